@@ -1,0 +1,835 @@
+// Host driver of the gfx950 Louvain engine and the C ABI of include/gala.h.
+//
+// Replaces the reference's round driver (louvain_gpu, LinXi-lx/GALA-mg
+// src/louvain_gpu/louvain.cu:136-229, with the multi-round loop of :193-226
+// made live), its phase-1 driver (louvain_main_process,
+// src/louvain_gpu/louvain_main_process.cu:237-1060) and phase-2 driver
+// (build_compressed_graph, src/louvain_gpu/build_compressed_graph.cu:5-173).
+//
+// Device residency: the input CSR is uploaded once; every round's graph,
+// state and frontier stay in HBM.  Per iteration the host reads back one
+// 128-byte counter block (exact sums for Q, move/sweep counts) -- the only
+// synchronisation, needed for the reference's convergence test.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gala.h"
+#include "gala_kernels.h"
+
+using namespace gala;
+
+#define HIPOK(x)                                    \
+  do {                                              \
+    hipError_t e_ = (x);                            \
+    if (e_ != hipSuccess) {                         \
+      ctx->last_hip = e_;                           \
+      return GALA_E_HIP;                            \
+    }                                               \
+  } while (0)
+
+namespace {
+
+constexpr int64_t M2_LIMIT = 1000000000LL;  // 4*m2^2 < 2^63 for the exact gain tests
+
+template <class P>
+hipError_t dalloc(P** p, int64_t elems) {
+  *p = nullptr;
+  size_t bytes = (size_t)std::max<int64_t>(elems, 1) * sizeof(P);
+  return hipMalloc((void**)p, bytes);
+}
+
+}  // namespace
+
+struct gala_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  gala_opts opts{};
+  hipError_t last_hip = hipSuccess;
+
+  // input graph (sizes) and mode
+  int32_t n0 = 0;
+  int64_t m0 = 0;
+  int wmode = GALA_W_NONE;
+  bool have_graph = false;
+  double t_h2d_ms = 0;
+
+  // CSR buffers: [0] = input (read-only, kept for re-runs), [1]/[2] = work
+  // buffers the contraction ping-pongs between.  Weights are int32 (int path)
+  // or double (FP path); the input may have implicit unit weights.
+  int64_t* off[3] = {nullptr, nullptr, nullptr};
+  int32_t* nbr[3] = {nullptr, nullptr, nullptr};
+  void* w[3] = {nullptr, nullptr, nullptr};
+  bool unit0 = true;
+  int src = 0;  // buffer holding the current round graph
+
+  // per-vertex state (sized n0)
+  void *K = nullptr, *Self = nullptr, *Tot = nullptr, *In = nullptr, *dTot = nullptr;
+  int32_t *size = nullptr, *dsize = nullptr, *prev = nullptr, *cur = nullptr;
+  uint8_t* active = nullptr;
+  int8_t* binid = nullptr;
+  int32_t *order = nullptr, *front = nullptr, *fin = nullptr;
+  int32_t *bstart_full = nullptr, *bstart_front = nullptr;
+  int64_t* hub_off = nullptr;
+  int64_t* hub_tmp = nullptr;
+  int32_t* hub_keys = nullptr;
+  void* hub_vals = nullptr;
+  int64_t hub_cap = 0;
+  int32_t* part_tmp = nullptr;
+  int64_t* scan64_tmp = nullptr;
+  Counters* ctr = nullptr;
+  Counters* ctr_h = nullptr;  // pinned
+
+  // contraction buffers
+  int32_t *c_cnt = nullptr, *c_nid = nullptr, *c_skey = nullptr, *c_dcnt = nullptr,
+          *c_big_list = nullptr, *c_nbig = nullptr, *c_i32_tmp = nullptr, *c_i32_scalar = nullptr;
+  int64_t *c_ecnt = nullptr, *c_eoff = nullptr, *c_ecur = nullptr, *c_big_tab = nullptr,
+          *c_i64_tmp = nullptr, *c_i64_scalar = nullptr;
+  void* c_sval = nullptr;
+  int32_t* arena_keys = nullptr;
+  void* arena_vals = nullptr;
+  uint32_t* arena_bits = nullptr;
+  int64_t arena_slots = 0, arena_words = 0;
+
+  std::vector<void*> owned;
+  gala_result res{};
+  bool have_result = false;
+  std::vector<int32_t> comm_host;
+
+  hipEvent_t ev[8];
+};
+
+namespace {
+
+bool is_fp(const gala_ctx* ctx) { return ctx->wmode == GALA_W_F32; }
+
+void free_all(gala_ctx* ctx) {
+  for (void* p : ctx->owned) (void)hipFree(p);
+  ctx->owned.clear();
+  if (ctx->hub_keys) (void)hipFree(ctx->hub_keys);
+  if (ctx->hub_vals) (void)hipFree(ctx->hub_vals);
+  ctx->hub_keys = nullptr; ctx->hub_vals = nullptr; ctx->hub_cap = 0;
+  if (ctx->arena_keys) (void)hipFree(ctx->arena_keys);
+  if (ctx->arena_vals) (void)hipFree(ctx->arena_vals);
+  if (ctx->arena_bits) (void)hipFree(ctx->arena_bits);
+  ctx->arena_keys = nullptr; ctx->arena_vals = nullptr; ctx->arena_bits = nullptr;
+  ctx->arena_slots = 0; ctx->arena_words = 0;
+  ctx->have_graph = false;
+}
+
+template <class P>
+int own(gala_ctx* ctx, P** p, int64_t elems) {
+  hipError_t e = dalloc(p, elems);
+  if (e != hipSuccess) { ctx->last_hip = e; return GALA_E_OOM; }
+  ctx->owned.push_back((void*)*p);
+  return GALA_OK;
+}
+
+#define OWN(p, n)                                   \
+  do {                                              \
+    int rc_ = own(ctx, &(p), (n));                  \
+    if (rc_) { free_all(ctx); return rc_; }         \
+  } while (0)
+
+// Allocate every device buffer for an input of n vertices / m entries.
+int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
+  free_all(ctx);
+  const int64_t N = std::max<int32_t>(n, 1), M = std::max<int64_t>(m, 1);
+  const size_t tsz = fp ? 8 : 4;  // tot_t / w_t / acc_t size
+  for (int k = 0; k < 3; k++) {
+    OWN(ctx->off[k], N + 1);
+    OWN(ctx->nbr[k], M);
+    char* p;
+    OWN(p, M * tsz);
+    ctx->w[k] = p;
+  }
+  { char* p;
+    OWN(p, N * tsz); ctx->K = p; OWN(p, N * tsz); ctx->Self = p; OWN(p, N * tsz); ctx->Tot = p;
+    OWN(p, N * tsz); ctx->In = p; OWN(p, N * tsz); ctx->dTot = p; }
+  OWN(ctx->size, N); OWN(ctx->dsize, N); OWN(ctx->prev, N); OWN(ctx->cur, N);
+  OWN(ctx->active, N); OWN(ctx->binid, N);
+  OWN(ctx->order, N); OWN(ctx->front, N); OWN(ctx->fin, N);
+  OWN(ctx->bstart_full, NBINS + 1); OWN(ctx->bstart_front, NBINS + 1);
+  OWN(ctx->hub_off, N + 1); OWN(ctx->hub_tmp, 2 * (N + 1));
+  OWN(ctx->part_tmp, part_tmp_elems(N));
+  OWN(ctx->scan64_tmp, scan_tmp_elems(N) + 2);
+  OWN(ctx->ctr, 1);
+  // contraction
+  OWN(ctx->c_cnt, N); OWN(ctx->c_nid, N); OWN(ctx->c_skey, M);
+  { char* p; OWN(p, M * tsz); ctx->c_sval = p; }
+  OWN(ctx->c_dcnt, N + 1); OWN(ctx->c_big_list, N); OWN(ctx->c_nbig, 1);
+  OWN(ctx->c_i32_tmp, N / 4 + 2 + scan_tmp_elems(N));
+  OWN(ctx->c_i32_scalar, 4);
+  OWN(ctx->c_ecnt, N + 1); OWN(ctx->c_eoff, N + 1); OWN(ctx->c_ecur, N);
+  OWN(ctx->c_big_tab, N + 1); OWN(ctx->c_i64_tmp, 2 * (N + 1) + scan_tmp_elems(N));
+  OWN(ctx->c_i64_scalar, 4);
+  if (!ctx->ctr_h) {
+    hipError_t e = hipHostMalloc((void**)&ctx->ctr_h, sizeof(Counters), hipHostMallocDefault);
+    if (e != hipSuccess) { ctx->last_hip = e; free_all(ctx); return GALA_E_OOM; }
+  }
+  return GALA_OK;
+}
+
+int grow_hub(gala_ctx* ctx, int64_t slots, bool fp) {
+  if (slots <= ctx->hub_cap) return GALA_OK;
+  if (ctx->hub_keys) (void)hipFree(ctx->hub_keys);
+  if (ctx->hub_vals) (void)hipFree(ctx->hub_vals);
+  ctx->hub_keys = nullptr; ctx->hub_vals = nullptr;
+  int64_t cap = slots + slots / 4;
+  hipError_t e = dalloc(&ctx->hub_keys, cap);
+  if (e == hipSuccess) {
+    char* p;
+    e = dalloc(&p, cap * (fp ? 8 : 4));
+    ctx->hub_vals = p;
+  }
+  if (e != hipSuccess) { ctx->last_hip = e; ctx->hub_cap = 0; return GALA_E_OOM; }
+  ctx->hub_cap = cap;
+  return GALA_OK;
+}
+
+int grow_arena(gala_ctx* ctx, int64_t slots, int64_t words, bool fp) {
+  if (slots > ctx->arena_slots) {
+    if (ctx->arena_keys) (void)hipFree(ctx->arena_keys);
+    if (ctx->arena_vals) (void)hipFree(ctx->arena_vals);
+    ctx->arena_keys = nullptr; ctx->arena_vals = nullptr;
+    int64_t cap = slots + slots / 4;
+    hipError_t e = dalloc(&ctx->arena_keys, cap);
+    if (e == hipSuccess) { char* p; e = dalloc(&p, cap * (fp ? 8 : 4)); ctx->arena_vals = p; }
+    if (e != hipSuccess) { ctx->last_hip = e; ctx->arena_slots = 0; return GALA_E_OOM; }
+    ctx->arena_slots = cap;
+  }
+  if (words > ctx->arena_words) {
+    if (ctx->arena_bits) (void)hipFree(ctx->arena_bits);
+    ctx->arena_bits = nullptr;
+    hipError_t e = dalloc(&ctx->arena_bits, words);
+    if (e != hipSuccess) { ctx->last_hip = e; ctx->arena_words = 0; return GALA_E_OOM; }
+    ctx->arena_words = words;
+  }
+  return GALA_OK;
+}
+
+// --------------------------------------------------------------------------
+// per-vertex hub-table offsets for the bins that use the global arena
+// --------------------------------------------------------------------------
+__global__ void k_hub_sizes(const int32_t* __restrict__ order, const int32_t* __restrict__ bstart,
+                            int gbin, const int64_t* __restrict__ off, int64_t* __restrict__ tsz) {
+  const int lo = bstart[gbin], hi = bstart[NBINS];
+  for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
+    int v = order[i];
+    int64_t d = off[v + 1] - off[v];
+    int64_t t = 1;
+    while (t < 2 * d) t <<= 1;
+    tsz[i - lo] = t;
+  }
+}
+__global__ void k_hub_scatter(const int32_t* __restrict__ order, const int32_t* __restrict__ bstart,
+                              int gbin, const int64_t* __restrict__ pos, int64_t* __restrict__ hub_off) {
+  const int lo = bstart[gbin], hi = bstart[NBINS];
+  for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x)
+    hub_off[order[i]] = pos[i - lo];
+}
+__global__ void k_offsets_from_incl(const int64_t* __restrict__ end_incl, int32_t n, int64_t* __restrict__ off) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x)
+    off[i] = i == 0 ? 0 : end_incl[i - 1];
+}
+__global__ void k_iota(int32_t* __restrict__ p, int32_t n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+}
+__global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (double)a[i];
+}
+template <class T>
+__global__ void k_sum_w(const typename T::w_t* __restrict__ w, int64_t m, Counters* c) {
+  typename T::tot_t s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    s += (typename T::tot_t)w[i];
+  // 64-bit accumulate for the int path: weights <= 2^31, up to 2^31 entries per thread sum
+  if (T::FP) atomicAdd(&c->sum_in_f, (double)s);
+  else atomicAdd(&c->sum_in, (unsigned long long)(long long)s);
+}
+__global__ void k_sum_w64(const int32_t* __restrict__ w, int64_t m, Counters* c) {
+  long long s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    s += w[i];
+  if (s) atomicAdd(&c->sum_in, (unsigned long long)s);
+}
+
+struct RoundOut {
+  double q;
+  int iters;
+  double t_dm, t_upd;
+};
+
+template <class T>
+DevGraph<T> graph_at(gala_ctx* ctx, int k, int32_t n, int64_t m) {
+  DevGraph<T> g;
+  g.n = n;
+  g.m = m;
+  g.off = ctx->off[k];
+  g.nbr = ctx->nbr[k];
+  g.w = (k == 0 && ctx->unit0) ? nullptr : (const typename T::w_t*)ctx->w[k];
+  return g;
+}
+inline int dst_of(int src) { return src == 1 ? 2 : 1; }
+
+template <class T>
+DevState<T> dstate(gala_ctx* ctx) {
+  using tt = typename T::tot_t;
+  DevState<T> st;
+  st.K = (tt*)ctx->K; st.Self = (tt*)ctx->Self; st.Tot = (tt*)ctx->Tot; st.In = (tt*)ctx->In;
+  st.dTot = (tt*)ctx->dTot;
+  st.size = ctx->size; st.dsize = ctx->dsize; st.prev = ctx->prev; st.cur = ctx->cur;
+  st.active = ctx->active; st.binid = ctx->binid; st.hub_off = ctx->hub_off;
+  st.hub_keys = ctx->hub_keys; st.hub_vals = (typename T::acc_t*)ctx->hub_vals;
+  st.ctr = ctx->ctr;
+  return st;
+}
+
+template <class T>
+double q_from(const Counters& c, double m2) {
+  if (T::FP) return c.sum_in_f / m2 - c.sum_tot2_f / (m2 * m2);
+  return (double)c.sum_in / m2 - (double)c.sum_tot2 / (m2 * m2);
+}
+
+// Bins, order, hub offsets and community init for the round graph.
+// caps[b] = vertices of bin b; *nitems = vertices in any bin (degree > 0),
+// i.e. the valid prefix of ctx->order.
+template <class T>
+int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nitems) {
+  hipStream_t s = ctx->stream;
+  DevState<T> st = dstate<T>(ctx);
+  launch_init<T>(g, st, s);
+  partition_bins(0, nullptr, ctx->binid, nullptr, g.n, ctx->order, ctx->bstart_full, ctx->part_tmp, s);
+  int32_t hb[NBINS + 1];
+  HIPOK(hipMemcpyAsync(hb, ctx->bstart_full, sizeof hb, hipMemcpyDeviceToHost, s));
+  HIPOK(hipStreamSynchronize(s));
+  for (int b = 0; b < NBINS; b++) caps[b] = hb[b + 1] - hb[b];
+  *nitems = hb[NBINS];
+  const int gb = global_bin_min<T>();
+  const int nh = hb[NBINS] - hb[gb];
+  if (nh > 0) {
+    int nb = std::min((nh + 255) / 256, 4096);
+    k_hub_sizes<<<nb, 256, 0, s>>>(ctx->order, ctx->bstart_full, gb, g.off, ctx->hub_tmp);
+    int64_t* pos = ctx->hub_tmp + nh + 1;
+    int64_t* tot = ctx->scan64_tmp;
+    scan_exclusive<int64_t, int64_t>(ctx->hub_tmp, pos, nh, ctx->c_i64_tmp, tot, s);
+    int64_t slots = 0;
+    HIPOK(hipMemcpyAsync(&slots, tot, sizeof slots, hipMemcpyDeviceToHost, s));
+    HIPOK(hipStreamSynchronize(s));
+    int rc = grow_hub(ctx, slots, T::FP);
+    if (rc) return rc;
+    k_hub_scatter<<<nb, 256, 0, s>>>(ctx->order, ctx->bstart_full, gb, pos, ctx->hub_off);
+  }
+  return GALA_OK;
+}
+
+template <class T>
+int read_counters(gala_ctx* ctx, Counters* out) {
+  HIPOK(hipMemcpyAsync(ctx->ctr_h, ctx->ctr, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+  HIPOK(hipStreamSynchronize(ctx->stream));
+  *out = *ctx->ctr_h;
+  if (out->overflow) return GALA_E_OVERFLOW;
+  return GALA_OK;
+}
+
+// One round of phase 1 (louvain_main_process).  On return ctx->prev holds the
+// accepted assignment of the round graph.
+template <class T>
+int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int round, gala_iter_cb icb,
+              void* user, RoundOut* ro, std::vector<int32_t>& hostvec) {
+  using tt = typename T::tot_t;
+  hipStream_t s = ctx->stream;
+  const double m2d = (double)m2;
+  int32_t caps[NBINS], nitems = 0;
+  int rc = round_setup<T>(ctx, g, caps, &nitems);
+  if (rc) return rc;
+  DevState<T> st = dstate<T>(ctx);
+  // Iteration 0: Q on the singleton partition (louvain_main_process.cu:538-548)
+  reset_counters(ctx->ctr, (long long)m2, m2d, s);
+  launch_apply<T>(st, g.n, m2, s);
+  for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], m2, s);
+  Counters c;
+  if ((rc = read_counters<T>(ctx, &c))) return rc;
+  double q = q_from<T>(c, m2d), qprev = q;
+  if (icb) {
+    const int32_t* vec = nullptr;
+    if (ctx->opts.dump_iterations) {
+      hostvec.resize(g.n);
+      HIPOK(hipMemcpy(hostvec.data(), ctx->prev, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost));
+      vec = hostvec.data();
+    }
+    icb(user, round, 0, q, 0, vec, g.n);
+  }
+  // first frontier: every vertex (fill(active,1), louvain_main_process.cu:574)
+  copy_i32(ctx->order, ctx->front, nitems, s);
+  copy_i32(ctx->bstart_full, ctx->bstart_front, NBINS + 1, s);
+  const int max_it = ctx->opts.max_iterations > 0 ? ctx->opts.max_iterations : 100000;
+  double t_dm = 0, t_upd = 0;
+  int it = 1;
+  for (;; it++) {
+    reset_counters(ctx->ctr, (long long)m2, m2d, s);
+    HIPOK(hipEventRecord(ctx->ev[0], s));
+    for (int b = 0; b < NBINS; b++)
+      launch_decide<T>(b, g, st, ctx->front, ctx->bstart_front, caps[b], m2, s);
+    HIPOK(hipEventRecord(ctx->ev[1], s));
+    launch_apply<T>(st, g.n, m2, s);
+    for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], m2, s);
+    partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front,
+                   ctx->part_tmp, s);
+    HIPOK(hipEventRecord(ctx->ev[2], s));
+    if ((rc = read_counters<T>(ctx, &c))) return rc;
+    float a = 0, b2 = 0;
+    (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
+    (void)hipEventElapsedTime(&b2, ctx->ev[1], ctx->ev[2]);
+    t_dm += a;
+    t_upd += b2;
+    ctx->res.swept_vertices += (int64_t)c.swept;
+    ctx->res.swept_edges += (int64_t)c.swept_edges;
+    if (!g.w) {
+      ctx->res.swept_vertices_unit += (int64_t)c.swept;
+      ctx->res.swept_edges_unit += (int64_t)c.swept_edges;
+    }
+    q = q_from<T>(c, m2d);
+    if (icb) {
+      const int32_t* vec = nullptr;
+      if (ctx->opts.dump_iterations) {
+        hostvec.resize(g.n);
+        HIPOK(hipMemcpy(hostvec.data(), ctx->cur, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost));
+        vec = hostvec.data();
+      }
+      icb(user, round, it, q, (int64_t)c.moved, vec, g.n);
+    }
+    if (q - qprev < ctx->opts.min_modularity || it >= max_it) break;
+    copy_i32(ctx->cur, ctx->prev, g.n, s);  // prev = cur (louvain_main_process.cu:1029)
+    qprev = q;
+  }
+  ro->q = qprev;
+  ro->iters = it;
+  ro->t_dm = t_dm;
+  ro->t_upd = t_upd;
+  (void)sizeof(tt);
+  return GALA_OK;
+}
+
+template <class T>
+ContractBufs<T> cbufs(gala_ctx* ctx) {
+  ContractBufs<T> cb;
+  cb.cnt = ctx->c_cnt; cb.nid = ctx->c_nid; cb.ecnt = ctx->c_ecnt; cb.eoff = ctx->c_eoff;
+  cb.ecur = ctx->c_ecur; cb.skey = ctx->c_skey; cb.sval = (typename T::acc_t*)ctx->c_sval;
+  cb.dcnt = ctx->c_dcnt; cb.noff = nullptr; cb.big_list = ctx->c_big_list; cb.nbig = ctx->c_nbig;
+  cb.big_tab = ctx->c_big_tab; cb.arena_keys = ctx->arena_keys;
+  cb.arena_vals = (typename T::acc_t*)ctx->arena_vals; cb.arena_bits = ctx->arena_bits;
+  cb.arena_slots = ctx->arena_slots; cb.arena_words = ctx->arena_words;
+  cb.i32_tmp = ctx->c_i32_tmp; cb.i64_tmp = ctx->c_i64_tmp; cb.i32_scalar = ctx->c_i32_scalar;
+  cb.i64_scalar = ctx->c_i64_scalar;
+  return cb;
+}
+
+// Contract the round graph under comm into buffer `dst`.  Returns C and new m.
+template <class T>
+int contract(gala_ctx* ctx, const DevGraph<T>& g, const int32_t* comm, int dst, int32_t* C_out,
+             int64_t* m_out) {
+  hipStream_t s = ctx->stream;
+  ContractBufs<T> cb = cbufs<T>(ctx);
+  int32_t C = 0;
+  int64_t staged = 0;
+  if (contract_prepare<T>(g, comm, cb, &C, &staged, s)) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
+  ContractOut<T> out;
+  out.off = ctx->off[dst];
+  out.nbr = ctx->nbr[dst];
+  out.w = (typename T::w_t*)ctx->w[dst];
+  int64_t m = 0;
+  for (int attempt = 0; attempt < 8; attempt++) {
+    cb = cbufs<T>(ctx);
+    int rc = contract_build<T>(g, comm, cb, C, staged, out, &m, s);
+    if (rc == 0) break;
+    if (rc == -1) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
+    // grow the row-merge arena: slots for all long rows, bitmap for 64 of them
+    int64_t need_slots = 0;
+    int32_t nbig = 0;
+    HIPOK(hipMemcpy(&nbig, ctx->c_nbig, sizeof nbig, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(&need_slots, ctx->c_big_tab + nbig, sizeof need_slots, hipMemcpyDeviceToHost));
+    int64_t words = (int64_t)((C + 31) / 32) * std::min<int64_t>(std::max(nbig, 1), 64);
+    int r2 = grow_arena(ctx, need_slots, rc == -3 ? words : ctx->arena_words, T::FP);
+    if (r2) return r2;
+    if (attempt == 7) return GALA_E_OOM;
+  }
+  Counters c;
+  HIPOK(hipMemcpyAsync(ctx->ctr_h, ctx->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIPOK(hipStreamSynchronize(s));
+  int32_t ovf = 0;
+  HIPOK(hipMemcpy(&ovf, ctx->c_i32_scalar + 3, sizeof ovf, hipMemcpyDeviceToHost));
+  (void)c;
+  if (ovf) return GALA_E_OVERFLOW;
+  *C_out = C;
+  *m_out = m;
+  return GALA_OK;
+}
+
+template <class T>
+int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala_result* out) {
+  using tt = typename T::tot_t;
+  hipStream_t s = ctx->stream;
+  gala_result& R = ctx->res;
+  std::memset(&R, 0, sizeof R);
+  R.n_vertices = ctx->n0;
+  R.input_edges = ctx->m0 / 2;
+  R.t_h2d_ms = ctx->t_h2d_ms;
+  // m2 = sum of weights (louvain.cu:165-167)
+  reset_counters(ctx->ctr, 0, 0.0, s);
+  DevGraph<T> g = graph_at<T>(ctx, 0, ctx->n0, ctx->m0);
+  int src = 0;
+  double m2d = 0;
+  if (g.w) {
+    int nb = (int)std::min<int64_t>((g.m + 255) / 256, 4096);
+    if (T::FP) k_sum_w<T><<<nb, 256, 0, s>>>(g.w, g.m, ctx->ctr);
+    else k_sum_w64<<<nb, 256, 0, s>>>((const int32_t*)g.w, g.m, ctx->ctr);
+    Counters c;
+    int rc = read_counters<T>(ctx, &c);
+    if (rc) return rc;
+    m2d = T::FP ? c.sum_in_f : (double)c.sum_in;
+  } else {
+    m2d = (double)g.m;
+  }
+  if (!(m2d > 0)) return GALA_E_INVALID;
+  if (!T::FP && m2d > (double)M2_LIMIT) return GALA_E_OVERFLOW;
+  const tt m2 = (tt)m2d;
+
+  HIPOK(hipEventRecord(ctx->ev[4], s));
+  k_iota<<<std::min((ctx->n0 + 255) / 256, 8192), 256, 0, s>>>(ctx->fin, ctx->n0);
+  double prev_mod = -1, cur_mod = -1;
+  int round = 0;
+  std::vector<int32_t> hostvec;
+  const int max_rounds = ctx->opts.max_rounds > 0 ? std::min(ctx->opts.max_rounds, 64) : 64;
+  for (;;) {
+    if (round > 0) prev_mod = cur_mod;
+    HIPOK(hipEventRecord(ctx->ev[5], s));
+    RoundOut ro;
+    int rc = run_round<T>(ctx, g, m2, round, icb, user, &ro, hostvec);
+    if (rc) return rc;
+    cur_mod = ro.q;
+    HIPOK(hipEventRecord(ctx->ev[6], s));
+    int32_t C = 0;
+    int64_t m = 0;
+    const int dst = dst_of(src);
+    rc = contract<T>(ctx, g, ctx->prev, dst, &C, &m);
+    if (rc) return rc;
+    compose_map(ctx->fin, ctx->n0, ctx->prev, ctx->c_nid, s);
+    HIPOK(hipEventRecord(ctx->ev[7], s));
+    HIPOK(hipEventSynchronize(ctx->ev[7]));
+    float t_round = 0, t_build = 0;
+    (void)hipEventElapsedTime(&t_round, ctx->ev[5], ctx->ev[6]);
+    (void)hipEventElapsedTime(&t_build, ctx->ev[6], ctx->ev[7]);
+    R.t_decide_ms += ro.t_dm;
+    R.t_update_ms += ro.t_upd;
+    R.t_contract_ms += t_build;
+    R.total_iterations += ro.iters;
+    if (round < 64) {
+      R.round_q[round] = cur_mod;
+      R.round_nc[round] = C;
+      R.round_iters[round] = ro.iters;
+      R.round_ms[round] = t_round;
+    }
+    if (round < 64) {
+      R.round_decide_ms[round] = ro.t_dm;
+      R.round_update_ms[round] = ro.t_upd;
+      R.round_build_ms[round] = t_build;
+    }
+    R.rounds = round + 1;
+    R.n_communities = C;
+    if (rcb) {
+      std::vector<int32_t> cm(g.n);
+      HIPOK(hipMemcpy(cm.data(), ctx->prev, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost));
+      if (ctx->opts.dump_rounds) {
+        std::vector<int64_t> off(C + 1), endi(C);
+        std::vector<int32_t> nb(m);
+        std::vector<int64_t> wi;
+        std::vector<double> wf;
+        HIPOK(hipMemcpy(off.data(), ctx->off[dst], sizeof(int64_t) * (C + 1), hipMemcpyDeviceToHost));
+        HIPOK(hipMemcpy(nb.data(), ctx->nbr[dst], sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+        for (int32_t i = 0; i < C; i++) endi[i] = off[i + 1];
+        if (T::FP) {
+          wf.resize(m);
+          HIPOK(hipMemcpy(wf.data(), ctx->w[dst], sizeof(double) * m, hipMemcpyDeviceToHost));
+        } else {
+          std::vector<int32_t> w32(m);
+          HIPOK(hipMemcpy(w32.data(), ctx->w[dst], sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+          wi.assign(w32.begin(), w32.end());
+        }
+        rcb(user, round, cur_mod, g.n, cm.data(), C, endi.data(), nb.data(),
+            T::FP ? nullptr : wi.data(), T::FP ? wf.data() : nullptr);
+      } else {
+        rcb(user, round, cur_mod, g.n, cm.data(), C, nullptr, nullptr, nullptr, nullptr);
+      }
+    }
+    // next round's graph = contracted graph
+    src = dst;
+    g = graph_at<T>(ctx, src, C, m);
+    round++;
+    R.rounds = round;
+    R.n_communities = C;
+    if (!(cur_mod - prev_mod >= ctx->opts.min_modularity)) break;
+    if (round >= max_rounds) break;
+  }
+  HIPOK(hipEventRecord(ctx->ev[3], s));
+  HIPOK(hipEventSynchronize(ctx->ev[3]));
+  float tt_ms = 0;
+  (void)hipEventElapsedTime(&tt_ms, ctx->ev[4], ctx->ev[3]);
+  R.t_total_ms = tt_ms;
+  R.modularity = cur_mod;
+  ctx->comm_host.resize(ctx->n0);
+  HIPOK(hipMemcpy(ctx->comm_host.data(), ctx->fin, sizeof(int32_t) * ctx->n0, hipMemcpyDeviceToHost));
+  ctx->have_result = true;
+  if (out) *out = R;
+  return GALA_OK;
+}
+
+// Upload a graph given as inclusive ends into buffer A (+ weights).
+int upload(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr, const void* w,
+           int weight_mode) {
+  if (n < 0 || (n > 0 && (!end_incl || !nbr))) return GALA_E_INVALID;
+  int64_t m = n ? end_incl[n - 1] : 0;
+  int64_t prev = 0;
+  for (int32_t i = 0; i < n; i++) {
+    if (end_incl[i] < prev) return GALA_E_FORMAT;
+    prev = end_incl[i];
+  }
+  for (int64_t j = 0; j < m; j++)
+    if (nbr[j] < 0 || nbr[j] >= n) return GALA_E_FORMAT;
+  if (weight_mode != GALA_W_NONE && m > 0 && !w) return GALA_E_INVALID;
+  if (weight_mode == GALA_W_I32)
+    for (int64_t j = 0; j < m; j++)
+      if (((const int32_t*)w)[j] <= 0) return GALA_E_FORMAT;  // Louvain needs positive weights
+  if (weight_mode == GALA_W_F32)
+    for (int64_t j = 0; j < m; j++)
+      if (!(((const float*)w)[j] > 0) || !std::isfinite(((const float*)w)[j])) return GALA_E_FORMAT;
+  ctx->wmode = weight_mode;
+  bool fp = weight_mode == GALA_W_F32;
+  int rc = alloc_buffers(ctx, n, m, fp);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  HIPOK(hipEventRecord(ctx->ev[0], s));
+  int64_t* dend = ctx->off[1];  // staging of the inclusive ends
+  if (n > 0) HIPOK(hipMemcpyAsync(dend, end_incl, sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+  k_offsets_from_incl<<<std::min((n + 256) / 256 + 1, 8192), 256, 0, s>>>(dend, n, ctx->off[0]);
+  if (m > 0) HIPOK(hipMemcpyAsync(ctx->nbr[0], nbr, sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+  ctx->unit0 = weight_mode == GALA_W_NONE;
+  if (weight_mode == GALA_W_I32 && m > 0)
+    HIPOK(hipMemcpyAsync(ctx->w[0], w, sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+  if (weight_mode == GALA_W_F32 && m > 0) {
+    float* tmpf = (float*)ctx->c_sval;  // staging (8 bytes per entry available)
+    HIPOK(hipMemcpyAsync(tmpf, w, sizeof(float) * m, hipMemcpyHostToDevice, s));
+    k_f32_to_f64<<<(int)std::min<int64_t>((m + 255) / 256, 8192), 256, 0, s>>>(tmpf, (double*)ctx->w[0], m);
+  }
+  HIPOK(hipEventRecord(ctx->ev[1], s));
+  HIPOK(hipEventSynchronize(ctx->ev[1]));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+  ctx->t_h2d_ms = ms;
+  ctx->n0 = n;
+  ctx->m0 = m;
+  ctx->have_graph = true;
+  ctx->have_result = false;
+  return GALA_OK;
+}
+
+const char* kErr[] = {"ok", "I/O error", "malformed graph", "out of memory", "HIP runtime error",
+                      "exact-integer range exceeded or hash overflow", "invalid argument or state"};
+
+}  // namespace
+
+extern "C" {
+
+const char* gala_version(void) { return "gala-mi355x 0.1 (gfx950)"; }
+
+const char* gala_strerror(int code) {
+  int k = -code;
+  if (k < 0 || k > 6) return "unknown error";
+  return kErr[k];
+}
+
+void gala_default_opts(gala_opts* o) {
+  std::memset(o, 0, sizeof *o);
+  o->min_modularity = 1e-6;
+  o->learned_tau = 0.5f;
+}
+
+int gala_create(gala_ctx** out, int device, const gala_opts* opts) {
+  if (!out) return GALA_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return GALA_E_HIP;
+  if (device < 0 || device >= ndev) return GALA_E_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return GALA_E_HIP;
+  gala_ctx* ctx = new gala_ctx();
+  ctx->device = device;
+  if (opts) ctx->opts = *opts;
+  else gala_default_opts(&ctx->opts);
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return GALA_E_HIP; }
+  for (auto& e : ctx->ev)
+    if (hipEventCreate(&e) != hipSuccess) { delete ctx; return GALA_E_HIP; }
+  *out = ctx;
+  return GALA_OK;
+}
+
+void gala_destroy(gala_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_all(ctx);
+  if (ctx->ctr_h) (void)hipHostFree(ctx->ctr_h);
+  for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int gala_set_graph(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr, const void* w,
+                   int weight_mode) {
+  if (!ctx || weight_mode < 0 || weight_mode > 2) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  return upload(ctx, n, end_incl, nbr, w, weight_mode);
+}
+
+int gala_load_bin(gala_ctx* ctx, const char* path, int weight_mode) {
+  if (!ctx || weight_mode < 0 || weight_mode > 2) return GALA_E_INVALID;
+  int32_t n = 0;
+  int64_t m = 0;
+  int rc = gala_read_bin(path, weight_mode, &n, &m, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  std::vector<int64_t> end(n);
+  std::vector<int32_t> nbr(m);
+  std::vector<int32_t> w(weight_mode != GALA_W_NONE ? m : 0);
+  rc = gala_read_bin(path, weight_mode, &n, &m, end.data(), nbr.data(), w.empty() ? nullptr : w.data());
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  return upload(ctx, n, end.data(), nbr.data(), w.empty() ? nullptr : w.data(), weight_mode);
+}
+
+int gala_run(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala_result* out) {
+  if (!ctx || !ctx->have_graph) return GALA_E_INVALID;
+  if (ctx->n0 == 0) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  // the input buffers are read-only: a context can run the same graph again
+  return is_fp(ctx) ? run_all<FltW>(ctx, icb, rcb, user, out) : run_all<IntW>(ctx, icb, rcb, user, out);
+}
+
+int gala_partial_result(gala_ctx* ctx, gala_result* out) {
+  if (!ctx || !out) return GALA_E_INVALID;
+  *out = ctx->res;
+  return GALA_OK;
+}
+
+int gala_get_communities(gala_ctx* ctx, int32_t* out_n0) {
+  if (!ctx || !out_n0 || !ctx->have_result) return GALA_E_INVALID;
+  std::memcpy(out_n0, ctx->comm_host.data(), sizeof(int32_t) * ctx->comm_host.size());
+  return GALA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// phase-level entry points
+// ---------------------------------------------------------------------------
+int gala_phase_decide(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr,
+                      const int32_t* w, int64_t m2, const int32_t* K, const int32_t* Self,
+                      const int32_t* Tot, const int32_t* size, const int32_t* prev,
+                      const uint8_t* frontier, int32_t* cur_out) {
+  if (!ctx || n <= 0 || !K || !Self || !Tot || !size || !prev || !cur_out) return GALA_E_INVALID;
+  if (m2 <= 0 || m2 > M2_LIMIT) return GALA_E_OVERFLOW;
+  (void)hipSetDevice(ctx->device);
+  int rc = upload(ctx, n, end_incl, nbr, w, w ? GALA_W_I32 : GALA_W_NONE);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  DevGraph<IntW> g = graph_at<IntW>(ctx, 0, n, ctx->m0);
+  int32_t caps[NBINS], nitems = 0;
+  if ((rc = round_setup<IntW>(ctx, g, caps, &nitems))) return rc;  // bins, order, hub tables
+  HIPOK(hipMemcpyAsync(ctx->K, K, 4ll * n, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ctx->Self, Self, 4ll * n, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ctx->Tot, Tot, 4ll * n, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ctx->size, size, 4ll * n, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ctx->prev, prev, 4ll * n, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ctx->cur, prev, 4ll * n, hipMemcpyHostToDevice, s));
+  if (frontier) HIPOK(hipMemcpyAsync(ctx->active, frontier, n, hipMemcpyHostToDevice, s));
+  else fill<uint8_t>(ctx->active, n, 1, s);
+  partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp, s);
+  DevState<IntW> st = dstate<IntW>(ctx);
+  reset_counters(ctx->ctr, m2, (double)m2, s);
+  for (int b = 0; b < NBINS; b++) launch_decide<IntW>(b, g, st, ctx->front, ctx->bstart_front, caps[b], (int32_t)m2, s);
+  Counters c;
+  if ((rc = read_counters<IntW>(ctx, &c))) return rc;
+  HIPOK(hipMemcpy(cur_out, ctx->cur, 4ll * n, hipMemcpyDeviceToHost));
+  ctx->have_graph = false;
+  return GALA_OK;
+}
+
+int gala_phase_weight(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr,
+                      const int32_t* w, int64_t m2, const int32_t* K, const int32_t* Self,
+                      const int32_t* Tot, const int32_t* cur, int64_t min_tot, int32_t* In_out,
+                      uint8_t* active_out) {
+  if (!ctx || n <= 0 || !K || !Self || !Tot || !cur || !In_out || !active_out) return GALA_E_INVALID;
+  if (m2 <= 0 || m2 > M2_LIMIT) return GALA_E_OVERFLOW;
+  (void)hipSetDevice(ctx->device);
+  int rc = upload(ctx, n, end_incl, nbr, w, w ? GALA_W_I32 : GALA_W_NONE);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  DevGraph<IntW> g = graph_at<IntW>(ctx, 0, n, ctx->m0);
+  int32_t caps[NBINS], nitems = 0;
+  if ((rc = round_setup<IntW>(ctx, g, caps, &nitems))) return rc;
+  HIPOK(hipMemcpyAsync(ctx->K, K, 4ll * n, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ctx->Self, Self, 4ll * n, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ctx->Tot, Tot, 4ll * n, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ctx->cur, cur, 4ll * n, hipMemcpyHostToDevice, s));
+  DevState<IntW> st = dstate<IntW>(ctx);
+  reset_counters(ctx->ctr, min_tot, (double)min_tot, s);
+  // degree-0 vertices are in no bin: In stays 0 and active 1, as the
+  // reference's zero-initialised In and fill(active, 1) leave them
+  fill<int32_t>((int32_t*)ctx->In, n, 0, s);
+  fill<uint8_t>(ctx->active, n, 1, s);
+  for (int b = 0; b < NBINS; b++) launch_weight<IntW>(b, g, st, ctx->order, ctx->bstart_full, caps[b], (int32_t)m2, s);
+  Counters c;
+  if ((rc = read_counters<IntW>(ctx, &c))) return rc;
+  HIPOK(hipMemcpy(In_out, ctx->In, 4ll * n, hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(active_out, ctx->active, n, hipMemcpyDeviceToHost));
+  ctx->have_graph = false;
+  return GALA_OK;
+}
+
+int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr,
+                        const int32_t* w, const int32_t* comm, int32_t* out_C, int64_t* out_m,
+                        int32_t* new_id, int64_t* out_end_incl, int32_t* out_nbr, int64_t* out_w) {
+  if (!ctx || n <= 0 || !comm || !out_C || !out_m) return GALA_E_INVALID;
+  for (int32_t v = 0; v < n; v++)
+    if (comm[v] < 0 || comm[v] >= n) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  int rc = upload(ctx, n, end_incl, nbr, w, w ? GALA_W_I32 : GALA_W_NONE);
+  if (rc) return rc;
+  DevGraph<IntW> g = graph_at<IntW>(ctx, 0, n, ctx->m0);
+  HIPOK(hipMemcpy(ctx->prev, comm, 4ll * n, hipMemcpyHostToDevice));
+  reset_counters(ctx->ctr, 0, 0.0, ctx->stream);
+  int32_t C = 0;
+  int64_t m = 0;
+  if ((rc = contract<IntW>(ctx, g, ctx->prev, 1, &C, &m))) return rc;
+  *out_C = C;
+  *out_m = m;
+  if (new_id) {
+    std::vector<int32_t> cnt(n), nid(n);
+    HIPOK(hipMemcpy(cnt.data(), ctx->c_cnt, 4ll * n, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(nid.data(), ctx->c_nid, 4ll * n, hipMemcpyDeviceToHost));
+    for (int32_t c = 0; c < n; c++) new_id[c] = cnt[c] > 0 ? nid[c] : -1;
+  }
+  if (out_end_incl && out_nbr && out_w) {
+    std::vector<int64_t> off(C + 1);
+    HIPOK(hipMemcpy(off.data(), ctx->off[1], 8ll * (C + 1), hipMemcpyDeviceToHost));
+    for (int32_t c = 0; c < C; c++) out_end_incl[c] = off[c + 1];
+    HIPOK(hipMemcpy(out_nbr, ctx->nbr[1], 4ll * m, hipMemcpyDeviceToHost));
+    std::vector<int32_t> w32(m);
+    HIPOK(hipMemcpy(w32.data(), ctx->w[1], 4ll * m, hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < m; j++) out_w[j] = w32[j];
+  }
+  ctx->have_graph = false;
+  return GALA_OK;
+}
+
+}  // extern "C"

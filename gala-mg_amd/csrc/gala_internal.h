@@ -1,0 +1,78 @@
+// Internal declarations shared by the gfx950 kernels and the host engine.
+// Not part of the C ABI (include/gala.h is).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gala {
+
+// ---------------------------------------------------------------------------
+// Degree bins for decide-and-move and weight update (wave64 re-cut of the
+// reference's eight bins, src/louvain_gpu/louvain_main_process.cu:276-288).
+//   bin 0: 1..3      4-lane sub-wave group, ring exchange
+//   bin 1: 4..7      8-lane group
+//   bin 2: 8..15     16-lane group
+//   bin 3: 16..31    32-lane group
+//   bin 4: 32..127   one wave per vertex, 256-slot LDS hash per wave
+//   bin 5: 128..511  one wave per vertex, 1024-slot LDS hash per wave
+//   bin 6: 512..4095 one 256-thread workgroup per vertex, 8192-slot LDS hash
+//   bin 7: 4096..8191 one 1024-thread workgroup per vertex, 16384-slot LDS hash
+//   bin 8: >= 8192   one 1024-thread workgroup per vertex, global hash table
+// Degree-0 vertices belong to no bin (they are never swept).
+// ---------------------------------------------------------------------------
+constexpr int NBINS = 9;
+constexpr int BIN_LO[NBINS] = {1, 4, 8, 16, 32, 128, 512, 4096, 8192};
+
+__host__ __device__ inline int bin_of_degree(int64_t d) {
+  if (d <= 0) return -1;
+  if (d < 4) return 0;
+  if (d < 8) return 1;
+  if (d < 16) return 2;
+  if (d < 32) return 3;
+  if (d < 128) return 4;
+  if (d < 512) return 5;
+  if (d < 4096) return 6;
+  if (d < 8192) return 7;
+  return 8;
+}
+
+// First bin that uses the global hash arena (FP64 tables do not fit 16384
+// slots in LDS, so the FP path sends bin 7 to the arena as well).
+template <class T>
+constexpr int global_bin_min() { return T::FP ? 7 : 8; }
+
+// Per-iteration device counters (one small struct read back per iteration).
+struct Counters {
+  unsigned long long sum_in;     // sum over v of In[v]            (exact, int path)
+  unsigned long long sum_tot2;   // sum over c of Tot[c]^2         (exact, int path)
+  long long min_tot;             // min non-zero Tot (zero -> m2)
+  unsigned long long moved;      // vertices that changed community this sweep
+  unsigned long long swept;      // vertices evaluated by decide-and-move
+  unsigned long long swept_edges;// row entries read by decide-and-move
+  unsigned long long active;     // vertices kept in the next frontier
+  int nonempty;                  // communities with size > 0
+  int overflow;                  // hash-table overflow flag (must stay 0)
+  double sum_in_f;               // FP path: sum In (FP64)
+  double sum_tot2_f;             // FP path: sum Tot^2 (FP64)
+  double min_tot_f;              // FP path: min non-zero Tot
+  long long pad[3];
+};
+
+// Weight traits: exact integer path and FP64-accumulating float path.
+struct IntW {
+  using w_t = int32_t;     // edge weight as stored
+  using acc_t = int32_t;   // per-community neighbour weight sum (<= K < 2^31)
+  using tot_t = int32_t;   // K, Self, Tot, In (all <= m2 < 2^31)
+  using gain_t = long long;
+  static constexpr bool FP = false;
+};
+struct FltW {
+  using w_t = double;      // fp32 input weights are widened on upload; contracted
+                           // rounds carry FP64 sums
+  using acc_t = double;
+  using tot_t = double;
+  using gain_t = double;
+  static constexpr bool FP = true;
+};
+
+}  // namespace gala

@@ -1,0 +1,388 @@
+// Host-side graph IO and synthetic generators (no device code).
+//
+// .bin layout (LinXi-lx/GALA-mg src/graph/graph.cpp:153-219): little-endian
+//   int32 n; int64 end[n] (inclusive row ends, no leading 0);
+//   int32 nbr[end[n-1]]; optional weight section of end[n-1] 4-byte values.
+// The weight section is selected out of band (the reference's -w flag); this
+// build adds float32 in the same slot.
+//
+// Generators (deterministic per seed, xoshiro256** seeded by splitmix64):
+//   SBM       : every pair i<j drawn once, Bernoulli(p_in | p_out), geometric skipping
+//   Chung-Lu  : expected degrees w_i = min(cap, c (i+1)^(-1/(gamma-1))), c set by
+//               bisection for the mean; edges by the Miller-Hagberg O(n+m) skip method
+//   R-MAT     : scale/edge factor/a,b,c, fp32 weights U[lo,hi]; symmetrised,
+//               duplicates and self-loops merged by summing weights
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/gala.h"
+
+namespace {
+
+struct Rng {
+  uint64_t s[4];
+  static uint64_t splitmix(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  explicit Rng(uint64_t seed) {
+    uint64_t x = seed;
+    for (auto& v : s) v = splitmix(x);
+  }
+  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  uint64_t next() {
+    uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+    s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = rotl(s[3], 45);
+    return r;
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }  // [0,1)
+  double uniform_pos() {  // (0,1]
+    return 1.0 - uniform();
+  }
+};
+
+// Symmetric CSR from an undirected edge list (u,v) (u != v); rows sorted, unique.
+struct Csr {
+  int32_t n = 0;
+  std::vector<int64_t> end;  // inclusive ends
+  std::vector<int32_t> nbr;
+  std::vector<float> w;
+};
+
+void csr_from_pairs(int32_t n, const std::vector<std::pair<int32_t, int32_t>>& e, Csr& out) {
+  std::vector<int64_t> deg(n + 1, 0);
+  for (auto& p : e) { deg[p.first + 1]++; deg[p.second + 1]++; }
+  for (int32_t i = 0; i < n; i++) deg[i + 1] += deg[i];
+  std::vector<int32_t> nb(deg[n]);
+  std::vector<int64_t> cur(deg.begin(), deg.end() - 1);
+  for (auto& p : e) { nb[cur[p.first]++] = p.second; nb[cur[p.second]++] = p.first; }
+  out.n = n;
+  out.end.assign(n, 0);
+  out.nbr.clear();
+  out.nbr.reserve(nb.size());
+  int64_t pos = 0;
+  for (int32_t i = 0; i < n; i++) {
+    auto b = nb.begin() + deg[i], en = nb.begin() + deg[i + 1];
+    std::sort(b, en);
+    auto last = std::unique(b, en);
+    for (auto it = b; it != last; ++it) out.nbr.push_back(*it);
+    pos += last - b;
+    out.end[i] = pos;
+  }
+}
+
+int emit(const Csr& c, int32_t* out_n, int64_t* out_m, int64_t* end_incl, int32_t* nbr, float* w) {
+  if (out_n) *out_n = c.n;
+  if (out_m) *out_m = c.n ? c.end[c.n - 1] : 0;
+  if (end_incl) std::memcpy(end_incl, c.end.data(), sizeof(int64_t) * c.n);
+  if (nbr) std::memcpy(nbr, c.nbr.data(), sizeof(int32_t) * c.nbr.size());
+  if (w && !c.w.empty()) std::memcpy(w, c.w.data(), sizeof(float) * c.w.size());
+  return GALA_OK;
+}
+
+// Bernoulli(p) over a run of `len` consecutive candidates: calls f(offset) for hits.
+template <class F>
+void bernoulli_run(Rng& r, int64_t len, double p, F f) {
+  if (p <= 0 || len <= 0) return;
+  if (p >= 1) { for (int64_t k = 0; k < len; k++) f(k); return; }
+  const double lq = std::log1p(-p);
+  int64_t k = -1;
+  for (;;) {
+    double u = r.uniform_pos();
+    int64_t skip = (int64_t)std::floor(std::log(u) / lq);
+    k += 1 + skip;
+    if (k >= len) break;
+    f(k);
+  }
+}
+
+// cache of the last generated graph so the two-call protocol does not regenerate
+struct GenCache {
+  std::string key;
+  Csr csr;
+} g_cache;
+
+}  // namespace
+
+extern "C" {
+
+int gala_read_bin(const char* path, int weight_mode, int32_t* n, int64_t* m, int64_t* end_incl,
+                  int32_t* nbr, void* w) {
+  if (!path || !n || !m) return GALA_E_INVALID;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return GALA_E_IO;
+  std::fseek(f, 0, SEEK_END);
+  long long fsz = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  int32_t nn = 0;
+  if (std::fread(&nn, 4, 1, f) != 1) { std::fclose(f); return GALA_E_FORMAT; }
+  if (nn < 0 || 4 + 8ll * nn > fsz) { std::fclose(f); return GALA_E_FORMAT; }
+  int64_t last = 0;
+  if (nn > 0) {
+    std::fseek(f, 4 + 8ll * (nn - 1), SEEK_SET);
+    if (std::fread(&last, 8, 1, f) != 1) { std::fclose(f); return GALA_E_FORMAT; }
+  }
+  long long need = 4 + 8ll * nn + 4ll * last + (weight_mode != GALA_W_NONE ? 4ll * last : 0);
+  if (last < 0 || fsz < need) { std::fclose(f); return GALA_E_FORMAT; }
+  *n = nn;
+  *m = last;
+  if (!end_incl) { std::fclose(f); return GALA_OK; }
+  std::fseek(f, 4, SEEK_SET);
+  if (nn > 0 && std::fread(end_incl, 8, nn, f) != (size_t)nn) { std::fclose(f); return GALA_E_IO; }
+  int64_t prev = 0;
+  for (int32_t i = 0; i < nn; i++) {
+    if (end_incl[i] < prev) { std::fclose(f); return GALA_E_FORMAT; }
+    prev = end_incl[i];
+  }
+  if (nbr && last > 0 && std::fread(nbr, 4, last, f) != (size_t)last) { std::fclose(f); return GALA_E_IO; }
+  if (nbr)
+    for (int64_t j = 0; j < last; j++)
+      if (nbr[j] < 0 || nbr[j] >= nn) { std::fclose(f); return GALA_E_FORMAT; }
+  if (w && weight_mode != GALA_W_NONE && last > 0 &&
+      std::fread(w, 4, last, f) != (size_t)last) { std::fclose(f); return GALA_E_IO; }
+  std::fclose(f);
+  return GALA_OK;
+}
+
+int gala_write_bin(const char* path, int32_t n, const int64_t* end_incl, const int32_t* nbr,
+                   const void* w, int weight_mode) {
+  if (!path || n < 0 || (n > 0 && (!end_incl || !nbr))) return GALA_E_INVALID;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return GALA_E_IO;
+  int64_t m = n ? end_incl[n - 1] : 0;
+  bool ok = std::fwrite(&n, 4, 1, f) == 1;
+  if (n) ok = ok && std::fwrite(end_incl, 8, n, f) == (size_t)n;
+  if (m) ok = ok && std::fwrite(nbr, 4, m, f) == (size_t)m;
+  if (w && weight_mode != GALA_W_NONE && m) ok = ok && std::fwrite(w, 4, m, f) == (size_t)m;
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? GALA_OK : GALA_E_IO;
+}
+
+int gala_ingest_text(const char* path, int weight_mode, int32_t* n, int64_t* m, int64_t* end_incl,
+                     int32_t* nbr, void* w) {
+  if (!path || !n || !m) return GALA_E_INVALID;
+  std::string key = std::string("txt:") + path + ":" + std::to_string(weight_mode);
+  if (g_cache.key != key) {
+    std::ifstream fin(path);
+    if (!fin.is_open()) return GALA_E_IO;
+    const bool wt = weight_mode != GALA_W_NONE;
+    std::vector<std::vector<std::pair<int64_t, double>>> rows;
+    std::string line;
+    while (std::getline(fin, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      std::istringstream iss(line);
+      long long a, b;
+      double x = 1;
+      if (!(iss >> a >> b)) continue;
+      if (wt) iss >> x;
+      if (a < 0 || b < 0 || a > INT32_MAX || b > INT32_MAX) return GALA_E_FORMAT;
+      size_t need = (size_t)std::max(a, b) + 1;
+      if (rows.size() < need) rows.resize(need);
+      rows[a].push_back({b, x});
+      rows[b].push_back({a, x});
+    }
+    // per-row dedup, ascending (graph.cpp:54-75); weights summed when weighted
+    std::vector<int32_t> renum(rows.size(), -1);
+    std::vector<char> linked(rows.size(), 0);
+    for (size_t i = 0; i < rows.size(); i++)
+      for (auto& p : rows[i]) { linked[i] = 1; linked[p.first] = 1; }
+    int32_t nn = 0;
+    for (size_t i = 0; i < rows.size(); i++)
+      if (linked[i]) renum[i] = nn++;
+    Csr c;
+    c.n = nn;
+    c.end.assign(nn, 0);
+    int64_t pos = 0;
+    for (size_t i = 0; i < rows.size(); i++) {
+      if (!linked[i]) continue;
+      std::map<int32_t, double> mp;
+      for (auto& p : rows[i]) {
+        auto it = mp.find(renum[p.first]);
+        if (it == mp.end()) mp.emplace(renum[p.first], p.second);
+        else if (wt) it->second += p.second;
+      }
+      // a self-loop line pushes the entry twice into its own row; keep one entry
+      for (auto& kv : mp) {
+        c.nbr.push_back(kv.first);
+        if (wt) c.w.push_back((float)kv.second);
+        pos++;
+      }
+      c.end[renum[i]] = pos;
+    }
+    g_cache.key = key;
+    g_cache.csr = std::move(c);
+  }
+  const Csr& c = g_cache.csr;
+  *n = c.n;
+  *m = c.n ? c.end[c.n - 1] : 0;
+  if (end_incl) std::memcpy(end_incl, c.end.data(), 8 * c.n);
+  if (nbr) std::memcpy(nbr, c.nbr.data(), 4 * c.nbr.size());
+  if (w && weight_mode == GALA_W_F32) std::memcpy(w, c.w.data(), 4 * c.w.size());
+  if (w && weight_mode == GALA_W_I32)
+    for (size_t j = 0; j < c.w.size(); j++) ((int32_t*)w)[j] = (int32_t)std::lround(c.w[j]);
+  return GALA_OK;
+}
+
+int gala_gen_sbm(int32_t n, int32_t blocks, double p_in, double p_out, uint64_t seed, int32_t* out_n,
+                 int64_t* out_m, int64_t* end_incl, int32_t* nbr) {
+  if (n < 0 || blocks < 1 || blocks > std::max(n, 1)) return GALA_E_INVALID;
+  char kb[256];
+  std::snprintf(kb, sizeof kb, "sbm:%d:%d:%.17g:%.17g:%llu", n, blocks, p_in, p_out,
+                (unsigned long long)seed);
+  if (g_cache.key != kb) {
+    Rng r(seed);
+    auto blk = [&](int64_t v) { return (int32_t)((v * blocks) / n); };
+    auto bstart = [&](int32_t b) {  // first vertex of block b
+      return (int32_t)(((int64_t)b * n + blocks - 1) / blocks);
+    };
+    std::vector<std::pair<int32_t, int32_t>> e;
+    for (int32_t i = 0; i < n; i++) {
+      int32_t bi = blk(i);
+      // runs of constant probability over j > i: rest of own block, then other blocks
+      int32_t j0 = i + 1;
+      for (int32_t b = bi; b < blocks && j0 < n; b++) {
+        int32_t j1 = (b + 1 < blocks) ? bstart(b + 1) : n;
+        if (j1 <= j0) continue;
+        double p = (b == bi) ? p_in : p_out;
+        bernoulli_run(r, j1 - j0, p, [&](int64_t k) { e.push_back({i, (int32_t)(j0 + k)}); });
+        j0 = j1;
+      }
+    }
+    Csr c;
+    csr_from_pairs(n, e, c);
+    g_cache.key = kb;
+    g_cache.csr = std::move(c);
+  }
+  return emit(g_cache.csr, out_n, out_m, end_incl, nbr, nullptr);
+}
+
+int gala_gen_chung_lu(int32_t n, double gamma, double mean_deg, double max_deg, uint64_t seed,
+                      int32_t* out_n, int64_t* out_m, int64_t* end_incl, int32_t* nbr) {
+  if (n < 2 || gamma <= 1 || mean_deg <= 0 || max_deg <= 0) return GALA_E_INVALID;
+  char kb[256];
+  std::snprintf(kb, sizeof kb, "cl:%d:%.17g:%.17g:%.17g:%llu", n, gamma, mean_deg, max_deg,
+                (unsigned long long)seed);
+  if (g_cache.key != kb) {
+    const double ex = -1.0 / (gamma - 1.0);
+    std::vector<double> base(n);
+    for (int32_t i = 0; i < n; i++) base[i] = std::pow((double)i + 1.0, ex);
+    auto mean_for = [&](double c) {
+      double s = 0;
+      for (int32_t i = 0; i < n; i++) s += std::min(max_deg, c * base[i]);
+      return s / n;
+    };
+    double lo = 0, hi = 1;
+    while (mean_for(hi) < mean_deg && hi < 1e300) hi *= 2;
+    for (int it = 0; it < 200; it++) {
+      double mid = 0.5 * (lo + hi);
+      (mean_for(mid) < mean_deg ? lo : hi) = mid;
+    }
+    std::vector<double> wv(n);
+    double S = 0;
+    for (int32_t i = 0; i < n; i++) { wv[i] = std::min(max_deg, hi * base[i]); S += wv[i]; }
+    Rng r(seed);
+    std::vector<std::pair<int32_t, int32_t>> e;
+    e.reserve((size_t)(S / 2 * 1.02));
+    for (int32_t u = 0; u < n - 1; u++) {
+      int64_t v = u + 1;
+      double p = std::min(wv[u] * wv[v] / S, 1.0);
+      while (v < n && p > 0) {
+        if (p != 1.0) {
+          double rr = r.uniform_pos();
+          v += (int64_t)std::floor(std::log(rr) / std::log1p(-p));
+        }
+        if (v < n) {
+          double q = std::min(wv[u] * wv[v] / S, 1.0);
+          if (r.uniform() < q / p) e.push_back({u, (int32_t)v});
+          p = q;
+          v += 1;
+        }
+      }
+    }
+    Csr c;
+    csr_from_pairs(n, e, c);
+    g_cache.key = kb;
+    g_cache.csr = std::move(c);
+  }
+  return emit(g_cache.csr, out_n, out_m, end_incl, nbr, nullptr);
+}
+
+int gala_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                  float w_lo, float w_hi, int32_t* out_n, int64_t* out_m, int64_t* end_incl,
+                  int32_t* nbr, float* w) {
+  if (scale < 1 || scale > 30 || edge_factor < 1 || a < 0 || b < 0 || c < 0 || a + b + c > 1)
+    return GALA_E_INVALID;
+  char kb[256];
+  std::snprintf(kb, sizeof kb, "rmat:%d:%d:%.17g:%.17g:%.17g:%llu:%g:%g", scale, edge_factor, a, b, c,
+                (unsigned long long)seed, (double)w_lo, (double)w_hi);
+  if (g_cache.key != kb) {
+    const int32_t n = 1 << scale;
+    const int64_t ne = (int64_t)edge_factor << scale;
+    Rng r(seed);
+    std::vector<int32_t> src(ne), dst(ne);
+    std::vector<float> wt(ne);
+    for (int64_t k = 0; k < ne; k++) {
+      int32_t u = 0, v = 0;
+      for (int lvl = 0; lvl < scale; lvl++) {
+        double x = r.uniform();
+        int bu = 0, bv = 0;
+        if (x < a) {}
+        else if (x < a + b) bv = 1;
+        else if (x < a + b + c) bu = 1;
+        else { bu = 1; bv = 1; }
+        u = (u << 1) | bu;
+        v = (v << 1) | bv;
+      }
+      src[k] = u; dst[k] = v;
+      wt[k] = (float)(w_lo + (w_hi - w_lo) * r.uniform());
+    }
+    // symmetrise into rows; a self-loop contributes one entry
+    std::vector<int64_t> deg(n + 1, 0);
+    for (int64_t k = 0; k < ne; k++) {
+      deg[src[k] + 1]++;
+      if (src[k] != dst[k]) deg[dst[k] + 1]++;
+    }
+    for (int32_t i = 0; i < n; i++) deg[i + 1] += deg[i];
+    std::vector<std::pair<int32_t, float>> ent(deg[n]);
+    std::vector<int64_t> cur(deg.begin(), deg.end() - 1);
+    for (int64_t k = 0; k < ne; k++) {
+      ent[cur[src[k]]++] = {dst[k], wt[k]};
+      if (src[k] != dst[k]) ent[cur[dst[k]]++] = {src[k], wt[k]};
+    }
+    src.clear(); src.shrink_to_fit(); dst.clear(); dst.shrink_to_fit(); wt.clear(); wt.shrink_to_fit();
+    Csr cs;
+    cs.n = n;
+    cs.end.assign(n, 0);
+    int64_t pos = 0;
+    for (int32_t i = 0; i < n; i++) {
+      auto bgn = ent.begin() + deg[i], en = ent.begin() + deg[i + 1];
+      std::sort(bgn, en, [](const std::pair<int32_t, float>& x, const std::pair<int32_t, float>& y) {
+        return x.first < y.first;
+      });
+      for (auto it = bgn; it != en;) {
+        int32_t k = it->first;
+        double s = 0;
+        while (it != en && it->first == k) { s += it->second; ++it; }
+        cs.nbr.push_back(k);
+        cs.w.push_back((float)s);
+        pos++;
+      }
+      cs.end[i] = pos;
+    }
+    g_cache.key = kb;
+    g_cache.csr = std::move(cs);
+  }
+  return emit(g_cache.csr, out_n, out_m, end_incl, nbr, w);
+}
+
+}  // extern "C"

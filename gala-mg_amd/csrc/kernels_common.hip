@@ -1,0 +1,286 @@
+// Scans, stable partition, fills and small reductions used by every phase.
+// All kernels are written for gfx950 wave64; block sizes are multiples of 64.
+#include "gala_kernels.h"
+
+namespace gala {
+
+// ---------------------------------------------------------------------------
+// wave / block helpers
+// ---------------------------------------------------------------------------
+template <class V>
+__device__ __forceinline__ V wave_incl_scan(V x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 1; k < 64; k <<= 1) {
+    V y = __shfl_up(x, k, 64);
+    if (lane >= k) x += y;
+  }
+  return x;
+}
+
+template <class V>
+__device__ __forceinline__ V wave_sum(V x) {
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) x += __shfl_xor(x, k, 64);
+  return x;
+}
+
+// Exclusive block scan for blockDim.x <= 1024; returns the prefix, *total = sum.
+template <class V>
+__device__ __forceinline__ V block_excl_scan(V x, V* sh /*[16]*/, V* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  V inc = wave_incl_scan(x);
+  if (lane == 63) sh[wid] = inc;
+  __syncthreads();
+  if (wid == 0) {
+    V s = lane < nw ? sh[lane] : V(0);
+    V si = wave_incl_scan(s);
+    if (lane < nw) sh[lane] = si - s;
+    if (lane == nw - 1) sh[16] = si;
+  }
+  __syncthreads();
+  V r = inc - x + sh[wid];
+  *total = sh[16];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// device-wide exclusive scan (reduce-then-scan, three launches)
+// ---------------------------------------------------------------------------
+constexpr int SCAN_T = 256, SCAN_IPT = 16, SCAN_TILE = SCAN_T * SCAN_IPT;
+
+template <class In, class V>
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const In* __restrict__ in, int64_t n,
+                                                        V* __restrict__ part) {
+  __shared__ V sh[17];
+  int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  V s = 0;
+  for (int k = 0; k < SCAN_IPT; k++) {
+    int64_t i = base + (int64_t)k * SCAN_T + threadIdx.x;
+    if (i < n) s += (V)in[i];
+  }
+  V tot;
+  block_excl_scan(s, sh, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+template <class V>
+__global__ __launch_bounds__(1024) void k_scan_top(V* __restrict__ part, int nb, V* __restrict__ total) {
+  __shared__ V sh[17];
+  // each thread owns a contiguous chunk
+  int per = (nb + 1023) / 1024;
+  int lo = threadIdx.x * per, hi = min(nb, lo + per);
+  V s = 0;
+  for (int i = lo; i < hi; i++) s += part[i];
+  V tot;
+  V pre = block_excl_scan(s, sh, &tot);
+  for (int i = lo; i < hi; i++) { V t = part[i]; part[i] = pre; pre += t; }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+
+template <class In, class V>
+__global__ __launch_bounds__(SCAN_T) void k_scan_down(const In* __restrict__ in, int64_t n,
+                                                      const V* __restrict__ part, V* __restrict__ out) {
+  __shared__ V sh[17];
+  int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_IPT;
+  V loc[SCAN_IPT];
+  V s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_IPT; k++) {
+    int64_t i = base + k;
+    loc[k] = i < n ? (V)in[i] : V(0);
+    s += loc[k];
+  }
+  V tot;
+  V pre = block_excl_scan(s, sh, &tot) + part[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_IPT; k++) {
+    int64_t i = base + k;
+    if (i < n) out[i] = pre;
+    pre += loc[k];
+  }
+}
+
+int64_t scan_tmp_elems(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+template <class In, class V>
+void scan_exclusive(const In* in, V* out, int64_t n, V* tmp, V* total, hipStream_t s) {
+  int nb = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
+  if (nb == 0) {
+    (void)hipMemsetAsync(total, 0, sizeof(V), s);
+    return;
+  }
+  k_scan_reduce<In, V><<<nb, SCAN_T, 0, s>>>(in, n, tmp);
+  k_scan_top<V><<<1, 1024, 0, s>>>(tmp, nb, total);
+  k_scan_down<In, V><<<nb, SCAN_T, 0, s>>>(in, n, tmp, out);
+}
+
+template void scan_exclusive<int32_t, int32_t>(const int32_t*, int32_t*, int64_t, int32_t*, int32_t*, hipStream_t);
+template void scan_exclusive<int32_t, int64_t>(const int32_t*, int64_t*, int64_t, int64_t*, int64_t*, hipStream_t);
+template void scan_exclusive<int64_t, int64_t>(const int64_t*, int64_t*, int64_t, int64_t*, int64_t*, hipStream_t);
+template void scan_exclusive<uint8_t, int32_t>(const uint8_t*, int32_t*, int64_t, int32_t*, int32_t*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Stable multi-way partition into NBINS bins (key < 0 drops the item).
+//   mode 0: items are 0..n-1, key = binid[i]
+//   mode 1: items are order[i], key = active[v] ? binid[v] : -1
+// out receives the kept items grouped by bin in input order; bstart[NBINS+1].
+// ---------------------------------------------------------------------------
+constexpr int PART_T = 256, PART_IPT = 8, PART_TILE = PART_T * PART_IPT;
+
+__device__ __forceinline__ void part_item(int mode, const int32_t* order, const int8_t* binid,
+                                          const uint8_t* active, int64_t i, int& v, int& key) {
+  if (mode == 0) {
+    v = (int)i;
+    key = binid[i];
+  } else {
+    v = order[i];
+    key = active[v] ? (int)binid[v] : -1;
+  }
+}
+
+__global__ __launch_bounds__(PART_T) void k_part_count(int mode, const int32_t* __restrict__ order,
+                                                       const int8_t* __restrict__ binid,
+                                                       const uint8_t* __restrict__ active, int64_t n,
+                                                       int32_t* __restrict__ cnt, int nblk) {
+  __shared__ int sc[NBINS];
+  if (threadIdx.x < NBINS) sc[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * PART_TILE + (int64_t)threadIdx.x * PART_IPT;
+  int c[NBINS];
+#pragma unroll
+  for (int b = 0; b < NBINS; b++) c[b] = 0;
+  for (int k = 0; k < PART_IPT; k++) {
+    int64_t i = base + k;
+    if (i < n) {
+      int v, key;
+      part_item(mode, order, binid, active, i, v, key);
+#pragma unroll
+      for (int b = 0; b < NBINS; b++) c[b] += (key == b);
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NBINS; b++) {
+    int s = wave_sum(c[b]);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&sc[b], s);
+  }
+  __syncthreads();
+  if (threadIdx.x < NBINS) cnt[threadIdx.x * nblk + blockIdx.x] = sc[threadIdx.x];
+}
+
+__global__ __launch_bounds__(1024) void k_part_top(int32_t* __restrict__ cnt, int nblk,
+                                                   int32_t* __restrict__ bstart) {
+  __shared__ int32_t sh[17];
+  int total = NBINS * nblk;
+  int per = (total + 1023) / 1024;
+  int lo = threadIdx.x * per, hi = min(total, lo + per);
+  int32_t s = 0;
+  for (int i = lo; i < hi; i++) s += cnt[i];
+  int32_t tot;
+  int32_t pre = block_excl_scan(s, sh, &tot);
+  for (int i = lo; i < hi; i++) {
+    int32_t t = cnt[i];
+    cnt[i] = pre;
+    if (i % nblk == 0) bstart[i / nblk] = pre;
+    pre += t;
+  }
+  if (threadIdx.x == 0) bstart[NBINS] = tot;
+}
+
+__global__ __launch_bounds__(PART_T) void k_part_scatter(int mode, const int32_t* __restrict__ order,
+                                                         const int8_t* __restrict__ binid,
+                                                         const uint8_t* __restrict__ active, int64_t n,
+                                                         const int32_t* __restrict__ cnt, int nblk,
+                                                         int32_t* __restrict__ out) {
+  __shared__ int32_t sh[17];
+  int64_t base = (int64_t)blockIdx.x * PART_TILE + (int64_t)threadIdx.x * PART_IPT;
+  int vs[PART_IPT], ks[PART_IPT];
+  int c[NBINS];
+#pragma unroll
+  for (int b = 0; b < NBINS; b++) c[b] = 0;
+#pragma unroll
+  for (int k = 0; k < PART_IPT; k++) {
+    int64_t i = base + k;
+    ks[k] = -1;
+    vs[k] = 0;
+    if (i < n) part_item(mode, order, binid, active, i, vs[k], ks[k]);
+#pragma unroll
+    for (int b = 0; b < NBINS; b++) c[b] += (ks[k] == b);
+  }
+  int pos[NBINS];
+#pragma unroll
+  for (int b = 0; b < NBINS; b++) {
+    int32_t tot;
+    pos[b] = block_excl_scan<int32_t>(c[b], sh, &tot) + cnt[b * nblk + blockIdx.x];
+  }
+#pragma unroll
+  for (int k = 0; k < PART_IPT; k++) {
+    int key = ks[k];
+    if (key >= 0) {
+      int p = 0;
+#pragma unroll
+      for (int b = 0; b < NBINS; b++)
+        if (key == b) { p = pos[b]; pos[b] = p + 1; }
+      out[p] = vs[k];
+    }
+  }
+}
+
+int64_t part_tmp_elems(int64_t n) { return (int64_t)NBINS * ((n + PART_TILE - 1) / PART_TILE + 1); }
+
+void partition_bins(int mode, const int32_t* order, const int8_t* binid, const uint8_t* active,
+                    int64_t n, int32_t* out, int32_t* bstart, int32_t* tmp, hipStream_t s) {
+  int nblk = (int)((n + PART_TILE - 1) / PART_TILE);
+  if (nblk == 0) nblk = 1;
+  k_part_count<<<nblk, PART_T, 0, s>>>(mode, order, binid, active, n, tmp, nblk);
+  k_part_top<<<1, 1024, 0, s>>>(tmp, nblk, bstart);
+  k_part_scatter<<<nblk, PART_T, 0, s>>>(mode, order, binid, active, n, tmp, nblk, out);
+}
+
+// ---------------------------------------------------------------------------
+// fills / copies
+// ---------------------------------------------------------------------------
+template <class V>
+__global__ void k_fill(V* __restrict__ p, int64_t n, V x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = x;
+}
+
+template <class V>
+void fill(V* p, int64_t n, V x, hipStream_t s) {
+  if (n <= 0) return;
+  int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  k_fill<V><<<nb, 256, 0, s>>>(p, n, x);
+}
+template void fill<int32_t>(int32_t*, int64_t, int32_t, hipStream_t);
+template void fill<int64_t>(int64_t*, int64_t, int64_t, hipStream_t);
+template void fill<uint8_t>(uint8_t*, int64_t, uint8_t, hipStream_t);
+template void fill<double>(double*, int64_t, double, hipStream_t);
+template void fill<float>(float*, int64_t, float, hipStream_t);
+
+__global__ void k_copy_i32(const int32_t* __restrict__ a, int32_t* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+void copy_i32(const int32_t* a, int32_t* b, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  k_copy_i32<<<nb, 256, 0, s>>>(a, b, n);
+}
+
+__global__ void k_reset_counters(Counters* c, long long min_init, double min_init_f) {
+  if (threadIdx.x == 0) {
+    c->sum_in = 0; c->sum_tot2 = 0; c->min_tot = min_init; c->moved = 0; c->swept = 0;
+    c->swept_edges = 0; c->active = 0; c->nonempty = 0; c->overflow = 0;
+    c->sum_in_f = 0; c->sum_tot2_f = 0; c->min_tot_f = min_init_f;
+  }
+}
+void reset_counters(Counters* c, long long min_init, double min_init_f, hipStream_t s) {
+  k_reset_counters<<<1, 64, 0, s>>>(c, min_init, min_init_f);
+}
+
+}  // namespace gala

@@ -1,0 +1,650 @@
+// Phase-1 kernels for gfx950: community init, decide-and-move (four kernel
+// families over nine degree bins), Tot/size apply with the modularity sums,
+// and the weight update (In + sound prune).
+//
+// Semantics follow LinXi-lx/GALA-mg src/louvain_gpu/kernel_functions.cu:
+//   init_communities :34-73, build_and_select_in_warp :289-441,
+//   build_and_select :473-593, build_and_select_blk :640-813,
+//   compute_In / compute_In_blk :889-1034,
+// with gains compared exactly:  H(d) = w_d*m2 - K*(Tot_d - Tot_c + K)  (= dq*m2/2),
+// best = argmax H over d != c with H > 0, ties to the lowest id, move iff
+// H(best) - m2*(w_c - Self) > 0 and not (size[best]==size[c]==1 && best > c).
+#include "gala_kernels.h"
+#include <climits>
+
+namespace gala {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned hash_slot(int key, int lg) {
+  // multiplicative hashing onto a power-of-two table
+  return ((unsigned)key * 2654435769u) >> (32 - lg);
+}
+
+__device__ __forceinline__ int ilog2_ceil(int x) {  // x >= 1
+  return x <= 1 ? 0 : 32 - __clz(x - 1);
+}
+
+template <class G>
+__device__ __forceinline__ bool better(G h, int d, G bh, int bd) {
+  return h > bh || (h == bh && d < bd);
+}
+
+// All lanes of the wave (or sub-group of width W) reach the same (H, d).
+template <class G, int W>
+__device__ __forceinline__ void argmax_xor(G& h, int& d) {
+#pragma unroll
+  for (int k = W / 2; k >= 1; k >>= 1) {
+    G ho = __shfl_xor(h, k, W);
+    int dd = __shfl_xor(d, k, W);
+    if (better(ho, dd, h, d)) { h = ho; d = dd; }
+  }
+}
+
+template <class V, int W>
+__device__ __forceinline__ V sum_xor(V x) {
+#pragma unroll
+  for (int k = W / 2; k >= 1; k >>= 1) x += __shfl_xor(x, k, W);
+  return x;
+}
+
+__device__ __forceinline__ void lds_fence_wave() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <class T>
+__device__ __forceinline__ typename T::w_t load_w(const typename T::w_t* w, int64_t j) {
+  return w ? w[j] : (typename T::w_t)1;
+}
+
+// Decision and state write-back shared by every decide kernel (one thread).
+template <class T>
+__device__ __forceinline__ bool commit_move(int v, int c, typename T::gain_t bestH, int bestD,
+                                            typename T::acc_t wc, typename T::tot_t K,
+                                            typename T::tot_t S, typename T::tot_t m2,
+                                            const int32_t* __restrict__ size, int32_t* __restrict__ cur,
+                                            typename T::tot_t* dTot, int32_t* dsize) {
+  using G = typename T::gain_t;
+  int nc = c;
+  if (bestD != INT_MAX) {
+    G gain = bestH - (G)m2 * ((G)wc - (G)S);
+    if (gain > 0 && !(size[bestD] == 1 && size[c] == 1 && bestD > c)) nc = bestD;
+  }
+  cur[v] = nc;
+  if (nc != c) {
+    atomicAdd(&dTot[nc], K);
+    atomicAdd(&dTot[c], -K);
+    atomicAdd(&dsize[nc], 1);
+    atomicAdd(&dsize[c], -1);
+    return true;
+  }
+  return false;
+}
+
+template <class V>
+__device__ __forceinline__ void atomic_add_ull(unsigned long long* p, V x) {
+  if (x) atomicAdd(p, (unsigned long long)x);
+}
+
+// ---------------------------------------------------------------------------
+// init: K, Self, Tot=K, size=1, prev=cur=v, bin id, active=1
+// (init_communities, kernel_functions.cu:34-73)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> st) {
+  using tt = typename T::tot_t;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += gridDim.x * blockDim.x) {
+    int64_t o = g.off[v], e = g.off[v + 1];
+    int64_t d = e - o;
+    st.binid[v] = (int8_t)bin_of_degree(d);
+    st.size[v] = 1;
+    st.prev[v] = v;
+    st.cur[v] = v;
+    st.active[v] = 1;
+    st.dTot[v] = 0;
+    st.dsize[v] = 0;
+    if (d > 64) continue;  // wide rows: k_init_wide
+    tt k = 0, self = 0;
+    for (int64_t j = o; j < e; j++) {
+      tt w = (tt)load_w<T>(g.w, j);
+      k += w;
+      if (g.nbr[j] == v) self = w;
+    }
+    st.K[v] = k;
+    st.Tot[v] = k;
+    st.Self[v] = self;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_init_wide(DevGraph<T> g, DevState<T> st) {
+  using tt = typename T::tot_t;
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  for (int v = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); v < g.n; v += nw) {
+    int64_t o = g.off[v], e = g.off[v + 1];
+    if (e - o <= 64) continue;
+    tt k = 0, self = 0;
+    for (int64_t j = o + lane; j < e; j += 64) {
+      tt w = (tt)load_w<T>(g.w, j);
+      k += w;
+      if (g.nbr[j] == v) self = w;
+    }
+    k = sum_xor<tt, 64>(k);
+    self = sum_xor<tt, 64>(self);  // at most one self entry per row
+    if (lane == 0) {
+      st.K[v] = k;
+      st.Tot[v] = k;
+      st.Self[v] = self;
+    }
+  }
+}
+
+template <class T>
+void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s) {
+  if (g.n <= 0) return;
+  int nb = std::min((g.n + 255) / 256, 16384);
+  k_init_small<T><<<nb, 256, 0, s>>>(g, st);
+  int nb2 = std::min((g.n + 3) / 4, 16384);
+  k_init_wide<T><<<nb2, 256, 0, s>>>(g, st);
+}
+
+// ---------------------------------------------------------------------------
+// decide-and-move, sub-wave ring kernels (degree < 32)
+// Each group of W lanes owns one vertex, lane l holds entry l.  A xor
+// exchange over every other lane of the group sums the weight of equal
+// communities; the lowest lane of each community ("leader") evaluates its
+// gain once, and an xor argmax selects the best (ties -> lowest id).
+// (build_and_select_in_warp, kernel_functions.cu:289-441)
+// ---------------------------------------------------------------------------
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> st,
+                                                      const int32_t* __restrict__ front,
+                                                      const int32_t* __restrict__ bstart, int bin,
+                                                      typename T::tot_t m2) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  __shared__ unsigned long long s_moved, s_swept, s_edges;
+  if (threadIdx.x == 0) { s_moved = 0; s_swept = 0; s_edges = 0; }
+  __syncthreads();
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & (W - 1);
+  const int groups = gridDim.x * (blockDim.x / W);
+  unsigned moved = 0, swept = 0, edges = 0;
+  for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W; i < hi; i += groups) {
+    const int v = front[i];
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    const int c = st.prev[v];
+    int cl = -1;
+    acc wl = 0;
+    if (lane < d) {
+      cl = st.prev[g.nbr[o + lane]];
+      wl = (acc)load_w<T>(g.w, o + lane);
+    }
+    acc s = wl, wc = (cl == c) ? wl : acc(0);
+    bool leader = cl >= 0;
+#pragma unroll
+    for (int k = 1; k < W; k++) {
+      int co = __shfl_xor(cl, k, W);
+      acc wo = __shfl_xor(wl, k, W);
+      if (co == cl) {
+        s += wo;
+        if ((lane ^ k) < lane) leader = false;
+      }
+      if (co == c) wc += wo;
+    }
+    const tt K = st.K[v];
+    const tt Tc = st.Tot[c];
+    G h = 0;
+    int bd = INT_MAX;
+    if (leader && cl != c) {
+      G hh = (G)s * (G)m2 - (G)K * ((G)st.Tot[cl] - (G)Tc + (G)K);
+      if (hh > 0) { h = hh; bd = cl; }
+    }
+    argmax_xor<G, W>(h, bd);
+    if (lane == 0) {
+      swept++;
+      edges += d;
+      if (commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize))
+        moved++;
+    }
+  }
+  if (lane == 0) {
+    if (moved) atomicAdd(&s_moved, (unsigned long long)moved);
+    if (swept) atomicAdd(&s_swept, (unsigned long long)swept);
+    if (edges) atomicAdd(&s_edges, (unsigned long long)edges);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, s_moved);
+    atomic_add_ull(&st.ctr->swept, s_swept);
+    atomic_add_ull(&st.ctr->swept_edges, s_edges);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// decide-and-move, one wave per vertex with a per-wave LDS hash (32..511)
+// Phase 1 inserts (community -> summed weight) with LDS CAS/add; phase 2 scans
+// the table so each distinct community is evaluated once (one Tot gather per
+// community rather than per entry).  (build_and_select, kernel_functions.cu:473-593)
+// ---------------------------------------------------------------------------
+template <int TMAX, class T>
+__global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> st,
+                                                      const int32_t* __restrict__ front,
+                                                      const int32_t* __restrict__ bstart, int bin,
+                                                      typename T::tot_t m2) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  __shared__ int s_keys[4][TMAX];
+  __shared__ acc s_vals[4][TMAX];
+  __shared__ unsigned long long s_moved, s_swept, s_edges;
+  if (threadIdx.x == 0) { s_moved = 0; s_swept = 0; s_edges = 0; }
+  __syncthreads();
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int* keys = s_keys[wid];
+  acc* vals = s_vals[wid];
+  const int nwaves = gridDim.x * 4;
+  unsigned moved = 0, swept = 0, edges = 0;
+  for (int i = lo + blockIdx.x * 4 + wid; i < hi; i += nwaves) {
+    const int v = front[i];
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    const int c = st.prev[v];
+    int lg = ilog2_ceil(2 * d);
+    if (lg < 6) lg = 6;
+    const int tsz = 1 << lg;  // <= TMAX by bin construction
+    for (int t = lane; t < tsz; t += 64) { keys[t] = 0; vals[t] = 0; }
+    lds_fence_wave();
+    for (int j = lane; j < d; j += 64) {
+      const int cl = st.prev[g.nbr[o + j]];
+      const acc w = (acc)load_w<T>(g.w, o + j);
+      unsigned p = hash_slot(cl, lg);
+      for (int probe = 0; probe < tsz; probe++) {
+        int k = atomicCAS(&keys[p], 0, cl + 1);
+        if (k == 0 || k == cl + 1) { atomicAdd(&vals[p], w); break; }
+        p = (p + 1) & (tsz - 1);
+      }
+    }
+    lds_fence_wave();
+    const tt K = st.K[v];
+    const tt Tc = st.Tot[c];
+    G h = 0;
+    int bd = INT_MAX;
+    acc wc = 0;
+    for (int t = lane; t < tsz; t += 64) {
+      const int k = keys[t];
+      if (k) {
+        const int dd = k - 1;
+        const acc a = vals[t];
+        if (dd == c) {
+          wc = a;
+        } else {
+          G hh = (G)a * (G)m2 - (G)K * ((G)st.Tot[dd] - (G)Tc + (G)K);
+          if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; }
+        }
+      }
+    }
+    argmax_xor<G, 64>(h, bd);
+    wc = sum_xor<acc, 64>(wc);
+    if (lane == 0) {
+      swept++;
+      edges += d;
+      if (commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize))
+        moved++;
+    }
+    lds_fence_wave();
+  }
+  if (lane == 0) {
+    if (moved) atomicAdd(&s_moved, (unsigned long long)moved);
+    if (swept) atomicAdd(&s_swept, (unsigned long long)swept);
+    if (edges) atomicAdd(&s_edges, (unsigned long long)edges);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, s_moved);
+    atomic_add_ull(&st.ctr->swept, s_swept);
+    atomic_add_ull(&st.ctr->swept_edges, s_edges);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// decide-and-move, one workgroup per vertex (512 and up).  TMAX > 0: LDS hash
+// of up to TMAX slots; TMAX == 0: the vertex's slice of the global hash arena
+// (hub_off, sized next_pow2(2*deg) per hub per round -- no overlap, no drop).
+// (build_and_select_blk, kernel_functions.cu:640-813 and :815-887)
+// ---------------------------------------------------------------------------
+template <int NT, int TMAX, class T>
+__global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> st,
+                                                      const int32_t* __restrict__ front,
+                                                      const int32_t* __restrict__ bstart, int bin,
+                                                      typename T::tot_t m2) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  constexpr int TL = TMAX > 0 ? TMAX : 1;
+  __shared__ int s_keys[TL];
+  __shared__ acc s_vals[TL];
+  __shared__ G r_h[NT / 64];
+  __shared__ int r_d[NT / 64];
+  __shared__ acc r_wc[NT / 64];
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned moved = 0, swept = 0;
+  unsigned long long edges = 0;
+  for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
+    const int v = front[i];
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    const int c = st.prev[v];
+    int lg = ilog2_ceil(2 * d);
+    const int tsz = 1 << lg;
+    int* keys;
+    acc* vals;
+    if (TMAX > 0) {
+      keys = s_keys;
+      vals = s_vals;
+    } else {
+      keys = st.hub_keys + st.hub_off[v];
+      vals = st.hub_vals + st.hub_off[v];
+    }
+    for (int t = threadIdx.x; t < tsz; t += NT) { keys[t] = 0; vals[t] = 0; }
+    __syncthreads();
+    for (int j = threadIdx.x; j < d; j += NT) {
+      const int cl = st.prev[g.nbr[o + j]];
+      const acc w = (acc)load_w<T>(g.w, o + j);
+      unsigned p = hash_slot(cl, lg);
+      int probe = 0;
+      for (; probe < tsz; probe++) {
+        int k = atomicCAS(&keys[p], 0, cl + 1);
+        if (k == 0 || k == cl + 1) { atomicAdd(&vals[p], w); break; }
+        p = (p + 1) & (tsz - 1);
+      }
+      if (probe == tsz) st.ctr->overflow = 1;
+    }
+    __syncthreads();
+    const tt K = st.K[v];
+    const tt Tc = st.Tot[c];
+    G h = 0;
+    int bd = INT_MAX;
+    acc wc = 0;
+    for (int t = threadIdx.x; t < tsz; t += NT) {
+      const int k = keys[t];
+      if (k) {
+        const int dd = k - 1;
+        const acc a = vals[t];
+        if (dd == c) {
+          wc = a;
+        } else {
+          G hh = (G)a * (G)m2 - (G)K * ((G)st.Tot[dd] - (G)Tc + (G)K);
+          if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; }
+        }
+      }
+    }
+    argmax_xor<G, 64>(h, bd);
+    wc = sum_xor<acc, 64>(wc);
+    if (lane == 0) { r_h[wid] = h; r_d[wid] = bd; r_wc[wid] = wc; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 1; q < NT / 64; q++) {
+        if (better(r_h[q], r_d[q], h, bd)) { h = r_h[q]; bd = r_d[q]; }
+        wc += r_wc[q];
+      }
+      swept++;
+      edges += d;
+      if (commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize))
+        moved++;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, moved);
+    atomic_add_ull(&st.ctr->swept, swept);
+    atomic_add_ull(&st.ctr->swept_edges, edges);
+  }
+}
+
+static int grid_for(int cap, int per_block, int max_blocks) {
+  int nb = (cap + per_block - 1) / per_block;
+  if (nb < 1) nb = 1;
+  return std::min(nb, max_blocks);
+}
+
+template <class T>
+void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
+                   const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s) {
+  if (cap <= 0) return;
+  switch (bin) {
+    case 0: k_decide_ring<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, 0, m2); break;
+    case 1: k_decide_ring<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, 1, m2); break;
+    case 2: k_decide_ring<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, 2, m2); break;
+    case 3: k_decide_ring<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, 3, m2); break;
+    case 4: k_decide_wave<256, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, 4, m2); break;
+    case 5: k_decide_wave<1024, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, 5, m2); break;
+    case 6: k_decide_block<256, 8192, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, 6, m2); break;
+    case 7:
+      if constexpr (T::FP)
+        k_decide_block<1024, 0, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, 7, m2);
+      else
+        k_decide_block<1024, 16384, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, 7, m2);
+      break;
+    case 8: k_decide_block<1024, 0, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, 8, m2); break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// apply: Tot += dTot, size += dsize (louvain_main_process.cu:949-953),
+// min non-zero Tot (Tot_op, kernel_functions.cuh:112-121), sum Tot^2, non-empty count.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_apply(DevState<T> st, int32_t n, typename T::tot_t m2) {
+  using tt = typename T::tot_t;
+  __shared__ unsigned long long sh_t2[4];
+  __shared__ double sh_t2f[4];
+  __shared__ unsigned long long sh_min[4];
+  __shared__ int sh_ne[4];
+  unsigned long long t2 = 0, mn;
+  double t2f = 0;
+  if (T::FP) {
+    double m2d = (double)m2;
+    mn = __double_as_longlong(m2d);
+  } else {
+    mn = (unsigned long long)(long long)m2;
+  }
+  int ne = 0;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    tt t = st.Tot[c] + st.dTot[c];
+    int sz = st.size[c] + st.dsize[c];
+    st.Tot[c] = t;
+    st.size[c] = sz;
+    st.dTot[c] = 0;
+    st.dsize[c] = 0;
+    ne += sz > 0;
+    if (T::FP) {
+      double tf = (double)t;
+      t2f += tf * tf;
+      if (tf > 0) {
+        unsigned long long b = (unsigned long long)__double_as_longlong(tf);
+        mn = b < mn ? b : mn;
+      }
+    } else {
+      long long ti = (long long)t;
+      t2 += (unsigned long long)(ti * ti);
+      if (ti != 0 && (unsigned long long)ti < mn) mn = (unsigned long long)ti;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  t2 = sum_xor<unsigned long long, 64>(t2);
+  t2f = sum_xor<double, 64>(t2f);
+  ne = sum_xor<int, 64>(ne);
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) {
+    unsigned long long o = __shfl_xor(mn, k, 64);
+    mn = o < mn ? o : mn;
+  }
+  if (lane == 0) { sh_t2[wid] = t2; sh_t2f[wid] = t2f; sh_min[wid] = mn; sh_ne[wid] = ne; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < 4; q++) {
+      t2 += sh_t2[q]; t2f += sh_t2f[q]; ne += sh_ne[q];
+      mn = sh_min[q] < mn ? sh_min[q] : mn;
+    }
+    if (T::FP) {
+      atomicAdd(&st.ctr->sum_tot2_f, t2f);
+      atomicMin((unsigned long long*)&st.ctr->min_tot_f, mn);
+    } else {
+      atomic_add_ull(&st.ctr->sum_tot2, t2);
+      atomicMin((unsigned long long*)&st.ctr->min_tot, mn);
+    }
+    if (ne) atomicAdd(&st.ctr->nonempty, ne);
+  }
+}
+
+template <class T>
+void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s) {
+  if (n <= 0) return;
+  int nb = std::min((n + 255) / 256, 4096);
+  k_apply<T><<<nb, 256, 0, s>>>(st, n, m2);
+}
+
+// ---------------------------------------------------------------------------
+// weight update: In[v] = sum w over cur[u]==cur[v], prune flag
+// (compute_In / compute_In_blk, kernel_functions.cu:889-1034).  The prune
+// m2*(2In - S - K) + K*(minTot - (Tot_c - K)) > 0 proves every gain < 0.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ uint8_t prune_keep(typename T::tot_t in, typename T::tot_t K,
+                                              typename T::tot_t S, typename T::tot_t Tc,
+                                              typename T::tot_t minTot, typename T::tot_t m2) {
+  using G = typename T::gain_t;
+  G P = (G)m2 * (2 * (G)in - (G)S - (G)K) + (G)K * ((G)minTot - ((G)Tc - (G)K));
+  return P > 0 ? 0 : 1;
+}
+
+template <class T>
+__device__ __forceinline__ typename T::tot_t read_min_tot(const Counters* c) {
+  if (T::FP) return (typename T::tot_t)c->min_tot_f;
+  return (typename T::tot_t)c->min_tot;
+}
+
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_weight_group(DevGraph<T> g, DevState<T> st,
+                                                       const int32_t* __restrict__ order,
+                                                       const int32_t* __restrict__ bstart, int bin,
+                                                       typename T::tot_t m2) {
+  using tt = typename T::tot_t;
+  __shared__ unsigned long long s_in, s_act;
+  __shared__ double s_inf;
+  if (threadIdx.x == 0) { s_in = 0; s_act = 0; s_inf = 0; }
+  __syncthreads();
+  const tt minTot = read_min_tot<T>(st.ctr);
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & (W - 1);
+  const int groups = gridDim.x * (blockDim.x / W);
+  tt insum = 0;
+  unsigned act = 0;
+  for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W; i < hi; i += groups) {
+    const int v = order[i];
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    const int c = st.cur[v];
+    tt x = 0;
+    for (int j = lane; j < d; j += W)
+      if (st.cur[g.nbr[o + j]] == c) x += (tt)load_w<T>(g.w, o + j);
+    x = sum_xor<tt, W>(x);
+    if (lane == 0) {
+      st.In[v] = x;
+      uint8_t keep = prune_keep<T>(x, st.K[v], st.Self[v], st.Tot[c], minTot, m2);
+      st.active[v] = keep;
+      act += keep;
+      insum += x;
+    }
+  }
+  if (lane == 0) {
+    if (T::FP) { if (insum != 0) atomicAdd(&s_inf, (double)insum); }
+    else if (insum) atomicAdd(&s_in, (unsigned long long)(long long)insum);
+    if (act) atomicAdd(&s_act, (unsigned long long)act);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (T::FP) { if (s_inf != 0) atomicAdd(&st.ctr->sum_in_f, s_inf); }
+    else atomic_add_ull(&st.ctr->sum_in, s_in);
+    atomic_add_ull(&st.ctr->active, s_act);
+  }
+}
+
+template <int NT, class T>
+__global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> st,
+                                                      const int32_t* __restrict__ order,
+                                                      const int32_t* __restrict__ bstart, int bin,
+                                                      typename T::tot_t m2) {
+  using tt = typename T::tot_t;
+  __shared__ tt r[NT / 64];
+  const tt minTot = read_min_tot<T>(st.ctr);
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  tt insum = 0;
+  unsigned act = 0;
+  for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
+    const int v = order[i];
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    const int c = st.cur[v];
+    tt x = 0;
+    for (int j = threadIdx.x; j < d; j += NT)
+      if (st.cur[g.nbr[o + j]] == c) x += (tt)load_w<T>(g.w, o + j);
+    x = sum_xor<tt, 64>(x);
+    if (lane == 0) r[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      x = 0;
+      for (int q = 0; q < NT / 64; q++) x += r[q];
+      st.In[v] = x;
+      uint8_t keep = prune_keep<T>(x, st.K[v], st.Self[v], st.Tot[c], minTot, m2);
+      st.active[v] = keep;
+      act += keep;
+      insum += x;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (T::FP) { if (insum != 0) atomicAdd(&st.ctr->sum_in_f, (double)insum); }
+    else if (insum) atomicAdd(&st.ctr->sum_in, (unsigned long long)(long long)insum);
+    if (act) atomicAdd(&st.ctr->active, (unsigned long long)act);
+  }
+}
+
+template <class T>
+void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
+                   const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s) {
+  if (cap <= 0) return;
+  switch (bin) {
+    case 0: k_weight_group<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, order, bstart, 0, m2); break;
+    case 1: k_weight_group<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, order, bstart, 1, m2); break;
+    case 2: k_weight_group<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, order, bstart, 2, m2); break;
+    case 3: k_weight_group<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, order, bstart, 3, m2); break;
+    case 4: k_weight_group<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, order, bstart, 4, m2); break;
+    case 5: k_weight_group<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, order, bstart, 5, m2); break;
+    case 6: k_weight_block<256, T><<<grid_for(cap, 1, 4096), 256, 0, s>>>(g, st, order, bstart, 6, m2); break;
+    case 7: k_weight_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, order, bstart, 7, m2); break;
+    case 8: k_weight_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, order, bstart, 8, m2); break;
+  }
+}
+
+// explicit instantiations
+#define GALA_INST(T)                                                                              \
+  template void launch_init<T>(const DevGraph<T>&, DevState<T>&, hipStream_t);                    \
+  template void launch_decide<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,           \
+                                 const int32_t*, int, typename T::tot_t, hipStream_t);            \
+  template void launch_apply<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t);           \
+  template void launch_weight<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,           \
+                                 const int32_t*, int, typename T::tot_t, hipStream_t);
+GALA_INST(IntW)
+GALA_INST(FltW)
+
+}  // namespace gala

@@ -1,0 +1,363 @@
+"""Python host mirror of the gfx950 Louvain engine (ctypes over libgala.so).
+
+Mirrors the reference's in-process interface (LinXi-lx/GALA-mg):
+  * ``Graph`` with ``load_bin_graph`` / ``store_bin_graph``   (src/graph/graph.h:7-19)
+  * ``louvain_gpu(graph, min_modularity)`` -> (Q, communities) (src/louvain_gpu/louvain.cuh:8)
+plus phase-level calls (decide, weight update, contraction) for parity tests
+and the synthetic generators.  Every call goes through the C ABI declared in
+include/gala.h; there is no CPU fallback -- if libgala.so is missing or has no
+device, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgala.so")
+
+W_NONE, W_I32, W_F32 = 0, 1, 2
+
+EXPORTS = [
+    "gala_version", "gala_strerror", "gala_default_opts", "gala_create", "gala_destroy",
+    "gala_load_bin", "gala_set_graph", "gala_run", "gala_partial_result",
+    "gala_get_communities", "gala_phase_decide", "gala_phase_weight", "gala_phase_contract",
+    "gala_read_bin", "gala_write_bin", "gala_ingest_text", "gala_gen_sbm", "gala_gen_chung_lu",
+    "gala_gen_rmat",
+]
+
+
+class GalaError(RuntimeError):
+    pass
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("min_modularity", ctypes.c_double), ("max_iterations", ctypes.c_int),
+                ("max_rounds", ctypes.c_int), ("dump_iterations", ctypes.c_int),
+                ("dump_rounds", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("frontier_mode", ctypes.c_int), ("learned_tau", ctypes.c_float),
+                ("reserved", ctypes.c_int * 8)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("modularity", ctypes.c_double), ("n_vertices", ctypes.c_int32),
+                ("n_communities", ctypes.c_int32), ("rounds", ctypes.c_int32),
+                ("total_iterations", ctypes.c_int32), ("t_total_ms", ctypes.c_double),
+                ("t_decide_ms", ctypes.c_double), ("t_update_ms", ctypes.c_double),
+                ("t_contract_ms", ctypes.c_double), ("t_h2d_ms", ctypes.c_double),
+                ("swept_vertices", ctypes.c_int64), ("swept_edges", ctypes.c_int64),
+                ("input_edges", ctypes.c_int64), ("swept_vertices_unit", ctypes.c_int64),
+                ("swept_edges_unit", ctypes.c_int64), ("round_q", ctypes.c_double * 64),
+                ("round_nc", ctypes.c_int32 * 64), ("round_iters", ctypes.c_int32 * 64),
+                ("round_ms", ctypes.c_double * 64), ("round_decide_ms", ctypes.c_double * 64),
+                ("round_update_ms", ctypes.c_double * 64), ("round_build_ms", ctypes.c_double * 64)]
+
+    def to_dict(self):
+        r = self.rounds
+        return dict(modularity=self.modularity, n_vertices=self.n_vertices,
+                    n_communities=self.n_communities, rounds=r,
+                    total_iterations=self.total_iterations, t_total_ms=self.t_total_ms,
+                    t_decide_ms=self.t_decide_ms, t_update_ms=self.t_update_ms,
+                    t_contract_ms=self.t_contract_ms, t_h2d_ms=self.t_h2d_ms,
+                    swept_vertices=self.swept_vertices, swept_edges=self.swept_edges,
+                    input_edges=self.input_edges, swept_vertices_unit=self.swept_vertices_unit,
+                    swept_edges_unit=self.swept_edges_unit, round_q=list(self.round_q[:r]),
+                    round_nc=list(self.round_nc[:r]), round_iters=list(self.round_iters[:r]),
+                    round_ms=list(self.round_ms[:r]),
+                    round_decide_ms=list(self.round_decide_ms[:r]),
+                    round_update_ms=list(self.round_update_ms[:r]),
+                    round_build_ms=list(self.round_build_ms[:r]))
+
+
+ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                           ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32)
+ROUND_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int32,
+                            ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
+                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
+                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double))
+
+_lib = None
+I64P = ctypes.POINTER(ctypes.c_int64)
+I32P = ctypes.POINTER(ctypes.c_int32)
+U8P = ctypes.POINTER(ctypes.c_uint8)
+VP = ctypes.c_void_p
+
+
+def lib():
+    """Load libgala.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GalaError(f"{LIB_PATH} not built: run `make -C gala-mg_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.gala_version.restype = ctypes.c_char_p
+        L.gala_strerror.restype = ctypes.c_char_p
+        L.gala_strerror.argtypes = [ctypes.c_int]
+        L.gala_default_opts.argtypes = [ctypes.POINTER(Opts)]
+        L.gala_create.argtypes = [ctypes.POINTER(VP), ctypes.c_int, ctypes.POINTER(Opts)]
+        L.gala_destroy.argtypes = [VP]
+        L.gala_load_bin.argtypes = [VP, ctypes.c_char_p, ctypes.c_int]
+        L.gala_set_graph.argtypes = [VP, ctypes.c_int32, I64P, I32P, VP, ctypes.c_int]
+        L.gala_run.argtypes = [VP, ITER_CB, ROUND_CB, VP, ctypes.POINTER(Result)]
+        L.gala_partial_result.argtypes = [VP, ctypes.POINTER(Result)]
+        L.gala_get_communities.argtypes = [VP, I32P]
+        L.gala_phase_decide.argtypes = [VP, ctypes.c_int32, I64P, I32P, I32P, ctypes.c_int64, I32P,
+                                        I32P, I32P, I32P, I32P, U8P, I32P]
+        L.gala_phase_weight.argtypes = [VP, ctypes.c_int32, I64P, I32P, I32P, ctypes.c_int64, I32P,
+                                        I32P, I32P, I32P, ctypes.c_int64, I32P, U8P]
+        L.gala_phase_contract.argtypes = [VP, ctypes.c_int32, I64P, I32P, I32P, I32P, I32P, I64P,
+                                          I32P, I64P, I32P, I64P]
+        L.gala_read_bin.argtypes = [ctypes.c_char_p, ctypes.c_int, I32P, I64P, I64P, I32P, VP]
+        L.gala_write_bin.argtypes = [ctypes.c_char_p, ctypes.c_int32, I64P, I32P, VP, ctypes.c_int]
+        L.gala_ingest_text.argtypes = [ctypes.c_char_p, ctypes.c_int, I32P, I64P, I64P, I32P, VP]
+        L.gala_gen_sbm.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_uint64, I32P, I64P, I64P, I32P]
+        L.gala_gen_chung_lu.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_uint64, I32P, I64P, I64P, I32P]
+        L.gala_gen_rmat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_uint64, ctypes.c_float,
+                                    ctypes.c_float, I32P, I64P, I64P, I32P,
+                                    ctypes.POINTER(ctypes.c_float)]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise GalaError(f"{what}: {lib().gala_strerror(rc).decode()} ({rc})")
+
+
+def _p(a, ct):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+# ---------------------------------------------------------------------------
+# Graph (src/graph/graph.h:7-19)
+# ---------------------------------------------------------------------------
+class Graph:
+    """CSR graph in the reference layout: ``degrees`` are inclusive row ends."""
+
+    def __init__(self, degrees=None, neighbors=None, weights=None, weight_mode=W_NONE):
+        self.degrees = np.zeros(0, np.int64) if degrees is None else np.ascontiguousarray(degrees, np.int64)
+        self.neighbors = np.zeros(0, np.int32) if neighbors is None else np.ascontiguousarray(neighbors, np.int32)
+        self.weight_mode = weight_mode
+        if weights is None:
+            self.weights = None
+        else:
+            self.weights = np.ascontiguousarray(weights, np.float32 if weight_mode == W_F32 else np.int32)
+
+    @property
+    def vertex_num(self):
+        return len(self.degrees)
+
+    @property
+    def edge_num(self):
+        return int(self.degrees[-1]) // 2 if len(self.degrees) else 0
+
+    @property
+    def is_weighted(self):
+        return self.weight_mode != W_NONE
+
+    def offsets(self):
+        return np.concatenate([[0], self.degrees]).astype(np.int64)
+
+    @staticmethod
+    def load_bin_graph(path, weight_mode=W_NONE):
+        """Graph::load_bin_graph (graph.cpp:177-219)."""
+        L = lib()
+        n, m = ctypes.c_int32(), ctypes.c_int64()
+        _check(L.gala_read_bin(path.encode(), weight_mode, ctypes.byref(n), ctypes.byref(m), None, None,
+                               None), f"read {path}")
+        end = np.zeros(max(n.value, 1), np.int64)
+        nbr = np.zeros(max(m.value, 1), np.int32)
+        w = None
+        if weight_mode != W_NONE:
+            w = np.zeros(max(m.value, 1), np.float32 if weight_mode == W_F32 else np.int32)
+        _check(L.gala_read_bin(path.encode(), weight_mode, ctypes.byref(n), ctypes.byref(m),
+                               _p(end, ctypes.c_int64), _p(nbr, ctypes.c_int32),
+                               None if w is None else w.ctypes.data_as(VP)), f"read {path}")
+        return Graph(end[:n.value], nbr[:m.value], None if w is None else w[:m.value], weight_mode)
+
+    def store_bin_graph(self, path):
+        """Graph::store_bin_graph (graph.cpp:153-175)."""
+        w = self.weights
+        _check(lib().gala_write_bin(path.encode(), self.vertex_num, _p(self.degrees, ctypes.c_int64),
+                                    _p(self.neighbors, ctypes.c_int32),
+                                    None if w is None else w.ctypes.data_as(VP), self.weight_mode),
+               f"write {path}")
+
+    @staticmethod
+    def from_text(path, weight_mode=W_NONE):
+        """Graph::Graph(file, is_wt) (graph.cpp:18-151), i.e. preprocess's ingest."""
+        L = lib()
+        n, m = ctypes.c_int32(), ctypes.c_int64()
+        _check(L.gala_ingest_text(path.encode(), weight_mode, ctypes.byref(n), ctypes.byref(m), None,
+                                  None, None), f"ingest {path}")
+        end = np.zeros(max(n.value, 1), np.int64)
+        nbr = np.zeros(max(m.value, 1), np.int32)
+        w = None if weight_mode == W_NONE else np.zeros(max(m.value, 1),
+                                                          np.float32 if weight_mode == W_F32 else np.int32)
+        _check(L.gala_ingest_text(path.encode(), weight_mode, ctypes.byref(n), ctypes.byref(m),
+                                  _p(end, ctypes.c_int64), _p(nbr, ctypes.c_int32),
+                                  None if w is None else w.ctypes.data_as(VP)), f"ingest {path}")
+        return Graph(end[:n.value], nbr[:m.value], None if w is None else w[:m.value], weight_mode)
+
+
+# ---------------------------------------------------------------------------
+# generators
+# ---------------------------------------------------------------------------
+def _two_call(fn, args, with_w=False):
+    n, m = ctypes.c_int32(), ctypes.c_int64()
+    if with_w:
+        _check(fn(*args, ctypes.byref(n), ctypes.byref(m), None, None, None), fn.__name__)
+    else:
+        _check(fn(*args, ctypes.byref(n), ctypes.byref(m), None, None), fn.__name__)
+    end = np.zeros(max(n.value, 1), np.int64)
+    nbr = np.zeros(max(m.value, 1), np.int32)
+    if with_w:
+        w = np.zeros(max(m.value, 1), np.float32)
+        _check(fn(*args, ctypes.byref(n), ctypes.byref(m), _p(end, ctypes.c_int64),
+                  _p(nbr, ctypes.c_int32), _p(w, ctypes.c_float)), fn.__name__)
+        return Graph(end[:n.value], nbr[:m.value], w[:m.value], W_F32)
+    _check(fn(*args, ctypes.byref(n), ctypes.byref(m), _p(end, ctypes.c_int64), _p(nbr, ctypes.c_int32)),
+           fn.__name__)
+    return Graph(end[:n.value], nbr[:m.value])
+
+
+def gen_sbm(n, blocks, p_in, p_out, seed):
+    return _two_call(lib().gala_gen_sbm, (n, blocks, p_in, p_out, seed))
+
+
+def gen_chung_lu(n, gamma=2.1, mean_deg=17.35, max_deg=15000.0, seed=0):
+    return _two_call(lib().gala_gen_chung_lu, (n, gamma, mean_deg, max_deg, seed))
+
+
+def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=0, w_lo=0.5, w_hi=2.0):
+    return _two_call(lib().gala_gen_rmat, (scale, edge_factor, a, b, c, seed, w_lo, w_hi), with_w=True)
+
+
+# ---------------------------------------------------------------------------
+# engine
+# ---------------------------------------------------------------------------
+class Engine:
+    """One device context (gala_create ... gala_destroy)."""
+
+    def __init__(self, device=0, min_modularity=1e-6, dump_iterations=False, dump_rounds=False,
+                 max_rounds=0, max_iterations=0):
+        L = lib()
+        self.opts = Opts()
+        L.gala_default_opts(ctypes.byref(self.opts))
+        self.opts.min_modularity = min_modularity
+        self.opts.dump_iterations = int(dump_iterations)
+        self.opts.dump_rounds = int(dump_rounds)
+        self.opts.max_rounds = max_rounds
+        self.opts.max_iterations = max_iterations
+        self.ctx = VP()
+        _check(L.gala_create(ctypes.byref(self.ctx), device, ctypes.byref(self.opts)), "gala_create")
+
+    def close(self):
+        if self.ctx:
+            lib().gala_destroy(self.ctx)
+            self.ctx = VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_graph(self, g):
+        w = g.weights
+        _check(lib().gala_set_graph(self.ctx, g.vertex_num, _p(g.degrees, ctypes.c_int64),
+                                    _p(g.neighbors, ctypes.c_int32),
+                                    None if w is None else w.ctypes.data_as(VP), g.weight_mode),
+               "gala_set_graph")
+
+    def load_bin(self, path, weight_mode=W_NONE):
+        _check(lib().gala_load_bin(self.ctx, path.encode(), weight_mode), "gala_load_bin")
+
+    def run(self, trace=None, rounds=None):
+        """gala_run.  trace: list receiving (round, it, Q, moved, cur|None);
+        rounds: list receiving dicts with the contracted CSR when dump_rounds."""
+        def icb(user, rnd, it, q, moved, cur, n):
+            if trace is not None:
+                vec = np.ctypeslib.as_array(cur, shape=(n,)).copy() if cur else None
+                trace.append((rnd, it, q, int(moved), vec))
+
+        def rcb(user, rnd, q, n, comm, C, end, nbr, wi, wf):
+            if rounds is not None:
+                d = dict(round=rnd, q=q, n=n, C=C,
+                         comm=np.ctypeslib.as_array(comm, shape=(n,)).copy())
+                if end:
+                    d["end"] = np.ctypeslib.as_array(end, shape=(max(C, 1),))[:C].copy()
+                    m = int(d["end"][-1]) if C else 0
+                    d["nbr"] = np.ctypeslib.as_array(nbr, shape=(max(m, 1),))[:m].copy()
+                    if wi:
+                        d["w"] = np.ctypeslib.as_array(wi, shape=(max(m, 1),))[:m].copy()
+                    if wf:
+                        d["w"] = np.ctypeslib.as_array(wf, shape=(max(m, 1),))[:m].copy()
+                rounds.append(d)
+
+        self._icb, self._rcb = ITER_CB(icb), ROUND_CB(rcb)
+        res = Result()
+        _check(lib().gala_run(self.ctx, self._icb, self._rcb, None, ctypes.byref(res)), "gala_run")
+        return res.to_dict()
+
+    def communities(self, n):
+        out = np.zeros(max(n, 1), np.int32)
+        _check(lib().gala_get_communities(self.ctx, _p(out, ctypes.c_int32)), "gala_get_communities")
+        return out[:n]
+
+    # ---- phase-level ----
+    def phase_decide(self, g, m2, K, Self, Tot, size, prev, frontier=None):
+        n = g.vertex_num
+        cur = np.zeros(n, np.int32)
+        a = [np.ascontiguousarray(x, np.int32) for x in (K, Self, Tot, size, prev)]
+        fr = None if frontier is None else np.ascontiguousarray(frontier, np.uint8)
+        w = None if g.weights is None else np.ascontiguousarray(g.weights, np.int32)
+        _check(lib().gala_phase_decide(self.ctx, n, _p(g.degrees, ctypes.c_int64),
+                                       _p(g.neighbors, ctypes.c_int32), _p(w, ctypes.c_int32), int(m2),
+                                       *[_p(x, ctypes.c_int32) for x in a], _p(fr, ctypes.c_uint8),
+                                       _p(cur, ctypes.c_int32)), "gala_phase_decide")
+        return cur
+
+    def phase_weight(self, g, m2, K, Self, Tot, cur, min_tot):
+        n = g.vertex_num
+        In = np.zeros(n, np.int32)
+        act = np.zeros(n, np.uint8)
+        a = [np.ascontiguousarray(x, np.int32) for x in (K, Self, Tot, cur)]
+        w = None if g.weights is None else np.ascontiguousarray(g.weights, np.int32)
+        _check(lib().gala_phase_weight(self.ctx, n, _p(g.degrees, ctypes.c_int64),
+                                       _p(g.neighbors, ctypes.c_int32), _p(w, ctypes.c_int32), int(m2),
+                                       *[_p(x, ctypes.c_int32) for x in a], int(min_tot),
+                                       _p(In, ctypes.c_int32), _p(act, ctypes.c_uint8)),
+               "gala_phase_weight")
+        return In, act
+
+    def phase_contract(self, g, comm):
+        n = g.vertex_num
+        comm = np.ascontiguousarray(comm, np.int32)
+        w = None if g.weights is None else np.ascontiguousarray(g.weights, np.int32)
+        C, m = ctypes.c_int32(), ctypes.c_int64()
+        nid = np.zeros(n, np.int32)
+        args = (self.ctx, n, _p(g.degrees, ctypes.c_int64), _p(g.neighbors, ctypes.c_int32),
+                _p(w, ctypes.c_int32), _p(comm, ctypes.c_int32), ctypes.byref(C), ctypes.byref(m))
+        _check(lib().gala_phase_contract(*args, _p(nid, ctypes.c_int32), None, None, None),
+               "gala_phase_contract")
+        end = np.zeros(max(C.value, 1), np.int64)
+        nbr = np.zeros(max(m.value, 1), np.int32)
+        ww = np.zeros(max(m.value, 1), np.int64)
+        _check(lib().gala_phase_contract(*args, None, _p(end, ctypes.c_int64), _p(nbr, ctypes.c_int32),
+                                         _p(ww, ctypes.c_int64)), "gala_phase_contract")
+        return nid, C.value, end[:C.value], nbr[:m.value], ww[:m.value]
+
+
+def louvain_gpu(graph, min_modularity=1e-6, device=0):
+    """louvain_gpu (louvain.cu:136-229): returns (final Q, composed communities)."""
+    eng = Engine(device=device, min_modularity=min_modularity)
+    try:
+        eng.set_graph(graph)
+        res = eng.run()
+        return res["modularity"], eng.communities(graph.vertex_num)
+    finally:
+        eng.close()

@@ -1,0 +1,176 @@
+/*
+ * gala.h -- C ABI of the gfx950 Louvain engine (libgala.so).
+ *
+ * Drop-in boundary for the single-GPU path of LinXi-lx/GALA-mg.  The reference
+ * has no plugin/FFI layer; its in-process interface is C++ over thrust vectors:
+ *   double louvain_gpu(Graph&, vertex_t* h_community, double min_modularity,
+ *                      int gpu_id, int gpu_num)          src/louvain_gpu/louvain.cuh:8
+ *   louvain_main_process(...)  (phase 1, one round)       src/louvain_gpu/louvain.cuh:10-17
+ *   build_compressed_graph(...) (phase 2, contraction)    src/louvain_gpu/louvain.cuh:19-22
+ *   Graph::load_bin_graph / store_bin_graph (.bin format) src/graph/graph.cpp:153-219
+ * Each entry point below names the reference interface it replaces.
+ *
+ * Conventions: every function returns 0 on success or a negative GALA_E_* code;
+ * the caller owns all host buffers, the context owns device memory; a context
+ * is single-threaded (independent contexts may run concurrently).  No C++ or
+ * HIP types cross this boundary.
+ */
+#ifndef GALA_H
+#define GALA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GALA_OK 0
+#define GALA_E_IO -1        /* file missing / unreadable / short read */
+#define GALA_E_FORMAT -2    /* malformed .bin or CSR (offsets, ids, sizes) */
+#define GALA_E_OOM -3       /* device or host allocation failed */
+#define GALA_E_HIP -4       /* HIP runtime error / no device */
+#define GALA_E_OVERFLOW -5  /* exact-integer range exceeded (m2 > 1e9) or hash overflow */
+#define GALA_E_INVALID -6   /* bad argument or state (no graph, m2 == 0, ...) */
+
+/* weight modes of the .bin weight section (out of band, as the reference's -w flag) */
+#define GALA_W_NONE 0 /* no weight section: every weight is 1 (graph.cpp:205-212) */
+#define GALA_W_I32 1  /* int32 weights (graph.cpp:201-204) */
+#define GALA_W_F32 2  /* IEEE-754 float32 weights in the same slot (FP64 accumulation) */
+
+typedef struct gala_ctx gala_ctx;
+
+typedef struct {
+  double min_modularity; /* convergence threshold, 1e-6 in gala_main.cu:48 */
+  int max_iterations;    /* per-round sweep cap, 0 = 100000 */
+  int max_rounds;        /* 0 = until the round's dQ < min_modularity */
+  int dump_iterations;   /* 1: iteration callback receives the community vector */
+  int dump_rounds;       /* 1: round callback receives the contracted CSR */
+  int verbose;           /* 1: print the reference's stdout lines from gala_run */
+  int frontier_mode;     /* 0 plain (sound prune), 1 learned frontier + certification */
+  float learned_tau;     /* learned frontier threshold on the move probability */
+  int reserved[8];
+} gala_opts;
+
+/* Called after every iteration (Iteration:0 is Q before any move).
+ * cur is a host copy of the proposed assignment when dump_iterations=1, else NULL. */
+typedef void (*gala_iter_cb)(void* user, int round, int iteration, double q, int64_t n_moved,
+                             const int32_t* cur, int32_t n);
+
+/* Called after each round's contraction.  comm = accepted (pre-renumbering)
+ * assignment of the round graph (n entries); the contracted CSR (C rows,
+ * inclusive ends, int64 weights) is given when dump_rounds=1, else NULLs. */
+typedef void (*gala_round_cb)(void* user, int round, double q, int32_t n, const int32_t* comm,
+                              int32_t C, const int64_t* end_incl, const int32_t* nbr,
+                              const int64_t* w_i64, const double* w_f64);
+
+typedef struct {
+  double modularity;       /* final Q */
+  int32_t n_vertices;      /* input vertices */
+  int32_t n_communities;   /* final communities */
+  int32_t rounds;
+  int32_t total_iterations;/* sweeps over all rounds (including reverted ones) */
+  double t_total_ms;       /* graph resident in HBM -> final composed map on device */
+  double t_decide_ms;      /* decide-and-move */
+  double t_update_ms;      /* Tot/size apply + weight update + modularity + frontier */
+  double t_contract_ms;    /* contraction incl. composition */
+  double t_h2d_ms;         /* upload of the input graph (outside t_total_ms) */
+  int64_t swept_vertices;  /* vertices evaluated by decide-and-move */
+  int64_t swept_edges;     /* row entries read by decide-and-move */
+  int64_t input_edges;     /* undirected input edges (directed entries / 2) */
+  int64_t swept_vertices_unit; /* ...of which on graphs with implicit unit weights */
+  int64_t swept_edges_unit;
+  double round_q[64];
+  int32_t round_nc[64];
+  int32_t round_iters[64];
+  double round_ms[64];        /* phase 1 of the round (louvain_main_process) */
+  double round_decide_ms[64];
+  double round_update_ms[64];
+  double round_build_ms[64];  /* contraction of the round */
+} gala_result;
+
+/* Version string of the library. */
+const char* gala_version(void);
+const char* gala_strerror(int code);
+void gala_default_opts(gala_opts* o);
+
+/* Create a context on HIP device `device` (replaces cudaSetDevice + thrust
+ * state set-up, louvain.cu:149-173).  opts may be NULL for defaults. */
+int gala_create(gala_ctx** out, int device, const gala_opts* opts);
+void gala_destroy(gala_ctx* ctx);
+
+/* Load a reference .bin file (Graph::load_bin_graph, graph.cpp:177-219) and
+ * upload it (GraphGPU, graph_gpu.h:13-19).  weight_mode = GALA_W_*. */
+int gala_load_bin(gala_ctx* ctx, const char* path, int weight_mode);
+
+/* Same from host arrays: n vertices, end_incl[n] inclusive row ends (the .bin
+ * layout), nbr[end_incl[n-1]], w = NULL or int32/float32 per weight_mode. */
+int gala_set_graph(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr,
+                   const void* w, int weight_mode);
+
+/* Multi-round Louvain (louvain_gpu, louvain.cu:136-229 with the intended round
+ * loop of :193-226).  Callbacks may be NULL. */
+int gala_run(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala_result* out);
+
+/* Result so far (valid inside a round callback for the rounds completed). */
+int gala_partial_result(gala_ctx* ctx, gala_result* out);
+
+/* Composed final community of every input vertex (the reference's
+ * h_community output, louvain.cu:226), renumbered 0..C-1. */
+int gala_get_communities(gala_ctx* ctx, int32_t* out_n0);
+
+/* ---- phase-level entry points (host arrays in and out), for parity tests ---- */
+
+/* One synchronous decide-and-move sweep (the build_and_select* family,
+ * kernel_functions.cu:289-887) over the vertices with frontier[v] != 0
+ * (frontier NULL = all), on a given state.  int32 weights (NULL = unit). */
+int gala_phase_decide(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr,
+                      const int32_t* w, int64_t m2, const int32_t* K, const int32_t* Self,
+                      const int32_t* Tot, const int32_t* size, const int32_t* prev,
+                      const uint8_t* frontier, int32_t* cur_out);
+
+/* Weight update + prune (compute_In / compute_In_blk, kernel_functions.cu:889-1034)
+ * for all vertices: In_out[v], active_out[v]. */
+int gala_phase_weight(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr,
+                      const int32_t* w, int64_t m2, const int32_t* K, const int32_t* Self,
+                      const int32_t* Tot, const int32_t* cur, int64_t min_tot, int32_t* In_out,
+                      uint8_t* active_out);
+
+/* Contraction (build_compressed_graph, build_compressed_graph.cu:5-173).
+ * Two calls: with out arrays NULL it returns C and the entry count through
+ * out_C / out_m; then with arrays of those sizes it fills them.  new_id[n]
+ * receives the renumbering (-1 for empty communities). */
+int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr,
+                        const int32_t* w, const int32_t* comm, int32_t* out_C, int64_t* out_m,
+                        int32_t* new_id, int64_t* out_end_incl, int32_t* out_nbr, int64_t* out_w);
+
+/* ---- .bin IO and generators (host only; no device needed) ---- */
+
+/* Read a .bin: first call with NULL arrays returns n and m (directed entries). */
+int gala_read_bin(const char* path, int weight_mode, int32_t* n, int64_t* m, int64_t* end_incl,
+                  int32_t* nbr, void* w);
+/* Write a .bin (Graph::store_bin_graph, graph.cpp:153-175); w may be NULL. */
+int gala_write_bin(const char* path, int32_t n, const int64_t* end_incl, const int32_t* nbr,
+                   const void* w, int weight_mode);
+/* Text edge list -> CSR with the reference preprocess semantics (graph.cpp:18-151):
+ * '#' comments, both directions, per-row dedup in ascending order (weights summed
+ * when weighted), dense renumbering of linked vertices.  Self-loops are kept as
+ * one entry (the reference truncates them).  Two-call protocol as gala_read_bin. */
+int gala_ingest_text(const char* path, int weight_mode, int32_t* n, int64_t* m, int64_t* end_incl,
+                     int32_t* nbr, void* w);
+
+/* Synthetic generators (deterministic for a seed).  Two-call protocol: with
+ * NULL arrays they return n and m; the arrays are then filled on a second call
+ * with the same arguments.  Rows are sorted and duplicate-free. */
+int gala_gen_sbm(int32_t n, int32_t blocks, double p_in, double p_out, uint64_t seed, int32_t* out_n,
+                 int64_t* out_m, int64_t* end_incl, int32_t* nbr);
+int gala_gen_chung_lu(int32_t n, double gamma, double mean_deg, double max_deg, uint64_t seed,
+                      int32_t* out_n, int64_t* out_m, int64_t* end_incl, int32_t* nbr);
+int gala_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                  float w_lo, float w_hi, int32_t* out_n, int64_t* out_m, int64_t* end_incl,
+                  int32_t* nbr, float* w);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GALA_H */

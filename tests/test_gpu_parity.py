@@ -1,0 +1,195 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bar: bit-exact for every integer quantity -- per-iteration community vectors,
+moved counts, per-round renumbered contracted CSR (rows canonical), the
+composed final partition -- and identical doubles for Q, which both sides
+compute as (double)sumIn/m2 - (double)sumTot2/m2^2 from exact integer sums.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import gala
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def full_trace_compare(eng, g, keep_vectors=True):
+    trace, rounds = [], []
+    eng.set_graph(g)
+    res = eng.run(trace=trace, rounds=rounds)
+    t = oracle.Trace(keep_vectors=keep_vectors)
+    ref = oracle.louvain(g.vertex_num, g.degrees, g.neighbors,
+                         None if g.weights is None else g.weights, trace=t)
+    assert len(trace) == len(t.rows), (len(trace), len(t.rows))
+    for a, b in zip(trace, t.rows):
+        assert a[0] == b[0] and a[1] == b[1], (a[:2], b[:2])
+        assert a[2] == b[2], f"Q differs at round {a[0]} it {a[1]}: {a[2]!r} vs {b[2]!r}"
+        assert a[3] == b[3], f"moved differs at round {a[0]} it {a[1]}"
+        if keep_vectors and a[4] is not None:
+            assert np.array_equal(a[4], b[4]), f"community vector differs at round {a[0]} it {a[1]}"
+    assert res["rounds"] == len(ref["round_q"])
+    assert list(res["round_nc"]) == list(ref["round_nc"])
+    assert res["modularity"] == ref["Q"]
+    assert np.array_equal(eng.communities(g.vertex_num), ref["final_comm"])
+    return res, rounds
+
+
+def test_karate_known_answer(golden):
+    say("karate")
+    g = gala.Graph.load_bin_graph(os.path.join(golden, "karate.txt.bin"))
+    eng = gala.Engine(dump_iterations=True, dump_rounds=True)
+    res, rounds = full_trace_compare(eng, g)
+    assert f"{res['modularity']:.6f}" == "0.408695" and res["n_communities"] == 4
+    assert [r["C"] for r in rounds] == [11, 4, 4]
+    eng.close()
+
+
+def test_sbm1024_per_iteration_and_contraction(golden):
+    say("sbm1024")
+    g = gala.Graph.load_bin_graph(os.path.join(golden, "sbm1024_s0.bin"))
+    eng = gala.Engine(dump_iterations=True, dump_rounds=True)
+    res, rounds = full_trace_compare(eng, g)
+    # every contracted graph equals the oracle's contraction of the same partition
+    cur = g
+    w = np.ones(len(g.neighbors), np.int64)
+    for r in rounds:
+        nid, C, end, nb, ww = oracle.contract(cur.vertex_num, cur.degrees, cur.neighbors, w, r["comm"])
+        assert C == r["C"]
+        assert np.array_equal(end, r["end"]) and np.array_equal(nb, r["nbr"])
+        assert np.array_equal(ww, r["w"])
+        cur = gala.Graph(end, nb, ww.astype(np.int32), gala.W_I32)
+        w = ww
+    eng.close()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_chung_lu_hub_bins(seed):
+    # 200k vertices with hubs up to ~12k: every degree bin incl. the global-hash hubs
+    say(f"chung_lu 200k seed {seed}")
+    g = gala.gen_chung_lu(200000, 2.1, 17.35, 12000.0, seed)
+    deg = np.diff(g.offsets())
+    assert deg.max() >= 8192
+    eng = gala.Engine(dump_iterations=True, dump_rounds=False)
+    full_trace_compare(eng, g)
+    eng.close()
+
+
+def test_weighted_int_graph():
+    say("weighted int32")
+    rng = np.random.default_rng(3)
+    g = gala.gen_chung_lu(30000, 2.3, 12.0, 3000.0, 5)
+    off = g.offsets()
+    w = np.zeros(len(g.neighbors), np.int32)
+    # symmetric integer weights 1..7
+    for v in range(g.vertex_num):
+        for j in range(off[v], off[v + 1]):
+            u = g.neighbors[j]
+            if u > v:
+                x = int(rng.integers(1, 8))
+                w[j] = x
+                k = off[u] + np.searchsorted(g.neighbors[off[u]:off[u + 1]], v)
+                w[k] = x
+    gw = gala.Graph(g.degrees, g.neighbors, w, gala.W_I32)
+    eng = gala.Engine(dump_iterations=True)
+    full_trace_compare(eng, gw)
+    eng.close()
+
+
+def random_state(rng, g, ncomm):
+    n = g.vertex_num
+    off = g.offsets()
+    deg = np.diff(off)
+    K = deg.astype(np.int64) if g.weights is None else np.add.reduceat(g.weights.astype(np.int64), off[:-1]) * (deg > 0)
+    Self = np.zeros(n, np.int64)
+    prev = rng.integers(0, ncomm, n).astype(np.int32)
+    Tot = np.bincount(prev, weights=K, minlength=n).astype(np.int64)
+    size = np.bincount(prev, minlength=n).astype(np.int32)
+    return K, Self, Tot, size, prev
+
+
+@pytest.mark.parametrize("ncomm", [3, 50, 5000])
+def test_phase_decide_random_states(engine, ncomm):
+    # random assignments create many equal-weight candidates: exercises the
+    # lowest-id tie-break and the singleton guard in every bin
+    say(f"phase decide ncomm={ncomm}")
+    rng = np.random.default_rng(ncomm)
+    g = gala.gen_chung_lu(60000, 2.1, 17.35, 12000.0, 11)
+    K, Self, Tot, size, prev = random_state(rng, g, ncomm)
+    m2 = int(K.sum())
+    w = np.ones(len(g.neighbors), np.int64)
+    fr = (rng.random(g.vertex_num) < 0.7).astype(np.uint8)
+    for frontier in (None, fr):
+        got = engine.phase_decide(g, m2, K, Self, Tot, size, prev, frontier)
+        exp = oracle.decide_step(g.vertex_num, g.degrees, g.neighbors, w, m2, K, Self, Tot, size, prev,
+                                 frontier)
+        assert np.array_equal(got, exp)
+
+
+def test_phase_decide_singletons(engine):
+    say("phase decide singletons")
+    g = gala.gen_chung_lu(50000, 2.1, 17.35, 9000.0, 4)
+    n = g.vertex_num
+    K = np.diff(g.offsets()).astype(np.int64)
+    prev = np.arange(n, dtype=np.int32)
+    got = engine.phase_decide(g, int(K.sum()), K, np.zeros(n), K, np.ones(n), prev)
+    exp = oracle.decide_step(n, g.degrees, g.neighbors, np.ones(len(g.neighbors), np.int64), int(K.sum()), K,
+                             np.zeros(n, np.int64), K, np.ones(n, np.int32), prev)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("ncomm", [7, 900])
+def test_phase_weight(engine, ncomm):
+    say(f"phase weight ncomm={ncomm}")
+    rng = np.random.default_rng(ncomm + 1)
+    g = gala.gen_chung_lu(40000, 2.1, 17.35, 9000.0, 2)
+    K, Self, Tot, size, cur = random_state(rng, g, ncomm)
+    m2 = int(K.sum())
+    nz = Tot[Tot > 0]
+    min_tot = int(nz.min())
+    In, act = engine.phase_weight(g, m2, K, Self, Tot, cur, min_tot)
+    off = g.offsets()
+    src = np.repeat(np.arange(g.vertex_num), np.diff(off))
+    same = cur[g.neighbors] == cur[src]
+    In_ref = np.bincount(src[same], minlength=g.vertex_num).astype(np.int64)
+    assert np.array_equal(In, In_ref)
+    c = cur.astype(np.int64)
+    P = [m2 * (2 * int(In_ref[v]) - int(Self[v]) - int(K[v])) + int(K[v]) * (min_tot - (int(Tot[c[v]]) - int(K[v])))
+         for v in range(g.vertex_num)]
+    act_ref = np.array([0 if p > 0 else 1 for p in P], np.uint8)
+    assert np.array_equal(act, act_ref)
+
+
+@pytest.mark.parametrize("ncomm", [1, 2, 40, 3000, 30000])
+def test_phase_contract_random(engine, ncomm):
+    # ncomm small => very long rows (global-hash + bitmap path); large => LDS sort path
+    say(f"phase contract ncomm={ncomm}")
+    rng = np.random.default_rng(ncomm + 7)
+    g = gala.gen_chung_lu(60000, 2.1, 17.35, 12000.0, 3)
+    comm = rng.integers(0, ncomm, g.vertex_num).astype(np.int32)
+    nid, C, end, nb, ww = engine.phase_contract(g, comm)
+    rnid, rC, rend, rnb, rww = oracle.contract(g.vertex_num, g.degrees, g.neighbors,
+                                               np.ones(len(g.neighbors), np.int64), comm)
+    assert C == rC
+    used = np.bincount(comm, minlength=g.vertex_num) > 0
+    assert np.array_equal(nid[used], rnid[used]) and np.all(nid[~used] == -1)
+    assert np.array_equal(end, rend) and np.array_equal(nb, rnb) and np.array_equal(ww, rww)
+
+
+def test_determinism_twice(golden):
+    say("determinism")
+    g = gala.gen_chung_lu(100000, 2.1, 17.35, 10000.0, 9)
+    eng = gala.Engine()
+    eng.set_graph(g)
+    r1 = eng.run()
+    c1 = eng.communities(g.vertex_num)
+    r2 = eng.run()  # same context, input buffers are read-only
+    c2 = eng.communities(g.vertex_num)
+    assert r1["modularity"] == r2["modularity"] and np.array_equal(c1, c2)
+    eng.close()
