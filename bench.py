@@ -81,11 +81,10 @@ def main():
     wall_ms = wall * 1e3 / args.steps
     edges = g.edge_num
     last = runs[-1]
-    # decide-and-move algorithmic bytes: 16*d + 40 per swept vertex (12*d + 40 in
-    # round 0, whose unit weights are implicit and never read)
-    sv, se = last["swept_vertices"], last["swept_edges"]
-    sv_u, se_u = last.get("swept_vertices_unit", 0), last.get("swept_edges_unit", 0)
-    dm_bytes = 12 * se_u + 40 * sv_u + 16 * (se - se_u) + 40 * (sv - sv_u)
+    # decide-and-move algorithmic bytes (per swept vertex of degree d): 16d + 40,
+    # 12d + 40 with implicit unit weights; the first sweep of a round needs no
+    # prev[] gathers: 12d + 40 (8d + 40 unit).  Summed by the engine per sweep.
+    dm_bytes = last["decide_bytes"]
     dm_s = last["t_decide_ms"] * 1e-3
     achieved = dm_bytes / dm_s / 1e9 if dm_s > 0 else 0.0
 

@@ -64,12 +64,14 @@ struct gala_ctx {
   int32_t* nbr[3] = {nullptr, nullptr, nullptr};
   void* w[3] = {nullptr, nullptr, nullptr};
   bool unit0 = true;
+  bool unique0 = false;  // input rows strictly ascending (duplicate-free)
   int src = 0;  // buffer holding the current round graph
 
   // per-vertex state (sized n0)
   void *K = nullptr, *Self = nullptr, *Tot = nullptr, *In = nullptr, *dTot = nullptr;
   int32_t *size = nullptr, *dsize = nullptr, *prev = nullptr, *cur = nullptr;
   uint8_t* active = nullptr;
+  uint8_t* dirty = nullptr;
   int8_t* binid = nullptr;
   int32_t *order = nullptr, *front = nullptr, *fin = nullptr;
   int32_t *bstart_full = nullptr, *bstart_front = nullptr;
@@ -84,7 +86,7 @@ struct gala_ctx {
   Counters* ctr_h = nullptr;  // pinned
 
   // contraction buffers
-  int32_t *c_cnt = nullptr, *c_nid = nullptr, *c_skey = nullptr, *c_dcnt = nullptr,
+  int32_t *c_cnt = nullptr, *c_nid = nullptr, *c_label = nullptr, *c_skey = nullptr, *c_dcnt = nullptr,
           *c_big_list = nullptr, *c_nbig = nullptr, *c_i32_tmp = nullptr, *c_i32_scalar = nullptr;
   int64_t *c_ecnt = nullptr, *c_eoff = nullptr, *c_ecur = nullptr, *c_big_tab = nullptr,
           *c_i64_tmp = nullptr, *c_i64_scalar = nullptr;
@@ -150,7 +152,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
     OWN(p, N * tsz); ctx->K = p; OWN(p, N * tsz); ctx->Self = p; OWN(p, N * tsz); ctx->Tot = p;
     OWN(p, N * tsz); ctx->In = p; OWN(p, N * tsz); ctx->dTot = p; }
   OWN(ctx->size, N); OWN(ctx->dsize, N); OWN(ctx->prev, N); OWN(ctx->cur, N);
-  OWN(ctx->active, N); OWN(ctx->binid, N);
+  OWN(ctx->active, N); OWN(ctx->dirty, N); OWN(ctx->binid, N);
   OWN(ctx->order, N); OWN(ctx->front, N); OWN(ctx->fin, N);
   OWN(ctx->bstart_full, NBINS + 1); OWN(ctx->bstart_front, NBINS + 1);
   OWN(ctx->hub_off, N + 1); OWN(ctx->hub_tmp, 2 * (N + 1));
@@ -158,9 +160,9 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->scan64_tmp, scan_tmp_elems(N) + 2);
   OWN(ctx->ctr, 1);
   // contraction
-  OWN(ctx->c_cnt, N); OWN(ctx->c_nid, N); OWN(ctx->c_skey, M);
+  OWN(ctx->c_cnt, N); OWN(ctx->c_nid, N); OWN(ctx->c_label, N); OWN(ctx->c_skey, M);
   { char* p; OWN(p, M * tsz); ctx->c_sval = p; }
-  OWN(ctx->c_dcnt, N + 1); OWN(ctx->c_big_list, N); OWN(ctx->c_nbig, 1);
+  OWN(ctx->c_dcnt, N + 1); OWN(ctx->c_big_list, 2 * N); OWN(ctx->c_nbig, 2);
   OWN(ctx->c_i32_tmp, N / 4 + 2 + scan_tmp_elems(N));
   OWN(ctx->c_i32_scalar, 4);
   OWN(ctx->c_ecnt, N + 1); OWN(ctx->c_eoff, N + 1); OWN(ctx->c_ecur, N);
@@ -186,6 +188,10 @@ int grow_hub(gala_ctx* ctx, int64_t slots, bool fp) {
     ctx->hub_vals = p;
   }
   if (e != hipSuccess) { ctx->last_hip = e; ctx->hub_cap = 0; return GALA_E_OOM; }
+  // the hub kernels keep overflow slices all-zero between uses
+  e = hipMemset(ctx->hub_keys, 0, sizeof(int32_t) * cap);
+  if (e == hipSuccess) e = hipMemset(ctx->hub_vals, 0, (size_t)cap * (fp ? 8 : 4));
+  if (e != hipSuccess) { ctx->last_hip = e; return GALA_E_HIP; }
   ctx->hub_cap = cap;
   return GALA_OK;
 }
@@ -195,7 +201,7 @@ int grow_arena(gala_ctx* ctx, int64_t slots, int64_t words, bool fp) {
     if (ctx->arena_keys) (void)hipFree(ctx->arena_keys);
     if (ctx->arena_vals) (void)hipFree(ctx->arena_vals);
     ctx->arena_keys = nullptr; ctx->arena_vals = nullptr;
-    int64_t cap = slots + slots / 4;
+    int64_t cap = slots + slots / 4;  // slack so later rounds rarely reallocate
     hipError_t e = dalloc(&ctx->arena_keys, cap);
     if (e == hipSuccess) { char* p; e = dalloc(&p, cap * (fp ? 8 : 4)); ctx->arena_vals = p; }
     if (e != hipSuccess) { ctx->last_hip = e; ctx->arena_slots = 0; return GALA_E_OOM; }
@@ -283,7 +289,7 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.K = (tt*)ctx->K; st.Self = (tt*)ctx->Self; st.Tot = (tt*)ctx->Tot; st.In = (tt*)ctx->In;
   st.dTot = (tt*)ctx->dTot;
   st.size = ctx->size; st.dsize = ctx->dsize; st.prev = ctx->prev; st.cur = ctx->cur;
-  st.active = ctx->active; st.binid = ctx->binid; st.hub_off = ctx->hub_off;
+  st.active = ctx->active; st.dirty = ctx->dirty; st.binid = ctx->binid; st.hub_off = ctx->hub_off;
   st.hub_keys = ctx->hub_keys; st.hub_vals = (typename T::acc_t*)ctx->hub_vals;
   st.ctr = ctx->ctr;
   return st;
@@ -339,8 +345,8 @@ int read_counters(gala_ctx* ctx, Counters* out) {
 // One round of phase 1 (louvain_main_process).  On return ctx->prev holds the
 // accepted assignment of the round graph.
 template <class T>
-int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int round, gala_iter_cb icb,
-              void* user, RoundOut* ro, std::vector<int32_t>& hostvec) {
+int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int round, bool unique_rows,
+              gala_iter_cb icb, void* user, RoundOut* ro, std::vector<int32_t>& hostvec) {
   using tt = typename T::tot_t;
   hipStream_t s = ctx->stream;
   const double m2d = (double)m2;
@@ -351,7 +357,8 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   // Iteration 0: Q on the singleton partition (louvain_main_process.cu:538-548)
   reset_counters(ctx->ctr, (long long)m2, m2d, s);
   launch_apply<T>(st, g.n, m2, s);
-  for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], m2, s);
+  for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], false, s);
+  launch_prune<T>(st, g.n, m2, s);
   Counters c;
   if ((rc = read_counters<T>(ctx, &c))) return rc;
   double q = q_from<T>(c, m2d), qprev = q;
@@ -373,11 +380,16 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   for (;; it++) {
     reset_counters(ctx->ctr, (long long)m2, m2d, s);
     HIPOK(hipEventRecord(ctx->ev[0], s));
-    for (int b = 0; b < NBINS; b++)
-      launch_decide<T>(b, g, st, ctx->front, ctx->bstart_front, caps[b], m2, s);
+    const bool first = it == 1 && unique_rows;  // singleton partition: no aggregation needed
+    for (int b = 0; b < NBINS; b++) {
+      if (first) launch_decide_first<T>(b, g, st, ctx->front, ctx->bstart_front, caps[b], m2, s);
+      else launch_decide<T>(b, g, st, ctx->front, ctx->bstart_front, caps[b], m2, s);
+    }
     HIPOK(hipEventRecord(ctx->ev[1], s));
     launch_apply<T>(st, g.n, m2, s);
-    for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], m2, s);
+    // In changes only where a vertex or one of its neighbours moved
+    for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s);
+    launch_prune<T>(st, g.n, m2, s);
     partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front,
                    ctx->part_tmp, s);
     HIPOK(hipEventRecord(ctx->ev[2], s));
@@ -392,6 +404,10 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     if (!g.w) {
       ctx->res.swept_vertices_unit += (int64_t)c.swept;
       ctx->res.swept_edges_unit += (int64_t)c.swept_edges;
+    }
+    {
+      const int64_t per_edge = (first ? 12 : 16) - (g.w ? 0 : 4);
+      ctx->res.decide_bytes += per_edge * (int64_t)c.swept_edges + 40 * (int64_t)c.swept;
     }
     q = q_from<T>(c, m2d);
     if (icb) {
@@ -418,7 +434,8 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
 template <class T>
 ContractBufs<T> cbufs(gala_ctx* ctx) {
   ContractBufs<T> cb;
-  cb.cnt = ctx->c_cnt; cb.nid = ctx->c_nid; cb.ecnt = ctx->c_ecnt; cb.eoff = ctx->c_eoff;
+  cb.cnt = ctx->c_cnt; cb.nid = ctx->c_nid; cb.label = ctx->c_label; cb.ecnt = ctx->c_ecnt;
+  cb.eoff = ctx->c_eoff;
   cb.ecur = ctx->c_ecur; cb.skey = ctx->c_skey; cb.sval = (typename T::acc_t*)ctx->c_sval;
   cb.dcnt = ctx->c_dcnt; cb.noff = nullptr; cb.big_list = ctx->c_big_list; cb.nbig = ctx->c_nbig;
   cb.big_tab = ctx->c_big_tab; cb.arena_keys = ctx->arena_keys;
@@ -429,43 +446,33 @@ ContractBufs<T> cbufs(gala_ctx* ctx) {
   return cb;
 }
 
-// Contract the round graph under comm into buffer `dst`.  Returns C and new m.
+// Contract the round graph under comm into buffer `dst`.  Returns C and new m;
+// ctx->c_label then holds new_id(comm[v]) for the composition.
 template <class T>
 int contract(gala_ctx* ctx, const DevGraph<T>& g, const int32_t* comm, int dst, int32_t* C_out,
              int64_t* m_out) {
   hipStream_t s = ctx->stream;
   ContractBufs<T> cb = cbufs<T>(ctx);
-  int32_t C = 0;
-  int64_t staged = 0;
-  if (contract_prepare<T>(g, comm, cb, &C, &staged, s)) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
+  ContractPlan plan;
+  if (contract_prepare<T>(g, comm, cb, &plan, s)) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
+  int rc = grow_arena(ctx, plan.slots, ctx->arena_words, T::FP);
+  if (rc) return rc;
   ContractOut<T> out;
   out.off = ctx->off[dst];
   out.nbr = ctx->nbr[dst];
   out.w = (typename T::w_t*)ctx->w[dst];
   int64_t m = 0;
-  for (int attempt = 0; attempt < 8; attempt++) {
+  cb = cbufs<T>(ctx);
+  int r = contract_build<T>(g, cb, plan, out, &m, s);
+  if (r == -3) {  // rows with > 16384 distinct neighbours and C > 2^20: bitmap arena
+    int64_t words = (int64_t)((plan.C + 31) / 32) * 64;
+    if ((rc = grow_arena(ctx, plan.slots, words, T::FP))) return rc;
     cb = cbufs<T>(ctx);
-    int rc = contract_build<T>(g, comm, cb, C, staged, out, &m, s);
-    if (rc == 0) break;
-    if (rc == -1) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
-    // grow the row-merge arena: slots for all long rows, bitmap for 64 of them
-    int64_t need_slots = 0;
-    int32_t nbig = 0;
-    HIPOK(hipMemcpy(&nbig, ctx->c_nbig, sizeof nbig, hipMemcpyDeviceToHost));
-    HIPOK(hipMemcpy(&need_slots, ctx->c_big_tab + nbig, sizeof need_slots, hipMemcpyDeviceToHost));
-    int64_t words = (int64_t)((C + 31) / 32) * std::min<int64_t>(std::max(nbig, 1), 64);
-    int r2 = grow_arena(ctx, need_slots, rc == -3 ? words : ctx->arena_words, T::FP);
-    if (r2) return r2;
-    if (attempt == 7) return GALA_E_OOM;
+    fill<int64_t>(cb.ecur, plan.C, 0, s);
+    r = contract_build<T>(g, cb, plan, out, &m, s);
   }
-  Counters c;
-  HIPOK(hipMemcpyAsync(ctx->ctr_h, ctx->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
-  HIPOK(hipStreamSynchronize(s));
-  int32_t ovf = 0;
-  HIPOK(hipMemcpy(&ovf, ctx->c_i32_scalar + 3, sizeof ovf, hipMemcpyDeviceToHost));
-  (void)c;
-  if (ovf) return GALA_E_OVERFLOW;
-  *C_out = C;
+  if (r) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
+  *C_out = plan.C;
   *m_out = m;
   return GALA_OK;
 }
@@ -509,7 +516,8 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
     if (round > 0) prev_mod = cur_mod;
     HIPOK(hipEventRecord(ctx->ev[5], s));
     RoundOut ro;
-    int rc = run_round<T>(ctx, g, m2, round, icb, user, &ro, hostvec);
+    const bool unique_rows = src != 0 || ctx->unique0;  // contracted rows are canonical
+    int rc = run_round<T>(ctx, g, m2, round, unique_rows, icb, user, &ro, hostvec);
     if (rc) return rc;
     cur_mod = ro.q;
     HIPOK(hipEventRecord(ctx->ev[6], s));
@@ -518,7 +526,7 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
     const int dst = dst_of(src);
     rc = contract<T>(ctx, g, ctx->prev, dst, &C, &m);
     if (rc) return rc;
-    compose_map(ctx->fin, ctx->n0, ctx->prev, ctx->c_nid, s);
+    compose_map(ctx->fin, ctx->n0, ctx->c_label, s);
     HIPOK(hipEventRecord(ctx->ev[7], s));
     HIPOK(hipEventSynchronize(ctx->ev[7]));
     float t_round = 0, t_build = 0;
@@ -600,6 +608,13 @@ int upload(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr
   }
   for (int64_t j = 0; j < m; j++)
     if (nbr[j] < 0 || nbr[j] >= n) return GALA_E_FORMAT;
+  bool unique = true;  // strictly ascending rows (what preprocess and the generators write)
+  for (int32_t i = 0, j0 = 0; i < n && unique; i++) {
+    int64_t a = i == 0 ? 0 : end_incl[i - 1];
+    for (int64_t j = a + 1; j < end_incl[i]; j++)
+      if (nbr[j] <= nbr[j - 1]) { unique = false; break; }
+    (void)j0;
+  }
   if (weight_mode != GALA_W_NONE && m > 0 && !w) return GALA_E_INVALID;
   if (weight_mode == GALA_W_I32)
     for (int64_t j = 0; j < m; j++)
@@ -632,6 +647,7 @@ int upload(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr
   ctx->t_h2d_ms = ms;
   ctx->n0 = n;
   ctx->m0 = m;
+  ctx->unique0 = unique;
   ctx->have_graph = true;
   ctx->have_result = false;
   return GALA_OK;
@@ -787,7 +803,8 @@ int gala_phase_weight(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const i
   // reference's zero-initialised In and fill(active, 1) leave them
   fill<int32_t>((int32_t*)ctx->In, n, 0, s);
   fill<uint8_t>(ctx->active, n, 1, s);
-  for (int b = 0; b < NBINS; b++) launch_weight<IntW>(b, g, st, ctx->order, ctx->bstart_full, caps[b], (int32_t)m2, s);
+  for (int b = 0; b < NBINS; b++) launch_weight<IntW>(b, g, st, ctx->order, ctx->bstart_full, caps[b], false, s);
+  launch_prune<IntW>(st, n, (int32_t)m2, s);
   Counters c;
   if ((rc = read_counters<IntW>(ctx, &c))) return rc;
   HIPOK(hipMemcpy(In_out, ctx->In, 4ll * n, hipMemcpyDeviceToHost));

@@ -35,6 +35,7 @@ struct DevState {
   typename T::tot_t *K, *Self, *Tot, *In, *dTot;
   int32_t *size, *dsize, *prev, *cur;
   uint8_t* active;
+  uint8_t* dirty;    // In must be recomputed (the vertex or a neighbour moved)
   int8_t* binid;
   int64_t* hub_off;  // per-vertex offset into the global hash arena (bin 8), in slots
   int32_t* hub_keys;
@@ -51,20 +52,36 @@ template <class T>
 void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
                    const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s);
 
+// First sweep of a round (singleton partition, duplicate-free rows): no
+// aggregation needed, every entry is its own candidate community.
+template <class T>
+void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
+                         const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s);
+
 // Tot += dTot, size += dsize, sums of Tot^2, min Tot, non-empty count.
 template <class T>
 void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);
 
-// In[v] and the prune flag over order[bstart[b] .. bstart[b+1]).
+// In[v] over order[bstart[b] .. bstart[b+1]) (only the dirty vertices when only_dirty).
 template <class T>
 void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
-                   const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s);
+                   const int32_t* bstart, int cap, bool only_dirty, hipStream_t s);
+// prune flags, sum In, active count; clears the dirty marks.
+template <class T>
+void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);
 
 // ---- contraction (kernels_contract.hip) ----
+struct ContractPlan {
+  int32_t C;      // communities (rows of the contracted graph)
+  int32_t nbig;   // rows with more than 4096 staged entries
+  int64_t slots;  // global hash slots those rows need
+};
+
 template <class T>
 struct ContractBufs {
   int32_t* cnt;       // [n]   community sizes
   int32_t* nid;       // [n]   new id of community (exclusive scan of cnt > 0)
+  int32_t* label;     // [n]   new id of every vertex's community
   int64_t* ecnt;      // [n+1] entries per new community
   int64_t* eoff;      // [n+1]
   int64_t* ecur;      // [n]
@@ -72,8 +89,8 @@ struct ContractBufs {
   typename T::acc_t* sval;  // [m]   staging values
   int32_t* dcnt;      // [n+1] distinct neighbours per new community
   int64_t* noff;      // [n+1]
-  int32_t* big_list;  // [n]   communities taking the global-hash path
-  int32_t* nbig;      // [1]
+  int32_t* big_list;  // [2n]  communities taking the global-hash path, then the bitmap queue
+  int32_t* nbig;      // [2]   long rows, rows beyond the LDS sort
   int64_t* big_tab;   // [n+1] arena offsets (slots) of each large community's table
   int32_t* arena_keys;
   typename T::acc_t* arena_vals;
@@ -93,18 +110,19 @@ struct ContractOut {
   typename T::w_t* w;
 };
 
-// Renumber + member lists + per-community entry counts; returns C and total
-// staged entries through host pointers (one sync).
+// Renumber (nid, label), per-community entry counts and offsets, long-row
+// classification and their hash-table offsets.  Fills *plan (two syncs).
 template <class T>
-int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& cb, int32_t* C_out,
-                     int64_t* staged_out, hipStream_t s);
-// Merge rows, canonical order, write the new CSR into out (sized by caller
-// from the returned entry count).  Returns the new directed-entry count.
+int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& cb, ContractPlan* plan,
+                     hipStream_t s);
+// Stage, merge rows in canonical order, write the new CSR into out.  Needs
+// arena_slots >= plan.slots.  Returns 0, -1 (HIP error) or -3 (bitmap arena
+// too small: grow arena_words to 64*ceil(C/32) and call again).
 template <class T>
-int contract_build(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& cb, int32_t C,
-                   int64_t staged, ContractOut<T>& out, int64_t* m_out, hipStream_t s);
+int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan& plan,
+                   ContractOut<T>& out, int64_t* m_out, hipStream_t s);
 
-// final[v] = nid[comm[final[v]]]
-void compose_map(int32_t* fin, int32_t n0, const int32_t* comm, const int32_t* nid, hipStream_t s);
+// final[v] = label[final[v]]
+void compose_map(int32_t* fin, int32_t n0, const int32_t* label, hipStream_t s);
 
 }  // namespace gala

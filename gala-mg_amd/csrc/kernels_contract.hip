@@ -8,24 +8,29 @@
 // and hash insertion order, :1101-1111, :1264) every output row is canonical:
 // ascending neighbour id, the order preprocess's std::map produces.
 //
-//   rows with <= 4096 staged entries: bitonic sort of (key, weight) in LDS,
-//     then reduce-by-key;
-//   longer rows: one workgroup, a global open-addressing table per row
-//     (sized next_pow2(2*min(L, C)), no drops), then an ordered extraction
-//     through a C-bit bitmap (LDS when C <= 2^20, else a global slice).
+// Pipeline: label[v] = new_id(comm[v]) -> per-community entry counts -> stage
+// every entry (label of the neighbour, weight) in its community's segment ->
+// merge each segment into a sorted, duplicate-free row:
+//   L <= 64      one wave, bitonic sort in registers + segmented scan
+//   L <= 512     64-thread workgroup: LDS hash aggregate, bitonic sort of the keys
+//   L <= 4096    256-thread workgroup, same with larger LDS tables
+//   L >  4096    1024-thread workgroup: global hash (next_pow2(2 min(L, C))
+//                slots, no drops), distinct keys sorted in LDS (<= 16384,
+//                8192 for FP64 values); beyond that an ordered extraction
+//                through a C-bit bitmap
+// -> exclusive scan of the distinct counts -> compaction into the new CSR.
 #include "gala_kernels.h"
+#include "gala_device.h"
 #include <climits>
 
 namespace gala {
 
-constexpr int SMALL_A = 512;    // one 64-thread workgroup
-constexpr int SMALL_B = 4096;   // one 256-thread workgroup
 constexpr int BITMAP_LDS_WORDS = 32768;  // 2^20 bits = 128 KB
 
 __device__ __forceinline__ unsigned chash(int key, int lg) {
   return ((unsigned)key * 2654435769u) >> (32 - lg);
 }
-__device__ __forceinline__ int clog2(long long x) {
+__host__ __device__ __forceinline__ int clog2(long long x) {
   int l = 0;
   while ((1ll << l) < x) l++;
   return l;
@@ -42,55 +47,17 @@ __global__ void k_nonempty(const int32_t* __restrict__ cnt, int32_t n, uint8_t* 
     flag[c] = cnt[c] > 0;
 }
 
-template <class T>
-__global__ void k_ecount(DevGraph<T> g, const int32_t* __restrict__ comm, const int32_t* __restrict__ nid,
+__global__ void k_label(const int32_t* __restrict__ comm, const int32_t* __restrict__ nid, int32_t n,
+                        int32_t* __restrict__ label) {
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    label[v] = nid[comm[v]];
+}
+
+__global__ void k_ecount(const int64_t* __restrict__ off, int32_t n, const int32_t* __restrict__ label,
                          int64_t* __restrict__ ecnt) {
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += gridDim.x * blockDim.x) {
-    int64_t d = g.off[v + 1] - g.off[v];
-    if (d) atomicAdd((unsigned long long*)&ecnt[nid[comm[v]]], (unsigned long long)d);
-  }
-}
-
-// stage every row entry as (new id of the neighbour's community, weight) in the
-// segment of the owner's new community
-template <class T>
-__global__ void k_stage_small(DevGraph<T> g, const int32_t* __restrict__ comm,
-                              const int32_t* __restrict__ nid, const int64_t* __restrict__ eoff,
-                              int64_t* __restrict__ ecur, int32_t* __restrict__ skey,
-                              typename T::acc_t* __restrict__ sval) {
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += gridDim.x * blockDim.x) {
-    int64_t o = g.off[v], e = g.off[v + 1];
-    int64_t d = e - o;
-    if (d == 0 || d > 64) continue;
-    int nc = nid[comm[v]];
-    int64_t base = eoff[nc] + (int64_t)atomicAdd((unsigned long long*)&ecur[nc], (unsigned long long)d);
-    for (int64_t j = o; j < e; j++) {
-      skey[base + (j - o)] = nid[comm[g.nbr[j]]];
-      sval[base + (j - o)] = (typename T::acc_t)(g.w ? g.w[j] : (typename T::w_t)1);
-    }
-  }
-}
-
-template <class T>
-__global__ void k_stage_wide(DevGraph<T> g, const int32_t* __restrict__ comm,
-                             const int32_t* __restrict__ nid, const int64_t* __restrict__ eoff,
-                             int64_t* __restrict__ ecur, int32_t* __restrict__ skey,
-                             typename T::acc_t* __restrict__ sval) {
-  const int lane = threadIdx.x & 63;
-  const int nw = gridDim.x * (blockDim.x >> 6);
-  for (int v = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); v < g.n; v += nw) {
-    int64_t o = g.off[v], e = g.off[v + 1];
-    int64_t d = e - o;
-    if (d <= 64) continue;
-    int nc = nid[comm[v]];
-    int64_t base = 0;
-    if (lane == 0)
-      base = eoff[nc] + (int64_t)atomicAdd((unsigned long long*)&ecur[nc], (unsigned long long)d);
-    base = __shfl(base, 0, 64);
-    for (int64_t j = o + lane; j < e; j += 64) {
-      skey[base + (j - o)] = nid[comm[g.nbr[j]]];
-      sval[base + (j - o)] = (typename T::acc_t)(g.w ? g.w[j] : (typename T::w_t)1);
-    }
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    int64_t d = off[v + 1] - off[v];
+    if (d) atomicAdd((unsigned long long*)&ecnt[label[v]], (unsigned long long)d);
   }
 }
 
@@ -99,7 +66,7 @@ __global__ void k_classify(const int64_t* __restrict__ ecnt, int32_t C, int32_t*
                            int32_t* __restrict__ nbig, int64_t* __restrict__ big_tsz) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
     long long L = ecnt[c];
-    if (L > SMALL_B) {
+    if (L > 4096) {
       int k = atomicAdd(nbig, 1);
       big_list[k] = c;
       long long lim = L < (long long)C ? L : (long long)C;
@@ -109,86 +76,165 @@ __global__ void k_classify(const int64_t* __restrict__ ecnt, int32_t C, int32_t*
 }
 
 // ---------------------------------------------------------------------------
-// short rows: bitonic sort in LDS then reduce-by-key, in place
+// staging: (label of the neighbour, weight) into the owner's segment.  W lanes
+// per vertex read the row with coalesced loads; lane 0 reserves the slot range.
 // ---------------------------------------------------------------------------
-template <int NT, int CAP, class T>
-__global__ __launch_bounds__(NT) void k_merge_small(const int64_t* __restrict__ ecnt,
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_stage(DevGraph<T> g, const int32_t* __restrict__ label,
+                                               const int64_t* __restrict__ eoff, int64_t* __restrict__ ecur,
+                                               int64_t dlo, int64_t dhi, int32_t* __restrict__ skey,
+                                               typename T::acc_t* __restrict__ sval) {
+  using acc = typename T::acc_t;
+  const int lane = threadIdx.x & (W - 1);
+  const int groups = gridDim.x * (blockDim.x / W);
+  for (int v = (blockIdx.x * blockDim.x + threadIdx.x) / W; v < g.n; v += groups) {
+    const int64_t o = g.off[v];
+    const int64_t d = g.off[v + 1] - o;
+    if (d < dlo || d > dhi) continue;  // uniform per group
+    int64_t base = 0;
+    if (lane == 0) {
+      const int nc = label[v];
+      base = eoff[nc] + (int64_t)atomicAdd((unsigned long long*)&ecur[nc], (unsigned long long)d);
+    }
+    base = __shfl(base, 0, W);
+    for (int64_t j = lane; j < d; j += W) {
+      skey[base + j] = label[g.nbr[o + j]];
+      sval[base + j] = g.w ? (acc)g.w[o + j] : acc(1);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// L <= 64: one wave per community, everything in registers
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_merge_tiny(const int64_t* __restrict__ ecnt,
                                                      const int64_t* __restrict__ eoff, int32_t C,
-                                                     int lo_excl, int32_t* __restrict__ skey,
+                                                     int32_t* __restrict__ skey,
                                                      typename T::acc_t* __restrict__ sval,
                                                      int32_t* __restrict__ dcnt) {
   using acc = typename T::acc_t;
-  __shared__ int k_s[CAP];
-  __shared__ acc v_s[CAP];
-  __shared__ int h_s[CAP];
-  __shared__ int sh[17];
-  for (int c = blockIdx.x; c < C; c += gridDim.x) {
-    const long long L = ecnt[c];
-    if (L <= lo_excl || L > CAP) continue;  // uniform per block
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  for (int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < C; c += nw) {
+    const int L = (int)ecnt[c];
+    if (L == 0 || L > 64) continue;  // uniform per wave
     const int64_t base = eoff[c];
-    int P = 1;
-    while (P < L) P <<= 1;
-    for (int t = threadIdx.x; t < P; t += NT) {
-      if (t < L) { k_s[t] = skey[base + t]; v_s[t] = sval[base + t]; }
-      else { k_s[t] = INT_MAX; v_s[t] = 0; }
-    }
-    __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
+    int key = INT_MAX;
+    acc val = 0;
+    if (lane < L) { key = skey[base + lane]; val = sval[base + lane]; }
+    // bitonic sort over 64 lanes (ascending)
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
       for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int t = threadIdx.x; t < P; t += NT) {
-          int u = t ^ j;
-          if (u > t) {
-            bool up = (t & k) == 0;
-            int a = k_s[t], b = k_s[u];
-            if ((a > b) == up) {
-              k_s[t] = b; k_s[u] = a;
-              acc x = v_s[t]; v_s[t] = v_s[u]; v_s[u] = x;
-            }
+        int pk = __shfl_xor(key, j, 64);
+        acc pv = __shfl_xor(val, j, 64);
+        const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+        const bool take = (lower == up) ? (pk < key) : (pk > key);
+        if (take) { key = pk; val = pv; }
+      }
+    }
+    const int prevk = __shfl_up(key, 1, 64);
+    const bool head = lane < L && (lane == 0 || prevk != key);
+    // segmented inclusive scan of val (runs start at heads)
+    bool f = head;
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+      acc y = __shfl_up(val, k, 64);
+      int fy = __shfl_up((int)f, k, 64);
+      if (lane >= k && !f) { val += y; f = fy; }
+    }
+    const unsigned long long heads = __ballot(head);
+    const int nextk = __shfl_down(key, 1, 64);
+    const bool tail = lane < L && (lane == L - 1 || nextk != key);
+    if (tail) {
+      const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+      const int rank = __popcll(heads & le) - 1;
+      skey[base + rank] = key;
+      sval[base + rank] = val;
+    }
+    if (lane == 0) dcnt[c] = __popcll(heads);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// bitonic sort of P (power of two) unique keys + values in LDS by NT threads
+// ---------------------------------------------------------------------------
+template <int NT, class V>
+__device__ __forceinline__ void lds_bitonic(int* k_s, V* v_s, int P) {
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < P; t += NT) {
+        const int u = t ^ j;
+        if (u > t) {
+          const bool up = (t & k) == 0;
+          const int a = k_s[t], b = k_s[u];
+          if ((a > b) == up) {
+            k_s[t] = b; k_s[u] = a;
+            V x = v_s[t]; v_s[t] = v_s[u]; v_s[u] = x;
           }
         }
-        __syncthreads();
-      }
-    }
-    // heads of runs
-    for (int t = threadIdx.x; t < P; t += NT)
-      h_s[t] = (t < L) && (t == 0 || k_s[t] != k_s[t - 1]);
-    __syncthreads();
-    // exclusive scan of head flags over P items (each thread a contiguous chunk)
-    const int per = (P + NT - 1) / NT;
-    const int a0 = threadIdx.x * per, a1 = min(P, a0 + per);
-    int s = 0;
-    for (int t = a0; t < a1; t++) s += h_s[t];
-    // block exclusive scan of s
-    {
-      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-      int x = s;
-#pragma unroll
-      for (int q = 1; q < 64; q <<= 1) {
-        int y = __shfl_up(x, q, 64);
-        if (lane >= q) x += y;
-      }
-      if (lane == 63) sh[wid] = x;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        int run = 0;
-        for (int w = 0; w < NT / 64; w++) { int tmp = sh[w]; sh[w] = run; run += tmp; }
-        sh[16] = run;
       }
       __syncthreads();
-      s = x - s + sh[wid];
     }
-    const int D = sh[16];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LO < L <= LMAX: one workgroup per community: LDS hash aggregate, then sort
+// the distinct keys (unique, so no reduction after the sort)
+// ---------------------------------------------------------------------------
+template <int NT, int LO, int LMAX, class T>
+__global__ __launch_bounds__(NT) void k_merge_mid(const int64_t* __restrict__ ecnt,
+                                                   const int64_t* __restrict__ eoff, int32_t C,
+                                                   int32_t* __restrict__ skey,
+                                                   typename T::acc_t* __restrict__ sval,
+                                                   int32_t* __restrict__ dcnt) {
+  using acc = typename T::acc_t;
+  constexpr int TMAX = 2 * LMAX;
+  __shared__ int h_k[TMAX];
+  __shared__ acc h_v[TMAX];
+  __shared__ int s_k[LMAX];
+  __shared__ acc s_v[LMAX];
+  __shared__ int s_sh[17];
+  for (int c = blockIdx.x; c < C; c += gridDim.x) {
+    const long long L = ecnt[c];
+    if (L <= LO || L > LMAX) continue;  // uniform per block
+    const int64_t base = eoff[c];
+    const long long lim = L < (long long)C ? L : (long long)C;
+    const int lg = clog2(2 * lim) < 1 ? 1 : clog2(2 * lim);
+    const int tsz = 1 << lg;
+    for (int t = threadIdx.x; t < tsz; t += NT) { h_k[t] = 0; h_v[t] = 0; }
     __syncthreads();
-    int pos = s;
-    for (int t = a0; t < a1; t++) {
-      if (h_s[t]) {
-        acc sum = 0;
-        int key = k_s[t];
-        for (int u = t; u < L && k_s[u] == key; u++) sum += v_s[u];
-        skey[base + pos] = key;
-        sval[base + pos] = sum;
-        pos++;
+    for (int j = threadIdx.x; j < L; j += NT) {
+      const int key = skey[base + j];
+      const acc w = sval[base + j];
+      unsigned p = chash(key, lg);
+      for (;;) {
+        const int k = atomicCAS(&h_k[p], 0, key + 1);
+        if (k == 0 || k == key + 1) { atomicAdd(&h_v[p], w); break; }
+        p = (p + 1) & (tsz - 1);
       }
+    }
+    __syncthreads();
+    int cnt = 0;
+    for (int t = threadIdx.x; t < tsz; t += NT) cnt += h_k[t] != 0;
+    int D;
+    int q = block_excl_scan<int>(cnt, s_sh, &D);
+    for (int t = threadIdx.x; t < tsz; t += NT) {
+      const int k = h_k[t];
+      if (k) { s_k[q] = k - 1; s_v[q] = h_v[t]; q++; }
+    }
+    __syncthreads();
+    int P = 1;
+    while (P < D) P <<= 1;
+    for (int t = D + threadIdx.x; t < P; t += NT) { s_k[t] = INT_MAX; s_v[t] = 0; }
+    __syncthreads();
+    lds_bitonic<NT, acc>(s_k, s_v, P);
+    for (int t = threadIdx.x; t < D; t += NT) {
+      skey[base + t] = s_k[t];
+      sval[base + t] = s_v[t];
     }
     if (threadIdx.x == 0) dcnt[c] = D;
     __syncthreads();
@@ -196,24 +242,26 @@ __global__ __launch_bounds__(NT) void k_merge_small(const int64_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// long rows: global hash + ordered extraction through a bitmap of C bits
+// L > 4096: global hash per community, distinct keys sorted in LDS; rows with
+// more than DMAX distinct keys are queued for the bitmap extraction
 // ---------------------------------------------------------------------------
-template <class T>
+template <int DMAX, class T>
 __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ ecnt,
                                                      const int64_t* __restrict__ eoff, int32_t C,
                                                      const int32_t* __restrict__ big_list, int nbig,
                                                      const int64_t* __restrict__ big_off,
                                                      int32_t* __restrict__ arena_keys,
                                                      typename T::acc_t* __restrict__ arena_vals,
-                                                     uint32_t* __restrict__ arena_bits,
                                                      int32_t* __restrict__ skey,
                                                      typename T::acc_t* __restrict__ sval,
-                                                     int32_t* __restrict__ dcnt, int* overflow) {
+                                                     int32_t* __restrict__ dcnt,
+                                                     int32_t* __restrict__ huge_list,
+                                                     int32_t* __restrict__ nhuge) {
   using acc = typename T::acc_t;
   constexpr int NT = 1024;
-  __shared__ uint32_t bits_lds[BITMAP_LDS_WORDS];
-  __shared__ int sh[17];
-  const int nwords = (C + 31) >> 5;
+  __shared__ int s_k[DMAX];
+  __shared__ acc s_v[DMAX];
+  __shared__ int s_sh[17];
   for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
     const int c = big_list[b];
     const long long L = ecnt[c];
@@ -223,29 +271,84 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
     const int lg = clog2(tsz);
     int* keys = arena_keys + toff;
     acc* vals = arena_vals + toff;
-    uint32_t* bits = nwords <= BITMAP_LDS_WORDS ? bits_lds : arena_bits + (int64_t)blockIdx.x * nwords;
     for (long long t = threadIdx.x; t < tsz; t += NT) { keys[t] = 0; vals[t] = 0; }
-    for (int t = threadIdx.x; t < nwords; t += NT) bits[t] = 0;
     __syncthreads();
     for (long long j = threadIdx.x; j < L; j += NT) {
-      int key = skey[base + j];
-      acc w = sval[base + j];
+      const int key = skey[base + j];
+      const acc w = sval[base + j];
       unsigned p = chash(key, lg);
-      long long probe = 0;
-      for (; probe < tsz; probe++) {
-        int k = atomicCAS(&keys[p], 0, key + 1);
+      for (;;) {
+        const int k = atomicCAS(&keys[p], 0, key + 1);
         if (k == 0 || k == key + 1) { atomicAdd(&vals[p], w); break; }
         p = (p + 1) & (unsigned)(tsz - 1);
       }
-      if (probe == tsz) *overflow = 1;
     }
     __syncthreads();
+    // count distinct keys first: rows beyond DMAX go to the bitmap path
+    int cnt = 0;
+    for (long long t = threadIdx.x; t < tsz; t += NT) cnt += keys[t] != 0;
+    int D;
+    int q = block_excl_scan<int>(cnt, s_sh, &D);
+    if (D > DMAX) {
+      if (threadIdx.x == 0) {
+        huge_list[atomicAdd(nhuge, 1)] = b;
+        dcnt[c] = D;
+      }
+      __syncthreads();
+      continue;
+    }
     for (long long t = threadIdx.x; t < tsz; t += NT) {
-      int k = keys[t];
+      const int k = keys[t];
+      if (k) { s_k[q] = k - 1; s_v[q] = vals[t]; q++; }
+    }
+    __syncthreads();
+    int P = 1;
+    while (P < D) P <<= 1;
+    for (int t = D + threadIdx.x; t < P; t += NT) { s_k[t] = INT_MAX; s_v[t] = 0; }
+    __syncthreads();
+    lds_bitonic<NT, acc>(s_k, s_v, P);
+    for (int t = threadIdx.x; t < D; t += NT) {
+      skey[base + t] = s_k[t];
+      sval[base + t] = s_v[t];
+    }
+    if (threadIdx.x == 0) dcnt[c] = D;
+    __syncthreads();
+  }
+}
+
+// Ordered extraction through a bitmap of C bits for rows with > DMAX distinct
+// keys (the global hash built by k_merge_big is still intact).
+template <class T>
+__global__ __launch_bounds__(1024) void k_merge_huge(const int64_t* __restrict__ eoff, int32_t C,
+                                                      const int32_t* __restrict__ big_list,
+                                                      const int32_t* __restrict__ huge_list, int nhuge,
+                                                      const int64_t* __restrict__ big_off,
+                                                      const int32_t* __restrict__ arena_keys,
+                                                      const typename T::acc_t* __restrict__ arena_vals,
+                                                      uint32_t* __restrict__ arena_bits,
+                                                      int32_t* __restrict__ skey,
+                                                      typename T::acc_t* __restrict__ sval) {
+  constexpr int NT = 1024;
+  __shared__ uint32_t bits_lds[BITMAP_LDS_WORDS];
+  __shared__ int sh[17];
+  const int nwords = (C + 31) >> 5;
+  uint32_t* bits = nwords <= BITMAP_LDS_WORDS ? bits_lds : arena_bits + (int64_t)blockIdx.x * nwords;
+  for (int h = blockIdx.x; h < nhuge; h += gridDim.x) {
+    const int b = huge_list[h];
+    const int c = big_list[b];
+    const int64_t base = eoff[c];
+    const int64_t toff = big_off[b];
+    const long long tsz = big_off[b + 1] - toff;
+    const int lg = clog2(tsz);
+    const int* keys = arena_keys + toff;
+    const typename T::acc_t* vals = arena_vals + toff;
+    for (int t = threadIdx.x; t < nwords; t += NT) bits[t] = 0;
+    __syncthreads();
+    for (long long t = threadIdx.x; t < tsz; t += NT) {
+      const int k = keys[t];
       if (k) atomicOr(&bits[(k - 1) >> 5], 1u << ((k - 1) & 31));
     }
     __syncthreads();
-    // ordered extraction: contiguous word chunk per thread
     const int per = (nwords + NT - 1) / NT;
     const int a0 = threadIdx.x * per, a1 = min(nwords, a0 + per);
     int s = 0;
@@ -268,14 +371,13 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
       __syncthreads();
       s = x - s + sh[wid];
     }
-    const int D = sh[16];
     int pos = s;
     for (int t = a0; t < a1; t++) {
       uint32_t wbits = bits[t];
       while (wbits) {
-        int bit = __ffs(wbits) - 1;
+        const int bit = __ffs(wbits) - 1;
         wbits &= wbits - 1;
-        int key = (t << 5) + bit;
+        const int key = (t << 5) + bit;
         unsigned p = chash(key, lg);
         while (keys[p] != key + 1) p = (p + 1) & (unsigned)(tsz - 1);
         skey[base + pos] = key;
@@ -283,7 +385,6 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
         pos++;
       }
     }
-    if (threadIdx.x == 0) dcnt[c] = D;
     __syncthreads();
   }
 }
@@ -294,27 +395,26 @@ __global__ void k_compact(const int64_t* __restrict__ eoff, const int64_t* __res
                           const int32_t* __restrict__ dcnt, int32_t C, const int32_t* __restrict__ skey,
                           const typename T::acc_t* __restrict__ sval, int32_t* __restrict__ nbr,
                           typename T::w_t* __restrict__ w) {
-  const int lane = threadIdx.x & 63;
-  const int nw = gridDim.x * (blockDim.x >> 6);
-  for (int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < C; c += nw) {
+  const int lane = threadIdx.x & 15;
+  const int ng = gridDim.x * (blockDim.x >> 4);
+  for (int c = blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4); c < C; c += ng) {
     const int64_t src = eoff[c], dst = noff[c];
     const int D = dcnt[c];
-    for (int t = lane; t < D; t += 64) {
+    for (int t = lane; t < D; t += 16) {
       nbr[dst + t] = skey[src + t];
       w[dst + t] = (typename T::w_t)sval[src + t];
     }
   }
 }
 
-__global__ void k_compose(int32_t* __restrict__ fin, int32_t n0, const int32_t* __restrict__ comm,
-                          const int32_t* __restrict__ nid) {
+__global__ void k_compose(int32_t* __restrict__ fin, int32_t n0, const int32_t* __restrict__ label) {
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n0; v += gridDim.x * blockDim.x)
-    fin[v] = nid[comm[fin[v]]];
+    fin[v] = label[fin[v]];
 }
 
-void compose_map(int32_t* fin, int32_t n0, const int32_t* comm, const int32_t* nid, hipStream_t s) {
+void compose_map(int32_t* fin, int32_t n0, const int32_t* label, hipStream_t s) {
   if (n0 <= 0) return;
-  k_compose<<<std::min((n0 + 255) / 256, 8192), 256, 0, s>>>(fin, n0, comm, nid);
+  k_compose<<<std::min((n0 + 255) / 256, 8192), 256, 0, s>>>(fin, n0, label);
 }
 
 static inline int nblk(int64_t n, int per = 256, int cap = 8192) {
@@ -324,62 +424,76 @@ static inline int nblk(int64_t n, int per = 256, int cap = 8192) {
 }
 
 template <class T>
-int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& cb, int32_t* C_out,
-                     int64_t* staged_out, hipStream_t s) {
+int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& cb, ContractPlan* plan,
+                     hipStream_t s) {
   const int32_t n = g.n;
   fill<int32_t>(cb.cnt, n, 0, s);
   k_hist<<<nblk(n), 256, 0, s>>>(comm, n, cb.cnt);
   uint8_t* flag = (uint8_t*)cb.i32_tmp;  // n bytes fit in the n-int scratch
   k_nonempty<<<nblk(n), 256, 0, s>>>(cb.cnt, n, flag);
   scan_exclusive<uint8_t, int32_t>(flag, cb.nid, n, cb.i32_tmp + (n + 3) / 4 + 1, cb.i32_scalar, s);
+  k_label<<<nblk(n), 256, 0, s>>>(comm, cb.nid, n, cb.label);
   int32_t C = 0;
   if (hipMemcpyAsync(&C, cb.i32_scalar, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
   fill<int64_t>(cb.ecnt, C + 1, 0, s);
-  k_ecount<T><<<nblk(n), 256, 0, s>>>(g, comm, cb.nid, cb.ecnt);
+  k_ecount<<<nblk(n), 256, 0, s>>>(g.off, n, cb.label, cb.ecnt);
   scan_exclusive<int64_t, int64_t>(cb.ecnt, cb.eoff, C, cb.i64_tmp, cb.eoff + C, s);
-  *C_out = C;
-  *staged_out = g.m;
+  fill<int32_t>(cb.nbig, 2, 0, s);  // nbig, nhuge
+  k_classify<<<nblk(C), 256, 0, s>>>(cb.ecnt, C, cb.big_list, cb.nbig, cb.i64_tmp);
+  int32_t nbig = 0;
+  if (hipMemcpyAsync(&nbig, cb.nbig, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  int64_t slots = 0;
+  if (nbig > 0) {
+    scan_exclusive<int64_t, int64_t>(cb.i64_tmp, cb.big_tab, nbig, cb.i64_tmp + nbig + 1, cb.big_tab + nbig, s);
+    if (hipMemcpyAsync(&slots, cb.big_tab + nbig, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+      return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  }
+  plan->C = C;
+  plan->nbig = nbig;
+  plan->slots = slots;
   return 0;
 }
 
 template <class T>
-int contract_build(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& cb, int32_t C,
-                   int64_t staged, ContractOut<T>& out, int64_t* m_out, hipStream_t s) {
+int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan& plan,
+                   ContractOut<T>& out, int64_t* m_out, hipStream_t s) {
   const int32_t n = g.n;
-  (void)staged;
+  const int32_t C = plan.C;
   fill<int64_t>(cb.ecur, C, 0, s);
-  k_stage_small<T><<<nblk(n), 256, 0, s>>>(g, comm, cb.nid, cb.eoff, cb.ecur, cb.skey, cb.sval);
-  k_stage_wide<T><<<nblk(n, 4, 16384), 256, 0, s>>>(g, comm, cb.nid, cb.eoff, cb.ecur, cb.skey, cb.sval);
+  fill<int32_t>(cb.nbig + 1, 1, 0, s);  // bitmap-queue length (a call may be repeated)
+  k_stage<16, T><<<nblk(n, 16, 16384), 256, 0, s>>>(g, cb.label, cb.eoff, cb.ecur, 1, 256, cb.skey, cb.sval);
+  k_stage<64, T><<<nblk(n, 4, 16384), 256, 0, s>>>(g, cb.label, cb.eoff, cb.ecur, 257, INT64_MAX, cb.skey,
+                                                     cb.sval);
   fill<int32_t>(cb.dcnt, C + 1, 0, s);
-  fill<int32_t>(cb.nbig, 1, 0, s);
-  k_classify<<<nblk(C), 256, 0, s>>>(cb.ecnt, C, cb.big_list, cb.nbig, cb.i64_tmp);
-  k_merge_small<64, SMALL_A, T><<<nblk(C, 1, 16384), 64, 0, s>>>(cb.ecnt, cb.eoff, C, 0, cb.skey, cb.sval, cb.dcnt);
-  k_merge_small<256, SMALL_B, T><<<nblk(C, 1, 4096), 256, 0, s>>>(cb.ecnt, cb.eoff, C, SMALL_A, cb.skey, cb.sval, cb.dcnt);
-  int32_t nbig = 0;
-  if (hipMemcpyAsync(&nbig, cb.nbig, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-  if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  if (nbig > 0) {
-    // table offsets: exclusive scan of sizes (written by k_classify into i64_tmp)
-    int64_t* big_off = cb.big_tab;
-    scan_exclusive<int64_t, int64_t>(cb.i64_tmp, big_off, nbig, cb.i64_tmp + nbig + 1, big_off + nbig, s);
-    int64_t slots = 0;
-    if (hipMemcpyAsync(&slots, big_off + nbig, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  k_merge_tiny<T><<<nblk(C, 4, 16384), 256, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval, cb.dcnt);
+  k_merge_mid<64, 64, 512, T><<<nblk(C, 1, 16384), 64, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval, cb.dcnt);
+  k_merge_mid<256, 512, 4096, T><<<nblk(C, 1, 4096), 256, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval,
+                                                                   cb.dcnt);
+  if (plan.nbig > 0) {
+    constexpr int DMAX = T::FP ? 8192 : 16384;
+    int32_t* huge_list = cb.big_list + plan.nbig;  // big_list has n entries, nbig <= n/4097
+    k_merge_big<DMAX, T><<<std::min(plan.nbig, 2048), 1024, 0, s>>>(
+        cb.ecnt, cb.eoff, C, cb.big_list, plan.nbig, cb.big_tab, cb.arena_keys, cb.arena_vals, cb.skey, cb.sval,
+        cb.dcnt, huge_list, cb.nbig + 1);
+    int32_t nhuge = 0;
+    if (hipMemcpyAsync(&nhuge, cb.nbig + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
-    const int nwords = (C + 31) / 32;
-    int grid = std::min(nbig, 1024);
-    if (nwords > BITMAP_LDS_WORDS) {
-      if (cb.arena_words < nwords) return -3;  // caller grows the bitmap arena
-      grid = (int)std::min<int64_t>(grid, std::max<int64_t>(1, cb.arena_words / nwords));
+    if (nhuge > 0) {
+      const int nwords = (C + 31) / 32;
+      int grid = std::min(nhuge, 1024);
+      if (nwords > BITMAP_LDS_WORDS) {
+        if (cb.arena_words < nwords) return -3;  // caller grows the bitmap arena and calls again
+        grid = (int)std::min<int64_t>(grid, cb.arena_words / nwords);
+      }
+      k_merge_huge<T><<<grid, 1024, 0, s>>>(cb.eoff, C, cb.big_list, huge_list, nhuge, cb.big_tab,
+                                            cb.arena_keys, cb.arena_vals, cb.arena_bits, cb.skey, cb.sval);
     }
-    if (slots > cb.arena_slots) return -2;  // caller grows the arena and retries
-    int* ovf = cb.i32_scalar + 3;
-    fill<int32_t>(ovf, 1, 0, s);
-    k_merge_big<T><<<grid, 1024, 0, s>>>(cb.ecnt, cb.eoff, C, cb.big_list, nbig, big_off, cb.arena_keys,
-                                         cb.arena_vals, cb.arena_bits, cb.skey, cb.sval, cb.dcnt, ovf);
   }
   scan_exclusive<int32_t, int64_t>(cb.dcnt, out.off, C, cb.i64_tmp, out.off + C, s);
-  k_compact<T><<<nblk(C, 4, 16384), 256, 0, s>>>(cb.eoff, out.off, cb.dcnt, C, cb.skey, cb.sval, out.nbr, out.w);
+  k_compact<T><<<nblk(C, 16, 16384), 256, 0, s>>>(cb.eoff, out.off, cb.dcnt, C, cb.skey, cb.sval, out.nbr, out.w);
   int64_t m = 0;
   if (hipMemcpyAsync(&m, out.off + C, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
@@ -387,12 +501,11 @@ int contract_build(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& c
   return 0;
 }
 
-// host-side helper: arena need of the current contraction (after classify)
-#define GALA_CINST(T)                                                                             \
-  template int contract_prepare<T>(const DevGraph<T>&, const int32_t*, ContractBufs<T>&, int32_t*, \
-                                   int64_t*, hipStream_t);                                        \
-  template int contract_build<T>(const DevGraph<T>&, const int32_t*, ContractBufs<T>&, int32_t,   \
-                                 int64_t, ContractOut<T>&, int64_t*, hipStream_t);
+#define GALA_CINST(T)                                                                               \
+  template int contract_prepare<T>(const DevGraph<T>&, const int32_t*, ContractBufs<T>&, ContractPlan*, \
+                                   hipStream_t);                                                    \
+  template int contract_build<T>(const DevGraph<T>&, ContractBufs<T>&, const ContractPlan&,         \
+                                 ContractOut<T>&, int64_t*, hipStream_t);
 GALA_CINST(IntW)
 GALA_CINST(FltW)
 

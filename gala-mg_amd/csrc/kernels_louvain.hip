@@ -59,6 +59,29 @@ __device__ __forceinline__ typename T::w_t load_w(const typename T::w_t* w, int6
   return w ? w[j] : (typename T::w_t)1;
 }
 
+// Scaled gain H(d) = w_d*m2 - K*(Tot_d - Tot_c + K).  Exact in int64 on the
+// integer path; on the FP path every operation is rounded separately (no FMA
+// contraction), the same sequence the FP64 oracle evaluates.
+template <class T>
+__device__ __forceinline__ typename T::gain_t gain_h(typename T::gain_t a, typename T::gain_t m2,
+                                                     typename T::gain_t K, typename T::gain_t Td,
+                                                     typename T::gain_t Tc) {
+  if constexpr (T::FP) {
+    const double t = __dadd_rn(__dsub_rn(Td, Tc), K);
+    return __dsub_rn(__dmul_rn(a, m2), __dmul_rn(K, t));
+  } else {
+    return a * m2 - K * (Td - Tc + K);
+  }
+}
+
+// Move test H(best) - m2*(w_c - S) > 0 (same rounding discipline).
+template <class T>
+__device__ __forceinline__ typename T::gain_t gain_net(typename T::gain_t h, typename T::gain_t m2,
+                                                       typename T::gain_t wc, typename T::gain_t S) {
+  if constexpr (T::FP) return __dsub_rn(h, __dmul_rn(m2, __dsub_rn(wc, S)));
+  else return h - m2 * (wc - S);
+}
+
 // Decision and state write-back shared by every decide kernel (one thread).
 template <class T>
 __device__ __forceinline__ bool commit_move(int v, int c, typename T::gain_t bestH, int bestD,
@@ -69,7 +92,7 @@ __device__ __forceinline__ bool commit_move(int v, int c, typename T::gain_t bes
   using G = typename T::gain_t;
   int nc = c;
   if (bestD != INT_MAX) {
-    G gain = bestH - (G)m2 * ((G)wc - (G)S);
+    G gain = gain_net<T>(bestH, (G)m2, (G)wc, (G)S);
     if (gain > 0 && !(size[bestD] == 1 && size[c] == 1 && bestD > c)) nc = bestD;
   }
   cur[v] = nc;
@@ -81,6 +104,14 @@ __device__ __forceinline__ bool commit_move(int v, int c, typename T::gain_t bes
     return true;
   }
   return false;
+}
+
+// A mover invalidates In of itself and of every neighbour (plain byte stores:
+// every writer stores 1).
+__device__ __forceinline__ void mark_row(uint8_t* dirty, const int32_t* __restrict__ nbr, int64_t o, int d,
+                                         int v, int lane, int width) {
+  for (int j = lane; j < d; j += width) dirty[nbr[o + j]] = 1;
+  if (lane == 0) dirty[v] = 1;
 }
 
 template <class V>
@@ -105,6 +136,7 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
     st.active[v] = 1;
     st.dTot[v] = 0;
     st.dsize[v] = 0;
+    st.dirty[v] = 0;
     if (d > 64) continue;  // wide rows: k_init_wide
     tt k = 0, self = 0;
     for (int64_t j = o; j < e; j++) {
@@ -179,10 +211,11 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
     const int c = st.prev[v];
-    int cl = -1;
+    int cl = -1, ul = 0;
     acc wl = 0;
     if (lane < d) {
-      cl = st.prev[g.nbr[o + lane]];
+      ul = g.nbr[o + lane];
+      cl = st.prev[ul];
       wl = (acc)load_w<T>(g.w, o + lane);
     }
     acc s = wl, wc = (cl == c) ? wl : acc(0);
@@ -202,15 +235,20 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
     G h = 0;
     int bd = INT_MAX;
     if (leader && cl != c) {
-      G hh = (G)s * (G)m2 - (G)K * ((G)st.Tot[cl] - (G)Tc + (G)K);
+      G hh = gain_h<T>((G)s, (G)m2, (G)K, (G)st.Tot[cl], (G)Tc);
       if (hh > 0) { h = hh; bd = cl; }
     }
     argmax_xor<G, W>(h, bd);
+    int mv = 0;
     if (lane == 0) {
       swept++;
       edges += d;
-      if (commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize))
-        moved++;
+      mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize);
+      moved += mv;
+    }
+    if (st.dirty && __shfl(mv, 0, W)) {  // In of the mover and its neighbours must be redone
+      if (lane < d) st.dirty[ul] = 1;
+      if (lane == 0) st.dirty[v] = 1;
     }
   }
   if (lane == 0) {
@@ -285,19 +323,21 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
         if (dd == c) {
           wc = a;
         } else {
-          G hh = (G)a * (G)m2 - (G)K * ((G)st.Tot[dd] - (G)Tc + (G)K);
+          G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)st.Tot[dd], (G)Tc);
           if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; }
         }
       }
     }
     argmax_xor<G, 64>(h, bd);
     wc = sum_xor<acc, 64>(wc);
+    int mv = 0;
     if (lane == 0) {
       swept++;
       edges += d;
-      if (commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize))
-        moved++;
+      mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize);
+      moved += mv;
     }
+    if (st.dirty && __shfl(mv, 0, 64)) mark_row(st.dirty, g.nbr, o, d, v, lane, 64);
     lds_fence_wave();
   }
   if (lane == 0) {
@@ -333,6 +373,7 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
   __shared__ G r_h[NT / 64];
   __shared__ int r_d[NT / 64];
   __shared__ acc r_wc[NT / 64];
+  __shared__ int s_mv;
   const int lo = bstart[bin], hi = bstart[bin + 1];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   unsigned moved = 0, swept = 0;
@@ -381,7 +422,7 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
         if (dd == c) {
           wc = a;
         } else {
-          G hh = (G)a * (G)m2 - (G)K * ((G)st.Tot[dd] - (G)Tc + (G)K);
+          G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)st.Tot[dd], (G)Tc);
           if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; }
         }
       }
@@ -397,10 +438,244 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
       }
       swept++;
       edges += d;
-      if (commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize))
-        moved++;
+      const int mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize);
+      moved += mv;
+      s_mv = mv;
     }
     __syncthreads();
+    if (st.dirty && s_mv) mark_row(st.dirty, g.nbr, o, d, v, threadIdx.x, NT);
+  }
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, moved);
+    atomic_add_ull(&st.ctr->swept, swept);
+    atomic_add_ull(&st.ctr->swept_edges, edges);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// decide-and-move for hubs: one 1024-thread workgroup per vertex, a LT-slot
+// LDS table tried first (bounded probing) and the vertex's slice of the global
+// arena for the overflow.  Global slices are kept all-zero between uses: the
+// scan that reads an overflow slot clears it.
+// ---------------------------------------------------------------------------
+template <int LT, class T>
+__global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> st,
+                                                      const int32_t* __restrict__ front,
+                                                      const int32_t* __restrict__ bstart, int bin,
+                                                      typename T::tot_t m2) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  constexpr int NT = 1024, PMAX = 32;
+  __shared__ int s_keys[LT];
+  __shared__ acc s_vals[LT];
+  __shared__ G r_h[NT / 64];
+  __shared__ int r_d[NT / 64];
+  __shared__ acc r_wc[NT / 64];
+  __shared__ int s_ovf, s_mv;
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned moved = 0, swept = 0;
+  unsigned long long edges = 0;
+  for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
+    const int v = front[i];
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    const int c = st.prev[v];
+    const int glg = ilog2_ceil(2 * d);
+    const int gsz = 1 << glg;
+    const int llg = glg < __builtin_ctz(LT) ? glg : __builtin_ctz(LT);
+    const int lsz = 1 << llg;
+    int* gkeys = st.hub_keys + st.hub_off[v];
+    acc* gvals = st.hub_vals + st.hub_off[v];
+    for (int t = threadIdx.x; t < lsz; t += NT) { s_keys[t] = 0; s_vals[t] = 0; }
+    if (threadIdx.x == 0) s_ovf = 0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < d; j += NT) {
+      const int cl = st.prev[g.nbr[o + j]];
+      const acc w = (acc)load_w<T>(g.w, o + j);
+      unsigned p = hash_slot(cl, llg);
+      bool placed = false;
+      for (int probe = 0; probe < PMAX; probe++) {
+        const int k = atomicCAS(&s_keys[p], 0, cl + 1);
+        if (k == 0 || k == cl + 1) { atomicAdd(&s_vals[p], w); placed = true; break; }
+        p = (p + 1) & (lsz - 1);
+      }
+      if (!placed) {
+        s_ovf = 1;
+        unsigned q = hash_slot(cl, glg);
+        int probe = 0;
+        for (; probe < gsz; probe++) {
+          const int k = atomicCAS(&gkeys[q], 0, cl + 1);
+          if (k == 0 || k == cl + 1) { atomicAdd(&gvals[q], w); break; }
+          q = (q + 1) & (gsz - 1);
+        }
+        if (probe == gsz) st.ctr->overflow = 1;
+      }
+    }
+    __syncthreads();
+    const tt K = st.K[v];
+    const tt Tc = st.Tot[c];
+    G h = 0;
+    int bd = INT_MAX;
+    acc wc = 0;
+    for (int t = threadIdx.x; t < lsz; t += NT) {
+      const int k = s_keys[t];
+      if (k) {
+        const int dd = k - 1;
+        const acc a = s_vals[t];
+        if (dd == c) {
+          wc += a;
+        } else {
+          G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)st.Tot[dd], (G)Tc);
+          if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; }
+        }
+      }
+    }
+    if (s_ovf) {
+      for (int t = threadIdx.x; t < gsz; t += NT) {
+        const int k = gkeys[t];
+        if (k) {
+          const int dd = k - 1;
+          const acc a = gvals[t];
+          gkeys[t] = 0;
+          gvals[t] = 0;
+          if (dd == c) {
+            wc += a;
+          } else {
+            G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)st.Tot[dd], (G)Tc);
+            if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; }
+          }
+        }
+      }
+    }
+    argmax_xor<G, 64>(h, bd);
+    wc = sum_xor<acc, 64>(wc);
+    if (lane == 0) { r_h[wid] = h; r_d[wid] = bd; r_wc[wid] = wc; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 1; q < NT / 64; q++) {
+        if (better(r_h[q], r_d[q], h, bd)) { h = r_h[q]; bd = r_d[q]; }
+        wc += r_wc[q];
+      }
+      swept++;
+      edges += d;
+      const int mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize);
+      moved += mv;
+      s_mv = mv;
+    }
+    __syncthreads();
+    if (st.dirty && s_mv) mark_row(st.dirty, g.nbr, o, d, v, threadIdx.x, NT);
+  }
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, moved);
+    atomic_add_ull(&st.ctr->swept, swept);
+    atomic_add_ull(&st.ctr->swept_edges, edges);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// First sweep of a round (every vertex its own community, rows duplicate-free):
+// each entry is its own candidate, so no aggregation and no prev[] gathers:
+// H(u) = w(v,u) m2 - K (Tot_u - Tot_v + K), w_c = Self[v].  Streaming argmax.
+// ---------------------------------------------------------------------------
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_decide_first(DevGraph<T> g, DevState<T> st,
+                                                       const int32_t* __restrict__ front,
+                                                       const int32_t* __restrict__ bstart, int bin,
+                                                       typename T::tot_t m2) {
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  __shared__ unsigned long long s_moved, s_swept, s_edges;
+  if (threadIdx.x == 0) { s_moved = 0; s_swept = 0; s_edges = 0; }
+  __syncthreads();
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & (W - 1);
+  const int groups = gridDim.x * (blockDim.x / W);
+  unsigned moved = 0, swept = 0, edges = 0;
+  for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W; i < hi; i += groups) {
+    const int v = front[i];
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    const tt K = st.K[v];
+    const tt Tc = st.Tot[v];
+    G h = 0;
+    int bd = INT_MAX;
+    for (int j = lane; j < d; j += W) {
+      const int u = g.nbr[o + j];
+      if (u != v) {
+        const G hh = gain_h<T>((G)load_w<T>(g.w, o + j), (G)m2, (G)K, (G)st.Tot[u], (G)Tc);
+        if (hh > 0 && better(hh, u, h, bd)) { h = hh; bd = u; }
+      }
+    }
+    argmax_xor<G, W>(h, bd);
+    int mv = 0;
+    if (lane == 0) {
+      swept++;
+      edges += d;
+      const tt S = st.Self[v];
+      mv = commit_move<T>(v, v, h, bd, (typename T::acc_t)S, K, S, m2, st.size, st.cur, st.dTot, st.dsize);
+      moved += mv;
+    }
+    if (st.dirty && __shfl(mv, 0, W)) mark_row(st.dirty, g.nbr, o, d, v, lane, W);
+  }
+  if (lane == 0) {
+    if (moved) atomicAdd(&s_moved, (unsigned long long)moved);
+    if (swept) atomicAdd(&s_swept, (unsigned long long)swept);
+    if (edges) atomicAdd(&s_edges, (unsigned long long)edges);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, s_moved);
+    atomic_add_ull(&st.ctr->swept, s_swept);
+    atomic_add_ull(&st.ctr->swept_edges, s_edges);
+  }
+}
+
+template <int NT, class T>
+__global__ __launch_bounds__(NT) void k_decide_first_block(DevGraph<T> g, DevState<T> st,
+                                                            const int32_t* __restrict__ front,
+                                                            const int32_t* __restrict__ bstart, int bin,
+                                                            typename T::tot_t m2) {
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  __shared__ G r_h[NT / 64];
+  __shared__ int r_d[NT / 64];
+  __shared__ int s_mv;
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned moved = 0, swept = 0;
+  unsigned long long edges = 0;
+  for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
+    const int v = front[i];
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    const tt K = st.K[v];
+    const tt Tc = st.Tot[v];
+    G h = 0;
+    int bd = INT_MAX;
+    for (int j = threadIdx.x; j < d; j += NT) {
+      const int u = g.nbr[o + j];
+      if (u != v) {
+        const G hh = gain_h<T>((G)load_w<T>(g.w, o + j), (G)m2, (G)K, (G)st.Tot[u], (G)Tc);
+        if (hh > 0 && better(hh, u, h, bd)) { h = hh; bd = u; }
+      }
+    }
+    argmax_xor<G, 64>(h, bd);
+    if (lane == 0) { r_h[wid] = h; r_d[wid] = bd; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 1; q < NT / 64; q++)
+        if (better(r_h[q], r_d[q], h, bd)) { h = r_h[q]; bd = r_d[q]; }
+      swept++;
+      edges += d;
+      const tt S = st.Self[v];
+      const int mv = commit_move<T>(v, v, h, bd, (typename T::acc_t)S, K, S, m2, st.size, st.cur, st.dTot, st.dsize);
+      moved += mv;
+      s_mv = mv;
+    }
+    __syncthreads();
+    if (st.dirty && s_mv) mark_row(st.dirty, g.nbr, o, d, v, threadIdx.x, NT);
   }
   if (threadIdx.x == 0) {
     atomic_add_ull(&st.ctr->moved, moved);
@@ -419,6 +694,7 @@ template <class T>
 void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
                    const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s) {
   if (cap <= 0) return;
+  constexpr int HUB_LT = T::FP ? 8192 : 16384;
   switch (bin) {
     case 0: k_decide_ring<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, 0, m2); break;
     case 1: k_decide_ring<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, 1, m2); break;
@@ -429,11 +705,28 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
     case 6: k_decide_block<256, 8192, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, 6, m2); break;
     case 7:
       if constexpr (T::FP)
-        k_decide_block<1024, 0, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, 7, m2);
+        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, 7, m2);
       else
         k_decide_block<1024, 16384, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, 7, m2);
       break;
-    case 8: k_decide_block<1024, 0, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, 8, m2); break;
+    case 8: k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, 8, m2); break;
+  }
+}
+
+template <class T>
+void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
+                         const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s) {
+  if (cap <= 0) return;
+  switch (bin) {
+    case 0: k_decide_first<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, 0, m2); break;
+    case 1: k_decide_first<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, 1, m2); break;
+    case 2: k_decide_first<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, 2, m2); break;
+    case 3: k_decide_first<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, 3, m2); break;
+    case 4: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, 4, m2); break;
+    case 5: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, 5, m2); break;
+    case 6: k_decide_first_block<256, T><<<grid_for(cap, 1, 4096), 256, 0, s>>>(g, st, front, bstart, 6, m2); break;
+    case 7: k_decide_first_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, 7, m2); break;
+    case 8: k_decide_first_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, 8, m2); break;
   }
 }
 
@@ -536,20 +829,14 @@ template <int W, class T>
 __global__ __launch_bounds__(256) void k_weight_group(DevGraph<T> g, DevState<T> st,
                                                        const int32_t* __restrict__ order,
                                                        const int32_t* __restrict__ bstart, int bin,
-                                                       typename T::tot_t m2) {
+                                                       int only_dirty) {
   using tt = typename T::tot_t;
-  __shared__ unsigned long long s_in, s_act;
-  __shared__ double s_inf;
-  if (threadIdx.x == 0) { s_in = 0; s_act = 0; s_inf = 0; }
-  __syncthreads();
-  const tt minTot = read_min_tot<T>(st.ctr);
   const int lo = bstart[bin], hi = bstart[bin + 1];
   const int lane = threadIdx.x & (W - 1);
   const int groups = gridDim.x * (blockDim.x / W);
-  tt insum = 0;
-  unsigned act = 0;
   for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W; i < hi; i += groups) {
     const int v = order[i];
+    if (only_dirty && !st.dirty[v]) continue;  // uniform per group
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
     const int c = st.cur[v];
@@ -557,24 +844,7 @@ __global__ __launch_bounds__(256) void k_weight_group(DevGraph<T> g, DevState<T>
     for (int j = lane; j < d; j += W)
       if (st.cur[g.nbr[o + j]] == c) x += (tt)load_w<T>(g.w, o + j);
     x = sum_xor<tt, W>(x);
-    if (lane == 0) {
-      st.In[v] = x;
-      uint8_t keep = prune_keep<T>(x, st.K[v], st.Self[v], st.Tot[c], minTot, m2);
-      st.active[v] = keep;
-      act += keep;
-      insum += x;
-    }
-  }
-  if (lane == 0) {
-    if (T::FP) { if (insum != 0) atomicAdd(&s_inf, (double)insum); }
-    else if (insum) atomicAdd(&s_in, (unsigned long long)(long long)insum);
-    if (act) atomicAdd(&s_act, (unsigned long long)act);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (T::FP) { if (s_inf != 0) atomicAdd(&st.ctr->sum_in_f, s_inf); }
-    else atomic_add_ull(&st.ctr->sum_in, s_in);
-    atomic_add_ull(&st.ctr->active, s_act);
+    if (lane == 0) st.In[v] = x;
   }
 }
 
@@ -582,16 +852,14 @@ template <int NT, class T>
 __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> st,
                                                       const int32_t* __restrict__ order,
                                                       const int32_t* __restrict__ bstart, int bin,
-                                                      typename T::tot_t m2) {
+                                                      int only_dirty) {
   using tt = typename T::tot_t;
   __shared__ tt r[NT / 64];
-  const tt minTot = read_min_tot<T>(st.ctr);
   const int lo = bstart[bin], hi = bstart[bin + 1];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  tt insum = 0;
-  unsigned act = 0;
   for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
     const int v = order[i];
+    if (only_dirty && !st.dirty[v]) continue;  // uniform per block
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
     const int c = st.cur[v];
@@ -605,35 +873,69 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
       x = 0;
       for (int q = 0; q < NT / 64; q++) x += r[q];
       st.In[v] = x;
-      uint8_t keep = prune_keep<T>(x, st.K[v], st.Self[v], st.Tot[c], minTot, m2);
-      st.active[v] = keep;
-      act += keep;
-      insum += x;
     }
     __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    if (T::FP) { if (insum != 0) atomicAdd(&st.ctr->sum_in_f, (double)insum); }
-    else if (insum) atomicAdd(&st.ctr->sum_in, (unsigned long long)(long long)insum);
-    if (act) atomicAdd(&st.ctr->active, (unsigned long long)act);
   }
 }
 
 template <class T>
 void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
-                   const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s) {
+                   const int32_t* bstart, int cap, bool only_dirty, hipStream_t s) {
   if (cap <= 0) return;
+  const int od = only_dirty ? 1 : 0;
   switch (bin) {
-    case 0: k_weight_group<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, order, bstart, 0, m2); break;
-    case 1: k_weight_group<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, order, bstart, 1, m2); break;
-    case 2: k_weight_group<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, order, bstart, 2, m2); break;
-    case 3: k_weight_group<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, order, bstart, 3, m2); break;
-    case 4: k_weight_group<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, order, bstart, 4, m2); break;
-    case 5: k_weight_group<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, order, bstart, 5, m2); break;
-    case 6: k_weight_block<256, T><<<grid_for(cap, 1, 4096), 256, 0, s>>>(g, st, order, bstart, 6, m2); break;
-    case 7: k_weight_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, order, bstart, 7, m2); break;
-    case 8: k_weight_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, order, bstart, 8, m2); break;
+    case 0: k_weight_group<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, order, bstart, 0, od); break;
+    case 1: k_weight_group<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, order, bstart, 1, od); break;
+    case 2: k_weight_group<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, order, bstart, 2, od); break;
+    case 3: k_weight_group<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, order, bstart, 3, od); break;
+    case 4: k_weight_group<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, order, bstart, 4, od); break;
+    case 5: k_weight_group<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, order, bstart, 5, od); break;
+    case 6: k_weight_block<256, T><<<grid_for(cap, 1, 4096), 256, 0, s>>>(g, st, order, bstart, 6, od); break;
+    case 7: k_weight_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, order, bstart, 7, od); break;
+    case 8: k_weight_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, order, bstart, 8, od); break;
   }
+}
+
+// ---------------------------------------------------------------------------
+// prune + sum In over every vertex of positive degree, clears the dirty marks.
+// (the prune test of compute_In, kernel_functions.cu:944,1028)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_prune(DevState<T> st, int32_t n, typename T::tot_t m2) {
+  using tt = typename T::tot_t;
+  __shared__ unsigned long long s_in[4], s_act[4];
+  __shared__ double s_inf[4];
+  const tt minTot = read_min_tot<T>(st.ctr);
+  unsigned long long insum = 0, act = 0;
+  double insumf = 0;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    if (st.binid[v] < 0) continue;
+    const tt x = st.In[v];
+    const uint8_t keep = prune_keep<T>(x, st.K[v], st.Self[v], st.Tot[st.cur[v]], minTot, m2);
+    st.active[v] = keep;
+    st.dirty[v] = 0;
+    act += keep;
+    if (T::FP) insumf += (double)x;
+    else insum += (unsigned long long)(long long)x;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  insum = sum_xor<unsigned long long, 64>(insum);
+  act = sum_xor<unsigned long long, 64>(act);
+  insumf = sum_xor<double, 64>(insumf);
+  if (lane == 0) { s_in[wid] = insum; s_act[wid] = act; s_inf[wid] = insumf; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < 4; q++) { insum += s_in[q]; act += s_act[q]; insumf += s_inf[q]; }
+    if (T::FP) { if (insumf != 0) atomicAdd(&st.ctr->sum_in_f, insumf); }
+    else atomic_add_ull(&st.ctr->sum_in, insum);
+    atomic_add_ull(&st.ctr->active, act);
+  }
+}
+
+template <class T>
+void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s) {
+  if (n <= 0) return;
+  k_prune<T><<<std::min((n + 255) / 256, 4096), 256, 0, s>>>(st, n, m2);
 }
 
 // explicit instantiations
@@ -641,9 +943,12 @@ void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
   template void launch_init<T>(const DevGraph<T>&, DevState<T>&, hipStream_t);                    \
   template void launch_decide<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,           \
                                  const int32_t*, int, typename T::tot_t, hipStream_t);            \
+  template void launch_decide_first<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,     \
+                                       const int32_t*, int, typename T::tot_t, hipStream_t);      \
   template void launch_apply<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t);           \
   template void launch_weight<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,           \
-                                 const int32_t*, int, typename T::tot_t, hipStream_t);
+                                 const int32_t*, int, bool, hipStream_t);                         \
+  template void launch_prune<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t);
 GALA_INST(IntW)
 GALA_INST(FltW)
 

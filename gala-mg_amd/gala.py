@@ -47,7 +47,8 @@ class Result(ctypes.Structure):
                 ("t_contract_ms", ctypes.c_double), ("t_h2d_ms", ctypes.c_double),
                 ("swept_vertices", ctypes.c_int64), ("swept_edges", ctypes.c_int64),
                 ("input_edges", ctypes.c_int64), ("swept_vertices_unit", ctypes.c_int64),
-                ("swept_edges_unit", ctypes.c_int64), ("round_q", ctypes.c_double * 64),
+                ("swept_edges_unit", ctypes.c_int64), ("decide_bytes", ctypes.c_int64),
+                ("round_q", ctypes.c_double * 64),
                 ("round_nc", ctypes.c_int32 * 64), ("round_iters", ctypes.c_int32 * 64),
                 ("round_ms", ctypes.c_double * 64), ("round_decide_ms", ctypes.c_double * 64),
                 ("round_update_ms", ctypes.c_double * 64), ("round_build_ms", ctypes.c_double * 64)]
@@ -61,7 +62,8 @@ class Result(ctypes.Structure):
                     t_contract_ms=self.t_contract_ms, t_h2d_ms=self.t_h2d_ms,
                     swept_vertices=self.swept_vertices, swept_edges=self.swept_edges,
                     input_edges=self.input_edges, swept_vertices_unit=self.swept_vertices_unit,
-                    swept_edges_unit=self.swept_edges_unit, round_q=list(self.round_q[:r]),
+                    swept_edges_unit=self.swept_edges_unit, decide_bytes=self.decide_bytes,
+                    round_q=list(self.round_q[:r]),
                     round_nc=list(self.round_nc[:r]), round_iters=list(self.round_iters[:r]),
                     round_ms=list(self.round_ms[:r]),
                     round_decide_ms=list(self.round_decide_ms[:r]),

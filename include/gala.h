@@ -79,6 +79,10 @@ typedef struct {
   int64_t input_edges;     /* undirected input edges (directed entries / 2) */
   int64_t swept_vertices_unit; /* ...of which on graphs with implicit unit weights */
   int64_t swept_edges_unit;
+  int64_t decide_bytes;    /* algorithmic bytes of all decide-and-move sweeps:
+                              16d+40 per swept vertex of degree d (12d+40 with implicit
+                              unit weights); first sweeps of a round need no prev[]
+                              gathers: 12d+40 (8d+40 unit) */
   double round_q[64];
   int32_t round_nc[64];
   int32_t round_iters[64];

@@ -106,7 +106,8 @@ def random_state(rng, g, ncomm):
     n = g.vertex_num
     off = g.offsets()
     deg = np.diff(off)
-    K = deg.astype(np.int64) if g.weights is None else np.add.reduceat(g.weights.astype(np.int64), off[:-1]) * (deg > 0)
+    src = np.repeat(np.arange(n), deg)
+    K = deg.astype(np.int64) if g.weights is None else np.bincount(src, weights=g.weights, minlength=n).astype(np.int64)
     Self = np.zeros(n, np.int64)
     prev = rng.integers(0, ncomm, n).astype(np.int32)
     Tot = np.bincount(prev, weights=K, minlength=n).astype(np.int64)
@@ -192,4 +193,81 @@ def test_determinism_twice(golden):
     r2 = eng.run()  # same context, input buffers are read-only
     c2 = eng.communities(g.vertex_num)
     assert r1["modularity"] == r2["modularity"] and np.array_equal(c1, c2)
+    eng.close()
+
+
+def star_plus_random(nleaf, extra, seed):
+    """Hub 0 joined to 1..nleaf, plus `extra` random edges among the leaves."""
+    rng = np.random.default_rng(seed)
+    n = nleaf + 1
+    e = {(0, v) for v in range(1, n)}
+    while len(e) < nleaf + extra:
+        a, b = rng.integers(1, n, 2)
+        if a != b:
+            e.add((min(a, b), max(a, b)))
+    rows = [[] for _ in range(n)]
+    for a, b in e:
+        rows[a].append(b)
+        rows[b].append(a)
+    end = np.cumsum([len(r) for r in rows]).astype(np.int64)
+    nbr = np.concatenate([np.sort(np.array(r, np.int32)) for r in rows]).astype(np.int32)
+    return gala.Graph(end, nbr)
+
+
+def test_phase_decide_hub_overflow(engine):
+    # a 50k-degree hub whose neighbours sit in ~40k distinct communities: more
+    # than the 16384-slot LDS table, so part of its row goes to the global slice
+    say("phase decide hub overflow")
+    g = star_plus_random(50000, 60000, 1)
+    rng = np.random.default_rng(5)
+    K, Self, Tot, size, prev = random_state(rng, g, 40000)
+    m2 = int(K.sum())
+    w = np.ones(len(g.neighbors), np.int64)
+    for rep in range(2):  # the second call checks the overflow slice was left zeroed
+        got = engine.phase_decide(g, m2, K, Self, Tot, size, prev)
+        exp = oracle.decide_step(g.vertex_num, g.degrees, g.neighbors, w, m2, K, Self, Tot, size, prev)
+        assert np.array_equal(got, exp), rep
+
+
+def test_phase_contract_huge_row(engine):
+    # the hub's community row has 50k distinct neighbours: bitmap extraction path
+    say("phase contract huge row")
+    g = star_plus_random(50000, 20000, 2)
+    comm = np.arange(g.vertex_num, dtype=np.int32)
+    comm[1:50] = 0
+    nid, C, end, nb, ww = engine.phase_contract(g, comm)
+    rnid, rC, rend, rnb, rww = oracle.contract(g.vertex_num, g.degrees, g.neighbors,
+                                               np.ones(len(g.neighbors), np.int64), comm)
+    assert C == rC and np.array_equal(end, rend) and np.array_equal(nb, rnb) and np.array_equal(ww, rww)
+
+
+def test_star_graph_full_run():
+    say("star full run")
+    g = star_plus_random(30000, 40000, 3)
+    eng = gala.Engine(dump_iterations=True)
+    full_trace_compare(eng, g)
+    eng.close()
+
+
+@pytest.mark.parametrize("scale", [12, 15])
+def test_weighted_fp32_rmat(scale):
+    # fp32 weights U[0.5, 2], FP64 accumulation: final Q within 1e-9 of the FP64 oracle
+    say(f"weighted fp32 rmat scale {scale}")
+    g = gala.gen_rmat(scale, 16, seed=scale)
+    eng = gala.Engine()
+    eng.set_graph(g)
+    res = eng.run()
+    ref = oracle.louvain_f(g.vertex_num, g.degrees, g.neighbors, g.weights)
+    assert abs(res["modularity"] - ref["Q"]) <= 1e-9, (res["modularity"], ref["Q"])
+    assert res["n_communities"] == ref["nc"]
+    comm = eng.communities(g.vertex_num)
+    # the reported Q is the modularity of the returned partition
+    off = g.offsets()
+    src = np.repeat(np.arange(g.vertex_num), np.diff(off))
+    w = g.weights.astype(np.float64)
+    m2 = w.sum()
+    K = np.bincount(src, weights=w, minlength=g.vertex_num)
+    tot = np.bincount(comm, weights=K, minlength=g.vertex_num)
+    q = w[comm[g.neighbors] == comm[src]].sum() / m2 - (tot ** 2).sum() / m2 ** 2
+    assert abs(q - res["modularity"]) <= 1e-9
     eng.close()
