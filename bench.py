@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gala-mg_amd"))
 
 HBM_PEAK_GBS = 8000.0
+BIN_DEG = ["1-3", "4-7", "8-15", "16-31", "32-127", "128-511", "512-4095", "4096-8191", ">=8192"]
 
 
 def log(msg):
@@ -76,6 +77,16 @@ def main():
     wall = time.perf_counter() - t1
     comm = eng.communities(g.vertex_num)
     eng.close()
+    # one more run with per-bin HIP events (outside the timed steps)
+    peng = gala.Engine(device=0, profile_bins=True)
+    peng.set_graph(g)
+    prof = peng.run()
+    peng.close()
+    bins = []
+    for b in range(9):
+        ms_b, by = prof["bin_decide_ms"][b], prof["bin_decide_bytes"][b]
+        bins.append({"bin": b, "degrees": BIN_DEG[b], "bytes": int(by), "entries": int(prof["bin_swept_edges"][b]),
+                     "ms": ms_b, "GBps": (by / ms_b / 1e6) if ms_b > 0 else 0.0})
 
     ms = float(np.mean([r["t_total_ms"] for r in runs]))
     wall_ms = wall * 1e3 / args.steps
@@ -133,6 +144,10 @@ def main():
         "iterations": last["total_iterations"],
         "phase_ms": {"decide": last["t_decide_ms"], "update": last["t_update_ms"],
                      "contract": last["t_contract_ms"]},
+        "decide_bins": bins,
+        "rounds_detail": {"communities": last["round_nc"], "iterations": last["round_iters"],
+                          "phase1_ms": last["round_ms"], "decide_ms": last["round_decide_ms"],
+                          "update_ms": last["round_update_ms"], "contract_ms": last["round_build_ms"]},
         "parity_vs_oracle": parity,
     }
     print(json.dumps(out))

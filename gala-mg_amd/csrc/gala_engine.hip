@@ -93,8 +93,15 @@ struct gala_ctx {
   void* c_sval = nullptr;
   int32_t* arena_keys = nullptr;
   void* arena_vals = nullptr;
-  uint32_t* arena_bits = nullptr;
-  int64_t arena_slots = 0, arena_words = 0;
+  int64_t arena_slots = 0;
+  void* dense = nullptr;  // dense-row accumulators (all-zero between uses)
+  int64_t dense_elems = 0;
+  DenseChunk* chunks = nullptr;
+  int64_t chunks_cap = 0;
+  int32_t* dense_c = nullptr;
+  int32_t* dense_tcnt = nullptr;
+  int64_t* dense_toff = nullptr;
+  int64_t dense_cap = 0;  // slots of dense_c, tiles of dense_tcnt/toff
 
   std::vector<void*> owned;
   gala_result res{};
@@ -102,6 +109,7 @@ struct gala_ctx {
   std::vector<int32_t> comm_host;
 
   hipEvent_t ev[8];
+  hipEvent_t bev[NBINS + 1];  // per-bin decide timing (opts.profile_bins)
 };
 
 namespace {
@@ -116,9 +124,12 @@ void free_all(gala_ctx* ctx) {
   ctx->hub_keys = nullptr; ctx->hub_vals = nullptr; ctx->hub_cap = 0;
   if (ctx->arena_keys) (void)hipFree(ctx->arena_keys);
   if (ctx->arena_vals) (void)hipFree(ctx->arena_vals);
-  if (ctx->arena_bits) (void)hipFree(ctx->arena_bits);
-  ctx->arena_keys = nullptr; ctx->arena_vals = nullptr; ctx->arena_bits = nullptr;
-  ctx->arena_slots = 0; ctx->arena_words = 0;
+  ctx->arena_keys = nullptr; ctx->arena_vals = nullptr;
+  ctx->arena_slots = 0;
+  for (void* p : {ctx->dense, (void*)ctx->chunks, (void*)ctx->dense_c, (void*)ctx->dense_tcnt, (void*)ctx->dense_toff})
+    if (p) (void)hipFree(p);
+  ctx->dense = nullptr; ctx->chunks = nullptr; ctx->dense_c = nullptr; ctx->dense_tcnt = nullptr;
+  ctx->dense_toff = nullptr; ctx->dense_elems = 0; ctx->chunks_cap = 0; ctx->dense_cap = 0;
   ctx->have_graph = false;
 }
 
@@ -196,7 +207,7 @@ int grow_hub(gala_ctx* ctx, int64_t slots, bool fp) {
   return GALA_OK;
 }
 
-int grow_arena(gala_ctx* ctx, int64_t slots, int64_t words, bool fp) {
+int grow_arena(gala_ctx* ctx, int64_t slots, bool fp) {
   if (slots > ctx->arena_slots) {
     if (ctx->arena_keys) (void)hipFree(ctx->arena_keys);
     if (ctx->arena_vals) (void)hipFree(ctx->arena_vals);
@@ -205,14 +216,52 @@ int grow_arena(gala_ctx* ctx, int64_t slots, int64_t words, bool fp) {
     hipError_t e = dalloc(&ctx->arena_keys, cap);
     if (e == hipSuccess) { char* p; e = dalloc(&p, cap * (fp ? 8 : 4)); ctx->arena_vals = p; }
     if (e != hipSuccess) { ctx->last_hip = e; ctx->arena_slots = 0; return GALA_E_OOM; }
+    // the row-merge kernels keep arena slices all-zero between uses
+    e = hipMemset(ctx->arena_keys, 0, sizeof(int32_t) * cap);
+    if (e == hipSuccess) e = hipMemset(ctx->arena_vals, 0, (size_t)cap * (fp ? 8 : 4));
+    if (e != hipSuccess) { ctx->last_hip = e; return GALA_E_HIP; }
     ctx->arena_slots = cap;
   }
-  if (words > ctx->arena_words) {
-    if (ctx->arena_bits) (void)hipFree(ctx->arena_bits);
-    ctx->arena_bits = nullptr;
-    hipError_t e = dalloc(&ctx->arena_bits, words);
-    if (e != hipSuccess) { ctx->last_hip = e; ctx->arena_words = 0; return GALA_E_OOM; }
-    ctx->arena_words = words;
+  return GALA_OK;
+}
+
+// Dense-row buffers for a contraction plan: accumulators for as many rows at
+// once as fit in DENSE_BYTES (HBM is plentiful), the chunk table, and tiles.
+constexpr int64_t DENSE_BYTES = 8ll << 30;
+int grow_dense(gala_ctx* ctx, const ContractPlan& plan, bool fp, int64_t* per_batch) {
+  const int64_t esz = fp ? 8 : 4;
+  const int64_t C = std::max<int32_t>(plan.C, 1);
+  int64_t pb = std::max<int64_t>(1, std::min<int64_t>(plan.ndense, DENSE_BYTES / (C * esz)));
+  *per_batch = pb;
+  const int64_t elems = pb * C;
+  hipError_t e = hipSuccess;
+  if (elems > ctx->dense_elems) {
+    if (ctx->dense) (void)hipFree(ctx->dense);
+    ctx->dense = nullptr;
+    e = hipMalloc(&ctx->dense, (size_t)(elems * esz));
+    if (e == hipSuccess) e = hipMemset(ctx->dense, 0, (size_t)(elems * esz));
+    if (e != hipSuccess) { ctx->last_hip = e; ctx->dense_elems = 0; return GALA_E_OOM; }
+    ctx->dense_elems = elems;
+  }
+  const int64_t nch = (int64_t)plan.chunks.size();
+  if (nch > ctx->chunks_cap) {
+    if (ctx->chunks) (void)hipFree(ctx->chunks);
+    ctx->chunks = nullptr;
+    e = dalloc(&ctx->chunks, nch);
+    if (e != hipSuccess) { ctx->last_hip = e; ctx->chunks_cap = 0; return GALA_E_OOM; }
+    ctx->chunks_cap = nch;
+  }
+  const int64_t tiles = pb * ((C + 4095) / 4096) + 1;
+  const int64_t need = std::max<int64_t>(tiles, pb);
+  if (need > ctx->dense_cap) {
+    for (void* p : {(void*)ctx->dense_c, (void*)ctx->dense_tcnt, (void*)ctx->dense_toff})
+      if (p) (void)hipFree(p);
+    ctx->dense_c = nullptr; ctx->dense_tcnt = nullptr; ctx->dense_toff = nullptr;
+    e = dalloc(&ctx->dense_c, need);
+    if (e == hipSuccess) e = dalloc(&ctx->dense_tcnt, need);
+    if (e == hipSuccess) e = dalloc(&ctx->dense_toff, need + 1);
+    if (e != hipSuccess) { ctx->last_hip = e; ctx->dense_cap = 0; return GALA_E_OOM; }
+    ctx->dense_cap = need;
   }
   return GALA_OK;
 }
@@ -381,10 +430,13 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     reset_counters(ctx->ctr, (long long)m2, m2d, s);
     HIPOK(hipEventRecord(ctx->ev[0], s));
     const bool first = it == 1 && unique_rows;  // singleton partition: no aggregation needed
+    const bool prof = ctx->opts.profile_bins != 0;
     for (int b = 0; b < NBINS; b++) {
+      if (prof) HIPOK(hipEventRecord(ctx->bev[b], s));
       if (first) launch_decide_first<T>(b, g, st, ctx->front, ctx->bstart_front, caps[b], m2, s);
       else launch_decide<T>(b, g, st, ctx->front, ctx->bstart_front, caps[b], m2, s);
     }
+    if (prof) HIPOK(hipEventRecord(ctx->bev[NBINS], s));
     HIPOK(hipEventRecord(ctx->ev[1], s));
     launch_apply<T>(st, g.n, m2, s);
     // In changes only where a vertex or one of its neighbours moved
@@ -408,6 +460,15 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     {
       const int64_t per_edge = (first ? 12 : 16) - (g.w ? 0 : 4);
       ctx->res.decide_bytes += per_edge * (int64_t)c.swept_edges + 40 * (int64_t)c.swept;
+      for (int b = 0; b < NBINS; b++) {
+        ctx->res.bin_decide_bytes[b] += per_edge * (int64_t)c.bin_edges[b] + 40 * (int64_t)c.bin_swept[b];
+        ctx->res.bin_swept_edges[b] += (int64_t)c.bin_edges[b];
+        if (prof) {
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, ctx->bev[b], ctx->bev[b + 1]);
+          ctx->res.bin_decide_ms[b] += ms;
+        }
+      }
     }
     q = q_from<T>(c, m2d);
     if (icb) {
@@ -439,8 +500,11 @@ ContractBufs<T> cbufs(gala_ctx* ctx) {
   cb.ecur = ctx->c_ecur; cb.skey = ctx->c_skey; cb.sval = (typename T::acc_t*)ctx->c_sval;
   cb.dcnt = ctx->c_dcnt; cb.noff = nullptr; cb.big_list = ctx->c_big_list; cb.nbig = ctx->c_nbig;
   cb.big_tab = ctx->c_big_tab; cb.arena_keys = ctx->arena_keys;
-  cb.arena_vals = (typename T::acc_t*)ctx->arena_vals; cb.arena_bits = ctx->arena_bits;
-  cb.arena_slots = ctx->arena_slots; cb.arena_words = ctx->arena_words;
+  cb.arena_vals = (typename T::acc_t*)ctx->arena_vals; cb.arena_slots = ctx->arena_slots;
+  cb.dense = (typename T::acc_t*)ctx->dense;
+  cb.dense_slots = 0;
+  cb.chunks = ctx->chunks; cb.dense_c = ctx->dense_c; cb.dense_tcnt = ctx->dense_tcnt;
+  cb.dense_toff = ctx->dense_toff;
   cb.i32_tmp = ctx->c_i32_tmp; cb.i64_tmp = ctx->c_i64_tmp; cb.i32_scalar = ctx->c_i32_scalar;
   cb.i64_scalar = ctx->c_i64_scalar;
   return cb;
@@ -455,23 +519,18 @@ int contract(gala_ctx* ctx, const DevGraph<T>& g, const int32_t* comm, int dst, 
   ContractBufs<T> cb = cbufs<T>(ctx);
   ContractPlan plan;
   if (contract_prepare<T>(g, comm, cb, &plan, s)) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
-  int rc = grow_arena(ctx, plan.slots, ctx->arena_words, T::FP);
+  int rc = grow_arena(ctx, plan.slots, T::FP);
   if (rc) return rc;
+  int64_t per_batch = 0;
+  if (plan.ndense > 0 && (rc = grow_dense(ctx, plan, T::FP, &per_batch))) return rc;
   ContractOut<T> out;
   out.off = ctx->off[dst];
   out.nbr = ctx->nbr[dst];
   out.w = (typename T::w_t*)ctx->w[dst];
   int64_t m = 0;
   cb = cbufs<T>(ctx);
-  int r = contract_build<T>(g, cb, plan, out, &m, s);
-  if (r == -3) {  // rows with > 16384 distinct neighbours and C > 2^20: bitmap arena
-    int64_t words = (int64_t)((plan.C + 31) / 32) * 64;
-    if ((rc = grow_arena(ctx, plan.slots, words, T::FP))) return rc;
-    cb = cbufs<T>(ctx);
-    fill<int64_t>(cb.ecur, plan.C, 0, s);
-    r = contract_build<T>(g, cb, plan, out, &m, s);
-  }
-  if (r) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
+  cb.dense_slots = per_batch;
+  if (contract_build<T>(g, cb, plan, out, &m, s)) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
   *C_out = plan.C;
   *m_out = m;
   return GALA_OK;
@@ -688,6 +747,8 @@ int gala_create(gala_ctx** out, int device, const gala_opts* opts) {
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return GALA_E_HIP; }
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) { delete ctx; return GALA_E_HIP; }
+  for (auto& e : ctx->bev)
+    if (hipEventCreate(&e) != hipSuccess) { delete ctx; return GALA_E_HIP; }
   *out = ctx;
   return GALA_OK;
 }
@@ -699,6 +760,7 @@ void gala_destroy(gala_ctx* ctx) {
   free_all(ctx);
   if (ctx->ctr_h) (void)hipHostFree(ctx->ctr_h);
   for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+  for (auto& e : ctx->bev) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -55,7 +55,8 @@ struct Counters {
   double sum_in_f;               // FP path: sum In (FP64)
   double sum_tot2_f;             // FP path: sum Tot^2 (FP64)
   double min_tot_f;              // FP path: min non-zero Tot
-  long long pad[3];
+  unsigned long long bin_swept[NBINS];  // decide-and-move per degree bin
+  unsigned long long bin_edges[NBINS];
 };
 
 // Weight traits: exact integer path and FP64-accumulating float path.

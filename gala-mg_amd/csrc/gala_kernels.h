@@ -2,6 +2,7 @@
 #pragma once
 #include "gala_internal.h"
 #include <algorithm>
+#include <vector>
 
 namespace gala {
 
@@ -71,10 +72,20 @@ template <class T>
 void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);
 
 // ---- contraction (kernels_contract.hip) ----
+// One chunk of a dense row's staged segment (see kernels_contract.hip).
+struct DenseChunk {
+  int32_t slot, c;  // dense-accumulator slot, community
+  int64_t start;    // offset inside the community's staged segment
+  int32_t len, pad;
+};
+
 struct ContractPlan {
   int32_t C;      // communities (rows of the contracted graph)
-  int32_t nbig;   // rows with more than 4096 staged entries
-  int64_t slots;  // global hash slots those rows need
+  int32_t nbig;   // rows with 4096 < L <= LT staged entries
+  int64_t slots;  // global overflow slots those rows need
+  int32_t ndense; // rows with L > LT (dense accumulation)
+  std::vector<int32_t> dense_c;     // their communities, ascending
+  std::vector<DenseChunk> chunks;   // their segments cut into LT/2-entry chunks
 };
 
 template <class T>
@@ -89,14 +100,18 @@ struct ContractBufs {
   typename T::acc_t* sval;  // [m]   staging values
   int32_t* dcnt;      // [n+1] distinct neighbours per new community
   int64_t* noff;      // [n+1]
-  int32_t* big_list;  // [2n]  communities taking the global-hash path, then the bitmap queue
-  int32_t* nbig;      // [2]   long rows, rows beyond the LDS sort
+  int32_t* big_list;  // [2n]  big rows in [0, n), dense rows from n
+  int32_t* nbig;      // [2]   counts of big and dense rows
   int64_t* big_tab;   // [n+1] arena offsets (slots) of each large community's table
-  int32_t* arena_keys;
+  int32_t* arena_keys;            // big-row overflow slices (all-zero between uses)
   typename T::acc_t* arena_vals;
-  uint32_t* arena_bits;
   int64_t arena_slots;
-  int64_t arena_words;
+  typename T::acc_t* dense;       // dense_slots x C accumulators (all-zero between uses)
+  int64_t dense_slots;
+  DenseChunk* chunks;             // chunk table of the current dense batch
+  int32_t* dense_c;               // community of each dense slot in the batch
+  int32_t* dense_tcnt;            // per (slot, tile) non-zero counts
+  int64_t* dense_toff;            // their exclusive scan (+ total)
   int32_t* i32_tmp;
   int64_t* i64_tmp;
   int32_t* i32_scalar;  // [4]
@@ -116,8 +131,8 @@ template <class T>
 int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& cb, ContractPlan* plan,
                      hipStream_t s);
 // Stage, merge rows in canonical order, write the new CSR into out.  Needs
-// arena_slots >= plan.slots.  Returns 0, -1 (HIP error) or -3 (bitmap arena
-// too small: grow arena_words to 64*ceil(C/32) and call again).
+// arena_slots >= plan.slots, dense_slots >= 1 (x C accumulators) when
+// plan.ndense > 0, chunk/tile buffers sized for the plan.  Returns 0 or -1.
 template <class T>
 int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan& plan,
                    ContractOut<T>& out, int64_t* m_out, hipStream_t s);

@@ -237,6 +237,7 @@ __global__ void k_reset_counters(Counters* c, long long min_init, double min_ini
     c->swept_edges = 0; c->active = 0; c->nonempty = 0; c->overflow = 0;
     c->sum_in_f = 0; c->sum_tot2_f = 0; c->min_tot_f = min_init_f;
   }
+  if (threadIdx.x < NBINS) { c->bin_swept[threadIdx.x] = 0; c->bin_edges[threadIdx.x] = 0; }
 }
 void reset_counters(Counters* c, long long min_init, double min_init_f, hipStream_t s) {
   k_reset_counters<<<1, 64, 0, s>>>(c, min_init, min_init_f);

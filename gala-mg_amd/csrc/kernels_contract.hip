@@ -10,22 +10,26 @@
 //
 // Pipeline: label[v] = new_id(comm[v]) -> per-community entry counts -> stage
 // every entry (label of the neighbour, weight) in its community's segment ->
-// merge each segment into a sorted, duplicate-free row:
-//   L <= 64      one wave, bitonic sort in registers + segmented scan
-//   L <= 512     64-thread workgroup: LDS hash aggregate, bitonic sort of the keys
-//   L <= 4096    256-thread workgroup, same with larger LDS tables
-//   L >  4096    1024-thread workgroup: global hash (next_pow2(2 min(L, C))
-//                slots, no drops), distinct keys sorted in LDS (<= 16384,
-//                8192 for FP64 values); beyond that an ordered extraction
-//                through a C-bit bitmap
+// merge each segment into a sorted, duplicate-free row, by segment length L:
+//   L <= 64       one wave, bitonic sort in registers + segmented scan
+//   L <= 512      64-thread workgroup: LDS hash aggregate, bitonic sort of the keys
+//   L <= 4096     256-thread workgroup, same with larger LDS tables
+//   L <= LT       1024-thread workgroup: LT-slot LDS table (LT = 16384, 8192 on
+//                 the FP path), overflow into a global slice, LDS sort of <= LT keys
+//   L >  LT       dense rows: the segment is cut into chunks, one workgroup per
+//                 chunk pre-aggregates in LDS and adds into a C-wide accumulator
+//                 for the row (HBM is plentiful); a tiled scan of the accumulator
+//                 then emits the row already in key order
 // -> exclusive scan of the distinct counts -> compaction into the new CSR.
 #include "gala_kernels.h"
 #include "gala_device.h"
+#include <algorithm>
 #include <climits>
+#include <vector>
 
 namespace gala {
 
-constexpr int BITMAP_LDS_WORDS = 32768;  // 2^20 bits = 128 KB
+constexpr int DENSE_TILE = 4096;  // keys per workgroup in the dense-row scans
 
 __device__ __forceinline__ unsigned chash(int key, int lg) {
   return ((unsigned)key * 2654435769u) >> (32 - lg);
@@ -61,15 +65,19 @@ __global__ void k_ecount(const int64_t* __restrict__ off, int32_t n, const int32
   }
 }
 
-// classify rows: long rows are appended to big_list with their table size
-__global__ void k_classify(const int64_t* __restrict__ ecnt, int32_t C, int32_t* __restrict__ big_list,
-                           int32_t* __restrict__ nbig, int64_t* __restrict__ big_tsz) {
+// classify rows: 4096 < L <= lt -> big_list (with the size of its global
+// overflow slice); L > lt -> dense_list
+__global__ void k_classify(const int64_t* __restrict__ ecnt, int32_t C, long long lt,
+                           int32_t* __restrict__ big_list, int32_t* __restrict__ counts,
+                           int64_t* __restrict__ big_tsz, int32_t* __restrict__ dense_list) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
-    long long L = ecnt[c];
-    if (L > 4096) {
-      int k = atomicAdd(nbig, 1);
+    const long long L = ecnt[c];
+    if (L > lt) {
+      dense_list[atomicAdd(&counts[1], 1)] = c;
+    } else if (L > 4096) {
+      const int k = atomicAdd(&counts[0], 1);
       big_list[k] = c;
-      long long lim = L < (long long)C ? L : (long long)C;
+      const long long lim = L < (long long)C ? L : (long long)C;
       big_tsz[k] = 1ll << clog2(2 * lim);
     }
   }
@@ -242,10 +250,14 @@ __global__ __launch_bounds__(NT) void k_merge_mid(const int64_t* __restrict__ ec
 }
 
 // ---------------------------------------------------------------------------
-// L > 4096: global hash per community, distinct keys sorted in LDS; rows with
-// more than DMAX distinct keys are queued for the bitmap extraction
+// 4096 < L <= LT: one 1024-thread workgroup per community.  An LT-slot LDS
+// table is tried first (bounded probing); entries that find no LDS slot go to
+// the community's slice of the global arena (next_pow2(2 min(L, C)) slots, so
+// nothing is dropped).  A key lives in exactly one of the two tables, and
+// D <= L <= LT keys are then sorted in LDS.  Global slices are kept all-zero
+// between uses (the reader clears what it read).
 // ---------------------------------------------------------------------------
-template <int DMAX, class T>
+template <int LT, class T>
 __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ ecnt,
                                                      const int64_t* __restrict__ eoff, int32_t C,
                                                      const int32_t* __restrict__ big_list, int nbig,
@@ -254,52 +266,81 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
                                                      typename T::acc_t* __restrict__ arena_vals,
                                                      int32_t* __restrict__ skey,
                                                      typename T::acc_t* __restrict__ sval,
-                                                     int32_t* __restrict__ dcnt,
-                                                     int32_t* __restrict__ huge_list,
-                                                     int32_t* __restrict__ nhuge) {
+                                                     int32_t* __restrict__ dcnt) {
   using acc = typename T::acc_t;
-  constexpr int NT = 1024;
-  __shared__ int s_k[DMAX];
-  __shared__ acc s_v[DMAX];
+  constexpr int NT = 1024, PMAX = 32, PER = LT / NT;
+  constexpr int LLG = __builtin_ctz(LT);
+  __shared__ int s_k[LT];
+  __shared__ acc s_v[LT];
   __shared__ int s_sh[17];
+  __shared__ int s_ovf;
   for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
     const int c = big_list[b];
     const long long L = ecnt[c];
     const int64_t base = eoff[c];
     const int64_t toff = big_off[b];
-    const long long tsz = big_off[b + 1] - toff;
-    const int lg = clog2(tsz);
+    const long long gsz = big_off[b + 1] - toff;
+    const int glg = clog2(gsz);
     int* keys = arena_keys + toff;
     acc* vals = arena_vals + toff;
-    for (long long t = threadIdx.x; t < tsz; t += NT) { keys[t] = 0; vals[t] = 0; }
+    for (int t = threadIdx.x; t < LT; t += NT) { s_k[t] = 0; s_v[t] = 0; }
+    if (threadIdx.x == 0) s_ovf = 0;
     __syncthreads();
     for (long long j = threadIdx.x; j < L; j += NT) {
       const int key = skey[base + j];
       const acc w = sval[base + j];
-      unsigned p = chash(key, lg);
-      for (;;) {
-        const int k = atomicCAS(&keys[p], 0, key + 1);
-        if (k == 0 || k == key + 1) { atomicAdd(&vals[p], w); break; }
-        p = (p + 1) & (unsigned)(tsz - 1);
+      unsigned p = chash(key, LLG);
+      bool placed = false;
+      for (int probe = 0; probe < PMAX; probe++) {
+        const int k = atomicCAS(&s_k[p], 0, key + 1);
+        if (k == 0 || k == key + 1) { atomicAdd(&s_v[p], w); placed = true; break; }
+        p = (p + 1) & (LT - 1);
+      }
+      if (!placed) {
+        s_ovf = 1;
+        unsigned q = chash(key, glg);
+        for (;;) {
+          const int k = atomicCAS(&keys[q], 0, key + 1);
+          if (k == 0 || k == key + 1) { atomicAdd(&vals[q], w); break; }
+          q = (q + 1) & (unsigned)(gsz - 1);
+        }
       }
     }
     __syncthreads();
-    // count distinct keys first: rows beyond DMAX go to the bitmap path
-    int cnt = 0;
-    for (long long t = threadIdx.x; t < tsz; t += NT) cnt += keys[t] != 0;
-    int D;
-    int q = block_excl_scan<int>(cnt, s_sh, &D);
-    if (D > DMAX) {
-      if (threadIdx.x == 0) {
-        huge_list[atomicAdd(nhuge, 1)] = b;
-        dcnt[c] = D;
-      }
-      __syncthreads();
-      continue;
+    const bool ovf = s_ovf != 0;
+    int cl = 0, cg = 0;
+    for (int t = threadIdx.x; t < LT; t += NT) cl += s_k[t] != 0;
+    if (ovf)
+      for (long long t = threadIdx.x; t < gsz; t += NT) cg += keys[t] != 0;
+    int Dl, Dg;
+    const int ql = block_excl_scan<int>(cl, s_sh, &Dl);
+    const int qg = block_excl_scan<int>(cg, s_sh, &Dg);
+    const int D = Dl + Dg;
+    // compact: LDS pairs to [0, Dl), overflow pairs to [Dl, D)
+    int rk[PER];
+    acc rv[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      rk[i] = s_k[threadIdx.x + i * NT];
+      rv[i] = s_v[threadIdx.x + i * NT];
     }
-    for (long long t = threadIdx.x; t < tsz; t += NT) {
-      const int k = keys[t];
-      if (k) { s_k[q] = k - 1; s_v[q] = vals[t]; q++; }
+    __syncthreads();
+    int q = ql;
+#pragma unroll
+    for (int i = 0; i < PER; i++)
+      if (rk[i]) { s_k[q] = rk[i] - 1; s_v[q] = rv[i]; q++; }
+    if (ovf) {
+      q = Dl + qg;
+      for (long long t = threadIdx.x; t < gsz; t += NT) {
+        const int k = keys[t];
+        if (k) {
+          s_k[q] = k - 1;
+          s_v[q] = vals[t];
+          q++;
+          keys[t] = 0;
+          vals[t] = 0;
+        }
+      }
     }
     __syncthreads();
     int P = 1;
@@ -316,91 +357,141 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
   }
 }
 
-// Ordered extraction through a bitmap of C bits for rows with > DMAX distinct
-// keys (the global hash built by k_merge_big is still intact).
-template <class T>
-__global__ __launch_bounds__(1024) void k_merge_huge(const int64_t* __restrict__ eoff, int32_t C,
-                                                      const int32_t* __restrict__ big_list,
-                                                      const int32_t* __restrict__ huge_list, int nhuge,
-                                                      const int64_t* __restrict__ big_off,
-                                                      const int32_t* __restrict__ arena_keys,
-                                                      const typename T::acc_t* __restrict__ arena_vals,
-                                                      uint32_t* __restrict__ arena_bits,
-                                                      int32_t* __restrict__ skey,
-                                                      typename T::acc_t* __restrict__ sval) {
-  constexpr int NT = 1024;
-  __shared__ uint32_t bits_lds[BITMAP_LDS_WORDS];
-  __shared__ int sh[17];
-  const int nwords = (C + 31) >> 5;
-  uint32_t* bits = nwords <= BITMAP_LDS_WORDS ? bits_lds : arena_bits + (int64_t)blockIdx.x * nwords;
-  for (int h = blockIdx.x; h < nhuge; h += gridDim.x) {
-    const int b = huge_list[h];
-    const int c = big_list[b];
-    const int64_t base = eoff[c];
-    const int64_t toff = big_off[b];
-    const long long tsz = big_off[b + 1] - toff;
-    const int lg = clog2(tsz);
-    const int* keys = arena_keys + toff;
-    const typename T::acc_t* vals = arena_vals + toff;
-    for (int t = threadIdx.x; t < nwords; t += NT) bits[t] = 0;
+// ---------------------------------------------------------------------------
+// dense rows (L > LT).  chunk table: (dense slot, community, start, length)
+// ---------------------------------------------------------------------------
+template <int LT, class T>
+__global__ __launch_bounds__(1024) void k_dense_accum(const DenseChunk* __restrict__ chunks, int nchunks,
+                                                      const int64_t* __restrict__ eoff, int32_t C,
+                                                      const int32_t* __restrict__ skey,
+                                                      const typename T::acc_t* __restrict__ sval,
+                                                      typename T::acc_t* __restrict__ dense) {
+  using acc = typename T::acc_t;
+  constexpr int NT = 1024, LLG = __builtin_ctz(LT);
+  __shared__ int s_k[LT];
+  __shared__ acc s_v[LT];
+  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const DenseChunk cc = chunks[ch];
+    const int64_t base = eoff[cc.c] + cc.start;
+    acc* row = dense + (int64_t)cc.slot * C;
+    for (int t = threadIdx.x; t < LT; t += NT) { s_k[t] = 0; s_v[t] = 0; }
     __syncthreads();
-    for (long long t = threadIdx.x; t < tsz; t += NT) {
-      const int k = keys[t];
-      if (k) atomicOr(&bits[(k - 1) >> 5], 1u << ((k - 1) & 31));
+    // a chunk holds at most LT/2 entries, so the table never fills
+    for (int j = threadIdx.x; j < cc.len; j += NT) {
+      const int key = skey[base + j];
+      const acc w = sval[base + j];
+      unsigned p = chash(key, LLG);
+      for (;;) {
+        const int k = atomicCAS(&s_k[p], 0, key + 1);
+        if (k == 0 || k == key + 1) { atomicAdd(&s_v[p], w); break; }
+        p = (p + 1) & (LT - 1);
+      }
     }
     __syncthreads();
-    const int per = (nwords + NT - 1) / NT;
-    const int a0 = threadIdx.x * per, a1 = min(nwords, a0 + per);
-    int s = 0;
-    for (int t = a0; t < a1; t++) s += __popc(bits[t]);
-    {
-      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-      int x = s;
-#pragma unroll
-      for (int q = 1; q < 64; q <<= 1) {
-        int y = __shfl_up(x, q, 64);
-        if (lane >= q) x += y;
-      }
-      if (lane == 63) sh[wid] = x;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        int run = 0;
-        for (int w = 0; w < NT / 64; w++) { int tmp = sh[w]; sh[w] = run; run += tmp; }
-        sh[16] = run;
-      }
-      __syncthreads();
-      s = x - s + sh[wid];
-    }
-    int pos = s;
-    for (int t = a0; t < a1; t++) {
-      uint32_t wbits = bits[t];
-      while (wbits) {
-        const int bit = __ffs(wbits) - 1;
-        wbits &= wbits - 1;
-        const int key = (t << 5) + bit;
-        unsigned p = chash(key, lg);
-        while (keys[p] != key + 1) p = (p + 1) & (unsigned)(tsz - 1);
-        skey[base + pos] = key;
-        sval[base + pos] = vals[p];
-        pos++;
-      }
+    for (int t = threadIdx.x; t < LT; t += NT) {
+      const int k = s_k[t];
+      if (k) atomicAdd(&row[k - 1], s_v[t]);
     }
     __syncthreads();
   }
 }
 
-// copy each merged row to its final place
+// non-zero count of every DENSE_TILE-key tile of every dense row
 template <class T>
-__global__ void k_compact(const int64_t* __restrict__ eoff, const int64_t* __restrict__ noff,
-                          const int32_t* __restrict__ dcnt, int32_t C, const int32_t* __restrict__ skey,
-                          const typename T::acc_t* __restrict__ sval, int32_t* __restrict__ nbr,
-                          typename T::w_t* __restrict__ w) {
+__global__ __launch_bounds__(256) void k_dense_count(const typename T::acc_t* __restrict__ dense, int nslots,
+                                                     int32_t C, int ntiles, int32_t* __restrict__ tcnt) {
+  __shared__ int32_t sh[17];
+  const int total = nslots * ntiles;
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    const int slot = t / ntiles, tile = t % ntiles;
+    const typename T::acc_t* row = dense + (int64_t)slot * C;
+    const int k0 = tile * DENSE_TILE, k1 = min(C, k0 + DENSE_TILE);
+    int cnt = 0;
+    for (int k = k0 + threadIdx.x; k < k1; k += 256) cnt += row[k] != 0;
+    int tot;
+    block_excl_scan<int32_t>(cnt, sh, &tot);
+    if (threadIdx.x == 0) tcnt[t] = tot;
+  }
+}
+
+// ordered extraction: each tile writes its keys (ascending) at the row's
+// running position and clears the accumulator entries it read
+template <class T>
+__global__ __launch_bounds__(256) void k_dense_scatter(typename T::acc_t* __restrict__ dense, int nslots,
+                                                       int32_t C, int ntiles, const int32_t* __restrict__ slot_c,
+                                                       const int64_t* __restrict__ toff,
+                                                       const int64_t* __restrict__ eoff,
+                                                       int32_t* __restrict__ skey,
+                                                       typename T::acc_t* __restrict__ sval,
+                                                       int32_t* __restrict__ dcnt) {
+  using acc = typename T::acc_t;
+  constexpr int PER = DENSE_TILE / 256;
+  __shared__ int32_t sh[17];
+  const int total = nslots * ntiles;
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    const int slot = t / ntiles, tile = t % ntiles;
+    acc* row = dense + (int64_t)slot * C;
+    const int c = slot_c[slot];
+    const int64_t rbase = toff[slot * ntiles];
+    const int k0 = tile * DENSE_TILE + threadIdx.x * PER;
+    int cnt = 0;
+#pragma unroll
+    for (int i = 0; i < PER; i++) cnt += (k0 + i < C) && row[k0 + i] != 0;
+    int tot;
+    int pos = block_excl_scan<int32_t>(cnt, sh, &tot);
+    const int64_t out = eoff[c] + (toff[t] - rbase) + pos;
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int k = k0 + i;
+      if (k < C) {
+        const acc v = row[k];
+        if (v != 0) {
+          skey[out + q] = k;
+          sval[out + q] = v;
+          row[k] = 0;
+          q++;
+        }
+      }
+    }
+    if (tile == 0 && threadIdx.x == 0) dcnt[c] = (int32_t)(toff[(slot + 1) * ntiles] - rbase);
+  }
+}
+
+// copy each merged row to its final place: 16-lane groups for rows of at
+// most 4096 staged entries, one workgroup per row for the listed long rows
+template <class T>
+__global__ __launch_bounds__(256) void k_compact(const int64_t* __restrict__ ecnt, const int64_t* __restrict__ eoff,
+                                                  const int64_t* __restrict__ noff, const int32_t* __restrict__ dcnt,
+                                                  int32_t C, const int32_t* __restrict__ skey,
+                                                  const typename T::acc_t* __restrict__ sval,
+                                                  int32_t* __restrict__ nbr, typename T::w_t* __restrict__ w) {
   const int lane = threadIdx.x & 15;
   const int ng = gridDim.x * (blockDim.x >> 4);
   for (int c = blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4); c < C; c += ng) {
+    if (ecnt[c] > 4096) continue;  // uniform per group: copied by k_compact_list
     const int64_t src = eoff[c], dst = noff[c];
     const int D = dcnt[c];
     for (int t = lane; t < D; t += 16) {
+      nbr[dst + t] = skey[src + t];
+      w[dst + t] = (typename T::w_t)sval[src + t];
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_compact_list(const int32_t* __restrict__ list, int nlist,
+                                                       const int64_t* __restrict__ eoff,
+                                                       const int64_t* __restrict__ noff,
+                                                       const int32_t* __restrict__ dcnt,
+                                                       const int32_t* __restrict__ skey,
+                                                       const typename T::acc_t* __restrict__ sval,
+                                                       int32_t* __restrict__ nbr, typename T::w_t* __restrict__ w) {
+  for (int i = blockIdx.x; i < nlist; i += gridDim.x) {
+    const int c = list[i];
+    const int64_t src = eoff[c], dst = noff[c];
+    const int D = dcnt[c];
+    for (int t = threadIdx.x; t < D; t += 256) {
       nbr[dst + t] = skey[src + t];
       w[dst + t] = (typename T::w_t)sval[src + t];
     }
@@ -426,6 +517,7 @@ static inline int nblk(int64_t n, int per = 256, int cap = 8192) {
 template <class T>
 int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& cb, ContractPlan* plan,
                      hipStream_t s) {
+  constexpr long long LT = T::FP ? 8192 : 16384;
   const int32_t n = g.n;
   fill<int32_t>(cb.cnt, n, 0, s);
   k_hist<<<nblk(n), 256, 0, s>>>(comm, n, cb.cnt);
@@ -439,11 +531,13 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   fill<int64_t>(cb.ecnt, C + 1, 0, s);
   k_ecount<<<nblk(n), 256, 0, s>>>(g.off, n, cb.label, cb.ecnt);
   scan_exclusive<int64_t, int64_t>(cb.ecnt, cb.eoff, C, cb.i64_tmp, cb.eoff + C, s);
-  fill<int32_t>(cb.nbig, 2, 0, s);  // nbig, nhuge
-  k_classify<<<nblk(C), 256, 0, s>>>(cb.ecnt, C, cb.big_list, cb.nbig, cb.i64_tmp);
-  int32_t nbig = 0;
-  if (hipMemcpyAsync(&nbig, cb.nbig, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  fill<int32_t>(cb.nbig, 2, 0, s);  // big rows, dense rows
+  int32_t* dense_list = cb.big_list + n;
+  k_classify<<<nblk(C), 256, 0, s>>>(cb.ecnt, C, LT, cb.big_list, cb.nbig, cb.i64_tmp, dense_list);
+  int32_t counts[2] = {0, 0};
+  if (hipMemcpyAsync(counts, cb.nbig, sizeof counts, hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  const int32_t nbig = counts[0], ndense = counts[1];
   int64_t slots = 0;
   if (nbig > 0) {
     scan_exclusive<int64_t, int64_t>(cb.i64_tmp, cb.big_tab, nbig, cb.i64_tmp + nbig + 1, cb.big_tab + nbig, s);
@@ -454,16 +548,34 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   plan->C = C;
   plan->nbig = nbig;
   plan->slots = slots;
+  plan->ndense = ndense;
+  plan->dense_c.clear();
+  plan->chunks.clear();
+  if (ndense > 0) {
+    // dense rows in ascending community order (deterministic slot assignment)
+    std::vector<int32_t> dl(ndense);
+    if (hipMemcpy(dl.data(), dense_list, sizeof(int32_t) * ndense, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    std::sort(dl.begin(), dl.end());
+    std::vector<int64_t> ec(C + 1);
+    if (hipMemcpy(ec.data(), cb.ecnt, sizeof(int64_t) * C, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    plan->dense_c = dl;
+    const int64_t CH = LT / 2;  // entries per chunk: the LDS table stays at most half full
+    for (int32_t i = 0; i < ndense; i++) {
+      const int64_t L = ec[dl[i]];
+      for (int64_t st = 0; st < L; st += CH)
+        plan->chunks.push_back({i, dl[i], st, (int32_t)std::min<int64_t>(CH, L - st)});
+    }
+  }
   return 0;
 }
 
 template <class T>
 int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan& plan,
                    ContractOut<T>& out, int64_t* m_out, hipStream_t s) {
+  constexpr int LT = T::FP ? 8192 : 16384;
   const int32_t n = g.n;
   const int32_t C = plan.C;
   fill<int64_t>(cb.ecur, C, 0, s);
-  fill<int32_t>(cb.nbig + 1, 1, 0, s);  // bitmap-queue length (a call may be repeated)
   k_stage<16, T><<<nblk(n, 16, 16384), 256, 0, s>>>(g, cb.label, cb.eoff, cb.ecur, 1, 256, cb.skey, cb.sval);
   k_stage<64, T><<<nblk(n, 4, 16384), 256, 0, s>>>(g, cb.label, cb.eoff, cb.ecur, 257, INT64_MAX, cb.skey,
                                                      cb.sval);
@@ -472,28 +584,48 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
   k_merge_mid<64, 64, 512, T><<<nblk(C, 1, 16384), 64, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval, cb.dcnt);
   k_merge_mid<256, 512, 4096, T><<<nblk(C, 1, 4096), 256, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval,
                                                                    cb.dcnt);
-  if (plan.nbig > 0) {
-    constexpr int DMAX = T::FP ? 8192 : 16384;
-    int32_t* huge_list = cb.big_list + plan.nbig;  // big_list has n entries, nbig <= n/4097
-    k_merge_big<DMAX, T><<<std::min(plan.nbig, 2048), 1024, 0, s>>>(
-        cb.ecnt, cb.eoff, C, cb.big_list, plan.nbig, cb.big_tab, cb.arena_keys, cb.arena_vals, cb.skey, cb.sval,
-        cb.dcnt, huge_list, cb.nbig + 1);
-    int32_t nhuge = 0;
-    if (hipMemcpyAsync(&nhuge, cb.nbig + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-    if (hipStreamSynchronize(s) != hipSuccess) return -1;
-    if (nhuge > 0) {
-      const int nwords = (C + 31) / 32;
-      int grid = std::min(nhuge, 1024);
-      if (nwords > BITMAP_LDS_WORDS) {
-        if (cb.arena_words < nwords) return -3;  // caller grows the bitmap arena and calls again
-        grid = (int)std::min<int64_t>(grid, cb.arena_words / nwords);
+  if (plan.nbig > 0)
+    k_merge_big<LT, T><<<std::min(plan.nbig, 2048), 1024, 0, s>>>(cb.ecnt, cb.eoff, C, cb.big_list, plan.nbig,
+                                                                   cb.big_tab, cb.arena_keys, cb.arena_vals,
+                                                                   cb.skey, cb.sval, cb.dcnt);
+  if (plan.ndense > 0) {
+    const int ntiles = (C + DENSE_TILE - 1) / DENSE_TILE;
+    const int per_batch = (int)std::max<int64_t>(1, std::min<int64_t>(plan.ndense, cb.dense_slots));
+    size_t ci = 0;
+    for (int s0 = 0; s0 < plan.ndense; s0 += per_batch) {
+      const int nb = std::min(per_batch, plan.ndense - s0);
+      // chunks of this batch, slots renumbered from 0
+      std::vector<DenseChunk> ch;
+      while (ci < plan.chunks.size() && plan.chunks[ci].slot < s0 + nb) {
+        DenseChunk d = plan.chunks[ci++];
+        d.slot -= s0;
+        ch.push_back(d);
       }
-      k_merge_huge<T><<<grid, 1024, 0, s>>>(cb.eoff, C, cb.big_list, huge_list, nhuge, cb.big_tab,
-                                            cb.arena_keys, cb.arena_vals, cb.arena_bits, cb.skey, cb.sval);
+      if (hipMemcpyAsync(cb.chunks, ch.data(), sizeof(DenseChunk) * ch.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+        return -1;
+      if (hipMemcpyAsync(cb.dense_c, plan.dense_c.data() + s0, sizeof(int32_t) * nb, hipMemcpyHostToDevice, s) !=
+          hipSuccess)
+        return -1;
+      k_dense_accum<LT, T><<<(int)std::min<size_t>(ch.size(), 4096), 1024, 0, s>>>(
+          cb.chunks, (int)ch.size(), cb.eoff, C, cb.skey, cb.sval, cb.dense);
+      k_dense_count<T><<<std::min(nb * ntiles, 16384), 256, 0, s>>>(cb.dense, nb, C, ntiles, cb.dense_tcnt);
+      scan_exclusive<int32_t, int64_t>(cb.dense_tcnt, cb.dense_toff, (int64_t)nb * ntiles, cb.i64_tmp,
+                                       cb.dense_toff + (int64_t)nb * ntiles, s);
+      k_dense_scatter<T><<<std::min(nb * ntiles, 16384), 256, 0, s>>>(cb.dense, nb, C, ntiles, cb.dense_c,
+                                                                       cb.dense_toff, cb.eoff, cb.skey, cb.sval,
+                                                                       cb.dcnt);
+      if (hipStreamSynchronize(s) != hipSuccess) return -1;  // host chunk vector reused
     }
   }
   scan_exclusive<int32_t, int64_t>(cb.dcnt, out.off, C, cb.i64_tmp, out.off + C, s);
-  k_compact<T><<<nblk(C, 16, 16384), 256, 0, s>>>(cb.eoff, out.off, cb.dcnt, C, cb.skey, cb.sval, out.nbr, out.w);
+  k_compact<T><<<nblk(C, 16, 16384), 256, 0, s>>>(cb.ecnt, cb.eoff, out.off, cb.dcnt, C, cb.skey, cb.sval, out.nbr,
+                                                  out.w);
+  if (plan.nbig > 0)
+    k_compact_list<T><<<std::min(plan.nbig, 8192), 256, 0, s>>>(cb.big_list, plan.nbig, cb.eoff, out.off, cb.dcnt,
+                                                                 cb.skey, cb.sval, out.nbr, out.w);
+  if (plan.ndense > 0)
+    k_compact_list<T><<<std::min(plan.ndense, 8192), 256, 0, s>>>(cb.big_list + n, plan.ndense, cb.eoff, out.off,
+                                                                   cb.dcnt, cb.skey, cb.sval, out.nbr, out.w);
   int64_t m = 0;
   if (hipMemcpyAsync(&m, out.off + C, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;

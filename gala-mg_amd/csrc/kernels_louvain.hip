@@ -261,6 +261,8 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
     atomic_add_ull(&st.ctr->moved, s_moved);
     atomic_add_ull(&st.ctr->swept, s_swept);
     atomic_add_ull(&st.ctr->swept_edges, s_edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
   }
 }
 
@@ -350,6 +352,8 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
     atomic_add_ull(&st.ctr->moved, s_moved);
     atomic_add_ull(&st.ctr->swept, s_swept);
     atomic_add_ull(&st.ctr->swept_edges, s_edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
   }
 }
 
@@ -449,6 +453,8 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
     atomic_add_ull(&st.ctr->moved, moved);
     atomic_add_ull(&st.ctr->swept, swept);
     atomic_add_ull(&st.ctr->swept_edges, edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], edges);
   }
 }
 
@@ -571,6 +577,8 @@ __global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> 
     atomic_add_ull(&st.ctr->moved, moved);
     atomic_add_ull(&st.ctr->swept, swept);
     atomic_add_ull(&st.ctr->swept_edges, edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], edges);
   }
 }
 
@@ -629,6 +637,8 @@ __global__ __launch_bounds__(256) void k_decide_first(DevGraph<T> g, DevState<T>
     atomic_add_ull(&st.ctr->moved, s_moved);
     atomic_add_ull(&st.ctr->swept, s_swept);
     atomic_add_ull(&st.ctr->swept_edges, s_edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
   }
 }
 
@@ -681,6 +691,8 @@ __global__ __launch_bounds__(NT) void k_decide_first_block(DevGraph<T> g, DevSta
     atomic_add_ull(&st.ctr->moved, moved);
     atomic_add_ull(&st.ctr->swept, swept);
     atomic_add_ull(&st.ctr->swept_edges, edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], edges);
   }
 }
 

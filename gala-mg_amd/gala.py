@@ -36,7 +36,7 @@ class Opts(ctypes.Structure):
                 ("max_rounds", ctypes.c_int), ("dump_iterations", ctypes.c_int),
                 ("dump_rounds", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("frontier_mode", ctypes.c_int), ("learned_tau", ctypes.c_float),
-                ("reserved", ctypes.c_int * 8)]
+                ("profile_bins", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
 
 
 class Result(ctypes.Structure):
@@ -51,7 +51,9 @@ class Result(ctypes.Structure):
                 ("round_q", ctypes.c_double * 64),
                 ("round_nc", ctypes.c_int32 * 64), ("round_iters", ctypes.c_int32 * 64),
                 ("round_ms", ctypes.c_double * 64), ("round_decide_ms", ctypes.c_double * 64),
-                ("round_update_ms", ctypes.c_double * 64), ("round_build_ms", ctypes.c_double * 64)]
+                ("round_update_ms", ctypes.c_double * 64), ("round_build_ms", ctypes.c_double * 64),
+                ("bin_decide_bytes", ctypes.c_int64 * 9), ("bin_swept_edges", ctypes.c_int64 * 9),
+                ("bin_decide_ms", ctypes.c_double * 9)]
 
     def to_dict(self):
         r = self.rounds
@@ -68,7 +70,9 @@ class Result(ctypes.Structure):
                     round_ms=list(self.round_ms[:r]),
                     round_decide_ms=list(self.round_decide_ms[:r]),
                     round_update_ms=list(self.round_update_ms[:r]),
-                    round_build_ms=list(self.round_build_ms[:r]))
+                    round_build_ms=list(self.round_build_ms[:r]),
+                    bin_decide_bytes=list(self.bin_decide_bytes), bin_swept_edges=list(self.bin_swept_edges),
+                    bin_decide_ms=list(self.bin_decide_ms))
 
 
 ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -245,7 +249,7 @@ class Engine:
     """One device context (gala_create ... gala_destroy)."""
 
     def __init__(self, device=0, min_modularity=1e-6, dump_iterations=False, dump_rounds=False,
-                 max_rounds=0, max_iterations=0):
+                 max_rounds=0, max_iterations=0, profile_bins=False):
         L = lib()
         self.opts = Opts()
         L.gala_default_opts(ctypes.byref(self.opts))
@@ -254,6 +258,7 @@ class Engine:
         self.opts.dump_rounds = int(dump_rounds)
         self.opts.max_rounds = max_rounds
         self.opts.max_iterations = max_iterations
+        self.opts.profile_bins = int(profile_bins)
         self.ctx = VP()
         _check(L.gala_create(ctypes.byref(self.ctx), device, ctypes.byref(self.opts)), "gala_create")
 

@@ -48,7 +48,8 @@ typedef struct {
   int verbose;           /* 1: print the reference's stdout lines from gala_run */
   int frontier_mode;     /* 0 plain (sound prune), 1 learned frontier + certification */
   float learned_tau;     /* learned frontier threshold on the move probability */
-  int reserved[8];
+  int profile_bins;      /* 1: time every degree bin's decide launch with HIP events */
+  int reserved[7];
 } gala_opts;
 
 /* Called after every iteration (Iteration:0 is Q before any move).
@@ -90,6 +91,9 @@ typedef struct {
   double round_decide_ms[64];
   double round_update_ms[64];
   double round_build_ms[64];  /* contraction of the round */
+  int64_t bin_decide_bytes[9]; /* decide-and-move algorithmic bytes per degree bin */
+  int64_t bin_swept_edges[9];
+  double bin_decide_ms[9];     /* per-bin decide time (profile_bins=1), HIP events */
 } gala_result;
 
 /* Version string of the library. */

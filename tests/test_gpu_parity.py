@@ -27,13 +27,13 @@ def full_trace_compare(eng, g, keep_vectors=True):
     t = oracle.Trace(keep_vectors=keep_vectors)
     ref = oracle.louvain(g.vertex_num, g.degrees, g.neighbors,
                          None if g.weights is None else g.weights, trace=t)
-    assert len(trace) == len(t.rows), (len(trace), len(t.rows))
     for a, b in zip(trace, t.rows):
         assert a[0] == b[0] and a[1] == b[1], (a[:2], b[:2])
         assert a[2] == b[2], f"Q differs at round {a[0]} it {a[1]}: {a[2]!r} vs {b[2]!r}"
         assert a[3] == b[3], f"moved differs at round {a[0]} it {a[1]}"
         if keep_vectors and a[4] is not None:
             assert np.array_equal(a[4], b[4]), f"community vector differs at round {a[0]} it {a[1]}"
+    assert len(trace) == len(t.rows), (len(trace), len(t.rows))
     assert res["rounds"] == len(ref["round_q"])
     assert list(res["round_nc"]) == list(ref["round_nc"])
     assert res["modularity"] == ref["Q"]
