@@ -96,6 +96,8 @@ struct gala_ctx {
   int64_t arena_slots = 0;
   void* dense = nullptr;  // dense-row accumulators (all-zero between uses)
   int64_t dense_elems = 0;
+  uint32_t* dense_bits = nullptr;  // their presence bitmaps (all-zero between uses)
+  int64_t dense_words = 0;
   DenseChunk* chunks = nullptr;
   int64_t chunks_cap = 0;
   int32_t* dense_c = nullptr;
@@ -126,10 +128,12 @@ void free_all(gala_ctx* ctx) {
   if (ctx->arena_vals) (void)hipFree(ctx->arena_vals);
   ctx->arena_keys = nullptr; ctx->arena_vals = nullptr;
   ctx->arena_slots = 0;
-  for (void* p : {ctx->dense, (void*)ctx->chunks, (void*)ctx->dense_c, (void*)ctx->dense_tcnt, (void*)ctx->dense_toff})
+  for (void* p : {ctx->dense, (void*)ctx->dense_bits, (void*)ctx->chunks, (void*)ctx->dense_c,
+                  (void*)ctx->dense_tcnt, (void*)ctx->dense_toff})
     if (p) (void)hipFree(p);
-  ctx->dense = nullptr; ctx->chunks = nullptr; ctx->dense_c = nullptr; ctx->dense_tcnt = nullptr;
-  ctx->dense_toff = nullptr; ctx->dense_elems = 0; ctx->chunks_cap = 0; ctx->dense_cap = 0;
+  ctx->dense = nullptr; ctx->dense_bits = nullptr; ctx->chunks = nullptr; ctx->dense_c = nullptr;
+  ctx->dense_tcnt = nullptr; ctx->dense_toff = nullptr; ctx->dense_elems = 0; ctx->dense_words = 0;
+  ctx->chunks_cap = 0; ctx->dense_cap = 0;
   ctx->have_graph = false;
 }
 
@@ -243,6 +247,15 @@ int grow_dense(gala_ctx* ctx, const ContractPlan& plan, bool fp, int64_t* per_ba
     if (e != hipSuccess) { ctx->last_hip = e; ctx->dense_elems = 0; return GALA_E_OOM; }
     ctx->dense_elems = elems;
   }
+  const int64_t words = pb * ((C + 31) / 32);
+  if (words > ctx->dense_words) {
+    if (ctx->dense_bits) (void)hipFree(ctx->dense_bits);
+    ctx->dense_bits = nullptr;
+    e = dalloc(&ctx->dense_bits, words);
+    if (e == hipSuccess) e = hipMemset(ctx->dense_bits, 0, sizeof(uint32_t) * words);
+    if (e != hipSuccess) { ctx->last_hip = e; ctx->dense_words = 0; return GALA_E_OOM; }
+    ctx->dense_words = words;
+  }
   const int64_t nch = (int64_t)plan.chunks.size();
   if (nch > ctx->chunks_cap) {
     if (ctx->chunks) (void)hipFree(ctx->chunks);
@@ -251,7 +264,7 @@ int grow_dense(gala_ctx* ctx, const ContractPlan& plan, bool fp, int64_t* per_ba
     if (e != hipSuccess) { ctx->last_hip = e; ctx->chunks_cap = 0; return GALA_E_OOM; }
     ctx->chunks_cap = nch;
   }
-  const int64_t tiles = pb * ((C + 4095) / 4096) + 1;
+  const int64_t tiles = pb * ((C + 4095) / 4096) + 1;  // DENSE_TILE keys per tile
   const int64_t need = std::max<int64_t>(tiles, pb);
   if (need > ctx->dense_cap) {
     for (void* p : {(void*)ctx->dense_c, (void*)ctx->dense_tcnt, (void*)ctx->dense_toff})
@@ -317,6 +330,7 @@ struct RoundOut {
   double q;
   int iters;
   double t_dm, t_upd;
+  int32_t caps[NBINS];  // degree-bin sizes of the round graph
 };
 
 template <class T>
@@ -486,6 +500,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   }
   ro->q = qprev;
   ro->iters = it;
+  for (int b = 0; b < NBINS; b++) ro->caps[b] = caps[b];
   ro->t_dm = t_dm;
   ro->t_upd = t_upd;
   (void)sizeof(tt);
@@ -501,7 +516,10 @@ ContractBufs<T> cbufs(gala_ctx* ctx) {
   cb.dcnt = ctx->c_dcnt; cb.noff = nullptr; cb.big_list = ctx->c_big_list; cb.nbig = ctx->c_nbig;
   cb.big_tab = ctx->c_big_tab; cb.arena_keys = ctx->arena_keys;
   cb.arena_vals = (typename T::acc_t*)ctx->arena_vals; cb.arena_slots = ctx->arena_slots;
+  cb.order = ctx->order;  // degree-bin order of the round graph (round_setup)
+  cb.bstart = ctx->bstart_full;
   cb.dense = (typename T::acc_t*)ctx->dense;
+  cb.dense_bits = ctx->dense_bits;
   cb.dense_slots = 0;
   cb.chunks = ctx->chunks; cb.dense_c = ctx->dense_c; cb.dense_tcnt = ctx->dense_tcnt;
   cb.dense_toff = ctx->dense_toff;
@@ -513,11 +531,12 @@ ContractBufs<T> cbufs(gala_ctx* ctx) {
 // Contract the round graph under comm into buffer `dst`.  Returns C and new m;
 // ctx->c_label then holds new_id(comm[v]) for the composition.
 template <class T>
-int contract(gala_ctx* ctx, const DevGraph<T>& g, const int32_t* comm, int dst, int32_t* C_out,
-             int64_t* m_out) {
+int contract(gala_ctx* ctx, const DevGraph<T>& g, const int32_t* comm, int dst, const int32_t* caps,
+             int32_t* C_out, int64_t* m_out) {
   hipStream_t s = ctx->stream;
   ContractBufs<T> cb = cbufs<T>(ctx);
   ContractPlan plan;
+  for (int b = 0; b < NBINS; b++) plan.caps[b] = caps[b];
   if (contract_prepare<T>(g, comm, cb, &plan, s)) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
   int rc = grow_arena(ctx, plan.slots, T::FP);
   if (rc) return rc;
@@ -583,7 +602,7 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
     int32_t C = 0;
     int64_t m = 0;
     const int dst = dst_of(src);
-    rc = contract<T>(ctx, g, ctx->prev, dst, &C, &m);
+    rc = contract<T>(ctx, g, ctx->prev, dst, ro.caps, &C, &m);
     if (rc) return rc;
     compose_map(ctx->fin, ctx->n0, ctx->c_label, s);
     HIPOK(hipEventRecord(ctx->ev[7], s));
@@ -885,11 +904,13 @@ int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const
   int rc = upload(ctx, n, end_incl, nbr, w, w ? GALA_W_I32 : GALA_W_NONE);
   if (rc) return rc;
   DevGraph<IntW> g = graph_at<IntW>(ctx, 0, n, ctx->m0);
+  int32_t caps[NBINS], nitems = 0;
+  if ((rc = round_setup<IntW>(ctx, g, caps, &nitems))) return rc;  // degree bins for the staging kernels
   HIPOK(hipMemcpy(ctx->prev, comm, 4ll * n, hipMemcpyHostToDevice));
   reset_counters(ctx->ctr, 0, 0.0, ctx->stream);
   int32_t C = 0;
   int64_t m = 0;
-  if ((rc = contract<IntW>(ctx, g, ctx->prev, 1, &C, &m))) return rc;
+  if ((rc = contract<IntW>(ctx, g, ctx->prev, 1, caps, &C, &m))) return rc;
   *out_C = C;
   *out_m = m;
   if (new_id) {

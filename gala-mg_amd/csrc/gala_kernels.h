@@ -80,6 +80,7 @@ struct DenseChunk {
 };
 
 struct ContractPlan {
+  int32_t caps[NBINS];  // vertices per degree bin of the graph being contracted
   int32_t C;      // communities (rows of the contracted graph)
   int32_t nbig;   // rows with 4096 < L <= LT staged entries
   int64_t slots;  // global overflow slots those rows need
@@ -93,6 +94,8 @@ struct ContractBufs {
   int32_t* cnt;       // [n]   community sizes
   int32_t* nid;       // [n]   new id of community (exclusive scan of cnt > 0)
   int32_t* label;     // [n]   new id of every vertex's community
+  const int32_t* order;   // the graph's vertices grouped by degree bin
+  const int32_t* bstart;  // [NBINS+1] bin offsets into order (device)
   int64_t* ecnt;      // [n+1] entries per new community
   int64_t* eoff;      // [n+1]
   int64_t* ecur;      // [n]
@@ -107,6 +110,7 @@ struct ContractBufs {
   typename T::acc_t* arena_vals;
   int64_t arena_slots;
   typename T::acc_t* dense;       // dense_slots x C accumulators (all-zero between uses)
+  uint32_t* dense_bits;           // dense_slots x ceil(C/32) presence bitmaps (all-zero between uses)
   int64_t dense_slots;
   DenseChunk* chunks;             // chunk table of the current dense batch
   int32_t* dense_c;               // community of each dense slot in the batch

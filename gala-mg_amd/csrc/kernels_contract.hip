@@ -13,7 +13,8 @@
 // merge each segment into a sorted, duplicate-free row, by segment length L:
 //   L <= 64       one wave, bitonic sort in registers + segmented scan
 //   L <= 512      64-thread workgroup: LDS hash aggregate, bitonic sort of the keys
-//   L <= 4096     256-thread workgroup, same with larger LDS tables
+//   L <= 2048     128-thread workgroup, same with larger LDS tables
+//   L <= 4096     256-thread workgroup
 //   L <= LT       1024-thread workgroup: LT-slot LDS table (LT = 16384, 8192 on
 //                 the FP path), overflow into a global slice, LDS sort of <= LT keys
 //   L >  LT       dense rows: the segment is cut into chunks, one workgroup per
@@ -84,28 +85,105 @@ __global__ void k_classify(const int64_t* __restrict__ ecnt, int32_t C, long lon
 }
 
 // ---------------------------------------------------------------------------
-// staging: (label of the neighbour, weight) into the owner's segment.  W lanes
-// per vertex read the row with coalesced loads; lane 0 reserves the slot range.
+// staging: (label of the neighbour, weight) into the owner's segment.
+// Rows of fewer than 512 entries: a block owns 256 consecutive vertices and
+// reserves space with one global atomic per distinct community in the block
+// (LDS aggregation first, so a community with many members is not a hot spot),
+// then all threads copy the block's rows.  Longer rows: k_stage_rows.
 // ---------------------------------------------------------------------------
-template <int W, class T>
+template <class T>
 __global__ __launch_bounds__(256) void k_stage(DevGraph<T> g, const int32_t* __restrict__ label,
-                                               const int64_t* __restrict__ eoff, int64_t* __restrict__ ecur,
-                                               int64_t dlo, int64_t dhi, int32_t* __restrict__ skey,
+                                               const int64_t* __restrict__ eoff,
+                                               unsigned long long* __restrict__ ecur,
+                                               int32_t* __restrict__ skey,
                                                typename T::acc_t* __restrict__ sval) {
   using acc = typename T::acc_t;
-  const int lane = threadIdx.x & (W - 1);
-  const int groups = gridDim.x * (blockDim.x / W);
-  for (int v = (blockIdx.x * blockDim.x + threadIdx.x) / W; v < g.n; v += groups) {
+  constexpr int HS = 512;
+  __shared__ int64_t s_ofs[257];
+  __shared__ int64_t s_dst[256];
+  __shared__ int64_t s_row[256];
+  __shared__ int s_hk[HS];
+  __shared__ unsigned long long s_hv[HS];
+  __shared__ unsigned long long s_hb[HS];
+  __shared__ int64_t s_sh[17];
+  for (int vb = blockIdx.x * 256; vb < g.n; vb += gridDim.x * 256) {
+    const int nloc = min(256, g.n - vb);
+    const int v = vb + threadIdx.x;
+    const bool valid = threadIdx.x < nloc;
+    const int64_t o = valid ? g.off[v] : 0;
+    const int64_t d0 = valid ? g.off[v + 1] - o : 0;
+    const int64_t d = d0 < 512 ? d0 : 0;  // longer rows (degree bins 6-8): k_stage_rows
+    const int nc = (valid && d) ? label[v] : -1;
+    for (int t = threadIdx.x; t < HS; t += 256) { s_hk[t] = 0; s_hv[t] = 0; }
+    int64_t tot;
+    const int64_t pre = block_excl_scan<int64_t>(d, s_sh, &tot);
+    s_ofs[threadIdx.x] = pre;
+    if (threadIdx.x == 0) s_ofs[nloc] = tot;
+    __syncthreads();
+    unsigned p = 0;
+    unsigned long long intra = 0;
+    if (nc >= 0) {
+      p = chash(nc, 9);
+      for (;;) {
+        const int k = atomicCAS(&s_hk[p], 0, nc + 1);
+        if (k == 0 || k == nc + 1) break;
+        p = (p + 1) & (HS - 1);
+      }
+      intra = atomicAdd(&s_hv[p], (unsigned long long)d);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < HS; t += 256)
+      if (s_hk[t]) s_hb[t] = atomicAdd(&ecur[s_hk[t] - 1], s_hv[t]);
+    __syncthreads();
+    if (nc >= 0) s_dst[threadIdx.x] = eoff[nc] + (int64_t)(s_hb[p] + intra);
+    __syncthreads();
+    if (valid) s_row[threadIdx.x] = o;
+    __syncthreads();
+    for (int64_t q = threadIdx.x; q < tot; q += 256) {
+      int a = 0, z = nloc;  // largest i with s_ofs[i] <= q
+      while (z - a > 1) {
+        const int mid = (a + z) >> 1;
+        if (s_ofs[mid] <= q) a = mid; else z = mid;
+      }
+      const int64_t j = s_row[a] + (q - s_ofs[a]);
+      const int64_t dst = s_dst[a] + (q - s_ofs[a]);
+      skey[dst] = label[g.nbr[j]];
+      sval[dst] = g.w ? (acc)g.w[j] : acc(1);
+    }
+    __syncthreads();
+  }
+}
+
+// rows of the vertices order[bstart[blo] .. bstart[bhi]) (the round's degree
+// bins): NT threads per vertex, one reservation each
+template <int NT, class T>
+__global__ __launch_bounds__(256) void k_stage_rows(DevGraph<T> g, const int32_t* __restrict__ order,
+                                                    const int32_t* __restrict__ bstart, int blo, int bhi,
+                                                    const int32_t* __restrict__ label,
+                                                    const int64_t* __restrict__ eoff,
+                                                    unsigned long long* __restrict__ ecur,
+                                                    int32_t* __restrict__ skey,
+                                                    typename T::acc_t* __restrict__ sval) {
+  using acc = typename T::acc_t;
+  constexpr int G = 256 / NT;  // vertices per block step
+  const int lo = bstart[blo], hi = bstart[bhi];
+  const int lane = threadIdx.x % NT;
+  for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / NT; i < hi; i += gridDim.x * G) {
+    const int v = order[i];
     const int64_t o = g.off[v];
     const int64_t d = g.off[v + 1] - o;
-    if (d < dlo || d > dhi) continue;  // uniform per group
+    const int nc = label[v];
     int64_t base = 0;
-    if (lane == 0) {
-      const int nc = label[v];
-      base = eoff[nc] + (int64_t)atomicAdd((unsigned long long*)&ecur[nc], (unsigned long long)d);
+    if (lane == 0) base = eoff[nc] + (int64_t)atomicAdd(&ecur[nc], (unsigned long long)d);
+    base = __shfl(base, 0, NT > 64 ? 64 : NT);
+    if (NT > 64) {  // broadcast across the waves of the group through LDS
+      __shared__ int64_t s_b;
+      if (threadIdx.x == 0) s_b = base;
+      __syncthreads();
+      base = s_b;
+      __syncthreads();
     }
-    base = __shfl(base, 0, W);
-    for (int64_t j = lane; j < d; j += W) {
+    for (int64_t j = lane; j < d; j += NT) {
       skey[base + j] = label[g.nbr[o + j]];
       sval[base + j] = g.w ? (acc)g.w[o + j] : acc(1);
     }
@@ -365,7 +443,8 @@ __global__ __launch_bounds__(1024) void k_dense_accum(const DenseChunk* __restri
                                                       const int64_t* __restrict__ eoff, int32_t C,
                                                       const int32_t* __restrict__ skey,
                                                       const typename T::acc_t* __restrict__ sval,
-                                                      typename T::acc_t* __restrict__ dense) {
+                                                      typename T::acc_t* __restrict__ dense,
+                                                      uint32_t* __restrict__ dbits, int64_t words) {
   using acc = typename T::acc_t;
   constexpr int NT = 1024, LLG = __builtin_ctz(LT);
   __shared__ int s_k[LT];
@@ -374,6 +453,7 @@ __global__ __launch_bounds__(1024) void k_dense_accum(const DenseChunk* __restri
     const DenseChunk cc = chunks[ch];
     const int64_t base = eoff[cc.c] + cc.start;
     acc* row = dense + (int64_t)cc.slot * C;
+    uint32_t* bits = dbits + (int64_t)cc.slot * words;
     for (int t = threadIdx.x; t < LT; t += NT) { s_k[t] = 0; s_v[t] = 0; }
     __syncthreads();
     // a chunk holds at most LT/2 entries, so the table never fills
@@ -390,69 +470,67 @@ __global__ __launch_bounds__(1024) void k_dense_accum(const DenseChunk* __restri
     __syncthreads();
     for (int t = threadIdx.x; t < LT; t += NT) {
       const int k = s_k[t];
-      if (k) atomicAdd(&row[k - 1], s_v[t]);
+      if (k) {
+        atomicAdd(&row[k - 1], s_v[t]);
+        atomicOr(&bits[((k - 1) >> 5)], 1u << ((k - 1) & 31));
+      }
     }
     __syncthreads();
   }
 }
 
-// non-zero count of every DENSE_TILE-key tile of every dense row
-template <class T>
-__global__ __launch_bounds__(256) void k_dense_count(const typename T::acc_t* __restrict__ dense, int nslots,
-                                                     int32_t C, int ntiles, int32_t* __restrict__ tcnt) {
+// present-key count of every DENSE_TILE-key tile of every dense row (bitmap popcounts)
+__global__ __launch_bounds__(128) void k_dense_count(const uint32_t* __restrict__ dbits, int nslots, int64_t words,
+                                                     int ntiles, int32_t* __restrict__ tcnt) {
   __shared__ int32_t sh[17];
+  constexpr int TW = DENSE_TILE / 32;  // words per tile (one per thread)
   const int total = nslots * ntiles;
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
     const int slot = t / ntiles, tile = t % ntiles;
-    const typename T::acc_t* row = dense + (int64_t)slot * C;
-    const int k0 = tile * DENSE_TILE, k1 = min(C, k0 + DENSE_TILE);
-    int cnt = 0;
-    for (int k = k0 + threadIdx.x; k < k1; k += 256) cnt += row[k] != 0;
+    const int64_t wi = (int64_t)tile * TW + threadIdx.x;
+    const int cnt = wi < words ? __popc(dbits[(int64_t)slot * words + wi]) : 0;
     int tot;
     block_excl_scan<int32_t>(cnt, sh, &tot);
     if (threadIdx.x == 0) tcnt[t] = tot;
   }
 }
 
-// ordered extraction: each tile writes its keys (ascending) at the row's
-// running position and clears the accumulator entries it read
+// ordered extraction: each tile writes its present keys (ascending) at the
+// row's running position and clears the accumulator and bitmap words it read
 template <class T>
-__global__ __launch_bounds__(256) void k_dense_scatter(typename T::acc_t* __restrict__ dense, int nslots,
-                                                       int32_t C, int ntiles, const int32_t* __restrict__ slot_c,
+__global__ __launch_bounds__(128) void k_dense_scatter(typename T::acc_t* __restrict__ dense,
+                                                       uint32_t* __restrict__ dbits, int nslots, int32_t C,
+                                                       int64_t words, int ntiles,
+                                                       const int32_t* __restrict__ slot_c,
                                                        const int64_t* __restrict__ toff,
                                                        const int64_t* __restrict__ eoff,
                                                        int32_t* __restrict__ skey,
                                                        typename T::acc_t* __restrict__ sval,
                                                        int32_t* __restrict__ dcnt) {
   using acc = typename T::acc_t;
-  constexpr int PER = DENSE_TILE / 256;
+  constexpr int TW = DENSE_TILE / 32;
   __shared__ int32_t sh[17];
   const int total = nslots * ntiles;
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
     const int slot = t / ntiles, tile = t % ntiles;
     acc* row = dense + (int64_t)slot * C;
+    uint32_t* bits = dbits + (int64_t)slot * words;
     const int c = slot_c[slot];
     const int64_t rbase = toff[slot * ntiles];
-    const int k0 = tile * DENSE_TILE + threadIdx.x * PER;
-    int cnt = 0;
-#pragma unroll
-    for (int i = 0; i < PER; i++) cnt += (k0 + i < C) && row[k0 + i] != 0;
+    const int64_t wi = (int64_t)tile * TW + threadIdx.x;
+    uint32_t wbits = wi < words ? bits[wi] : 0u;
     int tot;
-    int pos = block_excl_scan<int32_t>(cnt, sh, &tot);
-    const int64_t out = eoff[c] + (toff[t] - rbase) + pos;
-    int q = 0;
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int k = k0 + i;
-      if (k < C) {
-        const acc v = row[k];
-        if (v != 0) {
-          skey[out + q] = k;
-          sval[out + q] = v;
-          row[k] = 0;
-          q++;
-        }
-      }
+    const int pos = block_excl_scan<int32_t>(__popc(wbits), sh, &tot);
+    int64_t out = eoff[c] + (toff[t] - rbase) + pos;
+    if (wbits) bits[wi] = 0;
+    while (wbits) {
+      const int bit = __ffs(wbits) - 1;
+      wbits &= wbits - 1;
+      const int k = (int)(wi * 32 + bit);
+      skey[out] = k;
+      sval[out] = row[k];
+      row[k] = 0;
+      out++;
     }
     if (tile == 0 && threadIdx.x == 0) dcnt[c] = (int32_t)(toff[(slot + 1) * ntiles] - rbase);
   }
@@ -576,14 +654,19 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
   const int32_t n = g.n;
   const int32_t C = plan.C;
   fill<int64_t>(cb.ecur, C, 0, s);
-  k_stage<16, T><<<nblk(n, 16, 16384), 256, 0, s>>>(g, cb.label, cb.eoff, cb.ecur, 1, 256, cb.skey, cb.sval);
-  k_stage<64, T><<<nblk(n, 4, 16384), 256, 0, s>>>(g, cb.label, cb.eoff, cb.ecur, 257, INT64_MAX, cb.skey,
-                                                     cb.sval);
+  unsigned long long* ecur = (unsigned long long*)cb.ecur;
+  k_stage<T><<<nblk(n, 256, 16384), 256, 0, s>>>(g, cb.label, cb.eoff, ecur, cb.skey, cb.sval);
+  k_stage_rows<64, T><<<nblk(plan.caps[6], 4, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 6, 7, cb.label, cb.eoff,
+                                                                    ecur, cb.skey, cb.sval);
+  k_stage_rows<256, T><<<nblk(plan.caps[7] + plan.caps[8], 1, 4096), 256, 0, s>>>(
+      g, cb.order, cb.bstart, 7, NBINS, cb.label, cb.eoff, ecur, cb.skey, cb.sval);
   fill<int32_t>(cb.dcnt, C + 1, 0, s);
   k_merge_tiny<T><<<nblk(C, 4, 16384), 256, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval, cb.dcnt);
   k_merge_mid<64, 64, 512, T><<<nblk(C, 1, 16384), 64, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval, cb.dcnt);
-  k_merge_mid<256, 512, 4096, T><<<nblk(C, 1, 4096), 256, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval,
+  k_merge_mid<128, 512, 2048, T><<<nblk(C, 1, 8192), 128, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval,
                                                                    cb.dcnt);
+  k_merge_mid<256, 2048, 4096, T><<<nblk(C, 1, 4096), 256, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval,
+                                                                    cb.dcnt);
   if (plan.nbig > 0)
     k_merge_big<LT, T><<<std::min(plan.nbig, 2048), 1024, 0, s>>>(cb.ecnt, cb.eoff, C, cb.big_list, plan.nbig,
                                                                    cb.big_tab, cb.arena_keys, cb.arena_vals,
@@ -606,14 +689,15 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
       if (hipMemcpyAsync(cb.dense_c, plan.dense_c.data() + s0, sizeof(int32_t) * nb, hipMemcpyHostToDevice, s) !=
           hipSuccess)
         return -1;
+      const int64_t words = (C + 31) / 32;
       k_dense_accum<LT, T><<<(int)std::min<size_t>(ch.size(), 4096), 1024, 0, s>>>(
-          cb.chunks, (int)ch.size(), cb.eoff, C, cb.skey, cb.sval, cb.dense);
-      k_dense_count<T><<<std::min(nb * ntiles, 16384), 256, 0, s>>>(cb.dense, nb, C, ntiles, cb.dense_tcnt);
+          cb.chunks, (int)ch.size(), cb.eoff, C, cb.skey, cb.sval, cb.dense, cb.dense_bits, words);
+      k_dense_count<<<std::min(nb * ntiles, 16384), 128, 0, s>>>(cb.dense_bits, nb, words, ntiles, cb.dense_tcnt);
       scan_exclusive<int32_t, int64_t>(cb.dense_tcnt, cb.dense_toff, (int64_t)nb * ntiles, cb.i64_tmp,
                                        cb.dense_toff + (int64_t)nb * ntiles, s);
-      k_dense_scatter<T><<<std::min(nb * ntiles, 16384), 256, 0, s>>>(cb.dense, nb, C, ntiles, cb.dense_c,
-                                                                       cb.dense_toff, cb.eoff, cb.skey, cb.sval,
-                                                                       cb.dcnt);
+      k_dense_scatter<T><<<std::min(nb * ntiles, 16384), 128, 0, s>>>(cb.dense, cb.dense_bits, nb, C, words,
+                                                                       ntiles, cb.dense_c, cb.dense_toff, cb.eoff,
+                                                                       cb.skey, cb.sval, cb.dcnt);
       if (hipStreamSynchronize(s) != hipSuccess) return -1;  // host chunk vector reused
     }
   }
