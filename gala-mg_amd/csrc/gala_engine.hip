@@ -105,6 +105,21 @@ struct gala_ctx {
   int64_t* dense_toff = nullptr;
   int64_t dense_cap = 0;  // slots of dense_c, tiles of dense_tcnt/toff
 
+  // learned frontier
+  float* last_gain = nullptr;
+  int32_t* moved_it = nullptr;
+  float* nbr_moved = nullptr;
+  int32_t* front2 = nullptr;
+  int32_t* bstart2 = nullptr;
+  uint8_t* keep = nullptr;
+  float* feat = nullptr;        // logged features of the frontier
+  uint8_t* lab = nullptr;
+  float* mlp_prm = nullptr;     // MLP_NPARAM fp32 (persistent across graphs)
+  short* mlp_w1b = nullptr;     // BF16 copy of W1
+  bool have_mlp = false;
+  gala_log_cb logger = nullptr;
+  void* logger_user = nullptr;
+
   std::vector<void*> owned;
   gala_result res{};
   bool have_result = false;
@@ -169,6 +184,9 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->size, N); OWN(ctx->dsize, N); OWN(ctx->prev, N); OWN(ctx->cur, N);
   OWN(ctx->active, N); OWN(ctx->dirty, N); OWN(ctx->binid, N);
   OWN(ctx->order, N); OWN(ctx->front, N); OWN(ctx->fin, N);
+  OWN(ctx->last_gain, N); OWN(ctx->moved_it, N); OWN(ctx->nbr_moved, N);
+  OWN(ctx->front2, N); OWN(ctx->bstart2, NBINS + 1); OWN(ctx->keep, N);
+  OWN(ctx->feat, N * MLP_IN); OWN(ctx->lab, N);
   OWN(ctx->bstart_full, NBINS + 1); OWN(ctx->bstart_front, NBINS + 1);
   OWN(ctx->hub_off, N + 1); OWN(ctx->hub_tmp, 2 * (N + 1));
   OWN(ctx->part_tmp, part_tmp_elems(N));
@@ -355,6 +373,11 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.active = ctx->active; st.dirty = ctx->dirty; st.binid = ctx->binid; st.hub_off = ctx->hub_off;
   st.hub_keys = ctx->hub_keys; st.hub_vals = (typename T::acc_t*)ctx->hub_vals;
   st.ctr = ctx->ctr;
+  st.track = 0;
+  st.iter = 0;
+  st.last_gain = ctx->last_gain;
+  st.moved_it = ctx->moved_it;
+  st.nbr_moved = ctx->nbr_moved;
   return st;
 }
 
@@ -368,9 +391,10 @@ double q_from(const Counters& c, double m2) {
 // caps[b] = vertices of bin b; *nitems = vertices in any bin (degree > 0),
 // i.e. the valid prefix of ctx->order.
 template <class T>
-int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nitems) {
+int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nitems, int track = 0) {
   hipStream_t s = ctx->stream;
   DevState<T> st = dstate<T>(ctx);
+  st.track = track;
   launch_init<T>(g, st, s);
   partition_bins(0, nullptr, ctx->binid, nullptr, g.n, ctx->order, ctx->bstart_full, ctx->part_tmp, s);
   int32_t hb[NBINS + 1];
@@ -413,10 +437,14 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   using tt = typename T::tot_t;
   hipStream_t s = ctx->stream;
   const double m2d = (double)m2;
+  const bool learned = ctx->opts.frontier_mode == 1 && ctx->have_mlp;
+  const bool logging = ctx->logger != nullptr;
+  const int track = (learned || logging) ? 1 : 0;
   int32_t caps[NBINS], nitems = 0;
-  int rc = round_setup<T>(ctx, g, caps, &nitems);
+  int rc = round_setup<T>(ctx, g, caps, &nitems, track);
   if (rc) return rc;
   DevState<T> st = dstate<T>(ctx);
+  st.track = track;
   // Iteration 0: Q on the singleton partition (louvain_main_process.cu:538-548)
   reset_counters(ctx->ctr, (long long)m2, m2d, s);
   launch_apply<T>(st, g.n, m2, s);
@@ -437,21 +465,49 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   // first frontier: every vertex (fill(active,1), louvain_main_process.cu:574)
   copy_i32(ctx->order, ctx->front, nitems, s);
   copy_i32(ctx->bstart_full, ctx->bstart_front, NBINS + 1, s);
+  int32_t nfront = nitems;  // vertices in the plain frontier
   const int max_it = ctx->opts.max_iterations > 0 ? ctx->opts.max_iterations : 100000;
+  const float tau = ctx->opts.learned_tau > 0 ? ctx->opts.learned_tau : 0.5f;
   double t_dm = 0, t_upd = 0;
+  bool certify = false;
+  std::vector<float> hfeat;
+  std::vector<uint8_t> hlab;
   int it = 1;
   for (;; it++) {
+    st.iter = it;
     reset_counters(ctx->ctr, (long long)m2, m2d, s);
+    // a learned sweep visits only the predicted movers of the plain frontier;
+    // the first sweep of a round and certification sweeps are plain
+    const bool lsweep = learned && it >= 2 && !certify;
+    const int32_t* sfront = ctx->front;
+    const int32_t* sbstart = ctx->bstart_front;
+    if (lsweep) {
+      launch_mlp_filter<T>(g, st, ctx->front, nfront, it, ctx->mlp_prm, ctx->mlp_w1b, tau, ctx->keep, s);
+      partition_bins(1, ctx->front, ctx->binid, ctx->keep, nfront, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
+      sfront = ctx->front2;
+      sbstart = ctx->bstart2;
+    }
+    const bool log_it = logging && !lsweep && it >= 2;
+    if (log_it) launch_features<T>(g, st, ctx->front, nfront, it, ctx->feat, s);
     HIPOK(hipEventRecord(ctx->ev[0], s));
     const bool first = it == 1 && unique_rows;  // singleton partition: no aggregation needed
     const bool prof = ctx->opts.profile_bins != 0;
     for (int b = 0; b < NBINS; b++) {
       if (prof) HIPOK(hipEventRecord(ctx->bev[b], s));
-      if (first) launch_decide_first<T>(b, g, st, ctx->front, ctx->bstart_front, caps[b], m2, s);
-      else launch_decide<T>(b, g, st, ctx->front, ctx->bstart_front, caps[b], m2, s);
+      if (first) launch_decide_first<T>(b, g, st, sfront, sbstart, caps[b], m2, s);
+      else launch_decide<T>(b, g, st, sfront, sbstart, caps[b], m2, s);
     }
     if (prof) HIPOK(hipEventRecord(ctx->bev[NBINS], s));
     HIPOK(hipEventRecord(ctx->ev[1], s));
+    if (log_it) {
+      launch_labels(ctx->front, nfront, ctx->moved_it, it, ctx->lab, s);
+      hfeat.resize((size_t)nfront * MLP_IN);
+      hlab.resize(nfront);
+      HIPOK(hipMemcpyAsync(hfeat.data(), ctx->feat, sizeof(float) * hfeat.size(), hipMemcpyDeviceToHost, s));
+      HIPOK(hipMemcpyAsync(hlab.data(), ctx->lab, hlab.size(), hipMemcpyDeviceToHost, s));
+      HIPOK(hipStreamSynchronize(s));
+      ctx->logger(ctx->logger_user, round, it, nfront, hfeat.data(), hlab.data());
+    }
     launch_apply<T>(st, g.n, m2, s);
     // In changes only where a vertex or one of its neighbours moved
     for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s);
@@ -460,6 +516,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
                    ctx->part_tmp, s);
     HIPOK(hipEventRecord(ctx->ev[2], s));
     if ((rc = read_counters<T>(ctx, &c))) return rc;
+    nfront = (int32_t)c.active;
     float a = 0, b2 = 0;
     (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
     (void)hipEventElapsedTime(&b2, ctx->ev[1], ctx->ev[2]);
@@ -484,6 +541,8 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
         }
       }
     }
+    if (lsweep) ctx->res.learned_sweeps++;
+    if (certify) ctx->res.certify_sweeps++;
     q = q_from<T>(c, m2d);
     if (icb) {
       const int32_t* vec = nullptr;
@@ -494,7 +553,15 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       }
       icb(user, round, it, q, (int64_t)c.moved, vec, g.n);
     }
-    if (q - qprev < ctx->opts.min_modularity || it >= max_it) break;
+    if (lsweep) {
+      // a stalled learned sweep is kept and the level goes to certification:
+      // the next sweep is a full plain sweep over every prune-active vertex
+      if (q - qprev < ctx->opts.min_modularity) certify = true;
+      if (it >= max_it) break;
+    } else {
+      if (q - qprev < ctx->opts.min_modularity || it >= max_it) break;  // plain rule, with revert
+      certify = false;
+    }
     copy_i32(ctx->cur, ctx->prev, g.n, s);  // prev = cur (louvain_main_process.cu:1029)
     qprev = q;
   }
@@ -777,6 +844,8 @@ void gala_destroy(gala_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_all(ctx);
+  if (ctx->mlp_prm) (void)hipFree(ctx->mlp_prm);
+  if (ctx->mlp_w1b) (void)hipFree(ctx->mlp_w1b);
   if (ctx->ctr_h) (void)hipHostFree(ctx->ctr_h);
   for (auto& e : ctx->ev) (void)hipEventDestroy(e);
   for (auto& e : ctx->bev) (void)hipEventDestroy(e);
@@ -929,6 +998,179 @@ int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const
     for (int64_t j = 0; j < m; j++) out_w[j] = w32[j];
   }
   ctx->have_graph = false;
+  return GALA_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned frontier: MLP parameters, training, move logs
+// ---------------------------------------------------------------------------
+namespace {
+
+int ensure_mlp(gala_ctx* ctx) {
+  if (ctx->mlp_prm) return GALA_OK;
+  hipError_t e = hipMalloc((void**)&ctx->mlp_prm, sizeof(float) * MLP_NPARAM);
+  if (e == hipSuccess) e = hipMalloc((void**)&ctx->mlp_w1b, sizeof(short) * MLP_B1);
+  if (e != hipSuccess) { ctx->last_hip = e; return GALA_E_OOM; }
+  return GALA_OK;
+}
+
+int upload_mlp(gala_ctx* ctx, const float* params) {
+  int rc = ensure_mlp(ctx);
+  if (rc) return rc;
+  std::vector<float> p(params, params + MLP_NPARAM);
+  for (int k = MLP_IN; k < 16; k++)  // padded feature rows of W1 stay zero
+    for (int h = 0; h < MLP_H; h++) p[MLP_W1 + k * MLP_H + h] = 0.f;
+  HIPOK(hipMemcpy(ctx->mlp_prm, p.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
+  launch_w1_bf16(ctx->mlp_prm, ctx->mlp_w1b, ctx->stream);
+  HIPOK(hipStreamSynchronize(ctx->stream));
+  return GALA_OK;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int gala_set_logger(gala_ctx* ctx, gala_log_cb cb, void* user) {
+  if (!ctx) return GALA_E_INVALID;
+  ctx->logger = cb;
+  ctx->logger_user = user;
+  return GALA_OK;
+}
+
+int gala_set_mlp(gala_ctx* ctx, const float* params) {
+  if (!ctx || !params) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  int rc = upload_mlp(ctx, params);
+  if (rc) return rc;
+  ctx->have_mlp = true;
+  return GALA_OK;
+}
+
+int gala_get_mlp(gala_ctx* ctx, float* params) {
+  if (!ctx || !params || !ctx->have_mlp) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  HIPOK(hipMemcpy(params, ctx->mlp_prm, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+  return GALA_OK;
+}
+
+int gala_mlp_forward(gala_ctx* ctx, const float* params, const float* X, int64_t n, float* p_out) {
+  if (!ctx || !params || (n > 0 && (!X || !p_out)) || n < 0 || n > INT32_MAX) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  std::vector<float> keep_prm;
+  int rc = upload_mlp(ctx, params);
+  if (rc) return rc;
+  ctx->have_mlp = true;
+  DevBuf dx, dp;
+  HIPOK(hipMalloc(&dx.p, sizeof(float) * std::max<int64_t>(n, 1) * MLP_IN));
+  HIPOK(hipMalloc(&dp.p, sizeof(float) * std::max<int64_t>(n, 1)));
+  HIPOK(hipMemcpy(dx.p, X, sizeof(float) * n * MLP_IN, hipMemcpyHostToDevice));
+  launch_mlp_forward((const float*)dx.p, (int)n, ctx->mlp_prm, ctx->mlp_w1b, (float*)dp.p, ctx->stream);
+  HIPOK(hipStreamSynchronize(ctx->stream));
+  HIPOK(hipMemcpy(p_out, dp.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return GALA_OK;
+}
+
+int gala_mlp_grad(gala_ctx* ctx, const float* params, const float* X, const uint8_t* y, int64_t n, float pos_weight,
+                  float* grad_out, float* loss_out) {
+  if (!ctx || !params || !X || !y || !grad_out || n <= 0 || n > INT32_MAX) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  int rc = upload_mlp(ctx, params);
+  if (rc) return rc;
+  ctx->have_mlp = true;
+  hipStream_t s = ctx->stream;
+  double wsum = 0;
+  for (int64_t i = 0; i < n; i++) wsum += y[i] ? pos_weight : 1.0;
+  std::vector<int32_t> idx(n);
+  for (int64_t i = 0; i < n; i++) idx[i] = (int32_t)i;
+  DevBuf dx, dy, di, dg, dl;
+  HIPOK(hipMalloc(&dx.p, sizeof(float) * n * MLP_IN));
+  HIPOK(hipMalloc(&dy.p, n));
+  HIPOK(hipMalloc(&di.p, sizeof(int32_t) * n));
+  HIPOK(hipMalloc(&dg.p, sizeof(float) * MLP_NPARAM));
+  HIPOK(hipMalloc(&dl.p, sizeof(float)));
+  HIPOK(hipMemcpy(dx.p, X, sizeof(float) * n * MLP_IN, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(dy.p, y, n, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(di.p, idx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  fill<float>((float*)dg.p, MLP_NPARAM, 0.f, s);
+  fill<float>((float*)dl.p, 1, 0.f, s);
+  launch_mlp_grad((const float*)dx.p, (const uint8_t*)dy.p, (const int32_t*)di.p, (int)n, ctx->mlp_prm, ctx->mlp_w1b,
+                  pos_weight, (float)(1.0 / wsum), (float*)dg.p, (float*)dl.p, s);
+  HIPOK(hipStreamSynchronize(s));
+  HIPOK(hipMemcpy(grad_out, dg.p, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+  if (loss_out) HIPOK(hipMemcpy(loss_out, dl.p, sizeof(float), hipMemcpyDeviceToHost));
+  return GALA_OK;
+}
+
+int gala_mlp_train(gala_ctx* ctx, const float* X, const uint8_t* y, int64_t n, int epochs, int batch, float lr,
+                   float pos_weight, uint64_t seed, int init_given, float* params_io, float* loss_out) {
+  if (!ctx || !X || !y || !params_io || n <= 0 || n > INT32_MAX || epochs <= 0 || batch <= 0) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  hipStream_t s = ctx->stream;
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  auto rnd = [&]() {  // splitmix64 -> [0, 1)
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  std::vector<float> prm(MLP_NPARAM, 0.f);
+  if (init_given) {
+    std::copy(params_io, params_io + MLP_NPARAM, prm.begin());
+  } else {  // Glorot-uniform weights, zero biases
+    const double a1 = std::sqrt(6.0 / (MLP_IN + MLP_H)), a2 = std::sqrt(6.0 / (MLP_H + 1));
+    for (int k = 0; k < MLP_IN; k++)
+      for (int h = 0; h < MLP_H; h++) prm[MLP_W1 + k * MLP_H + h] = (float)((2 * rnd() - 1) * a1);
+    for (int h = 0; h < MLP_H; h++) prm[MLP_W2 + h] = (float)((2 * rnd() - 1) * a2);
+  }
+  int rc = upload_mlp(ctx, prm.data());
+  if (rc) return rc;
+  DevBuf dx, dy, di, dg, dl, dm, dv;
+  HIPOK(hipMalloc(&dx.p, sizeof(float) * n * MLP_IN));
+  HIPOK(hipMalloc(&dy.p, n));
+  HIPOK(hipMalloc(&di.p, sizeof(int32_t) * n));
+  HIPOK(hipMalloc(&dg.p, sizeof(float) * MLP_NPARAM));
+  HIPOK(hipMalloc(&dl.p, sizeof(float)));
+  HIPOK(hipMalloc(&dm.p, sizeof(float) * MLP_NPARAM));
+  HIPOK(hipMalloc(&dv.p, sizeof(float) * MLP_NPARAM));
+  HIPOK(hipMemcpy(dx.p, X, sizeof(float) * n * MLP_IN, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(dy.p, y, n, hipMemcpyHostToDevice));
+  fill<float>((float*)dm.p, MLP_NPARAM, 0.f, s);
+  fill<float>((float*)dv.p, MLP_NPARAM, 0.f, s);
+  std::vector<int32_t> idx(n);
+  for (int64_t i = 0; i < n; i++) idx[i] = (int32_t)i;
+  int step = 0;
+  for (int ep = 0; ep < epochs; ep++) {
+    for (int64_t i = n - 1; i > 0; i--) {  // Fisher-Yates
+      int64_t j = (int64_t)(rnd() * (double)(i + 1));
+      std::swap(idx[i], idx[j]);
+    }
+    HIPOK(hipMemcpy(di.p, idx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    fill<float>((float*)dl.p, 1, 0.f, s);
+    int nb = 0;
+    for (int64_t b0 = 0; b0 < n; b0 += batch, nb++) {
+      const int bn = (int)std::min<int64_t>(batch, n - b0);
+      double wsum = 0;
+      for (int k = 0; k < bn; k++) wsum += y[idx[b0 + k]] ? pos_weight : 1.0;
+      fill<float>((float*)dg.p, MLP_NPARAM, 0.f, s);
+      launch_mlp_grad((const float*)dx.p, (const uint8_t*)dy.p, (const int32_t*)di.p + b0, bn, ctx->mlp_prm,
+                      ctx->mlp_w1b, pos_weight, (float)(1.0 / wsum), (float*)dg.p, (float*)dl.p, s);
+      launch_adam(ctx->mlp_prm, (float*)dm.p, (float*)dv.p, (const float*)dg.p, lr, ++step, ctx->mlp_w1b, s);
+    }
+    float ls = 0;
+    HIPOK(hipMemcpyAsync(&ls, dl.p, sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPOK(hipStreamSynchronize(s));
+    if (loss_out) loss_out[ep] = ls / (float)nb;
+  }
+  HIPOK(hipMemcpy(params_io, ctx->mlp_prm, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+  ctx->have_mlp = true;
   return GALA_OK;
 }
 

@@ -41,6 +41,13 @@ __host__ __device__ inline int bin_of_degree(int64_t d) {
 template <class T>
 constexpr int global_bin_min() { return T::FP ? 7 : 8; }
 
+// Learned-frontier MLP: 6 features -> 64 hidden (ReLU) -> 1 logit.
+// Parameter vector layout (fp32): W1[16][64] (feature rows 6..15 are zero
+// padding for the 16-deep MFMA), b1[64], w2[64], b2.
+constexpr int MLP_IN = 6, MLP_H = 64;
+constexpr int MLP_W1 = 0, MLP_B1 = 16 * MLP_H, MLP_W2 = MLP_B1 + MLP_H, MLP_B2 = MLP_W2 + MLP_H;
+constexpr int MLP_NPARAM = MLP_B2 + 1;
+
 // Per-iteration device counters (one small struct read back per iteration).
 struct Counters {
   unsigned long long sum_in;     // sum over v of In[v]            (exact, int path)

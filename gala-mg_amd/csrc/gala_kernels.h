@@ -42,6 +42,12 @@ struct DevState {
   int32_t* hub_keys;
   typename T::acc_t* hub_vals;
   Counters* ctr;
+  // learned-frontier features (tracked only when track != 0)
+  int track;
+  int iter;          // sweep number within the round (1 = first sweep)
+  float* last_gain;  // normalised net gain of the best move at the vertex's last sweep
+  int32_t* moved_it; // sweep in which the vertex last moved (-1: never this round)
+  float* nbr_moved;  // fraction of neighbours that moved in the latest sweep
 };
 
 template <class T>
@@ -70,6 +76,25 @@ void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
 // prune flags, sum In, active count; clears the dirty marks.
 template <class T>
 void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);
+
+// ---- learned frontier (kernels_learned.hip) ----
+// Features of vertex v before sweep it (all in [-1, 1]):
+//   f0 log2(1 + degree)/16, f1 In/K (weight fraction inside its community),
+//   f2 normalised net gain of its best move at its last sweep,
+//   f3 moved in sweep it-1, f4 fraction of neighbours that moved in sweep it-1,
+//   f5 min(it, 16)/16.
+template <class T>
+void launch_mlp_filter(const DevGraph<T>& g, DevState<T>& st, const int32_t* front, int nfront, int it,
+                       const float* prm, const short* w1b, float tau, uint8_t* keep, hipStream_t s);
+template <class T>
+void launch_features(const DevGraph<T>& g, DevState<T>& st, const int32_t* front, int nfront, int it, float* out,
+                     hipStream_t s);
+void launch_labels(const int32_t* front, int nfront, const int32_t* moved_it, int it, uint8_t* lab, hipStream_t s);
+void launch_mlp_forward(const float* X, int n, const float* prm, const short* w1b, float* p, hipStream_t s);
+void launch_mlp_grad(const float* X, const uint8_t* y, const int32_t* idx, int n, const float* prm, const short* w1b,
+                     float pos_w, float inv_wsum, float* grad, float* loss, hipStream_t s);
+void launch_adam(float* prm, float* m, float* v, const float* grad, float lr, int step, short* w1b, hipStream_t s);
+void launch_w1_bf16(const float* prm, short* w1b, hipStream_t s);
 
 // ---- contraction (kernels_contract.hip) ----
 // One chunk of a dense row's staged segment (see kernels_contract.hip).

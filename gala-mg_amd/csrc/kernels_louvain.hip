@@ -87,15 +87,22 @@ template <class T>
 __device__ __forceinline__ bool commit_move(int v, int c, typename T::gain_t bestH, int bestD,
                                             typename T::acc_t wc, typename T::tot_t K,
                                             typename T::tot_t S, typename T::tot_t m2,
-                                            const int32_t* __restrict__ size, int32_t* __restrict__ cur,
-                                            typename T::tot_t* dTot, int32_t* dsize) {
+                                            const DevState<T>& st) {
   using G = typename T::gain_t;
+  const int32_t* __restrict__ size = st.size;
   int nc = c;
+  G gain = -(G)m2 * ((G)wc - (G)S);  // no positive candidate: the cost of leaving
   if (bestD != INT_MAX) {
-    G gain = gain_net<T>(bestH, (G)m2, (G)wc, (G)S);
+    gain = gain_net<T>(bestH, (G)m2, (G)wc, (G)S);
     if (gain > 0 && !(size[bestD] == 1 && size[c] == 1 && bestD > c)) nc = bestD;
   }
-  cur[v] = nc;
+  st.cur[v] = nc;
+  if (st.track) {
+    st.last_gain[v] = (float)((double)gain / ((double)m2 * (double)K));
+    if (nc != c) st.moved_it[v] = st.iter;
+  }
+  typename T::tot_t* dTot = st.dTot;
+  int32_t* dsize = st.dsize;
   if (nc != c) {
     atomicAdd(&dTot[nc], K);
     atomicAdd(&dTot[c], -K);
@@ -137,6 +144,11 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
     st.dTot[v] = 0;
     st.dsize[v] = 0;
     st.dirty[v] = 0;
+    if (st.track) {
+      st.last_gain[v] = 0.f;
+      st.moved_it[v] = -1;
+      st.nbr_moved[v] = 0.f;
+    }
     if (d > 64) continue;  // wide rows: k_init_wide
     tt k = 0, self = 0;
     for (int64_t j = o; j < e; j++) {
@@ -243,7 +255,7 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
     if (lane == 0) {
       swept++;
       edges += d;
-      mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize);
+      mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st);
       moved += mv;
     }
     if (st.dirty && __shfl(mv, 0, W)) {  // In of the mover and its neighbours must be redone
@@ -336,7 +348,7 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
     if (lane == 0) {
       swept++;
       edges += d;
-      mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize);
+      mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st);
       moved += mv;
     }
     if (st.dirty && __shfl(mv, 0, 64)) mark_row(st.dirty, g.nbr, o, d, v, lane, 64);
@@ -442,7 +454,7 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
       }
       swept++;
       edges += d;
-      const int mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize);
+      const int mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st);
       moved += mv;
       s_mv = mv;
     }
@@ -566,7 +578,7 @@ __global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> 
       }
       swept++;
       edges += d;
-      const int mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st.size, st.cur, st.dTot, st.dsize);
+      const int mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st);
       moved += mv;
       s_mv = mv;
     }
@@ -622,7 +634,7 @@ __global__ __launch_bounds__(256) void k_decide_first(DevGraph<T> g, DevState<T>
       swept++;
       edges += d;
       const tt S = st.Self[v];
-      mv = commit_move<T>(v, v, h, bd, (typename T::acc_t)S, K, S, m2, st.size, st.cur, st.dTot, st.dsize);
+      mv = commit_move<T>(v, v, h, bd, (typename T::acc_t)S, K, S, m2, st);
       moved += mv;
     }
     if (st.dirty && __shfl(mv, 0, W)) mark_row(st.dirty, g.nbr, o, d, v, lane, W);
@@ -680,7 +692,7 @@ __global__ __launch_bounds__(NT) void k_decide_first_block(DevGraph<T> g, DevSta
       swept++;
       edges += d;
       const tt S = st.Self[v];
-      const int mv = commit_move<T>(v, v, h, bd, (typename T::acc_t)S, K, S, m2, st.size, st.cur, st.dTot, st.dsize);
+      const int mv = commit_move<T>(v, v, h, bd, (typename T::acc_t)S, K, S, m2, st);
       moved += mv;
       s_mv = mv;
     }
@@ -853,10 +865,18 @@ __global__ __launch_bounds__(256) void k_weight_group(DevGraph<T> g, DevState<T>
     const int d = (int)(g.off[v + 1] - o);
     const int c = st.cur[v];
     tt x = 0;
-    for (int j = lane; j < d; j += W)
-      if (st.cur[g.nbr[o + j]] == c) x += (tt)load_w<T>(g.w, o + j);
+    int mvn = 0;
+    for (int j = lane; j < d; j += W) {
+      const int u = g.nbr[o + j];
+      if (st.cur[u] == c) x += (tt)load_w<T>(g.w, o + j);
+      if (st.track) mvn += st.moved_it[u] == st.iter;
+    }
     x = sum_xor<tt, W>(x);
-    if (lane == 0) st.In[v] = x;
+    if (st.track) mvn = sum_xor<int, W>(mvn);
+    if (lane == 0) {
+      st.In[v] = x;
+      if (st.track) st.nbr_moved[v] = d ? (float)mvn / (float)d : 0.f;
+    }
   }
 }
 
@@ -867,6 +887,7 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
                                                       int only_dirty) {
   using tt = typename T::tot_t;
   __shared__ tt r[NT / 64];
+  __shared__ int rm[NT / 64];
   const int lo = bstart[bin], hi = bstart[bin + 1];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
@@ -876,15 +897,22 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
     const int d = (int)(g.off[v + 1] - o);
     const int c = st.cur[v];
     tt x = 0;
-    for (int j = threadIdx.x; j < d; j += NT)
-      if (st.cur[g.nbr[o + j]] == c) x += (tt)load_w<T>(g.w, o + j);
+    int mvn = 0;
+    for (int j = threadIdx.x; j < d; j += NT) {
+      const int u = g.nbr[o + j];
+      if (st.cur[u] == c) x += (tt)load_w<T>(g.w, o + j);
+      if (st.track) mvn += st.moved_it[u] == st.iter;
+    }
     x = sum_xor<tt, 64>(x);
-    if (lane == 0) r[wid] = x;
+    mvn = sum_xor<int, 64>(mvn);
+    if (lane == 0) { r[wid] = x; rm[wid] = mvn; }
     __syncthreads();
     if (threadIdx.x == 0) {
       x = 0;
-      for (int q = 0; q < NT / 64; q++) x += r[q];
+      mvn = 0;
+      for (int q = 0; q < NT / 64; q++) { x += r[q]; mvn += rm[q]; }
       st.In[v] = x;
+      if (st.track) st.nbr_moved[v] = d ? (float)mvn / (float)d : 0.f;
     }
     __syncthreads();
   }
@@ -925,6 +953,7 @@ __global__ __launch_bounds__(256) void k_prune(DevState<T> st, int32_t n, typena
     const tt x = st.In[v];
     const uint8_t keep = prune_keep<T>(x, st.K[v], st.Self[v], st.Tot[st.cur[v]], minTot, m2);
     st.active[v] = keep;
+    if (st.track && !st.dirty[v]) st.nbr_moved[v] = 0.f;  // no neighbour moved
     st.dirty[v] = 0;
     act += keep;
     if (T::FP) insumf += (double)x;

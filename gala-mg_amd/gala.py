@@ -23,8 +23,12 @@ EXPORTS = [
     "gala_load_bin", "gala_set_graph", "gala_run", "gala_partial_result",
     "gala_get_communities", "gala_phase_decide", "gala_phase_weight", "gala_phase_contract",
     "gala_read_bin", "gala_write_bin", "gala_ingest_text", "gala_gen_sbm", "gala_gen_chung_lu",
-    "gala_gen_rmat",
+    "gala_gen_rmat", "gala_set_logger", "gala_set_mlp", "gala_get_mlp", "gala_mlp_train",
+    "gala_mlp_forward", "gala_mlp_grad",
 ]
+
+MLP_NFEAT = 6
+MLP_NPARAM = 1153  # W1[16][64] (rows 6..15 zero), b1[64], w2[64], b2
 
 
 class GalaError(RuntimeError):
@@ -52,6 +56,7 @@ class Result(ctypes.Structure):
                 ("round_nc", ctypes.c_int32 * 64), ("round_iters", ctypes.c_int32 * 64),
                 ("round_ms", ctypes.c_double * 64), ("round_decide_ms", ctypes.c_double * 64),
                 ("round_update_ms", ctypes.c_double * 64), ("round_build_ms", ctypes.c_double * 64),
+                ("learned_sweeps", ctypes.c_int32), ("certify_sweeps", ctypes.c_int32),
                 ("bin_decide_bytes", ctypes.c_int64 * 9), ("bin_swept_edges", ctypes.c_int64 * 9),
                 ("bin_decide_ms", ctypes.c_double * 9)]
 
@@ -71,6 +76,7 @@ class Result(ctypes.Structure):
                     round_decide_ms=list(self.round_decide_ms[:r]),
                     round_update_ms=list(self.round_update_ms[:r]),
                     round_build_ms=list(self.round_build_ms[:r]),
+                    learned_sweeps=self.learned_sweeps, certify_sweeps=self.certify_sweeps,
                     bin_decide_bytes=list(self.bin_decide_bytes), bin_swept_edges=list(self.bin_swept_edges),
                     bin_decide_ms=list(self.bin_decide_ms))
 
@@ -81,6 +87,9 @@ ROUND_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_double
                             ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double))
+
+LOG_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int32,
+                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint8))
 
 _lib = None
 I64P = ctypes.POINTER(ctypes.c_int64)
@@ -124,6 +133,14 @@ def lib():
                                     ctypes.c_double, ctypes.c_uint64, ctypes.c_float,
                                     ctypes.c_float, I32P, I64P, I64P, I32P,
                                     ctypes.POINTER(ctypes.c_float)]
+        F32P = ctypes.POINTER(ctypes.c_float)
+        L.gala_set_logger.argtypes = [VP, LOG_CB, VP]
+        L.gala_set_mlp.argtypes = [VP, F32P]
+        L.gala_get_mlp.argtypes = [VP, F32P]
+        L.gala_mlp_train.argtypes = [VP, F32P, U8P, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_uint64, ctypes.c_int, F32P, F32P]
+        L.gala_mlp_forward.argtypes = [VP, F32P, F32P, ctypes.c_int64, F32P]
+        L.gala_mlp_grad.argtypes = [VP, F32P, F32P, U8P, ctypes.c_int64, ctypes.c_float, F32P, F32P]
         _lib = L
     return _lib
 
@@ -249,7 +266,7 @@ class Engine:
     """One device context (gala_create ... gala_destroy)."""
 
     def __init__(self, device=0, min_modularity=1e-6, dump_iterations=False, dump_rounds=False,
-                 max_rounds=0, max_iterations=0, profile_bins=False):
+                 max_rounds=0, max_iterations=0, profile_bins=False, frontier_mode=0, learned_tau=0.5):
         L = lib()
         self.opts = Opts()
         L.gala_default_opts(ctypes.byref(self.opts))
@@ -259,6 +276,8 @@ class Engine:
         self.opts.max_rounds = max_rounds
         self.opts.max_iterations = max_iterations
         self.opts.profile_bins = int(profile_bins)
+        self.opts.frontier_mode = int(frontier_mode)
+        self.opts.learned_tau = float(learned_tau)
         self.ctx = VP()
         _check(L.gala_create(ctypes.byref(self.ctx), device, ctypes.byref(self.opts)), "gala_create")
 
@@ -309,6 +328,55 @@ class Engine:
         res = Result()
         _check(lib().gala_run(self.ctx, self._icb, self._rcb, None, ctypes.byref(res)), "gala_run")
         return res.to_dict()
+
+    # ---- learned frontier ----
+    def set_logger(self, sink):
+        """sink(round, iteration, features[n, 6], moved[n]) for every plain sweep after the first."""
+        def cb(user, rnd, it, n, feat, moved):
+            f = np.ctypeslib.as_array(feat, shape=(n, MLP_NFEAT)).copy() if n else np.zeros((0, MLP_NFEAT), np.float32)
+            m = np.ctypeslib.as_array(moved, shape=(n,)).copy() if n else np.zeros(0, np.uint8)
+            sink(rnd, it, f, m)
+        self._logcb = LOG_CB(cb) if sink is not None else LOG_CB()
+        _check(lib().gala_set_logger(self.ctx, self._logcb, None), "gala_set_logger")
+
+    def set_mlp(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == MLP_NPARAM
+        _check(lib().gala_set_mlp(self.ctx, _p(p, ctypes.c_float)), "gala_set_mlp")
+
+    def get_mlp(self):
+        p = np.zeros(MLP_NPARAM, np.float32)
+        _check(lib().gala_get_mlp(self.ctx, _p(p, ctypes.c_float)), "gala_get_mlp")
+        return p
+
+    def mlp_train(self, X, y, epochs=20, batch=4096, lr=1e-2, pos_weight=1.0, seed=0, init=None):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.uint8)
+        p = np.zeros(MLP_NPARAM, np.float32) if init is None else np.ascontiguousarray(init, np.float32).copy()
+        loss = np.zeros(epochs, np.float32)
+        _check(lib().gala_mlp_train(self.ctx, _p(X, ctypes.c_float), _p(y, ctypes.c_uint8), len(y), epochs, batch,
+                                    lr, pos_weight, seed, int(init is not None), _p(p, ctypes.c_float),
+                                    _p(loss, ctypes.c_float)), "gala_mlp_train")
+        return p, loss
+
+    def mlp_forward(self, params, X):
+        X = np.ascontiguousarray(X, np.float32)
+        p = np.ascontiguousarray(params, np.float32)
+        out = np.zeros(len(X), np.float32)
+        _check(lib().gala_mlp_forward(self.ctx, _p(p, ctypes.c_float), _p(X, ctypes.c_float), len(X),
+                                      _p(out, ctypes.c_float)), "gala_mlp_forward")
+        return out
+
+    def mlp_grad(self, params, X, y, pos_weight=1.0):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.uint8)
+        p = np.ascontiguousarray(params, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(lib().gala_mlp_grad(self.ctx, _p(p, ctypes.c_float), _p(X, ctypes.c_float), _p(y, ctypes.c_uint8),
+                                   len(y), pos_weight, _p(g, ctypes.c_float), _p(loss, ctypes.c_float)),
+               "gala_mlp_grad")
+        return g, float(loss[0])
 
     def communities(self, n):
         out = np.zeros(max(n, 1), np.int32)

@@ -91,6 +91,8 @@ typedef struct {
   double round_decide_ms[64];
   double round_update_ms[64];
   double round_build_ms[64];  /* contraction of the round */
+  int32_t learned_sweeps;      /* sweeps run on the learned frontier */
+  int32_t certify_sweeps;      /* plain sweeps run to certify a stalled learned level */
   int64_t bin_decide_bytes[9]; /* decide-and-move algorithmic bytes per degree bin */
   int64_t bin_swept_edges[9];
   double bin_decide_ms[9];     /* per-bin decide time (profile_bins=1), HIP events */
@@ -150,6 +152,35 @@ int gala_phase_weight(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const i
 int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr,
                         const int32_t* w, const int32_t* comm, int32_t* out_C, int64_t* out_m,
                         int32_t* new_id, int64_t* out_end_incl, int32_t* out_nbr, int64_t* out_w);
+
+/* ---- learned frontier (no counterpart in the reference) ----
+ * A 6 -> 64 -> 1 MLP (BF16 MFMA, FP32 accumulation, FP32 master weights)
+ * predicts which vertices the next sweep moves; with opts.frontier_mode = 1 a
+ * sweep only visits the predicted movers among the prune-active vertices, and
+ * a full plain sweep certifies every level before it is accepted.  Parameter
+ * vector: GALA_MLP_NPARAM floats, W1[16][64] (rows 6..15 zero), b1[64],
+ * w2[64], b2.  Features (6 per vertex) are defined in gala_kernels.h. */
+#define GALA_MLP_NFEAT 6
+#define GALA_MLP_NPARAM 1153
+
+/* Move logs of a plain run: before every sweep after the first, the features
+ * of the swept vertices and, after it, whether each one moved. */
+typedef void (*gala_log_cb)(void* user, int round, int iteration, int32_t n, const float* features,
+                            const uint8_t* moved);
+int gala_set_logger(gala_ctx* ctx, gala_log_cb cb, void* user);
+int gala_set_mlp(gala_ctx* ctx, const float* params);
+int gala_get_mlp(gala_ctx* ctx, float* params);
+/* Train on (X[n][6], y[n]) with the device's own forward/backward kernels and
+ * Adam; params_io holds the initial weights when init_given, else they are
+ * drawn from seed; receives the trained weights, which the context keeps.
+ * loss_out[epochs] (optional) receives the mean weighted BCE per epoch. */
+int gala_mlp_train(gala_ctx* ctx, const float* X, const uint8_t* y, int64_t n, int epochs, int batch, float lr,
+                   float pos_weight, uint64_t seed, int init_given, float* params_io, float* loss_out);
+/* Move probabilities for X[n][6] under params. */
+int gala_mlp_forward(gala_ctx* ctx, const float* params, const float* X, int64_t n, float* p_out);
+/* Gradient of the weighted BCE over (X, y) at params (no update). */
+int gala_mlp_grad(gala_ctx* ctx, const float* params, const float* X, const uint8_t* y, int64_t n, float pos_weight,
+                  float* grad_out, float* loss_out);
 
 /* ---- .bin IO and generators (host only; no device needed) ---- */
 
