@@ -14,6 +14,9 @@ Also reported:
   cpu_baseline  the oracle (exact-integer C port of the same algorithm, OpenMP)
                 on the same graph on the host, 1 run
   parity        final partition of the last GPU step == oracle's, bit for bit
+  learned       the learned-frontier form on the same graph (network trained on
+                SBM move logs, gala-mg_amd/models/frontier_mlp.json): time per
+                run, speedup over the plain path and final-Q difference
 """
 import argparse
 import json
@@ -30,6 +33,33 @@ BIN_DEG = ["1-3", "4-7", "8-15", "16-31", "32-127", "128-511", "512-4095", "4096
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def learned_leg(gala, g, args, plain):
+    """Same graph, learned frontier: K timed runs after one warmup."""
+    import numpy as np
+    import train_frontier as TF
+    if not os.path.exists(TF.DEFAULT_MODEL):
+        log("learned leg skipped: no trained model")
+        return None
+    params, tau = TF.load(TF.DEFAULT_MODEL)
+    eng = gala.Engine(device=0, frontier_mode=1, learned_tau=tau)
+    eng.set_mlp(params)
+    eng.set_graph(g)
+    eng.run()
+    runs = [eng.run() for _ in range(args.steps)]
+    eng.close()
+    r = runs[-1]
+    ms = float(np.mean([x["t_total_ms"] for x in runs]))
+    plain_ms = plain["t_total_ms"]
+    out = {"ms_per_step": ms, "speedup_vs_plain": plain_ms / ms, "plain_ms": plain_ms,
+           "modularity": r["modularity"], "dQ": r["modularity"] - plain["modularity"], "tau": tau,
+           "learned_sweeps": r["learned_sweeps"], "certify_sweeps": r["certify_sweeps"],
+           "kept_fraction": (r["learned_kept"] / r["learned_frontier"]) if r["learned_frontier"] else None,
+           "swept_edges": r["swept_edges"], "plain_swept_edges": plain["swept_edges"],
+           "phase_ms": {"decide": r["t_decide_ms"], "update": r["t_update_ms"], "contract": r["t_contract_ms"]}}
+    log(f"learned: {ms:.2f} ms (plain {plain_ms:.2f}) Q={r['modularity']:.6f} dQ={out['dQ']:+.6f}")
+    return out
 
 
 def main():
@@ -77,6 +107,7 @@ def main():
     wall = time.perf_counter() - t1
     comm = eng.communities(g.vertex_num)
     eng.close()
+    learned = learned_leg(gala, g, args, runs[-1])
     # one more run with per-bin HIP events (outside the timed steps)
     peng = gala.Engine(device=0, profile_bins=True)
     peng.set_graph(g)
@@ -149,6 +180,8 @@ def main():
                           "phase1_ms": last["round_ms"], "decide_ms": last["round_decide_ms"],
                           "update_ms": last["round_update_ms"], "contract_ms": last["round_build_ms"]},
         "parity_vs_oracle": parity,
+        "learned": learned,
+        "dQ_learned_vs_plain": learned["dQ"] if learned else None,
     }
     print(json.dumps(out))
 

@@ -109,8 +109,12 @@ struct gala_ctx {
   float* last_gain = nullptr;
   int32_t* moved_it = nullptr;
   float* nbr_moved = nullptr;
-  int32_t* front2 = nullptr;
+  int32_t* front2 = nullptr;   // predicted movers of a learned sweep, then its complement
   int32_t* bstart2 = nullptr;
+  int32_t* front3 = nullptr;   // second frontier buffer (the pre-sweep frontier survives a sweep)
+  int32_t* bstart3 = nullptr;
+  void* tot_snap = nullptr;    // Tot and size before a learned sweep
+  int32_t* size_snap = nullptr;
   uint8_t* keep = nullptr;
   float* feat = nullptr;        // logged features of the frontier
   uint8_t* lab = nullptr;
@@ -186,6 +190,8 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->order, N); OWN(ctx->front, N); OWN(ctx->fin, N);
   OWN(ctx->last_gain, N); OWN(ctx->moved_it, N); OWN(ctx->nbr_moved, N);
   OWN(ctx->front2, N); OWN(ctx->bstart2, NBINS + 1); OWN(ctx->keep, N);
+  OWN(ctx->front3, N); OWN(ctx->bstart3, NBINS + 1); OWN(ctx->size_snap, N);
+  { char* p; OWN(p, N * tsz); ctx->tot_snap = p; }
   OWN(ctx->feat, N * MLP_IN); OWN(ctx->lab, N);
   OWN(ctx->bstart_full, NBINS + 1); OWN(ctx->bstart_front, NBINS + 1);
   OWN(ctx->hub_off, N + 1); OWN(ctx->hub_tmp, 2 * (N + 1));
@@ -462,45 +468,93 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     }
     icb(user, round, 0, q, 0, vec, g.n);
   }
-  // first frontier: every vertex (fill(active,1), louvain_main_process.cu:574)
-  copy_i32(ctx->order, ctx->front, nitems, s);
-  copy_i32(ctx->bstart_full, ctx->bstart_front, NBINS + 1, s);
+  // first frontier: every vertex (fill(active,1), louvain_main_process.cu:574).
+  // Two frontier buffers alternate so that the frontier a sweep started from
+  // is still at hand when a stalled learned sweep must be certified.
+  int32_t* fcur = ctx->front;
+  int32_t* bcur = ctx->bstart_front;
+  int32_t* fnext = ctx->front3;
+  int32_t* bnext = ctx->bstart3;
+  copy_i32(ctx->order, fcur, nitems, s);
+  copy_i32(ctx->bstart_full, bcur, NBINS + 1, s);
   int32_t nfront = nitems;  // vertices in the plain frontier
   const int max_it = ctx->opts.max_iterations > 0 ? ctx->opts.max_iterations : 100000;
   const float tau = ctx->opts.learned_tau > 0 ? ctx->opts.learned_tau : 0.5f;
+  const bool prof = ctx->opts.profile_bins != 0;
+  const size_t tsz = sizeof(tt);
   double t_dm = 0, t_upd = 0;
-  bool certify = false;
   std::vector<float> hfeat;
   std::vector<uint8_t> hlab;
+  // decide-and-move over one frontier, timed into t_dm (and per bin)
+  auto decide = [&](DevState<T>& sd, const int32_t* fr, const int32_t* bs, bool first) -> int {
+    HIPOK(hipEventRecord(ctx->ev[0], s));
+    for (int b = 0; b < NBINS; b++) {
+      if (prof) HIPOK(hipEventRecord(ctx->bev[b], s));
+      if (first) launch_decide_first<T>(b, g, sd, fr, bs, caps[b], m2, s);
+      else launch_decide<T>(b, g, sd, fr, bs, caps[b], m2, s);
+    }
+    if (prof) HIPOK(hipEventRecord(ctx->bev[NBINS], s));
+    HIPOK(hipEventRecord(ctx->ev[1], s));
+    return GALA_OK;
+  };
+  // Tot/size update, In of the invalidated rows, prune, next frontier and the
+  // counters; accumulates the sweep's work and timings
+  auto update = [&](DevState<T>& sp, bool first, Counters* out) -> int {
+    launch_apply<T>(st, g.n, m2, s);
+    // In changes only where a vertex or one of its neighbours moved
+    for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s);
+    launch_prune<T>(sp, g.n, m2, s);
+    partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, fnext, bnext, ctx->part_tmp, s);
+    HIPOK(hipEventRecord(ctx->ev[2], s));
+    int r = read_counters<T>(ctx, out);
+    if (r) return r;
+    float a = 0, b2 = 0;
+    (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
+    (void)hipEventElapsedTime(&b2, ctx->ev[1], ctx->ev[2]);
+    t_dm += a;
+    t_upd += b2;
+    ctx->res.swept_vertices += (int64_t)out->swept;
+    ctx->res.swept_edges += (int64_t)out->swept_edges;
+    if (!g.w) {
+      ctx->res.swept_vertices_unit += (int64_t)out->swept;
+      ctx->res.swept_edges_unit += (int64_t)out->swept_edges;
+    }
+    const int64_t per_edge = (first ? 12 : 16) - (g.w ? 0 : 4);
+    ctx->res.decide_bytes += per_edge * (int64_t)out->swept_edges + 40 * (int64_t)out->swept;
+    for (int b = 0; b < NBINS; b++) {
+      ctx->res.bin_decide_bytes[b] += per_edge * (int64_t)out->bin_edges[b] + 40 * (int64_t)out->bin_swept[b];
+      ctx->res.bin_swept_edges[b] += (int64_t)out->bin_edges[b];
+      if (prof) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ctx->bev[b], ctx->bev[b + 1]);
+        ctx->res.bin_decide_ms[b] += ms;
+      }
+    }
+    std::swap(fcur, fnext);
+    std::swap(bcur, bnext);
+    return GALA_OK;
+  };
   int it = 1;
   for (;; it++) {
     st.iter = it;
     reset_counters(ctx->ctr, (long long)m2, m2d, s);
-    // a learned sweep visits only the predicted movers of the plain frontier;
-    // the first sweep of a round and certification sweeps are plain
-    const bool lsweep = learned && it >= 2 && !certify;
-    const int32_t* sfront = ctx->front;
-    const int32_t* sbstart = ctx->bstart_front;
+    // A learned sweep visits only the predicted movers P of the plain frontier
+    // F.  The first sweep of a round is plain.
+    const bool lsweep = learned && it >= 2;
+    const bool first = it == 1 && unique_rows;  // singleton partition: no aggregation needed
+    const int32_t nf0 = nfront;
     if (lsweep) {
-      launch_mlp_filter<T>(g, st, ctx->front, nfront, it, ctx->mlp_prm, ctx->mlp_w1b, tau, ctx->keep, s);
-      partition_bins(1, ctx->front, ctx->binid, ctx->keep, nfront, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
-      sfront = ctx->front2;
-      sbstart = ctx->bstart2;
+      launch_mlp_filter<T>(g, st, fcur, nfront, it, ctx->mlp_prm, ctx->mlp_w1b, tau, ctx->keep, s);
+      partition_bins(1, fcur, ctx->binid, ctx->keep, nfront, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
+      // pre-sweep Tot and size, read again if the sweep has to be certified
+      HIPOK(hipMemcpyAsync(ctx->tot_snap, st.Tot, tsz * g.n, hipMemcpyDeviceToDevice, s));
+      HIPOK(hipMemcpyAsync(ctx->size_snap, st.size, sizeof(int32_t) * g.n, hipMemcpyDeviceToDevice, s));
     }
     const bool log_it = logging && !lsweep && it >= 2;
-    if (log_it) launch_features<T>(g, st, ctx->front, nfront, it, ctx->feat, s);
-    HIPOK(hipEventRecord(ctx->ev[0], s));
-    const bool first = it == 1 && unique_rows;  // singleton partition: no aggregation needed
-    const bool prof = ctx->opts.profile_bins != 0;
-    for (int b = 0; b < NBINS; b++) {
-      if (prof) HIPOK(hipEventRecord(ctx->bev[b], s));
-      if (first) launch_decide_first<T>(b, g, st, sfront, sbstart, caps[b], m2, s);
-      else launch_decide<T>(b, g, st, sfront, sbstart, caps[b], m2, s);
-    }
-    if (prof) HIPOK(hipEventRecord(ctx->bev[NBINS], s));
-    HIPOK(hipEventRecord(ctx->ev[1], s));
+    if (log_it) launch_features<T>(g, st, fcur, nfront, it, ctx->feat, s);
+    if ((rc = decide(st, lsweep ? ctx->front2 : fcur, lsweep ? ctx->bstart2 : bcur, first))) return rc;
     if (log_it) {
-      launch_labels(ctx->front, nfront, ctx->moved_it, it, ctx->lab, s);
+      launch_labels(fcur, nfront, ctx->moved_it, it, ctx->lab, s);
       hfeat.resize((size_t)nfront * MLP_IN);
       hlab.resize(nfront);
       HIPOK(hipMemcpyAsync(hfeat.data(), ctx->feat, sizeof(float) * hfeat.size(), hipMemcpyDeviceToHost, s));
@@ -508,42 +562,35 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       HIPOK(hipStreamSynchronize(s));
       ctx->logger(ctx->logger_user, round, it, nfront, hfeat.data(), hlab.data());
     }
-    launch_apply<T>(st, g.n, m2, s);
-    // In changes only where a vertex or one of its neighbours moved
-    for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s);
-    launch_prune<T>(st, g.n, m2, s);
-    partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front,
-                   ctx->part_tmp, s);
-    HIPOK(hipEventRecord(ctx->ev[2], s));
-    if ((rc = read_counters<T>(ctx, &c))) return rc;
+    if ((rc = update(st, first, &c))) return rc;
     nfront = (int32_t)c.active;
-    float a = 0, b2 = 0;
-    (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
-    (void)hipEventElapsedTime(&b2, ctx->ev[1], ctx->ev[2]);
-    t_dm += a;
-    t_upd += b2;
-    ctx->res.swept_vertices += (int64_t)c.swept;
-    ctx->res.swept_edges += (int64_t)c.swept_edges;
-    if (!g.w) {
-      ctx->res.swept_vertices_unit += (int64_t)c.swept;
-      ctx->res.swept_edges_unit += (int64_t)c.swept_edges;
-    }
-    {
-      const int64_t per_edge = (first ? 12 : 16) - (g.w ? 0 : 4);
-      ctx->res.decide_bytes += per_edge * (int64_t)c.swept_edges + 40 * (int64_t)c.swept;
-      for (int b = 0; b < NBINS; b++) {
-        ctx->res.bin_decide_bytes[b] += per_edge * (int64_t)c.bin_edges[b] + 40 * (int64_t)c.bin_swept[b];
-        ctx->res.bin_swept_edges[b] += (int64_t)c.bin_edges[b];
-        if (prof) {
-          float ms = 0;
-          (void)hipEventElapsedTime(&ms, ctx->bev[b], ctx->bev[b + 1]);
-          ctx->res.bin_decide_ms[b] += ms;
-        }
-      }
-    }
-    if (lsweep) ctx->res.learned_sweeps++;
-    if (certify) ctx->res.certify_sweeps++;
+    int64_t moved = (int64_t)c.moved;
     q = q_from<T>(c, m2d);
+    if (lsweep) {
+      ctx->res.learned_sweeps++;
+      ctx->res.learned_frontier += nf0;
+      ctx->res.learned_kept += (int64_t)c.swept;
+    }
+    if (lsweep && q - qprev < ctx->opts.min_modularity) {
+      // Certification.  The sweep is synchronous, so the plain sweep from the
+      // same state is the learned one plus the decisions of F \ P, taken
+      // against the pre-sweep Tot and size (prev is still the pre-sweep
+      // assignment).  Their moves land on the updated state; rows none of
+      // them touches keep the In of the learned sweep, which is exact.
+      ctx->res.certify_sweeps++;
+      reset_counters(ctx->ctr, (long long)m2, m2d, s);
+      partition_bins(2, fnext, ctx->binid, ctx->keep, nf0, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
+      DevState<T> sc = st;
+      sc.Tot = (tt*)ctx->tot_snap;
+      sc.size = ctx->size_snap;
+      if ((rc = decide(sc, ctx->front2, ctx->bstart2, false))) return rc;
+      DevState<T> sp = st;
+      sp.track = 0;  // keep the neighbour-moved fractions of the learned sweep's rows
+      if ((rc = update(sp, false, &c))) return rc;
+      nfront = (int32_t)c.active;
+      moved += (int64_t)c.moved;
+      q = q_from<T>(c, m2d);
+    }
     if (icb) {
       const int32_t* vec = nullptr;
       if (ctx->opts.dump_iterations) {
@@ -551,17 +598,10 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
         HIPOK(hipMemcpy(hostvec.data(), ctx->cur, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost));
         vec = hostvec.data();
       }
-      icb(user, round, it, q, (int64_t)c.moved, vec, g.n);
+      icb(user, round, it, q, moved, vec, g.n);
     }
-    if (lsweep) {
-      // a stalled learned sweep is kept and the level goes to certification:
-      // the next sweep is a full plain sweep over every prune-active vertex
-      if (q - qprev < ctx->opts.min_modularity) certify = true;
-      if (it >= max_it) break;
-    } else {
-      if (q - qprev < ctx->opts.min_modularity || it >= max_it) break;  // plain rule, with revert
-      certify = false;
-    }
+    // plain rule, with revert (louvain_main_process.cu:1004-1007,1026-1031)
+    if (q - qprev < ctx->opts.min_modularity || it >= max_it) break;
     copy_i32(ctx->cur, ctx->prev, g.n, s);  // prev = cur (louvain_main_process.cu:1029)
     qprev = q;
   }

@@ -12,6 +12,9 @@ template <class In, class V>
 void scan_exclusive(const In* in, V* out, int64_t n, V* tmp, V* total, hipStream_t s);
 
 int64_t part_tmp_elems(int64_t n);
+// Stable partition into degree bins.  mode 0: every vertex 0..n-1 by binid;
+// mode 1: the vertices of order[0..n) with active[v] != 0; mode 2: those with
+// active[v] == 0.  bstart receives NBINS+1 bin starts.
 void partition_bins(int mode, const int32_t* order, const int8_t* binid, const uint8_t* active,
                     int64_t n, int32_t* out, int32_t* bstart, int32_t* tmp, hipStream_t s);
 

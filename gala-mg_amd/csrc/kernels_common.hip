@@ -95,8 +95,9 @@ __device__ __forceinline__ void part_item(int mode, const int32_t* order, const 
     v = (int)i;
     key = binid[i];
   } else {
+    // mode 1 keeps the flagged vertices of `order`, mode 2 the unflagged ones
     v = order[i];
-    key = active[v] ? (int)binid[v] : -1;
+    key = (active[v] != 0) == (mode == 1) ? (int)binid[v] : -1;
   }
 }
 

@@ -58,7 +58,8 @@ class Result(ctypes.Structure):
                 ("round_update_ms", ctypes.c_double * 64), ("round_build_ms", ctypes.c_double * 64),
                 ("learned_sweeps", ctypes.c_int32), ("certify_sweeps", ctypes.c_int32),
                 ("bin_decide_bytes", ctypes.c_int64 * 9), ("bin_swept_edges", ctypes.c_int64 * 9),
-                ("bin_decide_ms", ctypes.c_double * 9)]
+                ("bin_decide_ms", ctypes.c_double * 9),
+                ("learned_frontier", ctypes.c_int64), ("learned_kept", ctypes.c_int64)]
 
     def to_dict(self):
         r = self.rounds
@@ -78,7 +79,8 @@ class Result(ctypes.Structure):
                     round_build_ms=list(self.round_build_ms[:r]),
                     learned_sweeps=self.learned_sweeps, certify_sweeps=self.certify_sweeps,
                     bin_decide_bytes=list(self.bin_decide_bytes), bin_swept_edges=list(self.bin_swept_edges),
-                    bin_decide_ms=list(self.bin_decide_ms))
+                    bin_decide_ms=list(self.bin_decide_ms),
+                    learned_frontier=self.learned_frontier, learned_kept=self.learned_kept)
 
 
 ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,

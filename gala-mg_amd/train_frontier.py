@@ -46,15 +46,15 @@ def sbm_training_graphs(seeds):
     return [gala.gen_sbm(1024, 16, 0.10, 0.002, s) for s in seeds]
 
 
-def train(engine, X, y, epochs=40, batch=4096, lr=5e-3, seed=0):
-    """Class-balanced training; returns (params, per-epoch loss, threshold)."""
+def train(engine, X, y, epochs=40, batch=4096, lr=5e-3, seed=0, recall=0.98):
+    """Class-balanced training; returns (params, per-epoch loss, threshold).
+    The threshold keeps the given fraction of the movers of the training logs."""
     pos = max(int(y.sum()), 1)
     pos_w = float(len(y) - pos) / pos
     p, loss = engine.mlp_train(X, y, epochs=epochs, batch=batch, lr=lr, pos_weight=pos_w, seed=seed)
-    # threshold: keep recall of movers >= 0.98 on the training logs
     pr = engine.mlp_forward(p, X)
     movers = np.sort(pr[y > 0])
-    tau = float(movers[int(0.02 * len(movers))]) if len(movers) else 0.5
+    tau = float(movers[int((1.0 - recall) * len(movers))]) if len(movers) else 0.5
     return p, loss, tau
 
 
@@ -74,13 +74,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--seeds", type=int, default=32)
     ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--recall", type=float, default=0.98)
     ap.add_argument("--out", default=DEFAULT_MODEL)
     args = ap.parse_args()
     eng = gala.Engine()
     seeds = list(range(1, args.seeds + 1))
     X, y = collect_logs(eng, sbm_training_graphs(seeds))
-    p, loss, tau = train(eng, X, y, epochs=args.epochs)
+    p, loss, tau = train(eng, X, y, epochs=args.epochs, recall=args.recall)
     save(args.out, p, tau, {"train": f"SBM n=1024 blocks=16 p_in=0.10 p_out=0.002 seeds 1..{args.seeds}",
+                            "recall_target": args.recall, "epochs": args.epochs,
                             "samples": int(len(y)), "movers": int(y.sum()),
                             "loss_first": float(loss[0]), "loss_last": float(loss[-1])})
     print(f"samples={len(y)} movers={int(y.sum())} loss {loss[0]:.4f} -> {loss[-1]:.4f} tau={tau:.4f} -> {args.out}")

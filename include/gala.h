@@ -92,10 +92,12 @@ typedef struct {
   double round_update_ms[64];
   double round_build_ms[64];  /* contraction of the round */
   int32_t learned_sweeps;      /* sweeps run on the learned frontier */
-  int32_t certify_sweeps;      /* plain sweeps run to certify a stalled learned level */
+  int32_t certify_sweeps;      /* stalled learned sweeps certified by sweeping the rest of the frontier */
   int64_t bin_decide_bytes[9]; /* decide-and-move algorithmic bytes per degree bin */
   int64_t bin_swept_edges[9];
   double bin_decide_ms[9];     /* per-bin decide time (profile_bins=1), HIP events */
+  int64_t learned_frontier;    /* plain-frontier vertices at the learned sweeps */
+  int64_t learned_kept;        /* of those, vertices the network kept (swept) */
 } gala_result;
 
 /* Version string of the library. */
