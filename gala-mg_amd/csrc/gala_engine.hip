@@ -124,7 +124,28 @@ struct gala_ctx {
   gala_log_cb logger = nullptr;
   void* logger_user = nullptr;
 
+  // batched graphs (gala_batch_*): union layout and per-graph state
+  int32_t b_G = 0;
+  std::vector<int32_t> b_vstart;       // host copy of vstart[G+1]
+  int32_t* b_gid[3] = {nullptr, nullptr, nullptr};  // [0] input vertices, [1]/[2] round graphs
+  int32_t* b_m2g = nullptr;
+  uint8_t *b_live = nullptr, *b_accept = nullptr, *b_done = nullptr;
+  double *b_qprev = nullptr, *b_cur_mod = nullptr;
+  int32_t *b_iters = nullptr, *b_tot_iters = nullptr, *b_rounds = nullptr, *b_ncomm = nullptr;
+  int32_t* b_scalars = nullptr;        // [0] nlive, [1] nopen, [2] iteration counter
+  unsigned long long *b_gin = nullptr, *b_gtot2 = nullptr;
+  int32_t* b_host = nullptr;           // pinned readback of the scalars
+  int32_t b_m2max = 0;
+  std::vector<void*> bowned;
+  bool have_batch = false, have_batch_result = false;
+  std::vector<int32_t> b_comm_local;   // per-vertex community within its graph
+  std::vector<double> b_q;
+  std::vector<int32_t> b_nc, b_rd, b_it;
+
   std::vector<void*> owned;
+  int32_t cap_n = -1;  // sizes the owned buffers were allocated for
+  int64_t cap_m = -1;
+  bool cap_fp = false;
   gala_result res{};
   bool have_result = false;
   std::vector<int32_t> comm_host;
@@ -137,9 +158,20 @@ namespace {
 
 bool is_fp(const gala_ctx* ctx) { return ctx->wmode == GALA_W_F32; }
 
+void free_batch(gala_ctx* ctx) {
+  for (void* p : ctx->bowned) (void)hipFree(p);
+  ctx->bowned.clear();
+  ctx->have_batch = false;
+  ctx->have_batch_result = false;
+  ctx->b_G = 0;
+}
+
 void free_all(gala_ctx* ctx) {
   for (void* p : ctx->owned) (void)hipFree(p);
   ctx->owned.clear();
+  ctx->cap_n = -1;
+  ctx->cap_m = -1;
+  free_batch(ctx);
   if (ctx->hub_keys) (void)hipFree(ctx->hub_keys);
   if (ctx->hub_vals) (void)hipFree(ctx->hub_vals);
   ctx->hub_keys = nullptr; ctx->hub_vals = nullptr; ctx->hub_cap = 0;
@@ -170,8 +202,11 @@ int own(gala_ctx* ctx, P** p, int64_t elems) {
     if (rc_) { free_all(ctx); return rc_; }         \
   } while (0)
 
-// Allocate every device buffer for an input of n vertices / m entries.
+// Allocate every device buffer for an input of n vertices / m entries (kept
+// when the buffers of the previous graph are large enough).
 int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
+  free_batch(ctx);
+  if (!ctx->owned.empty() && n <= ctx->cap_n && m <= ctx->cap_m && fp == ctx->cap_fp) return GALA_OK;
   free_all(ctx);
   const int64_t N = std::max<int32_t>(n, 1), M = std::max<int64_t>(m, 1);
   const size_t tsz = fp ? 8 : 4;  // tot_t / w_t / acc_t size
@@ -211,6 +246,9 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
     hipError_t e = hipHostMalloc((void**)&ctx->ctr_h, sizeof(Counters), hipHostMallocDefault);
     if (e != hipSuccess) { ctx->last_hip = e; free_all(ctx); return GALA_E_OOM; }
   }
+  ctx->cap_n = n;
+  ctx->cap_m = m;
+  ctx->cap_fp = fp;
   return GALA_OK;
 }
 
@@ -838,6 +876,265 @@ int upload(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr
   return GALA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// batched graphs
+// ---------------------------------------------------------------------------
+template <class P>
+int bown(gala_ctx* ctx, P** p, int64_t elems) {
+  hipError_t e = dalloc(p, std::max<int64_t>(elems, 1));
+  if (e != hipSuccess) { ctx->last_hip = e; return GALA_E_OOM; }
+  ctx->bowned.push_back((void*)*p);
+  return GALA_OK;
+}
+#define BOWN(p, n)                                  \
+  do {                                              \
+    int rc_ = bown(ctx, &(p), (n));                 \
+    if (rc_) { free_batch(ctx); return rc_; }       \
+  } while (0)
+
+BatchDev bdev(gala_ctx* ctx) {
+  BatchDev b;
+  b.G = ctx->b_G;
+  b.m2g = ctx->b_m2g;
+  b.live = ctx->b_live; b.accept = ctx->b_accept; b.done = ctx->b_done;
+  b.qprev = ctx->b_qprev; b.cur_mod = ctx->b_cur_mod;
+  b.iters = ctx->b_iters; b.tot_iters = ctx->b_tot_iters; b.rounds = ctx->b_rounds;
+  b.g_in = ctx->b_gin; b.g_tot2 = ctx->b_gtot2;
+  b.nlive = ctx->b_scalars;
+  return b;
+}
+
+// Union upload plus per-graph m2 and the graph of every vertex.
+int batch_upload(gala_ctx* ctx, int32_t G, const int32_t* vstart, int32_t n, const int64_t* end_incl,
+                 const int32_t* nbr, const void* w, int weight_mode) {
+  if (G <= 0 || !vstart || weight_mode == GALA_W_F32 || weight_mode < 0 || weight_mode > 1) return GALA_E_INVALID;
+  if (vstart[0] != 0 || vstart[G] != n) return GALA_E_INVALID;
+  for (int32_t g = 0; g < G; g++)
+    if (vstart[g + 1] < vstart[g]) return GALA_E_INVALID;
+  if (n <= 0) return GALA_E_INVALID;
+  // rows stay inside their graph; per-graph m2 (exact, integer)
+  std::vector<int32_t> m2g(G), gid0(n);
+  int32_t m2max = 0;
+  for (int32_t g = 0; g < G; g++) {
+    int64_t m2 = 0;
+    for (int32_t v = vstart[g]; v < vstart[g + 1]; v++) {
+      gid0[v] = g;
+      const int64_t a = v == 0 ? 0 : end_incl[v - 1];
+      for (int64_t j = a; j < end_incl[v]; j++) {
+        if (nbr[j] < vstart[g] || nbr[j] >= vstart[g + 1]) return GALA_E_FORMAT;
+        m2 += weight_mode == GALA_W_I32 ? ((const int32_t*)w)[j] : 1;
+      }
+    }
+    if (m2 > M2_LIMIT) return GALA_E_OVERFLOW;
+    m2g[g] = (int32_t)m2;
+    m2max = std::max(m2max, (int32_t)m2);
+  }
+  int rc = upload(ctx, n, end_incl, nbr, w, weight_mode);
+  if (rc) return rc;
+  const int64_t N = n;
+  for (int k = 0; k < 3; k++) BOWN(ctx->b_gid[k], N);
+  BOWN(ctx->b_m2g, G); BOWN(ctx->b_live, G); BOWN(ctx->b_accept, G); BOWN(ctx->b_done, G);
+  BOWN(ctx->b_qprev, G); BOWN(ctx->b_cur_mod, G); BOWN(ctx->b_iters, G); BOWN(ctx->b_tot_iters, G);
+  BOWN(ctx->b_rounds, G); BOWN(ctx->b_ncomm, G); BOWN(ctx->b_scalars, 4); BOWN(ctx->b_gin, G);
+  BOWN(ctx->b_gtot2, G);
+  if (!ctx->b_host) {
+    hipError_t e = hipHostMalloc((void**)&ctx->b_host, 4 * sizeof(int32_t), hipHostMallocDefault);
+    if (e != hipSuccess) { ctx->last_hip = e; free_batch(ctx); return GALA_E_OOM; }
+  }
+  hipStream_t s = ctx->stream;
+  HIPOK(hipMemcpyAsync(ctx->b_gid[0], gid0.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ctx->b_m2g, m2g.data(), sizeof(int32_t) * G, hipMemcpyHostToDevice, s));
+  HIPOK(hipStreamSynchronize(s));
+  ctx->b_G = G;
+  ctx->b_vstart.assign(vstart, vstart + G + 1);
+  ctx->b_m2max = m2max;
+  ctx->have_batch = true;
+  ctx->have_batch_result = false;
+  return GALA_OK;
+}
+
+// All graphs of the batch, each with the semantics of its own run_all: the
+// union's sweeps, updates and contractions are shared, every per-graph
+// decision (iteration break with revert, round stop) is taken on the device
+// from that graph's own exact sums.
+int run_batch(gala_ctx* ctx, gala_batch_result* out) {
+  using T = IntW;
+  hipStream_t s = ctx->stream;
+  gala_batch_result R{};
+  R.n_graphs = ctx->b_G;
+  const int32_t G = ctx->b_G;
+  const int32_t m2max = std::max(ctx->b_m2max, 1);
+  const double min_mod = ctx->opts.min_modularity;
+  const int max_it = ctx->opts.max_iterations > 0 ? ctx->opts.max_iterations : 100000;
+  const int max_rounds = ctx->opts.max_rounds > 0 ? std::min(ctx->opts.max_rounds, 64) : 64;
+  const bool graphs = ctx->opts.no_hip_graphs == 0;
+  if (ctx->m0 == 0) {  // no graph has an edge: every vertex stays alone, Q = 0
+    ctx->b_q.assign(G, 0.0);
+    ctx->b_rd.assign(G, 0);
+    ctx->b_it.assign(G, 0);
+    ctx->b_nc.resize(G);
+    ctx->b_comm_local.resize(ctx->n0);
+    for (int32_t gg = 0; gg < G; gg++) {
+      ctx->b_nc[gg] = ctx->b_vstart[gg + 1] - ctx->b_vstart[gg];
+      for (int32_t v = ctx->b_vstart[gg]; v < ctx->b_vstart[gg + 1]; v++)
+        ctx->b_comm_local[v] = v - ctx->b_vstart[gg];
+    }
+    ctx->have_batch_result = true;
+    if (out) *out = R;
+    return GALA_OK;
+  }
+  BatchDev b = bdev(ctx);
+  int32_t* nlive = ctx->b_scalars;
+  int32_t* nopen = ctx->b_scalars + 1;
+  int32_t* it_dev = ctx->b_scalars + 2;
+  int src = 0;
+  DevGraph<T> g = graph_at<T>(ctx, 0, ctx->n0, ctx->m0);
+  fill<uint8_t>(ctx->b_done, G, 0, s);
+  fill<int32_t>(ctx->b_tot_iters, G, 0, s);
+  fill<int32_t>(ctx->b_rounds, G, 0, s);
+  fill<double>(ctx->b_cur_mod, G, 0.0, s);
+  copy_i32(ctx->b_gid[0], ctx->b_gid[1], ctx->n0, s);
+  reset_counters(ctx->ctr, m2max, (double)m2max, s);  // the decide totals start at zero
+  HIPOK(hipEventRecord(ctx->ev[4], s));
+  k_iota<<<std::min((ctx->n0 + 255) / 256, 8192), 256, 0, s>>>(ctx->fin, ctx->n0);
+  int gcur = 1;  // b_gid index of the round graph
+  int round = 0;
+  for (;;) {
+    HIPOK(hipEventRecord(ctx->ev[5], s));
+    int32_t* gid = ctx->b_gid[gcur];
+    int32_t caps[NBINS], nitems = 0;
+    int rc = round_setup<T>(ctx, g, caps, &nitems, 0);
+    if (rc) return rc;
+    DevState<T> st = dstate<T>(ctx);
+    st.gid = gid;
+    st.m2g = ctx->b_m2g;
+    launch_batch_live(b, it_dev, s);
+    // iteration 0: the singleton partition's Q of every running graph
+    launch_batch_reset(ctx->ctr, m2max, nlive, s);
+    launch_apply<T>(st, g.n, m2max, s);
+    for (int k = 0; k < NBINS; k++) launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], false, s);
+    launch_prune<T>(st, g.n, m2max, s);
+    launch_graph_sums(gid, ctx->binid, (const int32_t*)ctx->In, (const int32_t*)ctx->Tot, g.n, b.g_in, b.g_tot2, s);
+    launch_batch_step(b, it_dev, min_mod, max_it, s);
+    launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, true, it_dev, s);
+    partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp, s);
+    const bool unique_rows = src != 0 || ctx->unique0;
+    // one iteration for every running graph; a graph that stopped is inert
+    auto body = [&](bool first) {
+      launch_batch_reset(ctx->ctr, m2max, nlive, s);
+      for (int k = 0; k < NBINS; k++) {
+        if (first) launch_decide_first<T>(k, g, st, ctx->front, ctx->bstart_front, caps[k], m2max, s);
+        else launch_decide<T>(k, g, st, ctx->front, ctx->bstart_front, caps[k], m2max, s);
+      }
+      launch_apply<T>(st, g.n, m2max, s);
+      for (int k = 0; k < NBINS; k++) launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], true, s);
+      launch_prune<T>(st, g.n, m2max, s);
+      launch_graph_sums(gid, ctx->binid, (const int32_t*)ctx->In, (const int32_t*)ctx->Tot, g.n, b.g_in, b.g_tot2,
+                        s);
+      launch_batch_step(b, it_dev, min_mod, max_it, s);
+      launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, false, it_dev, s);
+      partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp,
+                     s);
+    };
+    auto live_after = [&](int32_t* out_live) -> int {
+      HIPOK(hipMemcpyAsync(ctx->b_host, nlive, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      HIPOK(hipStreamSynchronize(s));
+      *out_live = ctx->b_host[0];
+      return GALA_OK;
+    };
+    int32_t live = 0;
+    if ((rc = live_after(&live))) return rc;
+    if (round < 64) R.round_open[round] = live;
+    hipGraphExec_t exec = nullptr;
+    int it = 0;
+    while (live > 0 && it < max_it) {
+      it++;
+      if (it == 1 || !graphs) {
+        body(it == 1 && unique_rows);
+      } else {
+        if (!exec) {
+          hipGraph_t gr = nullptr;
+          HIPOK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          body(false);
+          hipError_t e = hipStreamEndCapture(s, &gr);
+          if (e != hipSuccess) { ctx->last_hip = e; return GALA_E_HIP; }
+          e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(gr);
+          if (e != hipSuccess) { ctx->last_hip = e; return GALA_E_HIP; }
+        }
+        hipError_t e = hipGraphLaunch(exec, s);
+        if (e != hipSuccess) { ctx->last_hip = e; (void)hipGraphExecDestroy(exec); return GALA_E_HIP; }
+      }
+      rc = live_after(&live);
+      if (rc) { if (exec) (void)hipGraphExecDestroy(exec); return rc; }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    HIPOK(hipEventRecord(ctx->ev[6], s));
+    // contraction of the whole union under the accepted assignments
+    int32_t C = 0;
+    int64_t m = 0;
+    const int dst = dst_of(src);
+    rc = contract<T>(ctx, g, ctx->prev, dst, caps, &C, &m);
+    if (rc) return rc;
+    compose_map(ctx->fin, ctx->n0, ctx->c_label, s);
+    const int gnext = gcur == 1 ? 2 : 1;
+    launch_batch_gid_next(gid, ctx->c_cnt, ctx->c_nid, g.n, ctx->b_gid[gnext], s);
+    fill<int32_t>(nopen, 1, 0, s);
+    launch_batch_round(b, round, min_mod, max_rounds, nopen, s);
+    HIPOK(hipEventRecord(ctx->ev[7], s));
+    HIPOK(hipMemcpyAsync(ctx->b_host + 1, nopen, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPOK(hipStreamSynchronize(s));
+    const int32_t open = ctx->b_host[1];
+    float t_p1 = 0, t_build = 0;
+    (void)hipEventElapsedTime(&t_p1, ctx->ev[5], ctx->ev[6]);
+    (void)hipEventElapsedTime(&t_build, ctx->ev[6], ctx->ev[7]);
+    R.t_phase1_ms += t_p1;
+    R.t_contract_ms += t_build;
+    R.iterations += it;
+    if (round < 64) {
+      R.round_iters[round] = it;
+      R.round_ms[round] = t_p1 + t_build;
+    }
+    src = dst;
+    gcur = gnext;
+    g = graph_at<T>(ctx, src, C, m);
+    round++;
+    R.rounds = round;
+    if (open == 0 || round >= max_rounds) break;
+  }
+  HIPOK(hipEventRecord(ctx->ev[3], s));
+  HIPOK(hipEventSynchronize(ctx->ev[3]));
+  float tt_ms = 0;
+  (void)hipEventElapsedTime(&tt_ms, ctx->ev[4], ctx->ev[3]);
+  R.t_total_ms = tt_ms;
+  Counters c;
+  int rc = read_counters<T>(ctx, &c);
+  if (rc) return rc;
+  R.swept_vertices = (int64_t)c.swept;
+  R.swept_edges = (int64_t)c.swept_edges;
+  // per-graph results; communities renumbered within each graph (the final
+  // union's vertices are grouped by graph in ascending order)
+  fill<int32_t>(ctx->b_ncomm, G, 0, s);
+  launch_batch_count(ctx->b_gid[gcur], g.n, ctx->b_ncomm, s);
+  ctx->b_q.resize(G); ctx->b_nc.resize(G); ctx->b_rd.resize(G); ctx->b_it.resize(G);
+  HIPOK(hipMemcpyAsync(ctx->b_q.data(), ctx->b_cur_mod, sizeof(double) * G, hipMemcpyDeviceToHost, s));
+  HIPOK(hipMemcpyAsync(ctx->b_nc.data(), ctx->b_ncomm, sizeof(int32_t) * G, hipMemcpyDeviceToHost, s));
+  HIPOK(hipMemcpyAsync(ctx->b_rd.data(), ctx->b_rounds, sizeof(int32_t) * G, hipMemcpyDeviceToHost, s));
+  HIPOK(hipMemcpyAsync(ctx->b_it.data(), ctx->b_tot_iters, sizeof(int32_t) * G, hipMemcpyDeviceToHost, s));
+  ctx->b_comm_local.resize(ctx->n0);
+  HIPOK(hipMemcpyAsync(ctx->b_comm_local.data(), ctx->fin, sizeof(int32_t) * ctx->n0, hipMemcpyDeviceToHost, s));
+  HIPOK(hipStreamSynchronize(s));
+  int64_t first = 0;
+  for (int32_t gg = 0; gg < G; gg++) {
+    for (int32_t v = ctx->b_vstart[gg]; v < ctx->b_vstart[gg + 1]; v++) ctx->b_comm_local[v] -= (int32_t)first;
+    first += ctx->b_nc[gg];
+  }
+  ctx->have_batch_result = true;
+  ctx->have_result = false;
+  if (out) *out = R;
+  return GALA_OK;
+}
+
 const char* kErr[] = {"ok", "I/O error", "malformed graph", "out of memory", "HIP runtime error",
                       "exact-integer range exceeded or hash overflow", "invalid argument or state"};
 
@@ -887,6 +1184,7 @@ void gala_destroy(gala_ctx* ctx) {
   if (ctx->mlp_prm) (void)hipFree(ctx->mlp_prm);
   if (ctx->mlp_w1b) (void)hipFree(ctx->mlp_w1b);
   if (ctx->ctr_h) (void)hipHostFree(ctx->ctr_h);
+  if (ctx->b_host) (void)hipHostFree(ctx->b_host);
   for (auto& e : ctx->ev) (void)hipEventDestroy(e);
   for (auto& e : ctx->bev) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -921,6 +1219,36 @@ int gala_run(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gal
   (void)hipSetDevice(ctx->device);
   // the input buffers are read-only: a context can run the same graph again
   return is_fp(ctx) ? run_all<FltW>(ctx, icb, rcb, user, out) : run_all<IntW>(ctx, icb, rcb, user, out);
+}
+
+int gala_batch_set(gala_ctx* ctx, int32_t n_graphs, const int32_t* vstart, int32_t n, const int64_t* end_incl,
+                   const int32_t* nbr, const void* w, int weight_mode) {
+  if (!ctx) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  return batch_upload(ctx, n_graphs, vstart, n, end_incl, nbr, w, weight_mode);
+}
+
+int gala_batch_run(gala_ctx* ctx, gala_batch_result* out) {
+  if (!ctx || !ctx->have_batch || !ctx->have_graph) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  return run_batch(ctx, out);
+}
+
+int gala_batch_graph_results(gala_ctx* ctx, double* modularity, int32_t* n_communities, int32_t* rounds,
+                             int32_t* iterations) {
+  if (!ctx || !ctx->have_batch_result) return GALA_E_INVALID;
+  const size_t G = ctx->b_q.size();
+  if (modularity) std::memcpy(modularity, ctx->b_q.data(), sizeof(double) * G);
+  if (n_communities) std::memcpy(n_communities, ctx->b_nc.data(), sizeof(int32_t) * G);
+  if (rounds) std::memcpy(rounds, ctx->b_rd.data(), sizeof(int32_t) * G);
+  if (iterations) std::memcpy(iterations, ctx->b_it.data(), sizeof(int32_t) * G);
+  return GALA_OK;
+}
+
+int gala_batch_communities(gala_ctx* ctx, int32_t* comm) {
+  if (!ctx || !comm || !ctx->have_batch_result) return GALA_E_INVALID;
+  std::memcpy(comm, ctx->b_comm_local.data(), sizeof(int32_t) * ctx->b_comm_local.size());
+  return GALA_OK;
 }
 
 int gala_partial_result(gala_ctx* ctx, gala_result* out) {

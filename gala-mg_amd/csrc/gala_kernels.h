@@ -51,6 +51,27 @@ struct DevState {
   float* last_gain;  // normalised net gain of the best move at the vertex's last sweep
   int32_t* moved_it; // sweep in which the vertex last moved (-1: never this round)
   float* nbr_moved;  // fraction of neighbours that moved in the latest sweep
+  // batched graphs (block-diagonal union): graph of every vertex and the m2 of
+  // every graph; null for a single graph
+  const int32_t* gid = nullptr;
+  const typename T::tot_t* m2g = nullptr;
+};
+
+// Per-graph state of a batch (one entry per graph; integer path).
+struct BatchDev {
+  int32_t G;
+  const int32_t* m2g;            // m2 of every graph
+  uint8_t* live;                 // still iterating in this round
+  uint8_t* accept;               // latest sweep accepted (prev = cur)
+  uint8_t* done;                 // round loop finished
+  double* qprev;                 // Q of the accepted assignment of the round
+  double* cur_mod;               // Q at the end of the latest round
+  int32_t* iters;                // iterations of the current round
+  int32_t* tot_iters;            // iterations over all rounds
+  int32_t* rounds;               // rounds run
+  unsigned long long* g_in;      // per-graph sum In   (zero between sweeps)
+  unsigned long long* g_tot2;    // per-graph sum Tot^2 (zero between sweeps)
+  int32_t* nlive;                // graphs still iterating after the latest sweep
 };
 
 template <class T>
@@ -79,6 +100,19 @@ void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
 // prune flags, sum In, active count; clears the dirty marks.
 template <class T>
 void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);
+
+// ---- batched graphs (kernels_batch.hip) ----
+void launch_graph_sums(const int32_t* gid, const int8_t* binid, const int32_t* In, const int32_t* Tot, int32_t n,
+                       unsigned long long* g_in, unsigned long long* g_tot2, hipStream_t s);
+void launch_batch_step(const BatchDev& b, const int32_t* it_dev, double min_mod, int max_it, hipStream_t s);
+void launch_batch_mask(const BatchDev& b, const int32_t* gid, const int32_t* cur, int32_t* prev, uint8_t* active,
+                       int32_t n, bool first, int32_t* it_dev, hipStream_t s);
+void launch_batch_live(const BatchDev& b, int32_t* it_dev, hipStream_t s);
+void launch_batch_round(const BatchDev& b, int round, double min_mod, int max_rounds, int32_t* nopen, hipStream_t s);
+void launch_batch_gid_next(const int32_t* gid, const int32_t* cnt, const int32_t* nid, int32_t n, int32_t* gid_next,
+                           hipStream_t s);
+void launch_batch_count(const int32_t* gid, int32_t n, int32_t* per_graph, hipStream_t s);
+void launch_batch_reset(Counters* c, long long min_init, int32_t* nlive, hipStream_t s);
 
 // ---- learned frontier (kernels_learned.hip) ----
 // Features of vertex v before sweep it (all in [-1, 1]):

@@ -21,6 +21,8 @@
 #include <map>
 #include <sstream>
 #include <string>
+#include <thread>
+#include <atomic>
 #include <vector>
 
 #include "../../include/gala.h"
@@ -106,11 +108,46 @@ void bernoulli_run(Rng& r, int64_t len, double p, F f) {
   }
 }
 
+// Planted-partition SBM: vertex v in block floor(v*blocks/n); every pair i<j
+// drawn once, p_in inside a block, p_out across.
+void sbm_csr(int32_t n, int32_t blocks, double p_in, double p_out, uint64_t seed, Csr& out) {
+  Rng r(seed);
+  auto blk = [&](int64_t v) { return (int32_t)((v * blocks) / n); };
+  auto bstart = [&](int32_t b) {  // first vertex of block b
+    return (int32_t)(((int64_t)b * n + blocks - 1) / blocks);
+  };
+  std::vector<std::pair<int32_t, int32_t>> e;
+  for (int32_t i = 0; i < n; i++) {
+    int32_t bi = blk(i);
+    // runs of constant probability over j > i: rest of own block, then other blocks
+    int32_t j0 = i + 1;
+    for (int32_t b = bi; b < blocks && j0 < n; b++) {
+      int32_t j1 = (b + 1 < blocks) ? bstart(b + 1) : n;
+      if (j1 <= j0) continue;
+      double p = (b == bi) ? p_in : p_out;
+      bernoulli_run(r, j1 - j0, p, [&](int64_t k) { e.push_back({i, (int32_t)(j0 + k)}); });
+      j0 = j1;
+    }
+  }
+  csr_from_pairs(n, e, out);
+}
+
 // cache of the last generated graph so the two-call protocol does not regenerate
 struct GenCache {
   std::string key;
   Csr csr;
 } g_cache;
+
+// union of a batch of graphs: graph g owns vertices [vstart[g], vstart[g+1])
+struct BatchCsr {
+  std::vector<int32_t> vstart;
+  std::vector<int64_t> end;  // inclusive ends
+  std::vector<int32_t> nbr;
+};
+struct BatchCache {
+  std::string key;
+  BatchCsr u;
+} g_batch;
 
 }  // namespace
 
@@ -240,30 +277,83 @@ int gala_gen_sbm(int32_t n, int32_t blocks, double p_in, double p_out, uint64_t 
   std::snprintf(kb, sizeof kb, "sbm:%d:%d:%.17g:%.17g:%llu", n, blocks, p_in, p_out,
                 (unsigned long long)seed);
   if (g_cache.key != kb) {
-    Rng r(seed);
-    auto blk = [&](int64_t v) { return (int32_t)((v * blocks) / n); };
-    auto bstart = [&](int32_t b) {  // first vertex of block b
-      return (int32_t)(((int64_t)b * n + blocks - 1) / blocks);
-    };
-    std::vector<std::pair<int32_t, int32_t>> e;
-    for (int32_t i = 0; i < n; i++) {
-      int32_t bi = blk(i);
-      // runs of constant probability over j > i: rest of own block, then other blocks
-      int32_t j0 = i + 1;
-      for (int32_t b = bi; b < blocks && j0 < n; b++) {
-        int32_t j1 = (b + 1 < blocks) ? bstart(b + 1) : n;
-        if (j1 <= j0) continue;
-        double p = (b == bi) ? p_in : p_out;
-        bernoulli_run(r, j1 - j0, p, [&](int64_t k) { e.push_back({i, (int32_t)(j0 + k)}); });
-        j0 = j1;
-      }
-    }
     Csr c;
-    csr_from_pairs(n, e, c);
+    sbm_csr(n, blocks, p_in, p_out, seed, c);
     g_cache.key = kb;
     g_cache.csr = std::move(c);
   }
   return emit(g_cache.csr, out_n, out_m, end_incl, nbr, nullptr);
+}
+
+int gala_gen_sbm_batch(int32_t n_graphs, const int32_t* n, const int32_t* blocks, const double* p_in,
+                       const double* p_out, const uint64_t* seeds, int32_t threads, int32_t* vstart,
+                       int64_t* out_m, int64_t* end_incl, int32_t* nbr) {
+  if (n_graphs < 0 || (n_graphs > 0 && (!n || !blocks || !p_in || !p_out || !seeds))) return GALA_E_INVALID;
+  int64_t nt = 0;
+  for (int32_t g = 0; g < n_graphs; g++) {
+    if (n[g] < 0 || blocks[g] < 1 || blocks[g] > std::max(n[g], 1)) return GALA_E_INVALID;
+    nt += n[g];
+  }
+  if (nt > INT32_MAX) return GALA_E_INVALID;
+  // key: FNV-1a over the parameter arrays
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void* p, size_t len) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < len; i++) { h ^= b[i]; h *= 1099511628211ull; }
+  };
+  mix(&n_graphs, sizeof n_graphs);
+  mix(n, sizeof(int32_t) * n_graphs); mix(blocks, sizeof(int32_t) * n_graphs);
+  mix(p_in, sizeof(double) * n_graphs); mix(p_out, sizeof(double) * n_graphs);
+  mix(seeds, sizeof(uint64_t) * n_graphs);
+  char kb[64];
+  std::snprintf(kb, sizeof kb, "sbmbatch:%d:%016llx", n_graphs, (unsigned long long)h);
+  if (g_batch.key != kb) {
+    std::vector<Csr> parts(n_graphs);
+    int nth = threads > 0 ? threads : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    // largest graphs first so the threads finish together
+    std::vector<int32_t> ord(n_graphs);
+    for (int32_t g = 0; g < n_graphs; g++) ord[g] = g;
+    std::sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return n[a] > n[b]; });
+    std::atomic<int32_t> next{0};
+    auto work = [&]() {
+      for (;;) {
+        int32_t k = next.fetch_add(1);
+        if (k >= n_graphs) break;
+        int32_t g = ord[k];
+        sbm_csr(n[g], blocks[g], p_in[g], p_out[g], seeds[g], parts[g]);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    BatchCsr u;
+    u.vstart.assign(n_graphs + 1, 0);
+    int64_t mt = 0;
+    for (int32_t g = 0; g < n_graphs; g++) {
+      u.vstart[g + 1] = u.vstart[g] + n[g];
+      mt += parts[g].nbr.size();
+    }
+    u.end.resize(nt);
+    u.nbr.resize(mt);
+    int64_t e0 = 0;
+    for (int32_t g = 0; g < n_graphs; g++) {
+      const Csr& c = parts[g];
+      const int32_t v0 = u.vstart[g];
+      for (int32_t i = 0; i < c.n; i++) u.end[v0 + i] = e0 + c.end[i];
+      for (size_t j = 0; j < c.nbr.size(); j++) u.nbr[e0 + j] = v0 + c.nbr[j];
+      e0 += c.nbr.size();
+      parts[g] = Csr();
+    }
+    g_batch.key = kb;
+    g_batch.u = std::move(u);
+  }
+  const BatchCsr& u = g_batch.u;
+  if (vstart) std::memcpy(vstart, u.vstart.data(), sizeof(int32_t) * (n_graphs + 1));
+  if (out_m) *out_m = (int64_t)u.nbr.size();
+  if (end_incl) std::memcpy(end_incl, u.end.data(), sizeof(int64_t) * u.end.size());
+  if (nbr) std::memcpy(nbr, u.nbr.data(), sizeof(int32_t) * u.nbr.size());
+  return GALA_OK;
 }
 
 int gala_gen_chung_lu(int32_t n, double gamma, double mean_deg, double max_deg, uint64_t seed,

@@ -126,6 +126,12 @@ __device__ __forceinline__ void atomic_add_ull(unsigned long long* p, V x) {
   if (x) atomicAdd(p, (unsigned long long)x);
 }
 
+// m2 of the vertex's graph: the batch's per-graph m2 when graphs are batched.
+template <class T>
+__device__ __forceinline__ typename T::tot_t vertex_m2(const DevState<T>& st, int v, typename T::tot_t m2_all) {
+  return st.gid ? st.m2g[st.gid[v]] : m2_all;
+}
+
 // ---------------------------------------------------------------------------
 // init: K, Self, Tot=K, size=1, prev=cur=v, bin id, active=1
 // (init_communities, kernel_functions.cu:34-73)
@@ -207,7 +213,7 @@ template <int W, class T>
 __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> st,
                                                       const int32_t* __restrict__ front,
                                                       const int32_t* __restrict__ bstart, int bin,
-                                                      typename T::tot_t m2) {
+                                                      typename T::tot_t m2_all) {
   using acc = typename T::acc_t;
   using tt = typename T::tot_t;
   using G = typename T::gain_t;
@@ -220,6 +226,7 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
   unsigned moved = 0, swept = 0, edges = 0;
   for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W; i < hi; i += groups) {
     const int v = front[i];
+    const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
     const int c = st.prev[v];
@@ -288,7 +295,7 @@ template <int TMAX, class T>
 __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> st,
                                                       const int32_t* __restrict__ front,
                                                       const int32_t* __restrict__ bstart, int bin,
-                                                      typename T::tot_t m2) {
+                                                      typename T::tot_t m2_all) {
   using acc = typename T::acc_t;
   using tt = typename T::tot_t;
   using G = typename T::gain_t;
@@ -305,6 +312,7 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
   unsigned moved = 0, swept = 0, edges = 0;
   for (int i = lo + blockIdx.x * 4 + wid; i < hi; i += nwaves) {
     const int v = front[i];
+    const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
     const int c = st.prev[v];
@@ -379,7 +387,7 @@ template <int NT, int TMAX, class T>
 __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> st,
                                                       const int32_t* __restrict__ front,
                                                       const int32_t* __restrict__ bstart, int bin,
-                                                      typename T::tot_t m2) {
+                                                      typename T::tot_t m2_all) {
   using acc = typename T::acc_t;
   using tt = typename T::tot_t;
   using G = typename T::gain_t;
@@ -396,6 +404,7 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
   unsigned long long edges = 0;
   for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
     const int v = front[i];
+    const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
     const int c = st.prev[v];
@@ -480,7 +489,7 @@ template <int LT, class T>
 __global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> st,
                                                       const int32_t* __restrict__ front,
                                                       const int32_t* __restrict__ bstart, int bin,
-                                                      typename T::tot_t m2) {
+                                                      typename T::tot_t m2_all) {
   using acc = typename T::acc_t;
   using tt = typename T::tot_t;
   using G = typename T::gain_t;
@@ -497,6 +506,7 @@ __global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> 
   unsigned long long edges = 0;
   for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
     const int v = front[i];
+    const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
     const int c = st.prev[v];
@@ -603,7 +613,7 @@ template <int W, class T>
 __global__ __launch_bounds__(256) void k_decide_first(DevGraph<T> g, DevState<T> st,
                                                        const int32_t* __restrict__ front,
                                                        const int32_t* __restrict__ bstart, int bin,
-                                                       typename T::tot_t m2) {
+                                                       typename T::tot_t m2_all) {
   using tt = typename T::tot_t;
   using G = typename T::gain_t;
   __shared__ unsigned long long s_moved, s_swept, s_edges;
@@ -615,6 +625,7 @@ __global__ __launch_bounds__(256) void k_decide_first(DevGraph<T> g, DevState<T>
   unsigned moved = 0, swept = 0, edges = 0;
   for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W; i < hi; i += groups) {
     const int v = front[i];
+    const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
     const tt K = st.K[v];
@@ -658,7 +669,7 @@ template <int NT, class T>
 __global__ __launch_bounds__(NT) void k_decide_first_block(DevGraph<T> g, DevState<T> st,
                                                             const int32_t* __restrict__ front,
                                                             const int32_t* __restrict__ bstart, int bin,
-                                                            typename T::tot_t m2) {
+                                                            typename T::tot_t m2_all) {
   using tt = typename T::tot_t;
   using G = typename T::gain_t;
   __shared__ G r_h[NT / 64];
@@ -670,6 +681,7 @@ __global__ __launch_bounds__(NT) void k_decide_first_block(DevGraph<T> g, DevSta
   unsigned long long edges = 0;
   for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
     const int v = front[i];
+    const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
     const tt K = st.K[v];
@@ -951,7 +963,7 @@ __global__ __launch_bounds__(256) void k_prune(DevState<T> st, int32_t n, typena
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     if (st.binid[v] < 0) continue;
     const tt x = st.In[v];
-    const uint8_t keep = prune_keep<T>(x, st.K[v], st.Self[v], st.Tot[st.cur[v]], minTot, m2);
+    const uint8_t keep = prune_keep<T>(x, st.K[v], st.Self[v], st.Tot[st.cur[v]], minTot, vertex_m2<T>(st, v, m2));
     st.active[v] = keep;
     if (st.track && !st.dirty[v]) st.nbr_moved[v] = 0.f;  // no neighbour moved
     st.dirty[v] = 0;

@@ -24,7 +24,8 @@ EXPORTS = [
     "gala_get_communities", "gala_phase_decide", "gala_phase_weight", "gala_phase_contract",
     "gala_read_bin", "gala_write_bin", "gala_ingest_text", "gala_gen_sbm", "gala_gen_chung_lu",
     "gala_gen_rmat", "gala_set_logger", "gala_set_mlp", "gala_get_mlp", "gala_mlp_train",
-    "gala_mlp_forward", "gala_mlp_grad",
+    "gala_mlp_forward", "gala_mlp_grad", "gala_batch_set", "gala_batch_run", "gala_batch_graph_results",
+    "gala_batch_communities", "gala_gen_sbm_batch",
 ]
 
 MLP_NFEAT = 6
@@ -40,7 +41,8 @@ class Opts(ctypes.Structure):
                 ("max_rounds", ctypes.c_int), ("dump_iterations", ctypes.c_int),
                 ("dump_rounds", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("frontier_mode", ctypes.c_int), ("learned_tau", ctypes.c_float),
-                ("profile_bins", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
+                ("profile_bins", ctypes.c_int), ("no_hip_graphs", ctypes.c_int),
+                ("reserved", ctypes.c_int * 6)]
 
 
 class Result(ctypes.Structure):
@@ -89,6 +91,22 @@ ROUND_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_double
                             ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double))
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("n_graphs", ctypes.c_int32), ("rounds", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32), ("t_total_ms", ctypes.c_double), ("t_phase1_ms", ctypes.c_double),
+                ("t_contract_ms", ctypes.c_double), ("swept_vertices", ctypes.c_int64),
+                ("swept_edges", ctypes.c_int64), ("round_open", ctypes.c_int32 * 64),
+                ("round_iters", ctypes.c_int32 * 64), ("round_ms", ctypes.c_double * 64)]
+
+    def to_dict(self):
+        r = self.rounds
+        return dict(n_graphs=self.n_graphs, rounds=r, iterations=self.iterations, t_total_ms=self.t_total_ms,
+                    t_phase1_ms=self.t_phase1_ms, t_contract_ms=self.t_contract_ms,
+                    swept_vertices=self.swept_vertices, swept_edges=self.swept_edges,
+                    round_open=list(self.round_open[:r]), round_iters=list(self.round_iters[:r]),
+                    round_ms=list(self.round_ms[:r]))
+
 
 LOG_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int32,
                           ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint8))
@@ -143,6 +161,13 @@ def lib():
                                      ctypes.c_float, ctypes.c_uint64, ctypes.c_int, F32P, F32P]
         L.gala_mlp_forward.argtypes = [VP, F32P, F32P, ctypes.c_int64, F32P]
         L.gala_mlp_grad.argtypes = [VP, F32P, F32P, U8P, ctypes.c_int64, ctypes.c_float, F32P, F32P]
+        L.gala_batch_set.argtypes = [VP, ctypes.c_int32, I32P, ctypes.c_int32, I64P, I32P, VP, ctypes.c_int]
+        L.gala_batch_run.argtypes = [VP, ctypes.POINTER(BatchResult)]
+        L.gala_batch_graph_results.argtypes = [VP, ctypes.POINTER(ctypes.c_double), I32P, I32P, I32P]
+        L.gala_batch_communities.argtypes = [VP, I32P]
+        L.gala_gen_sbm_batch.argtypes = [ctypes.c_int32, I32P, I32P, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+                                         ctypes.c_int32, I32P, I64P, I64P, I32P]
         _lib = L
     return _lib
 
@@ -261,6 +286,59 @@ def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=0, w_lo=0.5, w_
     return _two_call(lib().gala_gen_rmat, (scale, edge_factor, a, b, c, seed, w_lo, w_hi), with_w=True)
 
 
+def gen_sbm_batch(n, blocks, p_in, p_out, seeds, threads=0):
+    """Union of SBMs (graph g = gen_sbm(n[g], blocks[g], p_in[g], p_out[g], seeds[g]),
+    vertex ids offset by vstart[g]).  Returns (vstart[G+1], union Graph)."""
+    G = len(n)
+    n = np.ascontiguousarray(n, np.int32)
+    blocks = np.ascontiguousarray(blocks, np.int32)
+    p_in = np.ascontiguousarray(p_in, np.float64)
+    p_out = np.ascontiguousarray(p_out, np.float64)
+    seeds = np.ascontiguousarray(seeds, np.uint64)
+    vstart = np.zeros(G + 1, np.int32)
+    m = ctypes.c_int64()
+    D = ctypes.POINTER(ctypes.c_double)
+    args = (G, _p(n, ctypes.c_int32), _p(blocks, ctypes.c_int32), _p(p_in, ctypes.c_double),
+            _p(p_out, ctypes.c_double), _p(seeds, ctypes.c_uint64), threads, _p(vstart, ctypes.c_int32),
+            ctypes.byref(m))
+    _check(lib().gala_gen_sbm_batch(*args, None, None), "gala_gen_sbm_batch")
+    nt = int(vstart[-1])
+    end = np.zeros(max(nt, 1), np.int64)
+    nbr = np.zeros(max(m.value, 1), np.int32)
+    _check(lib().gala_gen_sbm_batch(*args, _p(end, ctypes.c_int64), _p(nbr, ctypes.c_int32)), "gala_gen_sbm_batch")
+    del D
+    return vstart, Graph(end[:nt], nbr[:m.value])
+
+
+def union_graphs(graphs):
+    """Block-diagonal union of graphs (same weight mode): (vstart[G+1], Graph)."""
+    sizes = [g.vertex_num for g in graphs]
+    vstart = np.zeros(len(graphs) + 1, np.int32)
+    vstart[1:] = np.cumsum(sizes)
+    nbr = np.concatenate([g.neighbors + vstart[i] for i, g in enumerate(graphs)]).astype(np.int32)
+    ends, base = [], 0
+    for g in graphs:
+        ends.append(g.degrees + base)
+        base += int(g.degrees[-1]) if g.vertex_num else 0
+    end = np.concatenate(ends).astype(np.int64)
+    mode = graphs[0].weight_mode
+    w = None if graphs[0].weights is None else np.concatenate([g.weights for g in graphs])
+    return vstart, Graph(end, nbr, w, mode)
+
+
+def batch_sbm_params(n_graphs=4096, seed=0, n_lo=64, n_hi=20000):
+    """Parameters of the batched SBM workload: vertex counts log-uniform in
+    [n_lo, n_hi], 4..64 blocks, p_in uniform in [0.05, 0.3],
+    p_out = 0.1 p_in / blocks, one seed per graph."""
+    rng = np.random.default_rng(seed)
+    n = np.exp(rng.uniform(np.log(n_lo), np.log(n_hi), n_graphs)).astype(np.int32)
+    blocks = np.minimum(rng.integers(4, 65, n_graphs), n).astype(np.int32)
+    p_in = rng.uniform(0.05, 0.3, n_graphs)
+    p_out = 0.1 * p_in / blocks
+    seeds = (np.uint64(seed) * np.uint64(1000003) + np.arange(n_graphs, dtype=np.uint64)).astype(np.uint64)
+    return n, blocks, p_in, p_out, seeds
+
+
 # ---------------------------------------------------------------------------
 # engine
 # ---------------------------------------------------------------------------
@@ -268,7 +346,8 @@ class Engine:
     """One device context (gala_create ... gala_destroy)."""
 
     def __init__(self, device=0, min_modularity=1e-6, dump_iterations=False, dump_rounds=False,
-                 max_rounds=0, max_iterations=0, profile_bins=False, frontier_mode=0, learned_tau=0.5):
+                 max_rounds=0, max_iterations=0, profile_bins=False, frontier_mode=0, learned_tau=0.5,
+                 hip_graphs=True):
         L = lib()
         self.opts = Opts()
         L.gala_default_opts(ctypes.byref(self.opts))
@@ -280,6 +359,7 @@ class Engine:
         self.opts.profile_bins = int(profile_bins)
         self.opts.frontier_mode = int(frontier_mode)
         self.opts.learned_tau = float(learned_tau)
+        self.opts.no_hip_graphs = 0 if hip_graphs else 1
         self.ctx = VP()
         _check(L.gala_create(ctypes.byref(self.ctx), device, ctypes.byref(self.opts)), "gala_create")
 
@@ -330,6 +410,38 @@ class Engine:
         res = Result()
         _check(lib().gala_run(self.ctx, self._icb, self._rcb, None, ctypes.byref(res)), "gala_run")
         return res.to_dict()
+
+    # ---- batched graphs ----
+    def batch_set(self, vstart, g):
+        """Upload a batch given as its union graph and vstart[G+1]."""
+        vstart = np.ascontiguousarray(vstart, np.int32)
+        w = g.weights
+        _check(lib().gala_batch_set(self.ctx, len(vstart) - 1, _p(vstart, ctypes.c_int32), g.vertex_num,
+                                    _p(g.degrees, ctypes.c_int64), _p(g.neighbors, ctypes.c_int32),
+                                    None if w is None else w.ctypes.data_as(VP), g.weight_mode), "gala_batch_set")
+        self._vstart = vstart
+
+    def batch_run(self):
+        res = BatchResult()
+        _check(lib().gala_batch_run(self.ctx, ctypes.byref(res)), "gala_batch_run")
+        return res.to_dict()
+
+    def batch_results(self):
+        """Per graph: modularity, communities, rounds, iterations (numpy arrays)."""
+        G = len(self._vstart) - 1
+        q = np.zeros(G, np.float64)
+        nc, rd, it = (np.zeros(G, np.int32) for _ in range(3))
+        _check(lib().gala_batch_graph_results(self.ctx, _p(q, ctypes.c_double), _p(nc, ctypes.c_int32),
+                                              _p(rd, ctypes.c_int32), _p(it, ctypes.c_int32)),
+               "gala_batch_graph_results")
+        return dict(modularity=q, communities=nc, rounds=rd, iterations=it)
+
+    def batch_communities(self):
+        """List of per-graph community vectors (ids 0.. within each graph)."""
+        vs = self._vstart
+        out = np.zeros(max(int(vs[-1]), 1), np.int32)
+        _check(lib().gala_batch_communities(self.ctx, _p(out, ctypes.c_int32)), "gala_batch_communities")
+        return [out[vs[g]:vs[g + 1]] for g in range(len(vs) - 1)]
 
     # ---- learned frontier ----
     def set_logger(self, sink):

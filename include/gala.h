@@ -49,7 +49,9 @@ typedef struct {
   int frontier_mode;     /* 0 plain (sound prune), 1 learned frontier + certification */
   float learned_tau;     /* learned frontier threshold on the move probability */
   int profile_bins;      /* 1: time every degree bin's decide launch with HIP events */
-  int reserved[7];
+  int no_hip_graphs;     /* batched runs replay each round's iteration body from a
+                            captured hipGraph; 1 = launch kernel by kernel instead */
+  int reserved[6];
 } gala_opts;
 
 /* Called after every iteration (Iteration:0 is Q before any move).
@@ -158,8 +160,11 @@ int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const
 /* ---- learned frontier (no counterpart in the reference) ----
  * A 6 -> 64 -> 1 MLP (BF16 MFMA, FP32 accumulation, FP32 master weights)
  * predicts which vertices the next sweep moves; with opts.frontier_mode = 1 a
- * sweep only visits the predicted movers among the prune-active vertices, and
- * a full plain sweep certifies every level before it is accepted.  Parameter
+ * sweep only visits the predicted movers among the prune-active vertices.  A
+ * learned sweep that does not raise Q by min_modularity is certified: the rest
+ * of the frontier is swept against the same pre-sweep state, which completes
+ * it into the plain sweep, and the plain break/revert rule decides.  A round
+ * therefore only ends on a plain sweep's verdict.  Parameter
  * vector: GALA_MLP_NPARAM floats, W1[16][64] (rows 6..15 zero), b1[64],
  * w2[64], b2.  Features (6 per vertex) are defined in gala_kernels.h. */
 #define GALA_MLP_NFEAT 6
@@ -184,6 +189,41 @@ int gala_mlp_forward(gala_ctx* ctx, const float* params, const float* X, int64_t
 int gala_mlp_grad(gala_ctx* ctx, const float* params, const float* X, const uint8_t* y, int64_t n, float pos_weight,
                   float* grad_out, float* loss_out);
 
+/* ---- batched graphs (no counterpart in the reference, which runs one graph
+ * per process) ----
+ * Many independent graphs run as one block-diagonal union: graph g owns input
+ * vertices [vstart[g], vstart[g+1]) and its rows may only name vertices of the
+ * same range.  Every graph gets exactly the result of its own gala_run (same
+ * per-iteration decisions, break/revert, round loop and Q), while the sweeps,
+ * updates and contractions of all graphs share launches.  Integer weights
+ * only (GALA_W_NONE or GALA_W_I32); each graph's m2 must be <= 1e9.  A graph
+ * with no edge keeps every vertex alone and reports Q = 0 and 0 rounds. */
+typedef struct {
+  int32_t n_graphs;
+  int32_t rounds;            /* rounds of the union (the longest graph) */
+  int32_t iterations;        /* iterations of the union, summed over rounds */
+  int32_t reserved0;
+  double t_total_ms;         /* device time of the run, HIP events */
+  double t_phase1_ms;        /* sweeps + updates, all rounds */
+  double t_contract_ms;      /* contractions, all rounds */
+  int64_t swept_vertices;    /* decide-and-move work over all graphs */
+  int64_t swept_edges;
+  int32_t round_open[64];    /* graphs still running in each round */
+  int32_t round_iters[64];   /* iterations of the union in each round */
+  double round_ms[64];
+} gala_batch_result;
+
+/* Upload a batch (the union in the gala_set_graph layout plus vstart[n_graphs+1]). */
+int gala_batch_set(gala_ctx* ctx, int32_t n_graphs, const int32_t* vstart, int32_t n, const int64_t* end_incl,
+                   const int32_t* nbr, const void* w, int weight_mode);
+int gala_batch_run(gala_ctx* ctx, gala_batch_result* out);
+/* Per graph: final Q, communities, rounds and iterations (any may be NULL). */
+int gala_batch_graph_results(gala_ctx* ctx, double* modularity, int32_t* n_communities, int32_t* rounds,
+                             int32_t* iterations);
+/* Final community of every union vertex, numbered 0..C_g-1 within its graph
+ * (the numbering gala_get_communities gives for that graph alone). */
+int gala_batch_communities(gala_ctx* ctx, int32_t* comm);
+
 /* ---- .bin IO and generators (host only; no device needed) ---- */
 
 /* Read a .bin: first call with NULL arrays returns n and m (directed entries). */
@@ -206,6 +246,13 @@ int gala_gen_sbm(int32_t n, int32_t blocks, double p_in, double p_out, uint64_t 
                  int64_t* out_m, int64_t* end_incl, int32_t* nbr);
 int gala_gen_chung_lu(int32_t n, double gamma, double mean_deg, double max_deg, uint64_t seed,
                       int32_t* out_n, int64_t* out_m, int64_t* end_incl, int32_t* nbr);
+/* A batch of SBMs (graph g as gala_gen_sbm(n[g], blocks[g], p_in[g], p_out[g],
+ * seeds[g])) laid out as one union: vstart[n_graphs+1], vertex ids of graph g
+ * offset by vstart[g].  threads = 0: up to 16 host threads.  Two-call protocol:
+ * the first call fills vstart and out_m. */
+int gala_gen_sbm_batch(int32_t n_graphs, const int32_t* n, const int32_t* blocks, const double* p_in,
+                       const double* p_out, const uint64_t* seeds, int32_t threads, int32_t* vstart,
+                       int64_t* out_m, int64_t* end_incl, int32_t* nbr);
 int gala_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
                   float w_lo, float w_hi, int32_t* out_n, int64_t* out_m, int64_t* end_incl,
                   int32_t* nbr, float* w);
