@@ -17,6 +17,8 @@ Also reported:
   learned       the learned-frontier form on the same graph (network trained on
                 SBM move logs, gala-mg_amd/models/frontier_mlp.json): time per
                 run, speedup over the plain path and final-Q difference
+  batched       4,096 independent SBMs run as one union against the same graphs
+                run one at a time
 """
 import argparse
 import json
@@ -62,6 +64,70 @@ def learned_leg(gala, g, args, plain):
     return out
 
 
+def batch_leg(gala, args):
+    """Batched workload: 4,096 SBMs (vertex counts log-uniform in [64, 20,000],
+    4..64 blocks, p_in in [0.05, 0.3], p_out = 0.1 p_in / blocks, unweighted)
+    run as one union (hipGraph-replayed iterations, then kernel-by-kernel),
+    against the same graphs run one at a time (per-graph launches).  Every
+    graph's result is checked against the single-graph path for a sample."""
+    import numpy as np
+    t0 = time.time()
+    P = gala.batch_sbm_params(args.batch_graphs, seed=0)
+    vstart, u = gala.gen_sbm_batch(*P)
+    log(f"batch: {args.batch_graphs} graphs, n={u.vertex_num} undirected_edges={u.edge_num} "
+        f"gen {time.time() - t0:.1f}s")
+    out = {"graphs": args.batch_graphs, "vertices": int(u.vertex_num), "undirected_edges": int(u.edge_num)}
+    for name, hg in (("batched_hipgraph", True), ("batched_launches", False)):
+        eng = gala.Engine(device=0, hip_graphs=hg)
+        eng.batch_set(vstart, u)
+        eng.batch_run()
+        ts = []
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            r = eng.batch_run()
+            ts.append(time.perf_counter() - t1)
+        res = eng.batch_results()
+        comm = eng.batch_communities() if hg else None
+        eng.close()
+        out[name] = {"wall_ms": 1e3 * float(np.mean(ts)), "device_ms": r["t_total_ms"],
+                     "phase1_ms": r["t_phase1_ms"], "contract_ms": r["t_contract_ms"], "rounds": r["rounds"],
+                     "iterations": r["iterations"], "round_open": r["round_open"]}
+        log(f"{name}: {out[name]['wall_ms']:.1f} ms wall, {r['t_total_ms']:.1f} ms device, "
+            f"rounds {r['rounds']} iterations {r['iterations']}")
+        if hg:
+            bres, bcomm = res, comm
+    # per-graph launches: the single-graph path, graph by graph (upload excluded)
+    off = u.offsets()
+    eng = gala.Engine(device=0)
+    tot = 0.0
+    dev = 0.0
+    rng = np.random.default_rng(1)
+    sample = set(rng.choice(args.batch_graphs, size=min(64, args.batch_graphs), replace=False).tolist())
+    same = True
+    for i in range(args.batch_graphs):
+        a, b = int(vstart[i]), int(vstart[i + 1])
+        gi = gala.Graph(u.degrees[a:b] - off[a], u.neighbors[off[a]:off[b]] - a)
+        if gi.edge_num == 0:
+            continue
+        eng.set_graph(gi)
+        t1 = time.perf_counter()
+        r = eng.run()
+        tot += time.perf_counter() - t1
+        dev += r["t_total_ms"]
+        if i in sample:
+            same &= r["modularity"] == bres["modularity"][i] and bool(
+                np.array_equal(eng.communities(gi.vertex_num), bcomm[i]))
+    eng.close()
+    out["per_graph_launches"] = {"wall_ms": 1e3 * tot, "device_ms": dev}
+    out["speedup_vs_per_graph"] = out["per_graph_launches"]["wall_ms"] / out["batched_hipgraph"]["wall_ms"]
+    out["hipgraph_gain"] = out["batched_launches"]["wall_ms"] / out["batched_hipgraph"]["wall_ms"]
+    out["parity_sample_vs_single_graph"] = bool(same)
+    out["mean_q"] = float(np.mean(bres["modularity"]))
+    log(f"per-graph launches: {1e3 * tot:.1f} ms wall ({dev:.1f} ms device); batched speedup "
+        f"{out['speedup_vs_per_graph']:.2f}x; parity sample {same}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +137,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batched leg (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on exactly one GPU (asked for {args.gpus}); "
@@ -108,6 +175,7 @@ def main():
     comm = eng.communities(g.vertex_num)
     eng.close()
     learned = learned_leg(gala, g, args, runs[-1])
+    batched = batch_leg(gala, args) if args.batch_graphs > 0 else None
     # one more run with per-bin HIP events (outside the timed steps)
     peng = gala.Engine(device=0, profile_bins=True)
     peng.set_graph(g)
@@ -181,6 +249,7 @@ def main():
                           "update_ms": last["round_update_ms"], "contract_ms": last["round_build_ms"]},
         "parity_vs_oracle": parity,
         "learned": learned,
+        "batched": batched,
         "dQ_learned_vs_plain": learned["dQ"] if learned else None,
     }
     print(json.dumps(out))
