@@ -19,6 +19,7 @@ Also reported:
                 run, speedup over the plain path and final-Q difference
   batched       4,096 independent SBMs run as one union against the same graphs
                 run one at a time
+  weighted_rmat R-MAT scale 24 with fp32 weights (FP64 accumulation)
 """
 import argparse
 import json
@@ -128,6 +129,45 @@ def batch_leg(gala, args):
     return out
 
 
+def rmat_leg(gala, args):
+    """Weighted R-MAT: scale S (2^S vertices), edge factor 16, a=0.57 b=c=0.19,
+    fp32 weights uniform [0.5, 2.0], self-loops/duplicates merged; FP64
+    accumulation.  Optionally checked against the FP64 oracle (Q within 1e-9)."""
+    import numpy as np
+    t0 = time.time()
+    g = gala.gen_rmat(args.rmat_scale, 16, 0.57, 0.19, 0.19, 0, 0.5, 2.0)
+    deg = np.diff(g.offsets())
+    log(f"rmat: scale {args.rmat_scale} n={g.vertex_num} undirected_edges={g.edge_num} max_degree={deg.max()} "
+        f"gen {time.time() - t0:.1f}s")
+    eng = gala.Engine(device=0)
+    eng.set_graph(g)
+    eng.run()
+    runs = [eng.run() for _ in range(args.steps)]
+    comm = eng.communities(g.vertex_num)
+    eng.close()
+    r = runs[-1]
+    ms = float(np.mean([x["t_total_ms"] for x in runs]))
+    dm_gbs = r["decide_bytes"] / (r["t_decide_ms"] * 1e-3) / 1e9 if r["t_decide_ms"] > 0 else 0.0
+    out = {"scale": args.rmat_scale, "vertices": int(g.vertex_num), "undirected_edges": int(g.edge_num),
+           "max_degree": int(deg.max()), "hubs_over_1e5": int((deg > 100000).sum()), "ms_per_run": ms,
+           "edge_teps": g.edge_num / (ms * 1e-3), "modularity": r["modularity"], "rounds": r["rounds"],
+           "iterations": r["total_iterations"],
+           "phase_ms": {"decide": r["t_decide_ms"], "update": r["t_update_ms"], "contract": r["t_contract_ms"]},
+           "decide_GBps": dm_gbs, "dtype": "f64 (fp32 weights)", "oracle_dq": None}
+    log(f"rmat: {ms:.1f} ms Q={r['modularity']:.9f} rounds {r['rounds']} decide {dm_gbs:.0f} GB/s")
+    if args.rmat_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        oracle.set_threads(args.cpu_threads or min(16, os.cpu_count() or 1))
+        t2 = time.perf_counter()
+        ref = oracle.louvain_f(g.vertex_num, g.degrees, g.neighbors, g.weights)
+        out["oracle_dq"] = r["modularity"] - ref["Q"]
+        out["oracle_s"] = time.perf_counter() - t2
+        out["oracle_same_partition"] = bool(np.array_equal(ref["final_comm"], comm))
+        log(f"rmat oracle: Q={ref['Q']:.12f} dQ={out['oracle_dq']:.3e} ({out['oracle_s']:.1f} s)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +178,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batched leg (0: skip it)")
+    ap.add_argument("--rmat-scale", type=int, default=24, help="scale of the weighted R-MAT leg (0: skip it)")
+    ap.add_argument("--rmat-oracle", action="store_true", help="check the R-MAT leg against the FP64 oracle")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on exactly one GPU (asked for {args.gpus}); "
@@ -176,6 +218,7 @@ def main():
     eng.close()
     learned = learned_leg(gala, g, args, runs[-1])
     batched = batch_leg(gala, args) if args.batch_graphs > 0 else None
+    rmat = rmat_leg(gala, args) if args.rmat_scale > 0 else None
     # one more run with per-bin HIP events (outside the timed steps)
     peng = gala.Engine(device=0, profile_bins=True)
     peng.set_graph(g)
@@ -250,6 +293,7 @@ def main():
         "parity_vs_oracle": parity,
         "learned": learned,
         "batched": batched,
+        "weighted_rmat": rmat,
         "dQ_learned_vs_plain": learned["dQ"] if learned else None,
     }
     print(json.dumps(out))

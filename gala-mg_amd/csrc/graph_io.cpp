@@ -132,6 +132,23 @@ void sbm_csr(int32_t n, int32_t blocks, double p_in, double p_out, uint64_t seed
   csr_from_pairs(n, e, out);
 }
 
+int host_threads() {
+  return (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+}
+
+// f(i) for i in [0, count) over nth threads (dynamic assignment).
+template <class F>
+void parallel_for(int64_t count, int nth, F f) {
+  std::atomic<int64_t> next{0};
+  auto work = [&]() {
+    for (int64_t i; (i = next.fetch_add(1)) < count;) f(i);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nth && t < count; t++) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+}
+
 // cache of the last generated graph so the two-call protocol does not regenerate
 struct GenCache {
   std::string key;
@@ -309,7 +326,7 @@ int gala_gen_sbm_batch(int32_t n_graphs, const int32_t* n, const int32_t* blocks
   std::snprintf(kb, sizeof kb, "sbmbatch:%d:%016llx", n_graphs, (unsigned long long)h);
   if (g_batch.key != kb) {
     std::vector<Csr> parts(n_graphs);
-    int nth = threads > 0 ? threads : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    int nth = threads > 0 ? threads : host_threads();
     // largest graphs first so the threads finish together
     std::vector<int32_t> ord(n_graphs);
     for (int32_t g = 0; g < n_graphs; g++) ord[g] = g;
@@ -418,57 +435,86 @@ int gala_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint
   if (g_cache.key != kb) {
     const int32_t n = 1 << scale;
     const int64_t ne = (int64_t)edge_factor << scale;
-    Rng r(seed);
+    const int nth = host_threads();
+    // edges in fixed chunks, each from its own stream (output independent of
+    // the thread count)
+    constexpr int64_t CHUNK = 1 << 20;
+    const int64_t nchunks = (ne + CHUNK - 1) / CHUNK;
     std::vector<int32_t> src(ne), dst(ne);
     std::vector<float> wt(ne);
-    for (int64_t k = 0; k < ne; k++) {
-      int32_t u = 0, v = 0;
-      for (int lvl = 0; lvl < scale; lvl++) {
-        double x = r.uniform();
-        int bu = 0, bv = 0;
-        if (x < a) {}
-        else if (x < a + b) bv = 1;
-        else if (x < a + b + c) bu = 1;
-        else { bu = 1; bv = 1; }
-        u = (u << 1) | bu;
-        v = (v << 1) | bv;
+    parallel_for(nchunks, nth, [&](int64_t ch) {
+      uint64_t x = seed ^ (0xA0761D6478BD642Full * (uint64_t)(ch + 1));
+      Rng r(Rng::splitmix(x));
+      for (int64_t k = ch * CHUNK; k < std::min(ne, (ch + 1) * CHUNK); k++) {
+        int32_t u = 0, v = 0;
+        for (int lvl = 0; lvl < scale; lvl++) {
+          double q = r.uniform();
+          int bu = 0, bv = 0;
+          if (q < a) {}
+          else if (q < a + b) bv = 1;
+          else if (q < a + b + c) bu = 1;
+          else { bu = 1; bv = 1; }
+          u = (u << 1) | bu;
+          v = (v << 1) | bv;
+        }
+        src[k] = u; dst[k] = v;
+        wt[k] = (float)(w_lo + (w_hi - w_lo) * r.uniform());
       }
-      src[k] = u; dst[k] = v;
-      wt[k] = (float)(w_lo + (w_hi - w_lo) * r.uniform());
-    }
+    });
     // symmetrise into rows; a self-loop contributes one entry
     std::vector<int64_t> deg(n + 1, 0);
-    for (int64_t k = 0; k < ne; k++) {
-      deg[src[k] + 1]++;
-      if (src[k] != dst[k]) deg[dst[k] + 1]++;
-    }
+    parallel_for(nchunks, nth, [&](int64_t ch) {
+      for (int64_t k = ch * CHUNK; k < std::min(ne, (ch + 1) * CHUNK); k++) {
+        __atomic_fetch_add(&deg[src[k] + 1], 1, __ATOMIC_RELAXED);
+        if (src[k] != dst[k]) __atomic_fetch_add(&deg[dst[k] + 1], 1, __ATOMIC_RELAXED);
+      }
+    });
     for (int32_t i = 0; i < n; i++) deg[i + 1] += deg[i];
     std::vector<std::pair<int32_t, float>> ent(deg[n]);
     std::vector<int64_t> cur(deg.begin(), deg.end() - 1);
-    for (int64_t k = 0; k < ne; k++) {
-      ent[cur[src[k]]++] = {dst[k], wt[k]};
-      if (src[k] != dst[k]) ent[cur[dst[k]]++] = {src[k], wt[k]};
-    }
+    parallel_for(nchunks, nth, [&](int64_t ch) {
+      for (int64_t k = ch * CHUNK; k < std::min(ne, (ch + 1) * CHUNK); k++) {
+        ent[__atomic_fetch_add(&cur[src[k]], 1, __ATOMIC_RELAXED)] = {dst[k], wt[k]};
+        if (src[k] != dst[k]) ent[__atomic_fetch_add(&cur[dst[k]], 1, __ATOMIC_RELAXED)] = {src[k], wt[k]};
+      }
+    });
     src.clear(); src.shrink_to_fit(); dst.clear(); dst.shrink_to_fit(); wt.clear(); wt.shrink_to_fit();
+    // rows sorted by (neighbour, weight) so the merge order is fixed; parallel
+    // entries summed in FP64
+    constexpr int64_t RCH = 4096;
+    const int64_t nrch = (n + RCH - 1) / RCH;
+    std::vector<int64_t> uniq(n + 1, 0);
+    parallel_for(nrch, nth, [&](int64_t ch) {
+      for (int64_t i = ch * RCH; i < std::min<int64_t>(n, (ch + 1) * RCH); i++) {
+        auto bgn = ent.begin() + deg[i], en = ent.begin() + deg[i + 1];
+        std::sort(bgn, en);
+        int64_t u = 0;
+        for (auto it = bgn; it != en; ++it)
+          if (it == bgn || it->first != (it - 1)->first) u++;
+        uniq[i + 1] = u;
+      }
+    });
+    for (int32_t i = 0; i < n; i++) uniq[i + 1] += uniq[i];
     Csr cs;
     cs.n = n;
     cs.end.assign(n, 0);
-    int64_t pos = 0;
-    for (int32_t i = 0; i < n; i++) {
-      auto bgn = ent.begin() + deg[i], en = ent.begin() + deg[i + 1];
-      std::sort(bgn, en, [](const std::pair<int32_t, float>& x, const std::pair<int32_t, float>& y) {
-        return x.first < y.first;
-      });
-      for (auto it = bgn; it != en;) {
-        int32_t k = it->first;
-        double s = 0;
-        while (it != en && it->first == k) { s += it->second; ++it; }
-        cs.nbr.push_back(k);
-        cs.w.push_back((float)s);
-        pos++;
+    cs.nbr.resize(uniq[n]);
+    cs.w.resize(uniq[n]);
+    parallel_for(nrch, nth, [&](int64_t ch) {
+      for (int64_t i = ch * RCH; i < std::min<int64_t>(n, (ch + 1) * RCH); i++) {
+        auto bgn = ent.begin() + deg[i], en = ent.begin() + deg[i + 1];
+        int64_t pos = uniq[i];
+        for (auto it = bgn; it != en;) {
+          int32_t k = it->first;
+          double sum = 0;
+          while (it != en && it->first == k) { sum += it->second; ++it; }
+          cs.nbr[pos] = k;
+          cs.w[pos] = (float)sum;
+          pos++;
+        }
+        cs.end[i] = uniq[i + 1];
       }
-      cs.end[i] = pos;
-    }
+    });
     g_cache.key = kb;
     g_cache.csr = std::move(cs);
   }
