@@ -7,10 +7,12 @@ on the graph already resident in HBM (the input CSR stays read-only on the
 device between steps).  value = edge-TEPS = undirected input edges / T_e2e.
 
 Also reported:
-  roofline      decide-and-move (the dominant phase) against HBM peak, from the
-                algorithmic bytes of the sweeps (16*d + 40 per swept vertex of
-                degree d, 12*d + 40 when unit weights are implicit) over the
-                phase time measured with HIP events on the engine's stream
+  roofline      the dominant decide-and-move kernel (largest total time in a run
+                with per-bin HIP events on the engine's stream) against HBM
+                peak: its algorithmic bytes (16*d + 40 per swept vertex of
+                degree d, 12*d + 40 with implicit unit weights; first sweeps
+                12*d + 40 / 8*d + 40) over its launch time; the whole decide
+                phase is reported beside it
   cpu_baseline  the oracle (exact-integer C port of the same algorithm, OpenMP)
                 on the same graph on the host, 1 run
   parity        final partition of the last GPU step == oracle's, bit for bit
@@ -32,6 +34,16 @@ sys.path.insert(0, os.path.join(ROOT, "gala-mg_amd"))
 
 HBM_PEAK_GBS = 8000.0
 BIN_DEG = ["1-3", "4-7", "8-15", "16-31", "32-127", "128-511", "512-4095", "4096-8191", ">=8192"]
+# decide-and-move kernel of every degree bin (integer path): general sweeps,
+# and the first sweep of a round
+KERNEL_GENERAL = ["k_decide_ring<4, IntW>", "k_decide_ring<8, IntW>", "k_decide_ring<16, IntW>",
+                  "k_decide_ring<32, IntW>", "k_decide_wave<256, IntW>", "k_decide_wave<1024, IntW>",
+                  "k_decide_block<256, 8192, IntW>", "k_decide_block<1024, 16384, IntW>",
+                  "k_decide_hub<16384, IntW>"]
+KERNEL_FIRST = ["k_decide_first<4, IntW>", "k_decide_first<8, IntW>", "k_decide_first<16, IntW>",
+                "k_decide_first<32, IntW>", "k_decide_first<64, IntW>", "k_decide_first<64, IntW>",
+                "k_decide_first_block<256, IntW>", "k_decide_first_block<1024, IntW>",
+                "k_decide_first_block<1024, IntW>"]
 
 
 def log(msg):
@@ -180,6 +192,7 @@ def main():
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batched leg (0: skip it)")
     ap.add_argument("--rmat-scale", type=int, default=24, help="scale of the weighted R-MAT leg (0: skip it)")
     ap.add_argument("--rmat-oracle", action="store_true", help="check the R-MAT leg against the FP64 oracle")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-frontier leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on exactly one GPU (asked for {args.gpus}); "
@@ -216,7 +229,7 @@ def main():
     wall = time.perf_counter() - t1
     comm = eng.communities(g.vertex_num)
     eng.close()
-    learned = learned_leg(gala, g, args, runs[-1])
+    learned = None if args.no_learned else learned_leg(gala, g, args, runs[-1])
     batched = batch_leg(gala, args) if args.batch_graphs > 0 else None
     rmat = rmat_leg(gala, args) if args.rmat_scale > 0 else None
     # one more run with per-bin HIP events (outside the timed steps)
@@ -226,9 +239,17 @@ def main():
     peng.close()
     bins = []
     for b in range(9):
-        ms_b, by = prof["bin_decide_ms"][b], prof["bin_decide_bytes"][b]
-        bins.append({"bin": b, "degrees": BIN_DEG[b], "bytes": int(by), "entries": int(prof["bin_swept_edges"][b]),
-                     "ms": ms_b, "GBps": (by / ms_b / 1e6) if ms_b > 0 else 0.0})
+        for kind, name, ms_b, by, ent, nl in (
+                ("general", KERNEL_GENERAL[b], prof["bin_decide_ms"][b], prof["bin_decide_bytes"][b],
+                 prof["bin_swept_edges"][b], prof["bin_launches"][b]),
+                ("first", KERNEL_FIRST[b], prof["bin_first_ms"][b], prof["bin_first_bytes"][b],
+                 prof["bin_first_edges"][b], prof["bin_first_launches"][b])):
+            bins.append({"bin": b, "degrees": BIN_DEG[b], "sweep": kind, "kernel": name, "bytes": int(by),
+                         "entries": int(ent), "launches": int(nl), "ms": ms_b,
+                         "avg_launch_us": 1e3 * ms_b / nl if nl else 0.0,
+                         "GBps": (by / ms_b / 1e6) if ms_b > 0 else 0.0})
+    # the dominant decide kernel: largest total time over one run
+    dom = max(bins, key=lambda x: x["ms"])
 
     ms = float(np.mean([r["t_total_ms"] for r in runs]))
     wall_ms = wall * 1e3 / args.steps
@@ -273,10 +294,13 @@ def main():
                    "vertices": int(g.vertex_num), "undirected_edges": int(edges),
                    "gamma": 2.1, "mean_degree": 17.35, "degree_cap": 15000, "seed": args.seed,
                    "weights": "unit (int32 CSR)", "mode": "plain"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "decide-and-move (all degree bins)",
-                     "algorithmic_bytes": int(dm_bytes), "time_ms": last["t_decide_ms"]},
+        "roofline": {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": dom["kernel"], "bin": dom["bin"], "sweep": dom["sweep"],
+                     "launches": dom["launches"], "bytes_per_launch": dom["bytes"] / max(dom["launches"], 1),
+                     "avg_launch_us": dom["avg_launch_us"],
+                     "decide_phase": {"achieved": achieved, "frac": achieved / HBM_PEAK_GBS,
+                                      "algorithmic_bytes": int(dm_bytes), "time_ms": last["t_decide_ms"]}},
         "cpu_baseline": cpu,
         "seconds_e2e": ms * 1e-3,
         "wall_ms_per_step": wall_ms,

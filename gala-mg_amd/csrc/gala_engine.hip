@@ -560,12 +560,20 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     const int64_t per_edge = (first ? 12 : 16) - (g.w ? 0 : 4);
     ctx->res.decide_bytes += per_edge * (int64_t)out->swept_edges + 40 * (int64_t)out->swept;
     for (int b = 0; b < NBINS; b++) {
-      ctx->res.bin_decide_bytes[b] += per_edge * (int64_t)out->bin_edges[b] + 40 * (int64_t)out->bin_swept[b];
-      ctx->res.bin_swept_edges[b] += (int64_t)out->bin_edges[b];
-      if (prof) {
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ctx->bev[b], ctx->bev[b + 1]);
+      const int64_t by = per_edge * (int64_t)out->bin_edges[b] + 40 * (int64_t)out->bin_swept[b];
+      float ms = 0;
+      if (prof) (void)hipEventElapsedTime(&ms, ctx->bev[b], ctx->bev[b + 1]);
+      const int launched = caps[b] > 0 ? 1 : 0;  // the decide launchers skip empty bins
+      if (first) {
+        ctx->res.bin_first_bytes[b] += by;
+        ctx->res.bin_first_edges[b] += (int64_t)out->bin_edges[b];
+        ctx->res.bin_first_ms[b] += ms;
+        ctx->res.bin_first_launches[b] += launched;
+      } else {
+        ctx->res.bin_decide_bytes[b] += by;
+        ctx->res.bin_swept_edges[b] += (int64_t)out->bin_edges[b];
         ctx->res.bin_decide_ms[b] += ms;
+        ctx->res.bin_launches[b] += launched;
       }
     }
     std::swap(fcur, fnext);

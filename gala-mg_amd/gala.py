@@ -61,7 +61,10 @@ class Result(ctypes.Structure):
                 ("learned_sweeps", ctypes.c_int32), ("certify_sweeps", ctypes.c_int32),
                 ("bin_decide_bytes", ctypes.c_int64 * 9), ("bin_swept_edges", ctypes.c_int64 * 9),
                 ("bin_decide_ms", ctypes.c_double * 9),
-                ("learned_frontier", ctypes.c_int64), ("learned_kept", ctypes.c_int64)]
+                ("learned_frontier", ctypes.c_int64), ("learned_kept", ctypes.c_int64),
+                ("bin_first_bytes", ctypes.c_int64 * 9), ("bin_first_edges", ctypes.c_int64 * 9),
+                ("bin_first_ms", ctypes.c_double * 9), ("bin_launches", ctypes.c_int32 * 9),
+                ("bin_first_launches", ctypes.c_int32 * 9)]
 
     def to_dict(self):
         r = self.rounds
@@ -82,7 +85,10 @@ class Result(ctypes.Structure):
                     learned_sweeps=self.learned_sweeps, certify_sweeps=self.certify_sweeps,
                     bin_decide_bytes=list(self.bin_decide_bytes), bin_swept_edges=list(self.bin_swept_edges),
                     bin_decide_ms=list(self.bin_decide_ms),
-                    learned_frontier=self.learned_frontier, learned_kept=self.learned_kept)
+                    learned_frontier=self.learned_frontier, learned_kept=self.learned_kept,
+                    bin_first_bytes=list(self.bin_first_bytes), bin_first_edges=list(self.bin_first_edges),
+                    bin_first_ms=list(self.bin_first_ms), bin_launches=list(self.bin_launches),
+                    bin_first_launches=list(self.bin_first_launches))
 
 
 ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,

@@ -100,6 +100,13 @@ typedef struct {
   double bin_decide_ms[9];     /* per-bin decide time (profile_bins=1), HIP events */
   int64_t learned_frontier;    /* plain-frontier vertices at the learned sweeps */
   int64_t learned_kept;        /* of those, vertices the network kept (swept) */
+  /* The bin_* arrays above count the general sweeps; these the first sweep of
+   * each round (singleton partition, streaming-argmax kernels). */
+  int64_t bin_first_bytes[9];
+  int64_t bin_first_edges[9];
+  double bin_first_ms[9];      /* profile_bins=1 */
+  int32_t bin_launches[9];     /* decide launches per bin, general sweeps */
+  int32_t bin_first_launches[9];
 } gala_result;
 
 /* Version string of the library. */
