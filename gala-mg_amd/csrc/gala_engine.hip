@@ -74,6 +74,7 @@ struct gala_ctx {
   uint8_t* dirty = nullptr;
   int8_t* binid = nullptr;
   int32_t *order = nullptr, *front = nullptr, *fin = nullptr;
+  void* cv = nullptr;  // community view, 8 (int) or 16 (FP) bytes per vertex
   int32_t *bstart_full = nullptr, *bstart_front = nullptr;
   int64_t* hub_off = nullptr;
   int64_t* hub_tmp = nullptr;
@@ -141,6 +142,13 @@ struct gala_ctx {
   std::vector<int32_t> b_comm_local;   // per-vertex community within its graph
   std::vector<double> b_q;
   std::vector<int32_t> b_nc, b_rd, b_it;
+
+  // decide kernel variant per degree bin, general and first sweeps; the
+  // defaults are what gala_autotune picks on the 4M-vertex power-law graph
+  // (pipelined kernels for degrees < 128)
+  int dvar[NBINS] = {2, 2, 2, 2, 3, 0, 0, 0, 0};
+  int fvar[NBINS] = {1, 1, 1, 1, 1, 0, 0, 0, 0};
+  int wvar[NBINS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // weight update (1 = pipelined, bins 0-4)
 
   std::vector<void*> owned;
   int32_t cap_n = -1;  // sizes the owned buffers were allocated for
@@ -223,6 +231,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->size, N); OWN(ctx->dsize, N); OWN(ctx->prev, N); OWN(ctx->cur, N);
   OWN(ctx->active, N); OWN(ctx->dirty, N); OWN(ctx->binid, N);
   OWN(ctx->order, N); OWN(ctx->front, N); OWN(ctx->fin, N);
+  { char* p; OWN(p, N * 16); ctx->cv = p; }
   OWN(ctx->last_gain, N); OWN(ctx->moved_it, N); OWN(ctx->nbr_moved, N);
   OWN(ctx->front2, N); OWN(ctx->bstart2, NBINS + 1); OWN(ctx->keep, N);
   OWN(ctx->front3, N); OWN(ctx->bstart3, NBINS + 1); OWN(ctx->size_snap, N);
@@ -422,6 +431,7 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.last_gain = ctx->last_gain;
   st.moved_it = ctx->moved_it;
   st.nbr_moved = ctx->nbr_moved;
+  st.cv = (typename T::cv_t*)ctx->cv;
   return st;
 }
 
@@ -492,7 +502,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   // Iteration 0: Q on the singleton partition (louvain_main_process.cu:538-548)
   reset_counters(ctx->ctr, (long long)m2, m2d, s);
   launch_apply<T>(st, g.n, m2, s);
-  for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], false, s);
+  for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], false, s, ctx->wvar[b]);
   launch_prune<T>(st, g.n, m2, s);
   Counters c;
   if ((rc = read_counters<T>(ctx, &c))) return rc;
@@ -524,12 +534,15 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   std::vector<float> hfeat;
   std::vector<uint8_t> hlab;
   // decide-and-move over one frontier, timed into t_dm (and per bin)
-  auto decide = [&](DevState<T>& sd, const int32_t* fr, const int32_t* bs, bool first) -> int {
+  // build_cv: refresh the community view first (a certification sweep reads
+  // the view of the pre-sweep state, which is still in place)
+  auto decide = [&](DevState<T>& sd, const int32_t* fr, const int32_t* bs, bool first, bool build_cv) -> int {
     HIPOK(hipEventRecord(ctx->ev[0], s));
+    if (build_cv) launch_build_cv<T>(sd, g.n, s);
     for (int b = 0; b < NBINS; b++) {
       if (prof) HIPOK(hipEventRecord(ctx->bev[b], s));
-      if (first) launch_decide_first<T>(b, g, sd, fr, bs, caps[b], m2, s);
-      else launch_decide<T>(b, g, sd, fr, bs, caps[b], m2, s);
+      if (first) launch_decide_first<T>(b, g, sd, fr, bs, caps[b], m2, s, ctx->fvar[b]);
+      else launch_decide<T>(b, g, sd, fr, bs, caps[b], m2, s, ctx->dvar[b]);
     }
     if (prof) HIPOK(hipEventRecord(ctx->bev[NBINS], s));
     HIPOK(hipEventRecord(ctx->ev[1], s));
@@ -540,7 +553,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   auto update = [&](DevState<T>& sp, bool first, Counters* out) -> int {
     launch_apply<T>(st, g.n, m2, s);
     // In changes only where a vertex or one of its neighbours moved
-    for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s);
+    for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s, ctx->wvar[b]);
     launch_prune<T>(sp, g.n, m2, s);
     partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, fnext, bnext, ctx->part_tmp, s);
     HIPOK(hipEventRecord(ctx->ev[2], s));
@@ -598,7 +611,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     }
     const bool log_it = logging && !lsweep && it >= 2;
     if (log_it) launch_features<T>(g, st, fcur, nfront, it, ctx->feat, s);
-    if ((rc = decide(st, lsweep ? ctx->front2 : fcur, lsweep ? ctx->bstart2 : bcur, first))) return rc;
+    if ((rc = decide(st, lsweep ? ctx->front2 : fcur, lsweep ? ctx->bstart2 : bcur, first, true))) return rc;
     if (log_it) {
       launch_labels(fcur, nfront, ctx->moved_it, it, ctx->lab, s);
       hfeat.resize((size_t)nfront * MLP_IN);
@@ -629,7 +642,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       DevState<T> sc = st;
       sc.Tot = (tt*)ctx->tot_snap;
       sc.size = ctx->size_snap;
-      if ((rc = decide(sc, ctx->front2, ctx->bstart2, false))) return rc;
+      if ((rc = decide(sc, ctx->front2, ctx->bstart2, false, false))) return rc;
       DevState<T> sp = st;
       sp.track = 0;  // keep the neighbour-moved fractions of the learned sweep's rows
       if ((rc = update(sp, false, &c))) return rc;
@@ -1020,7 +1033,7 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     // iteration 0: the singleton partition's Q of every running graph
     launch_batch_reset(ctx->ctr, m2max, nlive, s);
     launch_apply<T>(st, g.n, m2max, s);
-    for (int k = 0; k < NBINS; k++) launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], false, s);
+    for (int k = 0; k < NBINS; k++) launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], false, s, ctx->wvar[k]);
     launch_prune<T>(st, g.n, m2max, s);
     launch_graph_sums(gid, ctx->binid, (const int32_t*)ctx->In, (const int32_t*)ctx->Tot, g.n, b.g_in, b.g_tot2, s);
     launch_batch_step(b, it_dev, min_mod, max_it, s);
@@ -1030,12 +1043,13 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     // one iteration for every running graph; a graph that stopped is inert
     auto body = [&](bool first) {
       launch_batch_reset(ctx->ctr, m2max, nlive, s);
+      launch_build_cv<T>(st, g.n, s);
       for (int k = 0; k < NBINS; k++) {
-        if (first) launch_decide_first<T>(k, g, st, ctx->front, ctx->bstart_front, caps[k], m2max, s);
-        else launch_decide<T>(k, g, st, ctx->front, ctx->bstart_front, caps[k], m2max, s);
+        if (first) launch_decide_first<T>(k, g, st, ctx->front, ctx->bstart_front, caps[k], m2max, s, ctx->fvar[k]);
+        else launch_decide<T>(k, g, st, ctx->front, ctx->bstart_front, caps[k], m2max, s, ctx->dvar[k]);
       }
       launch_apply<T>(st, g.n, m2max, s);
-      for (int k = 0; k < NBINS; k++) launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], true, s);
+      for (int k = 0; k < NBINS; k++) launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], true, s, ctx->wvar[k]);
       launch_prune<T>(st, g.n, m2max, s);
       launch_graph_sums(gid, ctx->binid, (const int32_t*)ctx->In, (const int32_t*)ctx->Tot, g.n, b.g_in, b.g_tot2,
                         s);
@@ -1229,6 +1243,78 @@ int gala_run(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gal
   return is_fp(ctx) ? run_all<FltW>(ctx, icb, rcb, user, out) : run_all<IntW>(ctx, icb, rcb, user, out);
 }
 
+static int weight_variants(int bin) { return bin <= 4 ? 2 : 1; }
+
+int gala_set_variants(gala_ctx* ctx, const int32_t* variants) {
+  if (!ctx || !variants) return GALA_E_INVALID;
+  for (int b = 0; b < NBINS; b++) {
+    if (variants[b] < 0 || variants[b] >= decide_variants(b)) return GALA_E_INVALID;
+    if (variants[NBINS + b] < 0 || variants[NBINS + b] >= decide_first_variants(b)) return GALA_E_INVALID;
+    if (variants[2 * NBINS + b] < 0 || variants[2 * NBINS + b] >= weight_variants(b)) return GALA_E_INVALID;
+  }
+  for (int b = 0; b < NBINS; b++) {
+    ctx->dvar[b] = variants[b];
+    ctx->fvar[b] = variants[NBINS + b];
+    ctx->wvar[b] = variants[2 * NBINS + b];
+  }
+  return GALA_OK;
+}
+
+int gala_get_variants(gala_ctx* ctx, int32_t* variants) {
+  if (!ctx || !variants) return GALA_E_INVALID;
+  for (int b = 0; b < NBINS; b++) {
+    variants[b] = ctx->dvar[b];
+    variants[NBINS + b] = ctx->fvar[b];
+    variants[2 * NBINS + b] = ctx->wvar[b];
+  }
+  return GALA_OK;
+}
+
+int gala_autotune(gala_ctx* ctx, int32_t* variants_out) {
+  if (!ctx || !ctx->have_graph || ctx->n0 == 0) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  // one full plain run per variant index with per-bin HIP events; every
+  // variant takes the same decisions, so the runs are identical but for time.
+  // Decide kernels are chosen per bin; the weight-update kernels, timed as a
+  // phase, are chosen all together.
+  const gala_opts saved = ctx->opts;
+  ctx->opts.profile_bins = 1;
+  ctx->opts.frontier_mode = 0;
+  int nmax = 1;
+  for (int b = 0; b < NBINS; b++)
+    nmax = std::max(nmax, std::max(decide_variants(b), std::max(decide_first_variants(b), weight_variants(b))));
+  double best_d[NBINS], best_f[NBINS], best_w = 1e300;
+  int bd[NBINS], bf[NBINS], bw = 0;
+  for (int b = 0; b < NBINS; b++) { best_d[b] = best_f[b] = 1e300; bd[b] = bf[b] = 0; }
+  int rc = GALA_OK;
+  for (int k = 0; k < nmax && rc == GALA_OK; k++) {
+    for (int b = 0; b < NBINS; b++) {
+      ctx->dvar[b] = std::min(k, decide_variants(b) - 1);
+      ctx->fvar[b] = std::min(k, decide_first_variants(b) - 1);
+      ctx->wvar[b] = std::min(k, weight_variants(b) - 1);
+    }
+    gala_result r;
+    for (int rep = 0; rep < 2 && rc == GALA_OK; rep++)  // the second run is timed
+      rc = is_fp(ctx) ? run_all<FltW>(ctx, nullptr, nullptr, nullptr, &r) : run_all<IntW>(ctx, nullptr, nullptr, nullptr, &r);
+    if (rc) break;
+    for (int b = 0; b < NBINS; b++) {
+      if (k < decide_variants(b) && r.bin_decide_ms[b] < best_d[b]) { best_d[b] = r.bin_decide_ms[b]; bd[b] = k; }
+      if (k < decide_first_variants(b) && r.bin_first_ms[b] < best_f[b]) { best_f[b] = r.bin_first_ms[b]; bf[b] = k; }
+    }
+    if (k < 2 && r.t_update_ms < best_w) { best_w = r.t_update_ms; bw = k; }
+  }
+  ctx->opts = saved;
+  for (int b = 0; b < NBINS; b++) {
+    ctx->dvar[b] = bd[b];
+    ctx->fvar[b] = bf[b];
+    ctx->wvar[b] = std::min(bw, weight_variants(b) - 1);
+  }
+  ctx->have_result = false;
+  if (rc) return rc;
+  if (variants_out) gala_get_variants(ctx, variants_out);
+  return GALA_OK;
+}
+
 int gala_batch_set(gala_ctx* ctx, int32_t n_graphs, const int32_t* vstart, int32_t n, const int64_t* end_incl,
                    const int32_t* nbr, const void* w, int weight_mode) {
   if (!ctx) return GALA_E_INVALID;
@@ -1298,7 +1384,9 @@ int gala_phase_decide(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const i
   partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp, s);
   DevState<IntW> st = dstate<IntW>(ctx);
   reset_counters(ctx->ctr, m2, (double)m2, s);
-  for (int b = 0; b < NBINS; b++) launch_decide<IntW>(b, g, st, ctx->front, ctx->bstart_front, caps[b], (int32_t)m2, s);
+  launch_build_cv<IntW>(st, n, s);
+  for (int b = 0; b < NBINS; b++)
+    launch_decide<IntW>(b, g, st, ctx->front, ctx->bstart_front, caps[b], (int32_t)m2, s, ctx->dvar[b]);
   Counters c;
   if ((rc = read_counters<IntW>(ctx, &c))) return rc;
   HIPOK(hipMemcpy(cur_out, ctx->cur, 4ll * n, hipMemcpyDeviceToHost));
@@ -1329,7 +1417,7 @@ int gala_phase_weight(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const i
   // reference's zero-initialised In and fill(active, 1) leave them
   fill<int32_t>((int32_t*)ctx->In, n, 0, s);
   fill<uint8_t>(ctx->active, n, 1, s);
-  for (int b = 0; b < NBINS; b++) launch_weight<IntW>(b, g, st, ctx->order, ctx->bstart_full, caps[b], false, s);
+  for (int b = 0; b < NBINS; b++) launch_weight<IntW>(b, g, st, ctx->order, ctx->bstart_full, caps[b], false, s, ctx->wvar[b]);
   launch_prune<IntW>(st, n, (int32_t)m2, s);
   Counters c;
   if ((rc = read_counters<IntW>(ctx, &c))) return rc;

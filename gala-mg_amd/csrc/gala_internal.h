@@ -66,12 +66,27 @@ struct Counters {
   unsigned long long bin_edges[NBINS];
 };
 
+// Community view of a vertex for one sweep: its community, that community's
+// size and Tot, packed into 16 bytes so one gather gives all three.
+struct alignas(16) CvI {
+  int32_t c;
+  int32_t size;
+  int32_t tot;
+  int32_t pad;
+};
+struct alignas(16) CvF {
+  int32_t c;
+  int32_t size;
+  double tot;
+};
+
 // Weight traits: exact integer path and FP64-accumulating float path.
 struct IntW {
   using w_t = int32_t;     // edge weight as stored
   using acc_t = int32_t;   // per-community neighbour weight sum (<= K < 2^31)
   using tot_t = int32_t;   // K, Self, Tot, In (all <= m2 < 2^31)
   using gain_t = long long;
+  using cv_t = CvI;
   static constexpr bool FP = false;
 };
 struct FltW {
@@ -80,6 +95,7 @@ struct FltW {
   using acc_t = double;
   using tot_t = double;
   using gain_t = double;
+  using cv_t = CvF;
   static constexpr bool FP = true;
 };
 

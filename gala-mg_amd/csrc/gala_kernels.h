@@ -55,6 +55,10 @@ struct DevState {
   // every graph; null for a single graph
   const int32_t* gid = nullptr;
   const typename T::tot_t* m2g = nullptr;
+  // community view {prev[u], size[prev[u]], Tot[prev[u]]} of every vertex,
+  // rebuilt before each sweep (launch_build_cv); the ring, per-group hash and
+  // pipelined kernels read it
+  typename T::cv_t* cv = nullptr;
 };
 
 // Per-graph state of a batch (one entry per graph; integer path).
@@ -79,24 +83,32 @@ void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s);
 
 // decide-and-move for one bin over front[bstart[b] .. bstart[b+1]) (device counts);
 // `cap` = host upper bound of that range (used only to size the grid).
+// cv[u] = {prev[u], size[prev[u]], Tot[prev[u]]} for u < n, before a sweep.
+template <class T>
+void launch_build_cv(DevState<T>& st, int32_t n, hipStream_t s);
+
+// variant: kernel variant of the bin (0 = default); see decide_variants().
 template <class T>
 void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
-                   const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s);
+                   const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s, int variant = 0);
+int decide_variants(int bin);
+int decide_first_variants(int bin);
 
 // First sweep of a round (singleton partition, duplicate-free rows): no
 // aggregation needed, every entry is its own candidate community.
 template <class T>
 void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
-                         const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s);
+                         const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s, int variant = 0);
 
 // Tot += dTot, size += dsize, sums of Tot^2, min Tot, non-empty count.
 template <class T>
 void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);
 
 // In[v] over order[bstart[b] .. bstart[b+1]) (only the dirty vertices when only_dirty).
+// variant 1: pipelined kernels for bins 0-4 (same result).
 template <class T>
 void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
-                   const int32_t* bstart, int cap, bool only_dirty, hipStream_t s);
+                   const int32_t* bstart, int cap, bool only_dirty, hipStream_t s, int variant = 0);
 // prune flags, sum In, active count; clears the dirty marks.
 template <class T>
 void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);

@@ -113,6 +113,34 @@ __device__ __forceinline__ bool commit_move(int v, int c, typename T::gain_t bes
   return false;
 }
 
+// commit_move with the sizes of the best and the own community already at hand
+// (from the community view).
+template <class T>
+__device__ __forceinline__ bool commit_move_sz(int v, int c, typename T::gain_t bestH, int bestD, int szD,
+                                               int szC, typename T::acc_t wc, typename T::tot_t K,
+                                               typename T::tot_t S, typename T::tot_t m2, const DevState<T>& st) {
+  using G = typename T::gain_t;
+  int nc = c;
+  G gain = -(G)m2 * ((G)wc - (G)S);
+  if (bestD != INT_MAX) {
+    gain = gain_net<T>(bestH, (G)m2, (G)wc, (G)S);
+    if (gain > 0 && !(szD == 1 && szC == 1 && bestD > c)) nc = bestD;
+  }
+  st.cur[v] = nc;
+  if (st.track) {
+    st.last_gain[v] = (float)((double)gain / ((double)m2 * (double)K));
+    if (nc != c) st.moved_it[v] = st.iter;
+  }
+  if (nc != c) {
+    atomicAdd(&st.dTot[nc], K);
+    atomicAdd(&st.dTot[c], -K);
+    atomicAdd(&st.dsize[nc], 1);
+    atomicAdd(&st.dsize[c], -1);
+    return true;
+  }
+  return false;
+}
+
 // A mover invalidates In of itself and of every neighbour (plain byte stores:
 // every writer stores 1).
 __device__ __forceinline__ void mark_row(uint8_t* dirty, const int32_t* __restrict__ nbr, int64_t o, int d,
@@ -202,6 +230,28 @@ void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// community view for the sweep: cv[u] = {prev[u], size[prev[u]], Tot[prev[u]]}
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_build_cv(DevState<T> st, int32_t n) {
+  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const int c = st.prev[u];
+    typename T::cv_t e;
+    e.c = c;
+    e.size = st.size[c];
+    if constexpr (!T::FP) e.pad = 0;
+    e.tot = st.Tot[c];
+    st.cv[u] = e;
+  }
+}
+
+template <class T>
+void launch_build_cv(DevState<T>& st, int32_t n, hipStream_t s) {
+  if (n <= 0) return;
+  k_build_cv<T><<<std::min((n + 255) / 256, 8192), 256, 0, s>>>(st, n);
+}
+
+// ---------------------------------------------------------------------------
 // decide-and-move, sub-wave ring kernels (degree < 32)
 // Each group of W lanes owns one vertex, lane l holds entry l.  A xor
 // exchange over every other lane of the group sums the weight of equal
@@ -229,12 +279,16 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
     const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
-    const int c = st.prev[v];
+    const typename T::cv_t ev = st.cv[v];
+    const int c = ev.c;
     int cl = -1, ul = 0;
     acc wl = 0;
+    tt tl = 0;
     if (lane < d) {
       ul = g.nbr[o + lane];
-      cl = st.prev[ul];
+      const typename T::cv_t eu = st.cv[ul];
+      cl = eu.c;
+      tl = eu.tot;
       wl = (acc)load_w<T>(g.w, o + lane);
     }
     acc s = wl, wc = (cl == c) ? wl : acc(0);
@@ -250,11 +304,11 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
       if (co == c) wc += wo;
     }
     const tt K = st.K[v];
-    const tt Tc = st.Tot[c];
+    const tt Tc = ev.tot;
     G h = 0;
     int bd = INT_MAX;
     if (leader && cl != c) {
-      G hh = gain_h<T>((G)s, (G)m2, (G)K, (G)st.Tot[cl], (G)Tc);
+      G hh = gain_h<T>((G)s, (G)m2, (G)K, (G)tl, (G)Tc);
       if (hh > 0) { h = hh; bd = cl; }
     }
     argmax_xor<G, W>(h, bd);
@@ -286,12 +340,319 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
 }
 
 // ---------------------------------------------------------------------------
-// decide-and-move, one wave per vertex with a per-wave LDS hash (32..511)
+// decide-and-move, pipelined sub-wave ring kernels (degree < 32)
+// Same decision as k_decide_ring.  The kernels are bound by the latency of a
+// chain of dependent loads per vertex (front -> offsets -> row -> community
+// view), so each W-lane group keeps four vertices in flight: in one iteration
+// it decides the oldest while the loads of the three behind it, one chain link
+// each, are outstanding.  The community view carries the sizes the singleton
+// guard needs, so the commit waits on nothing.
+// ---------------------------------------------------------------------------
+template <int W, bool FIRST, class T>
+__global__ __launch_bounds__(256) void k_decide_pipe(DevGraph<T> g, DevState<T> st,
+                                                      const int32_t* __restrict__ front,
+                                                      const int32_t* __restrict__ bstart, int bin,
+                                                      typename T::tot_t m2_all) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  using CV = typename T::cv_t;
+  __shared__ unsigned long long s_moved, s_swept, s_edges;
+  if (threadIdx.x == 0) { s_moved = 0; s_swept = 0; s_edges = 0; }
+  __syncthreads();
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & (W - 1);
+  const int stride = gridDim.x * (blockDim.x / W);
+  unsigned moved = 0, swept = 0, edges = 0;
+  // stage A: vertex id loaded
+  int iA = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W;
+  bool okA = iA < hi;
+  int vA = okA ? front[iA] : 0;
+  // stage B: row header and own view loaded
+  bool okB = false;
+  int vB = 0, dB = 0;
+  int64_t oB = 0;
+  CV evB{};
+  tt KB = 0, SB = 0;
+  // stage C: row entry loaded
+  bool okC = false;
+  int vC = 0, dC = 0, uC = 0;
+  CV evC{};
+  tt KC = 0, SC = 0;
+  acc wC = 0;
+  // stage D: neighbour's community view loaded
+  bool okD = false;
+  int vD = 0, dD = 0, uD = 0;
+  CV evD{}, euD{};
+  tt KD = 0, SD = 0;
+  acc wD = 0;
+  while (okA || okB || okC || okD) {  // uniform per group
+    if (okD) {
+      const typename T::tot_t m2 = vertex_m2<T>(st, vD, m2_all);
+      const int c = evD.c;
+      const bool has = lane < dD;
+      const int cl = has ? euD.c : -1;
+      const acc wl = has ? wD : acc(0);
+      acc s = wl, wc = (cl == c) ? wl : acc(0);
+      bool leader = has;
+      if constexpr (FIRST) {
+        // singleton partition with duplicate-free rows: every entry is its own
+        // community, and the own community's weight is the self-loop
+        wc = (acc)SD;
+      } else {
+#pragma unroll 4
+        for (int k = 1; k < W; k++) {
+          const int co = __shfl_xor(cl, k, W);
+          const acc wo = __shfl_xor(wl, k, W);
+          if (co == cl) {
+            s += wo;
+            if ((lane ^ k) < lane) leader = false;
+          }
+          if (co == c) wc += wo;
+        }
+      }
+      G h = 0;
+      int bd = INT_MAX, bsz = 0;
+      if (leader && cl != c) {
+        const G hh = gain_h<T>((G)s, (G)m2, (G)KD, (G)euD.tot, (G)evD.tot);
+        if (hh > 0) { h = hh; bd = cl; bsz = euD.size; }
+      }
+#pragma unroll
+      for (int k = W / 2; k >= 1; k >>= 1) {
+        const G ho = __shfl_xor(h, k, W);
+        const int dd = __shfl_xor(bd, k, W);
+        const int so = __shfl_xor(bsz, k, W);
+        if (better(ho, dd, h, bd)) { h = ho; bd = dd; bsz = so; }
+      }
+      int mv = 0;
+      if (lane == 0) {
+        swept++;
+        edges += dD;
+        mv = commit_move_sz<T>(vD, c, h, bd, bsz, evD.size, wc, KD, SD, m2, st);
+        moved += mv;
+      }
+      if (st.dirty && __shfl(mv, 0, W)) {
+        if (has) st.dirty[uD] = 1;
+        if (lane == 0) st.dirty[vD] = 1;
+      }
+    }
+    // advance, issuing the load each stage needs for the next iteration
+    okD = okC;
+    if (okC) {
+      vD = vC; dD = dC; uD = uC; evD = evC; KD = KC; SD = SC; wD = wC;
+      if (lane < dC) euD = st.cv[uC];
+    }
+    okC = okB;
+    if (okB) {
+      vC = vB; dC = dB; evC = evB; KC = KB; SC = SB;
+      if (lane < dB) {
+        uC = g.nbr[oB + lane];
+        wC = (acc)load_w<T>(g.w, oB + lane);
+      }
+    }
+    okB = okA;
+    if (okA) {
+      vB = vA;
+      oB = g.off[vA];
+      dB = (int)(g.off[vA + 1] - oB);
+      evB = st.cv[vA];
+      KB = st.K[vA];
+      SB = st.Self[vA];
+      iA += stride;
+      okA = iA < hi;
+      if (okA) vA = front[iA];
+    }
+  }
+  if (lane == 0) {
+    if (moved) atomicAdd(&s_moved, (unsigned long long)moved);
+    if (swept) atomicAdd(&s_swept, (unsigned long long)swept);
+    if (edges) atomicAdd(&s_edges, (unsigned long long)edges);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, s_moved);
+    atomic_add_ull(&st.ctr->swept, s_swept);
+    atomic_add_ull(&st.ctr->swept_edges, s_edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// decide-and-move, pipelined per-group LDS hash (32..511)
+// W lanes per vertex, EPL row entries per lane (degree <= W*EPL), four
+// vertices in flight per group as in k_decide_pipe.  The table records each
+// community's Tot and size from the community view, so the scan and the
+// commit need no further gathers.
+// ---------------------------------------------------------------------------
+template <int TMAX, int W, int EPL, class T>
+__global__ __launch_bounds__(256) void k_decide_hpipe(DevGraph<T> g, DevState<T> st,
+                                                       const int32_t* __restrict__ front,
+                                                       const int32_t* __restrict__ bstart, int bin,
+                                                       typename T::tot_t m2_all) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  using CV = typename T::cv_t;
+  constexpr int GPB = 256 / W;
+  __shared__ int s_keys[GPB][TMAX];
+  __shared__ acc s_vals[GPB][TMAX];
+  __shared__ tt s_tots[GPB][TMAX];
+  __shared__ int s_sizes[GPB][TMAX];
+  __shared__ unsigned long long s_moved, s_swept, s_edges;
+  if (threadIdx.x == 0) { s_moved = 0; s_swept = 0; s_edges = 0; }
+  __syncthreads();
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & (W - 1), grp = threadIdx.x / W;
+  int* keys = s_keys[grp];
+  acc* vals = s_vals[grp];
+  tt* tots = s_tots[grp];
+  int* sizes = s_sizes[grp];
+  const int stride = gridDim.x * GPB;
+  unsigned moved = 0, swept = 0, edges = 0;
+  int iA = lo + blockIdx.x * GPB + grp;
+  bool okA = iA < hi;
+  int vA = okA ? front[iA] : 0;
+  bool okB = false;
+  int vB = 0, dB = 0;
+  int64_t oB = 0;
+  CV evB{};
+  tt KB = 0, SB = 0;
+  bool okC = false;
+  int vC = 0, dC = 0;
+  CV evC{};
+  tt KC = 0, SC = 0;
+  int uC[EPL];
+  acc wC[EPL];
+  bool okD = false;
+  int vD = 0, dD = 0;
+  CV evD{};
+  tt KD = 0, SD = 0;
+  int uD[EPL];
+  acc wD[EPL];
+  CV euD[EPL];
+#pragma unroll
+  for (int k = 0; k < EPL; k++) { uC[k] = 0; wC[k] = 0; uD[k] = 0; wD[k] = 0; euD[k] = CV{}; }
+  while (okA || okB || okC || okD) {  // uniform per group
+    if (okD) {
+      const typename T::tot_t m2 = vertex_m2<T>(st, vD, m2_all);
+      const int c = evD.c;
+      int lg = ilog2_ceil(2 * dD);
+      if (lg < 1) lg = 1;
+      const int tsz = 1 << lg;  // <= TMAX by bin construction
+      for (int t = lane; t < tsz; t += W) { keys[t] = 0; vals[t] = 0; }
+      lds_fence_wave();
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        if (lane + k * W < dD) {
+          const int cl = euD[k].c;
+          unsigned p = hash_slot(cl, lg);
+          for (int probe = 0; probe < tsz; probe++) {
+            const int kk = atomicCAS(&keys[p], 0, cl + 1);
+            if (kk == 0) { tots[p] = euD[k].tot; sizes[p] = euD[k].size; }
+            if (kk == 0 || kk == cl + 1) { atomicAdd(&vals[p], wD[k]); break; }
+            p = (p + 1) & (tsz - 1);
+          }
+        }
+      }
+      lds_fence_wave();
+      G h = 0;
+      int bd = INT_MAX, bsz = 0;
+      acc wc = 0;
+      for (int t = lane; t < tsz; t += W) {
+        const int kk = keys[t];
+        if (kk) {
+          const int dd = kk - 1;
+          const acc a = vals[t];
+          if (dd == c) {
+            wc = a;
+          } else {
+            const G hh = gain_h<T>((G)a, (G)m2, (G)KD, (G)tots[t], (G)evD.tot);
+            if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; bsz = sizes[t]; }
+          }
+        }
+      }
+#pragma unroll
+      for (int k = W / 2; k >= 1; k >>= 1) {
+        const G ho = __shfl_xor(h, k, W);
+        const int dd = __shfl_xor(bd, k, W);
+        const int so = __shfl_xor(bsz, k, W);
+        if (better(ho, dd, h, bd)) { h = ho; bd = dd; bsz = so; }
+      }
+      wc = sum_xor<acc, W>(wc);
+      int mv = 0;
+      if (lane == 0) {
+        swept++;
+        edges += dD;
+        mv = commit_move_sz<T>(vD, c, h, bd, bsz, evD.size, wc, KD, SD, m2, st);
+        moved += mv;
+      }
+      if (st.dirty && __shfl(mv, 0, W)) {
+#pragma unroll
+        for (int k = 0; k < EPL; k++)
+          if (lane + k * W < dD) st.dirty[uD[k]] = 1;
+        if (lane == 0) st.dirty[vD] = 1;
+      }
+      lds_fence_wave();
+    }
+    okD = okC;
+    if (okC) {
+      vD = vC; dD = dC; evD = evC; KD = KC; SD = SC;
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        uD[k] = uC[k];
+        wD[k] = wC[k];
+        if (lane + k * W < dC) euD[k] = st.cv[uC[k]];
+      }
+    }
+    okC = okB;
+    if (okB) {
+      vC = vB; dC = dB; evC = evB; KC = KB; SC = SB;
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        if (lane + k * W < dB) {
+          uC[k] = g.nbr[oB + lane + k * W];
+          wC[k] = (acc)load_w<T>(g.w, oB + lane + k * W);
+        }
+      }
+    }
+    okB = okA;
+    if (okA) {
+      vB = vA;
+      oB = g.off[vA];
+      dB = (int)(g.off[vA + 1] - oB);
+      evB = st.cv[vA];
+      KB = st.K[vA];
+      SB = st.Self[vA];
+      iA += stride;
+      okA = iA < hi;
+      if (okA) vA = front[iA];
+    }
+  }
+  if (lane == 0) {
+    if (moved) atomicAdd(&s_moved, (unsigned long long)moved);
+    if (swept) atomicAdd(&s_swept, (unsigned long long)swept);
+    if (edges) atomicAdd(&s_edges, (unsigned long long)edges);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, s_moved);
+    atomic_add_ull(&st.ctr->swept, s_swept);
+    atomic_add_ull(&st.ctr->swept_edges, s_edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// decide-and-move, W lanes per vertex with a per-group LDS hash (8..511)
 // Phase 1 inserts (community -> summed weight) with LDS CAS/add; phase 2 scans
 // the table so each distinct community is evaluated once (one Tot gather per
-// community rather than per entry).  (build_and_select, kernel_functions.cu:473-593)
+// community rather than per entry).  W < 64 runs 64/W vertices per wave, so a
+// wave keeps more independent row and community gathers in flight.
+// (build_and_select, kernel_functions.cu:473-593)
 // ---------------------------------------------------------------------------
-template <int TMAX, class T>
+template <int TMAX, int W, class T>
 __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> st,
                                                       const int32_t* __restrict__ front,
                                                       const int32_t* __restrict__ bstart, int bin,
@@ -299,45 +660,52 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
   using acc = typename T::acc_t;
   using tt = typename T::tot_t;
   using G = typename T::gain_t;
-  __shared__ int s_keys[4][TMAX];
-  __shared__ acc s_vals[4][TMAX];
+  constexpr int GPB = 256 / W;  // groups per workgroup
+  constexpr int LGMIN = W >= 64 ? 6 : 1;  // tables hold next_pow2(2d) <= TMAX slots
+  __shared__ int s_keys[GPB][TMAX];
+  __shared__ acc s_vals[GPB][TMAX];
+  __shared__ tt s_tots[GPB][TMAX];  // Tot of each slot's community, from the community view
   __shared__ unsigned long long s_moved, s_swept, s_edges;
   if (threadIdx.x == 0) { s_moved = 0; s_swept = 0; s_edges = 0; }
   __syncthreads();
   const int lo = bstart[bin], hi = bstart[bin + 1];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int* keys = s_keys[wid];
-  acc* vals = s_vals[wid];
-  const int nwaves = gridDim.x * 4;
+  const int lane = threadIdx.x & (W - 1), grp = threadIdx.x / W;
+  int* keys = s_keys[grp];
+  acc* vals = s_vals[grp];
+  tt* tots = s_tots[grp];
+  const int ngroups = gridDim.x * GPB;
   unsigned moved = 0, swept = 0, edges = 0;
-  for (int i = lo + blockIdx.x * 4 + wid; i < hi; i += nwaves) {
+  for (int i = lo + blockIdx.x * GPB + grp; i < hi; i += ngroups) {
     const int v = front[i];
     const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
-    const int c = st.prev[v];
+    const typename T::cv_t ev = st.cv[v];
+    const int c = ev.c;
     int lg = ilog2_ceil(2 * d);
-    if (lg < 6) lg = 6;
+    if (lg < LGMIN) lg = LGMIN;
     const int tsz = 1 << lg;  // <= TMAX by bin construction
-    for (int t = lane; t < tsz; t += 64) { keys[t] = 0; vals[t] = 0; }
+    for (int t = lane; t < tsz; t += W) { keys[t] = 0; vals[t] = 0; }
     lds_fence_wave();
-    for (int j = lane; j < d; j += 64) {
-      const int cl = st.prev[g.nbr[o + j]];
+    for (int j = lane; j < d; j += W) {
+      const typename T::cv_t eu = st.cv[g.nbr[o + j]];
+      const int cl = eu.c;
       const acc w = (acc)load_w<T>(g.w, o + j);
       unsigned p = hash_slot(cl, lg);
       for (int probe = 0; probe < tsz; probe++) {
         int k = atomicCAS(&keys[p], 0, cl + 1);
+        if (k == 0) tots[p] = eu.tot;  // the claiming lane records the community's Tot
         if (k == 0 || k == cl + 1) { atomicAdd(&vals[p], w); break; }
         p = (p + 1) & (tsz - 1);
       }
     }
     lds_fence_wave();
     const tt K = st.K[v];
-    const tt Tc = st.Tot[c];
+    const tt Tc = ev.tot;
     G h = 0;
     int bd = INT_MAX;
     acc wc = 0;
-    for (int t = lane; t < tsz; t += 64) {
+    for (int t = lane; t < tsz; t += W) {
       const int k = keys[t];
       if (k) {
         const int dd = k - 1;
@@ -345,13 +713,13 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
         if (dd == c) {
           wc = a;
         } else {
-          G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)st.Tot[dd], (G)Tc);
+          G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)tots[t], (G)Tc);
           if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; }
         }
       }
     }
-    argmax_xor<G, 64>(h, bd);
-    wc = sum_xor<acc, 64>(wc);
+    argmax_xor<G, W>(h, bd);
+    wc = sum_xor<acc, W>(wc);
     int mv = 0;
     if (lane == 0) {
       swept++;
@@ -359,7 +727,7 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
       mv = commit_move<T>(v, c, h, bd, wc, K, st.Self[v], m2, st);
       moved += mv;
     }
-    if (st.dirty && __shfl(mv, 0, 64)) mark_row(st.dirty, g.nbr, o, d, v, lane, 64);
+    if (st.dirty && __shfl(mv, 0, W)) mark_row(st.dirty, g.nbr, o, d, v, lane, W);
     lds_fence_wave();
   }
   if (lane == 0) {
@@ -726,43 +1094,100 @@ static int grid_for(int cap, int per_block, int max_blocks) {
   return std::min(nb, max_blocks);
 }
 
+// Decide kernel variants per bin (every variant takes the same exact decision;
+// they differ only in how lanes are mapped to vertices):
+//   bins 0-3: 0 = xor ring over W lanes, 1 = LDS hash over W lanes,
+//             2 = xor ring over W lanes, four vertices in flight per group
+//   bin 4:    0 = one wave per vertex, 1 = 32 lanes, 2 = 16 lanes,
+//             3 = pipelined, 32 lanes x 4 entries, 4 = pipelined, 16 lanes x 8 entries
+//   bin 5:    0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 64 lanes x 8 entries
+// First sweeps: bins 0-3 have 0 = streaming argmax, 1 = pipelined (four
+// vertices in flight per group); bin 4 has 0 = one wave per vertex, 1 = 32 lanes.
+int decide_variants(int bin) {
+  static const int nv[NBINS] = {3, 3, 3, 3, 5, 3, 1, 1, 1};
+  return bin >= 0 && bin < NBINS ? nv[bin] : 1;
+}
+int decide_first_variants(int bin) { return bin <= 4 ? 2 : 1; }
+
 template <class T>
 void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
-                   const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s) {
+                   const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s, int variant) {
   if (cap <= 0) return;
   constexpr int HUB_LT = T::FP ? 8192 : 16384;
-  switch (bin) {
-    case 0: k_decide_ring<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, 0, m2); break;
-    case 1: k_decide_ring<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, 1, m2); break;
-    case 2: k_decide_ring<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, 2, m2); break;
-    case 3: k_decide_ring<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, 3, m2); break;
-    case 4: k_decide_wave<256, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, 4, m2); break;
-    case 5: k_decide_wave<1024, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, 5, m2); break;
-    case 6: k_decide_block<256, 8192, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, 6, m2); break;
-    case 7:
-      if constexpr (T::FP)
-        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, 7, m2);
+  if (variant < 0 || variant >= decide_variants(bin)) variant = 0;
+  const int b = bin;
+  switch (bin * 8 + variant) {
+    // bin 0: 1..3
+    case 0: k_decide_ring<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 1: k_decide_wave<8, 4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 2: k_decide_pipe<4, false, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    // bin 1: 4..7
+    case 8: k_decide_ring<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 9: k_decide_wave<16, 8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 10: k_decide_pipe<8, false, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    // bin 2: 8..15
+    case 16: k_decide_ring<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 17: k_decide_wave<32, 16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 18: k_decide_pipe<16, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    // bin 3: 16..31
+    case 24: k_decide_ring<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 25: k_decide_wave<64, 32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 26: k_decide_pipe<32, false, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    // bin 4: 32..127
+    case 32: k_decide_wave<256, 64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 33: k_decide_wave<256, 32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 34: k_decide_wave<256, 16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 35: k_decide_hpipe<256, 32, 4, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 36: k_decide_hpipe<256, 16, 8, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    // bin 5: 128..511
+    case 40: k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 41:
+      if constexpr (T::FP)  // 8 groups x 1024 slots of FP64 values and totals exceed the LDS
+        k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       else
-        k_decide_block<1024, 16384, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, 7, m2);
+        k_decide_wave<1024, 32, T><<<grid_for(cap, 8, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       break;
-    case 8: k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, 8, m2); break;
+    case 42:
+      if constexpr (T::FP)
+        k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
+      else
+        k_decide_hpipe<1024, 64, 8, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
+      break;
+    // bin 6: 512..4095
+    case 48: k_decide_block<256, 8192, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    // bin 7: 4096..8191
+    case 56:
+      if constexpr (T::FP)
+        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
+      else
+        k_decide_block<1024, 16384, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
+      break;
+    // bin 8: >= 8192
+    case 64: k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2); break;
   }
 }
 
 template <class T>
 void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
-                         const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s) {
+                         const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s, int variant) {
   if (cap <= 0) return;
-  switch (bin) {
-    case 0: k_decide_first<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, 0, m2); break;
-    case 1: k_decide_first<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, 1, m2); break;
-    case 2: k_decide_first<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, 2, m2); break;
-    case 3: k_decide_first<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, 3, m2); break;
-    case 4: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, 4, m2); break;
-    case 5: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, 5, m2); break;
-    case 6: k_decide_first_block<256, T><<<grid_for(cap, 1, 4096), 256, 0, s>>>(g, st, front, bstart, 6, m2); break;
-    case 7: k_decide_first_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, 7, m2); break;
-    case 8: k_decide_first_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, 8, m2); break;
+  if (variant < 0 || variant >= decide_first_variants(bin)) variant = 0;
+  const int b = bin;
+  switch (bin * 8 + variant) {
+    case 0: k_decide_first<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 1: k_decide_pipe<4, true, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 8: k_decide_first<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 9: k_decide_pipe<8, true, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 16: k_decide_first<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 17: k_decide_pipe<16, true, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 24: k_decide_first<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 25: k_decide_pipe<32, true, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 32: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 33: k_decide_first<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 40: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 48: k_decide_first_block<256, T><<<grid_for(cap, 1, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 56: k_decide_first_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 64: k_decide_first_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2); break;
   }
 }
 
@@ -892,6 +1317,103 @@ __global__ __launch_bounds__(256) void k_weight_group(DevGraph<T> g, DevState<T>
   }
 }
 
+// Pipelined weight update: W lanes x EPL entries per vertex (degree <= W*EPL),
+// five items in flight per group, one link of the load chain each
+// (order -> dirty flag -> offsets and own community -> row -> neighbour
+// communities).  Clean vertices pass through without further loads.
+template <int W, int EPL, class T>
+__global__ __launch_bounds__(256) void k_weight_pipe(DevGraph<T> g, DevState<T> st,
+                                                      const int32_t* __restrict__ order,
+                                                      const int32_t* __restrict__ bstart, int bin,
+                                                      int only_dirty) {
+  using tt = typename T::tot_t;
+  using acc = typename T::acc_t;
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & (W - 1);
+  const int stride = gridDim.x * (blockDim.x / W);
+  const int track = st.track;
+  // A: vertex id
+  int iA = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W;
+  bool okA = iA < hi;
+  int vA = okA ? order[iA] : 0;
+  // B: dirty flag
+  bool okB = false, dyB = false;
+  int vB = 0;
+  // C: row header and own community
+  bool okC = false;
+  int vC = 0, dC = 0, cC = 0;
+  int64_t oC = 0;
+  // D: row entries
+  bool okD = false;
+  int vD = 0, dD = 0, cD = 0;
+  int uD[EPL];
+  acc wD[EPL];
+  // E: neighbour communities
+  bool okE = false;
+  int vE = 0, dE = 0, cE = 0;
+  int cuE[EPL], mtE[EPL];
+  acc wE[EPL];
+#pragma unroll
+  for (int k = 0; k < EPL; k++) { uD[k] = 0; wD[k] = 0; cuE[k] = -1; mtE[k] = 0; wE[k] = 0; }
+  while (okA || okB || okC || okD || okE) {  // uniform per group
+    if (okE) {
+      tt x = 0;
+      int mvn = 0;
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        if (lane + k * W < dE) {
+          if (cuE[k] == cE) x += (tt)wE[k];
+          mvn += mtE[k] == st.iter;
+        }
+      }
+      x = sum_xor<tt, W>(x);
+      if (track) mvn = sum_xor<int, W>(mvn);
+      if (lane == 0) {
+        st.In[vE] = x;
+        if (track) st.nbr_moved[vE] = dE ? (float)mvn / (float)dE : 0.f;
+      }
+    }
+    okE = okD;
+    if (okD) {
+      vE = vD; dE = dD; cE = cD;
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        wE[k] = wD[k];
+        if (lane + k * W < dD) {
+          cuE[k] = st.cur[uD[k]];
+          if (track) mtE[k] = st.moved_it[uD[k]];
+        }
+      }
+    }
+    okD = okC;
+    if (okC) {
+      vD = vC; dD = dC; cD = cC;
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        if (lane + k * W < dC) {
+          uD[k] = g.nbr[oC + lane + k * W];
+          wD[k] = (acc)load_w<T>(g.w, oC + lane + k * W);
+        }
+      }
+    }
+    okC = okB && dyB;  // clean vertices leave the pipeline here
+    if (okC) {
+      vC = vB;
+      oC = g.off[vB];
+      dC = (int)(g.off[vB + 1] - oC);
+      cC = st.cur[vB];
+    }
+    okB = okA;
+    if (okA) {
+      vB = vA;
+      dyB = !only_dirty || st.dirty[vA];
+      iA += stride;
+      okA = iA < hi;
+      if (okA) vA = order[iA];
+    }
+  }
+}
+
 template <int NT, class T>
 __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> st,
                                                       const int32_t* __restrict__ order,
@@ -932,9 +1454,19 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
 
 template <class T>
 void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
-                   const int32_t* bstart, int cap, bool only_dirty, hipStream_t s) {
+                   const int32_t* bstart, int cap, bool only_dirty, hipStream_t s, int variant) {
   if (cap <= 0) return;
   const int od = only_dirty ? 1 : 0;
+  if (variant == 1 && bin <= 4) {  // pipelined
+    switch (bin) {
+      case 0: k_weight_pipe<4, 1, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, order, bstart, 0, od); break;
+      case 1: k_weight_pipe<8, 1, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, order, bstart, 1, od); break;
+      case 2: k_weight_pipe<16, 1, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, order, bstart, 2, od); break;
+      case 3: k_weight_pipe<32, 1, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, order, bstart, 3, od); break;
+      case 4: k_weight_pipe<32, 4, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, order, bstart, 4, od); break;
+    }
+    return;
+  }
   switch (bin) {
     case 0: k_weight_group<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, order, bstart, 0, od); break;
     case 1: k_weight_group<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, order, bstart, 1, od); break;
@@ -994,13 +1526,14 @@ void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t 
 // explicit instantiations
 #define GALA_INST(T)                                                                              \
   template void launch_init<T>(const DevGraph<T>&, DevState<T>&, hipStream_t);                    \
+  template void launch_build_cv<T>(DevState<T>&, int32_t, hipStream_t);                           \
   template void launch_decide<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,           \
-                                 const int32_t*, int, typename T::tot_t, hipStream_t);            \
+                                 const int32_t*, int, typename T::tot_t, hipStream_t, int);       \
   template void launch_decide_first<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,     \
-                                       const int32_t*, int, typename T::tot_t, hipStream_t);      \
+                                       const int32_t*, int, typename T::tot_t, hipStream_t, int); \
   template void launch_apply<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t);           \
   template void launch_weight<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,           \
-                                 const int32_t*, int, bool, hipStream_t);                         \
+                                 const int32_t*, int, bool, hipStream_t, int);                    \
   template void launch_prune<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t);
 GALA_INST(IntW)
 GALA_INST(FltW)

@@ -25,8 +25,10 @@ EXPORTS = [
     "gala_read_bin", "gala_write_bin", "gala_ingest_text", "gala_gen_sbm", "gala_gen_chung_lu",
     "gala_gen_rmat", "gala_set_logger", "gala_set_mlp", "gala_get_mlp", "gala_mlp_train",
     "gala_mlp_forward", "gala_mlp_grad", "gala_batch_set", "gala_batch_run", "gala_batch_graph_results",
-    "gala_batch_communities", "gala_gen_sbm_batch",
+    "gala_batch_communities", "gala_gen_sbm_batch", "gala_set_variants", "gala_get_variants", "gala_autotune",
 ]
+N_VARIANT_SLOTS = 27  # kernel variant per bin: 9 general sweeps, 9 first sweeps, 9 weight updates
+VARIANT_COUNTS = [3, 3, 3, 3, 5, 3, 1, 1, 1] + [2, 2, 2, 2, 2, 1, 1, 1, 1] + [2, 2, 2, 2, 2, 1, 1, 1, 1]
 
 MLP_NFEAT = 6
 MLP_NPARAM = 1153  # W1[16][64] (rows 6..15 zero), b1[64], w2[64], b2
@@ -174,6 +176,9 @@ def lib():
         L.gala_gen_sbm_batch.argtypes = [ctypes.c_int32, I32P, I32P, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
                                          ctypes.c_int32, I32P, I64P, I64P, I32P]
+        L.gala_set_variants.argtypes = [VP, I32P]
+        L.gala_get_variants.argtypes = [VP, I32P]
+        L.gala_autotune.argtypes = [VP, I32P]
         _lib = L
     return _lib
 
@@ -416,6 +421,23 @@ class Engine:
         res = Result()
         _check(lib().gala_run(self.ctx, self._icb, self._rcb, None, ctypes.byref(res)), "gala_run")
         return res.to_dict()
+
+    # ---- decide kernel variants ----
+    def set_variants(self, v):
+        v = np.ascontiguousarray(v, np.int32)
+        assert v.size == N_VARIANT_SLOTS
+        _check(lib().gala_set_variants(self.ctx, _p(v, ctypes.c_int32)), "gala_set_variants")
+
+    def get_variants(self):
+        v = np.zeros(N_VARIANT_SLOTS, np.int32)
+        _check(lib().gala_get_variants(self.ctx, _p(v, ctypes.c_int32)), "gala_get_variants")
+        return v
+
+    def autotune(self):
+        """Times every decide variant on the loaded graph; keeps the fastest per bin."""
+        v = np.zeros(N_VARIANT_SLOTS, np.int32)
+        _check(lib().gala_autotune(self.ctx, _p(v, ctypes.c_int32)), "gala_autotune")
+        return v
 
     # ---- batched graphs ----
     def batch_set(self, vstart, g):

@@ -196,6 +196,20 @@ int gala_mlp_forward(gala_ctx* ctx, const float* params, const float* X, int64_t
 int gala_mlp_grad(gala_ctx* ctx, const float* params, const float* X, const uint8_t* y, int64_t n, float pos_weight,
                   float* grad_out, float* loss_out);
 
+/* ---- decide kernel variants and the autotuner (no counterpart in the
+ * reference, whose bins and block sizes are fixed) ----
+ * Every degree bin has one or more decide-and-move kernels that take exactly
+ * the same decisions and differ in how lanes map to vertices.  variants[0..8]
+ * select the kernel of bins 0..8 for general sweeps, variants[9..17] for the
+ * first sweep of a round, variants[18..26] the weight-update kernel.
+ * gala_autotune times every variant on the loaded graph (full plain runs with
+ * per-bin HIP events) and keeps the fastest for later runs of the context;
+ * results are unchanged by construction. */
+#define GALA_NVARIANTS 27
+int gala_set_variants(gala_ctx* ctx, const int32_t* variants);
+int gala_get_variants(gala_ctx* ctx, int32_t* variants);
+int gala_autotune(gala_ctx* ctx, int32_t* variants_out);
+
 /* ---- batched graphs (no counterpart in the reference, which runs one graph
  * per process) ----
  * Many independent graphs run as one block-diagonal union: graph g owns input

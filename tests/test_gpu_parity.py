@@ -81,6 +81,54 @@ def test_chung_lu_hub_bins(seed):
     eng.close()
 
 
+def variant_set(k):
+    return np.array([min(k, c - 1) for c in gala.VARIANT_COUNTS], np.int32)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_decide_variants_match_oracle(k):
+    # every decide kernel variant takes the same decisions as the oracle
+    say(f"decide variants {k}")
+    g = gala.gen_chung_lu(200000, 2.1, 17.35, 12000.0, 0)
+    eng = gala.Engine(dump_iterations=True)
+    eng.set_variants(variant_set(k))
+    assert np.array_equal(eng.get_variants(), variant_set(k))
+    full_trace_compare(eng, g)
+    eng.close()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_decide_variants_random_states(k):
+    say(f"decide variants {k} on random states")
+    rng = np.random.default_rng(40 + k)
+    g = gala.gen_chung_lu(20000, 2.1, 17.35, 3000.0, 9)
+    eng = gala.Engine()
+    eng.set_variants(variant_set(k))
+    for ncomm in (3, 50, 5000):
+        K, Self, Tot, size, prev = random_state(rng, g, ncomm)
+        m2 = int(K.sum())
+        got = eng.phase_decide(g, m2, K, Self, Tot, size, prev)
+        exp = oracle.decide_step(g.vertex_num, g.degrees, g.neighbors, np.ones(len(g.neighbors), np.int64), m2,
+                                 K, Self, Tot, size, prev)
+        assert np.array_equal(got, exp), ncomm
+    eng.close()
+
+
+def test_autotune_keeps_results():
+    say("autotune")
+    g = gala.gen_chung_lu(100000, 2.1, 17.35, 8000.0, 6)
+    eng = gala.Engine()
+    eng.set_graph(g)
+    r0 = eng.run()
+    c0 = eng.communities(g.vertex_num)
+    v = eng.autotune()
+    assert all(0 <= v[i] < gala.VARIANT_COUNTS[i] for i in range(len(v)))
+    assert np.array_equal(eng.get_variants(), v)
+    r1 = eng.run()
+    assert r1["modularity"] == r0["modularity"] and np.array_equal(eng.communities(g.vertex_num), c0)
+    eng.close()
+
+
 def test_weighted_int_graph():
     say("weighted int32")
     rng = np.random.default_rng(3)
