@@ -92,6 +92,8 @@ struct gala_ctx {
   int64_t *c_ecnt = nullptr, *c_eoff = nullptr, *c_ecur = nullptr, *c_big_tab = nullptr,
           *c_i64_tmp = nullptr, *c_i64_scalar = nullptr;
   void* c_sval = nullptr;
+  int8_t* c_tier = nullptr;
+  int32_t *c_tlist = nullptr, *c_tstart = nullptr;
   int32_t* arena_keys = nullptr;
   void* arena_vals = nullptr;
   int64_t arena_slots = 0;
@@ -248,6 +250,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->c_dcnt, N + 1); OWN(ctx->c_big_list, 2 * N); OWN(ctx->c_nbig, 2);
   OWN(ctx->c_i32_tmp, N / 4 + 2 + scan_tmp_elems(N));
   OWN(ctx->c_i32_scalar, 4);
+  OWN(ctx->c_tier, N); OWN(ctx->c_tlist, N); OWN(ctx->c_tstart, NBINS + 1);
   OWN(ctx->c_ecnt, N + 1); OWN(ctx->c_eoff, N + 1); OWN(ctx->c_ecur, N);
   OWN(ctx->c_big_tab, N + 1); OWN(ctx->c_i64_tmp, 2 * (N + 1) + scan_tmp_elems(N));
   OWN(ctx->c_i64_scalar, 4);
@@ -456,6 +459,7 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   HIPOK(hipStreamSynchronize(s));
   for (int b = 0; b < NBINS; b++) caps[b] = hb[b + 1] - hb[b];
   *nitems = hb[NBINS];
+  launch_init_huge<T>(g, st, ctx->order, ctx->bstart_full, caps[7] + caps[8], s);
   const int gb = global_bin_min<T>();
   const int nh = hb[NBINS] - hb[gb];
   if (nh > 0) {
@@ -690,6 +694,7 @@ ContractBufs<T> cbufs(gala_ctx* ctx) {
   cb.chunks = ctx->chunks; cb.dense_c = ctx->dense_c; cb.dense_tcnt = ctx->dense_tcnt;
   cb.dense_toff = ctx->dense_toff;
   cb.i32_tmp = ctx->c_i32_tmp; cb.i64_tmp = ctx->c_i64_tmp; cb.i32_scalar = ctx->c_i32_scalar;
+  cb.tier = ctx->c_tier; cb.tlist = ctx->c_tlist; cb.tstart = ctx->c_tstart; cb.part_tmp = ctx->part_tmp;
   cb.i64_scalar = ctx->c_i64_scalar;
   return cb;
 }

@@ -78,8 +78,13 @@ struct BatchDev {
   int32_t* nlive;                // graphs still iterating after the latest sweep
 };
 
+// K, Self, Tot = K, size = 1, prev = cur = v, degree bins; rows of 4096 entries
+// and more get K and Self from launch_init_huge, once the bins are partitioned.
 template <class T>
 void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s);
+template <class T>
+void launch_init_huge(const DevGraph<T>& g, DevState<T>& st, const int32_t* order, const int32_t* bstart, int cap,
+                      hipStream_t s);
 
 // decide-and-move for one bin over front[bstart[b] .. bstart[b+1]) (device counts);
 // `cap` = host upper bound of that range (used only to size the grid).
@@ -155,6 +160,7 @@ struct DenseChunk {
 
 struct ContractPlan {
   int32_t caps[NBINS];  // vertices per degree bin of the graph being contracted
+  int32_t tstart[NBINS + 1];  // host copy of the row length tier starts
   int32_t C;      // communities (rows of the contracted graph)
   int32_t nbig;   // rows with 4096 < L <= LT staged entries
   int64_t slots;  // global overflow slots those rows need
@@ -194,6 +200,11 @@ struct ContractBufs {
   int64_t* i64_tmp;
   int32_t* i32_scalar;  // [4]
   int64_t* i64_scalar;  // [4]
+  // communities grouped by row length tier (L <= 64, <= 512, <= 2048, <= 4096)
+  int8_t* tier;       // [n]
+  int32_t* tlist;     // [n]
+  int32_t* tstart;    // [NBINS+1] (device)
+  int32_t* part_tmp;  // partition scratch
 };
 
 template <class T>

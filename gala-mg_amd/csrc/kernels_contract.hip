@@ -68,11 +68,18 @@ __global__ void k_ecount(const int64_t* __restrict__ off, int32_t n, const int32
 
 // classify rows: 4096 < L <= lt -> big_list (with the size of its global
 // overflow slice); L > lt -> dense_list
+// Row length tiers of the merge: 0: 1..64, 1: 65..512, 2: 513..2048,
+// 3: 2049..4096; longer rows go to the big and dense lists, empty rows nowhere.
+constexpr int TIER_TINY = 0, TIER_MID1 = 1, TIER_MID2 = 2, TIER_MID3 = 3;
+
 __global__ void k_classify(const int64_t* __restrict__ ecnt, int32_t C, long long lt,
                            int32_t* __restrict__ big_list, int32_t* __restrict__ counts,
-                           int64_t* __restrict__ big_tsz, int32_t* __restrict__ dense_list) {
+                           int64_t* __restrict__ big_tsz, int32_t* __restrict__ dense_list,
+                           int8_t* __restrict__ tier) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
     const long long L = ecnt[c];
+    tier[c] = L == 0 ? -1 : L <= 64 ? TIER_TINY : L <= 512 ? TIER_MID1 : L <= 2048 ? TIER_MID2
+            : L <= 4096 ? TIER_MID3 : -1;
     if (L > lt) {
       dense_list[atomicAdd(&counts[1], 1)] = c;
     } else if (L > 4096) {
@@ -195,16 +202,19 @@ __global__ __launch_bounds__(256) void k_stage_rows(DevGraph<T> g, const int32_t
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void k_merge_tiny(const int64_t* __restrict__ ecnt,
-                                                     const int64_t* __restrict__ eoff, int32_t C,
+                                                     const int64_t* __restrict__ eoff,
+                                                     const int32_t* __restrict__ tlist,
+                                                     const int32_t* __restrict__ tstart,
                                                      int32_t* __restrict__ skey,
                                                      typename T::acc_t* __restrict__ sval,
                                                      int32_t* __restrict__ dcnt) {
   using acc = typename T::acc_t;
   const int lane = threadIdx.x & 63;
   const int nw = gridDim.x * (blockDim.x >> 6);
-  for (int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < C; c += nw) {
+  const int lo = tstart[TIER_TINY], hi = tstart[TIER_TINY + 1];
+  for (int i = lo + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < hi; i += nw) {
+    const int c = tlist[i];
     const int L = (int)ecnt[c];
-    if (L == 0 || L > 64) continue;  // uniform per wave
     const int64_t base = eoff[c];
     int key = INT_MAX;
     acc val = 0;
@@ -274,6 +284,8 @@ __device__ __forceinline__ void lds_bitonic(int* k_s, V* v_s, int P) {
 template <int NT, int LO, int LMAX, class T>
 __global__ __launch_bounds__(NT) void k_merge_mid(const int64_t* __restrict__ ecnt,
                                                    const int64_t* __restrict__ eoff, int32_t C,
+                                                   const int32_t* __restrict__ tlist,
+                                                   const int32_t* __restrict__ tstart, int tr,
                                                    int32_t* __restrict__ skey,
                                                    typename T::acc_t* __restrict__ sval,
                                                    int32_t* __restrict__ dcnt) {
@@ -284,9 +296,10 @@ __global__ __launch_bounds__(NT) void k_merge_mid(const int64_t* __restrict__ ec
   __shared__ int s_k[LMAX];
   __shared__ acc s_v[LMAX];
   __shared__ int s_sh[17];
-  for (int c = blockIdx.x; c < C; c += gridDim.x) {
+  const int lo = tstart[tr], hi = tstart[tr + 1];
+  for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
+    const int c = tlist[i];
     const long long L = ecnt[c];
-    if (L <= LO || L > LMAX) continue;  // uniform per block
     const int64_t base = eoff[c];
     const long long lim = L < (long long)C ? L : (long long)C;
     const int lg = clog2(2 * lim) < 1 ? 1 : clog2(2 * lim);
@@ -539,15 +552,18 @@ __global__ __launch_bounds__(128) void k_dense_scatter(typename T::acc_t* __rest
 // copy each merged row to its final place: 16-lane groups for rows of at
 // most 4096 staged entries, one workgroup per row for the listed long rows
 template <class T>
-__global__ __launch_bounds__(256) void k_compact(const int64_t* __restrict__ ecnt, const int64_t* __restrict__ eoff,
+__global__ __launch_bounds__(256) void k_compact(const int32_t* __restrict__ tlist, const int32_t* __restrict__ tstart,
+                                                  const int64_t* __restrict__ eoff,
                                                   const int64_t* __restrict__ noff, const int32_t* __restrict__ dcnt,
-                                                  int32_t C, const int32_t* __restrict__ skey,
+                                                  const int32_t* __restrict__ skey,
                                                   const typename T::acc_t* __restrict__ sval,
                                                   int32_t* __restrict__ nbr, typename T::w_t* __restrict__ w) {
   const int lane = threadIdx.x & 15;
   const int ng = gridDim.x * (blockDim.x >> 4);
-  for (int c = blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4); c < C; c += ng) {
-    if (ecnt[c] > 4096) continue;  // uniform per group: copied by k_compact_list
+  // rows of tiers 0..3 (up to 4096 entries); longer ones: k_compact_list
+  const int lo = tstart[TIER_TINY], hi = tstart[TIER_MID3 + 1];
+  for (int i = lo + blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4); i < hi; i += ng) {
+    const int c = tlist[i];
     const int64_t src = eoff[c], dst = noff[c];
     const int D = dcnt[c];
     for (int t = lane; t < D; t += 16) {
@@ -611,9 +627,12 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   scan_exclusive<int64_t, int64_t>(cb.ecnt, cb.eoff, C, cb.i64_tmp, cb.eoff + C, s);
   fill<int32_t>(cb.nbig, 2, 0, s);  // big rows, dense rows
   int32_t* dense_list = cb.big_list + n;
-  k_classify<<<nblk(C), 256, 0, s>>>(cb.ecnt, C, LT, cb.big_list, cb.nbig, cb.i64_tmp, dense_list);
+  k_classify<<<nblk(C), 256, 0, s>>>(cb.ecnt, C, LT, cb.big_list, cb.nbig, cb.i64_tmp, dense_list, cb.tier);
+  partition_bins(0, nullptr, cb.tier, nullptr, C, cb.tlist, cb.tstart, cb.part_tmp, s);
   int32_t counts[2] = {0, 0};
   if (hipMemcpyAsync(counts, cb.nbig, sizeof counts, hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  if (hipMemcpyAsync(plan->tstart, cb.tstart, sizeof plan->tstart, hipMemcpyDeviceToHost, s) != hipSuccess)
+    return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
   const int32_t nbig = counts[0], ndense = counts[1];
   int64_t slots = 0;
@@ -661,12 +680,20 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
   k_stage_rows<256, T><<<nblk(plan.caps[7] + plan.caps[8], 1, 4096), 256, 0, s>>>(
       g, cb.order, cb.bstart, 7, NBINS, cb.label, cb.eoff, ecur, cb.skey, cb.sval);
   fill<int32_t>(cb.dcnt, C + 1, 0, s);
-  k_merge_tiny<T><<<nblk(C, 4, 16384), 256, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval, cb.dcnt);
-  k_merge_mid<64, 64, 512, T><<<nblk(C, 1, 16384), 64, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval, cb.dcnt);
-  k_merge_mid<128, 512, 2048, T><<<nblk(C, 1, 8192), 128, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval,
-                                                                   cb.dcnt);
-  k_merge_mid<256, 2048, 4096, T><<<nblk(C, 1, 4096), 256, 0, s>>>(cb.ecnt, cb.eoff, C, cb.skey, cb.sval,
-                                                                    cb.dcnt);
+  const int32_t* ts = plan.tstart;
+  auto tcount = [&](int t) { return ts[t + 1] - ts[t]; };
+  if (tcount(TIER_TINY) > 0)
+    k_merge_tiny<T><<<nblk(tcount(TIER_TINY), 4, 16384), 256, 0, s>>>(cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey,
+                                                                        cb.sval, cb.dcnt);
+  if (tcount(TIER_MID1) > 0)
+    k_merge_mid<64, 64, 512, T><<<nblk(tcount(TIER_MID1), 1, 16384), 64, 0, s>>>(
+        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID1, cb.skey, cb.sval, cb.dcnt);
+  if (tcount(TIER_MID2) > 0)
+    k_merge_mid<128, 512, 2048, T><<<nblk(tcount(TIER_MID2), 1, 8192), 128, 0, s>>>(
+        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID2, cb.skey, cb.sval, cb.dcnt);
+  if (tcount(TIER_MID3) > 0)
+    k_merge_mid<256, 2048, 4096, T><<<nblk(tcount(TIER_MID3), 1, 4096), 256, 0, s>>>(
+        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID3, cb.skey, cb.sval, cb.dcnt);
   if (plan.nbig > 0)
     k_merge_big<LT, T><<<std::min(plan.nbig, 2048), 1024, 0, s>>>(cb.ecnt, cb.eoff, C, cb.big_list, plan.nbig,
                                                                    cb.big_tab, cb.arena_keys, cb.arena_vals,
@@ -702,8 +729,10 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     }
   }
   scan_exclusive<int32_t, int64_t>(cb.dcnt, out.off, C, cb.i64_tmp, out.off + C, s);
-  k_compact<T><<<nblk(C, 16, 16384), 256, 0, s>>>(cb.ecnt, cb.eoff, out.off, cb.dcnt, C, cb.skey, cb.sval, out.nbr,
-                                                  out.w);
+  const int32_t nsmall = ts[TIER_MID3 + 1] - ts[TIER_TINY];
+  if (nsmall > 0)
+    k_compact<T><<<nblk(nsmall, 16, 16384), 256, 0, s>>>(cb.tlist, cb.tstart, cb.eoff, out.off, cb.dcnt, cb.skey,
+                                                         cb.sval, out.nbr, out.w);
   if (plan.nbig > 0)
     k_compact_list<T><<<std::min(plan.nbig, 8192), 256, 0, s>>>(cb.big_list, plan.nbig, cb.eoff, out.off, cb.dcnt,
                                                                  cb.skey, cb.sval, out.nbr, out.w);

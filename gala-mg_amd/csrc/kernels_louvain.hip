@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
       st.moved_it[v] = -1;
       st.nbr_moved[v] = 0.f;
     }
-    if (d > 64) continue;  // wide rows: k_init_wide
+    if (d > 64) continue;  // wide rows: k_init_wide, k_init_huge
     tt k = 0, self = 0;
     for (int64_t j = o; j < e; j++) {
       tt w = (tt)load_w<T>(g.w, j);
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void k_init_wide(DevGraph<T> g, DevState<T> st
   const int nw = gridDim.x * (blockDim.x >> 6);
   for (int v = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); v < g.n; v += nw) {
     int64_t o = g.off[v], e = g.off[v + 1];
-    if (e - o <= 64) continue;
+    if (e - o <= 64 || e - o >= 4096) continue;  // rows of 4096 and up: k_init_huge
     tt k = 0, self = 0;
     for (int64_t j = o + lane; j < e; j += 64) {
       tt w = (tt)load_w<T>(g.w, j);
@@ -218,6 +218,59 @@ __global__ __launch_bounds__(256) void k_init_wide(DevGraph<T> g, DevState<T> st
       st.Self[v] = self;
     }
   }
+}
+
+// K and Self of rows of 4096 entries and more (degree bins 7 and 8), one
+// 1024-thread workgroup per row, four independent loads per thread in flight;
+// fixed reduction order, so FP sums are reproducible.
+template <class T>
+__global__ __launch_bounds__(1024) void k_init_huge(DevGraph<T> g, DevState<T> st, const int32_t* __restrict__ order,
+                                                     const int32_t* __restrict__ bstart) {
+  using tt = typename T::tot_t;
+  constexpr int NT = 1024, U = 4;
+  __shared__ tt r_k[NT / 64], r_s[NT / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int lo = bstart[7], hi = bstart[NBINS];
+  for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
+    const int v = order[i];
+    const int64_t o = g.off[v], e = g.off[v + 1];
+    tt k = 0, self = 0;
+    for (int64_t j0 = o + threadIdx.x; j0 < e; j0 += (int64_t)NT * U) {
+      int u[U];
+      tt w[U];
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        const int64_t j = j0 + (int64_t)q * NT;
+        u[q] = j < e ? g.nbr[j] : -1;
+        w[q] = j < e ? (tt)load_w<T>(g.w, j) : tt(0);
+      }
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        k += w[q];
+        if (u[q] == v) self = w[q];
+      }
+    }
+    k = sum_xor<tt, 64>(k);
+    self = sum_xor<tt, 64>(self);  // at most one self entry per row
+    if (lane == 0) { r_k[wid] = k; r_s[wid] = self; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      k = 0;
+      self = 0;
+      for (int q = 0; q < NT / 64; q++) { k += r_k[q]; self += r_s[q]; }
+      st.K[v] = k;
+      st.Tot[v] = k;
+      st.Self[v] = self;
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+void launch_init_huge(const DevGraph<T>& g, DevState<T>& st, const int32_t* order, const int32_t* bstart, int cap,
+                      hipStream_t s) {
+  if (cap <= 0) return;
+  k_init_huge<T><<<std::min(cap, 2048), 1024, 0, s>>>(g, st, order, bstart);
 }
 
 template <class T>
@@ -1527,6 +1580,8 @@ void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t 
 #define GALA_INST(T)                                                                              \
   template void launch_init<T>(const DevGraph<T>&, DevState<T>&, hipStream_t);                    \
   template void launch_build_cv<T>(DevState<T>&, int32_t, hipStream_t);                           \
+  template void launch_init_huge<T>(const DevGraph<T>&, DevState<T>&, const int32_t*, const int32_t*, int,   \
+                                    hipStream_t);                                                  \
   template void launch_decide<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,           \
                                  const int32_t*, int, typename T::tot_t, hipStream_t, int);       \
   template void launch_decide_first<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,     \
