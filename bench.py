@@ -34,16 +34,29 @@ sys.path.insert(0, os.path.join(ROOT, "gala-mg_amd"))
 
 HBM_PEAK_GBS = 8000.0
 BIN_DEG = ["1-3", "4-7", "8-15", "16-31", "32-127", "128-511", "512-4095", "4096-8191", ">=8192"]
-# decide-and-move kernel of every degree bin (integer path): general sweeps,
-# and the first sweep of a round
-KERNEL_GENERAL = ["k_decide_ring<4, IntW>", "k_decide_ring<8, IntW>", "k_decide_ring<16, IntW>",
-                  "k_decide_ring<32, IntW>", "k_decide_wave<256, IntW>", "k_decide_wave<1024, IntW>",
-                  "k_decide_block<256, 8192, IntW>", "k_decide_block<1024, 16384, IntW>",
-                  "k_decide_hub<16384, IntW>"]
-KERNEL_FIRST = ["k_decide_first<4, IntW>", "k_decide_first<8, IntW>", "k_decide_first<16, IntW>",
-                "k_decide_first<32, IntW>", "k_decide_first<64, IntW>", "k_decide_first<64, IntW>",
-                "k_decide_first_block<256, IntW>", "k_decide_first_block<1024, IntW>",
-                "k_decide_first_block<1024, IntW>"]
+# decide-and-move kernel of every (degree bin, variant) on the integer path,
+# as the engine launches them: general sweeps, and the first sweep of a round
+KERNEL_GENERAL = [
+    ["k_decide_ring<4, IntW>", "k_decide_wave<8, 4, IntW>", "k_decide_pipe<4, false, IntW>"],
+    ["k_decide_ring<8, IntW>", "k_decide_wave<16, 8, IntW>", "k_decide_pipe<8, false, IntW>"],
+    ["k_decide_ring<16, IntW>", "k_decide_wave<32, 16, IntW>", "k_decide_pipe<16, false, IntW>"],
+    ["k_decide_ring<32, IntW>", "k_decide_wave<64, 32, IntW>", "k_decide_pipe<32, false, IntW>"],
+    ["k_decide_wave<256, 64, IntW>", "k_decide_wave<256, 32, IntW>", "k_decide_wave<256, 16, IntW>",
+     "k_decide_hpipe<256, 32, 4, IntW>", "k_decide_hpipe<256, 16, 8, IntW>"],
+    ["k_decide_wave<1024, 64, IntW>", "k_decide_wave<1024, 32, IntW>", "k_decide_hpipe<1024, 64, 8, IntW>"],
+    ["k_decide_block<256, 8192, IntW>"],
+    ["k_decide_block<1024, 16384, IntW>"],
+    ["k_decide_hub<16384, IntW>"]]
+KERNEL_FIRST = [
+    ["k_decide_first<4, IntW>", "k_decide_pipe<4, true, IntW>"],
+    ["k_decide_first<8, IntW>", "k_decide_pipe<8, true, IntW>"],
+    ["k_decide_first<16, IntW>", "k_decide_pipe<16, true, IntW>"],
+    ["k_decide_first<32, IntW>", "k_decide_pipe<32, true, IntW>"],
+    ["k_decide_first<64, IntW>", "k_decide_first<32, IntW>"],
+    ["k_decide_first<64, IntW>"],
+    ["k_decide_first_block<256, IntW>"],
+    ["k_decide_first_block<1024, IntW>"],
+    ["k_decide_first_block<1024, IntW>"]]
 
 
 def log(msg):
@@ -180,6 +193,20 @@ def rmat_leg(gala, args):
     return out
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size: FETCH_SIZE and WRITE_SIZE of
+    its dispatches against mib MiB read and written give the factor that turns
+    those counters into HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy probe: 5 copies of {mib} MiB")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,11 +220,17 @@ def main():
     ap.add_argument("--rmat-scale", type=int, default=24, help="scale of the weighted R-MAT leg (0: skip it)")
     ap.add_argument("--rmat-oracle", action="store_true", help="check the R-MAT leg against the FP64 oracle")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-frontier leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer 5 times and exit (scales the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on exactly one GPU (asked for {args.gpus}); "
               "the multi-GPU design is described but not built", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
 
     import numpy as np
     import gala
@@ -236,13 +269,14 @@ def main():
     peng = gala.Engine(device=0, profile_bins=True)
     peng.set_graph(g)
     prof = peng.run()
+    var = peng.get_variants()
     peng.close()
     bins = []
     for b in range(9):
         for kind, name, ms_b, by, ent, nl in (
-                ("general", KERNEL_GENERAL[b], prof["bin_decide_ms"][b], prof["bin_decide_bytes"][b],
+                ("general", KERNEL_GENERAL[b][var[b]], prof["bin_decide_ms"][b], prof["bin_decide_bytes"][b],
                  prof["bin_swept_edges"][b], prof["bin_launches"][b]),
-                ("first", KERNEL_FIRST[b], prof["bin_first_ms"][b], prof["bin_first_bytes"][b],
+                ("first", KERNEL_FIRST[b][var[9 + b]], prof["bin_first_ms"][b], prof["bin_first_bytes"][b],
                  prof["bin_first_edges"][b], prof["bin_first_launches"][b])):
             bins.append({"bin": b, "degrees": BIN_DEG[b], "sweep": kind, "kernel": name, "bytes": int(by),
                          "entries": int(ent), "launches": int(nl), "ms": ms_b,

@@ -46,3 +46,66 @@ __device__ __forceinline__ V block_excl_scan(V x, V* sh, V* total) {
 }
 
 }  // namespace gala
+
+namespace gala {
+
+// Bitonic sort, ascending by key, of NT*E (key, value) pairs held in
+// registers: thread t holds positions t*E .. t*E+E-1.  Compare-exchanges
+// between positions of one thread stay in registers, partners in another lane
+// of the same wave are reached with lane shuffles, and partners in another
+// wave (NT > 64) through xk/xv: NT*E elements of LDS each, element-major so a
+// wave's accesses are contiguous.  Every thread of the workgroup must call it
+// (it has barriers when NT > 64); on return xk/xv are free again.
+template <int NT, int E, class V>
+__device__ __forceinline__ void reg_bitonic(int (&k)[E], V (&v)[E], int* xk, V* xv) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int kk = 2; kk <= NT * E; kk <<= 1) {
+#pragma unroll
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      if (j < E) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const int l = e ^ j;
+          if (l > e) {
+            const bool up = ((t * E + e) & kk) == 0;
+            if ((k[e] > k[l]) == up) {
+              const int tk = k[e];
+              k[e] = k[l];
+              k[l] = tk;
+              const V tv = v[e];
+              v[e] = v[l];
+              v[l] = tv;
+            }
+          }
+        }
+      } else {
+        const int m = j / E;  // partner thread: t ^ m
+        const bool keep_min = ((t & m) == 0) == (((t * E) & kk) == 0);
+        if (m < 64) {
+#pragma unroll
+          for (int e = 0; e < E; e++) {
+            const int ok = __shfl_xor(k[e], m, 64);
+            const V ov = __shfl_xor(v[e], m, 64);
+            if (keep_min ? ok < k[e] : ok > k[e]) { k[e] = ok; v[e] = ov; }
+          }
+        } else {
+          __syncthreads();
+#pragma unroll
+          for (int e = 0; e < E; e++) { xk[e * NT + t] = k[e]; xv[e * NT + t] = v[e]; }
+          __syncthreads();
+          const int p = t ^ m;
+#pragma unroll
+          for (int e = 0; e < E; e++) {
+            const int ok = xk[e * NT + p];
+            const V ov = xv[e * NT + p];
+            if (keep_min ? ok < k[e] : ok > k[e]) { k[e] = ok; v[e] = ov; }
+          }
+        }
+      }
+    }
+  }
+  if (NT > 64) __syncthreads();
+}
+
+}  // namespace gala

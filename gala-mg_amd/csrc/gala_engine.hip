@@ -75,6 +75,7 @@ struct gala_ctx {
   int8_t* binid = nullptr;
   int32_t *order = nullptr, *front = nullptr, *fin = nullptr;
   void* cv = nullptr;  // community view, 8 (int) or 16 (FP) bytes per vertex
+  BlockPart* part = nullptr;  // workgroup partials of the O(V) reductions
   int32_t *bstart_full = nullptr, *bstart_front = nullptr;
   int64_t* hub_off = nullptr;
   int64_t* hub_tmp = nullptr;
@@ -85,6 +86,7 @@ struct gala_ctx {
   int64_t* scan64_tmp = nullptr;
   Counters* ctr = nullptr;
   Counters* ctr_h = nullptr;  // pinned
+  char* hpin = nullptr;       // pinned scratch of the small readbacks (HPIN_BYTES)
 
   // contraction buffers
   int32_t *c_cnt = nullptr, *c_nid = nullptr, *c_label = nullptr, *c_skey = nullptr, *c_dcnt = nullptr,
@@ -160,8 +162,8 @@ struct gala_ctx {
   bool have_result = false;
   std::vector<int32_t> comm_host;
 
-  hipEvent_t ev[8];
-  hipEvent_t bev[NBINS + 1];  // per-bin decide timing (opts.profile_bins)
+  hipEvent_t ev[8] = {};
+  hipEvent_t bev[NBINS + 1] = {};  // per-bin decide timing (opts.profile_bins)
 };
 
 namespace {
@@ -244,6 +246,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->part_tmp, part_tmp_elems(N));
   OWN(ctx->scan64_tmp, scan_tmp_elems(N) + 2);
   OWN(ctx->ctr, 1);
+  OWN(ctx->part, MAX_PARTS);
   // contraction
   OWN(ctx->c_cnt, N); OWN(ctx->c_nid, N); OWN(ctx->c_label, N); OWN(ctx->c_skey, M);
   { char* p; OWN(p, M * tsz); ctx->c_sval = p; }
@@ -256,6 +259,10 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->c_i64_scalar, 4);
   if (!ctx->ctr_h) {
     hipError_t e = hipHostMalloc((void**)&ctx->ctr_h, sizeof(Counters), hipHostMallocDefault);
+    if (e != hipSuccess) { ctx->last_hip = e; free_all(ctx); return GALA_E_OOM; }
+  }
+  if (!ctx->hpin) {
+    hipError_t e = hipHostMalloc((void**)&ctx->hpin, HPIN_BYTES, hipHostMallocDefault);
     if (e != hipSuccess) { ctx->last_hip = e; free_all(ctx); return GALA_E_OOM; }
   }
   ctx->cap_n = n;
@@ -435,6 +442,7 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.moved_it = ctx->moved_it;
   st.nbr_moved = ctx->nbr_moved;
   st.cv = (typename T::cv_t*)ctx->cv;
+  st.part = ctx->part;
   return st;
 }
 
@@ -454,8 +462,8 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   st.track = track;
   launch_init<T>(g, st, s);
   partition_bins(0, nullptr, ctx->binid, nullptr, g.n, ctx->order, ctx->bstart_full, ctx->part_tmp, s);
-  int32_t hb[NBINS + 1];
-  HIPOK(hipMemcpyAsync(hb, ctx->bstart_full, sizeof hb, hipMemcpyDeviceToHost, s));
+  int32_t* hb = (int32_t*)ctx->hpin;  // NBINS + 1 bin starts
+  HIPOK(hipMemcpyAsync(hb, ctx->bstart_full, sizeof(int32_t) * (NBINS + 1), hipMemcpyDeviceToHost, s));
   HIPOK(hipStreamSynchronize(s));
   for (int b = 0; b < NBINS; b++) caps[b] = hb[b + 1] - hb[b];
   *nitems = hb[NBINS];
@@ -468,10 +476,10 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
     int64_t* pos = ctx->hub_tmp + nh + 1;
     int64_t* tot = ctx->scan64_tmp;
     scan_exclusive<int64_t, int64_t>(ctx->hub_tmp, pos, nh, ctx->c_i64_tmp, tot, s);
-    int64_t slots = 0;
-    HIPOK(hipMemcpyAsync(&slots, tot, sizeof slots, hipMemcpyDeviceToHost, s));
+    int64_t* hs = (int64_t*)(ctx->hpin + 64);
+    HIPOK(hipMemcpyAsync(hs, tot, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     HIPOK(hipStreamSynchronize(s));
-    int rc = grow_hub(ctx, slots, T::FP);
+    int rc = grow_hub(ctx, hs[0], T::FP);
     if (rc) return rc;
     k_hub_scatter<<<nb, 256, 0, s>>>(ctx->order, ctx->bstart_full, gb, pos, ctx->hub_off);
   }
@@ -696,6 +704,8 @@ ContractBufs<T> cbufs(gala_ctx* ctx) {
   cb.i32_tmp = ctx->c_i32_tmp; cb.i64_tmp = ctx->c_i64_tmp; cb.i32_scalar = ctx->c_i32_scalar;
   cb.tier = ctx->c_tier; cb.tlist = ctx->c_tlist; cb.tstart = ctx->c_tstart; cb.part_tmp = ctx->part_tmp;
   cb.i64_scalar = ctx->c_i64_scalar;
+  cb.hpin = ctx->hpin;
+  cb.hpin_bytes = HPIN_BYTES;
   return cb;
 }
 
@@ -777,7 +787,7 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
     if (rc) return rc;
     compose_map(ctx->fin, ctx->n0, ctx->c_label, s);
     HIPOK(hipEventRecord(ctx->ev[7], s));
-    HIPOK(hipEventSynchronize(ctx->ev[7]));
+    HIPOK(hipStreamSynchronize(s));
     float t_round = 0, t_build = 0;
     (void)hipEventElapsedTime(&t_round, ctx->ev[5], ctx->ev[6]);
     (void)hipEventElapsedTime(&t_build, ctx->ev[6], ctx->ev[7]);
@@ -833,7 +843,7 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
     if (round >= max_rounds) break;
   }
   HIPOK(hipEventRecord(ctx->ev[3], s));
-  HIPOK(hipEventSynchronize(ctx->ev[3]));
+  HIPOK(hipStreamSynchronize(s));
   float tt_ms = 0;
   (void)hipEventElapsedTime(&tt_ms, ctx->ev[4], ctx->ev[3]);
   R.t_total_ms = tt_ms;
@@ -1130,7 +1140,7 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     if (open == 0 || round >= max_rounds) break;
   }
   HIPOK(hipEventRecord(ctx->ev[3], s));
-  HIPOK(hipEventSynchronize(ctx->ev[3]));
+  HIPOK(hipStreamSynchronize(s));
   float tt_ms = 0;
   (void)hipEventElapsedTime(&tt_ms, ctx->ev[4], ctx->ev[3]);
   R.t_total_ms = tt_ms;
@@ -1196,9 +1206,9 @@ int gala_create(gala_ctx** out, int device, const gala_opts* opts) {
   else gala_default_opts(&ctx->opts);
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return GALA_E_HIP; }
   for (auto& e : ctx->ev)
-    if (hipEventCreate(&e) != hipSuccess) { delete ctx; return GALA_E_HIP; }
+    if (hipEventCreate(&e) != hipSuccess) { gala_destroy(ctx); return GALA_E_HIP; }
   for (auto& e : ctx->bev)
-    if (hipEventCreate(&e) != hipSuccess) { delete ctx; return GALA_E_HIP; }
+    if (hipEventCreate(&e) != hipSuccess) { gala_destroy(ctx); return GALA_E_HIP; }
   *out = ctx;
   return GALA_OK;
 }
@@ -1211,9 +1221,12 @@ void gala_destroy(gala_ctx* ctx) {
   if (ctx->mlp_prm) (void)hipFree(ctx->mlp_prm);
   if (ctx->mlp_w1b) (void)hipFree(ctx->mlp_w1b);
   if (ctx->ctr_h) (void)hipHostFree(ctx->ctr_h);
+  if (ctx->hpin) (void)hipHostFree(ctx->hpin);
   if (ctx->b_host) (void)hipHostFree(ctx->b_host);
-  for (auto& e : ctx->ev) (void)hipEventDestroy(e);
-  for (auto& e : ctx->bev) (void)hipEventDestroy(e);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->bev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

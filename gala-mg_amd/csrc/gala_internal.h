@@ -66,19 +66,48 @@ struct Counters {
   unsigned long long bin_edges[NBINS];
 };
 
+// Per-workgroup partial results of the O(V) reductions (k_apply, k_prune).
+// One workgroup combines them in a fixed order afterwards, so thousands of
+// workgroups do not serialise on the same counter addresses and FP64 sums are
+// reproducible run to run.
+struct BlockPart {
+  unsigned long long s0;  // apply: sum Tot^2          prune: sum In
+  unsigned long long s1;  // apply: min non-zero Tot   prune: vertices kept active
+  double f0;              // FP path: apply sum Tot^2, prune sum In
+  unsigned long long f1;  // FP path: apply min Tot (bits of a non-negative double)
+  long long i0;           // apply: non-empty communities
+  long long pad[3];
+};
+constexpr int MAX_PARTS = 4096;  // workgroups of k_apply / k_prune
+constexpr int64_t HPIN_BYTES = 1 << 20;  // pinned host scratch of the engine's small readbacks
+
 // Community view of a vertex for one sweep: its community, that community's
-// size and Tot, packed into 16 bytes so one gather gives all three.
-struct alignas(16) CvI {
+// Tot and whether the community is a singleton, so one gather gives all
+// three.  Integer path: 8 bytes, Tot in the low 31 bits of ts (Tot <= m2 <
+// 2^31) and the singleton flag in its sign bit.  FP path: 16 bytes.
+struct alignas(8) CvI {
   int32_t c;
-  int32_t size;
-  int32_t tot;
-  int32_t pad;
+  int32_t ts;
 };
 struct alignas(16) CvF {
   int32_t c;
   int32_t size;
   double tot;
 };
+__host__ __device__ inline int32_t cv_tot(const CvI& e) { return e.ts & 0x7fffffff; }
+__host__ __device__ inline double cv_tot(const CvF& e) { return e.tot; }
+// 1 for a singleton community, some other positive size otherwise
+__host__ __device__ inline int32_t cv_sz(const CvI& e) { return e.ts < 0 ? 1 : 2; }
+__host__ __device__ inline int32_t cv_sz(const CvF& e) { return e.size; }
+__host__ __device__ inline void cv_set(CvI& e, int32_t c, int32_t size, int32_t tot) {
+  e.c = c;
+  e.ts = size == 1 ? (int32_t)((uint32_t)tot | 0x80000000u) : tot;
+}
+__host__ __device__ inline void cv_set(CvF& e, int32_t c, int32_t size, double tot) {
+  e.c = c;
+  e.size = size;
+  e.tot = tot;
+}
 
 // Weight traits: exact integer path and FP64-accumulating float path.
 struct IntW {

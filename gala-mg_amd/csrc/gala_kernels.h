@@ -59,6 +59,7 @@ struct DevState {
   // rebuilt before each sweep (launch_build_cv); the ring, per-group hash and
   // pipelined kernels read it
   typename T::cv_t* cv = nullptr;
+  BlockPart* part = nullptr;  // MAX_PARTS per-workgroup partials of k_apply / k_prune
 };
 
 // Per-graph state of a batch (one entry per graph; integer path).
@@ -205,6 +206,10 @@ struct ContractBufs {
   int32_t* tlist;     // [n]
   int32_t* tstart;    // [NBINS+1] (device)
   int32_t* part_tmp;  // partition scratch
+  // pinned host scratch for the small readbacks (>= 64 bytes); dense-row
+  // lists that do not fit go through pageable copies
+  char* hpin;
+  int64_t hpin_bytes;
 };
 
 template <class T>

@@ -11,12 +11,15 @@
 // Pipeline: label[v] = new_id(comm[v]) -> per-community entry counts -> stage
 // every entry (label of the neighbour, weight) in its community's segment ->
 // merge each segment into a sorted, duplicate-free row, by segment length L:
-//   L <= 64       one wave, bitonic sort in registers + segmented scan
-//   L <= 512      64-thread workgroup: LDS hash aggregate, bitonic sort of the keys
-//   L <= 2048     128-thread workgroup, same with larger LDS tables
+//   L <= 16/32/64 one 16-, 32- or 64-lane group per row: bitonic sort across
+//                 the group's lanes + segmented scan
+//   L <= 512      64-thread workgroup: LDS hash aggregate, then a register
+//                 bitonic sort of the distinct keys (8 per lane, no barriers)
+//   L <= 2048     128-thread workgroup, same with 16 keys per thread
 //   L <= 4096     256-thread workgroup
 //   L <= LT       1024-thread workgroup: LT-slot LDS table (LT = 16384, 8192 on
-//                 the FP path), overflow into a global slice, LDS sort of <= LT keys
+//                 the FP path), overflow into a global slice, register bitonic
+//                 sort of <= LT keys (LDS only for exchanges between waves)
 //   L >  LT       dense rows: the segment is cut into chunks, one workgroup per
 //                 chunk pre-aggregates in LDS and adds into a C-wide accumulator
 //                 for the row (HBM is plentiful); a tiled scan of the accumulator
@@ -66,11 +69,18 @@ __global__ void k_ecount(const int64_t* __restrict__ off, int32_t n, const int32
   }
 }
 
+// staged length of each listed dense row
+__global__ void k_dense_lengths(const int32_t* __restrict__ list, int32_t nlist, const int64_t* __restrict__ ecnt,
+                                int64_t* __restrict__ len) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nlist; i += gridDim.x * blockDim.x) len[i] = ecnt[list[i]];
+}
+
 // classify rows: 4096 < L <= lt -> big_list (with the size of its global
 // overflow slice); L > lt -> dense_list
-// Row length tiers of the merge: 0: 1..64, 1: 65..512, 2: 513..2048,
-// 3: 2049..4096; longer rows go to the big and dense lists, empty rows nowhere.
-constexpr int TIER_TINY = 0, TIER_MID1 = 1, TIER_MID2 = 2, TIER_MID3 = 3;
+// Row length tiers of the merge: 0: 1..16, 1: 17..32, 2: 33..64, 3: 65..512,
+// 4: 513..2048, 5: 2049..4096; longer rows go to the big and dense lists,
+// empty rows nowhere.
+constexpr int TIER_T16 = 0, TIER_T32 = 1, TIER_T64 = 2, TIER_MID1 = 3, TIER_MID2 = 4, TIER_MID3 = 5;
 
 __global__ void k_classify(const int64_t* __restrict__ ecnt, int32_t C, long long lt,
                            int32_t* __restrict__ big_list, int32_t* __restrict__ counts,
@@ -78,8 +88,8 @@ __global__ void k_classify(const int64_t* __restrict__ ecnt, int32_t C, long lon
                            int8_t* __restrict__ tier) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
     const long long L = ecnt[c];
-    tier[c] = L == 0 ? -1 : L <= 64 ? TIER_TINY : L <= 512 ? TIER_MID1 : L <= 2048 ? TIER_MID2
-            : L <= 4096 ? TIER_MID3 : -1;
+    tier[c] = L == 0 ? -1 : L <= 16 ? TIER_T16 : L <= 32 ? TIER_T32 : L <= 64 ? TIER_T64
+            : L <= 512 ? TIER_MID1 : L <= 2048 ? TIER_MID2 : L <= 4096 ? TIER_MID3 : -1;
     if (L > lt) {
       dense_list[atomicAdd(&counts[1], 1)] = c;
     } else if (L > 4096) {
@@ -198,9 +208,9 @@ __global__ __launch_bounds__(256) void k_stage_rows(DevGraph<T> g, const int32_t
 }
 
 // ---------------------------------------------------------------------------
-// L <= 64: one wave per community, everything in registers
+// L <= W (16, 32 or 64): a W-lane group per community, everything in registers
 // ---------------------------------------------------------------------------
-template <class T>
+template <int W, int TR, class T>
 __global__ __launch_bounds__(256) void k_merge_tiny(const int64_t* __restrict__ ecnt,
                                                      const int64_t* __restrict__ eoff,
                                                      const int32_t* __restrict__ tlist,
@@ -209,40 +219,42 @@ __global__ __launch_bounds__(256) void k_merge_tiny(const int64_t* __restrict__ 
                                                      typename T::acc_t* __restrict__ sval,
                                                      int32_t* __restrict__ dcnt) {
   using acc = typename T::acc_t;
-  const int lane = threadIdx.x & 63;
-  const int nw = gridDim.x * (blockDim.x >> 6);
-  const int lo = tstart[TIER_TINY], hi = tstart[TIER_TINY + 1];
-  for (int i = lo + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < hi; i += nw) {
+  const int lane = threadIdx.x & (W - 1);
+  const int gbase = (threadIdx.x & 63) & ~(W - 1);  // first lane of the group in its wave
+  const unsigned long long gmask = W == 64 ? ~0ull : ((1ull << W) - 1);
+  const int ng = gridDim.x * (blockDim.x / W);
+  const int lo = tstart[TR], hi = tstart[TR + 1];
+  for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W; i < hi; i += ng) {  // uniform per group
     const int c = tlist[i];
     const int L = (int)ecnt[c];
     const int64_t base = eoff[c];
     int key = INT_MAX;
     acc val = 0;
     if (lane < L) { key = skey[base + lane]; val = sval[base + lane]; }
-    // bitonic sort over 64 lanes (ascending)
+    // bitonic sort over the group's W lanes (ascending)
 #pragma unroll
-    for (int k = 2; k <= 64; k <<= 1) {
+    for (int k = 2; k <= W; k <<= 1) {
 #pragma unroll
       for (int j = k >> 1; j > 0; j >>= 1) {
-        int pk = __shfl_xor(key, j, 64);
-        acc pv = __shfl_xor(val, j, 64);
+        int pk = __shfl_xor(key, j, W);
+        acc pv = __shfl_xor(val, j, W);
         const bool up = (lane & k) == 0, lower = (lane & j) == 0;
         const bool take = (lower == up) ? (pk < key) : (pk > key);
         if (take) { key = pk; val = pv; }
       }
     }
-    const int prevk = __shfl_up(key, 1, 64);
+    const int prevk = __shfl_up(key, 1, W);
     const bool head = lane < L && (lane == 0 || prevk != key);
     // segmented inclusive scan of val (runs start at heads)
     bool f = head;
 #pragma unroll
-    for (int k = 1; k < 64; k <<= 1) {
-      acc y = __shfl_up(val, k, 64);
-      int fy = __shfl_up((int)f, k, 64);
+    for (int k = 1; k < W; k <<= 1) {
+      acc y = __shfl_up(val, k, W);
+      int fy = __shfl_up((int)f, k, W);
       if (lane >= k && !f) { val += y; f = fy; }
     }
-    const unsigned long long heads = __ballot(head);
-    const int nextk = __shfl_down(key, 1, 64);
+    const unsigned long long heads = (__ballot(head) >> gbase) & gmask;
+    const int nextk = __shfl_down(key, 1, W);
     const bool tail = lane < L && (lane == L - 1 || nextk != key);
     if (tail) {
       const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
@@ -255,25 +267,58 @@ __global__ __launch_bounds__(256) void k_merge_tiny(const int64_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// bitonic sort of P (power of two) unique keys + values in LDS by NT threads
+// Sort the D distinct (key, value) pairs s_k/s_v[0, D) of a row (D <= NT*E)
+// in registers and write them to skey/sval[base, base + D).  xk/xv are the
+// exchange buffers of the sort (NT*E elements each, may alias s_k/s_v).  Run
+// by threads 0..NT-1; for NT > 64 that must be the whole workgroup.
 // ---------------------------------------------------------------------------
-template <int NT, class V>
-__device__ __forceinline__ void lds_bitonic(int* k_s, V* v_s, int P) {
-  for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = threadIdx.x; t < P; t += NT) {
-        const int u = t ^ j;
-        if (u > t) {
-          const bool up = (t & k) == 0;
-          const int a = k_s[t], b = k_s[u];
-          if ((a > b) == up) {
-            k_s[t] = b; k_s[u] = a;
-            V x = v_s[t]; v_s[t] = v_s[u]; v_s[u] = x;
-          }
-        }
-      }
-      __syncthreads();
+template <int NT, int E, class V>
+__device__ __forceinline__ void sort_row_out(const int* s_k, const V* s_v, int D, int* xk, V* xv,
+                                             int32_t* __restrict__ skey, V* __restrict__ sval, int64_t base) {
+  int k[E];
+  V v[E];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int i = t * E + e;
+    k[e] = i < D ? s_k[i] : INT_MAX;
+    v[e] = i < D ? s_v[i] : V(0);
+  }
+  reg_bitonic<NT, E, V>(k, v, xk, xv);
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int i = t * E + e;
+    if (i < D) {
+      skey[base + i] = k[e];
+      sval[base + i] = v[e];
     }
+  }
+}
+
+// The smallest register sort that holds the D distinct pairs of a row: one
+// wave with 1, 2, 4 or 8 keys per lane for D <= 512, else the whole NT-thread
+// workgroup with 1..EMAX keys per thread (xk/xv hold NT*EMAX elements).
+template <int NT, int EMAX, class V>
+__device__ __forceinline__ void sort_row(const int* s_k, const V* s_v, int D, int* xk, V* xv,
+                                         int32_t* __restrict__ skey, V* __restrict__ sval, int64_t base) {
+  if (D <= 512 || NT == 64) {
+    if (threadIdx.x < 64) {
+      if (D <= 64) sort_row_out<64, 1, V>(s_k, s_v, D, xk, xv, skey, sval, base);
+      else if (D <= 128) sort_row_out<64, 2, V>(s_k, s_v, D, xk, xv, skey, sval, base);
+      else if (D <= 256) sort_row_out<64, 4, V>(s_k, s_v, D, xk, xv, skey, sval, base);
+      else sort_row_out<64, 8, V>(s_k, s_v, D, xk, xv, skey, sval, base);
+    }
+    return;
+  }
+  if constexpr (NT > 64) {
+    if (D <= NT) { sort_row_out<NT, 1, V>(s_k, s_v, D, xk, xv, skey, sval, base); return; }
+    if constexpr (EMAX >= 2)
+      if (D <= NT * 2) { sort_row_out<NT, 2, V>(s_k, s_v, D, xk, xv, skey, sval, base); return; }
+    if constexpr (EMAX >= 4)
+      if (D <= NT * 4) { sort_row_out<NT, 4, V>(s_k, s_v, D, xk, xv, skey, sval, base); return; }
+    if constexpr (EMAX >= 8)
+      if (D <= NT * 8) { sort_row_out<NT, 8, V>(s_k, s_v, D, xk, xv, skey, sval, base); return; }
+    if constexpr (EMAX >= 16) sort_row_out<NT, 16, V>(s_k, s_v, D, xk, xv, skey, sval, base);
   }
 }
 
@@ -326,15 +371,7 @@ __global__ __launch_bounds__(NT) void k_merge_mid(const int64_t* __restrict__ ec
       if (k) { s_k[q] = k - 1; s_v[q] = h_v[t]; q++; }
     }
     __syncthreads();
-    int P = 1;
-    while (P < D) P <<= 1;
-    for (int t = D + threadIdx.x; t < P; t += NT) { s_k[t] = INT_MAX; s_v[t] = 0; }
-    __syncthreads();
-    lds_bitonic<NT, acc>(s_k, s_v, P);
-    for (int t = threadIdx.x; t < D; t += NT) {
-      skey[base + t] = s_k[t];
-      sval[base + t] = s_v[t];
-    }
+    sort_row<NT, LMAX / NT, acc>(s_k, s_v, D, h_k, h_v, skey, sval, base);
     if (threadIdx.x == 0) dcnt[c] = D;
     __syncthreads();
   }
@@ -434,15 +471,7 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
       }
     }
     __syncthreads();
-    int P = 1;
-    while (P < D) P <<= 1;
-    for (int t = D + threadIdx.x; t < P; t += NT) { s_k[t] = INT_MAX; s_v[t] = 0; }
-    __syncthreads();
-    lds_bitonic<NT, acc>(s_k, s_v, P);
-    for (int t = threadIdx.x; t < D; t += NT) {
-      skey[base + t] = s_k[t];
-      sval[base + t] = s_v[t];
-    }
+    sort_row<NT, LT / NT, acc>(s_k, s_v, D, s_k, s_v, skey, sval, base);
     if (threadIdx.x == 0) dcnt[c] = D;
     __syncthreads();
   }
@@ -560,8 +589,8 @@ __global__ __launch_bounds__(256) void k_compact(const int32_t* __restrict__ tli
                                                   int32_t* __restrict__ nbr, typename T::w_t* __restrict__ w) {
   const int lane = threadIdx.x & 15;
   const int ng = gridDim.x * (blockDim.x >> 4);
-  // rows of tiers 0..3 (up to 4096 entries); longer ones: k_compact_list
-  const int lo = tstart[TIER_TINY], hi = tstart[TIER_MID3 + 1];
+  // rows of the tiers up to 4096 entries; longer ones: k_compact_list
+  const int lo = tstart[TIER_T16], hi = tstart[TIER_MID3 + 1];
   for (int i = lo + blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4); i < hi; i += ng) {
     const int c = tlist[i];
     const int64_t src = eoff[c], dst = noff[c];
@@ -619,9 +648,11 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   k_nonempty<<<nblk(n), 256, 0, s>>>(cb.cnt, n, flag);
   scan_exclusive<uint8_t, int32_t>(flag, cb.nid, n, cb.i32_tmp + (n + 3) / 4 + 1, cb.i32_scalar, s);
   k_label<<<nblk(n), 256, 0, s>>>(comm, cb.nid, n, cb.label);
-  int32_t C = 0;
-  if (hipMemcpyAsync(&C, cb.i32_scalar, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  int32_t* hp32 = (int32_t*)cb.hpin;
+  int64_t* hp64 = (int64_t*)cb.hpin;
+  if (hipMemcpyAsync(hp32, cb.i32_scalar, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  const int32_t C = hp32[0];
   fill<int64_t>(cb.ecnt, C + 1, 0, s);
   k_ecount<<<nblk(n), 256, 0, s>>>(g.off, n, cb.label, cb.ecnt);
   scan_exclusive<int64_t, int64_t>(cb.ecnt, cb.eoff, C, cb.i64_tmp, cb.eoff + C, s);
@@ -629,18 +660,20 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   int32_t* dense_list = cb.big_list + n;
   k_classify<<<nblk(C), 256, 0, s>>>(cb.ecnt, C, LT, cb.big_list, cb.nbig, cb.i64_tmp, dense_list, cb.tier);
   partition_bins(0, nullptr, cb.tier, nullptr, C, cb.tlist, cb.tstart, cb.part_tmp, s);
-  int32_t counts[2] = {0, 0};
-  if (hipMemcpyAsync(counts, cb.nbig, sizeof counts, hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-  if (hipMemcpyAsync(plan->tstart, cb.tstart, sizeof plan->tstart, hipMemcpyDeviceToHost, s) != hipSuccess)
+  // hp32[0..1] = big and dense row counts, hp32[2..] = the tier starts
+  if (hipMemcpyAsync(hp32, cb.nbig, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  if (hipMemcpyAsync(hp32 + 2, cb.tstart, sizeof plan->tstart, hipMemcpyDeviceToHost, s) != hipSuccess)
     return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  const int32_t nbig = counts[0], ndense = counts[1];
+  const int32_t nbig = hp32[0], ndense = hp32[1];
+  std::copy(hp32 + 2, hp32 + 2 + NBINS + 1, plan->tstart);
   int64_t slots = 0;
   if (nbig > 0) {
     scan_exclusive<int64_t, int64_t>(cb.i64_tmp, cb.big_tab, nbig, cb.i64_tmp + nbig + 1, cb.big_tab + nbig, s);
-    if (hipMemcpyAsync(&slots, cb.big_tab + nbig, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+    if (hipMemcpyAsync(hp64, cb.big_tab + nbig, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess)
       return -1;
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    slots = hp64[0];
   }
   plan->C = C;
   plan->nbig = nbig;
@@ -649,18 +682,33 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   plan->dense_c.clear();
   plan->chunks.clear();
   if (ndense > 0) {
+    // (community, staged length) of every dense row, lengths gathered on the device
+    int64_t* lens = cb.i64_tmp;  // free again after the big-row scan
+    k_dense_lengths<<<nblk(ndense), 256, 0, s>>>(dense_list, ndense, cb.ecnt, lens);
+    std::vector<char> pv;
+    const int64_t bytes = (int64_t)(sizeof(int64_t) + sizeof(int32_t)) * ndense;
+    char* hv = cb.hpin;
+    if (bytes > cb.hpin_bytes) {
+      pv.resize(bytes);
+      hv = pv.data();
+    }
+    int64_t* hl = (int64_t*)hv;
+    int32_t* hc = (int32_t*)(hv + sizeof(int64_t) * ndense);
+    if (hipMemcpyAsync(hl, lens, sizeof(int64_t) * ndense, hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+    if (hipMemcpyAsync(hc, dense_list, sizeof(int32_t) * ndense, hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
     // dense rows in ascending community order (deterministic slot assignment)
-    std::vector<int32_t> dl(ndense);
-    if (hipMemcpy(dl.data(), dense_list, sizeof(int32_t) * ndense, hipMemcpyDeviceToHost) != hipSuccess) return -1;
-    std::sort(dl.begin(), dl.end());
-    std::vector<int64_t> ec(C + 1);
-    if (hipMemcpy(ec.data(), cb.ecnt, sizeof(int64_t) * C, hipMemcpyDeviceToHost) != hipSuccess) return -1;
-    plan->dense_c = dl;
+    std::vector<int32_t> idx(ndense);
+    for (int32_t i = 0; i < ndense; i++) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return hc[a] < hc[b]; });
+    plan->dense_c.resize(ndense);
     const int64_t CH = LT / 2;  // entries per chunk: the LDS table stays at most half full
     for (int32_t i = 0; i < ndense; i++) {
-      const int64_t L = ec[dl[i]];
+      const int32_t c = hc[idx[i]];
+      const int64_t L = hl[idx[i]];
+      plan->dense_c[i] = c;
       for (int64_t st = 0; st < L; st += CH)
-        plan->chunks.push_back({i, dl[i], st, (int32_t)std::min<int64_t>(CH, L - st)});
+        plan->chunks.push_back({i, c, st, (int32_t)std::min<int64_t>(CH, L - st)});
     }
   }
   return 0;
@@ -682,9 +730,15 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
   fill<int32_t>(cb.dcnt, C + 1, 0, s);
   const int32_t* ts = plan.tstart;
   auto tcount = [&](int t) { return ts[t + 1] - ts[t]; };
-  if (tcount(TIER_TINY) > 0)
-    k_merge_tiny<T><<<nblk(tcount(TIER_TINY), 4, 16384), 256, 0, s>>>(cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey,
-                                                                        cb.sval, cb.dcnt);
+  if (tcount(TIER_T16) > 0)
+    k_merge_tiny<16, TIER_T16, T><<<nblk(tcount(TIER_T16), 16, 16384), 256, 0, s>>>(
+        cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey, cb.sval, cb.dcnt);
+  if (tcount(TIER_T32) > 0)
+    k_merge_tiny<32, TIER_T32, T><<<nblk(tcount(TIER_T32), 8, 16384), 256, 0, s>>>(
+        cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey, cb.sval, cb.dcnt);
+  if (tcount(TIER_T64) > 0)
+    k_merge_tiny<64, TIER_T64, T><<<nblk(tcount(TIER_T64), 4, 16384), 256, 0, s>>>(
+        cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey, cb.sval, cb.dcnt);
   if (tcount(TIER_MID1) > 0)
     k_merge_mid<64, 64, 512, T><<<nblk(tcount(TIER_MID1), 1, 16384), 64, 0, s>>>(
         cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID1, cb.skey, cb.sval, cb.dcnt);
@@ -729,7 +783,7 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     }
   }
   scan_exclusive<int32_t, int64_t>(cb.dcnt, out.off, C, cb.i64_tmp, out.off + C, s);
-  const int32_t nsmall = ts[TIER_MID3 + 1] - ts[TIER_TINY];
+  const int32_t nsmall = ts[TIER_MID3 + 1] - ts[TIER_T16];
   if (nsmall > 0)
     k_compact<T><<<nblk(nsmall, 16, 16384), 256, 0, s>>>(cb.tlist, cb.tstart, cb.eoff, out.off, cb.dcnt, cb.skey,
                                                          cb.sval, out.nbr, out.w);
@@ -739,10 +793,10 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
   if (plan.ndense > 0)
     k_compact_list<T><<<std::min(plan.ndense, 8192), 256, 0, s>>>(cb.big_list + n, plan.ndense, cb.eoff, out.off,
                                                                    cb.dcnt, cb.skey, cb.sval, out.nbr, out.w);
-  int64_t m = 0;
-  if (hipMemcpyAsync(&m, out.off + C, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  int64_t* hp64 = (int64_t*)cb.hpin;
+  if (hipMemcpyAsync(hp64, out.off + C, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  *m_out = m;
+  *m_out = hp64[0];
   return 0;
 }
 

@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
       st.moved_it[v] = -1;
       st.nbr_moved[v] = 0.f;
     }
-    if (d > 64) continue;  // wide rows: k_init_wide, k_init_huge
+    if (d > 8) continue;  // longer rows: k_init_wide, k_init_huge
     tt k = 0, self = 0;
     for (int64_t j = o; j < e; j++) {
       tt w = (tt)load_w<T>(g.w, j);
@@ -196,22 +196,23 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
   }
 }
 
-template <class T>
+// K and Self of the rows with LO < degree <= HI, W lanes per row.
+template <int W, int LO, int HI, class T>
 __global__ __launch_bounds__(256) void k_init_wide(DevGraph<T> g, DevState<T> st) {
   using tt = typename T::tot_t;
-  const int lane = threadIdx.x & 63;
-  const int nw = gridDim.x * (blockDim.x >> 6);
-  for (int v = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); v < g.n; v += nw) {
+  const int lane = threadIdx.x & (W - 1);
+  const int ng = gridDim.x * (blockDim.x / W);
+  for (int v = (blockIdx.x * blockDim.x + threadIdx.x) / W; v < g.n; v += ng) {  // uniform per group
     int64_t o = g.off[v], e = g.off[v + 1];
-    if (e - o <= 64 || e - o >= 4096) continue;  // rows of 4096 and up: k_init_huge
+    if (e - o <= LO || e - o > HI) continue;
     tt k = 0, self = 0;
-    for (int64_t j = o + lane; j < e; j += 64) {
+    for (int64_t j = o + lane; j < e; j += W) {
       tt w = (tt)load_w<T>(g.w, j);
       k += w;
       if (g.nbr[j] == v) self = w;
     }
-    k = sum_xor<tt, 64>(k);
-    self = sum_xor<tt, 64>(self);  // at most one self entry per row
+    k = sum_xor<tt, W>(k);
+    self = sum_xor<tt, W>(self);  // at most one self entry per row
     if (lane == 0) {
       st.K[v] = k;
       st.Tot[v] = k;
@@ -278,8 +279,10 @@ void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s) {
   if (g.n <= 0) return;
   int nb = std::min((g.n + 255) / 256, 16384);
   k_init_small<T><<<nb, 256, 0, s>>>(g, st);
-  int nb2 = std::min((g.n + 3) / 4, 16384);
-  k_init_wide<T><<<nb2, 256, 0, s>>>(g, st);
+  // rows of 9..64 entries, 16 lanes each; 65..4095 entries, one wave each;
+  // 4096 and up: launch_init_huge
+  k_init_wide<16, 8, 64, T><<<std::min((g.n + 15) / 16, 16384), 256, 0, s>>>(g, st);
+  k_init_wide<64, 64, 4095, T><<<std::min((g.n + 3) / 4, 8192), 256, 0, s>>>(g, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -290,10 +293,7 @@ __global__ __launch_bounds__(256) void k_build_cv(DevState<T> st, int32_t n) {
   for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
     const int c = st.prev[u];
     typename T::cv_t e;
-    e.c = c;
-    e.size = st.size[c];
-    if constexpr (!T::FP) e.pad = 0;
-    e.tot = st.Tot[c];
+    cv_set(e, c, st.size[c], st.Tot[c]);
     st.cv[u] = e;
   }
 }
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
       ul = g.nbr[o + lane];
       const typename T::cv_t eu = st.cv[ul];
       cl = eu.c;
-      tl = eu.tot;
+      tl = cv_tot(eu);
       wl = (acc)load_w<T>(g.w, o + lane);
     }
     acc s = wl, wc = (cl == c) ? wl : acc(0);
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
       if (co == c) wc += wo;
     }
     const tt K = st.K[v];
-    const tt Tc = ev.tot;
+    const tt Tc = cv_tot(ev);
     G h = 0;
     int bd = INT_MAX;
     if (leader && cl != c) {
@@ -467,8 +467,8 @@ __global__ __launch_bounds__(256) void k_decide_pipe(DevGraph<T> g, DevState<T> 
       G h = 0;
       int bd = INT_MAX, bsz = 0;
       if (leader && cl != c) {
-        const G hh = gain_h<T>((G)s, (G)m2, (G)KD, (G)euD.tot, (G)evD.tot);
-        if (hh > 0) { h = hh; bd = cl; bsz = euD.size; }
+        const G hh = gain_h<T>((G)s, (G)m2, (G)KD, (G)cv_tot(euD), (G)cv_tot(evD));
+        if (hh > 0) { h = hh; bd = cl; bsz = cv_sz(euD); }
       }
 #pragma unroll
       for (int k = W / 2; k >= 1; k >>= 1) {
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(256) void k_decide_pipe(DevGraph<T> g, DevState<T> 
       if (lane == 0) {
         swept++;
         edges += dD;
-        mv = commit_move_sz<T>(vD, c, h, bd, bsz, evD.size, wc, KD, SD, m2, st);
+        mv = commit_move_sz<T>(vD, c, h, bd, bsz, cv_sz(evD), wc, KD, SD, m2, st);
         moved += mv;
       }
       if (st.dirty && __shfl(mv, 0, W)) {
@@ -602,7 +602,7 @@ __global__ __launch_bounds__(256) void k_decide_hpipe(DevGraph<T> g, DevState<T>
           unsigned p = hash_slot(cl, lg);
           for (int probe = 0; probe < tsz; probe++) {
             const int kk = atomicCAS(&keys[p], 0, cl + 1);
-            if (kk == 0) { tots[p] = euD[k].tot; sizes[p] = euD[k].size; }
+            if (kk == 0) { tots[p] = cv_tot(euD[k]); sizes[p] = cv_sz(euD[k]); }
             if (kk == 0 || kk == cl + 1) { atomicAdd(&vals[p], wD[k]); break; }
             p = (p + 1) & (tsz - 1);
           }
@@ -620,7 +620,7 @@ __global__ __launch_bounds__(256) void k_decide_hpipe(DevGraph<T> g, DevState<T>
           if (dd == c) {
             wc = a;
           } else {
-            const G hh = gain_h<T>((G)a, (G)m2, (G)KD, (G)tots[t], (G)evD.tot);
+            const G hh = gain_h<T>((G)a, (G)m2, (G)KD, (G)tots[t], (G)cv_tot(evD));
             if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; bsz = sizes[t]; }
           }
         }
@@ -637,7 +637,7 @@ __global__ __launch_bounds__(256) void k_decide_hpipe(DevGraph<T> g, DevState<T>
       if (lane == 0) {
         swept++;
         edges += dD;
-        mv = commit_move_sz<T>(vD, c, h, bd, bsz, evD.size, wc, KD, SD, m2, st);
+        mv = commit_move_sz<T>(vD, c, h, bd, bsz, cv_sz(evD), wc, KD, SD, m2, st);
         moved += mv;
       }
       if (st.dirty && __shfl(mv, 0, W)) {
@@ -747,14 +747,14 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
       unsigned p = hash_slot(cl, lg);
       for (int probe = 0; probe < tsz; probe++) {
         int k = atomicCAS(&keys[p], 0, cl + 1);
-        if (k == 0) tots[p] = eu.tot;  // the claiming lane records the community's Tot
+        if (k == 0) tots[p] = cv_tot(eu);  // the claiming lane records the community's Tot
         if (k == 0 || k == cl + 1) { atomicAdd(&vals[p], w); break; }
         p = (p + 1) & (tsz - 1);
       }
     }
     lds_fence_wave();
     const tt K = st.K[v];
-    const tt Tc = ev.tot;
+    const tt Tc = cv_tot(ev);
     G h = 0;
     int bd = INT_MAX;
     acc wc = 0;
@@ -1301,6 +1301,48 @@ __global__ __launch_bounds__(256) void k_apply(DevState<T> st, int32_t n, typena
       t2 += sh_t2[q]; t2f += sh_t2f[q]; ne += sh_ne[q];
       mn = sh_min[q] < mn ? sh_min[q] : mn;
     }
+    BlockPart bp{};
+    bp.s0 = t2;
+    bp.s1 = mn;
+    bp.f0 = t2f;
+    bp.f1 = mn;
+    bp.i0 = ne;
+    st.part[blockIdx.x] = bp;
+  }
+}
+
+// Combines the nb workgroup partials of k_apply (fixed order) into the counters.
+template <class T>
+__global__ __launch_bounds__(256) void k_apply_fin(DevState<T> st, int nb) {
+  __shared__ unsigned long long sh_t2[4], sh_min[4];
+  __shared__ double sh_t2f[4];
+  __shared__ long long sh_ne[4];
+  unsigned long long t2 = 0, mn = ~0ull;
+  double t2f = 0;
+  long long ne = 0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    const BlockPart p = st.part[b];
+    t2 += p.s0;
+    t2f += p.f0;
+    ne += p.i0;
+    mn = p.s1 < mn ? p.s1 : mn;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  t2 = sum_xor<unsigned long long, 64>(t2);
+  t2f = sum_xor<double, 64>(t2f);
+  ne = sum_xor<long long, 64>(ne);
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) {
+    unsigned long long o = __shfl_xor(mn, k, 64);
+    mn = o < mn ? o : mn;
+  }
+  if (lane == 0) { sh_t2[wid] = t2; sh_t2f[wid] = t2f; sh_min[wid] = mn; sh_ne[wid] = ne; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < 4; q++) {
+      t2 += sh_t2[q]; t2f += sh_t2f[q]; ne += sh_ne[q];
+      mn = sh_min[q] < mn ? sh_min[q] : mn;
+    }
     if (T::FP) {
       atomicAdd(&st.ctr->sum_tot2_f, t2f);
       atomicMin((unsigned long long*)&st.ctr->min_tot_f, mn);
@@ -1308,15 +1350,16 @@ __global__ __launch_bounds__(256) void k_apply(DevState<T> st, int32_t n, typena
       atomic_add_ull(&st.ctr->sum_tot2, t2);
       atomicMin((unsigned long long*)&st.ctr->min_tot, mn);
     }
-    if (ne) atomicAdd(&st.ctr->nonempty, ne);
+    if (ne) atomicAdd(&st.ctr->nonempty, (int)ne);
   }
 }
 
 template <class T>
 void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s) {
   if (n <= 0) return;
-  int nb = std::min((n + 255) / 256, 4096);
+  const int nb = std::min((n + 255) / 256, MAX_PARTS);
   k_apply<T><<<nb, 256, 0, s>>>(st, n, m2);
+  k_apply_fin<T><<<1, 256, 0, s>>>(st, nb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1564,6 +1607,35 @@ __global__ __launch_bounds__(256) void k_prune(DevState<T> st, int32_t n, typena
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int q = 1; q < 4; q++) { insum += s_in[q]; act += s_act[q]; insumf += s_inf[q]; }
+    BlockPart bp{};
+    bp.s0 = insum;
+    bp.s1 = act;
+    bp.f0 = insumf;
+    st.part[blockIdx.x] = bp;
+  }
+}
+
+// Combines the nb workgroup partials of k_prune (fixed order) into the counters.
+template <class T>
+__global__ __launch_bounds__(256) void k_prune_fin(DevState<T> st, int nb) {
+  __shared__ unsigned long long s_in[4], s_act[4];
+  __shared__ double s_inf[4];
+  unsigned long long insum = 0, act = 0;
+  double insumf = 0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    const BlockPart p = st.part[b];
+    insum += p.s0;
+    act += p.s1;
+    insumf += p.f0;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  insum = sum_xor<unsigned long long, 64>(insum);
+  act = sum_xor<unsigned long long, 64>(act);
+  insumf = sum_xor<double, 64>(insumf);
+  if (lane == 0) { s_in[wid] = insum; s_act[wid] = act; s_inf[wid] = insumf; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < 4; q++) { insum += s_in[q]; act += s_act[q]; insumf += s_inf[q]; }
     if (T::FP) { if (insumf != 0) atomicAdd(&st.ctr->sum_in_f, insumf); }
     else atomic_add_ull(&st.ctr->sum_in, insum);
     atomic_add_ull(&st.ctr->active, act);
@@ -1573,7 +1645,9 @@ __global__ __launch_bounds__(256) void k_prune(DevState<T> st, int32_t n, typena
 template <class T>
 void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s) {
   if (n <= 0) return;
-  k_prune<T><<<std::min((n + 255) / 256, 4096), 256, 0, s>>>(st, n, m2);
+  const int nb = std::min((n + 255) / 256, MAX_PARTS);
+  k_prune<T><<<nb, 256, 0, s>>>(st, n, m2);
+  k_prune_fin<T><<<1, 256, 0, s>>>(st, nb);
 }
 
 // explicit instantiations
