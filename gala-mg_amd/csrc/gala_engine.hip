@@ -99,16 +99,10 @@ struct gala_ctx {
   int32_t* arena_keys = nullptr;
   void* arena_vals = nullptr;
   int64_t arena_slots = 0;
-  void* dense = nullptr;  // dense-row accumulators (all-zero between uses)
-  int64_t dense_elems = 0;
-  uint32_t* dense_bits = nullptr;  // their presence bitmaps (all-zero between uses)
-  int64_t dense_words = 0;
-  DenseChunk* chunks = nullptr;
+  DenseChunk* chunks = nullptr;  // dense-row chunk table
   int64_t chunks_cap = 0;
-  int32_t* dense_c = nullptr;
-  int32_t* dense_tcnt = nullptr;
-  int64_t* dense_toff = nullptr;
-  int64_t dense_cap = 0;  // slots of dense_c, tiles of dense_tcnt/toff
+  char* dscratch = nullptr;      // scratch of the dense-row sort
+  int64_t dscratch_bytes = 0;
 
   // learned frontier
   float* last_gain = nullptr;
@@ -191,12 +185,9 @@ void free_all(gala_ctx* ctx) {
   if (ctx->arena_vals) (void)hipFree(ctx->arena_vals);
   ctx->arena_keys = nullptr; ctx->arena_vals = nullptr;
   ctx->arena_slots = 0;
-  for (void* p : {ctx->dense, (void*)ctx->dense_bits, (void*)ctx->chunks, (void*)ctx->dense_c,
-                  (void*)ctx->dense_tcnt, (void*)ctx->dense_toff})
+  for (void* p : {(void*)ctx->chunks, (void*)ctx->dscratch})
     if (p) (void)hipFree(p);
-  ctx->dense = nullptr; ctx->dense_bits = nullptr; ctx->chunks = nullptr; ctx->dense_c = nullptr;
-  ctx->dense_tcnt = nullptr; ctx->dense_toff = nullptr; ctx->dense_elems = 0; ctx->dense_words = 0;
-  ctx->chunks_cap = 0; ctx->dense_cap = 0;
+  ctx->chunks = nullptr; ctx->dscratch = nullptr; ctx->chunks_cap = 0; ctx->dscratch_bytes = 0;
   ctx->have_graph = false;
 }
 
@@ -310,33 +301,10 @@ int grow_arena(gala_ctx* ctx, int64_t slots, bool fp) {
   return GALA_OK;
 }
 
-// Dense-row buffers for a contraction plan: accumulators for as many rows at
-// once as fit in DENSE_BYTES (HBM is plentiful), the chunk table, and tiles.
-constexpr int64_t DENSE_BYTES = 8ll << 30;
-int grow_dense(gala_ctx* ctx, const ContractPlan& plan, bool fp, int64_t* per_batch) {
-  const int64_t esz = fp ? 8 : 4;
-  const int64_t C = std::max<int32_t>(plan.C, 1);
-  int64_t pb = std::max<int64_t>(1, std::min<int64_t>(plan.ndense, DENSE_BYTES / (C * esz)));
-  *per_batch = pb;
-  const int64_t elems = pb * C;
+// Dense-row buffers for a contraction plan: the chunk table and the scratch of
+// the segmented sort (kept for later rounds when large enough).
+int grow_dense(gala_ctx* ctx, const ContractPlan& plan, bool fp) {
   hipError_t e = hipSuccess;
-  if (elems > ctx->dense_elems) {
-    if (ctx->dense) (void)hipFree(ctx->dense);
-    ctx->dense = nullptr;
-    e = hipMalloc(&ctx->dense, (size_t)(elems * esz));
-    if (e == hipSuccess) e = hipMemset(ctx->dense, 0, (size_t)(elems * esz));
-    if (e != hipSuccess) { ctx->last_hip = e; ctx->dense_elems = 0; return GALA_E_OOM; }
-    ctx->dense_elems = elems;
-  }
-  const int64_t words = pb * ((C + 31) / 32);
-  if (words > ctx->dense_words) {
-    if (ctx->dense_bits) (void)hipFree(ctx->dense_bits);
-    ctx->dense_bits = nullptr;
-    e = dalloc(&ctx->dense_bits, words);
-    if (e == hipSuccess) e = hipMemset(ctx->dense_bits, 0, sizeof(uint32_t) * words);
-    if (e != hipSuccess) { ctx->last_hip = e; ctx->dense_words = 0; return GALA_E_OOM; }
-    ctx->dense_words = words;
-  }
   const int64_t nch = (int64_t)plan.chunks.size();
   if (nch > ctx->chunks_cap) {
     if (ctx->chunks) (void)hipFree(ctx->chunks);
@@ -345,17 +313,13 @@ int grow_dense(gala_ctx* ctx, const ContractPlan& plan, bool fp, int64_t* per_ba
     if (e != hipSuccess) { ctx->last_hip = e; ctx->chunks_cap = 0; return GALA_E_OOM; }
     ctx->chunks_cap = nch;
   }
-  const int64_t tiles = pb * ((C + 4095) / 4096) + 1;  // DENSE_TILE keys per tile
-  const int64_t need = std::max<int64_t>(tiles, pb);
-  if (need > ctx->dense_cap) {
-    for (void* p : {(void*)ctx->dense_c, (void*)ctx->dense_tcnt, (void*)ctx->dense_toff})
-      if (p) (void)hipFree(p);
-    ctx->dense_c = nullptr; ctx->dense_tcnt = nullptr; ctx->dense_toff = nullptr;
-    e = dalloc(&ctx->dense_c, need);
-    if (e == hipSuccess) e = dalloc(&ctx->dense_tcnt, need);
-    if (e == hipSuccess) e = dalloc(&ctx->dense_toff, need + 1);
-    if (e != hipSuccess) { ctx->last_hip = e; ctx->dense_cap = 0; return GALA_E_OOM; }
-    ctx->dense_cap = need;
+  const int64_t bytes = dense_scratch_bytes(nch, plan.ndense, plan.dense_entries, fp);
+  if (bytes > ctx->dscratch_bytes) {
+    if (ctx->dscratch) (void)hipFree(ctx->dscratch);
+    ctx->dscratch = nullptr;
+    e = hipMalloc((void**)&ctx->dscratch, (size_t)bytes);
+    if (e != hipSuccess) { ctx->last_hip = e; ctx->dscratch_bytes = 0; return GALA_E_OOM; }
+    ctx->dscratch_bytes = bytes;
   }
   return GALA_OK;
 }
@@ -696,11 +660,8 @@ ContractBufs<T> cbufs(gala_ctx* ctx) {
   cb.arena_vals = (typename T::acc_t*)ctx->arena_vals; cb.arena_slots = ctx->arena_slots;
   cb.order = ctx->order;  // degree-bin order of the round graph (round_setup)
   cb.bstart = ctx->bstart_full;
-  cb.dense = (typename T::acc_t*)ctx->dense;
-  cb.dense_bits = ctx->dense_bits;
-  cb.dense_slots = 0;
-  cb.chunks = ctx->chunks; cb.dense_c = ctx->dense_c; cb.dense_tcnt = ctx->dense_tcnt;
-  cb.dense_toff = ctx->dense_toff;
+  cb.chunks = ctx->chunks;
+  cb.dscratch = ctx->dscratch;
   cb.i32_tmp = ctx->c_i32_tmp; cb.i64_tmp = ctx->c_i64_tmp; cb.i32_scalar = ctx->c_i32_scalar;
   cb.tier = ctx->c_tier; cb.tlist = ctx->c_tlist; cb.tstart = ctx->c_tstart; cb.part_tmp = ctx->part_tmp;
   cb.i64_scalar = ctx->c_i64_scalar;
@@ -721,15 +682,13 @@ int contract(gala_ctx* ctx, const DevGraph<T>& g, const int32_t* comm, int dst, 
   if (contract_prepare<T>(g, comm, cb, &plan, s)) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
   int rc = grow_arena(ctx, plan.slots, T::FP);
   if (rc) return rc;
-  int64_t per_batch = 0;
-  if (plan.ndense > 0 && (rc = grow_dense(ctx, plan, T::FP, &per_batch))) return rc;
+  if (plan.ndense > 0 && (rc = grow_dense(ctx, plan, T::FP))) return rc;
   ContractOut<T> out;
   out.off = ctx->off[dst];
   out.nbr = ctx->nbr[dst];
   out.w = (typename T::w_t*)ctx->w[dst];
   int64_t m = 0;
   cb = cbufs<T>(ctx);
-  cb.dense_slots = per_batch;
   if (contract_build<T>(g, cb, plan, out, &m, s)) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
   *C_out = plan.C;
   *m_out = m;

@@ -154,9 +154,12 @@ void launch_w1_bf16(const float* prm, short* w1b, hipStream_t s);
 // ---- contraction (kernels_contract.hip) ----
 // One chunk of a dense row's staged segment (see kernels_contract.hip).
 struct DenseChunk {
-  int32_t slot, c;  // dense-accumulator slot, community
+  int32_t slot, c;  // dense row index (ascending community), community
   int64_t start;    // offset inside the community's staged segment
-  int32_t len, pad;
+  int32_t len;      // staged entries of the chunk (<= LT/2)
+  int32_t first;    // index of the row's first chunk
+  int32_t nch;      // chunks of the row
+  int32_t pad;
 };
 
 struct ContractPlan {
@@ -168,7 +171,11 @@ struct ContractPlan {
   int32_t ndense; // rows with L > LT (dense accumulation)
   std::vector<int32_t> dense_c;     // their communities, ascending
   std::vector<DenseChunk> chunks;   // their segments cut into LT/2-entry chunks
+  int64_t dense_entries = 0;        // staged entries of all dense rows
 };
+
+// Bytes of device scratch the dense-row sort of a plan needs.
+int64_t dense_scratch_bytes(int64_t nchunks, int64_t nrows, int64_t entries, bool fp);
 
 template <class T>
 struct ContractBufs {
@@ -190,13 +197,8 @@ struct ContractBufs {
   int32_t* arena_keys;            // big-row overflow slices (all-zero between uses)
   typename T::acc_t* arena_vals;
   int64_t arena_slots;
-  typename T::acc_t* dense;       // dense_slots x C accumulators (all-zero between uses)
-  uint32_t* dense_bits;           // dense_slots x ceil(C/32) presence bitmaps (all-zero between uses)
-  int64_t dense_slots;
-  DenseChunk* chunks;             // chunk table of the current dense batch
-  int32_t* dense_c;               // community of each dense slot in the batch
-  int32_t* dense_tcnt;            // per (slot, tile) non-zero counts
-  int64_t* dense_toff;            // their exclusive scan (+ total)
+  DenseChunk* chunks;             // chunk table of the dense rows
+  char* dscratch;                 // dense_scratch_bytes() of scratch for their sort
   int32_t* i32_tmp;
   int64_t* i64_tmp;
   int32_t* i32_scalar;  // [4]

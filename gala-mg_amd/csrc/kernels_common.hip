@@ -80,6 +80,7 @@ template void scan_exclusive<int32_t, int32_t>(const int32_t*, int32_t*, int64_t
 template void scan_exclusive<int32_t, int64_t>(const int32_t*, int64_t*, int64_t, int64_t*, int64_t*, hipStream_t);
 template void scan_exclusive<int64_t, int64_t>(const int64_t*, int64_t*, int64_t, int64_t*, int64_t*, hipStream_t);
 template void scan_exclusive<uint8_t, int32_t>(const uint8_t*, int32_t*, int64_t, int32_t*, int32_t*, hipStream_t);
+template void scan_exclusive<uint8_t, int64_t>(const uint8_t*, int64_t*, int64_t, int64_t*, int64_t*, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Stable multi-way partition into NBINS bins (key < 0 drops the item).

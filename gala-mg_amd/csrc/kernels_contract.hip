@@ -33,8 +33,6 @@
 
 namespace gala {
 
-constexpr int DENSE_TILE = 4096;  // keys per workgroup in the dense-row scans
-
 __device__ __forceinline__ unsigned chash(int key, int lg) {
   return ((unsigned)key * 2654435769u) >> (32 - lg);
 }
@@ -478,24 +476,39 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// dense rows (L > LT).  chunk table: (dense slot, community, start, length)
+// dense rows (L > LT): sort-based merge.
+//  1. k_dense_agg: every chunk (<= LT/2 staged entries) is aggregated in an
+//     LDS hash table and its distinct (key, value) pairs are written back to
+//     the start of the chunk's span (ccnt[ch] of them); k_dense_pack then
+//     packs the chunks' pairs row after row into one buffer.
+//  2. the packed rows are cut into tiles of <= LT/2 pairs (a tile table
+//     built on the host from the row lengths) and sorted by a stable LSD radix
+//     sort over 8-bit digits of the key, segmented by row: per pass k_rs_hist
+//     counts the digits of every tile into hist[256*first + digit*nt +
+//     tile-in-row], one exclusive scan of that array gives every (row, digit,
+//     tile) its absolute output position, and k_rs_scatter ranks each tile's
+//     pairs stably (per wave with eight ballots, then across waves) and
+//     writes them to the other packed buffer.
+//  3. k_rs_heads / scan / k_rs_emit: equal keys are adjacent; each run head
+//     sums its run (at most one pair per chunk of the row) and writes the
+//     row's distinct pairs in ascending key order to the start of the row's
+//     staged segment, dcnt[c] = their count.
+// A tile table entry (DenseChunk) holds: slot = row, c = community, start =
+// packed offset of the tile, len = pairs, first/nch = the row's first tile
+// and tile count.
 // ---------------------------------------------------------------------------
 template <int LT, class T>
-__global__ __launch_bounds__(1024) void k_dense_accum(const DenseChunk* __restrict__ chunks, int nchunks,
-                                                      const int64_t* __restrict__ eoff, int32_t C,
-                                                      const int32_t* __restrict__ skey,
-                                                      const typename T::acc_t* __restrict__ sval,
-                                                      typename T::acc_t* __restrict__ dense,
-                                                      uint32_t* __restrict__ dbits, int64_t words) {
+__global__ __launch_bounds__(1024) void k_dense_agg(const DenseChunk* __restrict__ chunks, int nch,
+                                                    const int64_t* __restrict__ eoff, int32_t* __restrict__ skey,
+                                                    typename T::acc_t* __restrict__ sval, int32_t* __restrict__ ccnt) {
   using acc = typename T::acc_t;
   constexpr int NT = 1024, LLG = __builtin_ctz(LT);
   __shared__ int s_k[LT];
   __shared__ acc s_v[LT];
-  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+  __shared__ int s_sh[17];
+  for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
     const DenseChunk cc = chunks[ch];
     const int64_t base = eoff[cc.c] + cc.start;
-    acc* row = dense + (int64_t)cc.slot * C;
-    uint32_t* bits = dbits + (int64_t)cc.slot * words;
     for (int t = threadIdx.x; t < LT; t += NT) { s_k[t] = 0; s_v[t] = 0; }
     __syncthreads();
     // a chunk holds at most LT/2 entries, so the table never fills
@@ -510,72 +523,204 @@ __global__ __launch_bounds__(1024) void k_dense_accum(const DenseChunk* __restri
       }
     }
     __syncthreads();
+    int cnt = 0;
+    for (int t = threadIdx.x; t < LT; t += NT) cnt += s_k[t] != 0;
+    int D;
+    int q = block_excl_scan<int>(cnt, s_sh, &D);
     for (int t = threadIdx.x; t < LT; t += NT) {
       const int k = s_k[t];
       if (k) {
-        atomicAdd(&row[k - 1], s_v[t]);
-        atomicOr(&bits[((k - 1) >> 5)], 1u << ((k - 1) & 31));
+        skey[base + q] = k - 1;
+        sval[base + q] = s_v[t];
+        q++;
+      }
+    }
+    if (threadIdx.x == 0) ccnt[ch] = D;
+    __syncthreads();
+  }
+}
+
+// chunk ch's pairs to [pstart[ch], pstart[ch] + ccnt[ch]) of the packed buffer;
+// bound[row] = packed start of every row, bound[nrows] = packed total
+template <class T>
+__global__ __launch_bounds__(256) void k_dense_pack(const DenseChunk* __restrict__ chunks, int nch,
+                                                    const int32_t* __restrict__ ccnt, const int64_t* __restrict__ pstart,
+                                                    const int64_t* __restrict__ eoff, const int32_t* __restrict__ skey,
+                                                    const typename T::acc_t* __restrict__ sval,
+                                                    int32_t* __restrict__ kout, typename T::acc_t* __restrict__ vout,
+                                                    int nrows, int64_t* __restrict__ bound) {
+  for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+    const DenseChunk cc = chunks[ch];
+    const int64_t src = eoff[cc.c] + cc.start, dst = pstart[ch];
+    const int n = ccnt[ch];
+    for (int i = threadIdx.x; i < n; i += 256) {
+      kout[dst + i] = skey[src + i];
+      vout[dst + i] = sval[src + i];
+    }
+    if (threadIdx.x == 0) {
+      if (ch == cc.first) bound[cc.slot] = dst;
+      if (ch == nch - 1) bound[nrows] = pstart[nch];
+    }
+  }
+}
+
+// digit counts of every tile (layout: see above)
+__global__ __launch_bounds__(256) void k_rs_hist(const DenseChunk* __restrict__ tiles, int ntiles,
+                                                 const int32_t* __restrict__ kin, int shift,
+                                                 int32_t* __restrict__ hist) {
+  __shared__ int cnt[256];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const DenseChunk tt = tiles[t];
+    const int32_t* src = kin + tt.start;
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < tt.len; i += 256) atomicAdd(&cnt[(src[i] >> shift) & 255], 1);
+    __syncthreads();
+    hist[256ll * tt.first + (int64_t)threadIdx.x * tt.nch + (t - tt.first)] = cnt[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// stable scatter of every tile's pairs to their digit positions
+template <int LT, class T>
+__global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restrict__ tiles, int ntiles,
+                                                     const int32_t* __restrict__ kin,
+                                                     const typename T::acc_t* __restrict__ vin, int shift,
+                                                     const int64_t* __restrict__ hscan, int32_t* __restrict__ kout,
+                                                     typename T::acc_t* __restrict__ vout) {
+  using acc = typename T::acc_t;
+  constexpr int NW = 16, R = (LT / 2) / (NW * 64);  // rounds of 64 pairs per wave
+  __shared__ int wcnt[NW][256];
+  __shared__ int64_t gofs[256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long below_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const DenseChunk tt = tiles[t];
+    const int n = tt.len;
+    const int64_t b0 = tt.start;
+    for (int q = threadIdx.x; q < NW * 256; q += 1024) (&wcnt[0][0])[q] = 0;
+    if (threadIdx.x < 256)
+      gofs[threadIdx.x] = hscan[256ll * tt.first + (int64_t)threadIdx.x * tt.nch + (t - tt.first)];
+    __syncthreads();
+    int key[R], rk[R];
+    acc val[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = (w * R + r) * 64 + lane;
+      const bool ok = i < n;
+      key[r] = ok ? kin[b0 + i] : 0;
+      val[r] = ok ? vin[b0 + i] : acc(0);
+      const int d = (key[r] >> shift) & 255;
+      unsigned long long peers = __ballot(ok);
+#pragma unroll
+      for (int bit = 0; bit < 8; bit++) {
+        const bool on = (d >> bit) & 1;
+        const unsigned long long bb = __ballot(on);
+        peers &= on ? bb : ~bb;
+      }
+      const int below = __popcll(peers & below_mask);
+      const int base = wcnt[w][d];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      if (ok && below == 0) wcnt[w][d] = base + __popcll(peers);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      rk[r] = base + below;
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) {  // exclusive prefix over the waves, per digit
+      int run = 0;
+      for (int q = 0; q < NW; q++) {
+        const int x = wcnt[q][threadIdx.x];
+        wcnt[q][threadIdx.x] = run;
+        run += x;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = (w * R + r) * 64 + lane;
+      if (i < n) {
+        const int d = (key[r] >> shift) & 255;
+        const int64_t pos = gofs[d] + wcnt[w][d] + rk[r];
+        kout[pos] = key[r];
+        vout[pos] = val[r];
       }
     }
     __syncthreads();
   }
 }
 
-// present-key count of every DENSE_TILE-key tile of every dense row (bitmap popcounts)
-__global__ __launch_bounds__(128) void k_dense_count(const uint32_t* __restrict__ dbits, int nslots, int64_t words,
-                                                     int ntiles, int32_t* __restrict__ tcnt) {
-  __shared__ int32_t sh[17];
-  constexpr int TW = DENSE_TILE / 32;  // words per tile (one per thread)
-  const int total = nslots * ntiles;
-  for (int t = blockIdx.x; t < total; t += gridDim.x) {
-    const int slot = t / ntiles, tile = t % ntiles;
-    const int64_t wi = (int64_t)tile * TW + threadIdx.x;
-    const int cnt = wi < words ? __popc(dbits[(int64_t)slot * words + wi]) : 0;
-    int tot;
-    block_excl_scan<int32_t>(cnt, sh, &tot);
-    if (threadIdx.x == 0) tcnt[t] = tot;
+// run heads of the sorted packed pairs (the first pair of a row is a head)
+__global__ __launch_bounds__(256) void k_rs_heads(const DenseChunk* __restrict__ tiles, int ntiles,
+                                                  const int32_t* __restrict__ keys, uint8_t* __restrict__ heads) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const DenseChunk tt = tiles[t];
+    const int64_t seg0 = tiles[tt.first].start;
+    for (int i = threadIdx.x; i < tt.len; i += 256) {
+      const int64_t g = tt.start + i;
+      heads[g] = (g == seg0 || keys[g] != keys[g - 1]) ? 1 : 0;
+    }
   }
 }
 
-// ordered extraction: each tile writes its present keys (ascending) at the
-// row's running position and clears the accumulator and bitmap words it read
+// every run head sums its run and writes the row's i-th distinct pair
 template <class T>
-__global__ __launch_bounds__(128) void k_dense_scatter(typename T::acc_t* __restrict__ dense,
-                                                       uint32_t* __restrict__ dbits, int nslots, int32_t C,
-                                                       int64_t words, int ntiles,
-                                                       const int32_t* __restrict__ slot_c,
-                                                       const int64_t* __restrict__ toff,
-                                                       const int64_t* __restrict__ eoff,
-                                                       int32_t* __restrict__ skey,
-                                                       typename T::acc_t* __restrict__ sval,
-                                                       int32_t* __restrict__ dcnt) {
+__global__ __launch_bounds__(256) void k_rs_emit(const DenseChunk* __restrict__ tiles, int ntiles,
+                                                 const int32_t* __restrict__ keys,
+                                                 const typename T::acc_t* __restrict__ vals,
+                                                 const uint8_t* __restrict__ heads, const int64_t* __restrict__ hs,
+                                                 const int64_t* __restrict__ eoff, int32_t* __restrict__ skey,
+                                                 typename T::acc_t* __restrict__ sval, int32_t* __restrict__ dcnt) {
   using acc = typename T::acc_t;
-  constexpr int TW = DENSE_TILE / 32;
-  __shared__ int32_t sh[17];
-  const int total = nslots * ntiles;
-  for (int t = blockIdx.x; t < total; t += gridDim.x) {
-    const int slot = t / ntiles, tile = t % ntiles;
-    acc* row = dense + (int64_t)slot * C;
-    uint32_t* bits = dbits + (int64_t)slot * words;
-    const int c = slot_c[slot];
-    const int64_t rbase = toff[slot * ntiles];
-    const int64_t wi = (int64_t)tile * TW + threadIdx.x;
-    uint32_t wbits = wi < words ? bits[wi] : 0u;
-    int tot;
-    const int pos = block_excl_scan<int32_t>(__popc(wbits), sh, &tot);
-    int64_t out = eoff[c] + (toff[t] - rbase) + pos;
-    if (wbits) bits[wi] = 0;
-    while (wbits) {
-      const int bit = __ffs(wbits) - 1;
-      wbits &= wbits - 1;
-      const int k = (int)(wi * 32 + bit);
-      skey[out] = k;
-      sval[out] = row[k];
-      row[k] = 0;
-      out++;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const DenseChunk tt = tiles[t];
+    const DenseChunk tl = tiles[tt.first + tt.nch - 1];
+    const int64_t seg0 = tiles[tt.first].start, seg1 = tl.start + tl.len;
+    const int64_t out0 = eoff[tt.c];
+    if (t == tt.first && threadIdx.x == 0) dcnt[tt.c] = (int32_t)(hs[seg1] - hs[seg0]);
+    for (int i = threadIdx.x; i < tt.len; i += 256) {
+      const int64_t g = tt.start + i;
+      if (!heads[g]) continue;
+      acc sum = vals[g];
+      for (int64_t j = g + 1; j < seg1 && !heads[j]; j++) sum += vals[j];
+      const int64_t o = out0 + (hs[g] - hs[seg0]);
+      skey[o] = keys[g];
+      sval[o] = sum;
     }
-    if (tile == 0 && threadIdx.x == 0) dcnt[c] = (int32_t)(toff[(slot + 1) * ntiles] - rbase);
   }
+}
+
+// Scratch layout of the dense-row sort (offsets in bytes, 256-byte aligned).
+struct DenseLayout {
+  int64_t ccnt, pstart, bound, tiles, hist, hscan, k0, k1, v0, v1, heads, hs, tmp, total;
+};
+// nch chunks, nrows rows, P staged entries; tiles of the packed rows are at
+// most nch + nrows
+static DenseLayout dense_layout(int64_t nch, int64_t nrows, int64_t P, int64_t vsz) {
+  auto al = [](int64_t x) { return (x + 255) & ~(int64_t)255; };
+  const int64_t nt = nch + nrows;
+  DenseLayout L{};
+  int64_t o = 0;
+  L.ccnt = o; o = al(o + 4 * nch);
+  L.pstart = o; o = al(o + 8 * (nch + 1));
+  L.bound = o; o = al(o + 8 * (nrows + 1));
+  L.tiles = o; o = al(o + (int64_t)sizeof(DenseChunk) * nt);
+  L.hist = o; o = al(o + 4 * 256 * nt);
+  L.hscan = o; o = al(o + 8 * (256 * nt + 1));
+  L.k0 = o; o = al(o + 4 * P);
+  L.k1 = o; o = al(o + 4 * P);
+  L.v0 = o; o = al(o + vsz * P);
+  L.v1 = o; o = al(o + vsz * P);
+  L.heads = o; o = al(o + P);
+  L.hs = o; o = al(o + 8 * (P + 1));
+  L.tmp = o; o = al(o + 8 * (scan_tmp_elems(std::max<int64_t>(P, 256 * nt)) + 1));
+  L.total = o;
+  return L;
+}
+
+int64_t dense_scratch_bytes(int64_t nchunks, int64_t nrows, int64_t entries, bool fp) {
+  return dense_layout(nchunks, nrows, entries, fp ? 8 : 4).total;
 }
 
 // copy each merged row to its final place: 16-lane groups for rows of at
@@ -702,13 +847,16 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
     for (int32_t i = 0; i < ndense; i++) idx[i] = i;
     std::sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return hc[a] < hc[b]; });
     plan->dense_c.resize(ndense);
+    plan->dense_entries = 0;
     const int64_t CH = LT / 2;  // entries per chunk: the LDS table stays at most half full
     for (int32_t i = 0; i < ndense; i++) {
       const int32_t c = hc[idx[i]];
       const int64_t L = hl[idx[i]];
       plan->dense_c[i] = c;
+      plan->dense_entries += L;
+      const int32_t first = (int32_t)plan->chunks.size(), nc = (int32_t)((L + CH - 1) / CH);
       for (int64_t st = 0; st < L; st += CH)
-        plan->chunks.push_back({i, c, st, (int32_t)std::min<int64_t>(CH, L - st)});
+        plan->chunks.push_back({i, c, st, (int32_t)std::min<int64_t>(CH, L - st), first, nc, 0});
     }
   }
   return 0;
@@ -753,34 +901,70 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
                                                                    cb.big_tab, cb.arena_keys, cb.arena_vals,
                                                                    cb.skey, cb.sval, cb.dcnt);
   if (plan.ndense > 0) {
-    const int ntiles = (C + DENSE_TILE - 1) / DENSE_TILE;
-    const int per_batch = (int)std::max<int64_t>(1, std::min<int64_t>(plan.ndense, cb.dense_slots));
-    size_t ci = 0;
-    for (int s0 = 0; s0 < plan.ndense; s0 += per_batch) {
-      const int nb = std::min(per_batch, plan.ndense - s0);
-      // chunks of this batch, slots renumbered from 0
-      std::vector<DenseChunk> ch;
-      while (ci < plan.chunks.size() && plan.chunks[ci].slot < s0 + nb) {
-        DenseChunk d = plan.chunks[ci++];
-        d.slot -= s0;
-        ch.push_back(d);
-      }
-      if (hipMemcpyAsync(cb.chunks, ch.data(), sizeof(DenseChunk) * ch.size(), hipMemcpyHostToDevice, s) != hipSuccess)
-        return -1;
-      if (hipMemcpyAsync(cb.dense_c, plan.dense_c.data() + s0, sizeof(int32_t) * nb, hipMemcpyHostToDevice, s) !=
-          hipSuccess)
-        return -1;
-      const int64_t words = (C + 31) / 32;
-      k_dense_accum<LT, T><<<(int)std::min<size_t>(ch.size(), 4096), 1024, 0, s>>>(
-          cb.chunks, (int)ch.size(), cb.eoff, C, cb.skey, cb.sval, cb.dense, cb.dense_bits, words);
-      k_dense_count<<<std::min(nb * ntiles, 16384), 128, 0, s>>>(cb.dense_bits, nb, words, ntiles, cb.dense_tcnt);
-      scan_exclusive<int32_t, int64_t>(cb.dense_tcnt, cb.dense_toff, (int64_t)nb * ntiles, cb.i64_tmp,
-                                       cb.dense_toff + (int64_t)nb * ntiles, s);
-      k_dense_scatter<T><<<std::min(nb * ntiles, 16384), 128, 0, s>>>(cb.dense, cb.dense_bits, nb, C, words,
-                                                                       ntiles, cb.dense_c, cb.dense_toff, cb.eoff,
-                                                                       cb.skey, cb.sval, cb.dcnt);
-      if (hipStreamSynchronize(s) != hipSuccess) return -1;  // host chunk vector reused
+    using acc = typename T::acc_t;
+    const int nch = (int)plan.chunks.size();
+    const int nrows = plan.ndense;
+    const int64_t P = plan.dense_entries;
+    const DenseLayout L = dense_layout(nch, nrows, P, sizeof(acc));
+    char* d = cb.dscratch;
+    int32_t* ccnt = (int32_t*)(d + L.ccnt);
+    int64_t* pstart = (int64_t*)(d + L.pstart);
+    int64_t* bound = (int64_t*)(d + L.bound);
+    DenseChunk* tiles = (DenseChunk*)(d + L.tiles);
+    int32_t* hist = (int32_t*)(d + L.hist);
+    int64_t* hscan = (int64_t*)(d + L.hscan);
+    int32_t* kb[2] = {(int32_t*)(d + L.k0), (int32_t*)(d + L.k1)};
+    acc* vb[2] = {(acc*)(d + L.v0), (acc*)(d + L.v1)};
+    uint8_t* heads = (uint8_t*)(d + L.heads);
+    int64_t* hs = (int64_t*)(d + L.hs);
+    int64_t* tmp = (int64_t*)(d + L.tmp);
+    if (hipMemcpyAsync(cb.chunks, plan.chunks.data(), sizeof(DenseChunk) * nch, hipMemcpyHostToDevice, s) !=
+        hipSuccess)
+      return -1;
+    k_dense_agg<LT, T><<<std::min(nch, 4096), 1024, 0, s>>>(cb.chunks, nch, cb.eoff, cb.skey, cb.sval, ccnt);
+    scan_exclusive<int32_t, int64_t>(ccnt, pstart, nch, tmp, pstart + nch, s);
+    // pass 0 reads kb[1]; pass p writes kb[p & 1]
+    k_dense_pack<T><<<std::min(nch, 8192), 256, 0, s>>>(cb.chunks, nch, ccnt, pstart, cb.eoff, cb.skey, cb.sval,
+                                                         kb[1], vb[1], nrows, bound);
+    // tile table of the packed rows
+    std::vector<int64_t> hbv;
+    int64_t* hb = (int64_t*)cb.hpin;
+    if ((int64_t)sizeof(int64_t) * (nrows + 1) > cb.hpin_bytes) {
+      hbv.resize(nrows + 1);
+      hb = hbv.data();
     }
+    if (hipMemcpyAsync(hb, bound, sizeof(int64_t) * (nrows + 1), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    constexpr int64_t TL = LT / 2;
+    std::vector<DenseChunk> tv;
+    tv.reserve(nch + nrows);
+    for (int32_t r = 0; r < nrows; r++) {
+      const int64_t len = hb[r + 1] - hb[r];
+      const int32_t first = (int32_t)tv.size(), nt = (int32_t)((len + TL - 1) / TL);
+      for (int64_t o = 0; o < len; o += TL)
+        tv.push_back({r, plan.dense_c[r], hb[r] + o, (int32_t)std::min<int64_t>(TL, len - o), first, nt, 0});
+    }
+    const int ntl = (int)tv.size();
+    if (hipMemcpyAsync(tiles, tv.data(), sizeof(DenseChunk) * ntl, hipMemcpyHostToDevice, s) != hipSuccess)
+      return -1;
+    const int bits = C > 1 ? 32 - __builtin_clz((unsigned)(C - 1)) : 1;
+    const int npass = (bits + 7) / 8;
+    for (int p = 0; p < npass; p++) {
+      const int32_t* kin = kb[(p + 1) & 1];
+      const acc* vin = vb[(p + 1) & 1];
+      k_rs_hist<<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kin, 8 * p, hist);
+      scan_exclusive<int32_t, int64_t>(hist, hscan, 256ll * ntl, tmp, hscan + 256ll * ntl, s);
+      k_rs_scatter<LT, T><<<std::min(ntl, 4096), 1024, 0, s>>>(tiles, ntl, kin, vin, 8 * p, hscan, kb[p & 1],
+                                                                vb[p & 1]);
+    }
+    const int32_t* kf = kb[(npass - 1) & 1];
+    const acc* vf = vb[(npass - 1) & 1];
+    const int64_t Pk = hb[nrows];  // packed pairs
+    k_rs_heads<<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kf, heads);
+    scan_exclusive<uint8_t, int64_t>(heads, hs, Pk, tmp, hs + Pk, s);
+    k_rs_emit<T><<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kf, vf, heads, hs, cb.eoff, cb.skey, cb.sval,
+                                                      cb.dcnt);
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;  // the host tile vector is in flight until here
   }
   scan_exclusive<int32_t, int64_t>(cb.dcnt, out.off, C, cb.i64_tmp, out.off + C, s);
   const int32_t nsmall = ts[TIER_MID3 + 1] - ts[TIER_T16];
