@@ -478,7 +478,13 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   // Iteration 0: Q on the singleton partition (louvain_main_process.cu:538-548)
   reset_counters(ctx->ctr, (long long)m2, m2d, s);
   launch_apply<T>(st, g.n, m2, s);
-  for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], false, s, ctx->wvar[b]);
+  if (unique_rows) {
+    // singleton partition, at most one self entry per row: In = Self
+    HIPOK(hipMemcpyAsync(st.In, st.Self, sizeof(tt) * g.n, hipMemcpyDeviceToDevice, s));
+  } else {
+    for (int b = 0; b < NBINS; b++)
+      launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], false, s, ctx->wvar[b]);
+  }
   launch_prune<T>(st, g.n, m2, s);
   Counters c;
   if ((rc = read_counters<T>(ctx, &c))) return rc;
@@ -1006,14 +1012,20 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     launch_batch_live(b, it_dev, s);
     // iteration 0: the singleton partition's Q of every running graph
     launch_batch_reset(ctx->ctr, m2max, nlive, s);
+    const bool unique_rows = src != 0 || ctx->unique0;
     launch_apply<T>(st, g.n, m2max, s);
-    for (int k = 0; k < NBINS; k++) launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], false, s, ctx->wvar[k]);
+    if (unique_rows) {
+      // singleton partition, at most one self entry per row: In = Self
+      HIPOK(hipMemcpyAsync(st.In, st.Self, sizeof(*st.In) * g.n, hipMemcpyDeviceToDevice, s));
+    } else {
+      for (int k = 0; k < NBINS; k++)
+        launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], false, s, ctx->wvar[k]);
+    }
     launch_prune<T>(st, g.n, m2max, s);
     launch_graph_sums(gid, ctx->binid, (const int32_t*)ctx->In, (const int32_t*)ctx->Tot, g.n, b.g_in, b.g_tot2, s);
     launch_batch_step(b, it_dev, min_mod, max_it, s);
     launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, true, it_dev, s);
     partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp, s);
-    const bool unique_rows = src != 0 || ctx->unique0;
     // one iteration for every running graph; a graph that stopped is inert
     auto body = [&](bool first) {
       launch_batch_reset(ctx->ctr, m2max, nlive, s);

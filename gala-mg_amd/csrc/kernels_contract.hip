@@ -101,76 +101,9 @@ __global__ void k_classify(const int64_t* __restrict__ ecnt, int32_t C, long lon
 
 // ---------------------------------------------------------------------------
 // staging: (label of the neighbour, weight) into the owner's segment.
-// Rows of fewer than 512 entries: a block owns 256 consecutive vertices and
-// reserves space with one global atomic per distinct community in the block
-// (LDS aggregation first, so a community with many members is not a hot spot),
-// then all threads copy the block's rows.  Longer rows: k_stage_rows.
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(256) void k_stage(DevGraph<T> g, const int32_t* __restrict__ label,
-                                               const int64_t* __restrict__ eoff,
-                                               unsigned long long* __restrict__ ecur,
-                                               int32_t* __restrict__ skey,
-                                               typename T::acc_t* __restrict__ sval) {
-  using acc = typename T::acc_t;
-  constexpr int HS = 512;
-  __shared__ int64_t s_ofs[257];
-  __shared__ int64_t s_dst[256];
-  __shared__ int64_t s_row[256];
-  __shared__ int s_hk[HS];
-  __shared__ unsigned long long s_hv[HS];
-  __shared__ unsigned long long s_hb[HS];
-  __shared__ int64_t s_sh[17];
-  for (int vb = blockIdx.x * 256; vb < g.n; vb += gridDim.x * 256) {
-    const int nloc = min(256, g.n - vb);
-    const int v = vb + threadIdx.x;
-    const bool valid = threadIdx.x < nloc;
-    const int64_t o = valid ? g.off[v] : 0;
-    const int64_t d0 = valid ? g.off[v + 1] - o : 0;
-    const int64_t d = d0 < 512 ? d0 : 0;  // longer rows (degree bins 6-8): k_stage_rows
-    const int nc = (valid && d) ? label[v] : -1;
-    for (int t = threadIdx.x; t < HS; t += 256) { s_hk[t] = 0; s_hv[t] = 0; }
-    int64_t tot;
-    const int64_t pre = block_excl_scan<int64_t>(d, s_sh, &tot);
-    s_ofs[threadIdx.x] = pre;
-    if (threadIdx.x == 0) s_ofs[nloc] = tot;
-    __syncthreads();
-    unsigned p = 0;
-    unsigned long long intra = 0;
-    if (nc >= 0) {
-      p = chash(nc, 9);
-      for (;;) {
-        const int k = atomicCAS(&s_hk[p], 0, nc + 1);
-        if (k == 0 || k == nc + 1) break;
-        p = (p + 1) & (HS - 1);
-      }
-      intra = atomicAdd(&s_hv[p], (unsigned long long)d);
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < HS; t += 256)
-      if (s_hk[t]) s_hb[t] = atomicAdd(&ecur[s_hk[t] - 1], s_hv[t]);
-    __syncthreads();
-    if (nc >= 0) s_dst[threadIdx.x] = eoff[nc] + (int64_t)(s_hb[p] + intra);
-    __syncthreads();
-    if (valid) s_row[threadIdx.x] = o;
-    __syncthreads();
-    for (int64_t q = threadIdx.x; q < tot; q += 256) {
-      int a = 0, z = nloc;  // largest i with s_ofs[i] <= q
-      while (z - a > 1) {
-        const int mid = (a + z) >> 1;
-        if (s_ofs[mid] <= q) a = mid; else z = mid;
-      }
-      const int64_t j = s_row[a] + (q - s_ofs[a]);
-      const int64_t dst = s_dst[a] + (q - s_ofs[a]);
-      skey[dst] = label[g.nbr[j]];
-      sval[dst] = g.w ? (acc)g.w[j] : acc(1);
-    }
-    __syncthreads();
-  }
-}
-
 // rows of the vertices order[bstart[blo] .. bstart[bhi]) (the round's degree
-// bins): NT threads per vertex, one reservation each
+// bins): NT lanes per vertex, one reservation each
 template <int NT, class T>
 __global__ __launch_bounds__(256) void k_stage_rows(DevGraph<T> g, const int32_t* __restrict__ order,
                                                     const int32_t* __restrict__ bstart, int blo, int bhi,
@@ -489,10 +422,11 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
 //     tile) its absolute output position, and k_rs_scatter ranks each tile's
 //     pairs stably (per wave with eight ballots, then across waves) and
 //     writes them to the other packed buffer.
-//  3. k_rs_heads / scan / k_rs_emit: equal keys are adjacent; each run head
-//     sums its run (at most one pair per chunk of the row) and writes the
-//     row's distinct pairs in ascending key order to the start of the row's
-//     staged segment, dcnt[c] = their count.
+//  3. k_rs_heads / scan / k_rs_emit / k_rs_sum: equal keys are adjacent;
+//     each run head writes its key at the run's rank from the start of the
+//     row's staged segment, and every pair adds its value into that slot, so
+//     the row's distinct pairs end up in ascending key order, dcnt[c] = their
+//     count.
 // A tile table entry (DenseChunk) holds: slot = row, c = community, start =
 // packed offset of the tile, len = pairs, first/nch = the row's first tile
 // and tile count.
@@ -664,15 +598,14 @@ __global__ __launch_bounds__(256) void k_rs_heads(const DenseChunk* __restrict__
   }
 }
 
-// every run head sums its run and writes the row's i-th distinct pair
+// every run head writes the row's i-th distinct key (value zeroed), the
+// row's first tile its distinct count ...
 template <class T>
 __global__ __launch_bounds__(256) void k_rs_emit(const DenseChunk* __restrict__ tiles, int ntiles,
-                                                 const int32_t* __restrict__ keys,
-                                                 const typename T::acc_t* __restrict__ vals,
-                                                 const uint8_t* __restrict__ heads, const int64_t* __restrict__ hs,
-                                                 const int64_t* __restrict__ eoff, int32_t* __restrict__ skey,
-                                                 typename T::acc_t* __restrict__ sval, int32_t* __restrict__ dcnt) {
-  using acc = typename T::acc_t;
+                                                 const int32_t* __restrict__ keys, const uint8_t* __restrict__ heads,
+                                                 const int64_t* __restrict__ hs, const int64_t* __restrict__ eoff,
+                                                 int32_t* __restrict__ skey, typename T::acc_t* __restrict__ sval,
+                                                 int32_t* __restrict__ dcnt) {
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const DenseChunk tt = tiles[t];
     const DenseChunk tl = tiles[tt.first + tt.nch - 1];
@@ -682,11 +615,27 @@ __global__ __launch_bounds__(256) void k_rs_emit(const DenseChunk* __restrict__ 
     for (int i = threadIdx.x; i < tt.len; i += 256) {
       const int64_t g = tt.start + i;
       if (!heads[g]) continue;
-      acc sum = vals[g];
-      for (int64_t j = g + 1; j < seg1 && !heads[j]; j++) sum += vals[j];
       const int64_t o = out0 + (hs[g] - hs[seg0]);
       skey[o] = keys[g];
-      sval[o] = sum;
+      sval[o] = 0;
+    }
+  }
+}
+
+// ... then every pair adds its value into its run's slot (runs hold at most
+// one pair per chunk of the row)
+template <class T>
+__global__ __launch_bounds__(256) void k_rs_sum(const DenseChunk* __restrict__ tiles, int ntiles,
+                                                const typename T::acc_t* __restrict__ vals,
+                                                const uint8_t* __restrict__ heads, const int64_t* __restrict__ hs,
+                                                const int64_t* __restrict__ eoff, typename T::acc_t* __restrict__ sval) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const DenseChunk tt = tiles[t];
+    const int64_t seg0 = tiles[tt.first].start;
+    const int64_t out0 = eoff[tt.c] - hs[seg0];
+    for (int i = threadIdx.x; i < tt.len; i += 256) {
+      const int64_t g = tt.start + i;
+      atomicAdd(&sval[out0 + hs[g] - (heads[g] ? 0 : 1)], vals[g]);
     }
   }
 }
@@ -870,11 +819,20 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
   const int32_t C = plan.C;
   fill<int64_t>(cb.ecur, C, 0, s);
   unsigned long long* ecur = (unsigned long long*)cb.ecur;
-  k_stage<T><<<nblk(n, 256, 16384), 256, 0, s>>>(g, cb.label, cb.eoff, ecur, cb.skey, cb.sval);
-  k_stage_rows<64, T><<<nblk(plan.caps[6], 4, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 6, 7, cb.label, cb.eoff,
-                                                                    ecur, cb.skey, cb.sval);
-  k_stage_rows<256, T><<<nblk(plan.caps[7] + plan.caps[8], 1, 4096), 256, 0, s>>>(
-      g, cb.order, cb.bstart, 7, NBINS, cb.label, cb.eoff, ecur, cb.skey, cb.sval);
+  // W lanes per row, W matched to the round's degree bins, one reservation per row
+  const int32_t* cp = plan.caps;
+  k_stage_rows<4, T><<<nblk(cp[0], 64, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 0, 1, cb.label, cb.eoff, ecur,
+                                                           cb.skey, cb.sval);
+  k_stage_rows<8, T><<<nblk(cp[1], 32, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 1, 2, cb.label, cb.eoff, ecur,
+                                                           cb.skey, cb.sval);
+  k_stage_rows<16, T><<<nblk(cp[2], 16, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 2, 3, cb.label, cb.eoff, ecur,
+                                                            cb.skey, cb.sval);
+  k_stage_rows<32, T><<<nblk(cp[3], 8, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 3, 4, cb.label, cb.eoff, ecur,
+                                                           cb.skey, cb.sval);
+  k_stage_rows<64, T><<<nblk(cp[4] + cp[5] + cp[6], 4, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 4, 7, cb.label,
+                                                                           cb.eoff, ecur, cb.skey, cb.sval);
+  k_stage_rows<256, T><<<nblk(cp[7] + cp[8], 1, 4096), 256, 0, s>>>(g, cb.order, cb.bstart, 7, NBINS, cb.label,
+                                                                    cb.eoff, ecur, cb.skey, cb.sval);
   fill<int32_t>(cb.dcnt, C + 1, 0, s);
   const int32_t* ts = plan.tstart;
   auto tcount = [&](int t) { return ts[t + 1] - ts[t]; };
@@ -962,8 +920,8 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     const int64_t Pk = hb[nrows];  // packed pairs
     k_rs_heads<<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kf, heads);
     scan_exclusive<uint8_t, int64_t>(heads, hs, Pk, tmp, hs + Pk, s);
-    k_rs_emit<T><<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kf, vf, heads, hs, cb.eoff, cb.skey, cb.sval,
-                                                      cb.dcnt);
+    k_rs_emit<T><<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kf, heads, hs, cb.eoff, cb.skey, cb.sval, cb.dcnt);
+    k_rs_sum<T><<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, vf, heads, hs, cb.eoff, cb.sval);
     if (hipStreamSynchronize(s) != hipSuccess) return -1;  // the host tile vector is in flight until here
   }
   scan_exclusive<int32_t, int64_t>(cb.dcnt, out.off, C, cb.i64_tmp, out.off + C, s);
