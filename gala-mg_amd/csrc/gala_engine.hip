@@ -143,10 +143,11 @@ struct gala_ctx {
 
   // decide kernel variant per degree bin, general and first sweeps; the
   // defaults are what gala_autotune picks on the 4M-vertex power-law graph
-  // (pipelined kernels for degrees < 128)
-  int dvar[NBINS] = {2, 2, 2, 2, 3, 0, 0, 0, 0};
-  int fvar[NBINS] = {1, 1, 1, 1, 1, 0, 0, 0, 0};
-  int wvar[NBINS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // weight update (1 = pipelined, bins 0-4)
+  // (one thread per vertex below degree 32, the pipelined hash kernel for
+  // 32..127)
+  int dvar[NBINS] = {3, 3, 3, 3, 3, 0, 0, 0, 0};
+  int fvar[NBINS] = {2, 2, 2, 2, 1, 0, 0, 0, 0};
+  int wvar[NBINS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // weight update (1 = pipelined, bins 0-4; 2 = one thread per vertex, bins 0-3)
 
   std::vector<void*> owned;
   int32_t cap_n = -1;  // sizes the owned buffers were allocated for
@@ -1232,7 +1233,7 @@ int gala_run(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gal
   return is_fp(ctx) ? run_all<FltW>(ctx, icb, rcb, user, out) : run_all<IntW>(ctx, icb, rcb, user, out);
 }
 
-static int weight_variants(int bin) { return bin <= 4 ? 2 : 1; }
+static int weight_variants(int bin) { return bin <= 3 ? 3 : bin == 4 ? 2 : 1; }
 
 int gala_set_variants(gala_ctx* ctx, const int32_t* variants) {
   if (!ctx || !variants) return GALA_E_INVALID;
@@ -1290,7 +1291,7 @@ int gala_autotune(gala_ctx* ctx, int32_t* variants_out) {
       if (k < decide_variants(b) && r.bin_decide_ms[b] < best_d[b]) { best_d[b] = r.bin_decide_ms[b]; bd[b] = k; }
       if (k < decide_first_variants(b) && r.bin_first_ms[b] < best_f[b]) { best_f[b] = r.bin_first_ms[b]; bf[b] = k; }
     }
-    if (k < 2 && r.t_update_ms < best_w) { best_w = r.t_update_ms; bw = k; }
+    if (k < weight_variants(0) && r.t_update_ms < best_w) { best_w = r.t_update_ms; bw = k; }
   }
   ctx->opts = saved;
   for (int b = 0; b < NBINS; b++) {

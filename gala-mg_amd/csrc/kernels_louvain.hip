@@ -532,6 +532,105 @@ __global__ __launch_bounds__(256) void k_decide_pipe(DevGraph<T> g, DevState<T> 
 }
 
 // ---------------------------------------------------------------------------
+// decide-and-move, one thread per vertex (degree <= DMAX, integer path)
+// Same decision as k_decide_ring.  Each lane loads its whole row, then the
+// community views of all DMAX entries, so a lane has up to DMAX independent
+// gathers in flight and a wave 64 * DMAX; the aggregation is a predicated
+// O(DMAX^2) pass in registers: an entry whose community occurs at no lower
+// index sums the weights of the equal communities after it and is evaluated.
+// The view's sign bit carries the singleton flag the guard needs.
+// ---------------------------------------------------------------------------
+template <int DMAX, bool FIRST, class T>
+__global__ __launch_bounds__(256) void k_decide_tpv(DevGraph<T> g, DevState<T> st,
+                                                     const int32_t* __restrict__ front,
+                                                     const int32_t* __restrict__ bstart, int bin,
+                                                     typename T::tot_t m2_all) {
+  static_assert(!T::FP, "integer path only");
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  using CV = typename T::cv_t;
+  __shared__ unsigned long long s_moved, s_swept, s_edges;
+  if (threadIdx.x == 0) { s_moved = 0; s_swept = 0; s_edges = 0; }
+  __syncthreads();
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  unsigned moved = 0, swept = 0, edges = 0;
+  for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
+    const int v = front[i];
+    const tt m2 = vertex_m2<T>(st, v, m2_all);
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    const CV ev = st.cv[v];
+    const int c = ev.c;
+    const tt K = st.K[v], S = st.Self[v], Tc = cv_tot(ev);
+    int cs[DMAX];
+    int32_t ts[DMAX];  // Tot with the singleton flag in the sign bit
+    acc ws[DMAX];
+#pragma unroll
+    for (int k = 0; k < DMAX; k++) {
+      cs[k] = k < d ? g.nbr[o + k] : 0;
+      ws[k] = k < d ? (acc)load_w<T>(g.w, o + k) : acc(0);
+    }
+#pragma unroll
+    for (int k = 0; k < DMAX; k++) {
+      CV e{};
+      if (k < d) e = st.cv[cs[k]];
+      cs[k] = k < d ? e.c : -1;
+      ts[k] = e.ts;
+    }
+    G h = 0;
+    int bd = INT_MAX, bsz = 0;
+    acc wc = 0;
+    if constexpr (FIRST) {
+      // singleton partition with duplicate-free rows: every entry is its own
+      // community, and the own community's weight is the self-loop
+      wc = (acc)S;
+#pragma unroll
+      for (int k = 0; k < DMAX; k++) {
+        if (cs[k] >= 0 && cs[k] != c) {
+          const G hh = gain_h<T>((G)ws[k], (G)m2, (G)K, (G)(ts[k] & 0x7fffffff), (G)Tc);
+          if (hh > 0 && better(hh, cs[k], h, bd)) { h = hh; bd = cs[k]; bsz = ts[k] < 0 ? 1 : 2; }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < DMAX; k++) {
+        bool dup = cs[k] < 0;
+#pragma unroll
+        for (int j = 0; j < k; j++) dup |= cs[j] == cs[k];
+        acc sum = ws[k];
+#pragma unroll
+        for (int j = k + 1; j < DMAX; j++) sum += cs[j] == cs[k] ? ws[j] : acc(0);
+        if (!dup) {
+          if (cs[k] == c) {
+            wc = sum;
+          } else {
+            const G hh = gain_h<T>((G)sum, (G)m2, (G)K, (G)(ts[k] & 0x7fffffff), (G)Tc);
+            if (hh > 0 && better(hh, cs[k], h, bd)) { h = hh; bd = cs[k]; bsz = ts[k] < 0 ? 1 : 2; }
+          }
+        }
+      }
+    }
+    swept++;
+    edges += d;
+    const int mv = commit_move_sz<T>(v, c, h, bd, bsz, cv_sz(ev), wc, K, S, m2, st);
+    moved += mv;
+    if (st.dirty && mv) mark_row(st.dirty, g.nbr, o, d, v, 0, 1);
+  }
+  if (moved) atomicAdd(&s_moved, (unsigned long long)moved);
+  if (swept) atomicAdd(&s_swept, (unsigned long long)swept);
+  if (edges) atomicAdd(&s_edges, (unsigned long long)edges);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, s_moved);
+    atomic_add_ull(&st.ctr->swept, s_swept);
+    atomic_add_ull(&st.ctr->swept_edges, s_edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // decide-and-move, pipelined per-group LDS hash (32..511)
 // W lanes per vertex, EPL row entries per lane (degree <= W*EPL), four
 // vertices in flight per group as in k_decide_pipe.  The table records each
@@ -1150,17 +1249,32 @@ static int grid_for(int cap, int per_block, int max_blocks) {
 // Decide kernel variants per bin (every variant takes the same exact decision;
 // they differ only in how lanes are mapped to vertices):
 //   bins 0-3: 0 = xor ring over W lanes, 1 = LDS hash over W lanes,
-//             2 = xor ring over W lanes, four vertices in flight per group
+//             2 = xor ring over W lanes, four vertices in flight per group,
+//             3 = one thread per vertex (integer path; FP falls back to 2)
 //   bin 4:    0 = one wave per vertex, 1 = 32 lanes, 2 = 16 lanes,
 //             3 = pipelined, 32 lanes x 4 entries, 4 = pipelined, 16 lanes x 8 entries
 //   bin 5:    0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 64 lanes x 8 entries
 // First sweeps: bins 0-3 have 0 = streaming argmax, 1 = pipelined (four
-// vertices in flight per group); bin 4 has 0 = one wave per vertex, 1 = 32 lanes.
+// vertices in flight per group), 2 = one thread per vertex (integer path; FP
+// falls back to 1); bin 4 has 0 = one wave per vertex, 1 = 32 lanes.
 int decide_variants(int bin) {
-  static const int nv[NBINS] = {3, 3, 3, 3, 5, 3, 1, 1, 1};
+  static const int nv[NBINS] = {4, 4, 4, 4, 5, 3, 1, 1, 1};
   return bin >= 0 && bin < NBINS ? nv[bin] : 1;
 }
-int decide_first_variants(int bin) { return bin <= 4 ? 2 : 1; }
+int decide_first_variants(int bin) {
+  static const int nv[NBINS] = {3, 3, 3, 3, 2, 1, 1, 1, 1};
+  return bin >= 0 && bin < NBINS ? nv[bin] : 1;
+}
+
+// one thread per vertex on the integer path, the pipelined W-lane kernel on
+// the FP path (its per-entry FP64 totals would not fit the registers)
+#define TPV_OR(DMAX, FIRST, W, PER)                                                                       \
+  do {                                                                                                     \
+    if constexpr (T::FP)                                                                                   \
+      k_decide_pipe<W, FIRST, T><<<grid_for(cap, PER, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2);   \
+    else                                                                                                   \
+      k_decide_tpv<DMAX, FIRST, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); \
+  } while (0)
 
 template <class T>
 void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
@@ -1174,18 +1288,22 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
     case 0: k_decide_ring<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 1: k_decide_wave<8, 4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 2: k_decide_pipe<4, false, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 3: TPV_OR(3, false, 4, 256); break;
     // bin 1: 4..7
     case 8: k_decide_ring<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 9: k_decide_wave<16, 8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 10: k_decide_pipe<8, false, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 11: TPV_OR(7, false, 8, 128); break;
     // bin 2: 8..15
     case 16: k_decide_ring<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 17: k_decide_wave<32, 16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 18: k_decide_pipe<16, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 19: TPV_OR(15, false, 16, 64); break;
     // bin 3: 16..31
     case 24: k_decide_ring<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 25: k_decide_wave<64, 32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 26: k_decide_pipe<32, false, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 27: TPV_OR(31, false, 32, 32); break;
     // bin 4: 32..127
     case 32: k_decide_wave<256, 64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 33: k_decide_wave<256, 32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
@@ -1229,12 +1347,16 @@ void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const i
   switch (bin * 8 + variant) {
     case 0: k_decide_first<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 1: k_decide_pipe<4, true, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 2: TPV_OR(3, true, 4, 256); break;
     case 8: k_decide_first<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 9: k_decide_pipe<8, true, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 10: TPV_OR(7, true, 8, 128); break;
     case 16: k_decide_first<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 17: k_decide_pipe<16, true, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 18: TPV_OR(15, true, 16, 64); break;
     case 24: k_decide_first<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 25: k_decide_pipe<32, true, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 26: TPV_OR(31, true, 32, 32); break;
     case 32: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 33: k_decide_first<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 40: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
@@ -1413,6 +1535,41 @@ __global__ __launch_bounds__(256) void k_weight_group(DevGraph<T> g, DevState<T>
   }
 }
 
+// Weight update, one thread per vertex (degree <= DMAX): the lane loads its
+// row, then the communities of all entries, DMAX independent gathers in flight.
+template <int DMAX, class T>
+__global__ __launch_bounds__(256) void k_weight_tpv(DevGraph<T> g, DevState<T> st,
+                                                     const int32_t* __restrict__ order,
+                                                     const int32_t* __restrict__ bstart, int bin, int only_dirty) {
+  using tt = typename T::tot_t;
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
+    const int v = order[i];
+    if (only_dirty && !st.dirty[v]) continue;
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    const int c = st.cur[v];
+    int us[DMAX];
+    tt ws[DMAX];
+#pragma unroll
+    for (int k = 0; k < DMAX; k++) {
+      us[k] = k < d ? g.nbr[o + k] : v;
+      ws[k] = k < d ? (tt)load_w<T>(g.w, o + k) : tt(0);
+    }
+    tt x = 0;
+    int mvn = 0;
+#pragma unroll
+    for (int k = 0; k < DMAX; k++) {
+      if (k < d) {
+        if (st.cur[us[k]] == c) x += ws[k];
+        if (st.track) mvn += st.moved_it[us[k]] == st.iter;
+      }
+    }
+    st.In[v] = x;
+    if (st.track) st.nbr_moved[v] = d ? (float)mvn / (float)d : 0.f;
+  }
+}
+
 // Pipelined weight update: W lanes x EPL entries per vertex (degree <= W*EPL),
 // five items in flight per group, one link of the load chain each
 // (order -> dirty flag -> offsets and own community -> row -> neighbour
@@ -1553,6 +1710,15 @@ void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
                    const int32_t* bstart, int cap, bool only_dirty, hipStream_t s, int variant) {
   if (cap <= 0) return;
   const int od = only_dirty ? 1 : 0;
+  if (variant == 2 && bin <= 3) {  // one thread per vertex
+    switch (bin) {
+      case 0: k_weight_tpv<3, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, order, bstart, 0, od); break;
+      case 1: k_weight_tpv<7, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, order, bstart, 1, od); break;
+      case 2: k_weight_tpv<15, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, order, bstart, 2, od); break;
+      case 3: k_weight_tpv<31, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, order, bstart, 3, od); break;
+    }
+    return;
+  }
   if (variant == 1 && bin <= 4) {  // pipelined
     switch (bin) {
       case 0: k_weight_pipe<4, 1, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, order, bstart, 0, od); break;

@@ -28,7 +28,7 @@ EXPORTS = [
     "gala_batch_communities", "gala_gen_sbm_batch", "gala_set_variants", "gala_get_variants", "gala_autotune",
 ]
 N_VARIANT_SLOTS = 27  # kernel variant per bin: 9 general sweeps, 9 first sweeps, 9 weight updates
-VARIANT_COUNTS = [3, 3, 3, 3, 5, 3, 1, 1, 1] + [2, 2, 2, 2, 2, 1, 1, 1, 1] + [2, 2, 2, 2, 2, 1, 1, 1, 1]
+VARIANT_COUNTS = [4, 4, 4, 4, 5, 3, 1, 1, 1] + [3, 3, 3, 3, 2, 1, 1, 1, 1] + [3, 3, 3, 3, 2, 1, 1, 1, 1]
 
 MLP_NFEAT = 6
 MLP_NPARAM = 1153  # W1[16][64] (rows 6..15 zero), b1[64], w2[64], b2
