@@ -839,16 +839,33 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
     const int tsz = 1 << lg;  // <= TMAX by bin construction
     for (int t = lane; t < tsz; t += W) { keys[t] = 0; vals[t] = 0; }
     lds_fence_wave();
-    for (int j = lane; j < d; j += W) {
-      const typename T::cv_t eu = st.cv[g.nbr[o + j]];
-      const int cl = eu.c;
-      const acc w = (acc)load_w<T>(g.w, o + j);
-      unsigned p = hash_slot(cl, lg);
-      for (int probe = 0; probe < tsz; probe++) {
-        int k = atomicCAS(&keys[p], 0, cl + 1);
-        if (k == 0) tots[p] = cv_tot(eu);  // the claiming lane records the community's Tot
-        if (k == 0 || k == cl + 1) { atomicAdd(&vals[p], w); break; }
-        p = (p + 1) & (tsz - 1);
+    // U entries per lane are loaded before any is inserted (U independent
+    // row loads, then U independent community-view gathers in flight)
+    constexpr int U = 4;
+    for (int j0 = lane; j0 < d; j0 += W * U) {
+      int un[U];
+      acc wv[U];
+      typename T::cv_t ev4[U];
+#pragma unroll
+      for (int r = 0; r < U; r++) {
+        const int j = j0 + r * W;
+        un[r] = j < d ? g.nbr[o + j] : -1;
+        wv[r] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+      }
+#pragma unroll
+      for (int r = 0; r < U; r++)
+        if (un[r] >= 0) ev4[r] = st.cv[un[r]];
+#pragma unroll
+      for (int r = 0; r < U; r++) {
+        if (un[r] < 0) continue;
+        const int cl = ev4[r].c;
+        unsigned p = hash_slot(cl, lg);
+        for (int probe = 0; probe < tsz; probe++) {
+          int k = atomicCAS(&keys[p], 0, cl + 1);
+          if (k == 0) tots[p] = cv_tot(ev4[r]);  // the claiming lane records the community's Tot
+          if (k == 0 || k == cl + 1) { atomicAdd(&vals[p], wv[r]); break; }
+          p = (p + 1) & (tsz - 1);
+        }
       }
     }
     lds_fence_wave();
@@ -941,17 +958,32 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
     }
     for (int t = threadIdx.x; t < tsz; t += NT) { keys[t] = 0; vals[t] = 0; }
     __syncthreads();
-    for (int j = threadIdx.x; j < d; j += NT) {
-      const int cl = st.prev[g.nbr[o + j]];
-      const acc w = (acc)load_w<T>(g.w, o + j);
-      unsigned p = hash_slot(cl, lg);
-      int probe = 0;
-      for (; probe < tsz; probe++) {
-        int k = atomicCAS(&keys[p], 0, cl + 1);
-        if (k == 0 || k == cl + 1) { atomicAdd(&vals[p], w); break; }
-        p = (p + 1) & (tsz - 1);
+    // U entries per thread are loaded before any is inserted: U independent
+    // row loads, then U independent community gathers in flight
+    constexpr int U = 4;
+    for (int j0 = threadIdx.x; j0 < d; j0 += NT * U) {
+      int cl[U];
+      acc wv[U];
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        const int j = j0 + q * NT;
+        cl[q] = j < d ? g.nbr[o + j] : -1;
+        wv[q] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
       }
-      if (probe == tsz) st.ctr->overflow = 1;
+#pragma unroll
+      for (int q = 0; q < U; q++) cl[q] = cl[q] >= 0 ? st.prev[cl[q]] : -1;
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        if (cl[q] < 0) continue;
+        unsigned p = hash_slot(cl[q], lg);
+        int probe = 0;
+        for (; probe < tsz; probe++) {
+          int k = atomicCAS(&keys[p], 0, cl[q] + 1);
+          if (k == 0 || k == cl[q] + 1) { atomicAdd(&vals[p], wv[q]); break; }
+          p = (p + 1) & (tsz - 1);
+        }
+        if (probe == tsz) st.ctr->overflow = 1;
+      }
     }
     __syncthreads();
     const tt K = st.K[v];
@@ -1039,26 +1071,43 @@ __global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> 
     for (int t = threadIdx.x; t < lsz; t += NT) { s_keys[t] = 0; s_vals[t] = 0; }
     if (threadIdx.x == 0) s_ovf = 0;
     __syncthreads();
-    for (int j = threadIdx.x; j < d; j += NT) {
-      const int cl = st.prev[g.nbr[o + j]];
-      const acc w = (acc)load_w<T>(g.w, o + j);
-      unsigned p = hash_slot(cl, llg);
-      bool placed = false;
-      for (int probe = 0; probe < PMAX; probe++) {
-        const int k = atomicCAS(&s_keys[p], 0, cl + 1);
-        if (k == 0 || k == cl + 1) { atomicAdd(&s_vals[p], w); placed = true; break; }
-        p = (p + 1) & (lsz - 1);
+    // U entries per thread are loaded before any is inserted: U independent
+    // row loads, then U independent community gathers in flight
+    constexpr int U = 8;
+    for (int j0 = threadIdx.x; j0 < d; j0 += NT * U) {
+      int cu[U];
+      acc wv[U];
+#pragma unroll
+      for (int r = 0; r < U; r++) {
+        const int j = j0 + r * NT;
+        cu[r] = j < d ? g.nbr[o + j] : -1;
+        wv[r] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
       }
-      if (!placed) {
-        s_ovf = 1;
-        unsigned q = hash_slot(cl, glg);
-        int probe = 0;
-        for (; probe < gsz; probe++) {
-          const int k = atomicCAS(&gkeys[q], 0, cl + 1);
-          if (k == 0 || k == cl + 1) { atomicAdd(&gvals[q], w); break; }
-          q = (q + 1) & (gsz - 1);
+#pragma unroll
+      for (int r = 0; r < U; r++) cu[r] = cu[r] >= 0 ? st.prev[cu[r]] : -1;
+#pragma unroll
+      for (int r = 0; r < U; r++) {
+        const int cl = cu[r];
+        if (cl < 0) continue;
+        const acc w = wv[r];
+        unsigned p = hash_slot(cl, llg);
+        bool placed = false;
+        for (int probe = 0; probe < PMAX; probe++) {
+          const int k = atomicCAS(&s_keys[p], 0, cl + 1);
+          if (k == 0 || k == cl + 1) { atomicAdd(&s_vals[p], w); placed = true; break; }
+          p = (p + 1) & (lsz - 1);
         }
-        if (probe == gsz) st.ctr->overflow = 1;
+        if (!placed) {
+          s_ovf = 1;
+          unsigned q = hash_slot(cl, glg);
+          int probe = 0;
+          for (; probe < gsz; probe++) {
+            const int k = atomicCAS(&gkeys[q], 0, cl + 1);
+            if (k == 0 || k == cl + 1) { atomicAdd(&gvals[q], w); break; }
+            q = (q + 1) & (gsz - 1);
+          }
+          if (probe == gsz) st.ctr->overflow = 1;
+        }
       }
     }
     __syncthreads();
