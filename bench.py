@@ -42,22 +42,25 @@ KERNEL_GENERAL = [
     ["k_decide_ring<8, IntW>", "k_decide_wave<16, 8, IntW>", "k_decide_pipe<8, false, IntW>",
      "k_decide_tpv<7, false, IntW>"],
     ["k_decide_ring<16, IntW>", "k_decide_wave<32, 16, IntW>", "k_decide_pipe<16, false, IntW>",
-     "k_decide_tpv<15, false, IntW>"],
+     "k_decide_tpv<15, false, IntW>", "k_decide_hlead<32, 8, 2, false, IntW>"],
     ["k_decide_ring<32, IntW>", "k_decide_wave<64, 32, IntW>", "k_decide_pipe<32, false, IntW>",
-     "k_decide_tpv<31, false, IntW>"],
+     "k_decide_tpv<31, false, IntW>", "k_decide_hlead<64, 16, 2, false, IntW>"],
     ["k_decide_wave<256, 64, IntW>", "k_decide_wave<256, 32, IntW>", "k_decide_wave<256, 16, IntW>",
-     "k_decide_hpipe<256, 32, 4, IntW>", "k_decide_hpipe<256, 16, 8, IntW>"],
-    ["k_decide_wave<1024, 64, IntW>", "k_decide_wave<1024, 32, IntW>", "k_decide_hpipe<1024, 64, 8, IntW>"],
-    ["k_decide_block<256, 8192, IntW>"],
-    ["k_decide_block<1024, 16384, IntW>"],
-    ["k_decide_hub<16384, IntW>"]]
+     "k_decide_hpipe<256, 32, 4, IntW>", "k_decide_hpipe<256, 16, 8, IntW>",
+     "k_decide_hlead<256, 32, 4, false, IntW>", "k_decide_hlead<256, 16, 8, false, IntW>"],
+    ["k_decide_wave<1024, 64, IntW>", "k_decide_wave<1024, 32, IntW>", "k_decide_hpipe<1024, 64, 8, IntW>",
+     "k_decide_hlead<1024, 64, 8, false, IntW>"],
+    ["k_decide_block<256, 8192, IntW>", "k_decide_blead<256, 8192, 16, IntW>"],
+    ["k_decide_block<1024, 16384, IntW>", "k_decide_blead<1024, 16384, 8, IntW>"],
+    ["k_decide_hub<16384, IntW>", "k_hub_plan/agg/scan/commit<IntW>"]]
 KERNEL_FIRST = [
     ["k_decide_first<4, IntW>", "k_decide_pipe<4, true, IntW>", "k_decide_tpv<3, true, IntW>"],
     ["k_decide_first<8, IntW>", "k_decide_pipe<8, true, IntW>", "k_decide_tpv<7, true, IntW>"],
     ["k_decide_first<16, IntW>", "k_decide_pipe<16, true, IntW>", "k_decide_tpv<15, true, IntW>"],
-    ["k_decide_first<32, IntW>", "k_decide_pipe<32, true, IntW>", "k_decide_tpv<31, true, IntW>"],
-    ["k_decide_first<64, IntW>", "k_decide_first<32, IntW>"],
-    ["k_decide_first<64, IntW>"],
+    ["k_decide_first<32, IntW>", "k_decide_pipe<32, true, IntW>", "k_decide_tpv<31, true, IntW>",
+     "k_decide_hlead<1, 16, 2, true, IntW>"],
+    ["k_decide_first<64, IntW>", "k_decide_first<32, IntW>", "k_decide_hlead<1, 16, 8, true, IntW>"],
+    ["k_decide_first<64, IntW>", "k_decide_hlead<1, 64, 8, true, IntW>"],
     ["k_decide_first_block<256, IntW>"],
     ["k_decide_first_block<1024, IntW>"],
     ["k_decide_first_block<1024, IntW>"]]

@@ -34,6 +34,20 @@ struct DevGraph {
   const typename T::w_t* w;  // nullptr => every weight is 1
 };
 
+// split hub decide (bin 8): row chunk of one aggregation workgroup, table
+// chunk of one scan workgroup, and the scan's per-chunk partial result
+constexpr int HUB_CH = 4096, HUB_TCH = 4096;
+template <class T>
+struct HubPart {
+  typename T::gain_t h;
+  typename T::acc_t wc;
+  int32_t d;
+};
+// capacities for a graph of m directed entries: bin-8 hubs have >= 8192
+// entries and a next_pow2(2d) < 4d slot table
+inline int64_t hub_plan_elems(int64_t m) { return 2 * (m / 8192 + 2); }
+inline int64_t hub_part_elems(int64_t m) { return 4 * m / HUB_TCH + m / 8192 + 2; }
+
 template <class T>
 struct DevState {
   typename T::tot_t *K, *Self, *Tot, *In, *dTot;
@@ -44,6 +58,8 @@ struct DevState {
   int64_t* hub_off;  // per-vertex offset into the global hash arena (bin 8), in slots
   int32_t* hub_keys;
   typename T::acc_t* hub_vals;
+  int32_t* hub_plan = nullptr;  // split hub decide: chunk starts (hub_plan_elems)
+  void* hub_part = nullptr;     // split hub decide: HubPart<T>[hub_part_elems]
   Counters* ctr;
   // learned-frontier features (tracked only when track != 0)
   int track;

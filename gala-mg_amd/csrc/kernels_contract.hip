@@ -42,10 +42,57 @@ __host__ __device__ __forceinline__ int clog2(long long x) {
   return l;
 }
 
-// community sizes
-__global__ void k_hist(const int32_t* __restrict__ comm, int32_t n, int32_t* __restrict__ cnt) {
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-    atomicAdd(&cnt[comm[v]], 1);
+// member count and row-length sum of every community.  A workgroup takes a
+// tile of CT vertices at a time and pre-aggregates them in an LDS hash keyed by
+// community, so a large community costs one global atomic per tile instead of
+// one per member (per-member atomics on a few hot addresses serialise).
+constexpr int CT = 1024, CTS = 2 * CT;
+__global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ comm, const int64_t* __restrict__ off,
+                                                   int32_t n, int32_t* __restrict__ cnt,
+                                                   unsigned long long* __restrict__ dsum) {
+  __shared__ int s_key[CTS];
+  __shared__ int s_cnt[CTS];
+  __shared__ unsigned long long s_sum[CTS];
+  for (int t = threadIdx.x; t < CTS; t += blockDim.x) { s_key[t] = 0; s_cnt[t] = 0; s_sum[t] = 0; }
+  __syncthreads();
+  constexpr int LG = 11;  // CTS = 2^LG
+  for (int64_t base = (int64_t)blockIdx.x * CT; base < n; base += (int64_t)gridDim.x * CT) {
+    for (int j = threadIdx.x; j < CT; j += blockDim.x) {
+      const int64_t v = base + j;
+      if (v >= n) break;
+      const int c = comm[v];
+      const unsigned long long d = (unsigned long long)(off[v + 1] - off[v]);
+      unsigned p = ((unsigned)c * 2654435769u) >> (32 - LG);
+      for (int probe = 0; probe < CTS; probe++) {
+        const int k = atomicCAS(&s_key[p], 0, c + 1);
+        if (k == 0 || k == c + 1) {
+          atomicAdd(&s_cnt[p], 1);
+          if (d) atomicAdd(&s_sum[p], d);
+          break;
+        }
+        p = (p + 1) & (CTS - 1);
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < CTS; t += blockDim.x) {
+      const int k = s_key[t];
+      if (k) {
+        atomicAdd(&cnt[k - 1], s_cnt[t]);
+        if (s_sum[t]) atomicAdd(&dsum[k - 1], s_sum[t]);
+        s_key[t] = 0;
+        s_cnt[t] = 0;
+        s_sum[t] = 0;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// entries of every new community: ecnt[nid[c]] = row-length sum of c
+__global__ void k_ecount_comm(const int32_t* __restrict__ cnt, const int32_t* __restrict__ nid,
+                              const unsigned long long* __restrict__ dsum, int32_t n, int64_t* __restrict__ ecnt) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+    if (cnt[c] > 0) ecnt[nid[c]] = (int64_t)dsum[c];
 }
 
 __global__ void k_nonempty(const int32_t* __restrict__ cnt, int32_t n, uint8_t* __restrict__ flag) {
@@ -57,14 +104,6 @@ __global__ void k_label(const int32_t* __restrict__ comm, const int32_t* __restr
                         int32_t* __restrict__ label) {
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
     label[v] = nid[comm[v]];
-}
-
-__global__ void k_ecount(const int64_t* __restrict__ off, int32_t n, const int32_t* __restrict__ label,
-                         int64_t* __restrict__ ecnt) {
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    int64_t d = off[v + 1] - off[v];
-    if (d) atomicAdd((unsigned long long*)&ecnt[label[v]], (unsigned long long)d);
-  }
 }
 
 // staged length of each listed dense row
@@ -737,7 +776,10 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   constexpr long long LT = T::FP ? 8192 : 16384;
   const int32_t n = g.n;
   fill<int32_t>(cb.cnt, n, 0, s);
-  k_hist<<<nblk(n), 256, 0, s>>>(comm, n, cb.cnt);
+  // the staging cursors are free until contract_build: row-length sums by community
+  unsigned long long* dsum = (unsigned long long*)cb.ecur;
+  fill<int64_t>(cb.ecur, n, 0, s);
+  k_comm_sums<<<std::max(1, std::min((n + CT - 1) / CT, 2048)), 256, 0, s>>>(comm, g.off, n, cb.cnt, dsum);
   uint8_t* flag = (uint8_t*)cb.i32_tmp;  // n bytes fit in the n-int scratch
   k_nonempty<<<nblk(n), 256, 0, s>>>(cb.cnt, n, flag);
   scan_exclusive<uint8_t, int32_t>(flag, cb.nid, n, cb.i32_tmp + (n + 3) / 4 + 1, cb.i32_scalar, s);
@@ -748,7 +790,7 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
   const int32_t C = hp32[0];
   fill<int64_t>(cb.ecnt, C + 1, 0, s);
-  k_ecount<<<nblk(n), 256, 0, s>>>(g.off, n, cb.label, cb.ecnt);
+  k_ecount_comm<<<nblk(n), 256, 0, s>>>(cb.cnt, cb.nid, dsum, n, cb.ecnt);
   scan_exclusive<int64_t, int64_t>(cb.ecnt, cb.eoff, C, cb.i64_tmp, cb.eoff + C, s);
   fill<int32_t>(cb.nbig, 2, 0, s);  // big rows, dense rows
   int32_t* dense_list = cb.big_list + n;

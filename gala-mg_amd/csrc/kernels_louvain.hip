@@ -10,6 +10,7 @@
 // best = argmax H over d != c with H > 0, ties to the lowest id, move iff
 // H(best) - m2*(w_c - Self) > 0 and not (size[best]==size[c]==1 && best > c).
 #include "gala_kernels.h"
+#include "gala_device.h"
 #include <climits>
 
 namespace gala {
@@ -797,6 +798,199 @@ __global__ __launch_bounds__(256) void k_decide_hpipe(DevGraph<T> g, DevState<T>
 }
 
 // ---------------------------------------------------------------------------
+// decide-and-move, pipelined per-group LDS hash with leader evaluation
+// (8..511).  Same pipeline as k_decide_hpipe (W lanes per vertex, EPL row
+// entries per lane, four vertices in flight per group), but no table scan:
+// the lane whose insert claimed a slot ("leader") keeps the slot index, reads
+// the completed sum after the inserts, evaluates that community's gain with
+// the Tot and size of its own community-view entry, and clears the slot.  The
+// tables are zeroed once at kernel start and stay zero between vertices, so
+// the per-vertex cost is O(d / W) per lane instead of O(next_pow2(2d) / W),
+// and a slot is 8 bytes of LDS (12 on the FP path).
+// ---------------------------------------------------------------------------
+// FIRST: the first sweep of a round (singleton partition, duplicate-free
+// rows): every entry other than the self-loop is its own candidate with the
+// entry's weight, the own-community weight is Self, and no table is used.
+template <int TMAX, int W, int EPL, bool FIRST, class T>
+__global__ __launch_bounds__(256) void k_decide_hlead(DevGraph<T> g, DevState<T> st,
+                                                       const int32_t* __restrict__ front,
+                                                       const int32_t* __restrict__ bstart, int bin,
+                                                       typename T::tot_t m2_all) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  using CV = typename T::cv_t;
+  constexpr int GPB = 256 / W;
+  constexpr int LEAD = 1 << 30;
+  __shared__ int s_keys[GPB][TMAX];
+  __shared__ acc s_vals[GPB][TMAX];
+  __shared__ unsigned long long s_moved, s_swept, s_edges;
+  for (int t = threadIdx.x; t < GPB * TMAX; t += blockDim.x) {
+    (&s_keys[0][0])[t] = 0;
+    (&s_vals[0][0])[t] = acc(0);
+  }
+  if (threadIdx.x == 0) { s_moved = 0; s_swept = 0; s_edges = 0; }
+  __syncthreads();
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & (W - 1), grp = threadIdx.x / W;
+  int* keys = s_keys[grp];
+  acc* vals = s_vals[grp];
+  const int stride = gridDim.x * GPB;
+  unsigned moved = 0, swept = 0, edges = 0;
+  int iA = lo + blockIdx.x * GPB + grp;
+  bool okA = iA < hi;
+  int vA = okA ? front[iA] : 0;
+  bool okB = false;
+  int vB = 0, dB = 0;
+  int64_t oB = 0;
+  CV evB{};
+  tt KB = 0, SB = 0;
+  bool okC = false;
+  int vC = 0, dC = 0;
+  CV evC{};
+  tt KC = 0, SC = 0;
+  int uC[EPL];
+  acc wC[EPL];
+  bool okD = false;
+  int vD = 0, dD = 0;
+  CV evD{};
+  tt KD = 0, SD = 0;
+  int uD[EPL];
+  acc wD[EPL];
+  CV euD[EPL];
+#pragma unroll
+  for (int k = 0; k < EPL; k++) { uC[k] = 0; wC[k] = 0; uD[k] = 0; wD[k] = 0; euD[k] = CV{}; }
+  while (okA || okB || okC || okD) {  // uniform per group
+    if (okD) {
+      const typename T::tot_t m2 = vertex_m2<T>(st, vD, m2_all);
+      const int c = evD.c;
+      int lg = ilog2_ceil(2 * dD);
+      if (lg < 1) lg = 1;
+      const int tmask = (1 << lg) - 1;  // next_pow2(2d) <= TMAX by bin construction
+      G h = 0;
+      int bd = INT_MAX, bsz = 0;
+      acc wc = 0;
+      if constexpr (FIRST) {
+        (void)tmask;
+#pragma unroll
+        for (int k = 0; k < EPL; k++) {
+          const int cl = euD[k].c;
+          if (lane + k * W < dD && cl != c) {
+            const G hh = gain_h<T>((G)wD[k], (G)m2, (G)KD, (G)cv_tot(euD[k]), (G)cv_tot(evD));
+            if (hh > 0 && better(hh, cl, h, bd)) { h = hh; bd = cl; bsz = cv_sz(euD[k]); }
+          }
+        }
+      } else {
+        int sl[EPL];
+#pragma unroll
+        for (int k = 0; k < EPL; k++) {
+          sl[k] = -1;
+          if (lane + k * W < dD) {
+            const int cl = euD[k].c;
+            unsigned p = hash_slot(cl, lg);
+            for (int probe = 0; probe <= tmask; probe++) {
+              const int kk = atomicCAS(&keys[p], 0, cl + 1);
+              if (kk == 0 || kk == cl + 1) {
+                atomicAdd(&vals[p], wD[k]);
+                sl[k] = (int)p | (kk == 0 ? LEAD : 0);
+                break;
+              }
+              p = (p + 1) & tmask;
+            }
+          }
+        }
+        lds_fence_wave();
+#pragma unroll
+        for (int k = 0; k < EPL; k++) {
+          if (sl[k] >= LEAD) {
+            const int p = sl[k] & (LEAD - 1);
+            const acc a = vals[p];
+            keys[p] = 0;
+            vals[p] = acc(0);
+            const int cl = euD[k].c;
+            if (cl == c) {
+              wc = a;
+            } else {
+              const G hh = gain_h<T>((G)a, (G)m2, (G)KD, (G)cv_tot(euD[k]), (G)cv_tot(evD));
+              if (hh > 0 && better(hh, cl, h, bd)) { h = hh; bd = cl; bsz = cv_sz(euD[k]); }
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int k = W / 2; k >= 1; k >>= 1) {
+        const G ho = __shfl_xor(h, k, W);
+        const int dd = __shfl_xor(bd, k, W);
+        const int so = __shfl_xor(bsz, k, W);
+        if (better(ho, dd, h, bd)) { h = ho; bd = dd; bsz = so; }
+      }
+      if constexpr (FIRST) wc = (acc)SD;
+      else wc = sum_xor<acc, W>(wc);
+      int mv = 0;
+      if (lane == 0) {
+        swept++;
+        edges += dD;
+        mv = commit_move_sz<T>(vD, c, h, bd, bsz, cv_sz(evD), wc, KD, SD, m2, st);
+        moved += mv;
+      }
+      if (st.dirty && __shfl(mv, 0, W)) {
+#pragma unroll
+        for (int k = 0; k < EPL; k++)
+          if (lane + k * W < dD) st.dirty[uD[k]] = 1;
+        if (lane == 0) st.dirty[vD] = 1;
+      }
+      lds_fence_wave();  // every slot cleared before the next vertex inserts
+    }
+    okD = okC;
+    if (okC) {
+      vD = vC; dD = dC; evD = evC; KD = KC; SD = SC;
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        uD[k] = uC[k];
+        wD[k] = wC[k];
+        if (lane + k * W < dC) euD[k] = st.cv[uC[k]];
+      }
+    }
+    okC = okB;
+    if (okB) {
+      vC = vB; dC = dB; evC = evB; KC = KB; SC = SB;
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        if (lane + k * W < dB) {
+          uC[k] = g.nbr[oB + lane + k * W];
+          wC[k] = (acc)load_w<T>(g.w, oB + lane + k * W);
+        }
+      }
+    }
+    okB = okA;
+    if (okA) {
+      vB = vA;
+      oB = g.off[vA];
+      dB = (int)(g.off[vA + 1] - oB);
+      evB = st.cv[vA];
+      KB = st.K[vA];
+      SB = st.Self[vA];
+      iA += stride;
+      okA = iA < hi;
+      if (okA) vA = front[iA];
+    }
+  }
+  if (lane == 0) {
+    if (moved) atomicAdd(&s_moved, (unsigned long long)moved);
+    if (swept) atomicAdd(&s_swept, (unsigned long long)swept);
+    if (edges) atomicAdd(&s_edges, (unsigned long long)edges);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, s_moved);
+    atomic_add_ull(&st.ctr->swept, s_swept);
+    atomic_add_ull(&st.ctr->swept_edges, s_edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // decide-and-move, W lanes per vertex with a per-group LDS hash (8..511)
 // Phase 1 inserts (community -> summed weight) with LDS CAS/add; phase 2 scans
 // the table so each distinct community is evaluated once (one Tot gather per
@@ -1032,6 +1226,136 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
 }
 
 // ---------------------------------------------------------------------------
+// decide-and-move, one workgroup per vertex with leader evaluation (512 and
+// up, d <= NT * EPL).  Every thread loads its EPL row entries and their
+// community views up front, inserts them into the LDS table, and after a
+// barrier each slot's claiming thread evaluates that community once (Tot and
+// size from its own view entry) and clears the slot.  The table is zeroed
+// once per kernel, so a vertex costs O(d / NT) per thread, not
+// O(next_pow2(2d) / NT), and needs no Tot gathers.
+// ---------------------------------------------------------------------------
+template <int NT, int TMAX, int EPL, class T>
+__global__ __launch_bounds__(NT) void k_decide_blead(DevGraph<T> g, DevState<T> st,
+                                                      const int32_t* __restrict__ front,
+                                                      const int32_t* __restrict__ bstart, int bin,
+                                                      typename T::tot_t m2_all) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  using CV = typename T::cv_t;
+  constexpr int LEAD = 1 << 30;
+  __shared__ int s_keys[TMAX];
+  __shared__ acc s_vals[TMAX];
+  __shared__ G r_h[NT / 64];
+  __shared__ int r_d[NT / 64];
+  __shared__ int r_sz[NT / 64];
+  __shared__ acc r_wc[NT / 64];
+  __shared__ int s_mv;
+  for (int t = threadIdx.x; t < TMAX; t += NT) { s_keys[t] = 0; s_vals[t] = acc(0); }
+  __syncthreads();
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned moved = 0, swept = 0;
+  unsigned long long edges = 0;
+  for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
+    const int v = front[i];
+    const tt m2 = vertex_m2<T>(st, v, m2_all);
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);  // <= NT * EPL by bin construction
+    const CV ev = st.cv[v];
+    const int c = ev.c;
+    int lg = ilog2_ceil(2 * d);
+    const int tmask = (1 << lg) - 1;  // <= TMAX slots
+    int u[EPL];
+    acc wv[EPL];
+    CV eu[EPL];
+#pragma unroll
+    for (int k = 0; k < EPL; k++) {
+      const int j = threadIdx.x + k * NT;
+      u[k] = j < d ? g.nbr[o + j] : -1;
+      wv[k] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+    }
+#pragma unroll
+    for (int k = 0; k < EPL; k++)
+      if (u[k] >= 0) eu[k] = st.cv[u[k]];
+    int sl[EPL];
+#pragma unroll
+    for (int k = 0; k < EPL; k++) {
+      sl[k] = -1;
+      if (u[k] >= 0) {
+        const int cl = eu[k].c;
+        unsigned p = hash_slot(cl, lg);
+        for (int probe = 0; probe <= tmask; probe++) {
+          const int kk = atomicCAS(&s_keys[p], 0, cl + 1);
+          if (kk == 0 || kk == cl + 1) {
+            atomicAdd(&s_vals[p], wv[k]);
+            sl[k] = (int)p | (kk == 0 ? LEAD : 0);
+            break;
+          }
+          p = (p + 1) & tmask;
+        }
+      }
+    }
+    __syncthreads();
+    G h = 0;
+    int bd = INT_MAX, bsz = 0;
+    acc wc = 0;
+    const tt K = st.K[v];
+#pragma unroll
+    for (int k = 0; k < EPL; k++) {
+      if (sl[k] >= LEAD) {
+        const int p = sl[k] & (LEAD - 1);
+        const acc a = s_vals[p];
+        s_keys[p] = 0;
+        s_vals[p] = acc(0);
+        const int cl = eu[k].c;
+        if (cl == c) {
+          wc = a;
+        } else {
+          const G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)cv_tot(eu[k]), (G)cv_tot(ev));
+          if (hh > 0 && better(hh, cl, h, bd)) { h = hh; bd = cl; bsz = cv_sz(eu[k]); }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) {
+      const G ho = __shfl_xor(h, k, 64);
+      const int dd = __shfl_xor(bd, k, 64);
+      const int so = __shfl_xor(bsz, k, 64);
+      if (better(ho, dd, h, bd)) { h = ho; bd = dd; bsz = so; }
+    }
+    wc = sum_xor<acc, 64>(wc);
+    if (lane == 0) { r_h[wid] = h; r_d[wid] = bd; r_sz[wid] = bsz; r_wc[wid] = wc; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 1; q < NT / 64; q++) {
+        if (better(r_h[q], r_d[q], h, bd)) { h = r_h[q]; bd = r_d[q]; bsz = r_sz[q]; }
+        wc += r_wc[q];
+      }
+      swept++;
+      edges += d;
+      const int mv = commit_move_sz<T>(v, c, h, bd, bsz, cv_sz(ev), wc, K, st.Self[v], m2, st);
+      moved += mv;
+      s_mv = mv;
+    }
+    __syncthreads();
+    if (st.dirty && s_mv) {
+#pragma unroll
+      for (int k = 0; k < EPL; k++)
+        if (u[k] >= 0) st.dirty[u[k]] = 1;
+      if (threadIdx.x == 0) st.dirty[v] = 1;
+    }
+  }
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, moved);
+    atomic_add_ull(&st.ctr->swept, swept);
+    atomic_add_ull(&st.ctr->swept_edges, edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], edges);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // decide-and-move for hubs: one 1024-thread workgroup per vertex, a LT-slot
 // LDS table tried first (bounded probing) and the vertex's slice of the global
 // arena for the overflow.  Global slices are kept all-zero between uses: the
@@ -1174,6 +1498,243 @@ __global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> 
 }
 
 // ---------------------------------------------------------------------------
+// decide-and-move for hubs, split over many workgroups.  One workgroup per
+// hub leaves most of the chip idle when a few contracted communities hold
+// rows of 10^5 entries or more.  Four launches instead:
+//   plan:   prefix sums over the frontier's hubs of their row chunks (HUB_CH
+//           entries) and of their table chunks (HUB_TCH slots of the hub's
+//           next_pow2(2d)-slot slice of the global arena)
+//   agg:    one workgroup per row chunk: its entries' communities are summed
+//           in an LDS hash (<= HUB_CH distinct keys in 2*HUB_CH slots, never
+//           full), then each distinct (community, sum) is added into the
+//           hub's global slice with one CAS/add
+//   scan:   one workgroup per table chunk: gains of the chunk's communities
+//           (clearing every slot it reads, so the slices are all-zero again),
+//           best (H, id) and the own-community weight -> one partial
+//   commit: one workgroup per hub: combines its partials in chunk order,
+//           commits the move and marks the row dirty.
+// The decision is the same as k_decide_hub's: the global table holds the
+// exact per-community sums and the argmax order is total (H, then lowest id).
+// ---------------------------------------------------------------------------
+// locate the hub whose chunk range [start[i], start[i+1]) holds q
+__device__ __forceinline__ int hub_find(const int32_t* __restrict__ start, int nh, int q) {
+  int a = 0, b = nh;  // start[a] <= q < start[b]
+  while (b - a > 1) {
+    const int mid = (a + b) >> 1;
+    if (start[mid] <= q) a = mid; else b = mid;
+  }
+  return a;
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_hub_plan(DevGraph<T> g, const int32_t* __restrict__ front,
+                                                   const int32_t* __restrict__ bstart, int bin,
+                                                   int32_t* __restrict__ plan) {
+  __shared__ int sh[17];
+  const int lo = bstart[bin], hi = bstart[bin + 1], nh = hi - lo;
+  int32_t* cstart = plan;           // nh + 1 row-chunk starts
+  int32_t* tstart = plan + nh + 1;  // nh + 1 table-chunk starts
+  int carry_c = 0, carry_t = 0;
+  for (int base = 0; base < nh; base += blockDim.x) {  // uniform trip count
+    const int i = base + threadIdx.x;
+    int nc = 0, nt = 0;
+    if (i < nh) {
+      const int v = front[lo + i];
+      const int64_t d = g.off[v + 1] - g.off[v];
+      int64_t tsz = 1;
+      while (tsz < 2 * d) tsz <<= 1;
+      nc = (int)((d + HUB_CH - 1) / HUB_CH);
+      nt = (int)((tsz + HUB_TCH - 1) / HUB_TCH);
+    }
+    int tot_c, tot_t;
+    const int xc = block_excl_scan<int>(nc, sh, &tot_c);
+    const int xt = block_excl_scan<int>(nt, sh, &tot_t);
+    if (i < nh) { cstart[i] = carry_c + xc; tstart[i] = carry_t + xt; }
+    carry_c += tot_c;
+    carry_t += tot_t;
+  }
+  if (threadIdx.x == 0) { cstart[nh] = carry_c; tstart[nh] = carry_t; }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_hub_agg(DevGraph<T> g, DevState<T> st, const int32_t* __restrict__ front,
+                                                 const int32_t* __restrict__ bstart, int bin,
+                                                 const int32_t* __restrict__ plan) {
+  using acc = typename T::acc_t;
+  constexpr int NT = 256, TS = 2 * HUB_CH, LG = 13, PER = HUB_CH / NT;
+  static_assert((1 << LG) == TS, "table size");
+  __shared__ int s_keys[TS];
+  __shared__ acc s_vals[TS];
+  for (int t = threadIdx.x; t < TS; t += NT) { s_keys[t] = 0; s_vals[t] = acc(0); }
+  __syncthreads();
+  const int lo = bstart[bin], nh = bstart[bin + 1] - lo;
+  const int32_t* cstart = plan;
+  const int nq = cstart[nh];
+  for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+    const int i = hub_find(cstart, nh, q);
+    const int v = front[lo + i];
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    const int j0 = (q - cstart[i]) * HUB_CH;
+    const int j1 = min(d, j0 + HUB_CH);
+    // PER entries per thread: all row loads, then all community gathers, in flight
+    int cu[PER];
+    acc wv[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int j = j0 + r * NT + threadIdx.x;
+      cu[r] = j < j1 ? g.nbr[o + j] : -1;
+      wv[r] = j < j1 ? (acc)load_w<T>(g.w, o + j) : acc(0);
+    }
+#pragma unroll
+    for (int r = 0; r < PER; r++) cu[r] = cu[r] >= 0 ? st.prev[cu[r]] : -1;
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int cl = cu[r];
+      if (cl < 0) continue;
+      unsigned p = hash_slot(cl, LG);
+      for (int probe = 0; probe < TS; probe++) {
+        const int k = atomicCAS(&s_keys[p], 0, cl + 1);
+        if (k == 0 || k == cl + 1) { atomicAdd(&s_vals[p], wv[r]); break; }
+        p = (p + 1) & (TS - 1);
+      }
+    }
+    __syncthreads();
+    // flush the distinct communities into the hub's global slice
+    const int glg = ilog2_ceil(2 * d);
+    const int gsz = 1 << glg;
+    int* gkeys = st.hub_keys + st.hub_off[v];
+    acc* gvals = st.hub_vals + st.hub_off[v];
+    for (int t = threadIdx.x; t < TS; t += NT) {
+      const int key = s_keys[t];
+      if (key) {
+        const acc a = s_vals[t];
+        s_keys[t] = 0;
+        s_vals[t] = acc(0);
+        unsigned p = hash_slot(key - 1, glg);
+        int probe = 0;
+        for (; probe < gsz; probe++) {
+          const int k = atomicCAS(&gkeys[p], 0, key);
+          if (k == 0 || k == key) { atomicAdd(&gvals[p], a); break; }
+          p = (p + 1) & (gsz - 1);
+        }
+        if (probe == gsz) st.ctr->overflow = 1;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_hub_scan(DevGraph<T> g, DevState<T> st, const int32_t* __restrict__ front,
+                                                  const int32_t* __restrict__ bstart, int bin,
+                                                  const int32_t* __restrict__ plan, HubPart<T>* __restrict__ part,
+                                                  typename T::tot_t m2_all) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  constexpr int NT = 256;
+  __shared__ G r_h[NT / 64];
+  __shared__ int r_d[NT / 64];
+  __shared__ acc r_wc[NT / 64];
+  const int lo = bstart[bin], nh = bstart[bin + 1] - lo;
+  const int32_t* tstart = plan + nh + 1;
+  const int nt = tstart[nh];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q = blockIdx.x; q < nt; q += gridDim.x) {
+    const int i = hub_find(tstart, nh, q);
+    const int v = front[lo + i];
+    const tt m2 = vertex_m2<T>(st, v, m2_all);
+    const int d = (int)(g.off[v + 1] - g.off[v]);
+    const int gsz = 1 << ilog2_ceil(2 * d);
+    const int s0 = (q - tstart[i]) * HUB_TCH;
+    const int s1 = min(gsz, s0 + HUB_TCH);
+    int* gkeys = st.hub_keys + st.hub_off[v];
+    acc* gvals = st.hub_vals + st.hub_off[v];
+    const int c = st.prev[v];
+    const tt K = st.K[v];
+    const tt Tc = st.Tot[c];
+    G h = 0;
+    int bd = INT_MAX;
+    acc wc = 0;
+    for (int t = s0 + threadIdx.x; t < s1; t += NT) {
+      const int k = gkeys[t];
+      if (k) {
+        const int dd = k - 1;
+        const acc a = gvals[t];
+        gkeys[t] = 0;
+        gvals[t] = acc(0);
+        if (dd == c) {
+          wc = a;
+        } else {
+          const G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)st.Tot[dd], (G)Tc);
+          if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; }
+        }
+      }
+    }
+    argmax_xor<G, 64>(h, bd);
+    wc = sum_xor<acc, 64>(wc);
+    if (lane == 0) { r_h[wid] = h; r_d[wid] = bd; r_wc[wid] = wc; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < NT / 64; w++) {
+        if (better(r_h[w], r_d[w], h, bd)) { h = r_h[w]; bd = r_d[w]; }
+        wc += r_wc[w];
+      }
+      part[q].h = h;
+      part[q].d = bd;
+      part[q].wc = wc;
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_hub_commit(DevGraph<T> g, DevState<T> st, const int32_t* __restrict__ front,
+                                                    const int32_t* __restrict__ bstart, int bin,
+                                                    const int32_t* __restrict__ plan,
+                                                    const HubPart<T>* __restrict__ part, typename T::tot_t m2_all) {
+  using acc = typename T::acc_t;
+  using G = typename T::gain_t;
+  constexpr int NT = 256;
+  __shared__ int s_mv;
+  const int lo = bstart[bin], nh = bstart[bin + 1] - lo;
+  const int32_t* tstart = plan + nh + 1;
+  unsigned moved = 0, swept = 0;
+  unsigned long long edges = 0;
+  for (int i = blockIdx.x; i < nh; i += gridDim.x) {
+    const int v = front[lo + i];
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    if (threadIdx.x == 0) {
+      G h = 0;
+      int bd = INT_MAX;
+      acc wc = 0;
+      for (int q = tstart[i]; q < tstart[i + 1]; q++) {  // chunk order: the same sums every run
+        if (better(part[q].h, part[q].d, h, bd)) { h = part[q].h; bd = part[q].d; }
+        wc += part[q].wc;
+      }
+      const int c = st.prev[v];
+      swept++;
+      edges += d;
+      const int mv = commit_move<T>(v, c, h, bd, wc, st.K[v], st.Self[v], vertex_m2<T>(st, v, m2_all), st);
+      moved += mv;
+      s_mv = mv;
+    }
+    __syncthreads();
+    if (st.dirty && s_mv) mark_row(st.dirty, g.nbr, o, d, v, threadIdx.x, NT);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, moved);
+    atomic_add_ull(&st.ctr->swept, swept);
+    atomic_add_ull(&st.ctr->swept_edges, edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], edges);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // First sweep of a round (every vertex its own community, rows duplicate-free):
 // each entry is its own candidate, so no aggregation and no prev[] gathers:
 // H(u) = w(v,u) m2 - K (Tot_u - Tot_v + K), w_c = Self[v].  Streaming argmax.
@@ -1299,19 +1860,28 @@ static int grid_for(int cap, int per_block, int max_blocks) {
 // they differ only in how lanes are mapped to vertices):
 //   bins 0-3: 0 = xor ring over W lanes, 1 = LDS hash over W lanes,
 //             2 = xor ring over W lanes, four vertices in flight per group,
-//             3 = one thread per vertex (integer path; FP falls back to 2)
+//             3 = one thread per vertex (integer path; FP falls back to 2),
+//             4 (bins 2-3) = pipelined leader hash, W/2 lanes x 2 entries
 //   bin 4:    0 = one wave per vertex, 1 = 32 lanes, 2 = 16 lanes,
-//             3 = pipelined, 32 lanes x 4 entries, 4 = pipelined, 16 lanes x 8 entries
-//   bin 5:    0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 64 lanes x 8 entries
+//             3 = pipelined, 32 lanes x 4 entries, 4 = pipelined, 16 lanes x 8 entries,
+//             5 = pipelined leader hash, 32 lanes x 4 entries, 6 = the same, 16 x 8
+//   bin 5:    0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 64 lanes x 8 entries,
+//             3 = pipelined leader hash, 64 lanes x 8 entries
+//   bins 6-7: 0 = one workgroup per vertex, table scan, 1 = one workgroup per
+//             vertex, leader evaluation (bin 7 FP: the hub kernel)
+//   bin 8:    0 = one workgroup per hub, 1 = hubs split over row and table
+//             chunks (plan / agg / scan / commit)
 // First sweeps: bins 0-3 have 0 = streaming argmax, 1 = pipelined (four
 // vertices in flight per group), 2 = one thread per vertex (integer path; FP
-// falls back to 1); bin 4 has 0 = one wave per vertex, 1 = 32 lanes.
+// falls back to 1), 3 (bin 3) = pipelined leader kernel, 16 lanes x 2 entries;
+// bin 4 has 0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 16 lanes x
+// 8 entries; bin 5 has 0 = one wave per vertex, 1 = pipelined, 64 lanes x 8.
 int decide_variants(int bin) {
-  static const int nv[NBINS] = {4, 4, 4, 4, 5, 3, 1, 1, 1};
+  static const int nv[NBINS] = {4, 4, 5, 5, 7, 4, 2, 2, 2};
   return bin >= 0 && bin < NBINS ? nv[bin] : 1;
 }
 int decide_first_variants(int bin) {
-  static const int nv[NBINS] = {3, 3, 3, 3, 2, 1, 1, 1, 1};
+  static const int nv[NBINS] = {3, 3, 3, 4, 3, 2, 1, 1, 1};
   return bin >= 0 && bin < NBINS ? nv[bin] : 1;
 }
 
@@ -1348,17 +1918,21 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
     case 17: k_decide_wave<32, 16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 18: k_decide_pipe<16, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 19: TPV_OR(15, false, 16, 64); break;
+    case 20: k_decide_hlead<32, 8, 2, false, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     // bin 3: 16..31
     case 24: k_decide_ring<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 25: k_decide_wave<64, 32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 26: k_decide_pipe<32, false, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 27: TPV_OR(31, false, 32, 32); break;
+    case 28: k_decide_hlead<64, 16, 2, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     // bin 4: 32..127
     case 32: k_decide_wave<256, 64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 33: k_decide_wave<256, 32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 34: k_decide_wave<256, 16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 35: k_decide_hpipe<256, 32, 4, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 36: k_decide_hpipe<256, 16, 8, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 37: k_decide_hlead<256, 32, 4, false, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 38: k_decide_hlead<256, 16, 8, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     // bin 5: 128..511
     case 40: k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 41:
@@ -1373,8 +1947,10 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
       else
         k_decide_hpipe<1024, 64, 8, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       break;
+    case 43: k_decide_hlead<1024, 64, 8, false, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     // bin 6: 512..4095
     case 48: k_decide_block<256, 8192, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 49: k_decide_blead<256, 8192, 16, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     // bin 7: 4096..8191
     case 56:
       if constexpr (T::FP)
@@ -1382,8 +1958,25 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
       else
         k_decide_block<1024, 16384, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
       break;
+    case 57:
+      if constexpr (T::FP)  // 16384 FP64 slots exceed the LDS: the hub kernel
+        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
+      else
+        k_decide_blead<1024, 16384, 8, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
+      break;
     // bin 8: >= 8192
     case 64: k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 65:
+      if (st.hub_plan && st.hub_part) {
+        k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan);
+        k_hub_agg<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
+        k_hub_scan<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, (HubPart<T>*)st.hub_part, m2);
+        k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan,
+                                                                (const HubPart<T>*)st.hub_part, m2);
+      } else {
+        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2);
+      }
+      break;
   }
 }
 
@@ -1406,9 +1999,12 @@ void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const i
     case 24: k_decide_first<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 25: k_decide_pipe<32, true, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 26: TPV_OR(31, true, 32, 32); break;
+    case 27: k_decide_hlead<1, 16, 2, true, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 32: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 33: k_decide_first<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 34: k_decide_hlead<1, 16, 8, true, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 40: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 41: k_decide_hlead<1, 64, 8, true, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 48: k_decide_first_block<256, T><<<grid_for(cap, 1, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 56: k_decide_first_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 64: k_decide_first_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2); break;

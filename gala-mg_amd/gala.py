@@ -26,9 +26,18 @@ EXPORTS = [
     "gala_gen_rmat", "gala_set_logger", "gala_set_mlp", "gala_get_mlp", "gala_mlp_train",
     "gala_mlp_forward", "gala_mlp_grad", "gala_batch_set", "gala_batch_run", "gala_batch_graph_results",
     "gala_batch_communities", "gala_gen_sbm_batch", "gala_set_variants", "gala_get_variants", "gala_autotune",
+    "gala_variant_counts",
 ]
 N_VARIANT_SLOTS = 27  # kernel variant per bin: 9 general sweeps, 9 first sweeps, 9 weight updates
-VARIANT_COUNTS = [4, 4, 4, 4, 5, 3, 1, 1, 1] + [3, 3, 3, 3, 2, 1, 1, 1, 1] + [3, 3, 3, 3, 2, 1, 1, 1, 1]
+
+
+def variant_counts():
+    """Kernel variants of every slot (9 general sweeps, 9 first sweeps, 9
+    weight updates), as the library defines them; needs no device."""
+    import numpy as np
+    c = np.zeros(N_VARIANT_SLOTS, np.int32)
+    _check(lib().gala_variant_counts(_p(c, ctypes.c_int32)), "gala_variant_counts")
+    return [int(x) for x in c]
 
 MLP_NFEAT = 6
 MLP_NPARAM = 1153  # W1[16][64] (rows 6..15 zero), b1[64], w2[64], b2
@@ -179,6 +188,7 @@ def lib():
         L.gala_set_variants.argtypes = [VP, I32P]
         L.gala_get_variants.argtypes = [VP, I32P]
         L.gala_autotune.argtypes = [VP, I32P]
+        L.gala_variant_counts.argtypes = [I32P]
         _lib = L
     return _lib
 

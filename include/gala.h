@@ -209,6 +209,10 @@ int gala_mlp_grad(gala_ctx* ctx, const float* params, const float* X, const uint
 int gala_set_variants(gala_ctx* ctx, const int32_t* variants);
 int gala_get_variants(gala_ctx* ctx, int32_t* variants);
 int gala_autotune(gala_ctx* ctx, int32_t* variants_out);
+/* Number of kernel variants of every slot (GALA_NVARIANTS entries, same order
+ * as gala_set_variants): a valid variant of slot i is 0 .. counts[i]-1.  Needs
+ * no device and no context. */
+int gala_variant_counts(int32_t* counts_out);
 
 /* ---- batched graphs (no counterpart in the reference, which runs one graph
  * per process) ----

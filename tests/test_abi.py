@@ -32,6 +32,19 @@ def test_library_exports_every_declared_symbol():
     assert gala.lib().gala_version().startswith(b"gala-mi355x")
 
 
+def test_variant_counts_match_bench_tables():
+    # the bench names the kernel of every (bin, variant) the library can launch
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_tables", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    counts = gala.variant_counts()
+    assert len(counts) == 27 and min(counts) >= 1
+    assert [len(x) for x in bench.KERNEL_GENERAL] == counts[:9]
+    assert [len(x) for x in bench.KERNEL_FIRST] == counts[9:18]
+    assert all(len(set(x)) == len(x) for x in bench.KERNEL_GENERAL)
+
+
 def test_strerror_codes():
     L = gala.lib()
     assert L.gala_strerror(0) == b"ok"

@@ -82,10 +82,10 @@ def test_chung_lu_hub_bins(seed):
 
 
 def variant_set(k):
-    return np.array([min(k, c - 1) for c in gala.VARIANT_COUNTS], np.int32)
+    return np.array([min(k, c - 1) for c in gala.variant_counts()], np.int32)
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6])
 def test_decide_variants_match_oracle(k):
     # every decide kernel variant takes the same decisions as the oracle
     say(f"decide variants {k}")
@@ -97,7 +97,7 @@ def test_decide_variants_match_oracle(k):
     eng.close()
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6])
 def test_decide_variants_random_states(k):
     say(f"decide variants {k} on random states")
     rng = np.random.default_rng(40 + k)
@@ -122,7 +122,8 @@ def test_autotune_keeps_results():
     r0 = eng.run()
     c0 = eng.communities(g.vertex_num)
     v = eng.autotune()
-    assert all(0 <= v[i] < gala.VARIANT_COUNTS[i] for i in range(len(v)))
+    counts = gala.variant_counts()
+    assert all(0 <= v[i] < counts[i] for i in range(len(v)))
     assert np.array_equal(eng.get_variants(), v)
     r1 = eng.run()
     assert r1["modularity"] == r0["modularity"] and np.array_equal(eng.communities(g.vertex_num), c0)
@@ -275,6 +276,39 @@ def test_phase_decide_hub_overflow(engine):
         got = engine.phase_decide(g, m2, K, Self, Tot, size, prev)
         exp = oracle.decide_step(g.vertex_num, g.degrees, g.neighbors, w, m2, K, Self, Tot, size, prev)
         assert np.array_equal(got, exp), rep
+
+
+@pytest.mark.parametrize("ncomm", [3, 40000])
+def test_phase_decide_split_hub(ncomm):
+    # bin 8 split over row chunks and table chunks (variant 1): a 50k-degree hub
+    # whose neighbours sit in few or in ~40k distinct communities; the second
+    # call checks the global slices were left zeroed
+    say(f"phase decide split hub {ncomm}")
+    g = star_plus_random(50000, 60000, 1)
+    rng = np.random.default_rng(6)
+    K, Self, Tot, size, prev = random_state(rng, g, ncomm)
+    m2 = int(K.sum())
+    w = np.ones(len(g.neighbors), np.int64)
+    eng = gala.Engine()
+    v = eng.get_variants()
+    v[8] = 1
+    eng.set_variants(v)
+    for rep in range(2):
+        got = eng.phase_decide(g, m2, K, Self, Tot, size, prev)
+        exp = oracle.decide_step(g.vertex_num, g.degrees, g.neighbors, w, m2, K, Self, Tot, size, prev)
+        assert np.array_equal(got, exp), rep
+    eng.close()
+
+
+def test_star_graph_split_hub_full_run():
+    say("star full run, split hub")
+    g = star_plus_random(30000, 40000, 3)
+    eng = gala.Engine(dump_iterations=True)
+    v = eng.get_variants()
+    v[8] = 1
+    eng.set_variants(v)
+    full_trace_compare(eng, g)
+    eng.close()
 
 
 def test_phase_contract_huge_row(engine):
