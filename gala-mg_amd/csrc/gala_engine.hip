@@ -147,7 +147,7 @@ struct gala_ctx {
   // defaults are what gala_autotune picks on the 4M-vertex power-law graph
   // (one thread per vertex below degree 32, the pipelined leader-hash kernels
   // for 32..511, leader evaluation in the workgroup kernels for 512..8191)
-  int dvar[NBINS] = {3, 3, 3, 3, 6, 3, 1, 1, 0};
+  int dvar[NBINS] = {3, 3, 3, 3, 6, 3, 1, 1, 2};
   int fvar[NBINS] = {1, 2, 2, 2, 2, 1, 0, 0, 0};
   int wvar[NBINS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // weight update (1 = pipelined, bins 0-4; 2 = one thread per vertex, bins 0-3)
 
@@ -524,7 +524,14 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   // decide-and-move over one frontier, timed into t_dm (and per bin)
   // build_cv: refresh the community view first (a certification sweep reads
   // the view of the pre-sweep state, which is still in place)
-  auto decide = [&](DevState<T>& sd, const int32_t* fr, const int32_t* bs, bool first, bool build_cv) -> int {
+  // The integer path updates In from the movers' rows alone (no dirty marks);
+  // the FP path and the tracked (learned / logged) runs rescan dirty rows,
+  // which keeps the FP summation order fixed and yields the neighbour-moved
+  // fractions.
+  const bool incr = !T::FP && !track;
+  auto decide = [&](DevState<T>& sd0, const int32_t* fr, const int32_t* bs, bool first, bool build_cv) -> int {
+    DevState<T> sd = sd0;
+    if (incr) sd.dirty = nullptr;
     HIPOK(hipEventRecord(ctx->ev[0], s));
     if (build_cv) launch_build_cv<T>(sd, g.n, s);
     for (int b = 0; b < NBINS; b++) {
@@ -541,7 +548,16 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   auto update = [&](DevState<T>& sp, bool first, Counters* out) -> int {
     launch_apply<T>(st, g.n, m2, s);
     // In changes only where a vertex or one of its neighbours moved
-    for (int b = 0; b < NBINS; b++) launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s, ctx->wvar[b]);
+    if constexpr (!T::FP) {
+      if (incr) {
+        for (int b = 0; b < NBINS; b++) launch_weight_moved<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], s);
+        for (int b = WEIGHT_RESCAN_BIN; b < NBINS; b++)  // stayed, with a moving neighbour
+          launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s, ctx->wvar[b]);
+      }
+    }
+    if (!incr)
+      for (int b = 0; b < NBINS; b++)
+        launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s, ctx->wvar[b]);
     launch_prune<T>(sp, g.n, m2, s);
     partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, fnext, bnext, ctx->part_tmp, s);
     HIPOK(hipEventRecord(ctx->ev[2], s));

@@ -131,6 +131,14 @@ void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t 
 template <class T>
 void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
                    const int32_t* bstart, int cap, bool only_dirty, hipStream_t s, int variant = 0);
+// Incremental In update of the integer path: the rows of the vertices that
+// moved (cur != prev) over order[bstart[b] .. bstart[b+1]); a mover rewrites
+// its own In and adds the change of every shared entry to the In of each
+// neighbour that stayed.  Decide-and-move needs no dirty marks for it.
+constexpr int WEIGHT_RESCAN_BIN = 6;
+template <class T>
+void launch_weight_moved(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
+                         const int32_t* bstart, int cap, hipStream_t s);
 // prune flags, sum In, active count; clears the dirty marks.
 template <class T>
 void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);

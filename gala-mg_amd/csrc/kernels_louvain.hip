@@ -1361,11 +1361,12 @@ __global__ __launch_bounds__(NT) void k_decide_blead(DevGraph<T> g, DevState<T> 
 // arena for the overflow.  Global slices are kept all-zero between uses: the
 // scan that reads an overflow slot clears it.
 // ---------------------------------------------------------------------------
+// split_above > 0: rows longer than that are left to the split hub kernels.
 template <int LT, class T>
 __global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> st,
                                                       const int32_t* __restrict__ front,
                                                       const int32_t* __restrict__ bstart, int bin,
-                                                      typename T::tot_t m2_all) {
+                                                      typename T::tot_t m2_all, int split_above) {
   using acc = typename T::acc_t;
   using tt = typename T::tot_t;
   using G = typename T::gain_t;
@@ -1385,6 +1386,7 @@ __global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> 
     const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
+    if (split_above > 0 && d > split_above) continue;  // uniform per workgroup
     const int c = st.prev[v];
     const int glg = ilog2_ceil(2 * d);
     const int gsz = 1 << glg;
@@ -1529,7 +1531,7 @@ __device__ __forceinline__ int hub_find(const int32_t* __restrict__ start, int n
 template <class T>
 __global__ __launch_bounds__(1024) void k_hub_plan(DevGraph<T> g, const int32_t* __restrict__ front,
                                                    const int32_t* __restrict__ bstart, int bin,
-                                                   int32_t* __restrict__ plan) {
+                                                   int32_t* __restrict__ plan, int split_above) {
   __shared__ int sh[17];
   const int lo = bstart[bin], hi = bstart[bin + 1], nh = hi - lo;
   int32_t* cstart = plan;           // nh + 1 row-chunk starts
@@ -1541,10 +1543,12 @@ __global__ __launch_bounds__(1024) void k_hub_plan(DevGraph<T> g, const int32_t*
     if (i < nh) {
       const int v = front[lo + i];
       const int64_t d = g.off[v + 1] - g.off[v];
+      if (d > split_above) {
       int64_t tsz = 1;
       while (tsz < 2 * d) tsz <<= 1;
       nc = (int)((d + HUB_CH - 1) / HUB_CH);
       nt = (int)((tsz + HUB_TCH - 1) / HUB_TCH);
+      }
     }
     int tot_c, tot_t;
     const int xc = block_excl_scan<int>(nc, sh, &tot_c);
@@ -1703,6 +1707,7 @@ __global__ __launch_bounds__(256) void k_hub_commit(DevGraph<T> g, DevState<T> s
   unsigned moved = 0, swept = 0;
   unsigned long long edges = 0;
   for (int i = blockIdx.x; i < nh; i += gridDim.x) {
+    if (tstart[i] == tstart[i + 1]) continue;  // left to k_decide_hub (uniform per workgroup)
     const int v = front[lo + i];
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
@@ -1870,14 +1875,15 @@ static int grid_for(int cap, int per_block, int max_blocks) {
 //   bins 6-7: 0 = one workgroup per vertex, table scan, 1 = one workgroup per
 //             vertex, leader evaluation (bin 7 FP: the hub kernel)
 //   bin 8:    0 = one workgroup per hub, 1 = hubs split over row and table
-//             chunks (plan / agg / scan / commit)
+//             chunks (plan / agg / scan / commit), 2 = rows longer than the
+//             LDS table split, the others one workgroup per hub
 // First sweeps: bins 0-3 have 0 = streaming argmax, 1 = pipelined (four
 // vertices in flight per group), 2 = one thread per vertex (integer path; FP
 // falls back to 1), 3 (bin 3) = pipelined leader kernel, 16 lanes x 2 entries;
 // bin 4 has 0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 16 lanes x
 // 8 entries; bin 5 has 0 = one wave per vertex, 1 = pipelined, 64 lanes x 8.
 int decide_variants(int bin) {
-  static const int nv[NBINS] = {4, 4, 5, 5, 7, 4, 2, 2, 2};
+  static const int nv[NBINS] = {4, 4, 5, 5, 7, 4, 2, 2, 3};
   return bin >= 0 && bin < NBINS ? nv[bin] : 1;
 }
 int decide_first_variants(int bin) {
@@ -1954,29 +1960,32 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
     // bin 7: 4096..8191
     case 56:
       if constexpr (T::FP)
-        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
+        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0);
       else
         k_decide_block<1024, 16384, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
       break;
     case 57:
       if constexpr (T::FP)  // 16384 FP64 slots exceed the LDS: the hub kernel
-        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
+        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0);
       else
         k_decide_blead<1024, 16384, 8, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
       break;
     // bin 8: >= 8192
-    case 64: k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 64: k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0); break;
     case 65:
-      if (st.hub_plan && st.hub_part) {
-        k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan);
-        k_hub_agg<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
-        k_hub_scan<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, (HubPart<T>*)st.hub_part, m2);
-        k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan,
-                                                                (const HubPart<T>*)st.hub_part, m2);
-      } else {
-        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2);
-      }
+    case 66: {
+      // 1: every hub split; 2: only rows longer than the LDS table, the rest
+      // one workgroup per hub
+      const int above = variant == 1 ? 0 : HUB_LT;
+      if (variant == 2)
+        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, HUB_LT);
+      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above);
+      k_hub_agg<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
+      k_hub_scan<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, (HubPart<T>*)st.hub_part, m2);
+      k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan,
+                                                              (const HubPart<T>*)st.hub_part, m2);
       break;
+    }
   }
 }
 
@@ -2350,6 +2359,109 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Incremental weight update (integer path): only the rows of the vertices that
+// moved in the sweep (cur != prev) are read.  A mover v recomputes In[v] over
+// its row; for every neighbour u that did not move, the entry's contribution
+// to In[u] changes by w * ([cur[v] == c_u] - [prev[v] == c_u]), which v adds
+// to In[u] (exact integer atomics).  A vertex none of whose neighbours moved
+// keeps its In.  Work per sweep: the movers' rows, against the rows of the
+// movers and all their neighbours for the dirty-row update.  Neighbours of
+// degree >= 512 (bins >= WEIGHT_RESCAN_BIN) are marked dirty instead and
+// rescanned whole afterwards (launch_weight, only_dirty): a hub with many
+// moving neighbours would otherwise take that many atomics on one address.
+// ---------------------------------------------------------------------------
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_weight_moved(DevGraph<T> g, DevState<T> st,
+                                                       const int32_t* __restrict__ order,
+                                                       const int32_t* __restrict__ bstart, int bin) {
+  using tt = typename T::tot_t;
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & (W - 1);
+  const int groups = gridDim.x * (blockDim.x / W);
+  for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W; i < hi; i += groups) {
+    const int v = order[i];
+    const int c = st.cur[v], p = st.prev[v];
+    if (c == p) continue;  // uniform per group
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    tt x = 0;
+    for (int j = lane; j < d; j += W) {
+      const int u = g.nbr[o + j];
+      const tt w = (tt)load_w<T>(g.w, o + j);
+      const int cu = st.cur[u];
+      if (cu == c) x += w;
+      if (cu == st.prev[u]) {  // u stayed: its In gains or loses this entry
+        const tt dl = (cu == c ? w : tt(0)) - (cu == p ? w : tt(0));
+        if (dl) {
+          if (st.binid[u] >= WEIGHT_RESCAN_BIN) st.dirty[u] = 1;  // rescanned whole
+          else atomicAdd(&st.In[u], dl);
+        }
+      }
+    }
+    x = sum_xor<tt, W>(x);
+    if (lane == 0) st.In[v] = x;
+  }
+}
+
+template <int NT, class T>
+__global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevState<T> st,
+                                                            const int32_t* __restrict__ order,
+                                                            const int32_t* __restrict__ bstart, int bin) {
+  using tt = typename T::tot_t;
+  __shared__ tt r[NT / 64];
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
+    const int v = order[i];
+    const int c = st.cur[v], p = st.prev[v];
+    if (c == p) continue;  // uniform per block
+    const int64_t o = g.off[v];
+    const int d = (int)(g.off[v + 1] - o);
+    tt x = 0;
+    for (int j = threadIdx.x; j < d; j += NT) {
+      const int u = g.nbr[o + j];
+      const tt w = (tt)load_w<T>(g.w, o + j);
+      const int cu = st.cur[u];
+      if (cu == c) x += w;
+      if (cu == st.prev[u]) {
+        const tt dl = (cu == c ? w : tt(0)) - (cu == p ? w : tt(0));
+        if (dl) {
+          if (st.binid[u] >= WEIGHT_RESCAN_BIN) st.dirty[u] = 1;
+          else atomicAdd(&st.In[u], dl);
+        }
+      }
+    }
+    x = sum_xor<tt, 64>(x);
+    if (lane == 0) r[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      x = 0;
+      for (int q = 0; q < NT / 64; q++) x += r[q];
+      st.In[v] = x;
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+void launch_weight_moved(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
+                         const int32_t* bstart, int cap, hipStream_t s) {
+  if (cap <= 0) return;
+  static_assert(!T::FP, "integer path only: the FP path keeps a deterministic summation order");
+  switch (bin) {
+    case 0: k_weight_moved<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, order, bstart, 0); break;
+    case 1: k_weight_moved<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, order, bstart, 1); break;
+    case 2: k_weight_moved<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, order, bstart, 2); break;
+    case 3: k_weight_moved<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, order, bstart, 3); break;
+    case 4: k_weight_moved<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, order, bstart, 4); break;
+    case 5: k_weight_moved<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, order, bstart, 5); break;
+    case 6: k_weight_moved_block<256, T><<<grid_for(cap, 1, 4096), 256, 0, s>>>(g, st, order, bstart, 6); break;
+    case 7: k_weight_moved_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, order, bstart, 7); break;
+    case 8: k_weight_moved_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, order, bstart, 8); break;
+  }
+}
+
 template <class T>
 void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
                    const int32_t* bstart, int cap, bool only_dirty, hipStream_t s, int variant) {
@@ -2477,5 +2589,7 @@ void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t 
   template void launch_prune<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t);
 GALA_INST(IntW)
 GALA_INST(FltW)
+template void launch_weight_moved<IntW>(int, const DevGraph<IntW>&, DevState<IntW>&, const int32_t*, const int32_t*,
+                                        int, hipStream_t);
 
 }  // namespace gala

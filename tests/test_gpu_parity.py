@@ -278,12 +278,13 @@ def test_phase_decide_hub_overflow(engine):
         assert np.array_equal(got, exp), rep
 
 
+@pytest.mark.parametrize("variant", [1, 2])
 @pytest.mark.parametrize("ncomm", [3, 40000])
-def test_phase_decide_split_hub(ncomm):
+def test_phase_decide_split_hub(ncomm, variant):
     # bin 8 split over row chunks and table chunks (variant 1): a 50k-degree hub
     # whose neighbours sit in few or in ~40k distinct communities; the second
     # call checks the global slices were left zeroed
-    say(f"phase decide split hub {ncomm}")
+    say(f"phase decide split hub {ncomm} variant {variant}")
     g = star_plus_random(50000, 60000, 1)
     rng = np.random.default_rng(6)
     K, Self, Tot, size, prev = random_state(rng, g, ncomm)
@@ -291,7 +292,7 @@ def test_phase_decide_split_hub(ncomm):
     w = np.ones(len(g.neighbors), np.int64)
     eng = gala.Engine()
     v = eng.get_variants()
-    v[8] = 1
+    v[8] = variant
     eng.set_variants(v)
     for rep in range(2):
         got = eng.phase_decide(g, m2, K, Self, Tot, size, prev)
@@ -300,12 +301,13 @@ def test_phase_decide_split_hub(ncomm):
     eng.close()
 
 
-def test_star_graph_split_hub_full_run():
-    say("star full run, split hub")
+@pytest.mark.parametrize("variant", [1, 2])
+def test_star_graph_split_hub_full_run(variant):
+    say(f"star full run, split hub variant {variant}")
     g = star_plus_random(30000, 40000, 3)
     eng = gala.Engine(dump_iterations=True)
     v = eng.get_variants()
-    v[8] = 1
+    v[8] = variant
     eng.set_variants(v)
     full_trace_compare(eng, g)
     eng.close()
