@@ -50,10 +50,11 @@ KERNEL_GENERAL = [
      "k_decide_hlead<256, 32, 4, false, IntW>", "k_decide_hlead<256, 16, 8, false, IntW>"],
     ["k_decide_wave<1024, 64, IntW>", "k_decide_wave<1024, 32, IntW>", "k_decide_hpipe<1024, 64, 8, IntW>",
      "k_decide_hlead<1024, 64, 8, false, IntW>"],
-    ["k_decide_block<256, 8192, IntW>", "k_decide_blead<256, 8192, 16, IntW>"],
-    ["k_decide_block<1024, 16384, IntW>", "k_decide_blead<1024, 16384, 8, IntW>"],
+    ["k_decide_block<256, 8192, IntW>", "k_decide_blead<256, 8192, 16, 1, IntW>"],
+    ["k_decide_block<1024, 16384, IntW>", "k_decide_blead<1024, 16384, 8, 1, IntW>"],
     ["k_decide_hub<16384, IntW>", "k_hub_plan/agg/scan/commit<IntW>",
-     "k_decide_hub<16384, IntW> + k_hub_plan/agg/scan/commit<IntW>"]]
+     "k_decide_hub<16384, IntW> + k_hub_plan/agg/scan/commit<IntW>",
+     "k_decide_blead<1024, 16384, 16, 2, IntW> + k_hub_plan/agg/scan/commit<IntW>"]]
 KERNEL_FIRST = [
     ["k_decide_first<4, IntW>", "k_decide_pipe<4, true, IntW>", "k_decide_tpv<3, true, IntW>"],
     ["k_decide_first<8, IntW>", "k_decide_pipe<8, true, IntW>", "k_decide_tpv<7, true, IntW>"],

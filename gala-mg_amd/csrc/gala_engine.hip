@@ -551,8 +551,6 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     if constexpr (!T::FP) {
       if (incr) {
         for (int b = 0; b < NBINS; b++) launch_weight_moved<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], s);
-        for (int b = WEIGHT_RESCAN_BIN; b < NBINS; b++)  // stayed, with a moving neighbour
-          launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s, ctx->wvar[b]);
       }
     }
     if (!incr)
@@ -700,12 +698,15 @@ ContractBufs<T> cbufs(gala_ctx* ctx) {
 
 // Contract the round graph under comm into buffer `dst`.  Returns C and new m;
 // ctx->c_label then holds new_id(comm[v]) for the composition.
+// canonical: rows in ascending neighbour order (when the CSR leaves the
+// engine); otherwise rows of up to the LDS table size keep hash order.
 template <class T>
 int contract(gala_ctx* ctx, const DevGraph<T>& g, const int32_t* comm, int dst, const int32_t* caps,
-             int32_t* C_out, int64_t* m_out) {
+             int32_t* C_out, int64_t* m_out, bool canonical) {
   hipStream_t s = ctx->stream;
   ContractBufs<T> cb = cbufs<T>(ctx);
   ContractPlan plan;
+  plan.canonical = canonical;
   for (int b = 0; b < NBINS; b++) plan.caps[b] = caps[b];
   if (contract_prepare<T>(g, comm, cb, &plan, s)) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
   int rc = grow_arena(ctx, plan.slots, T::FP);
@@ -762,7 +763,7 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
     if (round > 0) prev_mod = cur_mod;
     HIPOK(hipEventRecord(ctx->ev[5], s));
     RoundOut ro;
-    const bool unique_rows = src != 0 || ctx->unique0;  // contracted rows are canonical
+    const bool unique_rows = src != 0 || ctx->unique0;  // contracted rows are duplicate-free
     int rc = run_round<T>(ctx, g, m2, round, unique_rows, icb, user, &ro, hostvec);
     if (rc) return rc;
     cur_mod = ro.q;
@@ -770,7 +771,7 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
     int32_t C = 0;
     int64_t m = 0;
     const int dst = dst_of(src);
-    rc = contract<T>(ctx, g, ctx->prev, dst, ro.caps, &C, &m);
+    rc = contract<T>(ctx, g, ctx->prev, dst, ro.caps, &C, &m, ctx->opts.dump_rounds != 0);
     if (rc) return rc;
     compose_map(ctx->fin, ctx->n0, ctx->c_label, s);
     HIPOK(hipEventRecord(ctx->ev[7], s));
@@ -1104,7 +1105,7 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     int32_t C = 0;
     int64_t m = 0;
     const int dst = dst_of(src);
-    rc = contract<T>(ctx, g, ctx->prev, dst, caps, &C, &m);
+    rc = contract<T>(ctx, g, ctx->prev, dst, caps, &C, &m, false);
     if (rc) return rc;
     compose_map(ctx->fin, ctx->n0, ctx->c_label, s);
     const int gnext = gcur == 1 ? 2 : 1;
@@ -1464,7 +1465,7 @@ int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const
   reset_counters(ctx->ctr, 0, 0.0, ctx->stream);
   int32_t C = 0;
   int64_t m = 0;
-  if ((rc = contract<IntW>(ctx, g, ctx->prev, 1, caps, &C, &m))) return rc;
+  if ((rc = contract<IntW>(ctx, g, ctx->prev, 1, caps, &C, &m, true))) return rc;
   *out_C = C;
   *out_m = m;
   if (new_id) {

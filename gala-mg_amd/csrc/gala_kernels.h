@@ -135,7 +135,7 @@ void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
 // moved (cur != prev) over order[bstart[b] .. bstart[b+1]); a mover rewrites
 // its own In and adds the change of every shared entry to the In of each
 // neighbour that stayed.  Decide-and-move needs no dirty marks for it.
-constexpr int WEIGHT_RESCAN_BIN = 6;
+constexpr int WEIGHT_AGG_BIN = 6;
 template <class T>
 void launch_weight_moved(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
                          const int32_t* bstart, int cap, hipStream_t s);
@@ -196,6 +196,10 @@ struct ContractPlan {
   std::vector<int32_t> dense_c;     // their communities, ascending
   std::vector<DenseChunk> chunks;   // their segments cut into LT/2-entry chunks
   int64_t dense_entries = 0;        // staged entries of all dense rows
+  // rows of up to LT entries leave the merge in ascending neighbour order
+  // (the order preprocess writes); otherwise in hash-table order.  Louvain's
+  // decisions do not depend on the order of a row's entries.
+  bool canonical = true;
 };
 
 // Bytes of device scratch the dense-row sort of a plan needs.

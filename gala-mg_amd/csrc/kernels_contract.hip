@@ -303,7 +303,7 @@ __global__ __launch_bounds__(NT) void k_merge_mid(const int64_t* __restrict__ ec
                                                    const int32_t* __restrict__ tstart, int tr,
                                                    int32_t* __restrict__ skey,
                                                    typename T::acc_t* __restrict__ sval,
-                                                   int32_t* __restrict__ dcnt) {
+                                                   int32_t* __restrict__ dcnt, int canonical) {
   using acc = typename T::acc_t;
   constexpr int TMAX = 2 * LMAX;
   __shared__ int h_k[TMAX];
@@ -341,7 +341,11 @@ __global__ __launch_bounds__(NT) void k_merge_mid(const int64_t* __restrict__ ec
       if (k) { s_k[q] = k - 1; s_v[q] = h_v[t]; q++; }
     }
     __syncthreads();
-    sort_row<NT, LMAX / NT, acc>(s_k, s_v, D, h_k, h_v, skey, sval, base);
+    if (canonical) {
+      sort_row<NT, LMAX / NT, acc>(s_k, s_v, D, h_k, h_v, skey, sval, base);
+    } else {
+      for (int t = threadIdx.x; t < D; t += NT) { skey[base + t] = s_k[t]; sval[base + t] = s_v[t]; }
+    }
     if (threadIdx.x == 0) dcnt[c] = D;
     __syncthreads();
   }
@@ -364,7 +368,7 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
                                                      typename T::acc_t* __restrict__ arena_vals,
                                                      int32_t* __restrict__ skey,
                                                      typename T::acc_t* __restrict__ sval,
-                                                     int32_t* __restrict__ dcnt) {
+                                                     int32_t* __restrict__ dcnt, int canonical) {
   using acc = typename T::acc_t;
   constexpr int NT = 1024, PMAX = 32, PER = LT / NT;
   constexpr int LLG = __builtin_ctz(LT);
@@ -441,7 +445,11 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
       }
     }
     __syncthreads();
-    sort_row<NT, LT / NT, acc>(s_k, s_v, D, s_k, s_v, skey, sval, base);
+    if (canonical) {
+      sort_row<NT, LT / NT, acc>(s_k, s_v, D, s_k, s_v, skey, sval, base);
+    } else {
+      for (int t = threadIdx.x; t < D; t += NT) { skey[base + t] = s_k[t]; sval[base + t] = s_v[t]; }
+    }
     if (threadIdx.x == 0) dcnt[c] = D;
     __syncthreads();
   }
@@ -876,6 +884,7 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
   k_stage_rows<256, T><<<nblk(cp[7] + cp[8], 1, 4096), 256, 0, s>>>(g, cb.order, cb.bstart, 7, NBINS, cb.label,
                                                                     cb.eoff, ecur, cb.skey, cb.sval);
   fill<int32_t>(cb.dcnt, C + 1, 0, s);
+  const int canon = plan.canonical ? 1 : 0;
   const int32_t* ts = plan.tstart;
   auto tcount = [&](int t) { return ts[t + 1] - ts[t]; };
   if (tcount(TIER_T16) > 0)
@@ -889,17 +898,17 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
         cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey, cb.sval, cb.dcnt);
   if (tcount(TIER_MID1) > 0)
     k_merge_mid<64, 64, 512, T><<<nblk(tcount(TIER_MID1), 1, 16384), 64, 0, s>>>(
-        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID1, cb.skey, cb.sval, cb.dcnt);
+        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID1, cb.skey, cb.sval, cb.dcnt, canon);
   if (tcount(TIER_MID2) > 0)
     k_merge_mid<128, 512, 2048, T><<<nblk(tcount(TIER_MID2), 1, 8192), 128, 0, s>>>(
-        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID2, cb.skey, cb.sval, cb.dcnt);
+        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID2, cb.skey, cb.sval, cb.dcnt, canon);
   if (tcount(TIER_MID3) > 0)
     k_merge_mid<256, 2048, 4096, T><<<nblk(tcount(TIER_MID3), 1, 4096), 256, 0, s>>>(
-        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID3, cb.skey, cb.sval, cb.dcnt);
+        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID3, cb.skey, cb.sval, cb.dcnt, canon);
   if (plan.nbig > 0)
     k_merge_big<LT, T><<<std::min(plan.nbig, 2048), 1024, 0, s>>>(cb.ecnt, cb.eoff, C, cb.big_list, plan.nbig,
                                                                    cb.big_tab, cb.arena_keys, cb.arena_vals,
-                                                                   cb.skey, cb.sval, cb.dcnt);
+                                                                   cb.skey, cb.sval, cb.dcnt, canon);
   if (plan.ndense > 0) {
     using acc = typename T::acc_t;
     const int nch = (int)plan.chunks.size();

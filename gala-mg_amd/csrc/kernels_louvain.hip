@@ -1234,11 +1234,16 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
 // once per kernel, so a vertex costs O(d / NT) per thread, not
 // O(next_pow2(2d) / NT), and needs no Tot gathers.
 // ---------------------------------------------------------------------------
-template <int NT, int TMAX, int EPL, class T>
+// NPASS > 1 (rows of up to TMAX - 1 entries): the entries are split into
+// NPASS groups by a second hash of their community, and the table is filled,
+// evaluated and cleared once per group, so it stays at most about half full
+// while every entry stays in registers.  split_above > 0: longer rows are
+// left to the split hub kernels.
+template <int NT, int TMAX, int EPL, int NPASS, class T>
 __global__ __launch_bounds__(NT) void k_decide_blead(DevGraph<T> g, DevState<T> st,
                                                       const int32_t* __restrict__ front,
                                                       const int32_t* __restrict__ bstart, int bin,
-                                                      typename T::tot_t m2_all) {
+                                                      typename T::tot_t m2_all, int split_above) {
   using acc = typename T::acc_t;
   using tt = typename T::tot_t;
   using G = typename T::gain_t;
@@ -1262,10 +1267,13 @@ __global__ __launch_bounds__(NT) void k_decide_blead(DevGraph<T> g, DevState<T> 
     const tt m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);  // <= NT * EPL by bin construction
+    if (split_above > 0 && d > split_above) continue;  // uniform per workgroup
     const CV ev = st.cv[v];
     const int c = ev.c;
     int lg = ilog2_ceil(2 * d);
-    const int tmask = (1 << lg) - 1;  // <= TMAX slots
+    constexpr int LGMAX = __builtin_ctz(TMAX);
+    if (lg > LGMAX) lg = LGMAX;  // NPASS > 1 only: d < TMAX keeps a free slot
+    const int tmask = (1 << lg) - 1;
     int u[EPL];
     acc wv[EPL];
     CV eu[EPL];
@@ -1276,46 +1284,50 @@ __global__ __launch_bounds__(NT) void k_decide_blead(DevGraph<T> g, DevState<T> 
       wv[k] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
     }
 #pragma unroll
-    for (int k = 0; k < EPL; k++)
-      if (u[k] >= 0) eu[k] = st.cv[u[k]];
-    int sl[EPL];
-#pragma unroll
-    for (int k = 0; k < EPL; k++) {
-      sl[k] = -1;
-      if (u[k] >= 0) {
-        const int cl = eu[k].c;
-        unsigned p = hash_slot(cl, lg);
-        for (int probe = 0; probe <= tmask; probe++) {
-          const int kk = atomicCAS(&s_keys[p], 0, cl + 1);
-          if (kk == 0 || kk == cl + 1) {
-            atomicAdd(&s_vals[p], wv[k]);
-            sl[k] = (int)p | (kk == 0 ? LEAD : 0);
-            break;
-          }
-          p = (p + 1) & tmask;
-        }
-      }
-    }
-    __syncthreads();
+    for (int k = 0; k < EPL; k++) eu[k] = u[k] >= 0 ? st.cv[u[k]] : CV{};
     G h = 0;
     int bd = INT_MAX, bsz = 0;
     acc wc = 0;
     const tt K = st.K[v];
 #pragma unroll
-    for (int k = 0; k < EPL; k++) {
-      if (sl[k] >= LEAD) {
-        const int p = sl[k] & (LEAD - 1);
-        const acc a = s_vals[p];
-        s_keys[p] = 0;
-        s_vals[p] = acc(0);
+    for (int pass = 0; pass < NPASS; pass++) {
+      int sl[EPL];
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        sl[k] = -1;
         const int cl = eu[k].c;
-        if (cl == c) {
-          wc = a;
-        } else {
-          const G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)cv_tot(eu[k]), (G)cv_tot(ev));
-          if (hh > 0 && better(hh, cl, h, bd)) { h = hh; bd = cl; bsz = cv_sz(eu[k]); }
+        const bool mine = NPASS == 1 || (int)((((unsigned)cl * 0x85EBCA6Bu) >> 16) % NPASS) == pass;
+        if (u[k] >= 0 && mine) {
+          unsigned p = hash_slot(cl, lg);
+          for (int probe = 0; probe <= tmask; probe++) {
+            const int kk = atomicCAS(&s_keys[p], 0, cl + 1);
+            if (kk == 0 || kk == cl + 1) {
+              atomicAdd(&s_vals[p], wv[k]);
+              sl[k] = (int)p | (kk == 0 ? LEAD : 0);
+              break;
+            }
+            p = (p + 1) & tmask;
+          }
         }
       }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        if (sl[k] >= LEAD) {
+          const int p = sl[k] & (LEAD - 1);
+          const acc a = s_vals[p];
+          s_keys[p] = 0;
+          s_vals[p] = acc(0);
+          const int cl = eu[k].c;
+          if (cl == c) {
+            wc = a;
+          } else {
+            const G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)cv_tot(eu[k]), (G)cv_tot(ev));
+            if (hh > 0 && better(hh, cl, h, bd)) { h = hh; bd = cl; bsz = cv_sz(eu[k]); }
+          }
+        }
+      }
+      if (NPASS > 1) __syncthreads();  // slots cleared before the next group inserts
     }
 #pragma unroll
     for (int k = 32; k >= 1; k >>= 1) {
@@ -1876,14 +1888,16 @@ static int grid_for(int cap, int per_block, int max_blocks) {
 //             vertex, leader evaluation (bin 7 FP: the hub kernel)
 //   bin 8:    0 = one workgroup per hub, 1 = hubs split over row and table
 //             chunks (plan / agg / scan / commit), 2 = rows longer than the
-//             LDS table split, the others one workgroup per hub
+//             LDS table split, the others one workgroup per hub, 3 = rows
+//             longer than the LDS table split, the others one workgroup per
+//             hub with two-pass leader evaluation
 // First sweeps: bins 0-3 have 0 = streaming argmax, 1 = pipelined (four
 // vertices in flight per group), 2 = one thread per vertex (integer path; FP
 // falls back to 1), 3 (bin 3) = pipelined leader kernel, 16 lanes x 2 entries;
 // bin 4 has 0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 16 lanes x
 // 8 entries; bin 5 has 0 = one wave per vertex, 1 = pipelined, 64 lanes x 8.
 int decide_variants(int bin) {
-  static const int nv[NBINS] = {4, 4, 5, 5, 7, 4, 2, 2, 3};
+  static const int nv[NBINS] = {4, 4, 5, 5, 7, 4, 2, 2, 4};
   return bin >= 0 && bin < NBINS ? nv[bin] : 1;
 }
 int decide_first_variants(int bin) {
@@ -1956,7 +1970,7 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
     case 43: k_decide_hlead<1024, 64, 8, false, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     // bin 6: 512..4095
     case 48: k_decide_block<256, 8192, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 49: k_decide_blead<256, 8192, 16, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 49: k_decide_blead<256, 8192, 16, 1, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2, 0); break;
     // bin 7: 4096..8191
     case 56:
       if constexpr (T::FP)
@@ -1968,17 +1982,26 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
       if constexpr (T::FP)  // 16384 FP64 slots exceed the LDS: the hub kernel
         k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0);
       else
-        k_decide_blead<1024, 16384, 8, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
+        k_decide_blead<1024, 16384, 8, 1, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0);
       break;
     // bin 8: >= 8192
     case 64: k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0); break;
     case 65:
-    case 66: {
+    case 66:
+    case 67: {
       // 1: every hub split; 2: only rows longer than the LDS table, the rest
-      // one workgroup per hub
-      const int above = variant == 1 ? 0 : HUB_LT;
+      // one workgroup per hub; 3: the same with the two-pass leader kernel
+      // for rows shorter than the LDS table (integer path; FP: as 2)
+      const int above = variant == 1 ? 0 : variant == 2 ? HUB_LT : HUB_LT - 1;
       if (variant == 2)
         k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, HUB_LT);
+      if (variant == 3) {
+        if constexpr (T::FP)
+          k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, above);
+        else
+          k_decide_blead<1024, 16384, 16, 2, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b,
+                                                                                        m2, above);
+      }
       k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above);
       k_hub_agg<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
       k_hub_scan<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, (HubPart<T>*)st.hub_part, m2);
@@ -2366,16 +2389,47 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
 // to In[u] changes by w * ([cur[v] == c_u] - [prev[v] == c_u]), which v adds
 // to In[u] (exact integer atomics).  A vertex none of whose neighbours moved
 // keeps its In.  Work per sweep: the movers' rows, against the rows of the
-// movers and all their neighbours for the dirty-row update.  Neighbours of
-// degree >= 512 (bins >= WEIGHT_RESCAN_BIN) are marked dirty instead and
-// rescanned whole afterwards (launch_weight, only_dirty): a hub with many
-// moving neighbours would otherwise take that many atomics on one address.
+// movers and all their neighbours for the dirty-row update.  Deltas for
+// neighbours of degree >= 512 (bins >= WEIGHT_AGG_BIN) are first summed in a
+// per-workgroup LDS table and added once per workgroup at its end: a hub with
+// many moving neighbours would otherwise take that many atomics on one
+// address.  (Integer sums: the result does not depend on the order.)
 // ---------------------------------------------------------------------------
+constexpr int HD_SLOTS = 2048, HD_LG = 11, HD_PROBES = 8;
+
+template <class tt>
+__device__ __forceinline__ void hub_delta_add(int* s_hk, tt* s_hv, int u, tt dl, tt* In) {
+  unsigned p = hash_slot(u, HD_LG);
+  for (int probe = 0; probe < HD_PROBES; probe++) {
+    const int k = atomicCAS(&s_hk[p], 0, u + 1);
+    if (k == 0 || k == u + 1) { atomicAdd(&s_hv[p], dl); return; }
+    p = (p + 1) & (HD_SLOTS - 1);
+  }
+  atomicAdd(&In[u], dl);  // table crowded: straight to the target
+}
+
+template <class tt>
+__device__ __forceinline__ void hub_delta_init(int* s_hk, tt* s_hv) {
+  for (int t = threadIdx.x; t < HD_SLOTS; t += blockDim.x) { s_hk[t] = 0; s_hv[t] = 0; }
+  __syncthreads();
+}
+
+template <class tt>
+__device__ __forceinline__ void hub_delta_flush(const int* s_hk, const tt* s_hv, tt* In) {
+  __syncthreads();
+  for (int t = threadIdx.x; t < HD_SLOTS; t += blockDim.x) {
+    const int k = s_hk[t];
+    if (k && s_hv[t]) atomicAdd(&In[k - 1], s_hv[t]);
+  }
+}
 template <int W, class T>
 __global__ __launch_bounds__(256) void k_weight_moved(DevGraph<T> g, DevState<T> st,
                                                        const int32_t* __restrict__ order,
                                                        const int32_t* __restrict__ bstart, int bin) {
   using tt = typename T::tot_t;
+  __shared__ int s_hk[HD_SLOTS];
+  __shared__ tt s_hv[HD_SLOTS];
+  hub_delta_init(s_hk, s_hv);
   const int lo = bstart[bin], hi = bstart[bin + 1];
   const int lane = threadIdx.x & (W - 1);
   const int groups = gridDim.x * (blockDim.x / W);
@@ -2394,7 +2448,7 @@ __global__ __launch_bounds__(256) void k_weight_moved(DevGraph<T> g, DevState<T>
       if (cu == st.prev[u]) {  // u stayed: its In gains or loses this entry
         const tt dl = (cu == c ? w : tt(0)) - (cu == p ? w : tt(0));
         if (dl) {
-          if (st.binid[u] >= WEIGHT_RESCAN_BIN) st.dirty[u] = 1;  // rescanned whole
+          if (st.binid[u] >= WEIGHT_AGG_BIN) hub_delta_add(s_hk, s_hv, u, dl, st.In);
           else atomicAdd(&st.In[u], dl);
         }
       }
@@ -2402,6 +2456,7 @@ __global__ __launch_bounds__(256) void k_weight_moved(DevGraph<T> g, DevState<T>
     x = sum_xor<tt, W>(x);
     if (lane == 0) st.In[v] = x;
   }
+  hub_delta_flush(s_hk, s_hv, st.In);
 }
 
 template <int NT, class T>
@@ -2410,6 +2465,9 @@ __global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevSta
                                                             const int32_t* __restrict__ bstart, int bin) {
   using tt = typename T::tot_t;
   __shared__ tt r[NT / 64];
+  __shared__ int s_hk[HD_SLOTS];
+  __shared__ tt s_hv[HD_SLOTS];
+  hub_delta_init(s_hk, s_hv);
   const int lo = bstart[bin], hi = bstart[bin + 1];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
@@ -2427,7 +2485,7 @@ __global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevSta
       if (cu == st.prev[u]) {
         const tt dl = (cu == c ? w : tt(0)) - (cu == p ? w : tt(0));
         if (dl) {
-          if (st.binid[u] >= WEIGHT_RESCAN_BIN) st.dirty[u] = 1;
+          if (st.binid[u] >= WEIGHT_AGG_BIN) hub_delta_add(s_hk, s_hv, u, dl, st.In);
           else atomicAdd(&st.In[u], dl);
         }
       }
@@ -2442,6 +2500,7 @@ __global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevSta
     }
     __syncthreads();
   }
+  hub_delta_flush(s_hk, s_hv, st.In);
 }
 
 template <class T>
