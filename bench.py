@@ -68,6 +68,35 @@ KERNEL_FIRST = [
     ["k_decide_first_block<1024, IntW>"]]
 
 
+# HBM bytes per launch of the decide kernels on this workload, from two
+# rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
+# dispatch, averaged over the kernel's dispatches), each scaled by what the
+# same counter reads for a plain 256 MiB device copy: HBM bytes = 2.00 x
+# FETCH_SIZE + 1.00 x WRITE_SIZE.  Commands and the full table:
+# profiles/decide_traffic_chung_lu_4M.md.  A counter run cannot be made from
+# inside the timed process, so the figures are recorded here and reported
+# beside the live timing of the same kernel.
+PMC_FETCH_FACTOR, PMC_WRITE_FACTOR = 2.00, 1.00
+PMC_KIB = {  # kernel: (FETCH_SIZE KiB, WRITE_SIZE KiB) per dispatch
+    "k_decide_hlead<256, 16, 8, false, IntW>": (185488.3, 11409.4),
+    "k_decide_hlead<1024, 64, 8, false, IntW>": (105252.9, 1523.7),
+    "k_decide_blead<256, 8192, 16, 1, IntW>": (123793.4, 491.1),
+    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50920.7, 70.8),
+    "k_decide_tpv<31, false, IntW>": (203664.7, 10283.5),
+    "k_decide_tpv<15, false, IntW>": (119634.9, 9916.0),
+    "k_decide_tpv<7, false, IntW>": (67802.9, 8329.0),
+    "k_decide_tpv<3, false, IntW>": (26011.5, 3247.9),
+}
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of a decide kernel (None if not measured)."""
+    if kernel not in PMC_KIB:
+        return None
+    f, w = PMC_KIB[kernel]
+    return 1024.0 * (PMC_FETCH_FACTOR * f + PMC_WRITE_FACTOR * w)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -338,7 +367,9 @@ def main():
                    "gamma": 2.1, "mean_degree": 17.35, "degree_cap": 15000, "seed": args.seed,
                    "weights": "unit (int32 CSR)", "mode": "plain"},
         "roofline": {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": pmc_traffic(dom["kernel"]),
+                     "traffic_source": "profiles/decide_traffic_chung_lu_4M.md (rocprofv3 --pmc FETCH_SIZE, "
+                                       "WRITE_SIZE; copy-scaled)" if pmc_traffic(dom["kernel"]) else None,
                      "kernel": dom["kernel"], "bin": dom["bin"], "sweep": dom["sweep"],
                      "launches": dom["launches"], "bytes_per_launch": dom["bytes"] / max(dom["launches"], 1),
                      "avg_launch_us": dom["avg_launch_us"],

@@ -135,7 +135,6 @@ void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
 // moved (cur != prev) over order[bstart[b] .. bstart[b+1]); a mover rewrites
 // its own In and adds the change of every shared entry to the In of each
 // neighbour that stayed.  Decide-and-move needs no dirty marks for it.
-constexpr int WEIGHT_AGG_BIN = 6;
 template <class T>
 void launch_weight_moved(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
                          const int32_t* bstart, int cap, hipStream_t s);

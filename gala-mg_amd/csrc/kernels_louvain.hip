@@ -2389,47 +2389,15 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
 // to In[u] changes by w * ([cur[v] == c_u] - [prev[v] == c_u]), which v adds
 // to In[u] (exact integer atomics).  A vertex none of whose neighbours moved
 // keeps its In.  Work per sweep: the movers' rows, against the rows of the
-// movers and all their neighbours for the dirty-row update.  Deltas for
-// neighbours of degree >= 512 (bins >= WEIGHT_AGG_BIN) are first summed in a
-// per-workgroup LDS table and added once per workgroup at its end: a hub with
-// many moving neighbours would otherwise take that many atomics on one
-// address.  (Integer sums: the result does not depend on the order.)
+// movers and all their neighbours for the dirty-row update.  (Summing the
+// deltas for hubs per workgroup in LDS first, or rescanning staying hubs
+// whole, both measured slower on the power-law workloads.)
 // ---------------------------------------------------------------------------
-constexpr int HD_SLOTS = 2048, HD_LG = 11, HD_PROBES = 8;
-
-template <class tt>
-__device__ __forceinline__ void hub_delta_add(int* s_hk, tt* s_hv, int u, tt dl, tt* In) {
-  unsigned p = hash_slot(u, HD_LG);
-  for (int probe = 0; probe < HD_PROBES; probe++) {
-    const int k = atomicCAS(&s_hk[p], 0, u + 1);
-    if (k == 0 || k == u + 1) { atomicAdd(&s_hv[p], dl); return; }
-    p = (p + 1) & (HD_SLOTS - 1);
-  }
-  atomicAdd(&In[u], dl);  // table crowded: straight to the target
-}
-
-template <class tt>
-__device__ __forceinline__ void hub_delta_init(int* s_hk, tt* s_hv) {
-  for (int t = threadIdx.x; t < HD_SLOTS; t += blockDim.x) { s_hk[t] = 0; s_hv[t] = 0; }
-  __syncthreads();
-}
-
-template <class tt>
-__device__ __forceinline__ void hub_delta_flush(const int* s_hk, const tt* s_hv, tt* In) {
-  __syncthreads();
-  for (int t = threadIdx.x; t < HD_SLOTS; t += blockDim.x) {
-    const int k = s_hk[t];
-    if (k && s_hv[t]) atomicAdd(&In[k - 1], s_hv[t]);
-  }
-}
 template <int W, class T>
 __global__ __launch_bounds__(256) void k_weight_moved(DevGraph<T> g, DevState<T> st,
                                                        const int32_t* __restrict__ order,
                                                        const int32_t* __restrict__ bstart, int bin) {
   using tt = typename T::tot_t;
-  __shared__ int s_hk[HD_SLOTS];
-  __shared__ tt s_hv[HD_SLOTS];
-  hub_delta_init(s_hk, s_hv);
   const int lo = bstart[bin], hi = bstart[bin + 1];
   const int lane = threadIdx.x & (W - 1);
   const int groups = gridDim.x * (blockDim.x / W);
@@ -2447,16 +2415,12 @@ __global__ __launch_bounds__(256) void k_weight_moved(DevGraph<T> g, DevState<T>
       if (cu == c) x += w;
       if (cu == st.prev[u]) {  // u stayed: its In gains or loses this entry
         const tt dl = (cu == c ? w : tt(0)) - (cu == p ? w : tt(0));
-        if (dl) {
-          if (st.binid[u] >= WEIGHT_AGG_BIN) hub_delta_add(s_hk, s_hv, u, dl, st.In);
-          else atomicAdd(&st.In[u], dl);
-        }
+        if (dl) atomicAdd(&st.In[u], dl);
       }
     }
     x = sum_xor<tt, W>(x);
     if (lane == 0) st.In[v] = x;
   }
-  hub_delta_flush(s_hk, s_hv, st.In);
 }
 
 template <int NT, class T>
@@ -2465,9 +2429,6 @@ __global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevSta
                                                             const int32_t* __restrict__ bstart, int bin) {
   using tt = typename T::tot_t;
   __shared__ tt r[NT / 64];
-  __shared__ int s_hk[HD_SLOTS];
-  __shared__ tt s_hv[HD_SLOTS];
-  hub_delta_init(s_hk, s_hv);
   const int lo = bstart[bin], hi = bstart[bin + 1];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
@@ -2484,10 +2445,7 @@ __global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevSta
       if (cu == c) x += w;
       if (cu == st.prev[u]) {
         const tt dl = (cu == c ? w : tt(0)) - (cu == p ? w : tt(0));
-        if (dl) {
-          if (st.binid[u] >= WEIGHT_AGG_BIN) hub_delta_add(s_hk, s_hv, u, dl, st.In);
-          else atomicAdd(&st.In[u], dl);
-        }
+        if (dl) atomicAdd(&st.In[u], dl);
       }
     }
     x = sum_xor<tt, 64>(x);
@@ -2500,7 +2458,6 @@ __global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevSta
     }
     __syncthreads();
   }
-  hub_delta_flush(s_hk, s_hv, st.In);
 }
 
 template <class T>
