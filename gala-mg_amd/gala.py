@@ -427,7 +427,10 @@ class Engine:
                         d["w"] = np.ctypeslib.as_array(wf, shape=(max(m, 1),))[:m].copy()
                 rounds.append(d)
 
-        self._icb, self._rcb = ITER_CB(icb), ROUND_CB(rcb)
+        # no callback unless the caller collects its output: a round callback
+        # makes the engine copy the round's community vector to the host
+        self._icb = ITER_CB(icb) if trace is not None else ITER_CB()
+        self._rcb = ROUND_CB(rcb) if rounds is not None else ROUND_CB()
         res = Result()
         _check(lib().gala_run(self.ctx, self._icb, self._rcb, None, ctypes.byref(res)), "gala_run")
         return res.to_dict()
