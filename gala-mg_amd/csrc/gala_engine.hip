@@ -432,26 +432,30 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   st.track = track;
   launch_init<T>(g, st, s);
   partition_bins(0, nullptr, ctx->binid, nullptr, g.n, ctx->order, ctx->bstart_full, ctx->part_tmp, s);
+  // hub-table sizes and their prefix sums before the one readback: the scan
+  // runs over n slots (entries past the hub count are zero)
+  const int gb = global_bin_min<T>();
+  const int64_t nn = std::max<int32_t>(g.n, 1);
+  fill<int64_t>(ctx->hub_tmp, nn, 0, s);
+  k_hub_sizes<<<std::min<int64_t>((nn + 255) / 256, 1024), 256, 0, s>>>(ctx->order, ctx->bstart_full, gb, g.off,
+                                                                         ctx->hub_tmp);
+  int64_t* pos = ctx->hub_tmp + nn + 1;
+  int64_t* tot = ctx->scan64_tmp;
+  scan_exclusive<int64_t, int64_t>(ctx->hub_tmp, pos, nn, ctx->c_i64_tmp, tot, s);
   int32_t* hb = (int32_t*)ctx->hpin;  // NBINS + 1 bin starts
+  int64_t* hs = (int64_t*)(ctx->hpin + 64);  // hub-table slots
   HIPOK(hipMemcpyAsync(hb, ctx->bstart_full, sizeof(int32_t) * (NBINS + 1), hipMemcpyDeviceToHost, s));
+  HIPOK(hipMemcpyAsync(hs, tot, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   HIPOK(hipStreamSynchronize(s));
   for (int b = 0; b < NBINS; b++) caps[b] = hb[b + 1] - hb[b];
   *nitems = hb[NBINS];
   launch_init_huge<T>(g, st, ctx->order, ctx->bstart_full, caps[7] + caps[8], s);
-  const int gb = global_bin_min<T>();
   const int nh = hb[NBINS] - hb[gb];
   if (nh > 0) {
-    int nb = std::min((nh + 255) / 256, 4096);
-    k_hub_sizes<<<nb, 256, 0, s>>>(ctx->order, ctx->bstart_full, gb, g.off, ctx->hub_tmp);
-    int64_t* pos = ctx->hub_tmp + nh + 1;
-    int64_t* tot = ctx->scan64_tmp;
-    scan_exclusive<int64_t, int64_t>(ctx->hub_tmp, pos, nh, ctx->c_i64_tmp, tot, s);
-    int64_t* hs = (int64_t*)(ctx->hpin + 64);
-    HIPOK(hipMemcpyAsync(hs, tot, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    HIPOK(hipStreamSynchronize(s));
     int rc = grow_hub(ctx, hs[0], T::FP);
     if (rc) return rc;
-    k_hub_scatter<<<nb, 256, 0, s>>>(ctx->order, ctx->bstart_full, gb, pos, ctx->hub_off);
+    k_hub_scatter<<<std::min((nh + 255) / 256, 4096), 256, 0, s>>>(ctx->order, ctx->bstart_full, gb, pos,
+                                                                   ctx->hub_off);
   }
   return GALA_OK;
 }

@@ -119,11 +119,18 @@ __global__ void k_dense_lengths(const int32_t* __restrict__ list, int32_t nlist,
 // empty rows nowhere.
 constexpr int TIER_T16 = 0, TIER_T32 = 1, TIER_T64 = 2, TIER_MID1 = 3, TIER_MID2 = 4, TIER_MID3 = 5;
 
-__global__ void k_classify(const int64_t* __restrict__ ecnt, int32_t C, long long lt,
-                           int32_t* __restrict__ big_list, int32_t* __restrict__ counts,
+// C is read on the device (rows c >= C get tier -1), so the host needs no
+// readback before the launch; n is an upper bound of C.
+__global__ void k_classify(const int64_t* __restrict__ ecnt, const int32_t* __restrict__ Cdev, int32_t n,
+                           long long lt, int32_t* __restrict__ big_list, int32_t* __restrict__ counts,
                            int64_t* __restrict__ big_tsz, int32_t* __restrict__ dense_list,
                            int8_t* __restrict__ tier) {
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+  const int32_t C = *Cdev;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    if (c >= C) {
+      tier[c] = -1;
+      continue;
+    }
     const long long L = ecnt[c];
     tier[c] = L == 0 ? -1 : L <= 16 ? TIER_T16 : L <= 32 ? TIER_T32 : L <= 64 ? TIER_T64
             : L <= 512 ? TIER_MID1 : L <= 2048 ? TIER_MID2 : L <= 4096 ? TIER_MID3 : -1;
@@ -174,6 +181,40 @@ __global__ __launch_bounds__(256) void k_stage_rows(DevGraph<T> g, const int32_t
       skey[base + j] = label[g.nbr[o + j]];
       sval[base + j] = g.w ? (acc)g.w[o + j] : acc(1);
     }
+  }
+}
+
+// rows of order[bstart[blo] .. bstart[bhi]) cut into STAGE_CH-entry chunks:
+// workgroup (x, y) stages chunks y, y + gridDim.y, ... of row x, one
+// reservation per chunk, so a hub row of 10^5 entries is spread over
+// gridDim.y workgroups instead of one
+constexpr int STAGE_CH = 4096;
+template <class T>
+__global__ __launch_bounds__(256) void k_stage_chunks(DevGraph<T> g, const int32_t* __restrict__ order,
+                                                      const int32_t* __restrict__ bstart, int blo, int bhi,
+                                                      const int32_t* __restrict__ label,
+                                                      const int64_t* __restrict__ eoff,
+                                                      unsigned long long* __restrict__ ecur,
+                                                      int32_t* __restrict__ skey,
+                                                      typename T::acc_t* __restrict__ sval) {
+  using acc = typename T::acc_t;
+  __shared__ int64_t s_b;
+  const int i = bstart[blo] + blockIdx.x;
+  if (i >= bstart[bhi]) return;  // uniform per workgroup
+  const int v = order[i];
+  const int64_t o = g.off[v];
+  const int64_t d = g.off[v + 1] - o;
+  const int nc = label[v];
+  for (int64_t c0 = (int64_t)blockIdx.y * STAGE_CH; c0 < d; c0 += (int64_t)gridDim.y * STAGE_CH) {
+    const int64_t len = d - c0 < STAGE_CH ? d - c0 : STAGE_CH;
+    if (threadIdx.x == 0) s_b = eoff[nc] + (int64_t)atomicAdd(&ecur[nc], (unsigned long long)len);
+    __syncthreads();
+    const int64_t base = s_b;
+    for (int64_t j = threadIdx.x; j < len; j += blockDim.x) {
+      skey[base + j] = label[g.nbr[o + c0 + j]];
+      sval[base + j] = g.w ? (acc)g.w[o + c0 + j] : acc(1);
+    }
+    __syncthreads();  // s_b is rewritten by the next chunk
   }
 }
 
@@ -792,33 +833,37 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   k_nonempty<<<nblk(n), 256, 0, s>>>(cb.cnt, n, flag);
   scan_exclusive<uint8_t, int32_t>(flag, cb.nid, n, cb.i32_tmp + (n + 3) / 4 + 1, cb.i32_scalar, s);
   k_label<<<nblk(n), 256, 0, s>>>(comm, cb.nid, n, cb.label);
-  int32_t* hp32 = (int32_t*)cb.hpin;
-  int64_t* hp64 = (int64_t*)cb.hpin;
-  if (hipMemcpyAsync(hp32, cb.i32_scalar, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-  if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  const int32_t C = hp32[0];
-  fill<int64_t>(cb.ecnt, C + 1, 0, s);
+  // Everything up to the one readback below is sized by n >= C, so C, the
+  // tier starts, the big/dense counts and the big rows' overflow slots come
+  // back with a single synchronisation.  Rows c >= C are empty: ecnt[c] = 0,
+  // eoff[c] = the total, tier -1, big_tsz 0.
+  fill<int64_t>(cb.ecnt, (int64_t)n + 1, 0, s);
   k_ecount_comm<<<nblk(n), 256, 0, s>>>(cb.cnt, cb.nid, dsum, n, cb.ecnt);
-  scan_exclusive<int64_t, int64_t>(cb.ecnt, cb.eoff, C, cb.i64_tmp, cb.eoff + C, s);
+  scan_exclusive<int64_t, int64_t>(cb.ecnt, cb.eoff, n, cb.i64_tmp, cb.eoff + n, s);
   fill<int32_t>(cb.nbig, 2, 0, s);  // big rows, dense rows
   int32_t* dense_list = cb.big_list + n;
-  k_classify<<<nblk(C), 256, 0, s>>>(cb.ecnt, C, LT, cb.big_list, cb.nbig, cb.i64_tmp, dense_list, cb.tier);
-  partition_bins(0, nullptr, cb.tier, nullptr, C, cb.tlist, cb.tstart, cb.part_tmp, s);
-  // hp32[0..1] = big and dense row counts, hp32[2..] = the tier starts
-  if (hipMemcpyAsync(hp32, cb.nbig, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-  if (hipMemcpyAsync(hp32 + 2, cb.tstart, sizeof plan->tstart, hipMemcpyDeviceToHost, s) != hipSuccess)
+  int64_t* big_tsz = cb.i64_tmp;  // free again after the ecnt scan
+  fill<int64_t>(big_tsz, n, 0, s);
+  k_classify<<<nblk(n), 256, 0, s>>>(cb.ecnt, cb.i32_scalar, n, LT, cb.big_list, cb.nbig, big_tsz, dense_list,
+                                     cb.tier);
+  partition_bins(0, nullptr, cb.tier, nullptr, n, cb.tlist, cb.tstart, cb.part_tmp, s);
+  scan_exclusive<int64_t, int64_t>(big_tsz, cb.big_tab, n, cb.i64_tmp + n + 1, cb.big_tab + n, s);
+  // pinned readback: [0] C, [1..2] big and dense row counts, [3..] tier starts,
+  // then (8-byte aligned) the overflow slots of the big rows
+  int32_t* hp32 = (int32_t*)cb.hpin;
+  int64_t* hp64 = (int64_t*)cb.hpin;
+  constexpr int SLOT64 = (3 + NBINS + 1 + 1) / 2;
+  if (hipMemcpyAsync(hp32, cb.i32_scalar, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  if (hipMemcpyAsync(hp32 + 1, cb.nbig, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  if (hipMemcpyAsync(hp32 + 3, cb.tstart, sizeof plan->tstart, hipMemcpyDeviceToHost, s) != hipSuccess)
+    return -1;
+  if (hipMemcpyAsync(hp64 + SLOT64, cb.big_tab + n, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess)
     return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  const int32_t nbig = hp32[0], ndense = hp32[1];
-  std::copy(hp32 + 2, hp32 + 2 + NBINS + 1, plan->tstart);
-  int64_t slots = 0;
-  if (nbig > 0) {
-    scan_exclusive<int64_t, int64_t>(cb.i64_tmp, cb.big_tab, nbig, cb.i64_tmp + nbig + 1, cb.big_tab + nbig, s);
-    if (hipMemcpyAsync(hp64, cb.big_tab + nbig, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess)
-      return -1;
-    if (hipStreamSynchronize(s) != hipSuccess) return -1;
-    slots = hp64[0];
-  }
+  const int32_t C = hp32[0];
+  const int32_t nbig = hp32[1], ndense = hp32[2];
+  std::copy(hp32 + 3, hp32 + 3 + NBINS + 1, plan->tstart);
+  const int64_t slots = nbig > 0 ? hp64[SLOT64] : 0;
   plan->C = C;
   plan->nbig = nbig;
   plan->slots = slots;
@@ -881,8 +926,9 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
                                                            cb.skey, cb.sval);
   k_stage_rows<64, T><<<nblk(cp[4] + cp[5] + cp[6], 4, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 4, 7, cb.label,
                                                                            cb.eoff, ecur, cb.skey, cb.sval);
-  k_stage_rows<256, T><<<nblk(cp[7] + cp[8], 1, 4096), 256, 0, s>>>(g, cb.order, cb.bstart, 7, NBINS, cb.label,
-                                                                    cb.eoff, ecur, cb.skey, cb.sval);
+  if (cp[7] + cp[8] > 0)
+    k_stage_chunks<T><<<dim3(cp[7] + cp[8], 16), 256, 0, s>>>(g, cb.order, cb.bstart, 7, NBINS, cb.label, cb.eoff,
+                                                              ecur, cb.skey, cb.sval);
   fill<int32_t>(cb.dcnt, C + 1, 0, s);
   const int canon = plan.canonical ? 1 : 0;
   const int32_t* ts = plan.tstart;
