@@ -218,9 +218,11 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   const size_t tsz = fp ? 8 : 4;  // tot_t / w_t / acc_t size
   for (int k = 0; k < 3; k++) {
     OWN(ctx->off[k], N + 1);
-    OWN(ctx->nbr[k], M);
+    // 16 entries of padding: row loads with 16-byte vectors may read up to 3
+    // entries past the end of the last row
+    OWN(ctx->nbr[k], M + CSR_PAD);
     char* p;
-    OWN(p, M * tsz);
+    OWN(p, (M + CSR_PAD) * tsz);
     ctx->w[k] = p;
   }
   { char* p;

@@ -34,6 +34,9 @@ struct DevGraph {
   const typename T::w_t* w;  // nullptr => every weight is 1
 };
 
+// entries of padding after every CSR neighbour / weight array (vector row loads)
+constexpr int64_t CSR_PAD = 16;
+
 // split hub decide (bin 8): row chunk of one aggregation workgroup, table
 // chunk of one scan workgroup, and the scan's per-chunk partial result
 constexpr int HUB_CH = 4096, HUB_TCH = 4096;

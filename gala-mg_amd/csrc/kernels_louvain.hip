@@ -541,6 +541,34 @@ __global__ __launch_bounds__(256) void k_decide_pipe(DevGraph<T> g, DevState<T> 
 // index sums the weights of the equal communities after it and is evaluated.
 // The view's sign bit carries the singleton flag the guard needs.
 // ---------------------------------------------------------------------------
+// A row of up to DMAX int32 entries read by one lane with 16-byte loads from
+// the aligned address at or below its start (the CSR arrays carry 16 entries
+// of padding, so the last vector may run past the final row).  Four entries
+// per load instead of one: a wave's row loads touch the same lines 4x fewer
+// times, which keeps them from being evicted and fetched again while the
+// lane's later entries are still outstanding.
+template <int DMAX>
+__device__ __forceinline__ void load_row_vec(const int32_t* __restrict__ p, int64_t o, int d, int (&out)[DMAX]) {
+  constexpr int NV = (DMAX + 3 + 3) / 4;
+  const int mis = (int)(o & 3);
+  const int4* __restrict__ q = reinterpret_cast<const int4*>(p + (o - mis));
+  const int nv = (mis + d + 3) >> 2;
+  int e[NV * 4];
+#pragma unroll
+  for (int j = 0; j < NV; j++) {
+    const int4 x = j < nv ? q[j] : make_int4(0, 0, 0, 0);
+    e[4 * j] = x.x;
+    e[4 * j + 1] = x.y;
+    e[4 * j + 2] = x.z;
+    e[4 * j + 3] = x.w;
+  }
+#pragma unroll
+  for (int k = 0; k < DMAX; k++) {
+    const int val = mis == 0 ? e[k] : mis == 1 ? e[k + 1] : mis == 2 ? e[k + 2] : e[k + 3];
+    out[k] = k < d ? val : 0;
+  }
+}
+
 template <int DMAX, bool FIRST, class T>
 __global__ __launch_bounds__(256) void k_decide_tpv(DevGraph<T> g, DevState<T> st,
                                                      const int32_t* __restrict__ front,
@@ -567,10 +595,12 @@ __global__ __launch_bounds__(256) void k_decide_tpv(DevGraph<T> g, DevState<T> s
     int cs[DMAX];
     int32_t ts[DMAX];  // Tot with the singleton flag in the sign bit
     acc ws[DMAX];
+    load_row_vec<DMAX>(g.nbr, o, d, cs);
+    if (g.w) {
+      load_row_vec<DMAX>((const int32_t*)g.w, o, d, ws);
+    } else {
 #pragma unroll
-    for (int k = 0; k < DMAX; k++) {
-      cs[k] = k < d ? g.nbr[o + k] : 0;
-      ws[k] = k < d ? (acc)load_w<T>(g.w, o + k) : acc(0);
+      for (int k = 0; k < DMAX; k++) ws[k] = k < d ? acc(1) : acc(0);
     }
 #pragma unroll
     for (int k = 0; k < DMAX; k++) {
