@@ -619,7 +619,8 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     }
     const bool log_it = logging && !lsweep && it >= 2;
     if (log_it) launch_features<T>(g, st, fcur, nfront, it, ctx->feat, s);
-    if ((rc = decide(st, lsweep ? ctx->front2 : fcur, lsweep ? ctx->bstart2 : bcur, first, true))) return rc;
+    // the first-sweep kernels read Tot directly: no community view needed
+    if ((rc = decide(st, lsweep ? ctx->front2 : fcur, lsweep ? ctx->bstart2 : bcur, first, !first))) return rc;
     if (log_it) {
       launch_labels(fcur, nfront, ctx->moved_it, it, ctx->lab, s);
       hfeat.resize((size_t)nfront * MLP_IN);

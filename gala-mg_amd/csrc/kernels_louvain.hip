@@ -161,6 +161,21 @@ __device__ __forceinline__ typename T::tot_t vertex_m2(const DevState<T>& st, in
   return st.gid ? st.m2g[st.gid[v]] : m2_all;
 }
 
+// Community view of u.  In the first sweep of a round every vertex is a
+// singleton with Tot = K, so the view is built from one 4-byte Tot gather
+// (8 bytes on the FP path) instead of reading the full view, which is then
+// not built for that sweep.
+template <class T, bool FIRST>
+__device__ __forceinline__ typename T::cv_t view_of(const DevState<T>& st, int u) {
+  if constexpr (FIRST) {
+    typename T::cv_t e;
+    cv_set(e, u, 1, st.Tot[u]);
+    return e;
+  } else {
+    return st.cv[u];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // init: K, Self, Tot=K, size=1, prev=cur=v, bin id, active=1
 // (init_communities, kernel_functions.cu:34-73)
@@ -494,7 +509,7 @@ __global__ __launch_bounds__(256) void k_decide_pipe(DevGraph<T> g, DevState<T> 
     okD = okC;
     if (okC) {
       vD = vC; dD = dC; uD = uC; evD = evC; KD = KC; SD = SC; wD = wC;
-      if (lane < dC) euD = st.cv[uC];
+      if (lane < dC) euD = view_of<T, FIRST>(st, uC);
     }
     okC = okB;
     if (okB) {
@@ -509,7 +524,7 @@ __global__ __launch_bounds__(256) void k_decide_pipe(DevGraph<T> g, DevState<T> 
       vB = vA;
       oB = g.off[vA];
       dB = (int)(g.off[vA + 1] - oB);
-      evB = st.cv[vA];
+      evB = view_of<T, FIRST>(st, vA);
       KB = st.K[vA];
       SB = st.Self[vA];
       iA += stride;
@@ -589,7 +604,7 @@ __global__ __launch_bounds__(256) void k_decide_tpv(DevGraph<T> g, DevState<T> s
     const tt m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.off[v + 1] - o);
-    const CV ev = st.cv[v];
+    const CV ev = view_of<T, FIRST>(st, v);
     const int c = ev.c;
     const tt K = st.K[v], S = st.Self[v], Tc = cv_tot(ev);
     int cs[DMAX];
@@ -605,7 +620,7 @@ __global__ __launch_bounds__(256) void k_decide_tpv(DevGraph<T> g, DevState<T> s
 #pragma unroll
     for (int k = 0; k < DMAX; k++) {
       CV e{};
-      if (k < d) e = st.cv[cs[k]];
+      if (k < d) e = view_of<T, FIRST>(st, cs[k]);
       cs[k] = k < d ? e.c : -1;
       ts[k] = e.ts;
     }
@@ -978,7 +993,7 @@ __global__ __launch_bounds__(256) void k_decide_hlead(DevGraph<T> g, DevState<T>
       for (int k = 0; k < EPL; k++) {
         uD[k] = uC[k];
         wD[k] = wC[k];
-        if (lane + k * W < dC) euD[k] = st.cv[uC[k]];
+        if (lane + k * W < dC) euD[k] = view_of<T, FIRST>(st, uC[k]);
       }
     }
     okC = okB;
@@ -997,7 +1012,7 @@ __global__ __launch_bounds__(256) void k_decide_hlead(DevGraph<T> g, DevState<T>
       vB = vA;
       oB = g.off[vA];
       dB = (int)(g.off[vA + 1] - oB);
-      evB = st.cv[vA];
+      evB = view_of<T, FIRST>(st, vA);
       KB = st.K[vA];
       SB = st.Self[vA];
       iA += stride;
