@@ -61,6 +61,9 @@ struct gala_ctx {
   // buffers the contraction ping-pongs between.  Weights are int32 (int path)
   // or double (FP path); the input may have implicit unit weights.
   int64_t* off[3] = {nullptr, nullptr, nullptr};
+  // row ends of the work graphs when they are gapped (contracted inside a run)
+  int64_t* end[3] = {nullptr, nullptr, nullptr};
+  bool gapped[3] = {false, false, false};
   int32_t* nbr[3] = {nullptr, nullptr, nullptr};
   void* w[3] = {nullptr, nullptr, nullptr};
   bool unit0 = true;
@@ -218,6 +221,8 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   const size_t tsz = fp ? 8 : 4;  // tot_t / w_t / acc_t size
   for (int k = 0; k < 3; k++) {
     OWN(ctx->off[k], N + 1);
+    OWN(ctx->end[k], N);
+    ctx->gapped[k] = false;
     // 16 entries of padding: row loads with 16-byte vectors may read up to 3
     // entries past the end of the last row
     OWN(ctx->nbr[k], M + CSR_PAD);
@@ -335,11 +340,12 @@ int grow_dense(gala_ctx* ctx, const ContractPlan& plan, bool fp) {
 // per-vertex hub-table offsets for the bins that use the global arena
 // --------------------------------------------------------------------------
 __global__ void k_hub_sizes(const int32_t* __restrict__ order, const int32_t* __restrict__ bstart,
-                            int gbin, const int64_t* __restrict__ off, int64_t* __restrict__ tsz) {
+                            int gbin, const int64_t* __restrict__ off, const int64_t* __restrict__ end,
+                            int64_t* __restrict__ tsz) {
   const int lo = bstart[gbin], hi = bstart[NBINS];
   for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
     int v = order[i];
-    int64_t d = off[v + 1] - off[v];
+    int64_t d = end[v] - off[v];
     int64_t t = 1;
     while (t < 2 * d) t <<= 1;
     tsz[i - lo] = t;
@@ -391,6 +397,7 @@ DevGraph<T> graph_at(gala_ctx* ctx, int k, int32_t n, int64_t m) {
   g.n = n;
   g.m = m;
   g.off = ctx->off[k];
+  g.end = ctx->gapped[k] ? ctx->end[k] : ctx->off[k] + 1;
   g.nbr = ctx->nbr[k];
   g.w = (k == 0 && ctx->unit0) ? nullptr : (const typename T::w_t*)ctx->w[k];
   return g;
@@ -440,7 +447,7 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   const int64_t nn = std::max<int32_t>(g.n, 1);
   fill<int64_t>(ctx->hub_tmp, nn, 0, s);
   k_hub_sizes<<<std::min<int64_t>((nn + 255) / 256, 1024), 256, 0, s>>>(ctx->order, ctx->bstart_full, gb, g.off,
-                                                                         ctx->hub_tmp);
+                                                                         g.end, ctx->hub_tmp);
   int64_t* pos = ctx->hub_tmp + nn + 1;
   int64_t* tot = ctx->scan64_tmp;
   scan_exclusive<int64_t, int64_t>(ctx->hub_tmp, pos, nn, ctx->c_i64_tmp, tot, s);
@@ -712,6 +719,16 @@ int contract(gala_ctx* ctx, const DevGraph<T>& g, const int32_t* comm, int dst, 
              int32_t* C_out, int64_t* m_out, bool canonical) {
   hipStream_t s = ctx->stream;
   ContractBufs<T> cb = cbufs<T>(ctx);
+  // non-canonical: stage straight into the output graph's arrays and keep
+  // the merged rows there (gapped CSR); canonical: stage into the scratch
+  // arrays and compact into a sorted, gap-free CSR
+  auto gap_bufs = [&](ContractBufs<T>& b) {
+    if (canonical) return;
+    b.skey = ctx->nbr[dst];
+    b.sval = (typename T::acc_t*)ctx->w[dst];
+    b.eoff = ctx->off[dst];
+  };
+  gap_bufs(cb);
   ContractPlan plan;
   plan.canonical = canonical;
   for (int b = 0; b < NBINS; b++) plan.caps[b] = caps[b];
@@ -725,9 +742,12 @@ int contract(gala_ctx* ctx, const DevGraph<T>& g, const int32_t* comm, int dst, 
   out.w = (typename T::w_t*)ctx->w[dst];
   int64_t m = 0;
   cb = cbufs<T>(ctx);
+  gap_bufs(cb);
+  if (!canonical) out.end = ctx->end[dst];
   if (contract_build<T>(g, cb, plan, out, &m, s)) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
+  ctx->gapped[dst] = !canonical;
   *C_out = plan.C;
-  *m_out = m;
+  *m_out = canonical ? m : g.m;  // gapped: the extent of the staged arrays
   return GALA_OK;
 }
 

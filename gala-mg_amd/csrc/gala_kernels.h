@@ -28,8 +28,12 @@ void reset_counters(Counters* c, long long min_init, double min_init_f, hipStrea
 template <class T>
 struct DevGraph {
   int32_t n;
-  int64_t m;  // directed entries
+  int64_t m;  // directed entries (compact CSR) or the extent of the arrays (gapped)
+  // row v is nbr[off[v] .. end[v]); a compact CSR has end = off + 1, a graph
+  // contracted inside a run keeps each merged row at the start of the row's
+  // staging segment, with a gap up to the next row's off
   const int64_t* off;
+  const int64_t* end;
   const int32_t* nbr;
   const typename T::w_t* w;  // nullptr => every weight is 1
 };
@@ -249,10 +253,15 @@ struct ContractOut {
   int64_t* off;  // [C+1]
   int32_t* nbr;
   typename T::w_t* w;
+  // non-null: gapped output.  The merged rows stay where they were staged (the
+  // caller pointed the staging arrays and the segment offsets at the output
+  // graph's nbr / w / off), end[c] = off[c] + distinct count, no compaction.
+  int64_t* end = nullptr;
 };
 
 // Renumber (nid, label), per-community entry counts and offsets, long-row
-// classification and their hash-table offsets.  Fills *plan (two syncs).
+// classification and their hash-table offsets.  Fills *plan (one sync, two
+// with dense rows).
 template <class T>
 int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& cb, ContractPlan* plan,
                      hipStream_t s);

@@ -48,7 +48,7 @@ __host__ __device__ __forceinline__ int clog2(long long x) {
 // one per member (per-member atomics on a few hot addresses serialise).
 constexpr int CT = 1024, CTS = 2 * CT;
 __global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ comm, const int64_t* __restrict__ off,
-                                                   int32_t n, int32_t* __restrict__ cnt,
+                                                   const int64_t* __restrict__ end, int32_t n, int32_t* __restrict__ cnt,
                                                    unsigned long long* __restrict__ dsum) {
   __shared__ int s_key[CTS];
   __shared__ int s_cnt[CTS];
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ c
       const int64_t v = base + j;
       if (v >= n) break;
       const int c = comm[v];
-      const unsigned long long d = (unsigned long long)(off[v + 1] - off[v]);
+      const unsigned long long d = (unsigned long long)(end[v] - off[v]);
       unsigned p = ((unsigned)c * 2654435769u) >> (32 - LG);
       for (int probe = 0; probe < CTS; probe++) {
         const int k = atomicCAS(&s_key[p], 0, c + 1);
@@ -86,6 +86,12 @@ __global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ c
     }
     __syncthreads();
   }
+}
+
+// row ends of a gapped contracted graph: end[c] = off[c] + distinct count
+__global__ void k_row_ends(const int64_t* __restrict__ off, const int32_t* __restrict__ dcnt, int32_t C,
+                           int64_t* __restrict__ end) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) end[c] = off[c] + dcnt[c];
 }
 
 // entries of every new community: ecnt[nid[c]] = row-length sum of c
@@ -165,7 +171,7 @@ __global__ __launch_bounds__(256) void k_stage_rows(DevGraph<T> g, const int32_t
   for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / NT; i < hi; i += gridDim.x * G) {
     const int v = order[i];
     const int64_t o = g.off[v];
-    const int64_t d = g.off[v + 1] - o;
+    const int64_t d = g.end[v] - o;
     const int nc = label[v];
     int64_t base = 0;
     if (lane == 0) base = eoff[nc] + (int64_t)atomicAdd(&ecur[nc], (unsigned long long)d);
@@ -203,7 +209,7 @@ __global__ __launch_bounds__(256) void k_stage_chunks(DevGraph<T> g, const int32
   if (i >= bstart[bhi]) return;  // uniform per workgroup
   const int v = order[i];
   const int64_t o = g.off[v];
-  const int64_t d = g.off[v + 1] - o;
+  const int64_t d = g.end[v] - o;
   const int nc = label[v];
   for (int64_t c0 = (int64_t)blockIdx.y * STAGE_CH; c0 < d; c0 += (int64_t)gridDim.y * STAGE_CH) {
     const int64_t len = d - c0 < STAGE_CH ? d - c0 : STAGE_CH;
@@ -828,7 +834,7 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   // the staging cursors are free until contract_build: row-length sums by community
   unsigned long long* dsum = (unsigned long long*)cb.ecur;
   fill<int64_t>(cb.ecur, n, 0, s);
-  k_comm_sums<<<std::max(1, std::min((n + CT - 1) / CT, 2048)), 256, 0, s>>>(comm, g.off, n, cb.cnt, dsum);
+  k_comm_sums<<<std::max(1, std::min((n + CT - 1) / CT, 2048)), 256, 0, s>>>(comm, g.off, g.end, n, cb.cnt, dsum);
   uint8_t* flag = (uint8_t*)cb.i32_tmp;  // n bytes fit in the n-int scratch
   k_nonempty<<<nblk(n), 256, 0, s>>>(cb.cnt, n, flag);
   scan_exclusive<uint8_t, int32_t>(flag, cb.nid, n, cb.i32_tmp + (n + 3) / 4 + 1, cb.i32_scalar, s);
@@ -1020,6 +1026,10 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     k_rs_emit<T><<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kf, heads, hs, cb.eoff, cb.skey, cb.sval, cb.dcnt);
     k_rs_sum<T><<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, vf, heads, hs, cb.eoff, cb.sval);
     if (hipStreamSynchronize(s) != hipSuccess) return -1;  // the host tile vector is in flight until here
+  }
+  if (out.end) {  // gapped: every row stays at the start of its staging segment
+    k_row_ends<<<nblk(C), 256, 0, s>>>(cb.eoff, cb.dcnt, C, out.end);
+    return 0;
   }
   scan_exclusive<int32_t, int64_t>(cb.dcnt, out.off, C, cb.i64_tmp, out.off + C, s);
   const int32_t nsmall = ts[TIER_MID3 + 1] - ts[TIER_T16];

@@ -35,7 +35,7 @@ __device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __exp
 template <class T>
 __device__ __forceinline__ void vertex_features(const DevGraph<T>& g, const DevState<T>& st, int v, int it,
                                                 float f[MLP_IN]) {
-  const int64_t d = g.off[v + 1] - g.off[v];
+  const int64_t d = g.end[v] - g.off[v];
   const double K = (double)st.K[v];
   f[0] = __log2f(1.0f + (float)d) * (1.0f / 16.0f);
   f[1] = K > 0 ? (float)((double)st.In[v] / K) : 0.0f;

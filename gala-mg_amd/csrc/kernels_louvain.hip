@@ -184,7 +184,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> st) {
   using tt = typename T::tot_t;
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += gridDim.x * blockDim.x) {
-    int64_t o = g.off[v], e = g.off[v + 1];
+    int64_t o = g.off[v], e = g.end[v];
     int64_t d = e - o;
     st.binid[v] = (int8_t)bin_of_degree(d);
     st.size[v] = 1;
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_init_wide(DevGraph<T> g, DevState<T> st
   const int lane = threadIdx.x & (W - 1);
   const int ng = gridDim.x * (blockDim.x / W);
   for (int v = (blockIdx.x * blockDim.x + threadIdx.x) / W; v < g.n; v += ng) {  // uniform per group
-    int64_t o = g.off[v], e = g.off[v + 1];
+    int64_t o = g.off[v], e = g.end[v];
     if (e - o <= LO || e - o > HI) continue;
     tt k = 0, self = 0;
     for (int64_t j = o + lane; j < e; j += W) {
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(1024) void k_init_huge(DevGraph<T> g, DevState<T> s
   const int lo = bstart[7], hi = bstart[NBINS];
   for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
     const int v = order[i];
-    const int64_t o = g.off[v], e = g.off[v + 1];
+    const int64_t o = g.off[v], e = g.end[v];
     tt k = 0, self = 0;
     for (int64_t j0 = o + threadIdx.x; j0 < e; j0 += (int64_t)NT * U) {
       int u[U];
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
     const int v = front[i];
     const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     const typename T::cv_t ev = st.cv[v];
     const int c = ev.c;
     int cl = -1, ul = 0;
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(256) void k_decide_pipe(DevGraph<T> g, DevState<T> 
     if (okA) {
       vB = vA;
       oB = g.off[vA];
-      dB = (int)(g.off[vA + 1] - oB);
+      dB = (int)(g.end[vA] - oB);
       evB = view_of<T, FIRST>(st, vA);
       KB = st.K[vA];
       SB = st.Self[vA];
@@ -603,7 +603,7 @@ __global__ __launch_bounds__(256) void k_decide_tpv(DevGraph<T> g, DevState<T> s
     const int v = front[i];
     const tt m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     const CV ev = view_of<T, FIRST>(st, v);
     const int c = ev.c;
     const tt K = st.K[v], S = st.Self[v], Tc = cv_tot(ev);
@@ -818,7 +818,7 @@ __global__ __launch_bounds__(256) void k_decide_hpipe(DevGraph<T> g, DevState<T>
     if (okA) {
       vB = vA;
       oB = g.off[vA];
-      dB = (int)(g.off[vA + 1] - oB);
+      dB = (int)(g.end[vA] - oB);
       evB = st.cv[vA];
       KB = st.K[vA];
       SB = st.Self[vA];
@@ -1011,7 +1011,7 @@ __global__ __launch_bounds__(256) void k_decide_hlead(DevGraph<T> g, DevState<T>
     if (okA) {
       vB = vA;
       oB = g.off[vA];
-      dB = (int)(g.off[vA + 1] - oB);
+      dB = (int)(g.end[vA] - oB);
       evB = view_of<T, FIRST>(st, vA);
       KB = st.K[vA];
       SB = st.Self[vA];
@@ -1070,7 +1070,7 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
     const int v = front[i];
     const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     const typename T::cv_t ev = st.cv[v];
     const int c = ev.c;
     int lg = ilog2_ceil(2 * d);
@@ -1182,7 +1182,7 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
     const int v = front[i];
     const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     const int c = st.prev[v];
     int lg = ilog2_ceil(2 * d);
     const int tsz = 1 << lg;
@@ -1311,7 +1311,7 @@ __global__ __launch_bounds__(NT) void k_decide_blead(DevGraph<T> g, DevState<T> 
     const int v = front[i];
     const tt m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);  // <= NT * EPL by bin construction
+    const int d = (int)(g.end[v] - o);  // <= NT * EPL by bin construction
     if (split_above > 0 && d > split_above) continue;  // uniform per workgroup
     const CV ev = st.cv[v];
     const int c = ev.c;
@@ -1442,7 +1442,7 @@ __global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> 
     const int v = front[i];
     const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     if (split_above > 0 && d > split_above) continue;  // uniform per workgroup
     const int c = st.prev[v];
     const int glg = ilog2_ceil(2 * d);
@@ -1599,7 +1599,7 @@ __global__ __launch_bounds__(1024) void k_hub_plan(DevGraph<T> g, const int32_t*
     int nc = 0, nt = 0;
     if (i < nh) {
       const int v = front[lo + i];
-      const int64_t d = g.off[v + 1] - g.off[v];
+      const int64_t d = g.end[v] - g.off[v];
       if (d > split_above) {
       int64_t tsz = 1;
       while (tsz < 2 * d) tsz <<= 1;
@@ -1635,7 +1635,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(DevGraph<T> g, DevState<T> st, 
     const int i = hub_find(cstart, nh, q);
     const int v = front[lo + i];
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     const int j0 = (q - cstart[i]) * HUB_CH;
     const int j1 = min(d, j0 + HUB_CH);
     // PER entries per thread: all row loads, then all community gathers, in flight
@@ -1706,7 +1706,7 @@ __global__ __launch_bounds__(256) void k_hub_scan(DevGraph<T> g, DevState<T> st,
     const int i = hub_find(tstart, nh, q);
     const int v = front[lo + i];
     const tt m2 = vertex_m2<T>(st, v, m2_all);
-    const int d = (int)(g.off[v + 1] - g.off[v]);
+    const int d = (int)(g.end[v] - g.off[v]);
     const int gsz = 1 << ilog2_ceil(2 * d);
     const int s0 = (q - tstart[i]) * HUB_TCH;
     const int s1 = min(gsz, s0 + HUB_TCH);
@@ -1767,7 +1767,7 @@ __global__ __launch_bounds__(256) void k_hub_commit(DevGraph<T> g, DevState<T> s
     if (tstart[i] == tstart[i + 1]) continue;  // left to k_decide_hub (uniform per workgroup)
     const int v = front[lo + i];
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     if (threadIdx.x == 0) {
       G h = 0;
       int bd = INT_MAX;
@@ -1819,7 +1819,7 @@ __global__ __launch_bounds__(256) void k_decide_first(DevGraph<T> g, DevState<T>
     const int v = front[i];
     const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     const tt K = st.K[v];
     const tt Tc = st.Tot[v];
     G h = 0;
@@ -1875,7 +1875,7 @@ __global__ __launch_bounds__(NT) void k_decide_first_block(DevGraph<T> g, DevSta
     const int v = front[i];
     const typename T::tot_t m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     const tt K = st.K[v];
     const tt Tc = st.Tot[v];
     G h = 0;
@@ -2239,7 +2239,7 @@ __global__ __launch_bounds__(256) void k_weight_group(DevGraph<T> g, DevState<T>
     const int v = order[i];
     if (only_dirty && !st.dirty[v]) continue;  // uniform per group
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     const int c = st.cur[v];
     tt x = 0;
     int mvn = 0;
@@ -2269,7 +2269,7 @@ __global__ __launch_bounds__(256) void k_weight_tpv(DevGraph<T> g, DevState<T> s
     const int v = order[i];
     if (only_dirty && !st.dirty[v]) continue;
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     const int c = st.cur[v];
     int us[DMAX];
     tt ws[DMAX];
@@ -2375,7 +2375,7 @@ __global__ __launch_bounds__(256) void k_weight_pipe(DevGraph<T> g, DevState<T> 
     if (okC) {
       vC = vB;
       oC = g.off[vB];
-      dC = (int)(g.off[vB + 1] - oC);
+      dC = (int)(g.end[vB] - oC);
       cC = st.cur[vB];
     }
     okB = okA;
@@ -2403,7 +2403,7 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
     const int v = order[i];
     if (only_dirty && !st.dirty[v]) continue;  // uniform per block
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     const int c = st.cur[v];
     tt x = 0;
     int mvn = 0;
@@ -2451,7 +2451,7 @@ __global__ __launch_bounds__(256) void k_weight_moved(DevGraph<T> g, DevState<T>
     const int c = st.cur[v], p = st.prev[v];
     if (c == p) continue;  // uniform per group
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     tt x = 0;
     for (int j = lane; j < d; j += W) {
       const int u = g.nbr[o + j];
@@ -2481,7 +2481,7 @@ __global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevSta
     const int c = st.cur[v], p = st.prev[v];
     if (c == p) continue;  // uniform per block
     const int64_t o = g.off[v];
-    const int d = (int)(g.off[v + 1] - o);
+    const int d = (int)(g.end[v] - o);
     tt x = 0;
     for (int j = threadIdx.x; j < d; j += NT) {
       const int u = g.nbr[o + j];
