@@ -78,6 +78,7 @@ struct gala_ctx {
   int8_t* binid = nullptr;
   int32_t *order = nullptr, *front = nullptr, *fin = nullptr;
   void* cv = nullptr;  // community view, 8 (int) or 16 (FP) bytes per vertex
+  int2* cp = nullptr;  // {cur, prev} per vertex (integer path, incremental In)
   BlockPart* part = nullptr;  // workgroup partials of the O(V) reductions
   int32_t *bstart_full = nullptr, *bstart_front = nullptr;
   int64_t* hub_off = nullptr;
@@ -237,6 +238,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->active, N); OWN(ctx->dirty, N); OWN(ctx->binid, N);
   OWN(ctx->order, N); OWN(ctx->front, N); OWN(ctx->fin, N);
   { char* p; OWN(p, N * 16); ctx->cv = p; }
+  OWN(ctx->cp, N);
   OWN(ctx->last_gain, N); OWN(ctx->moved_it, N); OWN(ctx->nbr_moved, N);
   OWN(ctx->front2, N); OWN(ctx->bstart2, NBINS + 1); OWN(ctx->keep, N);
   OWN(ctx->front3, N); OWN(ctx->bstart3, NBINS + 1); OWN(ctx->size_snap, N);
@@ -422,6 +424,7 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.nbr_moved = ctx->nbr_moved;
   st.cv = (typename T::cv_t*)ctx->cv;
   st.part = ctx->part;
+  st.cp = T::FP ? nullptr : ctx->cp;
   return st;
 }
 
@@ -677,7 +680,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     }
     // plain rule, with revert (louvain_main_process.cu:1004-1007,1026-1031)
     if (q - qprev < ctx->opts.min_modularity || it >= max_it) break;
-    copy_i32(ctx->cur, ctx->prev, g.n, s);  // prev = cur (louvain_main_process.cu:1029)
+    accept_sweep(ctx->cur, ctx->prev, st.cp, g.n, s);  // prev = cur (louvain_main_process.cu:1029)
     qprev = q;
   }
   ro->q = qprev;

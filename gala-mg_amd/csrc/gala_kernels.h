@@ -21,6 +21,8 @@ void partition_bins(int mode, const int32_t* order, const int8_t* binid, const u
 template <class V>
 void fill(V* p, int64_t n, V x, hipStream_t s);
 void copy_i32(const int32_t* a, int32_t* b, int64_t n, hipStream_t s);
+// prev = cur after an accepted sweep; cp (if not null) = {cur, cur}
+void accept_sweep(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, hipStream_t s);
 void reset_counters(Counters* c, long long min_init, double min_init_f, hipStream_t s);
 
 // ---- Louvain phase kernels (kernels_louvain.hip) ----
@@ -83,6 +85,10 @@ struct DevState {
   // pipelined kernels read it
   typename T::cv_t* cv = nullptr;
   BlockPart* part = nullptr;  // MAX_PARTS per-workgroup partials of k_apply / k_prune
+  // {cur, prev} of every vertex for the sweep in progress (null: not kept):
+  // the commit of a swept vertex writes it, accept_sweep resets it to
+  // {cur, cur}, so the incremental In update reads both with one gather
+  int2* cp = nullptr;
 };
 
 // Per-graph state of a batch (one entry per graph; integer path).

@@ -227,6 +227,22 @@ __global__ void k_copy_i32(const int32_t* __restrict__ a, int32_t* __restrict__ 
        i += (int64_t)gridDim.x * blockDim.x)
     b[i] = a[i];
 }
+__global__ void k_accept(const int32_t* __restrict__ cur, int32_t* __restrict__ prev, int2* __restrict__ cp,
+                         int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = cur[i];
+    prev[i] = c;
+    cp[i] = make_int2(c, c);
+  }
+}
+void accept_sweep(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  if (!cp) { copy_i32(cur, prev, n, s); return; }
+  int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  k_accept<<<nb, 256, 0, s>>>(cur, prev, cp, n);
+}
+
 void copy_i32(const int32_t* a, int32_t* b, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);

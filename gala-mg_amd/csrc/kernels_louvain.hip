@@ -98,6 +98,7 @@ __device__ __forceinline__ bool commit_move(int v, int c, typename T::gain_t bes
     if (gain > 0 && !(size[bestD] == 1 && size[c] == 1 && bestD > c)) nc = bestD;
   }
   st.cur[v] = nc;
+  if (st.cp) st.cp[v] = make_int2(nc, c);
   if (st.track) {
     st.last_gain[v] = (float)((double)gain / ((double)m2 * (double)K));
     if (nc != c) st.moved_it[v] = st.iter;
@@ -128,6 +129,7 @@ __device__ __forceinline__ bool commit_move_sz(int v, int c, typename T::gain_t 
     if (gain > 0 && !(szD == 1 && szC == 1 && bestD > c)) nc = bestD;
   }
   st.cur[v] = nc;
+  if (st.cp) st.cp[v] = make_int2(nc, c);
   if (st.track) {
     st.last_gain[v] = (float)((double)gain / ((double)m2 * (double)K));
     if (nc != c) st.moved_it[v] = st.iter;
@@ -190,6 +192,7 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
     st.size[v] = 1;
     st.prev[v] = v;
     st.cur[v] = v;
+    if (st.cp) st.cp[v] = make_int2(v, v);
     st.active[v] = 1;
     st.dTot[v] = 0;
     st.dsize[v] = 0;
@@ -2429,7 +2432,7 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
 
 // ---------------------------------------------------------------------------
 // Incremental weight update (integer path): only the rows of the vertices that
-// moved in the sweep (cur != prev) are read.  A mover v recomputes In[v] over
+// moved in the sweep (cur != prev, read as one {cur, prev} pair) are read.  A mover v recomputes In[v] over
 // its row; for every neighbour u that did not move, the entry's contribution
 // to In[u] changes by w * ([cur[v] == c_u] - [prev[v] == c_u]), which v adds
 // to In[u] (exact integer atomics).  A vertex none of whose neighbours moved
@@ -2448,7 +2451,8 @@ __global__ __launch_bounds__(256) void k_weight_moved(DevGraph<T> g, DevState<T>
   const int groups = gridDim.x * (blockDim.x / W);
   for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W; i < hi; i += groups) {
     const int v = order[i];
-    const int c = st.cur[v], p = st.prev[v];
+    const int2 cpv = st.cp[v];
+    const int c = cpv.x, p = cpv.y;
     if (c == p) continue;  // uniform per group
     const int64_t o = g.off[v];
     const int d = (int)(g.end[v] - o);
@@ -2456,9 +2460,10 @@ __global__ __launch_bounds__(256) void k_weight_moved(DevGraph<T> g, DevState<T>
     for (int j = lane; j < d; j += W) {
       const int u = g.nbr[o + j];
       const tt w = (tt)load_w<T>(g.w, o + j);
-      const int cu = st.cur[u];
+      const int2 cpu = st.cp[u];  // {cur[u], prev[u]} in one gather
+      const int cu = cpu.x;
       if (cu == c) x += w;
-      if (cu == st.prev[u]) {  // u stayed: its In gains or loses this entry
+      if (cu == cpu.y) {  // u stayed: its In gains or loses this entry
         const tt dl = (cu == c ? w : tt(0)) - (cu == p ? w : tt(0));
         if (dl) atomicAdd(&st.In[u], dl);
       }
@@ -2478,7 +2483,8 @@ __global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevSta
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int i = lo + blockIdx.x; i < hi; i += gridDim.x) {
     const int v = order[i];
-    const int c = st.cur[v], p = st.prev[v];
+    const int2 cpv = st.cp[v];
+    const int c = cpv.x, p = cpv.y;
     if (c == p) continue;  // uniform per block
     const int64_t o = g.off[v];
     const int d = (int)(g.end[v] - o);
@@ -2486,9 +2492,10 @@ __global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevSta
     for (int j = threadIdx.x; j < d; j += NT) {
       const int u = g.nbr[o + j];
       const tt w = (tt)load_w<T>(g.w, o + j);
-      const int cu = st.cur[u];
+      const int2 cpu = st.cp[u];
+      const int cu = cpu.x;
       if (cu == c) x += w;
-      if (cu == st.prev[u]) {
+      if (cu == cpu.y) {
         const tt dl = (cu == c ? w : tt(0)) - (cu == p ? w : tt(0));
         if (dl) atomicAdd(&st.In[u], dl);
       }
