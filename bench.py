@@ -47,9 +47,10 @@ KERNEL_GENERAL = [
      "k_decide_tpv<31, false, IntW>", "k_decide_hlead<64, 16, 2, false, IntW>"],
     ["k_decide_wave<256, 64, IntW>", "k_decide_wave<256, 32, IntW>", "k_decide_wave<256, 16, IntW>",
      "k_decide_hpipe<256, 32, 4, IntW>", "k_decide_hpipe<256, 16, 8, IntW>",
-     "k_decide_hlead<256, 32, 4, false, IntW>", "k_decide_hlead<256, 16, 8, false, IntW>"],
+     "k_decide_hlead<256, 32, 4, false, IntW>", "k_decide_hlead<256, 16, 8, false, IntW>",
+     "k_decide_hlead<256, 8, 16, false, IntW>"],
     ["k_decide_wave<1024, 64, IntW>", "k_decide_wave<1024, 32, IntW>", "k_decide_hpipe<1024, 64, 8, IntW>",
-     "k_decide_hlead<1024, 64, 8, false, IntW>"],
+     "k_decide_hlead<1024, 64, 8, false, IntW>", "k_decide_hlead<1024, 32, 16, false, IntW>"],
     ["k_decide_block<256, 8192, IntW>", "k_decide_blead<256, 8192, 16, 1, IntW>"],
     ["k_decide_block<1024, 16384, IntW>", "k_decide_blead<1024, 16384, 8, 1, IntW>"],
     ["k_decide_hub<16384, IntW>", "k_hub_plan/agg/scan/commit<IntW>",

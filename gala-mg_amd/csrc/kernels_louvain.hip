@@ -1929,9 +1929,10 @@ static int grid_for(int cap, int per_block, int max_blocks) {
 //             4 (bins 2-3) = pipelined leader hash, W/2 lanes x 2 entries
 //   bin 4:    0 = one wave per vertex, 1 = 32 lanes, 2 = 16 lanes,
 //             3 = pipelined, 32 lanes x 4 entries, 4 = pipelined, 16 lanes x 8 entries,
-//             5 = pipelined leader hash, 32 lanes x 4 entries, 6 = the same, 16 x 8
+//             5 = pipelined leader hash, 32 lanes x 4 entries, 6 = the same, 16 x 8,
+//             7 = the same, 8 x 16
 //   bin 5:    0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 64 lanes x 8 entries,
-//             3 = pipelined leader hash, 64 lanes x 8 entries
+//             3 = pipelined leader hash, 64 lanes x 8 entries, 4 = the same, 32 x 16
 //   bins 6-7: 0 = one workgroup per vertex, table scan, 1 = one workgroup per
 //             vertex, leader evaluation (bin 7 FP: the hub kernel)
 //   bin 8:    0 = one workgroup per hub, 1 = hubs split over row and table
@@ -1945,7 +1946,7 @@ static int grid_for(int cap, int per_block, int max_blocks) {
 // bin 4 has 0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 16 lanes x
 // 8 entries; bin 5 has 0 = one wave per vertex, 1 = pipelined, 64 lanes x 8.
 int decide_variants(int bin) {
-  static const int nv[NBINS] = {4, 4, 5, 5, 7, 4, 2, 2, 4};
+  static const int nv[NBINS] = {4, 4, 5, 5, 8, 5, 2, 2, 4};
   return bin >= 0 && bin < NBINS ? nv[bin] : 1;
 }
 int decide_first_variants(int bin) {
@@ -2001,6 +2002,7 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
     case 36: k_decide_hpipe<256, 16, 8, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 37: k_decide_hlead<256, 32, 4, false, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 38: k_decide_hlead<256, 16, 8, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 39: k_decide_hlead<256, 8, 16, false, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     // bin 5: 128..511
     case 40: k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 41:
@@ -2016,6 +2018,12 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
         k_decide_hpipe<1024, 64, 8, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       break;
     case 43: k_decide_hlead<1024, 64, 8, false, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 44:
+      if constexpr (T::FP)  // 8 groups x 1024 FP64 slots exceed the LDS with the keys
+        k_decide_hlead<1024, 64, 8, false, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
+      else
+        k_decide_hlead<1024, 32, 16, false, T><<<grid_for(cap, 32, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
+      break;
     // bin 6: 512..4095
     case 48: k_decide_block<256, 8192, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 49: k_decide_blead<256, 8192, 16, 1, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2, 0); break;

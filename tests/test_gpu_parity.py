@@ -85,7 +85,7 @@ def variant_set(k):
     return np.array([min(k, c - 1) for c in gala.variant_counts()], np.int32)
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7])
 def test_decide_variants_match_oracle(k):
     # every decide kernel variant takes the same decisions as the oracle
     say(f"decide variants {k}")
@@ -97,7 +97,7 @@ def test_decide_variants_match_oracle(k):
     eng.close()
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7])
 def test_decide_variants_random_states(k):
     say(f"decide variants {k} on random states")
     rng = np.random.default_rng(40 + k)
