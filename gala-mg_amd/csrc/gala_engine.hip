@@ -88,6 +88,7 @@ struct gala_ctx {
   int64_t hub_cap = 0;
   int32_t* hub_plan = nullptr;  // split hub decide scratch
   void* hub_part = nullptr;
+  uint8_t* hub_fail = nullptr;
   int32_t* part_tmp = nullptr;
   int64_t* scan64_tmp = nullptr;
   Counters* ctr = nullptr;
@@ -151,7 +152,7 @@ struct gala_ctx {
   // defaults are what gala_autotune picks on the 4M-vertex power-law graph
   // (one thread per vertex below degree 32, the pipelined leader-hash kernels
   // for 32..511, leader evaluation in the workgroup kernels for 512..8191)
-  int dvar[NBINS] = {3, 3, 3, 3, 6, 3, 1, 1, 2};
+  int dvar[NBINS] = {3, 3, 3, 3, 6, 3, 1, 1, 4};
   int fvar[NBINS] = {1, 2, 2, 2, 2, 1, 0, 0, 0};
   int wvar[NBINS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // weight update (1 = pipelined, bins 0-4; 2 = one thread per vertex, bins 0-3)
 
@@ -247,6 +248,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->bstart_full, NBINS + 1); OWN(ctx->bstart_front, NBINS + 1);
   OWN(ctx->hub_off, N + 1); OWN(ctx->hub_tmp, 2 * (N + 1));
   OWN(ctx->hub_plan, hub_plan_elems(M));
+  OWN(ctx->hub_fail, hub_plan_elems(M));
   { char* p; OWN(p, hub_part_elems(M) * (int64_t)sizeof(HubPart<FltW>)); ctx->hub_part = p; }
   OWN(ctx->part_tmp, part_tmp_elems(N));
   OWN(ctx->scan64_tmp, scan_tmp_elems(N) + 2);
@@ -415,7 +417,7 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.size = ctx->size; st.dsize = ctx->dsize; st.prev = ctx->prev; st.cur = ctx->cur;
   st.active = ctx->active; st.dirty = ctx->dirty; st.binid = ctx->binid; st.hub_off = ctx->hub_off;
   st.hub_keys = ctx->hub_keys; st.hub_vals = (typename T::acc_t*)ctx->hub_vals;
-  st.hub_plan = ctx->hub_plan; st.hub_part = ctx->hub_part;
+  st.hub_plan = ctx->hub_plan; st.hub_part = ctx->hub_part; st.hub_fail = ctx->hub_fail;
   st.ctr = ctx->ctr;
   st.track = 0;
   st.iter = 0;

@@ -69,6 +69,7 @@ struct DevState {
   typename T::acc_t* hub_vals;
   int32_t* hub_plan = nullptr;  // split hub decide: chunk starts (hub_plan_elems)
   void* hub_part = nullptr;     // split hub decide: HubPart<T>[hub_part_elems]
+  uint8_t* hub_fail = nullptr;  // community-split hub decide: per-hub overflow flags
   Counters* ctr;
   // learned-frontier features (tracked only when track != 0)
   int track;

@@ -1588,10 +1588,13 @@ __device__ __forceinline__ int hub_find(const int32_t* __restrict__ start, int n
   return a;
 }
 
+// part_keys > 0: instead of table chunks, plan ceil(d / part_keys) community
+// partitions per hub (k_hub_part); only != null: plan only the hubs marked there.
 template <class T>
 __global__ __launch_bounds__(1024) void k_hub_plan(DevGraph<T> g, const int32_t* __restrict__ front,
                                                    const int32_t* __restrict__ bstart, int bin,
-                                                   int32_t* __restrict__ plan, int split_above) {
+                                                   int32_t* __restrict__ plan, int split_above, int part_keys,
+                                                   const uint8_t* __restrict__ only) {
   __shared__ int sh[17];
   const int lo = bstart[bin], hi = bstart[bin + 1], nh = hi - lo;
   int32_t* cstart = plan;           // nh + 1 row-chunk starts
@@ -1603,11 +1606,15 @@ __global__ __launch_bounds__(1024) void k_hub_plan(DevGraph<T> g, const int32_t*
     if (i < nh) {
       const int v = front[lo + i];
       const int64_t d = g.end[v] - g.off[v];
-      if (d > split_above) {
-      int64_t tsz = 1;
-      while (tsz < 2 * d) tsz <<= 1;
-      nc = (int)((d + HUB_CH - 1) / HUB_CH);
-      nt = (int)((tsz + HUB_TCH - 1) / HUB_TCH);
+      if (d > split_above && (!only || only[i])) {
+        if (part_keys > 0) {
+          nt = (int)((d + part_keys - 1) / part_keys);
+        } else {
+          int64_t tsz = 1;
+          while (tsz < 2 * d) tsz <<= 1;
+          nc = (int)((d + HUB_CH - 1) / HUB_CH);
+          nt = (int)((tsz + HUB_TCH - 1) / HUB_TCH);
+        }
       }
     }
     int tot_c, tot_t;
@@ -1753,11 +1760,122 @@ __global__ __launch_bounds__(256) void k_hub_scan(DevGraph<T> g, DevState<T> st,
   }
 }
 
+// Hubs split by community: workgroup (hub, p) of P streams the whole row and
+// aggregates only the entries whose community c has part(c, P) == p, in an
+// LT-slot LDS table (P is chosen so that a part expects at most 3/8 LT
+// distinct communities), then evaluates them: no global table, no global
+// atomics; the row and the prev[] lines it gathers are re-read from L2 by the
+// P workgroups.  A part whose table still fills up (an adversarial skew of the
+// hash) flags its hub in fail[]; those hubs are redone by the global-table
+// split kernels.
+__device__ __forceinline__ int hub_part_of(int c, int P) {
+  const unsigned x = ((unsigned)c ^ ((unsigned)c >> 15)) * 0x85EBCA6Bu;
+  return (int)((x >> 8) % (unsigned)P);
+}
+
+template <int LT, class T>
+__global__ __launch_bounds__(1024) void k_hub_part(DevGraph<T> g, DevState<T> st, const int32_t* __restrict__ front,
+                                                    const int32_t* __restrict__ bstart, int bin,
+                                                    const int32_t* __restrict__ plan, HubPart<T>* __restrict__ part,
+                                                    uint8_t* __restrict__ fail, typename T::tot_t m2_all) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  constexpr int NT = 1024, PMAX = 64, U = 8, LG = __builtin_ctz(LT);
+  __shared__ int s_keys[LT];
+  __shared__ acc s_vals[LT];
+  __shared__ G r_h[NT / 64];
+  __shared__ int r_d[NT / 64];
+  __shared__ acc r_wc[NT / 64];
+  __shared__ int s_fail;
+  for (int t = threadIdx.x; t < LT; t += NT) { s_keys[t] = 0; s_vals[t] = acc(0); }
+  const int lo = bstart[bin], nh = bstart[bin + 1] - lo;
+  const int32_t* tstart = plan + nh + 1;
+  const int nt = tstart[nh];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q = blockIdx.x; q < nt; q += gridDim.x) {
+    const int i = hub_find(tstart, nh, q);
+    const int pp = q - tstart[i], P = tstart[i + 1] - tstart[i];
+    const int v = front[lo + i];
+    const tt m2 = vertex_m2<T>(st, v, m2_all);
+    const int64_t o = g.off[v];
+    const int d = (int)(g.end[v] - o);
+    if (threadIdx.x == 0) s_fail = 0;
+    __syncthreads();  // also orders the table clears before the inserts
+    for (int j0 = threadIdx.x; j0 < d; j0 += NT * U) {
+      int cu[U];
+      acc wv[U];
+#pragma unroll
+      for (int r = 0; r < U; r++) {
+        const int j = j0 + r * NT;
+        cu[r] = j < d ? g.nbr[o + j] : -1;
+        wv[r] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+      }
+#pragma unroll
+      for (int r = 0; r < U; r++) cu[r] = cu[r] >= 0 ? st.prev[cu[r]] : -1;
+#pragma unroll
+      for (int r = 0; r < U; r++) {
+        const int cl = cu[r];
+        if (cl < 0 || hub_part_of(cl, P) != pp) continue;
+        unsigned p = hash_slot(cl, LG);
+        bool placed = false;
+        for (int probe = 0; probe < PMAX; probe++) {
+          const int k = atomicCAS(&s_keys[p], 0, cl + 1);
+          if (k == 0 || k == cl + 1) { atomicAdd(&s_vals[p], wv[r]); placed = true; break; }
+          p = (p + 1) & (LT - 1);
+        }
+        if (!placed) s_fail = 1;
+      }
+    }
+    __syncthreads();
+    const bool failed = s_fail != 0;
+    const int c = st.prev[v];
+    const tt K = st.K[v];
+    const tt Tc = st.Tot[c];
+    G h = 0;
+    int bd = INT_MAX;
+    acc wc = 0;
+    for (int t = threadIdx.x; t < LT; t += NT) {
+      const int k = s_keys[t];
+      if (k) {
+        const int dd = k - 1;
+        const acc a = s_vals[t];
+        s_keys[t] = 0;
+        s_vals[t] = acc(0);
+        if (!failed) {
+          if (dd == c) {
+            wc = a;
+          } else {
+            const G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)st.Tot[dd], (G)Tc);
+            if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; }
+          }
+        }
+      }
+    }
+    argmax_xor<G, 64>(h, bd);
+    wc = sum_xor<acc, 64>(wc);
+    if (lane == 0) { r_h[wid] = h; r_d[wid] = bd; r_wc[wid] = wc; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < NT / 64; w++) {
+        if (better(r_h[w], r_d[w], h, bd)) { h = r_h[w]; bd = r_d[w]; }
+        wc += r_wc[w];
+      }
+      part[q].h = h;
+      part[q].d = bd;
+      part[q].wc = wc;
+      if (failed) fail[i] = 1;
+    }
+    __syncthreads();
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_hub_commit(DevGraph<T> g, DevState<T> st, const int32_t* __restrict__ front,
                                                     const int32_t* __restrict__ bstart, int bin,
                                                     const int32_t* __restrict__ plan,
-                                                    const HubPart<T>* __restrict__ part, typename T::tot_t m2_all) {
+                                                    const HubPart<T>* __restrict__ part, typename T::tot_t m2_all,
+                                                    const uint8_t* __restrict__ skip) {
   using acc = typename T::acc_t;
   using G = typename T::gain_t;
   constexpr int NT = 256;
@@ -1767,7 +1885,8 @@ __global__ __launch_bounds__(256) void k_hub_commit(DevGraph<T> g, DevState<T> s
   unsigned moved = 0, swept = 0;
   unsigned long long edges = 0;
   for (int i = blockIdx.x; i < nh; i += gridDim.x) {
-    if (tstart[i] == tstart[i + 1]) continue;  // left to k_decide_hub (uniform per workgroup)
+    // no chunks: left to k_decide_hub; skipped: redone by the fallback
+    if (tstart[i] == tstart[i + 1] || (skip && skip[i])) continue;  // uniform per workgroup
     const int v = front[lo + i];
     const int64_t o = g.off[v];
     const int d = (int)(g.end[v] - o);
@@ -1939,14 +2058,16 @@ static int grid_for(int cap, int per_block, int max_blocks) {
 //             chunks (plan / agg / scan / commit), 2 = rows longer than the
 //             LDS table split, the others one workgroup per hub, 3 = rows
 //             longer than the LDS table split, the others one workgroup per
-//             hub with two-pass leader evaluation
+//             hub with two-pass leader evaluation, 4 = rows longer than the
+//             LDS table split by community over workgroups (k_hub_part),
+//             5 = the same with one partition per hub (overflow fallback)
 // First sweeps: bins 0-3 have 0 = streaming argmax, 1 = pipelined (four
 // vertices in flight per group), 2 = one thread per vertex (integer path; FP
 // falls back to 1), 3 (bin 3) = pipelined leader kernel, 16 lanes x 2 entries;
 // bin 4 has 0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 16 lanes x
 // 8 entries; bin 5 has 0 = one wave per vertex, 1 = pipelined, 64 lanes x 8.
 int decide_variants(int bin) {
-  static const int nv[NBINS] = {4, 4, 5, 5, 8, 5, 2, 2, 4};
+  static const int nv[NBINS] = {4, 4, 5, 5, 8, 5, 2, 2, 6};
   return bin >= 0 && bin < NBINS ? nv[bin] : 1;
 }
 int decide_first_variants(int bin) {
@@ -2058,11 +2179,32 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
           k_decide_blead<1024, 16384, 16, 2, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b,
                                                                                         m2, above);
       }
-      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above);
+      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, 0, nullptr);
       k_hub_agg<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
       k_hub_scan<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, (HubPart<T>*)st.hub_part, m2);
       k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan,
-                                                              (const HubPart<T>*)st.hub_part, m2);
+                                                              (const HubPart<T>*)st.hub_part, m2, nullptr);
+      break;
+    }
+    case 68:
+    case 69: {
+      // 4: rows longer than the LDS table split by community over workgroups
+      // (k_hub_part), the others one workgroup per hub; hubs whose partition
+      // overflowed are redone by the global-table split.  5: the same with
+      // one partition per hub, so every hub with more distinct communities
+      // than the table holds takes the fallback (kept for its tests).
+      const int PART_KEYS = variant == 4 ? HUB_LT / 8 * 3 : INT_MAX;
+      HubPart<T>* hp = (HubPart<T>*)st.hub_part;
+      k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, HUB_LT);
+      fill<uint8_t>(st.hub_fail, cap, 0, s);
+      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, HUB_LT, PART_KEYS, nullptr);
+      k_hub_part<HUB_LT, T><<<2048, 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, st.hub_fail, m2);
+      k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2,
+                                                              st.hub_fail);
+      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, HUB_LT, 0, st.hub_fail);
+      k_hub_agg<T><<<256, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
+      k_hub_scan<T><<<256, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2);
+      k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2, nullptr);
       break;
     }
   }

@@ -278,7 +278,7 @@ def test_phase_decide_hub_overflow(engine):
         assert np.array_equal(got, exp), rep
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("ncomm", [3, 40000])
 def test_phase_decide_split_hub(ncomm, variant):
     # bin 8 split over row chunks and table chunks (variant 1): a 50k-degree hub
@@ -301,7 +301,7 @@ def test_phase_decide_split_hub(ncomm, variant):
     eng.close()
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 def test_star_graph_split_hub_full_run(variant):
     say(f"star full run, split hub variant {variant}")
     g = star_plus_random(30000, 40000, 3)
