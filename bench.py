@@ -57,7 +57,8 @@ KERNEL_GENERAL = [
      "k_decide_hub<16384, IntW> + k_hub_plan/agg/scan/commit<IntW>",
      "k_decide_blead<1024, 16384, 16, 2, IntW> + k_hub_plan/agg/scan/commit<IntW>",
      "k_decide_hub<16384, IntW> + k_hub_part<16384, IntW>",
-     "k_decide_hub<16384, IntW> + k_hub_part<16384, IntW> (one part per hub)"]]
+     "k_decide_hub<16384, IntW> + k_hub_part<16384, IntW> (one part per hub)",
+     "k_hub_part<16384, IntW>"]]
 KERNEL_FIRST = [
     ["k_decide_first<4, IntW>", "k_decide_pipe<4, true, IntW>", "k_decide_tpv<3, true, IntW>"],
     ["k_decide_first<8, IntW>", "k_decide_pipe<8, true, IntW>", "k_decide_tpv<7, true, IntW>"],

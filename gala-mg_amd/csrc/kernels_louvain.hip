@@ -2060,14 +2060,15 @@ static int grid_for(int cap, int per_block, int max_blocks) {
 //             longer than the LDS table split, the others one workgroup per
 //             hub with two-pass leader evaluation, 4 = rows longer than the
 //             LDS table split by community over workgroups (k_hub_part),
-//             5 = the same with one partition per hub (overflow fallback)
+//             5 = the same with one partition per hub (overflow fallback),
+//             6 = every hub split by community
 // First sweeps: bins 0-3 have 0 = streaming argmax, 1 = pipelined (four
 // vertices in flight per group), 2 = one thread per vertex (integer path; FP
 // falls back to 1), 3 (bin 3) = pipelined leader kernel, 16 lanes x 2 entries;
 // bin 4 has 0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 16 lanes x
 // 8 entries; bin 5 has 0 = one wave per vertex, 1 = pipelined, 64 lanes x 8.
 int decide_variants(int bin) {
-  static const int nv[NBINS] = {4, 4, 5, 5, 8, 5, 2, 2, 6};
+  static const int nv[NBINS] = {4, 4, 5, 5, 8, 5, 2, 2, 7};
   return bin >= 0 && bin < NBINS ? nv[bin] : 1;
 }
 int decide_first_variants(int bin) {
@@ -2187,21 +2188,25 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
       break;
     }
     case 68:
-    case 69: {
+    case 69:
+    case 70: {
       // 4: rows longer than the LDS table split by community over workgroups
       // (k_hub_part), the others one workgroup per hub; hubs whose partition
       // overflowed are redone by the global-table split.  5: the same with
       // one partition per hub, so every hub with more distinct communities
       // than the table holds takes the fallback (kept for its tests).
-      const int PART_KEYS = variant == 4 ? HUB_LT / 8 * 3 : INT_MAX;
+      // 6: every hub split by community.
+      const int PART_KEYS = variant == 5 ? INT_MAX : HUB_LT / 8 * 3;
+      const int above = variant == 6 ? 0 : HUB_LT;
       HubPart<T>* hp = (HubPart<T>*)st.hub_part;
-      k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, HUB_LT);
+      if (above > 0)
+        k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, above);
       fill<uint8_t>(st.hub_fail, cap, 0, s);
-      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, HUB_LT, PART_KEYS, nullptr);
+      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, PART_KEYS, nullptr);
       k_hub_part<HUB_LT, T><<<2048, 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, st.hub_fail, m2);
       k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2,
                                                               st.hub_fail);
-      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, HUB_LT, 0, st.hub_fail);
+      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, 0, st.hub_fail);
       k_hub_agg<T><<<256, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
       k_hub_scan<T><<<256, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2);
       k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2, nullptr);

@@ -85,7 +85,7 @@ def variant_set(k):
     return np.array([min(k, c - 1) for c in gala.variant_counts()], np.int32)
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_decide_variants_match_oracle(k):
     # every decide kernel variant takes the same decisions as the oracle
     say(f"decide variants {k}")
@@ -97,7 +97,7 @@ def test_decide_variants_match_oracle(k):
     eng.close()
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_decide_variants_random_states(k):
     say(f"decide variants {k} on random states")
     rng = np.random.default_rng(40 + k)
@@ -278,7 +278,7 @@ def test_phase_decide_hub_overflow(engine):
         assert np.array_equal(got, exp), rep
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("ncomm", [3, 40000])
 def test_phase_decide_split_hub(ncomm, variant):
     # bin 8 split over row chunks and table chunks (variant 1): a 50k-degree hub
@@ -301,7 +301,7 @@ def test_phase_decide_split_hub(ncomm, variant):
     eng.close()
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
 def test_star_graph_split_hub_full_run(variant):
     say(f"star full run, split hub variant {variant}")
     g = star_plus_random(30000, 40000, 3)
