@@ -1079,23 +1079,33 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     launch_prune<T>(st, g.n, m2max, s);
     launch_graph_sums(gid, ctx->binid, (const int32_t*)ctx->In, (const int32_t*)ctx->Tot, g.n, b.g_in, b.g_tot2, s);
     launch_batch_step(b, it_dev, min_mod, max_it, s);
-    launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, true, it_dev, s);
+    launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, true, it_dev, nullptr, s);
     partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp, s);
     // one iteration for every running graph; a graph that stopped is inert
+    // integer path: In from the movers' rows ({cur, prev} pairs, reset by the
+    // mask after each sweep), so decide-and-move writes no dirty marks
+    constexpr bool incr = !T::FP;
+    DevState<T> sdec = st;
+    if (incr) sdec.dirty = nullptr;
     auto body = [&](bool first) {
       launch_batch_reset(ctx->ctr, m2max, nlive, s);
       launch_build_cv<T>(st, g.n, s);
       for (int k = 0; k < NBINS; k++) {
-        if (first) launch_decide_first<T>(k, g, st, ctx->front, ctx->bstart_front, caps[k], m2max, s, ctx->fvar[k]);
-        else launch_decide<T>(k, g, st, ctx->front, ctx->bstart_front, caps[k], m2max, s, ctx->dvar[k]);
+        if (first) launch_decide_first<T>(k, g, sdec, ctx->front, ctx->bstart_front, caps[k], m2max, s, ctx->fvar[k]);
+        else launch_decide<T>(k, g, sdec, ctx->front, ctx->bstart_front, caps[k], m2max, s, ctx->dvar[k]);
       }
       launch_apply<T>(st, g.n, m2max, s);
-      for (int k = 0; k < NBINS; k++) launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], true, s, ctx->wvar[k]);
+      if constexpr (incr) {
+        for (int k = 0; k < NBINS; k++) launch_weight_moved<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], s);
+      } else {
+        for (int k = 0; k < NBINS; k++)
+          launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], true, s, ctx->wvar[k]);
+      }
       launch_prune<T>(st, g.n, m2max, s);
       launch_graph_sums(gid, ctx->binid, (const int32_t*)ctx->In, (const int32_t*)ctx->Tot, g.n, b.g_in, b.g_tot2,
                         s);
       launch_batch_step(b, it_dev, min_mod, max_it, s);
-      launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, false, it_dev, s);
+      launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, false, it_dev, incr ? st.cp : nullptr, s);
       partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp,
                      s);
     };

@@ -161,7 +161,7 @@ void launch_graph_sums(const int32_t* gid, const int8_t* binid, const int32_t* I
                        unsigned long long* g_in, unsigned long long* g_tot2, hipStream_t s);
 void launch_batch_step(const BatchDev& b, const int32_t* it_dev, double min_mod, int max_it, hipStream_t s);
 void launch_batch_mask(const BatchDev& b, const int32_t* gid, const int32_t* cur, int32_t* prev, uint8_t* active,
-                       int32_t n, bool first, int32_t* it_dev, hipStream_t s);
+                       int32_t n, bool first, int32_t* it_dev, int2* cp, hipStream_t s);
 void launch_batch_live(const BatchDev& b, int32_t* it_dev, hipStream_t s);
 void launch_batch_round(const BatchDev& b, int round, double min_mod, int max_rounds, int32_t* nopen, hipStream_t s);
 void launch_batch_gid_next(const int32_t* gid, const int32_t* cnt, const int32_t* nid, int32_t n, int32_t* gid_next,

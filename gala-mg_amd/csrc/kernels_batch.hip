@@ -84,7 +84,7 @@ __global__ void k_batch_step(BatchDev b, const int32_t* __restrict__ it_dev, dou
 __global__ __launch_bounds__(256) void k_batch_mask(BatchDev b, const int32_t* __restrict__ gid,
                                                      const int32_t* __restrict__ cur, int32_t* __restrict__ prev,
                                                      uint8_t* __restrict__ active, int32_t n, int first,
-                                                     int32_t* __restrict__ it_dev) {
+                                                     int32_t* __restrict__ it_dev, int2* __restrict__ cp) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *it_dev += 1;
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     const int g = gid[v];
@@ -92,7 +92,9 @@ __global__ __launch_bounds__(256) void k_batch_mask(BatchDev b, const int32_t* _
       active[v] = b.live[g];
       continue;
     }
-    if (b.accept[g]) prev[v] = cur[v];
+    int p = prev[v];
+    if (b.accept[g]) prev[v] = p = cur[v];
+    if (cp) cp[v] = make_int2(p, p);  // no vertex has moved in the next sweep yet
     if (!b.live[g]) active[v] = 0;
   }
 }
@@ -174,8 +176,8 @@ void launch_batch_step(const BatchDev& b, const int32_t* it_dev, double min_mod,
   k_batch_step<<<blocks_for(b.G, 256, 1 << 20), 256, 0, s>>>(b, it_dev, min_mod, max_it);
 }
 void launch_batch_mask(const BatchDev& b, const int32_t* gid, const int32_t* cur, int32_t* prev, uint8_t* active,
-                       int32_t n, bool first, int32_t* it_dev, hipStream_t s) {
-  k_batch_mask<<<blocks_for(n, 256, 8192), 256, 0, s>>>(b, gid, cur, prev, active, n, first ? 1 : 0, it_dev);
+                       int32_t n, bool first, int32_t* it_dev, int2* cp, hipStream_t s) {
+  k_batch_mask<<<blocks_for(n, 256, 8192), 256, 0, s>>>(b, gid, cur, prev, active, n, first ? 1 : 0, it_dev, cp);
 }
 void launch_batch_live(const BatchDev& b, int32_t* it_dev, hipStream_t s) {
   k_batch_live<<<blocks_for(b.G, 256, 1 << 20), 256, 0, s>>>(b, it_dev);

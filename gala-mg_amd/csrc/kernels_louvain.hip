@@ -1589,12 +1589,16 @@ __device__ __forceinline__ int hub_find(const int32_t* __restrict__ start, int n
 }
 
 // part_keys > 0: instead of table chunks, plan ceil(d / part_keys) community
-// partitions per hub (k_hub_part); only != null: plan only the hubs marked there.
+// partitions per hub (k_hub_part) for rows of up to part_max entries; longer
+// rows (whose P workgroups would each stream the whole row) are marked in
+// fail_out for the global-table split.  only != null: plan only the hubs
+// marked there.
 template <class T>
 __global__ __launch_bounds__(1024) void k_hub_plan(DevGraph<T> g, const int32_t* __restrict__ front,
                                                    const int32_t* __restrict__ bstart, int bin,
                                                    int32_t* __restrict__ plan, int split_above, int part_keys,
-                                                   const uint8_t* __restrict__ only) {
+                                                   const uint8_t* __restrict__ only, int64_t part_max,
+                                                   uint8_t* __restrict__ fail_out) {
   __shared__ int sh[17];
   const int lo = bstart[bin], hi = bstart[bin + 1], nh = hi - lo;
   int32_t* cstart = plan;           // nh + 1 row-chunk starts
@@ -1608,7 +1612,8 @@ __global__ __launch_bounds__(1024) void k_hub_plan(DevGraph<T> g, const int32_t*
       const int64_t d = g.end[v] - g.off[v];
       if (d > split_above && (!only || only[i])) {
         if (part_keys > 0) {
-          nt = (int)((d + part_keys - 1) / part_keys);
+          if (d <= part_max) nt = (int)((d + part_keys - 1) / part_keys);
+          else fail_out[i] = 1;
         } else {
           int64_t tsz = 1;
           while (tsz < 2 * d) tsz <<= 1;
@@ -2180,7 +2185,7 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
           k_decide_blead<1024, 16384, 16, 2, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b,
                                                                                         m2, above);
       }
-      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, 0, nullptr);
+      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, 0, nullptr, 0, nullptr);
       k_hub_agg<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
       k_hub_scan<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, (HubPart<T>*)st.hub_part, m2);
       k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan,
@@ -2202,13 +2207,16 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
       if (above > 0)
         k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, above);
       fill<uint8_t>(st.hub_fail, cap, 0, s);
-      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, PART_KEYS, nullptr);
+      // rows beyond 16 partitions go to the global-table split straight away
+      const int64_t part_max = variant == 5 ? INT64_MAX : 16ll * PART_KEYS;
+      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, PART_KEYS, nullptr, part_max,
+                                        st.hub_fail);
       k_hub_part<HUB_LT, T><<<2048, 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, st.hub_fail, m2);
       k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2,
                                                               st.hub_fail);
-      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, 0, st.hub_fail);
-      k_hub_agg<T><<<256, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
-      k_hub_scan<T><<<256, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2);
+      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, 0, st.hub_fail, 0, nullptr);
+      k_hub_agg<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
+      k_hub_scan<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2);
       k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2, nullptr);
       break;
     }
