@@ -154,7 +154,10 @@ struct gala_ctx {
   // for 32..511, leader evaluation in the workgroup kernels for 512..8191)
   int dvar[NBINS] = {3, 3, 3, 3, 6, 3, 1, 1, 4};
   int fvar[NBINS] = {1, 2, 2, 2, 2, 1, 0, 0, 0};
-  int wvar[NBINS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // weight update (1 = pipelined, bins 0-4; 2 = one thread per vertex, bins 0-3)
+  // weight update: the incremental integer path with narrow lane groups
+  // (variant 2); the dirty-row path (FP, tracked runs) reads variant 2 as one
+  // thread per vertex for bins 0-3
+  int wvar[NBINS] = {2, 2, 2, 2, 2, 2, 0, 0, 0};
 
   std::vector<void*> owned;
   int32_t cap_n = -1;  // sizes the owned buffers were allocated for
@@ -568,7 +571,8 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     // In changes only where a vertex or one of its neighbours moved
     if constexpr (!T::FP) {
       if (incr) {
-        for (int b = 0; b < NBINS; b++) launch_weight_moved<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], s);
+        for (int b = 0; b < NBINS; b++)
+          launch_weight_moved<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], s, ctx->wvar[b]);
       }
     }
     if (!incr)
@@ -1096,7 +1100,8 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
       }
       launch_apply<T>(st, g.n, m2max, s);
       if constexpr (incr) {
-        for (int k = 0; k < NBINS; k++) launch_weight_moved<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], s);
+        for (int k = 0; k < NBINS; k++)
+          launch_weight_moved<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], s, ctx->wvar[k]);
       } else {
         for (int k = 0; k < NBINS; k++)
           launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], true, s, ctx->wvar[k]);
@@ -1297,7 +1302,10 @@ int gala_run(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gal
   return is_fp(ctx) ? run_all<FltW>(ctx, icb, rcb, user, out) : run_all<IntW>(ctx, icb, rcb, user, out);
 }
 
-static int weight_variants(int bin) { return bin <= 3 ? 3 : bin == 4 ? 2 : 1; }
+// weight-update kernel variants per bin.  Dirty-row path (FP, tracked runs):
+// 1 = pipelined (bins 0-4), 2 = one thread per vertex (bins 0-3).  Incremental
+// integer path: narrower lane groups for bins 1-5 (launch_weight_moved).
+static int weight_variants(int bin) { return bin <= 5 ? 3 : 1; }
 
 int gala_set_variants(gala_ctx* ctx, const int32_t* variants) {
   if (!ctx || !variants) return GALA_E_INVALID;

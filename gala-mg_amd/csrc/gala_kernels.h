@@ -151,7 +151,7 @@ void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
 // neighbour that stayed.  Decide-and-move needs no dirty marks for it.
 template <class T>
 void launch_weight_moved(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
-                         const int32_t* bstart, int cap, hipStream_t s);
+                         const int32_t* bstart, int cap, hipStream_t s, int variant = 0);
 // prune flags, sum In, active count; clears the dirty marks.
 template <class T>
 void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);

@@ -716,20 +716,37 @@ __global__ __launch_bounds__(256) void k_rs_emit(const DenseChunk* __restrict__ 
   }
 }
 
-// ... then every pair adds its value into its run's slot (runs hold at most
-// one pair per chunk of the row)
+// ... then every pair's value goes into its run's slot (runs hold at most
+// one pair per chunk of the row): summed per wave first, one atomic per run
+// and wave
 template <class T>
 __global__ __launch_bounds__(256) void k_rs_sum(const DenseChunk* __restrict__ tiles, int ntiles,
                                                 const typename T::acc_t* __restrict__ vals,
                                                 const uint8_t* __restrict__ heads, const int64_t* __restrict__ hs,
                                                 const int64_t* __restrict__ eoff, typename T::acc_t* __restrict__ sval) {
+  using acc = typename T::acc_t;
+  const int lane = threadIdx.x & 63;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const DenseChunk tt = tiles[t];
     const int64_t seg0 = tiles[tt.first].start;
     const int64_t out0 = eoff[tt.c] - hs[seg0];
-    for (int i = threadIdx.x; i < tt.len; i += 256) {
+    for (int i0 = 0; i0 < tt.len; i0 += 256) {  // uniform trip count per workgroup
+      const int i = i0 + threadIdx.x;
+      const bool ok = i < tt.len;
       const int64_t g = tt.start + i;
-      atomicAdd(&sval[out0 + hs[g] - (heads[g] ? 0 : 1)], vals[g]);
+      // run slot of the pair (runs are contiguous), -1 past the tile end
+      const int64_t r = ok ? out0 + hs[g] - (heads[g] ? 0 : 1) : -1;
+      acc x = ok ? vals[g] : acc(0);
+      // segmented suffix sums over the wave: the first lane of each run in
+      // the wave then holds the run's wave-local sum and adds it once
+#pragma unroll
+      for (int k = 1; k < 64; k <<= 1) {
+        const acc xo = __shfl_down(x, k, 64);
+        const int64_t ro = __shfl_down(r, k, 64);
+        if (lane + k < 64 && ro == r) x += xo;
+      }
+      const int64_t rp = __shfl_up(r, 1, 64);
+      if (ok && (lane == 0 || rp != r)) atomicAdd(&sval[r], x);
     }
   }
 }

@@ -2675,18 +2675,27 @@ __global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevSta
   }
 }
 
+// variant: lanes per row for bins 0-5 -- 0: 4, 8, 16, 32, 64, 64; 1: 4, 4, 8,
+// 16, 32, 32; 2: 4, 4, 8, 8, 16, 32 (narrower groups: more rows in flight per
+// wave, fewer idle lanes on short rows)
 template <class T>
 void launch_weight_moved(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
-                         const int32_t* bstart, int cap, hipStream_t s) {
+                         const int32_t* bstart, int cap, hipStream_t s, int variant) {
   if (cap <= 0) return;
   static_assert(!T::FP, "integer path only: the FP path keeps a deterministic summation order");
+  static const int lanes[3][6] = {{4, 8, 16, 32, 64, 64}, {4, 4, 8, 16, 32, 32}, {4, 4, 8, 8, 16, 32}};
+  if (variant < 0 || variant > 2) variant = 0;
+  if (bin <= 5) {
+    switch (lanes[variant][bin]) {
+      case 4: k_weight_moved<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, order, bstart, bin); break;
+      case 8: k_weight_moved<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, order, bstart, bin); break;
+      case 16: k_weight_moved<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, order, bstart, bin); break;
+      case 32: k_weight_moved<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, order, bstart, bin); break;
+      default: k_weight_moved<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, order, bstart, bin); break;
+    }
+    return;
+  }
   switch (bin) {
-    case 0: k_weight_moved<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, order, bstart, 0); break;
-    case 1: k_weight_moved<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, order, bstart, 1); break;
-    case 2: k_weight_moved<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, order, bstart, 2); break;
-    case 3: k_weight_moved<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, order, bstart, 3); break;
-    case 4: k_weight_moved<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, order, bstart, 4); break;
-    case 5: k_weight_moved<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, order, bstart, 5); break;
     case 6: k_weight_moved_block<256, T><<<grid_for(cap, 1, 4096), 256, 0, s>>>(g, st, order, bstart, 6); break;
     case 7: k_weight_moved_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, order, bstart, 7); break;
     case 8: k_weight_moved_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, order, bstart, 8); break;
@@ -2821,6 +2830,6 @@ void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t 
 GALA_INST(IntW)
 GALA_INST(FltW)
 template void launch_weight_moved<IntW>(int, const DevGraph<IntW>&, DevState<IntW>&, const int32_t*, const int32_t*,
-                                        int, hipStream_t);
+                                        int, hipStream_t, int);
 
 }  // namespace gala
