@@ -510,12 +510,13 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
 //     packs the chunks' pairs row after row into one buffer.
 //  2. the packed rows are cut into tiles of <= LT/2 pairs (a tile table
 //     built on the host from the row lengths) and sorted by a stable LSD radix
-//     sort over 8-bit digits of the key, segmented by row: per pass k_rs_hist
-//     counts the digits of every tile into hist[256*first + digit*nt +
-//     tile-in-row], one exclusive scan of that array gives every (row, digit,
-//     tile) its absolute output position, and k_rs_scatter ranks each tile's
-//     pairs stably (per wave with eight ballots, then across waves) and
-//     writes them to the other packed buffer.
+//     sort over RB-bit digits of the key (RB = 10 when the keys fit in 20
+//     bits, two passes; else 8), segmented by row: per pass k_rs_hist counts
+//     the digits of every tile into hist[2^RB*first + digit*nt + tile-in-row],
+//     one exclusive scan of that array gives every (row, digit, tile) its
+//     absolute output position, and k_rs_scatter ranks each tile's pairs
+//     stably (per wave with RB ballots, then across waves) and writes them to
+//     the other packed buffer.
 //  3. k_rs_heads / scan / k_rs_emit / k_rs_sum: equal keys are adjacent;
 //     each run head writes its key at the run's rank from the start of the
 //     row's staged segment, and every pair adds its value into that slot, so
@@ -593,24 +594,29 @@ __global__ __launch_bounds__(256) void k_dense_pack(const DenseChunk* __restrict
 }
 
 // digit counts of every tile (layout: see above)
+// RB-bit digits (NB = 2^RB buckets): 8 bits, or 10 when the keys have at most
+// 20 bits, so that two passes cover them
+template <int RB>
 __global__ __launch_bounds__(256) void k_rs_hist(const DenseChunk* __restrict__ tiles, int ntiles,
                                                  const int32_t* __restrict__ kin, int shift,
                                                  int32_t* __restrict__ hist) {
-  __shared__ int cnt[256];
+  constexpr int NB = 1 << RB;
+  __shared__ int cnt[NB];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const DenseChunk tt = tiles[t];
     const int32_t* src = kin + tt.start;
-    cnt[threadIdx.x] = 0;
+    for (int d = threadIdx.x; d < NB; d += 256) cnt[d] = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < tt.len; i += 256) atomicAdd(&cnt[(src[i] >> shift) & 255], 1);
+    for (int i = threadIdx.x; i < tt.len; i += 256) atomicAdd(&cnt[(src[i] >> shift) & (NB - 1)], 1);
     __syncthreads();
-    hist[256ll * tt.first + (int64_t)threadIdx.x * tt.nch + (t - tt.first)] = cnt[threadIdx.x];
+    for (int d = threadIdx.x; d < NB; d += 256)
+      hist[(int64_t)NB * tt.first + (int64_t)d * tt.nch + (t - tt.first)] = cnt[d];
     __syncthreads();
   }
 }
 
 // stable scatter of every tile's pairs to their digit positions
-template <int LT, class T>
+template <int LT, int RB, class T>
 __global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restrict__ tiles, int ntiles,
                                                      const int32_t* __restrict__ kin,
                                                      const typename T::acc_t* __restrict__ vin, int shift,
@@ -618,17 +624,18 @@ __global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restric
                                                      typename T::acc_t* __restrict__ vout) {
   using acc = typename T::acc_t;
   constexpr int NW = 16, R = (LT / 2) / (NW * 64);  // rounds of 64 pairs per wave
-  __shared__ int wcnt[NW][256];
-  __shared__ int64_t gofs[256];
+  constexpr int NB = 1 << RB;
+  __shared__ int wcnt[NW][NB];
+  __shared__ int64_t gofs[NB];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned long long below_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const DenseChunk tt = tiles[t];
     const int n = tt.len;
     const int64_t b0 = tt.start;
-    for (int q = threadIdx.x; q < NW * 256; q += 1024) (&wcnt[0][0])[q] = 0;
-    if (threadIdx.x < 256)
-      gofs[threadIdx.x] = hscan[256ll * tt.first + (int64_t)threadIdx.x * tt.nch + (t - tt.first)];
+    for (int q = threadIdx.x; q < NW * NB; q += 1024) (&wcnt[0][0])[q] = 0;
+    for (int d = threadIdx.x; d < NB; d += 1024)
+      gofs[d] = hscan[(int64_t)NB * tt.first + (int64_t)d * tt.nch + (t - tt.first)];
     __syncthreads();
     int key[R], rk[R];
     acc val[R];
@@ -638,10 +645,10 @@ __global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restric
       const bool ok = i < n;
       key[r] = ok ? kin[b0 + i] : 0;
       val[r] = ok ? vin[b0 + i] : acc(0);
-      const int d = (key[r] >> shift) & 255;
+      const int d = (key[r] >> shift) & (NB - 1);
       unsigned long long peers = __ballot(ok);
 #pragma unroll
-      for (int bit = 0; bit < 8; bit++) {
+      for (int bit = 0; bit < RB; bit++) {
         const bool on = (d >> bit) & 1;
         const unsigned long long bb = __ballot(on);
         peers &= on ? bb : ~bb;
@@ -656,11 +663,11 @@ __global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restric
       rk[r] = base + below;
     }
     __syncthreads();
-    if (threadIdx.x < 256) {  // exclusive prefix over the waves, per digit
+    for (int d = threadIdx.x; d < NB; d += 1024) {  // exclusive prefix over the waves, per digit
       int run = 0;
       for (int q = 0; q < NW; q++) {
-        const int x = wcnt[q][threadIdx.x];
-        wcnt[q][threadIdx.x] = run;
+        const int x = wcnt[q][d];
+        wcnt[q][d] = run;
         run += x;
       }
     }
@@ -669,7 +676,7 @@ __global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restric
     for (int r = 0; r < R; r++) {
       const int i = (w * R + r) * 64 + lane;
       if (i < n) {
-        const int d = (key[r] >> shift) & 255;
+        const int d = (key[r] >> shift) & (NB - 1);
         const int64_t pos = gofs[d] + wcnt[w][d] + rk[r];
         kout[pos] = key[r];
         vout[pos] = val[r];
@@ -752,6 +759,7 @@ __global__ __launch_bounds__(256) void k_rs_sum(const DenseChunk* __restrict__ t
 }
 
 // Scratch layout of the dense-row sort (offsets in bytes, 256-byte aligned).
+constexpr int64_t RS_NB_MAX = 1024;  // buckets of the widest radix digit (10 bits)
 struct DenseLayout {
   int64_t ccnt, pstart, bound, tiles, hist, hscan, k0, k1, v0, v1, heads, hs, tmp, total;
 };
@@ -766,15 +774,15 @@ static DenseLayout dense_layout(int64_t nch, int64_t nrows, int64_t P, int64_t v
   L.pstart = o; o = al(o + 8 * (nch + 1));
   L.bound = o; o = al(o + 8 * (nrows + 1));
   L.tiles = o; o = al(o + (int64_t)sizeof(DenseChunk) * nt);
-  L.hist = o; o = al(o + 4 * 256 * nt);
-  L.hscan = o; o = al(o + 8 * (256 * nt + 1));
+  L.hist = o; o = al(o + 4 * RS_NB_MAX * nt);
+  L.hscan = o; o = al(o + 8 * (RS_NB_MAX * nt + 1));
   L.k0 = o; o = al(o + 4 * P);
   L.k1 = o; o = al(o + 4 * P);
   L.v0 = o; o = al(o + vsz * P);
   L.v1 = o; o = al(o + vsz * P);
   L.heads = o; o = al(o + P);
   L.hs = o; o = al(o + 8 * (P + 1));
-  L.tmp = o; o = al(o + 8 * (scan_tmp_elems(std::max<int64_t>(P, 256 * nt)) + 1));
+  L.tmp = o; o = al(o + 8 * (scan_tmp_elems(std::max<int64_t>(P, RS_NB_MAX * nt)) + 1));
   L.total = o;
   return L;
 }
@@ -1026,14 +1034,23 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     if (hipMemcpyAsync(tiles, tv.data(), sizeof(DenseChunk) * ntl, hipMemcpyHostToDevice, s) != hipSuccess)
       return -1;
     const int bits = C > 1 ? 32 - __builtin_clz((unsigned)(C - 1)) : 1;
-    const int npass = (bits + 7) / 8;
+    const int rb = bits <= 20 ? 10 : 8;  // two passes of 10 bits cover C <= 2^20
+    const int npass = (bits + rb - 1) / rb;
+    const int64_t nb = 1ll << rb;
     for (int p = 0; p < npass; p++) {
       const int32_t* kin = kb[(p + 1) & 1];
       const acc* vin = vb[(p + 1) & 1];
-      k_rs_hist<<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kin, 8 * p, hist);
-      scan_exclusive<int32_t, int64_t>(hist, hscan, 256ll * ntl, tmp, hscan + 256ll * ntl, s);
-      k_rs_scatter<LT, T><<<std::min(ntl, 4096), 1024, 0, s>>>(tiles, ntl, kin, vin, 8 * p, hscan, kb[p & 1],
-                                                                vb[p & 1]);
+      if (rb == 10)
+        k_rs_hist<10><<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kin, rb * p, hist);
+      else
+        k_rs_hist<8><<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kin, rb * p, hist);
+      scan_exclusive<int32_t, int64_t>(hist, hscan, nb * ntl, tmp, hscan + nb * ntl, s);
+      if (rb == 10)
+        k_rs_scatter<LT, 10, T><<<std::min(ntl, 4096), 1024, 0, s>>>(tiles, ntl, kin, vin, rb * p, hscan, kb[p & 1],
+                                                                    vb[p & 1]);
+      else
+        k_rs_scatter<LT, 8, T><<<std::min(ntl, 4096), 1024, 0, s>>>(tiles, ntl, kin, vin, rb * p, hscan, kb[p & 1],
+                                                                   vb[p & 1]);
     }
     const int32_t* kf = kb[(npass - 1) & 1];
     const acc* vf = vb[(npass - 1) & 1];
