@@ -255,7 +255,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   { char* p; OWN(p, hub_part_elems(M) * (int64_t)sizeof(HubPart<FltW>)); ctx->hub_part = p; }
   OWN(ctx->part_tmp, part_tmp_elems(N));
   OWN(ctx->scan64_tmp, scan_tmp_elems(N) + 2);
-  OWN(ctx->ctr, 1);
+  OWN(ctx->ctr, 2);  // [1] holds a deferred iteration-0 copy
   OWN(ctx->part, MAX_PARTS);
   // contraction
   OWN(ctx->c_cnt, N); OWN(ctx->c_nid, N); OWN(ctx->c_label, N); OWN(ctx->c_skey, M);
@@ -268,7 +268,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->c_big_tab, N + 1); OWN(ctx->c_i64_tmp, 2 * (N + 1) + scan_tmp_elems(N));
   OWN(ctx->c_i64_scalar, 4);
   if (!ctx->ctr_h) {
-    hipError_t e = hipHostMalloc((void**)&ctx->ctr_h, sizeof(Counters), hipHostMallocDefault);
+    hipError_t e = hipHostMalloc((void**)&ctx->ctr_h, 2 * sizeof(Counters), hipHostMallocDefault);
     if (e != hipSuccess) { ctx->last_hip = e; free_all(ctx); return GALA_E_OOM; }
   }
   if (!ctx->hpin) {
@@ -477,11 +477,18 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   return GALA_OK;
 }
 
+// Reads ctr[0] into out and, when saved is given, the copy kept in ctr[1] as
+// well, in one transfer and one synchronisation.
 template <class T>
-int read_counters(gala_ctx* ctx, Counters* out) {
-  HIPOK(hipMemcpyAsync(ctx->ctr_h, ctx->ctr, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+int read_counters(gala_ctx* ctx, Counters* out, Counters* saved = nullptr) {
+  const int k = saved ? 2 : 1;
+  HIPOK(hipMemcpyAsync(ctx->ctr_h, ctx->ctr, k * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
   HIPOK(hipStreamSynchronize(ctx->stream));
-  *out = *ctx->ctr_h;
+  *out = ctx->ctr_h[0];
+  if (saved) {
+    *saved = ctx->ctr_h[1];
+    if (saved->overflow) return GALA_E_OVERFLOW;
+  }
   if (out->overflow) return GALA_E_OVERFLOW;
   return GALA_OK;
 }
@@ -514,16 +521,20 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   }
   launch_prune<T>(st, g.n, m2, s);
   Counters c;
-  if ((rc = read_counters<T>(ctx, &c))) return rc;
-  double q = q_from<T>(c, m2d), qprev = q;
-  if (icb) {
-    const int32_t* vec = nullptr;
-    if (ctx->opts.dump_iterations) {
-      hostvec.resize(g.n);
-      HIPOK(hipMemcpy(hostvec.data(), ctx->prev, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost));
-      vec = hostvec.data();
-    }
-    icb(user, round, 0, q, 0, vec, g.n);
+  double q = 0, qprev = 0;
+  // Q0 is first needed by iteration 1's stopping test, so its counters are
+  // parked in ctr[1] and read with iteration 1's, saving a synchronisation;
+  // a per-iteration dump needs prev as it is now and reads at once.
+  const bool dump0 = icb && ctx->opts.dump_iterations;
+  bool q0_pending = !dump0;
+  if (q0_pending) {
+    HIPOK(hipMemcpyAsync(ctx->ctr + 1, ctx->ctr, sizeof(Counters), hipMemcpyDeviceToDevice, s));
+  } else {
+    if ((rc = read_counters<T>(ctx, &c))) return rc;
+    q = qprev = q_from<T>(c, m2d);
+    hostvec.resize(g.n);
+    HIPOK(hipMemcpy(hostvec.data(), ctx->prev, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost));
+    icb(user, round, 0, q, 0, hostvec.data(), g.n);
   }
   // first frontier: every vertex (fill(active,1), louvain_main_process.cu:574).
   // Two frontier buffers alternate so that the frontier a sweep started from
@@ -581,8 +592,14 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     launch_prune<T>(sp, g.n, m2, s);
     partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, fnext, bnext, ctx->part_tmp, s);
     HIPOK(hipEventRecord(ctx->ev[2], s));
-    int r = read_counters<T>(ctx, out);
+    Counters c0;
+    int r = read_counters<T>(ctx, out, q0_pending ? &c0 : nullptr);
     if (r) return r;
+    if (q0_pending) {
+      q0_pending = false;
+      qprev = q_from<T>(c0, m2d);
+      if (icb) icb(user, round, 0, qprev, 0, nullptr, g.n);
+    }
     float a = 0, b2 = 0;
     (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
     (void)hipEventElapsedTime(&b2, ctx->ev[1], ctx->ev[2]);
