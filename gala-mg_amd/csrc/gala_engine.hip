@@ -448,7 +448,7 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   DevState<T> st = dstate<T>(ctx);
   st.track = track;
   launch_init<T>(g, st, s);
-  partition_bins(0, nullptr, ctx->binid, nullptr, g.n, ctx->order, ctx->bstart_full, ctx->part_tmp, s);
+  partition_bins(0, nullptr, nullptr, ctx->binid, nullptr, g.n, ctx->order, ctx->bstart_full, ctx->part_tmp, s);
   // hub-table sizes and their prefix sums before the one readback: the scan
   // runs over n slots (entries past the hub count are zero)
   const int gb = global_bin_min<T>();
@@ -590,7 +590,8 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       for (int b = 0; b < NBINS; b++)
         launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s, ctx->wvar[b]);
     launch_prune<T>(sp, g.n, m2, s);
-    partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, fnext, bnext, ctx->part_tmp, s);
+    partition_bins(1, ctx->order, ctx->bstart_full, ctx->binid, ctx->active, nitems, fnext, bnext, ctx->part_tmp,
+                   s);
     HIPOK(hipEventRecord(ctx->ev[2], s));
     Counters c0;
     int r = read_counters<T>(ctx, out, q0_pending ? &c0 : nullptr);
@@ -645,7 +646,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     const int32_t nf0 = nfront;
     if (lsweep) {
       launch_mlp_filter<T>(g, st, fcur, nfront, it, ctx->mlp_prm, ctx->mlp_w1b, tau, ctx->keep, s);
-      partition_bins(1, fcur, ctx->binid, ctx->keep, nfront, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
+      partition_bins(1, fcur, bcur, ctx->binid, ctx->keep, nfront, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
       // pre-sweep Tot and size, read again if the sweep has to be certified
       HIPOK(hipMemcpyAsync(ctx->tot_snap, st.Tot, tsz * g.n, hipMemcpyDeviceToDevice, s));
       HIPOK(hipMemcpyAsync(ctx->size_snap, st.size, sizeof(int32_t) * g.n, hipMemcpyDeviceToDevice, s));
@@ -680,7 +681,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       // them touches keep the In of the learned sweep, which is exact.
       ctx->res.certify_sweeps++;
       reset_counters(ctx->ctr, (long long)m2, m2d, s);
-      partition_bins(2, fnext, ctx->binid, ctx->keep, nf0, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
+      partition_bins(2, fnext, bnext, ctx->binid, ctx->keep, nf0, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
       DevState<T> sc = st;
       sc.Tot = (tt*)ctx->tot_snap;
       sc.size = ctx->size_snap;
@@ -1101,7 +1102,8 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     launch_graph_sums(gid, ctx->binid, (const int32_t*)ctx->In, (const int32_t*)ctx->Tot, g.n, b.g_in, b.g_tot2, s);
     launch_batch_step(b, it_dev, min_mod, max_it, s);
     launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, true, it_dev, nullptr, s);
-    partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp, s);
+    partition_bins(1, ctx->order, ctx->bstart_full, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front,
+                 ctx->part_tmp, s);
     // one iteration for every running graph; a graph that stopped is inert
     // integer path: In from the movers' rows ({cur, prev} pairs, reset by the
     // mask after each sweep), so decide-and-move writes no dirty marks
@@ -1128,8 +1130,8 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
                         s);
       launch_batch_step(b, it_dev, min_mod, max_it, s);
       launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, false, it_dev, incr ? st.cp : nullptr, s);
-      partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp,
-                     s);
+      partition_bins(1, ctx->order, ctx->bstart_full, ctx->binid, ctx->active, nitems, ctx->front,
+                     ctx->bstart_front, ctx->part_tmp, s);
     };
     auto live_after = [&](int32_t* out_live) -> int {
       HIPOK(hipMemcpyAsync(ctx->b_host, nlive, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1470,7 +1472,8 @@ int gala_phase_decide(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const i
   HIPOK(hipMemcpyAsync(ctx->cur, prev, 4ll * n, hipMemcpyHostToDevice, s));
   if (frontier) HIPOK(hipMemcpyAsync(ctx->active, frontier, n, hipMemcpyHostToDevice, s));
   else fill<uint8_t>(ctx->active, n, 1, s);
-  partition_bins(1, ctx->order, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp, s);
+  partition_bins(1, ctx->order, ctx->bstart_full, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front,
+                 ctx->part_tmp, s);
   DevState<IntW> st = dstate<IntW>(ctx);
   reset_counters(ctx->ctr, m2, (double)m2, s);
   launch_build_cv<IntW>(st, n, s);

@@ -14,9 +14,11 @@ void scan_exclusive(const In* in, V* out, int64_t n, V* tmp, V* total, hipStream
 int64_t part_tmp_elems(int64_t n);
 // Stable partition into degree bins.  mode 0: every vertex 0..n-1 by binid;
 // mode 1: the vertices of order[0..n) with active[v] != 0; mode 2: those with
-// active[v] == 0.  bstart receives NBINS+1 bin starts.
-void partition_bins(int mode, const int32_t* order, const int8_t* binid, const uint8_t* active,
-                    int64_t n, int32_t* out, int32_t* bstart, int32_t* tmp, hipStream_t s);
+// active[v] == 0.  bstart receives NBINS+1 bin starts.  Modes 1 and 2 take
+// order already grouped by bin, with its NBINS+1 bin starts in order_bstart
+// (device), and keep that grouping: a stable compaction.
+void partition_bins(int mode, const int32_t* order, const int32_t* order_bstart, const int8_t* binid,
+                    const uint8_t* active, int64_t n, int32_t* out, int32_t* bstart, int32_t* tmp, hipStream_t s);
 
 template <class V>
 void fill(V* p, int64_t n, V x, hipStream_t s);

@@ -131,23 +131,44 @@ __global__ __launch_bounds__(PART_T) void k_part_count(int mode, const int32_t* 
   if (threadIdx.x < NBINS) cnt[threadIdx.x * nblk + blockIdx.x] = sc[threadIdx.x];
 }
 
+// exclusive scan of the bin-major counts, staged through LDS in chunks of
+// PART_TOP_CH so that the global loads and stores stay coalesced
+constexpr int PART_TOP_CH = 16384;
 __global__ __launch_bounds__(1024) void k_part_top(int32_t* __restrict__ cnt, int nblk,
                                                    int32_t* __restrict__ bstart) {
+  constexpr int PER = PART_TOP_CH / 1024;
+  __shared__ int32_t buf[PART_TOP_CH];
   __shared__ int32_t sh[17];
-  int total = NBINS * nblk;
-  int per = (total + 1023) / 1024;
-  int lo = threadIdx.x * per, hi = min(total, lo + per);
-  int32_t s = 0;
-  for (int i = lo; i < hi; i++) s += cnt[i];
-  int32_t tot;
-  int32_t pre = block_excl_scan(s, sh, &tot);
-  for (int i = lo; i < hi; i++) {
-    int32_t t = cnt[i];
-    cnt[i] = pre;
-    if (i % nblk == 0) bstart[i / nblk] = pre;
-    pre += t;
+  const int total = NBINS * nblk;
+  int32_t carry = 0;
+  for (int c0 = 0; c0 < total; c0 += PART_TOP_CH) {
+    const int len = min(PART_TOP_CH, total - c0);
+    for (int i = threadIdx.x; i < len; i += 1024) buf[i] = cnt[c0 + i];
+    __syncthreads();
+    const int lo = threadIdx.x * PER;
+    int32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) s += lo + k < len ? buf[lo + k] : 0;
+    int32_t tot;
+    int32_t pre = block_excl_scan(s, sh, &tot) + carry;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      if (lo + k < len) {
+        const int32_t t = buf[lo + k];
+        buf[lo + k] = pre;
+        pre += t;
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < len; i += 1024) {
+      const int g = c0 + i;
+      cnt[g] = buf[i];
+      if (g % nblk == 0) bstart[g / nblk] = buf[i];
+    }
+    carry += tot;
+    __syncthreads();
   }
-  if (threadIdx.x == 0) bstart[NBINS] = tot;
+  if (threadIdx.x == 0) bstart[NBINS] = carry;
 }
 
 __global__ __launch_bounds__(PART_T) void k_part_scatter(int mode, const int32_t* __restrict__ order,
@@ -191,8 +212,103 @@ __global__ __launch_bounds__(PART_T) void k_part_scatter(int mode, const int32_t
 
 int64_t part_tmp_elems(int64_t n) { return (int64_t)NBINS * ((n + PART_TILE - 1) / PART_TILE + 1); }
 
-void partition_bins(int mode, const int32_t* order, const int8_t* binid, const uint8_t* active,
-                    int64_t n, int32_t* out, int32_t* bstart, int32_t* tmp, hipStream_t s) {
+// ---------------------------------------------------------------------------
+// Stable compaction of a bin-grouped list (modes 1 and 2): kept items keep
+// their order, so the bins stay grouped; the new bin starts are the kept
+// counts before the old ones.  Each wave owns 64 * CMP_IPT consecutive items
+// and ranks them with ballots.
+// ---------------------------------------------------------------------------
+constexpr int CMP_T = 256, CMP_IPT = 16, CMP_TILE = CMP_T * CMP_IPT;
+
+__device__ __forceinline__ bool cmp_keep(int mode, const uint8_t* __restrict__ flag, int v) {
+  return (flag[v] != 0) == (mode == 1);
+}
+
+__global__ __launch_bounds__(CMP_T) void k_cmp_count(int mode, const int32_t* __restrict__ order,
+                                                     const uint8_t* __restrict__ flag, int64_t n,
+                                                     int32_t* __restrict__ cnt) {
+  __shared__ int32_t sh[17];
+  const int64_t base = (int64_t)blockIdx.x * CMP_TILE;
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; k++) {
+    const int64_t i = base + k * CMP_T + threadIdx.x;
+    if (i < n) c += cmp_keep(mode, flag, order[i]) ? 1 : 0;
+  }
+  int32_t tot;
+  (void)block_excl_scan<int32_t>(c, sh, &tot);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+}
+
+// exclusive scan of the per-tile counts; bins that start at or past n start
+// at the total
+__global__ __launch_bounds__(1024) void k_cmp_top(int32_t* __restrict__ cnt, int nblk,
+                                                  const int32_t* __restrict__ bstart_in, int64_t n,
+                                                  int32_t* __restrict__ bstart) {
+  __shared__ int32_t sh[17];
+  int32_t carry = 0;
+  for (int c0 = 0; c0 < nblk; c0 += 1024) {
+    const int i = c0 + threadIdx.x;
+    const int32_t x = i < nblk ? cnt[i] : 0;
+    int32_t tot;
+    const int32_t pre = block_excl_scan(x, sh, &tot);
+    if (i < nblk) cnt[i] = pre + carry;
+    carry += tot;
+  }
+  if (threadIdx.x <= NBINS && bstart_in[threadIdx.x] >= n) bstart[threadIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(CMP_T) void k_cmp_scatter(int mode, const int32_t* __restrict__ order,
+                                                       const uint8_t* __restrict__ flag, int64_t n,
+                                                       const int32_t* __restrict__ cnt,
+                                                       const int32_t* __restrict__ bstart_in,
+                                                       int32_t* __restrict__ out, int32_t* __restrict__ bstart) {
+  __shared__ int32_t wsum[CMP_T / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const int64_t wbase = (int64_t)blockIdx.x * CMP_TILE + (int64_t)w * 64 * CMP_IPT;
+  int vs[CMP_IPT];
+  unsigned keep = 0;
+  int wc = 0;
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; k++) {
+    const int64_t i = wbase + k * 64 + lane;
+    const bool ok = i < n;
+    vs[k] = ok ? order[i] : 0;
+    const bool kp = ok && cmp_keep(mode, flag, vs[k]);
+    keep |= (kp ? 1u : 0u) << k;
+    wc += __popcll(__ballot(kp));
+  }
+  if (lane == 0) wsum[w] = wc;
+  __syncthreads();
+  int pre = cnt[blockIdx.x];
+  for (int q = 0; q < w; q++) pre += wsum[q];
+  int bs[NBINS];
+#pragma unroll
+  for (int b = 0; b < NBINS; b++) bs[b] = bstart_in[b];
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; k++) {
+    const int64_t i = wbase + k * 64 + lane;
+    const bool kp = (keep >> k) & 1u;
+    const unsigned long long bal = __ballot(kp);
+    const int pos = pre + __popcll(bal & below);  // kept items before i
+    if (kp) out[pos] = vs[k];
+#pragma unroll
+    for (int b = 0; b < NBINS; b++)
+      if (bs[b] == i) bstart[b] = pos;
+    pre += __popcll(bal);
+  }
+}
+
+void partition_bins(int mode, const int32_t* order, const int32_t* order_bstart, const int8_t* binid,
+                    const uint8_t* active, int64_t n, int32_t* out, int32_t* bstart, int32_t* tmp, hipStream_t s) {
+  if (mode != 0) {
+    const int nblk = std::max<int>(1, (int)((n + CMP_TILE - 1) / CMP_TILE));
+    k_cmp_count<<<nblk, CMP_T, 0, s>>>(mode, order, active, n, tmp);
+    k_cmp_top<<<1, 1024, 0, s>>>(tmp, nblk, order_bstart, n, bstart);
+    k_cmp_scatter<<<nblk, CMP_T, 0, s>>>(mode, order, active, n, tmp, order_bstart, out, bstart);
+    return;
+  }
   int nblk = (int)((n + PART_TILE - 1) / PART_TILE);
   if (nblk == 0) nblk = 1;
   k_part_count<<<nblk, PART_T, 0, s>>>(mode, order, binid, active, n, tmp, nblk);

@@ -877,7 +877,7 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   fill<int64_t>(big_tsz, n, 0, s);
   k_classify<<<nblk(n), 256, 0, s>>>(cb.ecnt, cb.i32_scalar, n, LT, cb.big_list, cb.nbig, big_tsz, dense_list,
                                      cb.tier);
-  partition_bins(0, nullptr, cb.tier, nullptr, n, cb.tlist, cb.tstart, cb.part_tmp, s);
+  partition_bins(0, nullptr, nullptr, cb.tier, nullptr, n, cb.tlist, cb.tstart, cb.part_tmp, s);
   scan_exclusive<int64_t, int64_t>(big_tsz, cb.big_tab, n, cb.i64_tmp + n + 1, cb.big_tab + n, s);
   // pinned readback: [0] C, [1..2] big and dense row counts, [3..] tier starts,
   // then (8-byte aligned) the overflow slots of the big rows

@@ -345,6 +345,32 @@ def test_phase_contract_huge_row(engine):
     assert C == rC and np.array_equal(end, rend) and np.array_equal(nb, rnb) and np.array_equal(ww, rww)
 
 
+def test_phase_contract_wide_keys(engine):
+    # more than 2^21 communities: the dense rows' key-range digit is 10 bits wide
+    say("phase contract wide keys")
+    n = 2_200_000
+    rng = np.random.default_rng(11)
+    a = [np.arange(n)]
+    b = [(np.arange(n) + 1) % n]
+    for hub, k in ((0, 60000), (1_000_000, 30000)):
+        nb = rng.choice(np.arange(hub + 2, hub + 900_000), k, replace=False)
+        a.append(np.full(k, hub))
+        b.append(nb)
+    a, b = np.concatenate(a), np.concatenate(b)
+    src, dst = np.concatenate([a, b]), np.concatenate([b, a])
+    o = np.lexsort((dst, src))
+    src, dst = src[o], dst[o]
+    g = gala.Graph(np.cumsum(np.bincount(src, minlength=n)).astype(np.int64), dst.astype(np.int32))
+    comm = np.arange(n, dtype=np.int32)
+    comm[1:50] = 0
+    comm[1_000_001:1_000_010] = 1_000_000
+    nid, C, end, nb, ww = engine.phase_contract(g, comm)
+    assert C > 1 << 21
+    rnid, rC, rend, rnb, rww = oracle.contract(g.vertex_num, g.degrees, g.neighbors,
+                                               np.ones(len(g.neighbors), np.int64), comm)
+    assert C == rC and np.array_equal(end, rend) and np.array_equal(nb, rnb) and np.array_equal(ww, rww)
+
+
 def test_star_graph_full_run():
     say("star full run")
     g = star_plus_random(30000, 40000, 3)
