@@ -466,7 +466,7 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   HIPOK(hipStreamSynchronize(s));
   for (int b = 0; b < NBINS; b++) caps[b] = hb[b + 1] - hb[b];
   *nitems = hb[NBINS];
-  launch_init_huge<T>(g, st, ctx->order, ctx->bstart_full, caps[7] + caps[8], s);
+  launch_init_rows<T>(g, st, ctx->order, ctx->bstart_full, caps, s);
   const int nh = hb[NBINS] - hb[gb];
   if (nh > 0) {
     int rc = grow_hub(ctx, hs[0], T::FP);

@@ -111,13 +111,14 @@ struct BatchDev {
   int32_t* nlive;                // graphs still iterating after the latest sweep
 };
 
-// K, Self, Tot = K, size = 1, prev = cur = v, degree bins; rows of 4096 entries
-// and more get K and Self from launch_init_huge, once the bins are partitioned.
+// K, Self, Tot = K, size = 1, prev = cur = v, degree bins; K and Self of the
+// rows of 8 entries and more come from launch_init_rows, once the bins are
+// partitioned (order, bstart: the bin-grouped vertices; caps: host bin sizes).
 template <class T>
 void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s);
 template <class T>
-void launch_init_huge(const DevGraph<T>& g, DevState<T>& st, const int32_t* order, const int32_t* bstart, int cap,
-                      hipStream_t s);
+void launch_init_rows(const DevGraph<T>& g, DevState<T>& st, const int32_t* order, const int32_t* bstart,
+                      const int32_t* caps, hipStream_t s);
 
 // decide-and-move for one bin over front[bstart[b] .. bstart[b+1]) (device counts);
 // `cap` = host upper bound of that range (used only to size the grid).

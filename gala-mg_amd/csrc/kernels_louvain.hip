@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
       st.moved_it[v] = -1;
       st.nbr_moved[v] = 0.f;
     }
-    if (d > 8) continue;  // longer rows: k_init_wide, k_init_huge
+    if (d >= 8) continue;  // bins 2 and up: launch_init_rows, once the bins are partitioned
     tt k = 0, self = 0;
     for (int64_t j = o; j < e; j++) {
       tt w = (tt)load_w<T>(g.w, j);
@@ -215,15 +215,18 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
   }
 }
 
-// K and Self of the rows with LO < degree <= HI, W lanes per row.
-template <int W, int LO, int HI, class T>
-__global__ __launch_bounds__(256) void k_init_wide(DevGraph<T> g, DevState<T> st) {
+// K and Self of the rows of degree bins BLO .. BHI-1 (order[bstart[BLO] ..
+// bstart[BHI])), W lanes per row.
+template <int W, int BLO, int BHI, class T>
+__global__ __launch_bounds__(256) void k_init_wide(DevGraph<T> g, DevState<T> st, const int32_t* __restrict__ order,
+                                                    const int32_t* __restrict__ bstart) {
   using tt = typename T::tot_t;
   const int lane = threadIdx.x & (W - 1);
   const int ng = gridDim.x * (blockDim.x / W);
-  for (int v = (blockIdx.x * blockDim.x + threadIdx.x) / W; v < g.n; v += ng) {  // uniform per group
-    int64_t o = g.off[v], e = g.end[v];
-    if (e - o <= LO || e - o > HI) continue;
+  const int lo = bstart[BLO], hi = bstart[BHI];
+  for (int i = lo + (blockIdx.x * blockDim.x + threadIdx.x) / W; i < hi; i += ng) {  // uniform per group
+    const int v = order[i];
+    const int64_t o = g.off[v], e = g.end[v];
     tt k = 0, self = 0;
     for (int64_t j = o + lane; j < e; j += W) {
       tt w = (tt)load_w<T>(g.w, j);
@@ -286,11 +289,15 @@ __global__ __launch_bounds__(1024) void k_init_huge(DevGraph<T> g, DevState<T> s
   }
 }
 
+// rows of degree bins 2-3 (8..31 entries): 16 lanes each; bins 4-6 (32..4095):
+// one wave each; bins 7-8: one workgroup each
 template <class T>
-void launch_init_huge(const DevGraph<T>& g, DevState<T>& st, const int32_t* order, const int32_t* bstart, int cap,
-                      hipStream_t s) {
-  if (cap <= 0) return;
-  k_init_huge<T><<<std::min(cap, 2048), 1024, 0, s>>>(g, st, order, bstart);
+void launch_init_rows(const DevGraph<T>& g, DevState<T>& st, const int32_t* order, const int32_t* bstart,
+                      const int32_t* caps, hipStream_t s) {
+  const int c23 = caps[2] + caps[3], c46 = caps[4] + caps[5] + caps[6], c78 = caps[7] + caps[8];
+  if (c23 > 0) k_init_wide<16, 2, 4, T><<<std::min((c23 + 15) / 16, 16384), 256, 0, s>>>(g, st, order, bstart);
+  if (c46 > 0) k_init_wide<64, 4, 7, T><<<std::min((c46 + 3) / 4, 8192), 256, 0, s>>>(g, st, order, bstart);
+  if (c78 > 0) k_init_huge<T><<<std::min(c78, 2048), 1024, 0, s>>>(g, st, order, bstart);
 }
 
 template <class T>
@@ -298,10 +305,6 @@ void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s) {
   if (g.n <= 0) return;
   int nb = std::min((g.n + 255) / 256, 16384);
   k_init_small<T><<<nb, 256, 0, s>>>(g, st);
-  // rows of 9..64 entries, 16 lanes each; 65..4095 entries, one wave each;
-  // 4096 and up: launch_init_huge
-  k_init_wide<16, 8, 64, T><<<std::min((g.n + 15) / 16, 16384), 256, 0, s>>>(g, st);
-  k_init_wide<64, 64, 4095, T><<<std::min((g.n + 3) / 4, 8192), 256, 0, s>>>(g, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -2817,8 +2820,8 @@ void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t 
 #define GALA_INST(T)                                                                              \
   template void launch_init<T>(const DevGraph<T>&, DevState<T>&, hipStream_t);                    \
   template void launch_build_cv<T>(DevState<T>&, int32_t, hipStream_t);                           \
-  template void launch_init_huge<T>(const DevGraph<T>&, DevState<T>&, const int32_t*, const int32_t*, int,   \
-                                    hipStream_t);                                                  \
+  template void launch_init_rows<T>(const DevGraph<T>&, DevState<T>&, const int32_t*, const int32_t*,         \
+                                    const int32_t*, hipStream_t);                                  \
   template void launch_decide<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,           \
                                  const int32_t*, int, typename T::tot_t, hipStream_t, int);       \
   template void launch_decide_first<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,     \
