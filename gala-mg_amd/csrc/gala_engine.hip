@@ -266,7 +266,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->c_tier, N); OWN(ctx->c_tlist, N); OWN(ctx->c_tstart, NBINS + 1);
   OWN(ctx->c_ecnt, N + 1); OWN(ctx->c_eoff, N + 1); OWN(ctx->c_ecur, N);
   OWN(ctx->c_big_tab, N + 1); OWN(ctx->c_i64_tmp, 2 * (N + 1) + scan_tmp_elems(N));
-  OWN(ctx->c_i64_scalar, 4);
+  OWN(ctx->c_i64_scalar, 16);
   if (!ctx->ctr_h) {
     hipError_t e = hipHostMalloc((void**)&ctx->ctr_h, 2 * sizeof(Counters), hipHostMallocDefault);
     if (e != hipSuccess) { ctx->last_hip = e; free_all(ctx); return GALA_E_OOM; }
@@ -368,6 +368,14 @@ __global__ void k_offsets_from_incl(const int64_t* __restrict__ end_incl, int32_
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x)
     off[i] = i == 0 ? 0 : end_incl[i - 1];
 }
+// gathers the round set-up readback into one 72-byte block (see round_setup)
+__global__ void k_setup_stage(const int32_t* __restrict__ bstart, const int64_t* __restrict__ slots,
+                              int64_t* __restrict__ stage) {
+  const int t = threadIdx.x;
+  if (t <= NBINS) ((int32_t*)stage)[t] = bstart[t];
+  if (t == 0) stage[8] = *slots;
+}
+
 __global__ void k_iota(int32_t* __restrict__ p, int32_t n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
 }
@@ -459,11 +467,12 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   int64_t* pos = ctx->hub_tmp + nn + 1;
   int64_t* tot = ctx->scan64_tmp;
   scan_exclusive<int64_t, int64_t>(ctx->hub_tmp, pos, nn, ctx->c_i64_tmp, tot, s);
-  int32_t* hb = (int32_t*)ctx->hpin;  // NBINS + 1 bin starts
-  int64_t* hs = (int64_t*)(ctx->hpin + 64);  // hub-table slots
-  HIPOK(hipMemcpyAsync(hb, ctx->bstart_full, sizeof(int32_t) * (NBINS + 1), hipMemcpyDeviceToHost, s));
-  HIPOK(hipMemcpyAsync(hs, tot, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  // one transfer: bytes [0, 40) the NBINS + 1 bin starts, [64, 72) the hub-table slots
+  k_setup_stage<<<1, 64, 0, s>>>(ctx->bstart_full, tot, ctx->c_i64_scalar);
+  HIPOK(hipMemcpyAsync(ctx->hpin, ctx->c_i64_scalar, 72, hipMemcpyDeviceToHost, s));
   HIPOK(hipStreamSynchronize(s));
+  const int32_t* hb = (const int32_t*)ctx->hpin;
+  const int64_t* hs = (const int64_t*)(ctx->hpin + 64);
   for (int b = 0; b < NBINS; b++) caps[b] = hb[b + 1] - hb[b];
   *nitems = hb[NBINS];
   launch_init_rows<T>(g, st, ctx->order, ctx->bstart_full, caps, s);

@@ -246,7 +246,7 @@ struct ContractBufs {
   int32_t* i32_tmp;
   int64_t* i64_tmp;
   int32_t* i32_scalar;  // [4]
-  int64_t* i64_scalar;  // [4]
+  int64_t* i64_scalar;  // [16] staging of the plan readback
   // communities grouped by row length tier (L <= 64, <= 512, <= 2048, <= 4096)
   int8_t* tier;       // [n]
   int32_t* tlist;     // [n]

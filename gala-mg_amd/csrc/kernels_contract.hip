@@ -850,6 +850,19 @@ static inline int nblk(int64_t n, int per = 256, int cap = 8192) {
   return (int)std::min<int64_t>(b, cap);
 }
 
+// gathers the contraction plan readback into one block (layout: contract_prepare)
+template <int SLOT64>
+__global__ void k_plan_stage(const int32_t* __restrict__ C, const int32_t* __restrict__ nbig,
+                             const int32_t* __restrict__ tstart, const int64_t* __restrict__ slots,
+                             int64_t* __restrict__ stage) {
+  int32_t* s32 = (int32_t*)stage;
+  const int t = threadIdx.x;
+  if (t == 0) s32[0] = *C;
+  if (t < 2) s32[1 + t] = nbig[t];
+  if (t <= NBINS) s32[3 + t] = tstart[t];
+  if (t == 0) stage[SLOT64] = *slots;
+}
+
 template <class T>
 int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& cb, ContractPlan* plan,
                      hipStream_t s) {
@@ -884,11 +897,8 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   int32_t* hp32 = (int32_t*)cb.hpin;
   int64_t* hp64 = (int64_t*)cb.hpin;
   constexpr int SLOT64 = (3 + NBINS + 1 + 1) / 2;
-  if (hipMemcpyAsync(hp32, cb.i32_scalar, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-  if (hipMemcpyAsync(hp32 + 1, cb.nbig, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-  if (hipMemcpyAsync(hp32 + 3, cb.tstart, sizeof plan->tstart, hipMemcpyDeviceToHost, s) != hipSuccess)
-    return -1;
-  if (hipMemcpyAsync(hp64 + SLOT64, cb.big_tab + n, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+  k_plan_stage<SLOT64><<<1, 64, 0, s>>>(cb.i32_scalar, cb.nbig, cb.tstart, cb.big_tab + n, cb.i64_scalar);
+  if (hipMemcpyAsync(hp64, cb.i64_scalar, sizeof(int64_t) * (SLOT64 + 1), hipMemcpyDeviceToHost, s) != hipSuccess)
     return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
   const int32_t C = hp32[0];

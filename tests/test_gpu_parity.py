@@ -371,6 +371,44 @@ def test_phase_contract_wide_keys(engine):
     assert C == rC and np.array_equal(end, rend) and np.array_equal(nb, rnb) and np.array_equal(ww, rww)
 
 
+def test_single_engine_edge_cases():
+    # degree-0 vertices (interleaved and trailing), one edge, a self-loop,
+    # components that never touch, a path: every per-iteration vector, Q and
+    # the final partition equal the oracle's
+    say("single-graph edge cases")
+    cases = [
+        ("one edge", gala.Graph(np.array([1, 2], np.int64), np.array([1, 0], np.int32))),
+        ("star + isolated", gala.Graph(np.concatenate([[300], 300 + np.arange(1, 301), [600, 600]]).astype(np.int64),
+                                       np.concatenate([np.arange(1, 301), np.zeros(300)]).astype(np.int32))),
+        ("triangle + self-loop", gala.Graph(np.array([3, 5, 7], np.int64),
+                                            np.array([0, 1, 2, 0, 2, 0, 1], np.int32))),
+    ]
+    # isolated vertices between two 6-cliques, and a 40-vertex path
+    rows = [[] for _ in range(15)]
+    for base in (0, 8):
+        for a in range(base, base + 6):
+            rows[a] = [b for b in range(base, base + 6) if b != a]
+    cases.append(("cliques + gaps", gala.Graph(np.cumsum([len(r) for r in rows]).astype(np.int64),
+                                               np.concatenate([np.array(r, np.int32) for r in rows]))))
+    prow = [[v - 1] * (v > 0) + [v + 1] * (v < 39) for v in range(40)]
+    cases.append(("path", gala.Graph(np.cumsum([len(r) for r in prow]).astype(np.int64),
+                                     np.concatenate([np.array(r, np.int32) for r in prow]))))
+    for name, g in cases:
+        say(f"  {name}")
+        eng = gala.Engine(dump_iterations=True)
+        full_trace_compare(eng, g)
+        eng.close()
+    # no edge at all: m2 = 0 is rejected by both
+    g = gala.Graph(np.zeros(5, np.int64), np.zeros(0, np.int32))
+    eng = gala.Engine()
+    eng.set_graph(g)
+    with pytest.raises(gala.GalaError):
+        eng.run()
+    eng.close()
+    with pytest.raises(RuntimeError):
+        oracle.louvain(g.vertex_num, g.degrees, g.neighbors)
+
+
 def test_star_graph_full_run():
     say("star full run")
     g = star_plus_random(30000, 40000, 3)
