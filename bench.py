@@ -82,15 +82,15 @@ KERNEL_FIRST = [
 # beside the live timing of the same kernel.
 PMC_FETCH_FACTOR, PMC_WRITE_FACTOR = 2.00, 1.00
 PMC_KIB = {  # kernel: (FETCH_SIZE KiB, WRITE_SIZE KiB) per dispatch
-    "k_decide_hlead<256, 16, 8, false, IntW>": (190296.9, 14015.2),
-    "k_decide_hlead<1024, 64, 8, false, IntW>": (105946.1, 1917.0),
-    "k_decide_hlead<1, 16, 8, true, IntW>": (105184.6, 9030.2),
-    "k_decide_blead<256, 8192, 16, 1, IntW>": (124321.1, 626.7),
-    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50874.6, 83.4),
-    "k_decide_tpv<31, false, IntW>": (102853.8, 13236.9),
-    "k_decide_tpv<15, false, IntW>": (86118.8, 13930.5),
-    "k_decide_tpv<7, false, IntW>": (64311.2, 13582.8),
-    "k_decide_tpv<3, false, IntW>": (25647.9, 6774.6),
+    "k_decide_hlead<256, 16, 8, false, IntW>": (190332.5, 14014.0),
+    "k_decide_hlead<1024, 64, 8, false, IntW>": (105996.4, 1916.3),
+    "k_decide_hlead<1, 16, 8, true, IntW>": (105187.1, 9035.0),
+    "k_decide_blead<256, 8192, 16, 1, IntW>": (124006.7, 628.3),
+    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50952.5, 83.3),
+    "k_decide_tpv<31, false, IntW>": (102923.1, 13241.7),
+    "k_decide_tpv<15, false, IntW>": (86101.9, 13930.5),
+    "k_decide_tpv<7, false, IntW>": (64309.7, 13583.8),
+    "k_decide_tpv<3, false, IntW>": (25655.0, 6775.2),
 }
 
 
