@@ -417,6 +417,32 @@ def test_star_graph_full_run():
     eng.close()
 
 
+def test_full_size_chung_lu():
+    # the 4.0M-vertex / 34.4M-edge power-law workload at full size: final
+    # partition, Q and per-round community counts equal the oracle's, and Q is
+    # the modularity of the returned partition (exact integer sums)
+    say("full-size Chung-Lu 4.0M")
+    g = gala.gen_chung_lu(4_000_000, 2.1, 17.35, 15000.0, 0)
+    eng = gala.Engine()
+    eng.set_graph(g)
+    res = eng.run()
+    comm = eng.communities(g.vertex_num)
+    eng.close()
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    ref = oracle.louvain(g.vertex_num, g.degrees, g.neighbors)
+    assert np.array_equal(comm, ref["final_comm"])
+    assert res["modularity"] == ref["Q"]
+    assert list(res["round_nc"]) == list(ref["round_nc"])
+    off = g.offsets()
+    deg = np.diff(off)
+    src = np.repeat(np.arange(g.vertex_num), deg)
+    m2 = int(len(g.neighbors))
+    s_in = int(np.count_nonzero(comm[g.neighbors] == comm[src]))
+    tot = np.bincount(comm, weights=deg, minlength=g.vertex_num).astype(np.int64)
+    q = s_in / m2 - float(int((tot * tot).sum())) / (m2 * m2)
+    assert q == res["modularity"]
+
+
 @pytest.mark.parametrize("scale", [12, 15])
 def test_weighted_fp32_rmat(scale):
     # fp32 weights U[0.5, 2], FP64 accumulation: final Q within 1e-9 of the FP64 oracle
