@@ -1770,8 +1770,9 @@ __global__ __launch_bounds__(256) void k_hub_scan(DevGraph<T> g, DevState<T> st,
 
 // Hubs split by community: workgroup (hub, p) of P streams the whole row and
 // aggregates only the entries whose community c has part(c, P) == p, in an
-// LT-slot LDS table (P is chosen so that a part expects at most 3/8 LT
-// distinct communities), then evaluates them: no global table, no global
+// LT-slot LDS table (P is chosen so that a part expects at most 5/8 LT
+// distinct communities: fewer parts, fewer re-reads of the row, against
+// longer probes), then evaluates them: no global table, no global
 // atomics; the row and the prev[] lines it gathers are re-read from L2 by the
 // P workgroups.  A part whose table still fills up (an adversarial skew of the
 // hash) flags its hub in fail[]; those hubs are redone by the global-table
@@ -2204,7 +2205,7 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
       // one partition per hub, so every hub with more distinct communities
       // than the table holds takes the fallback (kept for its tests).
       // 6: every hub split by community.
-      const int PART_KEYS = variant == 5 ? INT_MAX : HUB_LT / 8 * 3;
+      const int PART_KEYS = variant == 5 ? INT_MAX : HUB_LT / 8 * 5;
       const int above = variant == 6 ? 0 : HUB_LT;
       HubPart<T>* hp = (HubPart<T>*)st.hub_part;
       if (above > 0)
