@@ -154,6 +154,37 @@ __global__ void k_classify(const int64_t* __restrict__ ecnt, const int32_t* __re
 // ---------------------------------------------------------------------------
 // staging: (label of the neighbour, weight) into the owner's segment.
 // ---------------------------------------------------------------------------
+// entries [o, o + d) of the CSR to skey/sval[base ..] as (label[nbr], w), NT
+// lanes from `lane`: the row and label loads of STG_U entries per lane are
+// issued before their stores (the stores may alias the CSR for the compiler)
+constexpr int STG_U = 4;
+template <int NT, class T>
+__device__ __forceinline__ void stage_span(const DevGraph<T>& g, int64_t o, int64_t d, int lane,
+                                           const int32_t* __restrict__ label, int32_t* __restrict__ skey,
+                                           typename T::acc_t* __restrict__ sval, int64_t base) {
+  using acc = typename T::acc_t;
+  for (int64_t j0 = lane; j0 < d; j0 += (int64_t)NT * STG_U) {
+    int u[STG_U];
+    acc w[STG_U];
+#pragma unroll
+    for (int r = 0; r < STG_U; r++) {
+      const int64_t j = j0 + (int64_t)r * NT;
+      u[r] = j < d ? g.nbr[o + j] : 0;
+      w[r] = j < d && g.w ? (acc)g.w[o + j] : acc(1);
+    }
+#pragma unroll
+    for (int r = 0; r < STG_U; r++) u[r] = j0 + (int64_t)r * NT < d ? label[u[r]] : 0;
+#pragma unroll
+    for (int r = 0; r < STG_U; r++) {
+      const int64_t j = j0 + (int64_t)r * NT;
+      if (j < d) {
+        skey[base + j] = u[r];
+        sval[base + j] = w[r];
+      }
+    }
+  }
+}
+
 // rows of the vertices order[bstart[blo] .. bstart[bhi]) (the round's degree
 // bins): NT lanes per vertex, one reservation each
 template <int NT, class T>
@@ -164,7 +195,6 @@ __global__ __launch_bounds__(256) void k_stage_rows(DevGraph<T> g, const int32_t
                                                     unsigned long long* __restrict__ ecur,
                                                     int32_t* __restrict__ skey,
                                                     typename T::acc_t* __restrict__ sval) {
-  using acc = typename T::acc_t;
   constexpr int G = 256 / NT;  // vertices per block step
   const int lo = bstart[blo], hi = bstart[bhi];
   const int lane = threadIdx.x % NT;
@@ -183,10 +213,7 @@ __global__ __launch_bounds__(256) void k_stage_rows(DevGraph<T> g, const int32_t
       base = s_b;
       __syncthreads();
     }
-    for (int64_t j = lane; j < d; j += NT) {
-      skey[base + j] = label[g.nbr[o + j]];
-      sval[base + j] = g.w ? (acc)g.w[o + j] : acc(1);
-    }
+    stage_span<NT>(g, o, d, lane, label, skey, sval, base);
   }
 }
 
@@ -203,7 +230,6 @@ __global__ __launch_bounds__(256) void k_stage_chunks(DevGraph<T> g, const int32
                                                       unsigned long long* __restrict__ ecur,
                                                       int32_t* __restrict__ skey,
                                                       typename T::acc_t* __restrict__ sval) {
-  using acc = typename T::acc_t;
   __shared__ int64_t s_b;
   const int i = bstart[blo] + blockIdx.x;
   if (i >= bstart[bhi]) return;  // uniform per workgroup
@@ -216,10 +242,7 @@ __global__ __launch_bounds__(256) void k_stage_chunks(DevGraph<T> g, const int32
     if (threadIdx.x == 0) s_b = eoff[nc] + (int64_t)atomicAdd(&ecur[nc], (unsigned long long)len);
     __syncthreads();
     const int64_t base = s_b;
-    for (int64_t j = threadIdx.x; j < len; j += blockDim.x) {
-      skey[base + j] = label[g.nbr[o + c0 + j]];
-      sval[base + j] = g.w ? (acc)g.w[o + c0 + j] : acc(1);
-    }
+    stage_span<256>(g, o + c0, len, (int)threadIdx.x, label, skey, sval, base);
     __syncthreads();  // s_b is rewritten by the next chunk
   }
 }
