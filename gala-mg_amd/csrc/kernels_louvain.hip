@@ -2608,6 +2608,42 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
 // deltas for hubs per workgroup in LDS first, or rescanning staying hubs
 // whole, both measured slower on the power-law workloads.)
 // ---------------------------------------------------------------------------
+// One mover's row, NT lanes from `lane`: returns the lane's part of the
+// mover's new In (entries whose {cur, prev} pair has cur == c) and adds each
+// staying neighbour's change.  The row loads and pair gathers of MV_U entries
+// per lane are issued before their atomics (which may alias them for the
+// compiler).
+constexpr int MV_U = 4;
+template <int NT, class T>
+__device__ __forceinline__ typename T::tot_t moved_row_span(const DevGraph<T>& g, DevState<T>& st, int64_t o,
+                                                            int d, int lane, int c, int p) {
+  using tt = typename T::tot_t;
+  tt x = 0;
+  for (int j0 = lane; j0 < d; j0 += NT * MV_U) {
+    int u[MV_U];
+    tt w[MV_U];
+    int2 cpu[MV_U];
+#pragma unroll
+    for (int r = 0; r < MV_U; r++) {
+      const int j = j0 + r * NT;
+      u[r] = j < d ? g.nbr[o + j] : -1;
+      w[r] = j < d ? (tt)load_w<T>(g.w, o + j) : tt(0);
+    }
+#pragma unroll
+    for (int r = 0; r < MV_U; r++) cpu[r] = u[r] >= 0 ? st.cp[u[r]] : make_int2(-1, -2);  // {cur[u], prev[u]}
+#pragma unroll
+    for (int r = 0; r < MV_U; r++) {
+      const int cu = cpu[r].x;
+      if (cu == c) x += w[r];
+      if (cu == cpu[r].y) {  // u stayed: its In gains or loses this entry
+        const tt dl = (cu == c ? w[r] : tt(0)) - (cu == p ? w[r] : tt(0));
+        if (dl) atomicAdd(&st.In[u[r]], dl);
+      }
+    }
+  }
+  return x;
+}
+
 template <int W, class T>
 __global__ __launch_bounds__(256) void k_weight_moved(DevGraph<T> g, DevState<T> st,
                                                        const int32_t* __restrict__ order,
@@ -2623,19 +2659,7 @@ __global__ __launch_bounds__(256) void k_weight_moved(DevGraph<T> g, DevState<T>
     if (c == p) continue;  // uniform per group
     const int64_t o = g.off[v];
     const int d = (int)(g.end[v] - o);
-    tt x = 0;
-    for (int j = lane; j < d; j += W) {
-      const int u = g.nbr[o + j];
-      const tt w = (tt)load_w<T>(g.w, o + j);
-      const int2 cpu = st.cp[u];  // {cur[u], prev[u]} in one gather
-      const int cu = cpu.x;
-      if (cu == c) x += w;
-      if (cu == cpu.y) {  // u stayed: its In gains or loses this entry
-        const tt dl = (cu == c ? w : tt(0)) - (cu == p ? w : tt(0));
-        if (dl) atomicAdd(&st.In[u], dl);
-      }
-    }
-    x = sum_xor<tt, W>(x);
+    const tt x = sum_xor<tt, W>(moved_row_span<W, T>(g, st, o, d, lane, c, p));
     if (lane == 0) st.In[v] = x;
   }
 }
@@ -2655,18 +2679,7 @@ __global__ __launch_bounds__(NT) void k_weight_moved_block(DevGraph<T> g, DevSta
     if (c == p) continue;  // uniform per block
     const int64_t o = g.off[v];
     const int d = (int)(g.end[v] - o);
-    tt x = 0;
-    for (int j = threadIdx.x; j < d; j += NT) {
-      const int u = g.nbr[o + j];
-      const tt w = (tt)load_w<T>(g.w, o + j);
-      const int2 cpu = st.cp[u];
-      const int cu = cpu.x;
-      if (cu == c) x += w;
-      if (cu == cpu.y) {
-        const tt dl = (cu == c ? w : tt(0)) - (cu == p ? w : tt(0));
-        if (dl) atomicAdd(&st.In[u], dl);
-      }
-    }
+    tt x = moved_row_span<NT, T>(g, st, o, d, (int)threadIdx.x, c, p);
     x = sum_xor<tt, 64>(x);
     if (lane == 0) r[wid] = x;
     __syncthreads();
