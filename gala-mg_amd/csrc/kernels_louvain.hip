@@ -2396,6 +2396,35 @@ __device__ __forceinline__ typename T::tot_t read_min_tot(const Counters* c) {
   return (typename T::tot_t)c->min_tot;
 }
 
+// One row's part of In (entries whose neighbour is in community c) and of the
+// moved-neighbour count, NT lanes from `lane`: the row loads and community
+// gathers of WR_U entries per lane are issued together; each lane adds its
+// entries in row order, so FP sums are unchanged.
+constexpr int WR_U = 4;
+template <int NT, class T>
+__device__ __forceinline__ void row_in_span(const DevGraph<T>& g, const DevState<T>& st, int64_t o, int d,
+                                            int lane, int c, typename T::tot_t& x, int& mvn) {
+  using tt = typename T::tot_t;
+  for (int j0 = lane; j0 < d; j0 += NT * WR_U) {
+    int u[WR_U], cu[WR_U];
+    tt w[WR_U];
+#pragma unroll
+    for (int r = 0; r < WR_U; r++) {
+      const int j = j0 + r * NT;
+      u[r] = j < d ? g.nbr[o + j] : -1;
+      w[r] = j < d ? (tt)load_w<T>(g.w, o + j) : tt(0);
+    }
+#pragma unroll
+    for (int r = 0; r < WR_U; r++) cu[r] = u[r] >= 0 ? st.cur[u[r]] : -1;
+#pragma unroll
+    for (int r = 0; r < WR_U; r++) {
+      if (u[r] < 0) continue;
+      if (cu[r] == c) x += w[r];
+      if (st.track) mvn += st.moved_it[u[r]] == st.iter;
+    }
+  }
+}
+
 template <int W, class T>
 __global__ __launch_bounds__(256) void k_weight_group(DevGraph<T> g, DevState<T> st,
                                                        const int32_t* __restrict__ order,
@@ -2413,11 +2442,7 @@ __global__ __launch_bounds__(256) void k_weight_group(DevGraph<T> g, DevState<T>
     const int c = st.cur[v];
     tt x = 0;
     int mvn = 0;
-    for (int j = lane; j < d; j += W) {
-      const int u = g.nbr[o + j];
-      if (st.cur[u] == c) x += (tt)load_w<T>(g.w, o + j);
-      if (st.track) mvn += st.moved_it[u] == st.iter;
-    }
+    row_in_span<W, T>(g, st, o, d, lane, c, x, mvn);
     x = sum_xor<tt, W>(x);
     if (st.track) mvn = sum_xor<int, W>(mvn);
     if (lane == 0) {
@@ -2577,11 +2602,7 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
     const int c = st.cur[v];
     tt x = 0;
     int mvn = 0;
-    for (int j = threadIdx.x; j < d; j += NT) {
-      const int u = g.nbr[o + j];
-      if (st.cur[u] == c) x += (tt)load_w<T>(g.w, o + j);
-      if (st.track) mvn += st.moved_it[u] == st.iter;
-    }
+    row_in_span<NT, T>(g, st, o, d, (int)threadIdx.x, c, x, mvn);
     x = sum_xor<tt, 64>(x);
     mvn = sum_xor<int, 64>(mvn);
     if (lane == 0) { r[wid] = x; rm[wid] = mvn; }
