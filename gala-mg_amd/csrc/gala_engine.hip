@@ -151,8 +151,9 @@ struct gala_ctx {
   // decide kernel variant per degree bin, general and first sweeps; the
   // defaults are what gala_autotune picks on the 4M-vertex power-law graph
   // (one thread per vertex below degree 32, the pipelined leader-hash kernels
-  // for 32..511, leader evaluation in the workgroup kernels for 512..8191)
-  int dvar[NBINS] = {3, 3, 3, 3, 6, 3, 1, 1, 4};
+  // for 32..511, leader evaluation in the workgroup kernels for 512..8191,
+  // every hub of 8192+ split by community over workgroups)
+  int dvar[NBINS] = {3, 3, 3, 3, 6, 3, 1, 1, 6};
   int fvar[NBINS] = {1, 2, 2, 2, 2, 1, 0, 0, 0};
   // weight update: the incremental integer path with narrow lane groups
   // (variant 2); the dirty-row path (FP, tracked runs) reads variant 2 as one
