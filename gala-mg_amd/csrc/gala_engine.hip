@@ -85,6 +85,8 @@ struct gala_ctx {
   int64_t* hub_tmp = nullptr;
   int32_t* hub_keys = nullptr;
   void* hub_vals = nullptr;
+  int32_t* hub_rc = nullptr;  // resolved split-hub rows (hub_cap slots each)
+  void* hub_rw = nullptr;
   int64_t hub_cap = 0;
   int32_t* hub_plan = nullptr;  // split hub decide scratch
   void* hub_part = nullptr;
@@ -192,7 +194,10 @@ void free_all(gala_ctx* ctx) {
   free_batch(ctx);
   if (ctx->hub_keys) (void)hipFree(ctx->hub_keys);
   if (ctx->hub_vals) (void)hipFree(ctx->hub_vals);
+  if (ctx->hub_rc) (void)hipFree(ctx->hub_rc);
+  if (ctx->hub_rw) (void)hipFree(ctx->hub_rw);
   ctx->hub_keys = nullptr; ctx->hub_vals = nullptr; ctx->hub_cap = 0;
+  ctx->hub_rc = nullptr; ctx->hub_rw = nullptr;
   if (ctx->arena_keys) (void)hipFree(ctx->arena_keys);
   if (ctx->arena_vals) (void)hipFree(ctx->arena_vals);
   ctx->arena_keys = nullptr; ctx->arena_vals = nullptr;
@@ -286,13 +291,21 @@ int grow_hub(gala_ctx* ctx, int64_t slots, bool fp) {
   if (slots <= ctx->hub_cap) return GALA_OK;
   if (ctx->hub_keys) (void)hipFree(ctx->hub_keys);
   if (ctx->hub_vals) (void)hipFree(ctx->hub_vals);
-  ctx->hub_keys = nullptr; ctx->hub_vals = nullptr;
+  if (ctx->hub_rc) (void)hipFree(ctx->hub_rc);
+  if (ctx->hub_rw) (void)hipFree(ctx->hub_rw);
+  ctx->hub_keys = nullptr; ctx->hub_vals = nullptr; ctx->hub_rc = nullptr; ctx->hub_rw = nullptr;
   int64_t cap = slots + slots / 4;
   hipError_t e = dalloc(&ctx->hub_keys, cap);
   if (e == hipSuccess) {
     char* p;
     e = dalloc(&p, cap * (fp ? 8 : 4));
     ctx->hub_vals = p;
+  }
+  if (e == hipSuccess) e = dalloc(&ctx->hub_rc, cap);
+  if (e == hipSuccess) {
+    char* p;
+    e = dalloc(&p, cap * (fp ? 8 : 4));
+    ctx->hub_rw = p;
   }
   if (e != hipSuccess) { ctx->last_hip = e; ctx->hub_cap = 0; return GALA_E_OOM; }
   // the hub kernels keep overflow slices all-zero between uses
@@ -429,6 +442,7 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.size = ctx->size; st.dsize = ctx->dsize; st.prev = ctx->prev; st.cur = ctx->cur;
   st.active = ctx->active; st.dirty = ctx->dirty; st.binid = ctx->binid; st.hub_off = ctx->hub_off;
   st.hub_keys = ctx->hub_keys; st.hub_vals = (typename T::acc_t*)ctx->hub_vals;
+  st.hub_rc = ctx->hub_rc; st.hub_rw = (typename T::acc_t*)ctx->hub_rw;
   st.hub_plan = ctx->hub_plan; st.hub_part = ctx->hub_part; st.hub_fail = ctx->hub_fail;
   st.ctr = ctx->ctr;
   st.track = 0;

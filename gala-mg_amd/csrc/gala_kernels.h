@@ -69,6 +69,10 @@ struct DevState {
   int64_t* hub_off;  // per-vertex offset into the global hash arena (bin 8), in slots
   int32_t* hub_keys;
   typename T::acc_t* hub_vals;
+  // split hubs: each row's (community of the neighbour, weight) pairs, resolved
+  // once per sweep into the hub's arena-sized slice (not kept zero)
+  int32_t* hub_rc = nullptr;
+  typename T::acc_t* hub_rw = nullptr;
   int32_t* hub_plan = nullptr;  // split hub decide: chunk starts (hub_plan_elems)
   void* hub_part = nullptr;     // split hub decide: HubPart<T>[hub_part_elems]
   uint8_t* hub_fail = nullptr;  // community-split hub decide: per-hub overflow flags

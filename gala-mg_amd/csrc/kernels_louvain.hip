@@ -1782,6 +1782,50 @@ __device__ __forceinline__ int hub_part_of(int c, int P) {
   return (int)((x >> 8) % (unsigned)P);
 }
 
+// The rows of the hubs with at least HUB_RESOLVE_P parts, resolved once for
+// their P part workgroups: hub_rc/hub_rw[hub_off[v] + j] = (prev[nbr[j]],
+// w[j]); hubs with fewer parts gather in k_hub_part directly.  Workgroup
+// (x, y) covers hubs x, x + gridDim.x, ... and every gridDim.y-th run of 1024
+// entries of their rows.
+constexpr int HUB_RESOLVE_P = 3;
+template <class T>
+__global__ __launch_bounds__(1024) void k_hub_resolve(DevGraph<T> g, DevState<T> st,
+                                                       const int32_t* __restrict__ front,
+                                                       const int32_t* __restrict__ bstart, int bin,
+                                                       const int32_t* __restrict__ plan) {
+  using acc = typename T::acc_t;
+  constexpr int NT = 1024, U = 4;
+  const int lo = bstart[bin], nh = bstart[bin + 1] - lo;
+  const int32_t* tstart = plan + nh + 1;
+  for (int i = blockIdx.x; i < nh; i += gridDim.x) {
+    if (tstart[i + 1] - tstart[i] < HUB_RESOLVE_P) continue;  // uniform per workgroup
+    const int v = front[lo + i];
+    const int64_t o = g.off[v];
+    const int d = (int)(g.end[v] - o);
+    const int64_t rb = st.hub_off[v];
+    for (int j0 = blockIdx.y * NT + threadIdx.x; j0 < d; j0 += gridDim.y * NT * U) {
+      int u[U];
+      acc w[U];
+#pragma unroll
+      for (int r = 0; r < U; r++) {
+        const int j = j0 + r * gridDim.y * NT;
+        u[r] = j < d ? g.nbr[o + j] : -1;
+        w[r] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+      }
+#pragma unroll
+      for (int r = 0; r < U; r++) u[r] = u[r] >= 0 ? st.prev[u[r]] : -1;
+#pragma unroll
+      for (int r = 0; r < U; r++) {
+        const int j = j0 + r * gridDim.y * NT;
+        if (j < d) {
+          st.hub_rc[rb + j] = u[r];
+          st.hub_rw[rb + j] = w[r];
+        }
+      }
+    }
+  }
+}
+
 template <int LT, class T>
 __global__ __launch_bounds__(1024) void k_hub_part(DevGraph<T> g, DevState<T> st, const int32_t* __restrict__ front,
                                                     const int32_t* __restrict__ bstart, int bin,
@@ -1809,19 +1853,30 @@ __global__ __launch_bounds__(1024) void k_hub_part(DevGraph<T> g, DevState<T> st
     const tt m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.end[v] - o);
+    const bool res = P >= HUB_RESOLVE_P;  // uniform per workgroup
+    const int64_t rb = res ? st.hub_off[v] : 0;  // the row resolved by k_hub_resolve
     if (threadIdx.x == 0) s_fail = 0;
     __syncthreads();  // also orders the table clears before the inserts
     for (int j0 = threadIdx.x; j0 < d; j0 += NT * U) {
       int cu[U];
       acc wv[U];
+      if (res) {
 #pragma unroll
-      for (int r = 0; r < U; r++) {
-        const int j = j0 + r * NT;
-        cu[r] = j < d ? g.nbr[o + j] : -1;
-        wv[r] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+        for (int r = 0; r < U; r++) {
+          const int j = j0 + r * NT;
+          cu[r] = j < d ? st.hub_rc[rb + j] : -1;
+          wv[r] = j < d ? st.hub_rw[rb + j] : acc(0);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < U; r++) {
+          const int j = j0 + r * NT;
+          cu[r] = j < d ? g.nbr[o + j] : -1;
+          wv[r] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+        }
+#pragma unroll
+        for (int r = 0; r < U; r++) cu[r] = cu[r] >= 0 ? st.prev[cu[r]] : -1;
       }
-#pragma unroll
-      for (int r = 0; r < U; r++) cu[r] = cu[r] >= 0 ? st.prev[cu[r]] : -1;
 #pragma unroll
       for (int r = 0; r < U; r++) {
         const int cl = cu[r];
@@ -2215,6 +2270,7 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
       const int64_t part_max = variant == 5 ? INT64_MAX : 16ll * PART_KEYS;
       k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, PART_KEYS, nullptr, part_max,
                                         st.hub_fail);
+      k_hub_resolve<T><<<dim3(std::min(cap, 512), 8), 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
       k_hub_part<HUB_LT, T><<<2048, 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, st.hub_fail, m2);
       k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2,
                                                               st.hub_fail);
