@@ -82,15 +82,15 @@ KERNEL_FIRST = [
 # beside the live timing of the same kernel.
 PMC_FETCH_FACTOR, PMC_WRITE_FACTOR = 2.00, 1.00
 PMC_KIB = {  # kernel: (FETCH_SIZE KiB, WRITE_SIZE KiB) per dispatch
-    "k_decide_hlead<256, 16, 8, false, IntW>": (190311.3, 14012.4),
-    "k_decide_hlead<1024, 64, 8, false, IntW>": (105972.9, 1917.5),
-    "k_decide_hlead<1, 16, 8, true, IntW>": (105182.7, 9032.8),
-    "k_decide_blead<256, 8192, 16, 1, IntW>": (124008.8, 627.9),
-    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50969.3, 83.4),
-    "k_decide_tpv<31, false, IntW>": (102899.1, 13238.7),
-    "k_decide_tpv<15, false, IntW>": (86055.8, 13928.1),
-    "k_decide_tpv<7, false, IntW>": (64300.2, 13583.4),
-    "k_decide_tpv<3, false, IntW>": (25651.5, 6775.0),
+    "k_decide_hlead<256, 16, 8, false, IntW>": (190300.5, 14015.4),
+    "k_decide_hlead<1024, 64, 8, false, IntW>": (106035.1, 1917.2),
+    "k_decide_hlead<1, 16, 8, true, IntW>": (105188.1, 9030.6),
+    "k_decide_blead<256, 8192, 16, 1, IntW>": (123872.1, 627.8),
+    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50893.3, 83.4),
+    "k_decide_tpv<31, false, IntW>": (102932.4, 13240.2),
+    "k_decide_tpv<15, false, IntW>": (86090.8, 13929.8),
+    "k_decide_tpv<7, false, IntW>": (64310.8, 13583.2),
+    "k_decide_tpv<3, false, IntW>": (25668.5, 6774.4),
 }
 
 
