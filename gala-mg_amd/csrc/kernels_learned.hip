@@ -2,8 +2,8 @@
 // next decide-and-move sweep will move it.  BF16 operands on the gfx950 MFMA
 // units (v_mfma_f32_16x16x16_bf16) with FP32 accumulation; FP32 master weights
 // updated by Adam.  No counterpart in the reference (the nearest analogue is
-// its sound prune, kernel_functions.cu:944,1028): the plain full sweep stays
-// the judge and certifies every level before it is accepted.
+// its sound prune, kernel_functions.cu:944,1028): the plain full sweep has the
+// last word and certifies every level before it is accepted.
 //
 // MFMA 16x16x16 operand layout used throughout (64 lanes, l = lane):
 //   A (16 x 16, rows x k):  lane l holds A[l % 16][4 (l / 16) + i], i = 0..3

@@ -171,7 +171,7 @@ int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const
  * learned sweep that does not raise Q by min_modularity is certified: the rest
  * of the frontier is swept against the same pre-sweep state, which completes
  * it into the plain sweep, and the plain break/revert rule decides.  A round
- * therefore only ends on a plain sweep's verdict.  Parameter
+ * therefore only ends on the outcome of a plain sweep.  Parameter
  * vector: GALA_MLP_NPARAM floats, W1[16][64] (rows 6..15 zero), b1[64],
  * w2[64], b2.  Features (6 per vertex) are defined in gala_kernels.h. */
 #define GALA_MLP_NFEAT 6
