@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -1013,7 +1014,14 @@ int batch_upload(gala_ctx* ctx, int32_t G, const int32_t* vstart, int32_t n, con
   if (vstart[0] != 0 || vstart[G] != n) return GALA_E_INVALID;
   for (int32_t g = 0; g < G; g++)
     if (vstart[g + 1] < vstart[g]) return GALA_E_INVALID;
-  if (n <= 0) return GALA_E_INVALID;
+  if (n <= 0 || !end_incl || !nbr) return GALA_E_INVALID;
+  if (weight_mode == GALA_W_I32 && !w) return GALA_E_INVALID;
+  // the CSR checks upload() makes, before any row is walked here
+  for (int32_t v = 0; v < n; v++)
+    if (end_incl[v] < (v == 0 ? 0 : end_incl[v - 1])) return GALA_E_FORMAT;
+  if (weight_mode == GALA_W_I32)
+    for (int64_t j = 0; j < end_incl[n - 1]; j++)
+      if (((const int32_t*)w)[j] <= 0) return GALA_E_FORMAT;
   // rows stay inside their graph; per-graph m2 (exact, integer)
   std::vector<int32_t> m2g(G), gid0(n);
   int32_t m2max = 0;
@@ -1328,9 +1336,15 @@ int gala_load_bin(gala_ctx* ctx, const char* path, int weight_mode) {
   int64_t m = 0;
   int rc = gala_read_bin(path, weight_mode, &n, &m, nullptr, nullptr, nullptr);
   if (rc) return rc;
-  std::vector<int64_t> end(n);
-  std::vector<int32_t> nbr(m);
-  std::vector<int32_t> w(weight_mode != GALA_W_NONE ? m : 0);
+  std::vector<int64_t> end;
+  std::vector<int32_t> nbr, w;
+  try {  // host staging of the file: no C++ exception may cross the C ABI
+    end.resize(n);
+    nbr.resize(m);
+    w.resize(weight_mode != GALA_W_NONE ? m : 0);
+  } catch (const std::exception&) {
+    return GALA_E_OOM;
+  }
   rc = gala_read_bin(path, weight_mode, &n, &m, end.data(), nbr.data(), w.empty() ? nullptr : w.data());
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);

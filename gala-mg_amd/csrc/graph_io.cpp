@@ -186,8 +186,10 @@ int gala_read_bin(const char* path, int weight_mode, int32_t* n, int64_t* m, int
     std::fseek(f, 4 + 8ll * (nn - 1), SEEK_SET);
     if (std::fread(&last, 8, 1, f) != 1) { std::fclose(f); return GALA_E_FORMAT; }
   }
-  long long need = 4 + 8ll * nn + 4ll * last + (weight_mode != GALA_W_NONE ? 4ll * last : 0);
-  if (last < 0 || fsz < need) { std::fclose(f); return GALA_E_FORMAT; }
+  // the entry count comes from the file: bound it by the bytes present before
+  // multiplying, so a corrupt count cannot overflow the size test
+  const long long per_entry = weight_mode != GALA_W_NONE ? 8 : 4;
+  if (last < 0 || last > (fsz - 4 - 8ll * nn) / per_entry) { std::fclose(f); return GALA_E_FORMAT; }
   *n = nn;
   *m = last;
   if (!end_incl) { std::fclose(f); return GALA_OK; }

@@ -1,6 +1,7 @@
 """C-ABI boundary checks that need no GPU: libgala.so loads and exports every
 function include/gala.h declares; .bin IO is byte-exact against the
 reference's own preprocess output; text ingest and generators behave."""
+import ctypes
 import os
 import re
 
@@ -108,6 +109,16 @@ def test_truncated_and_malformed_bin(tmp_path, golden):
     bad.write_bytes(bytes(arr))
     with pytest.raises(gala.GalaError):
         gala.Graph.load_bin_graph(str(bad))
+    # an entry count so large that the size test would overflow if it
+    # multiplied before comparing: rejected as a format error, no allocation
+    for last in (2 ** 62, 2 ** 63 - 1, -5):
+        arr = bytearray(src)
+        arr[4 + 8 * 33: 4 + 8 * 34] = last.to_bytes(8, "little", signed=True)
+        bad.write_bytes(bytes(arr))
+        for wm in (gala.W_NONE, gala.W_I32):
+            n, m = ctypes.c_int32(), ctypes.c_int64()
+            rc = L.gala_read_bin(str(bad).encode(), wm, ctypes.byref(n), ctypes.byref(m), None, None, None)
+            assert rc == -2, (last, wm, rc)
     assert L is not None
 
 

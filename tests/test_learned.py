@@ -7,9 +7,6 @@ import pytest
 import gala
 import train_frontier as TF
 
-pytestmark = pytest.mark.gpu
-
-
 def say(msg):
     print(msg, flush=True)
 
@@ -60,6 +57,7 @@ def test_auc_helper():
     assert auc(np.array([0.5, 0.5, 0.5, 0.5]), np.array([0, 1, 0, 1])) == 0.5
 
 
+@pytest.mark.gpu
 def test_logger_features_in_range():
     say("logger")
     eng = gala.Engine()
@@ -71,6 +69,7 @@ def test_logger_features_in_range():
     assert 0 < y.mean() < 1
 
 
+@pytest.mark.gpu
 def test_predict_nobody_equals_plain():
     # a network that predicts no mover leaves every learned sweep empty; each
     # such sweep stalls and its certification sweeps the whole frontier, which
@@ -89,6 +88,7 @@ def test_predict_nobody_equals_plain():
     assert rl["learned_kept"] == 0 and rl["learned_frontier"] > 0
 
 
+@pytest.mark.gpu
 def test_predict_everybody_equals_plain():
     # keeping the whole frontier makes every learned sweep the plain sweep
     say("predict everybody")
@@ -103,6 +103,7 @@ def test_predict_everybody_equals_plain():
     assert rl["swept_vertices"] == rp["swept_vertices"]
 
 
+@pytest.mark.gpu
 def test_learned_close_to_plain(trained):
     p, tau = trained
     say(f"learned vs plain (tau={tau:.3f})")
@@ -121,6 +122,7 @@ def test_learned_close_to_plain(trained):
     assert abs(q - rl["modularity"]) < 1e-12
 
 
+@pytest.mark.gpu
 def test_network_gain_on_held_out_graph(trained):
     # What the trained network gains over the method without it: the plain
     # frontier (sound prune only) has to sweep every active vertex; the network
@@ -141,6 +143,7 @@ def test_network_gain_on_held_out_graph(trained):
     assert recall > kept + 0.1
 
 
+@pytest.mark.gpu
 def test_learned_sbm_quality(trained):
     p, tau = trained
     say("learned sbm held out")

@@ -69,6 +69,8 @@ def test_modularity_matches_networkx(golden):
 
 
 def test_sbm_fixture_trace(golden):
+    # regression check against the committed trace (written by the oracle);
+    # the trajectory itself is pinned by test_sbm_trajectory_from_definition
     g = load(golden, "sbm1024_s0.bin")
     exp = json.load(open(os.path.join(golden, "sbm1024_s0.json")))
     t = oracle.Trace(keep_vectors=False)
@@ -215,3 +217,94 @@ def test_weighted_fp_oracle_matches_int_on_integer_weights():
     rf = oracle.louvain_f(n, off[1:], nbr, w.astype(np.float32))
     assert ri["nc"] == rf["nc"] and ri["Q"] == pytest.approx(rf["Q"], abs=1e-12)
     assert np.array_equal(ri["final_comm"], rf["final_comm"])
+
+
+# ---------------------------------------------------------------------------
+# C1 (the 1,024-vertex SBM) end to end from the definition: every sweep by
+# exact-rational brute force, Q in exact rationals, the break/revert rule,
+# contraction by a dictionary merge -- none of it calls the oracle
+# ---------------------------------------------------------------------------
+def q_exact(off, nbr, w, comm):
+    m2 = sum(int(x) for x in w)
+    n = len(off) - 1
+    s_in = 0
+    tot = {}
+    for v in range(n):
+        for j in range(off[v], off[v + 1]):
+            if comm[nbr[j]] == comm[v]:
+                s_in += int(w[j])
+            tot[comm[v]] = tot.get(comm[v], 0) + int(w[j])
+    return Fraction(s_in, m2) - Fraction(sum(t * t for t in tot.values()), m2 * m2)
+
+
+def contract_dict(off, nbr, w, comm):
+    """Contraction from the definition: new id = rank of the community among
+    the non-empty ones, every member row merged, rows ascending."""
+    n = len(off) - 1
+    used = sorted(set(int(c) for c in comm))
+    nid = {c: i for i, c in enumerate(used)}
+    rows = [dict() for _ in used]
+    for v in range(n):
+        r = rows[nid[int(comm[v])]]
+        for j in range(off[v], off[v + 1]):
+            d = nid[int(comm[nbr[j]])]
+            r[d] = r.get(d, 0) + int(w[j])
+    o2, nb2, w2 = [0], [], []
+    for r in rows:
+        for d in sorted(r):
+            nb2.append(d)
+            w2.append(r[d])
+        o2.append(len(nb2))
+    return nid, np.array(o2, np.int64), np.array(nb2, np.int32), np.array(w2, np.int64)
+
+
+def louvain_from_definition(off, nbr, w, min_mod=1e-6):
+    """Rows (round, iteration, Q as Fraction, cur) and the round counts."""
+    rows, nc = [], []
+    final = np.arange(len(off) - 1)
+    prev_mod = None
+    for rnd in range(64):
+        n = len(off) - 1
+        m2 = int(w.sum())
+        K = np.array([int(w[off[v]:off[v + 1]].sum()) for v in range(n)], np.int64)
+        Self = np.array([sum(int(w[j]) for j in range(off[v], off[v + 1]) if nbr[j] == v) for v in range(n)],
+                        np.int64)
+        prev = np.arange(n, dtype=np.int32)
+        qprev = q_exact(off, nbr, w, prev)
+        rows.append((rnd, 0, qprev, prev.copy()))
+        for it in range(1, 100000):
+            Tot = np.zeros(n, np.int64)
+            size = np.zeros(n, np.int32)
+            for v in range(n):
+                Tot[prev[v]] += K[v]
+                size[prev[v]] += 1
+            cur = brute_decide(off, nbr, w, prev, K, Self, Tot, size, m2)
+            q = q_exact(off, nbr, w, cur)
+            rows.append((rnd, it, q, cur.copy()))
+            if float(q - qprev) < min_mod:
+                break
+            prev, qprev = cur, q
+        nid, off, nbr, w = contract_dict(off, nbr, w, prev)
+        final = np.array([nid[int(prev[c])] for c in final])
+        nc.append(len(off) - 1)
+        cur_mod = qprev
+        if prev_mod is not None and not (float(cur_mod - prev_mod) >= min_mod):
+            break
+        prev_mod = cur_mod
+    return rows, nc, final
+
+
+def test_sbm_trajectory_from_definition(golden):
+    g = load(golden, "sbm1024_s0.bin")
+    off = g.offsets()
+    w = np.ones(len(g.neighbors), np.int64)
+    rows, nc, final = louvain_from_definition(off, g.neighbors, w)
+    t = oracle.Trace()
+    r = oracle.louvain(g.vertex_num, g.degrees, g.neighbors, trace=t)
+    assert len(t.rows) == len(rows)
+    for (a, b, q, cur), (ra, rb, rq, _, rvec) in zip(rows, t.rows):
+        assert (a, b) == (ra, rb)
+        assert np.array_equal(cur, rvec), (a, b)
+        assert abs(float(q) - rq) <= 1e-15, (a, b)
+    assert list(r["round_nc"]) == nc
+    assert np.array_equal(final, r["final_comm"])
