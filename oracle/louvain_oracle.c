@@ -16,15 +16,15 @@
  *     (:381,:532,:748); move iff gain > 0 unless size[best]==size[c]==1 and best > c
  *     (:386,:536,:754); Tot/size deltas (:397-400).  Reads are all pre-iteration
  *     (synchronous).  Here dq is scaled by m2/2:  H(d) = w_d m2 - K (Tot_d - Tot_c + K),
- *     and the move test is H(best) - m2 (w_c - Self) > 0  -- exact in int64 while
- *     4 m2^2 < 2^63 (m2 <= 1.5e9, checked).
+ *     and the move test is H(best) - m2 (w_c - Self) > 0  -- exact in int128 with
+ *     every weight sum in int64 while m2 < 2^62 (checked).
  *   Tot update    src/louvain_gpu/louvain_main_process.cu:949-956 and Tot_op
  *                 kernel_functions.cuh:112-121 (min over non-zero Tot, zero -> m2)
  *   weight update src/louvain_gpu/kernel_functions.cu:889-1034
  *     In[v] = sum w(v,u) over cur[u]==cur[v]; prune (active=0) iff
  *     m2 (2 In - S - K) + K (minTot - (Tot_c - K)) > 0  (:944, :1028, scaled by m2)
  *   modularity    kernel_functions.cuh:77-87, louvain_main_process.cu:541-545,990-999
- *     Q = sum In / m2 - sum Tot^2 / m2^2, here from exact int64 sums:
+ *     Q = sum In / m2 - sum Tot^2 / m2^2, here from exact integer sums (Tot^2 in int128):
  *     Q = (double)sumIn / (double)m2 - (double)sumTot2 / ((double)m2 * (double)m2)
  *   iteration     louvain_main_process.cu:550-1059: Iteration:0 is Q before any move,
  *                 break when Q_i - Q_{i-1} < min_modularity, return the previous
@@ -38,8 +38,11 @@
  *   round loop    src/louvain_gpu/louvain.cu:176-224 (the commented-out multi-round
  *                 loop), final composed map final[v] = new_id_R(...new_id_0(v))
  *
- * Weighted FP (fp32 weights, FP64 sums) variant: same steps with double arithmetic,
- * used only for the 1e-9 modularity tolerance check.
+ * Weights: int32 (the reference's weight_t), int64 (contracted sums), or float32
+ * scaled exactly to int64 by orc_scale_f32 -- one exact-integer implementation
+ * for all of them.  A separate FP64 variant (orc_louvain_f: float32 weights,
+ * double sums in row order) is the FP64 CPU oracle the weighted path's final Q
+ * is compared with (1e-9).
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -49,6 +52,8 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+typedef __int128 i128;
 
 #define ORC_OK 0
 #define ORC_E_INVALID -1
@@ -157,16 +162,16 @@ static int32_t decide_vertex(const og_t *g, const st_t *s, int32_t v, spa_t *sp)
     }
     int64_t wc = sp->seen[c] ? sp->acc[c] : 0;
     int32_t best = -1;
-    int64_t bestH = 0;
+    i128 bestH = 0;
     for (int32_t t = 0; t < sp->nt; t++) {
         int32_t d = sp->touched[t];
         sp->seen[d] = 0;
         if (d == c) continue;
-        int64_t H = sp->acc[d] * m2 - K * (s->Tot[d] - Tc + K);
+        i128 H = (i128)sp->acc[d] * m2 - (i128)K * (s->Tot[d] - Tc + K);
         if (H > bestH || (H == bestH && H > 0 && d < best)) { bestH = H; best = d; }
     }
     if (best < 0) return c;
-    if (bestH - m2 * (wc - S) <= 0) return c;
+    if (bestH - (i128)m2 * (wc - S) <= 0) return c;
     if (s->size[best] == 1 && s->size[c] == 1 && best > c) return c; /* singleton guard */
     return best;
 }
@@ -179,21 +184,23 @@ static void weight_vertex(const og_t *g, st_t *s, int32_t v, int64_t minTot) {
         if (s->cur[g->nbr[j]] == c) in += g->w[j];
     s->In[v] = in;
     int64_t K = s->K[v], S = s->Self[v];
-    int64_t P = s->m2 * (2 * in - S - K) + K * (minTot - (s->Tot[c] - K));
+    i128 P = (i128)s->m2 * (2 * in - S - K) + (i128)K * (minTot - (s->Tot[c] - K));
     s->active[v] = (P > 0) ? 0 : 1;
 }
 
-static double q_of(int64_t sumIn, int64_t sumTot2, int64_t m2) {
+/* Q from exact sums, finished with three FP64 operations (the engine's order) */
+static double q_of(int64_t sumIn, i128 sumTot2, int64_t m2) {
     return (double)sumIn / (double)m2 - (double)sumTot2 / ((double)m2 * (double)m2);
 }
 
 /* Weight update over all vertices + modularity sums. */
 static double st_weight_and_q(const og_t *g, st_t *s, int64_t minTot) {
     int32_t n = g->n;
-    int64_t sumIn = 0, sumT2 = 0;
+    int64_t sumIn = 0;
+    i128 sumT2 = 0;
 #pragma omp parallel for schedule(dynamic, 1024) reduction(+ : sumIn) num_threads(g_threads)
     for (int32_t v = 0; v < n; v++) { weight_vertex(g, s, v, minTot); sumIn += s->In[v]; }
-    for (int32_t c = 0; c < n; c++) sumT2 += s->Tot[c] * s->Tot[c];
+    for (int32_t c = 0; c < n; c++) sumT2 += (i128)s->Tot[c] * s->Tot[c];
     return q_of(sumIn, sumT2, s->m2);
 }
 
@@ -355,7 +362,7 @@ int cmp_i32(const void *a, const void *b) {
 static void og_free(og_t *g) { free(g->off); free(g->nbr); free(g->w); free(g->wf); }
 
 static int og_from_incl(og_t *g, int32_t n, const int64_t *end_incl, const int32_t *nbr,
-                        const int32_t *w) {
+                        const int32_t *w, const int64_t *w64) {
     memset(g, 0, sizeof(*g));
     if (n < 0) return ORC_E_INVALID;
     g->n = n;
@@ -371,7 +378,8 @@ static int og_from_incl(og_t *g, int32_t n, const int64_t *end_incl, const int32
     for (int64_t j = 0; j < e; j++) {
         if (nbr[j] < 0 || nbr[j] >= n) return ORC_E_INVALID;
         g->nbr[j] = nbr[j];
-        g->w[j] = w ? (int64_t)w[j] : 1;
+        g->w[j] = w64 ? w64[j] : w ? (int64_t)w[j] : 1;
+        if (g->w[j] <= 0) return ORC_E_INVALID; /* Louvain needs positive weights */
     }
     return ORC_OK;
 }
@@ -380,11 +388,24 @@ static int og_from_incl(og_t *g, int32_t n, const int64_t *end_incl, const int32
 /* Public (ctypes) entry points                                        */
 /* ------------------------------------------------------------------ */
 
-/* Singleton-free helper: m2 and the int64 overflow bound. */
+/* m2 and its range: every weight sum (K, Tot, In, w_d) is <= m2 < 2^62, so
+ * they are exact in int64 and every gain / prune product fits int128. */
+#define ORC_M2_MAX ((int64_t)1 << 62)
 static int check_m2(int64_t m2) {
     if (m2 <= 0) return ORC_E_INVALID;
-    if (m2 > 1500000000LL) return ORC_E_OVERFLOW;
+    if (m2 >= ORC_M2_MAX) return ORC_E_OVERFLOW;
     return ORC_OK;
+}
+
+/* sum of positive int64 weights with the 2^62 range check (no wraparound) */
+static int sum_m2(const og_t *g, int64_t *m2) {
+    i128 s = 0;
+    for (int64_t j = 0; j < g->off[g->n]; j++) {
+        s += g->w[j];
+        if (s >= ORC_M2_MAX) return ORC_E_OVERFLOW;
+    }
+    *m2 = (int64_t)s;
+    return check_m2(*m2);
 }
 
 /*
@@ -392,16 +413,15 @@ static int check_m2(int64_t m2) {
  * final_comm[n0] receives the composed community of every input vertex.
  * stats (optional, int64[4]): rounds, total sweeps, swept vertices, swept edges.
  */
-int orc_louvain(int32_t n, const int64_t *end_incl, const int32_t *nbr, const int32_t *w,
-                double min_mod, int prune, int max_rounds, orc_iter_cb cb, void *user,
-                int32_t *final_comm, double *final_q, int32_t *final_nc,
-                double *round_q, int32_t *round_nc, int64_t *stats) {
+static int louvain_exact(int32_t n, const int64_t *end_incl, const int32_t *nbr, const int32_t *w,
+                         const int64_t *w64, double min_mod, int prune, int max_rounds, orc_iter_cb cb,
+                         void *user, int32_t *final_comm, double *final_q, int32_t *final_nc,
+                         double *round_q, int32_t *round_nc, int64_t *stats) {
     og_t g;
-    int rc = og_from_incl(&g, n, end_incl, nbr, w);
+    int rc = og_from_incl(&g, n, end_incl, nbr, w, w64);
     if (rc) { og_free(&g); return rc; }
     int64_t m2 = 0;
-    for (int64_t j = 0; j < g.off[n]; j++) m2 += g.w[j];
-    if ((rc = check_m2(m2))) { og_free(&g); return rc; }
+    if ((rc = sum_m2(&g, &m2))) { og_free(&g); return rc; }
     for (int32_t v = 0; v < n; v++) final_comm[v] = v;
     double prev_mod = -1, cur_mod = -1;
     int round = 0;
@@ -436,6 +456,67 @@ int orc_louvain(int32_t n, const int64_t *end_incl, const int32_t *nbr, const in
     *final_nc = g.n;
     if (stats) { stats[0] = round; stats[1] = st_sweeps; stats[2] = st_sv; stats[3] = st_se; }
     og_free(&g);
+    return ORC_OK;
+}
+
+int orc_louvain(int32_t n, const int64_t *end_incl, const int32_t *nbr, const int32_t *w,
+                double min_mod, int prune, int max_rounds, orc_iter_cb cb, void *user,
+                int32_t *final_comm, double *final_q, int32_t *final_nc,
+                double *round_q, int32_t *round_nc, int64_t *stats) {
+    return louvain_exact(n, end_incl, nbr, w, NULL, min_mod, prune, max_rounds, cb, user, final_comm,
+                         final_q, final_nc, round_q, round_nc, stats);
+}
+
+/*
+ * The same on int64 weights: contracted graphs, and float32 weights scaled to
+ * exact integers by orc_scale_f32 (Q and every decision are invariant under a
+ * common scale of the weights, so the run is the exact-arithmetic run on the
+ * float values).  m2 must stay below 2^62.
+ */
+int orc_louvain_w64(int32_t n, const int64_t *end_incl, const int32_t *nbr, const int64_t *w64,
+                    double min_mod, int prune, int max_rounds, orc_iter_cb cb, void *user,
+                    int32_t *final_comm, double *final_q, int32_t *final_nc,
+                    double *round_q, int32_t *round_nc, int64_t *stats) {
+    return louvain_exact(n, end_incl, nbr, NULL, w64, min_mod, prune, max_rounds, cb, user, final_comm,
+                         final_q, final_nc, round_q, round_nc, stats);
+}
+
+/*
+ * Exact integer image of positive finite float32 weights: w_j = W_j / 2^k with
+ * W_j integer and k >= 0 the smallest shift that makes every weight integral
+ * (k = 0 when all weights are integers).  Written from the IEEE-754 binary32
+ * layout: a normal value is (2^23 + mantissa) * 2^(exponent - 150), a
+ * subnormal one mantissa * 2^-149.  Returns ORC_E_INVALID on a non-positive or
+ * non-finite weight, ORC_E_OVERFLOW when the scaled sum reaches 2^62.
+ */
+int orc_scale_f32(int64_t m, const float *wf, int64_t *out, int32_t *k_out) {
+    int32_t lo = 0; /* smallest exponent of a weight's lowest set bit (<= 0) */
+    for (int64_t j = 0; j < m; j++) {
+        uint32_t b;
+        memcpy(&b, &wf[j], 4);
+        uint32_t ex = (b >> 23) & 0xff, man = b & 0x7fffff;
+        if ((b >> 31) || ex == 0xff || (ex == 0 && man == 0)) return ORC_E_INVALID;
+        uint64_t sig = ex ? (man | 0x800000u) : man;
+        int32_t e = ex ? (int32_t)ex - 150 : -149;
+        while (!(sig & 1)) { sig >>= 1; e++; }
+        if (e < lo) lo = e;
+    }
+    int32_t k = -lo;
+    i128 sum = 0;
+    for (int64_t j = 0; j < m; j++) {
+        uint32_t b;
+        memcpy(&b, &wf[j], 4);
+        uint32_t ex = (b >> 23) & 0xff, man = b & 0x7fffff;
+        uint64_t sig = ex ? (man | 0x800000u) : man;
+        int32_t e = (ex ? (int32_t)ex - 150 : -149) + k;
+        while (!(sig & 1)) { sig >>= 1; e++; } /* odd significand: e >= 0 by the choice of k */
+        if (e > 62) return ORC_E_OVERFLOW;
+        i128 W = (i128)sig << e;
+        sum += W;
+        if (W >= ORC_M2_MAX || sum >= ORC_M2_MAX) return ORC_E_OVERFLOW;
+        out[j] = (int64_t)W;
+    }
+    *k_out = k;
     return ORC_OK;
 }
 
@@ -712,7 +793,8 @@ int orc_louvain_f(int32_t n, const int64_t *end_incl, const int32_t *nbr, const 
 /* Modularity of an arbitrary partition (exact integer sums), for tests. */
 double orc_modularity(int32_t n, const int64_t *end_incl, const int32_t *nbr, const int64_t *w,
                       const int32_t *comm) {
-    int64_t m2 = 0, sumIn = 0, sumT2 = 0;
+    int64_t m2 = 0, sumIn = 0;
+    i128 sumT2 = 0;
     int64_t *tot = (int64_t *)calloc((size_t)(n > 0 ? n : 1), 8);
     int64_t prev = 0;
     for (int32_t v = 0; v < n; v++) {
@@ -723,7 +805,7 @@ double orc_modularity(int32_t n, const int64_t *end_incl, const int32_t *nbr, co
         }
         prev = end_incl[v];
     }
-    for (int32_t c = 0; c < n; c++) sumT2 += tot[c] * tot[c];
+    for (int32_t c = 0; c < n; c++) sumT2 += (i128)tot[c] * tot[c];
     free(tot);
     if (m2 == 0) return 0.0;
     return q_of(sumIn, sumT2, m2);

@@ -33,6 +33,13 @@ def lib():
                                   ctypes.c_int, ITER_CB, ctypes.c_void_p, i32p,
                                   ctypes.POINTER(ctypes.c_double), i32p,
                                   ctypes.POINTER(ctypes.c_double), i32p, i64p]
+        L.orc_louvain_w64.restype = ctypes.c_int
+        L.orc_louvain_w64.argtypes = [ctypes.c_int32, i64p, i32p, i64p, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_int, ITER_CB, ctypes.c_void_p, i32p,
+                                      ctypes.POINTER(ctypes.c_double), i32p,
+                                      ctypes.POINTER(ctypes.c_double), i32p, i64p]
+        L.orc_scale_f32.restype = ctypes.c_int
+        L.orc_scale_f32.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_float), i64p, i32p]
         L.orc_round.restype = ctypes.c_int
         L.orc_round.argtypes = [ctypes.c_int32, i64p, i32p, i64p, ctypes.c_int64, ctypes.c_double,
                                 ctypes.c_int, ctypes.c_int, ITER_CB, ctypes.c_void_p, i32p,
@@ -83,11 +90,17 @@ class Trace:
 
 
 def louvain(n, end, nbr, w=None, min_mod=1e-6, prune=1, max_rounds=0, trace=None):
-    """Full multi-round run.  Returns dict(final_comm, Q, nc, round_q, round_nc, stats)."""
+    """Full multi-round run on integer weights (None = unit, int32, or int64
+    e.g. scaled float32 weights).  Returns dict(final_comm, Q, nc, round_q,
+    round_nc, stats)."""
     L = lib()
     end = np.ascontiguousarray(end, np.int64)
     nbr = np.ascontiguousarray(nbr, np.int32)
-    w = np.ones(len(nbr), np.int32) if w is None else np.ascontiguousarray(w, np.int32)
+    wide = w is not None and np.asarray(w).dtype == np.int64
+    if wide:
+        w = np.ascontiguousarray(w, np.int64)
+    else:
+        w = np.ones(len(nbr), np.int32) if w is None else np.ascontiguousarray(w, np.int32)
     comm = np.zeros(max(n, 1), np.int32)
     q = ctypes.c_double()
     nc = ctypes.c_int32()
@@ -95,15 +108,30 @@ def louvain(n, end, nbr, w=None, min_mod=1e-6, prune=1, max_rounds=0, trace=None
     rnc = np.zeros(256, np.int32)
     stats = np.zeros(4, np.int64)
     cb = trace.cfun if trace is not None else ITER_CB()
-    rc = L.orc_louvain(n, _p(end, ctypes.c_int64), _p(nbr, ctypes.c_int32), _p(w, ctypes.c_int32),
-                       min_mod, prune, max_rounds, cb, None, _p(comm, ctypes.c_int32),
-                       ctypes.byref(q), ctypes.byref(nc), _p(rq, ctypes.c_double),
-                       _p(rnc, ctypes.c_int32), _p(stats, ctypes.c_int64))
+    fn = L.orc_louvain_w64 if wide else L.orc_louvain
+    rc = fn(n, _p(end, ctypes.c_int64), _p(nbr, ctypes.c_int32),
+            _p(w, ctypes.c_int64 if wide else ctypes.c_int32),
+            min_mod, prune, max_rounds, cb, None, _p(comm, ctypes.c_int32),
+            ctypes.byref(q), ctypes.byref(nc), _p(rq, ctypes.c_double),
+            _p(rnc, ctypes.c_int32), _p(stats, ctypes.c_int64))
     if rc != 0:
         raise RuntimeError(f"orc_louvain failed: {rc}")
     r = int(stats[0])
     return dict(final_comm=comm[:n], Q=q.value, nc=nc.value, round_q=rq[:r].copy(),
                 round_nc=rnc[:r].copy(), stats=stats.copy())
+
+
+def scale_f32(wf):
+    """Exact integer image of positive float32 weights -> (W int64[m], k) with
+    wf == W / 2^k exactly and k the smallest such shift."""
+    L = lib()
+    wf = np.ascontiguousarray(wf, np.float32)
+    out = np.zeros(max(len(wf), 1), np.int64)
+    k = ctypes.c_int32()
+    rc = L.orc_scale_f32(len(wf), _p(wf, ctypes.c_float), _p(out, ctypes.c_int64), ctypes.byref(k))
+    if rc != 0:
+        raise RuntimeError(f"orc_scale_f32 failed: {rc}")
+    return out[:len(wf)], k.value
 
 
 def louvain_f(n, end, nbr, wf, min_mod=1e-6, max_rounds=0):

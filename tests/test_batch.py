@@ -171,3 +171,70 @@ def test_batch_rejects_cross_graph_edges():
     with pytest.raises(gala.GalaError):
         eng.batch_set(vstart, gala.Graph(u.degrees, bad))
     eng.close()
+
+
+@pytest.mark.gpu
+def test_full_batch_4096_sbms():
+    # The batched workload at its full size: 4,096 SBMs, vertex counts
+    # log-uniform in [64, 20,000] (13.9M vertices, 1.29B directed entries in
+    # the union).  Every graph's reported Q is the modularity of its returned
+    # partition (exact integer sums); the oracle's own run matches every graph
+    # of 5,000+ vertices (the dense ones, degree bins 6-7) and a seeded sample
+    # of 256 of the rest, partition, Q, rounds and iterations.
+    from concurrent.futures import ThreadPoolExecutor
+    say("full batch: generate 4,096 SBMs")
+    P = gala.batch_sbm_params(4096, 0)
+    vstart, g = gala.gen_sbm_batch(*P)
+    say(f"  union: {g.vertex_num} vertices, {len(g.neighbors)} entries; run")
+    eng = gala.Engine()
+    eng.batch_set(vstart, g)
+    r = eng.batch_run()
+    res = eng.batch_results()
+    comm = np.concatenate(eng.batch_communities())
+    eng.close()
+    say(f"  batch run {r['t_total_ms']:.1f} ms, {r['rounds']} rounds; checking Q of every graph")
+    G = len(vstart) - 1
+    off = g.offsets()
+    deg = np.diff(off)
+    gid = np.repeat(np.arange(G), np.diff(vstart))
+    gcomm = comm + vstart[gid].astype(np.int32)  # globally unique community ids
+    src = np.repeat(np.arange(g.vertex_num, dtype=np.int32), deg)
+    same = gcomm[g.neighbors] == gcomm[src]
+    s_in = np.bincount(gid[src[same]], minlength=G).astype(np.int64)
+    m2 = np.bincount(gid, weights=deg, minlength=G).astype(np.int64)
+    tot = np.bincount(gcomm, weights=deg, minlength=g.vertex_num).astype(np.int64)
+    cgid = np.repeat(np.arange(G), np.diff(vstart))  # graph of every community slot
+    t2 = np.zeros(G, np.int64)
+    np.add.at(t2, cgid, tot * tot)
+    del src, same
+    for i in range(G):
+        if m2[i] == 0:
+            assert res["modularity"][i] == 0.0
+            continue
+        q = int(s_in[i]) / int(m2[i]) - float(int(t2[i])) / (int(m2[i]) * int(m2[i]))
+        assert res["modularity"][i] == q, i
+    n = P[0]
+    rng = np.random.default_rng(0)
+    small = np.where(n < 5000)[0]
+    check = np.concatenate([np.where(n >= 5000)[0], rng.choice(small, 256, replace=False)])
+    say(f"  oracle on {len(check)} graphs")
+    oracle.set_threads(1)
+
+    def one(i):
+        a, b = int(vstart[i]), int(vstart[i + 1])
+        d = g.degrees[a:b] - off[a]
+        nb = g.neighbors[off[a]:off[b]] - a
+        if len(nb) == 0:
+            return i, None
+        return i, oracle.louvain(b - a, d, nb)
+
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        for i, ref in ex.map(one, check.tolist()):
+            if ref is None:
+                continue
+            a, b = int(vstart[i]), int(vstart[i + 1])
+            assert np.array_equal(comm[a:b], ref["final_comm"]), i
+            assert res["modularity"][i] == ref["Q"], i
+            assert res["communities"][i] == ref["nc"], i
+            assert res["rounds"][i] == ref["stats"][0], i
+            assert res["iterations"][i] == ref["stats"][1], i

@@ -1,11 +1,14 @@
 """Learned frontier end to end on the GPU: move logs, training on the engine's
 own MFMA kernels, learned runs whose stalled sweeps are certified, and what
 the trained network gains over the method without it."""
+import os
+
 import numpy as np
 import pytest
 
 import gala
 import train_frontier as TF
+
 
 def say(msg):
     print(msg, flush=True)
@@ -151,3 +154,35 @@ def test_learned_sbm_quality(trained):
     rp, _ = run(g, 0)
     rl, _ = run(g, 1, p, tau)
     assert abs(rl["modularity"] - rp["modularity"]) <= 0.002
+
+
+def q_of_partition(g, comm):
+    """Modularity of `comm` on an unweighted graph from exact integer sums,
+    finished in the order the engine uses: sumIn/m2 - sumTot2/m2^2."""
+    off = g.offsets()
+    deg = np.diff(off)
+    src = np.repeat(np.arange(g.vertex_num), deg)
+    m2 = int(len(g.neighbors))
+    s_in = int(np.count_nonzero(comm[g.neighbors] == comm[src]))
+    tot = np.bincount(comm, weights=deg, minlength=g.vertex_num).astype(np.int64)
+    return s_in / m2 - float(int((tot * tot).sum())) / (m2 * m2)
+
+
+@pytest.mark.gpu
+def test_committed_model_on_held_out_4M():
+    # the committed network (trained on SBM move logs) on a held-out seed of the
+    # 4.0M-vertex power-law workload: final Q within 0.002 of the plain run and
+    # equal to the modularity of the returned partition
+    if not os.path.exists(TF.DEFAULT_MODEL):
+        pytest.skip("no committed model")
+    params, tau = TF.load(TF.DEFAULT_MODEL)
+    say("committed model, Chung-Lu 4.0M seed 1001")
+    g = gala.gen_chung_lu(4_000_000, 2.1, 17.35, 15000.0, 1001)
+    rp, cp = run(g, 0)
+    rl, cl = run(g, 1, params, tau)
+    say(f"  Q plain {rp['modularity']:.6f} learned {rl['modularity']:.6f} "
+        f"kept {rl['learned_kept']}/{rl['learned_frontier']} certified {rl['certify_sweeps']}")
+    assert rl["learned_sweeps"] > 0
+    assert abs(rl["modularity"] - rp["modularity"]) <= 0.002
+    assert q_of_partition(g, cl) == rl["modularity"]
+    assert q_of_partition(g, cp) == rp["modularity"]
