@@ -4,6 +4,33 @@
 
 namespace gala {
 
+// atomicAdd for signed 64-bit sums (two's complement: the unsigned add is the
+// same operation); the runtime's own overloads stay visible beside them.
+using ::atomicAdd;
+__device__ __forceinline__ long atomicAdd(long* p, long x) {
+  return (long)::atomicAdd((unsigned long long*)p, (unsigned long long)x);
+}
+__device__ __forceinline__ long long atomicAdd(long long* p, long long x) {
+  return (long long)::atomicAdd((unsigned long long*)p, (unsigned long long)x);
+}
+
+// Lane shuffles of 128-bit gains (two 64-bit halves); the runtime's overloads
+// cover the narrower types.
+using ::__shfl_xor;
+__device__ __forceinline__ __int128 __shfl_xor(__int128 x, int k, int w) {
+  const long long lo = ::__shfl_xor((long long)(unsigned long long)x, k, w);
+  const long long hi = ::__shfl_xor((long long)(x >> 64), k, w);
+  return ((__int128)hi << 64) | (__int128)(unsigned long long)lo;
+}
+using ::__shfl;
+__device__ __forceinline__ __int128 __shfl(__int128 x, int src, int w) {
+  const long long lo = ::__shfl((long long)(unsigned long long)x, src, w);
+  const long long hi = ::__shfl((long long)(x >> 64), src, w);
+  return ((__int128)hi << 64) | (__int128)(unsigned long long)lo;
+}
+using ::__shfl_up;
+using ::__shfl_down;
+
 template <class V>
 __device__ __forceinline__ V wave_incl_scan(V x) {
   const int lane = threadIdx.x & 63;

@@ -34,7 +34,7 @@ using namespace gala;
 
 namespace {
 
-constexpr int64_t M2_LIMIT = 1000000000LL;  // 4*m2^2 < 2^63 for the exact gain tests
+constexpr int64_t M2_LIMIT = 1000000000LL;  // int32 path: 4*m2^2 < 2^63 for the exact int64 gains
 
 template <class P>
 hipError_t dalloc(P** p, int64_t elems) {
@@ -55,12 +55,17 @@ struct gala_ctx {
   int32_t n0 = 0;
   int64_t m0 = 0;
   int wmode = GALA_W_NONE;
+  // exact wide path (int64 sums, int128 gains): float32 weights, or m2 > 1e9
+  bool wide = false;
+  int32_t wshift = 0;   // float32 weights: w * 2^wshift is an integer for every weight
+  int64_t m2_wide = 0;  // m2 of the wide path (scaled integer), computed on upload
   bool have_graph = false;
   double t_h2d_ms = 0;
 
   // CSR buffers: [0] = input (read-only, kept for re-runs), [1]/[2] = work
   // buffers the contraction ping-pongs between.  Weights are int32 (int path)
-  // or double (FP path); the input may have implicit unit weights.
+  // or int64 sums (wide path; its input round may hold float32 weights);
+  // the input may have implicit unit weights.
   int64_t* off[3] = {nullptr, nullptr, nullptr};
   // row ends of the work graphs when they are gapped (contracted inside a run)
   int64_t* end[3] = {nullptr, nullptr, nullptr};
@@ -68,6 +73,7 @@ struct gala_ctx {
   int32_t* nbr[3] = {nullptr, nullptr, nullptr};
   void* w[3] = {nullptr, nullptr, nullptr};
   bool unit0 = true;
+  bool f32in = false;  // input weights kept as float32 in w[0] (wide path, scaled on load)
   bool unique0 = false;  // input rows strictly ascending (duplicate-free)
   int src = 0;  // buffer holding the current round graph
 
@@ -78,7 +84,7 @@ struct gala_ctx {
   uint8_t* dirty = nullptr;
   int8_t* binid = nullptr;
   int32_t *order = nullptr, *front = nullptr, *fin = nullptr;
-  void* cv = nullptr;  // community view, 8 (int) or 16 (FP) bytes per vertex
+  void* cv = nullptr;  // community view, 8 (int) or 16 (wide) bytes per vertex
   int2* cp = nullptr;  // {cur, prev} per vertex (integer path, incremental In)
   BlockPart* part = nullptr;  // workgroup partials of the O(V) reductions
   int32_t *bstart_full = nullptr, *bstart_front = nullptr;
@@ -159,14 +165,14 @@ struct gala_ctx {
   int dvar[NBINS] = {3, 3, 3, 3, 6, 3, 1, 1, 6};
   int fvar[NBINS] = {1, 2, 2, 2, 2, 1, 0, 0, 0};
   // weight update: the incremental integer path with narrow lane groups
-  // (variant 2); the dirty-row path (FP, tracked runs) reads variant 2 as one
+  // (variant 2); the dirty-row path (tracked runs) reads variant 2 as one
   // thread per vertex for bins 0-3
   int wvar[NBINS] = {2, 2, 2, 2, 2, 2, 0, 0, 0};
 
   std::vector<void*> owned;
   int32_t cap_n = -1;  // sizes the owned buffers were allocated for
   int64_t cap_m = -1;
-  bool cap_fp = false;
+  bool cap_wide = false;
   gala_result res{};
   bool have_result = false;
   std::vector<int32_t> comm_host;
@@ -177,7 +183,7 @@ struct gala_ctx {
 
 namespace {
 
-bool is_fp(const gala_ctx* ctx) { return ctx->wmode == GALA_W_F32; }
+bool is_wide(const gala_ctx* ctx) { return ctx->wide; }
 
 void free_batch(gala_ctx* ctx) {
   for (void* p : ctx->bowned) (void)hipFree(p);
@@ -225,12 +231,12 @@ int own(gala_ctx* ctx, P** p, int64_t elems) {
 
 // Allocate every device buffer for an input of n vertices / m entries (kept
 // when the buffers of the previous graph are large enough).
-int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
+int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool wide) {
   free_batch(ctx);
-  if (!ctx->owned.empty() && n <= ctx->cap_n && m <= ctx->cap_m && fp == ctx->cap_fp) return GALA_OK;
+  if (!ctx->owned.empty() && n <= ctx->cap_n && m <= ctx->cap_m && wide == ctx->cap_wide) return GALA_OK;
   free_all(ctx);
   const int64_t N = std::max<int32_t>(n, 1), M = std::max<int64_t>(m, 1);
-  const size_t tsz = fp ? 8 : 4;  // tot_t / w_t / acc_t size
+  const size_t tsz = wide ? 8 : 4;  // tot_t / w_t / acc_t size
   for (int k = 0; k < 3; k++) {
     OWN(ctx->off[k], N + 1);
     OWN(ctx->end[k], N);
@@ -259,7 +265,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   OWN(ctx->hub_off, N + 1); OWN(ctx->hub_tmp, 2 * (N + 1));
   OWN(ctx->hub_plan, hub_plan_elems(M));
   OWN(ctx->hub_fail, hub_plan_elems(M));
-  { char* p; OWN(p, hub_part_elems(M) * (int64_t)sizeof(HubPart<FltW>)); ctx->hub_part = p; }
+  { char* p; OWN(p, hub_part_elems(M) * (int64_t)sizeof(HubPart<WideW>)); ctx->hub_part = p; }
   OWN(ctx->part_tmp, part_tmp_elems(N));
   OWN(ctx->scan64_tmp, scan_tmp_elems(N) + 2);
   OWN(ctx->ctr, 2);  // [1] holds a deferred iteration-0 copy
@@ -284,11 +290,11 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool fp) {
   }
   ctx->cap_n = n;
   ctx->cap_m = m;
-  ctx->cap_fp = fp;
+  ctx->cap_wide = wide;
   return GALA_OK;
 }
 
-int grow_hub(gala_ctx* ctx, int64_t slots, bool fp) {
+int grow_hub(gala_ctx* ctx, int64_t slots, bool wide) {
   if (slots <= ctx->hub_cap) return GALA_OK;
   if (ctx->hub_keys) (void)hipFree(ctx->hub_keys);
   if (ctx->hub_vals) (void)hipFree(ctx->hub_vals);
@@ -299,36 +305,36 @@ int grow_hub(gala_ctx* ctx, int64_t slots, bool fp) {
   hipError_t e = dalloc(&ctx->hub_keys, cap);
   if (e == hipSuccess) {
     char* p;
-    e = dalloc(&p, cap * (fp ? 8 : 4));
+    e = dalloc(&p, cap * (wide ? 8 : 4));
     ctx->hub_vals = p;
   }
   if (e == hipSuccess) e = dalloc(&ctx->hub_rc, cap);
   if (e == hipSuccess) {
     char* p;
-    e = dalloc(&p, cap * (fp ? 8 : 4));
+    e = dalloc(&p, cap * (wide ? 8 : 4));
     ctx->hub_rw = p;
   }
   if (e != hipSuccess) { ctx->last_hip = e; ctx->hub_cap = 0; return GALA_E_OOM; }
   // the hub kernels keep overflow slices all-zero between uses
   e = hipMemset(ctx->hub_keys, 0, sizeof(int32_t) * cap);
-  if (e == hipSuccess) e = hipMemset(ctx->hub_vals, 0, (size_t)cap * (fp ? 8 : 4));
+  if (e == hipSuccess) e = hipMemset(ctx->hub_vals, 0, (size_t)cap * (wide ? 8 : 4));
   if (e != hipSuccess) { ctx->last_hip = e; return GALA_E_HIP; }
   ctx->hub_cap = cap;
   return GALA_OK;
 }
 
-int grow_arena(gala_ctx* ctx, int64_t slots, bool fp) {
+int grow_arena(gala_ctx* ctx, int64_t slots, bool wide) {
   if (slots > ctx->arena_slots) {
     if (ctx->arena_keys) (void)hipFree(ctx->arena_keys);
     if (ctx->arena_vals) (void)hipFree(ctx->arena_vals);
     ctx->arena_keys = nullptr; ctx->arena_vals = nullptr;
     int64_t cap = slots + slots / 4;  // slack so later rounds rarely reallocate
     hipError_t e = dalloc(&ctx->arena_keys, cap);
-    if (e == hipSuccess) { char* p; e = dalloc(&p, cap * (fp ? 8 : 4)); ctx->arena_vals = p; }
+    if (e == hipSuccess) { char* p; e = dalloc(&p, cap * (wide ? 8 : 4)); ctx->arena_vals = p; }
     if (e != hipSuccess) { ctx->last_hip = e; ctx->arena_slots = 0; return GALA_E_OOM; }
     // the row-merge kernels keep arena slices all-zero between uses
     e = hipMemset(ctx->arena_keys, 0, sizeof(int32_t) * cap);
-    if (e == hipSuccess) e = hipMemset(ctx->arena_vals, 0, (size_t)cap * (fp ? 8 : 4));
+    if (e == hipSuccess) e = hipMemset(ctx->arena_vals, 0, (size_t)cap * (wide ? 8 : 4));
     if (e != hipSuccess) { ctx->last_hip = e; return GALA_E_HIP; }
     ctx->arena_slots = cap;
   }
@@ -337,7 +343,7 @@ int grow_arena(gala_ctx* ctx, int64_t slots, bool fp) {
 
 // Dense-row buffers for a contraction plan: the chunk table and the scratch of
 // the segmented sort (kept for later rounds when large enough).
-int grow_dense(gala_ctx* ctx, const ContractPlan& plan, bool fp) {
+int grow_dense(gala_ctx* ctx, const ContractPlan& plan, bool wide) {
   hipError_t e = hipSuccess;
   const int64_t nch = (int64_t)plan.chunks.size();
   if (nch > ctx->chunks_cap) {
@@ -347,7 +353,7 @@ int grow_dense(gala_ctx* ctx, const ContractPlan& plan, bool fp) {
     if (e != hipSuccess) { ctx->last_hip = e; ctx->chunks_cap = 0; return GALA_E_OOM; }
     ctx->chunks_cap = nch;
   }
-  const int64_t bytes = dense_scratch_bytes(nch, plan.ndense, plan.dense_entries, fp);
+  const int64_t bytes = dense_scratch_bytes(nch, plan.ndense, plan.dense_entries, wide);
   if (bytes > ctx->dscratch_bytes) {
     if (ctx->dscratch) (void)hipFree(ctx->dscratch);
     ctx->dscratch = nullptr;
@@ -391,21 +397,62 @@ __global__ void k_setup_stage(const int32_t* __restrict__ bstart, const int64_t*
   if (t == 0) stage[8] = *slots;
 }
 
+__global__ void k_i32_to_i64(const int32_t* __restrict__ a, int64_t* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+// integer-valued float32 weights below 2^31 (checked on the host): exact
+__global__ void k_f32_to_i32(const float* __restrict__ a, int32_t* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (int32_t)a[i];
+}
+
+// Exact integer image of positive finite float32 weights: the smallest shift
+// k >= 0 such that w * 2^k is an integer for every weight, the exact sum of the
+// scaled weights (< 2^62, else GALA_E_OVERFLOW) and whether every scaled
+// weight fits 31 bits.  From the binary32 layout: a normal value is
+// (2^23 + mantissa) * 2^(exponent - 150), a subnormal one mantissa * 2^-149;
+// with the significand made odd its exponent is that of the lowest set bit.
+int f32_exact_scale(const float* w, int64_t m, int32_t* shift, int64_t* m2, bool* fits32) {
+  auto odd = [](float x, uint64_t* sig, int32_t* e) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    const uint32_t ex = (b >> 23) & 0xff, man = b & 0x7fffff;
+    *sig = ex ? (man | 0x800000u) : man;
+    *e = ex ? (int32_t)ex - 150 : -149;
+    const int tz = __builtin_ctzll(*sig);
+    *sig >>= tz;
+    *e += tz;
+  };
+  int32_t lo = 0;
+  for (int64_t j = 0; j < m; j++) {
+    uint64_t sig;
+    int32_t e;
+    odd(w[j], &sig, &e);
+    lo = std::min(lo, e);
+  }
+  const int32_t k = -lo;
+  __int128 sum = 0;
+  bool f32 = true;
+  for (int64_t j = 0; j < m; j++) {
+    uint64_t sig;
+    int32_t e;
+    odd(w[j], &sig, &e);
+    e += k;  // >= 0 by the choice of k
+    if (e > 62) return GALA_E_OVERFLOW;
+    const __int128 W = (__int128)sig << e;
+    sum += W;
+    if (W >= WIDE_M2_LIMIT || sum >= WIDE_M2_LIMIT) return GALA_E_OVERFLOW;
+    f32 = f32 && W < ((__int128)1 << 31);
+  }
+  *shift = k;
+  *m2 = (int64_t)sum;
+  *fits32 = f32;
+  return GALA_OK;
+}
+
 __global__ void k_iota(int32_t* __restrict__ p, int32_t n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
-}
-__global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = (double)a[i];
-}
-template <class T>
-__global__ void k_sum_w(const typename T::w_t* __restrict__ w, int64_t m, Counters* c) {
-  typename T::tot_t s = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
-    s += (typename T::tot_t)w[i];
-  // 64-bit accumulate for the int path: weights <= 2^31, up to 2^31 entries per thread sum
-  if (T::FP) atomicAdd(&c->sum_in_f, (double)s);
-  else atomicAdd(&c->sum_in, (unsigned long long)(long long)s);
 }
 __global__ void k_sum_w64(const int32_t* __restrict__ w, int64_t m, Counters* c) {
   long long s = 0;
@@ -429,7 +476,11 @@ DevGraph<T> graph_at(gala_ctx* ctx, int k, int32_t n, int64_t m) {
   g.off = ctx->off[k];
   g.end = ctx->gapped[k] ? ctx->end[k] : ctx->off[k] + 1;
   g.nbr = ctx->nbr[k];
-  g.w = (k == 0 && ctx->unit0) ? nullptr : (const typename T::w_t*)ctx->w[k];
+  g.w = (k == 0 && (ctx->unit0 || ctx->f32in)) ? nullptr : (const typename T::w_t*)ctx->w[k];
+  if (T::WIDE && k == 0 && ctx->f32in) {
+    g.wf = (const float*)ctx->w[0];
+    g.wsc = std::ldexp(1.0, ctx->wshift);
+  }
   return g;
 }
 inline int dst_of(int src) { return src == 1 ? 2 : 1; }
@@ -453,14 +504,14 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.nbr_moved = ctx->nbr_moved;
   st.cv = (typename T::cv_t*)ctx->cv;
   st.part = ctx->part;
-  st.cp = T::FP ? nullptr : ctx->cp;
+  st.cp = ctx->cp;
   return st;
 }
 
-template <class T>
+// Q from the exact sums, finished with three FP64 operations
 double q_from(const Counters& c, double m2) {
-  if (T::FP) return c.sum_in_f / m2 - c.sum_tot2_f / (m2 * m2);
-  return (double)c.sum_in / m2 - (double)c.sum_tot2 / (m2 * m2);
+  const unsigned __int128 t2 = ((unsigned __int128)c.sum_tot2_hi << 64) | c.sum_tot2;
+  return (double)c.sum_in / m2 - (double)t2 / (m2 * m2);
 }
 
 // Bins, order, hub offsets and community init for the round graph.
@@ -494,7 +545,7 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   launch_init_rows<T>(g, st, ctx->order, ctx->bstart_full, caps, s);
   const int nh = hb[NBINS] - hb[gb];
   if (nh > 0) {
-    int rc = grow_hub(ctx, hs[0], T::FP);
+    int rc = grow_hub(ctx, hs[0], T::WIDE);
     if (rc) return rc;
     k_hub_scatter<<<std::min((nh + 255) / 256, 4096), 256, 0, s>>>(ctx->order, ctx->bstart_full, gb, pos,
                                                                    ctx->hub_off);
@@ -535,7 +586,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   DevState<T> st = dstate<T>(ctx);
   st.track = track;
   // Iteration 0: Q on the singleton partition (louvain_main_process.cu:538-548)
-  reset_counters(ctx->ctr, (long long)m2, m2d, s);
+  reset_counters(ctx->ctr, (long long)m2, s);
   launch_apply<T>(st, g.n, m2, s);
   if (unique_rows) {
     // singleton partition, at most one self entry per row: In = Self
@@ -556,7 +607,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     HIPOK(hipMemcpyAsync(ctx->ctr + 1, ctx->ctr, sizeof(Counters), hipMemcpyDeviceToDevice, s));
   } else {
     if ((rc = read_counters<T>(ctx, &c))) return rc;
-    q = qprev = q_from<T>(c, m2d);
+    q = qprev = q_from(c, m2d);
     hostvec.resize(g.n);
     HIPOK(hipMemcpy(hostvec.data(), ctx->prev, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost));
     icb(user, round, 0, q, 0, hostvec.data(), g.n);
@@ -581,11 +632,10 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   // decide-and-move over one frontier, timed into t_dm (and per bin)
   // build_cv: refresh the community view first (a certification sweep reads
   // the view of the pre-sweep state, which is still in place)
-  // The integer path updates In from the movers' rows alone (no dirty marks);
-  // the FP path and the tracked (learned / logged) runs rescan dirty rows,
-  // which keeps the FP summation order fixed and yields the neighbour-moved
-  // fractions.
-  const bool incr = !T::FP && !track;
+  // Plain runs update In from the movers' rows alone (no dirty marks); the
+  // tracked (learned / logged) runs rescan dirty rows, which yields the
+  // neighbour-moved fractions.
+  const bool incr = !track;
   auto decide = [&](DevState<T>& sd0, const int32_t* fr, const int32_t* bs, bool first, bool build_cv) -> int {
     DevState<T> sd = sd0;
     if (incr) sd.dirty = nullptr;
@@ -605,11 +655,9 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   auto update = [&](DevState<T>& sp, bool first, Counters* out) -> int {
     launch_apply<T>(st, g.n, m2, s);
     // In changes only where a vertex or one of its neighbours moved
-    if constexpr (!T::FP) {
-      if (incr) {
-        for (int b = 0; b < NBINS; b++)
-          launch_weight_moved<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], s, ctx->wvar[b]);
-      }
+    if (incr) {
+      for (int b = 0; b < NBINS; b++)
+        launch_weight_moved<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], s, ctx->wvar[b]);
     }
     if (!incr)
       for (int b = 0; b < NBINS; b++)
@@ -623,7 +671,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     if (r) return r;
     if (q0_pending) {
       q0_pending = false;
-      qprev = q_from<T>(c0, m2d);
+      qprev = q_from(c0, m2d);
       if (icb) icb(user, round, 0, qprev, 0, nullptr, g.n);
     }
     float a = 0, b2 = 0;
@@ -633,11 +681,16 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     t_upd += b2;
     ctx->res.swept_vertices += (int64_t)out->swept;
     ctx->res.swept_edges += (int64_t)out->swept_edges;
-    if (!g.w) {
+    if (!g.weighted()) {
       ctx->res.swept_vertices_unit += (int64_t)out->swept;
       ctx->res.swept_edges_unit += (int64_t)out->swept_edges;
     }
-    const int64_t per_edge = (first ? 12 : 16) - (g.w ? 0 : 4);
+    // per entry: 4 neighbour id + the weight + 4 prev[u] (not in a first
+    // sweep) + the community total (4 bytes, 8 on the wide path); the weight
+    // is 4 bytes (int32, or float32 of a wide input round), 8 for the int64
+    // sums of a wide contracted round, 0 when implicit
+    const int64_t wbytes = !g.weighted() ? 0 : (T::WIDE && g.w) ? 8 : 4;
+    const int64_t per_edge = 4 + wbytes + (first ? 0 : 4) + (T::WIDE ? 8 : 4);
     ctx->res.decide_bytes += per_edge * (int64_t)out->swept_edges + 40 * (int64_t)out->swept;
     for (int b = 0; b < NBINS; b++) {
       const int64_t by = per_edge * (int64_t)out->bin_edges[b] + 40 * (int64_t)out->bin_swept[b];
@@ -663,7 +716,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   int it = 1;
   for (;; it++) {
     st.iter = it;
-    reset_counters(ctx->ctr, (long long)m2, m2d, s);
+    reset_counters(ctx->ctr, (long long)m2, s);
     // A learned sweep visits only the predicted movers P of the plain frontier
     // F.  The first sweep of a round is plain.
     const bool lsweep = learned && it >= 2;
@@ -692,7 +745,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     if ((rc = update(st, first, &c))) return rc;
     nfront = (int32_t)c.active;
     int64_t moved = (int64_t)c.moved;
-    q = q_from<T>(c, m2d);
+    q = q_from(c, m2d);
     if (lsweep) {
       ctx->res.learned_sweeps++;
       ctx->res.learned_frontier += nf0;
@@ -705,7 +758,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       // assignment).  Their moves land on the updated state; rows none of
       // them touches keep the In of the learned sweep, which is exact.
       ctx->res.certify_sweeps++;
-      reset_counters(ctx->ctr, (long long)m2, m2d, s);
+      reset_counters(ctx->ctr, (long long)m2, s);
       partition_bins(2, fnext, bnext, ctx->binid, ctx->keep, nf0, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
       DevState<T> sc = st;
       sc.Tot = (tt*)ctx->tot_snap;
@@ -716,7 +769,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       if ((rc = update(sp, false, &c))) return rc;
       nfront = (int32_t)c.active;
       moved += (int64_t)c.moved;
-      q = q_from<T>(c, m2d);
+      q = q_from(c, m2d);
     }
     if (icb) {
       const int32_t* vec = nullptr;
@@ -785,9 +838,9 @@ int contract(gala_ctx* ctx, const DevGraph<T>& g, const int32_t* comm, int dst, 
   plan.canonical = canonical;
   for (int b = 0; b < NBINS; b++) plan.caps[b] = caps[b];
   if (contract_prepare<T>(g, comm, cb, &plan, s)) { ctx->last_hip = hipGetLastError(); return GALA_E_HIP; }
-  int rc = grow_arena(ctx, plan.slots, T::FP);
+  int rc = grow_arena(ctx, plan.slots, T::WIDE);
   if (rc) return rc;
-  if (plan.ndense > 0 && (rc = grow_dense(ctx, plan, T::FP))) return rc;
+  if (plan.ndense > 0 && (rc = grow_dense(ctx, plan, T::WIDE))) return rc;
   ContractOut<T> out;
   out.off = ctx->off[dst];
   out.nbr = ctx->nbr[dst];
@@ -812,25 +865,29 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
   R.n_vertices = ctx->n0;
   R.input_edges = ctx->m0 / 2;
   R.t_h2d_ms = ctx->t_h2d_ms;
-  // m2 = sum of weights (louvain.cu:165-167)
-  reset_counters(ctx->ctr, 0, 0.0, s);
+  R.wide = T::WIDE ? 1 : 0;
+  R.weight_shift = ctx->wshift;
+  // m2 = sum of weights (louvain.cu:165-167): summed exactly on upload for the
+  // wide path, on the device for int32 weights
+  reset_counters(ctx->ctr, 0, s);
   DevGraph<T> g = graph_at<T>(ctx, 0, ctx->n0, ctx->m0);
   int src = 0;
-  double m2d = 0;
-  if (g.w) {
+  int64_t m2i = 0;
+  if (T::WIDE) {
+    m2i = ctx->m2_wide;
+  } else if (g.w) {
     int nb = (int)std::min<int64_t>((g.m + 255) / 256, 4096);
-    if (T::FP) k_sum_w<T><<<nb, 256, 0, s>>>(g.w, g.m, ctx->ctr);
-    else k_sum_w64<<<nb, 256, 0, s>>>((const int32_t*)g.w, g.m, ctx->ctr);
+    k_sum_w64<<<nb, 256, 0, s>>>((const int32_t*)g.w, g.m, ctx->ctr);
     Counters c;
     int rc = read_counters<T>(ctx, &c);
     if (rc) return rc;
-    m2d = T::FP ? c.sum_in_f : (double)c.sum_in;
+    m2i = (int64_t)c.sum_in;
   } else {
-    m2d = (double)g.m;
+    m2i = g.m;
   }
-  if (!(m2d > 0)) return GALA_E_INVALID;
-  if (!T::FP && m2d > (double)M2_LIMIT) return GALA_E_OVERFLOW;
-  const tt m2 = (tt)m2d;
+  if (m2i <= 0) return GALA_E_INVALID;
+  if (m2i > (T::WIDE ? WIDE_M2_LIMIT - 1 : M2_LIMIT)) return GALA_E_OVERFLOW;
+  const tt m2 = (tt)m2i;
 
   HIPOK(hipEventRecord(ctx->ev[4], s));
   k_iota<<<std::min((ctx->n0 + 255) / 256, 8192), 256, 0, s>>>(ctx->fin, ctx->n0);
@@ -886,16 +943,19 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
         HIPOK(hipMemcpy(off.data(), ctx->off[dst], sizeof(int64_t) * (C + 1), hipMemcpyDeviceToHost));
         HIPOK(hipMemcpy(nb.data(), ctx->nbr[dst], sizeof(int32_t) * m, hipMemcpyDeviceToHost));
         for (int32_t i = 0; i < C; i++) endi[i] = off[i + 1];
-        if (T::FP) {
+        if (T::WIDE) {
+          // the exact integer sums, and their value on the input's scale
+          wi.resize(m);
+          HIPOK(hipMemcpy(wi.data(), ctx->w[dst], sizeof(int64_t) * m, hipMemcpyDeviceToHost));
           wf.resize(m);
-          HIPOK(hipMemcpy(wf.data(), ctx->w[dst], sizeof(double) * m, hipMemcpyDeviceToHost));
+          for (int64_t j = 0; j < m; j++) wf[j] = std::ldexp((double)wi[j], -ctx->wshift);
         } else {
           std::vector<int32_t> w32(m);
           HIPOK(hipMemcpy(w32.data(), ctx->w[dst], sizeof(int32_t) * m, hipMemcpyDeviceToHost));
           wi.assign(w32.begin(), w32.end());
         }
-        rcb(user, round, cur_mod, g.n, cm.data(), C, endi.data(), nb.data(),
-            T::FP ? nullptr : wi.data(), T::FP ? wf.data() : nullptr);
+        rcb(user, round, cur_mod, g.n, cm.data(), C, endi.data(), nb.data(), wi.data(),
+            T::WIDE ? wf.data() : nullptr);
       } else {
         rcb(user, round, cur_mod, g.n, cm.data(), C, nullptr, nullptr, nullptr, nullptr);
       }
@@ -948,10 +1008,32 @@ int upload(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr
   if (weight_mode == GALA_W_F32)
     for (int64_t j = 0; j < m; j++)
       if (!(((const float*)w)[j] > 0) || !std::isfinite(((const float*)w)[j])) return GALA_E_FORMAT;
+  // The path: int32 sums while m2 <= 1e9 and every weight is a 31-bit
+  // integer, else the wide path on the exact integer image of the weights.
+  bool wide = false, f32_as_i32 = false;
+  int32_t shift = 0;
+  int64_t m2x = m;
+  if (weight_mode == GALA_W_I32) {
+    __int128 sum = 0;
+    for (int64_t j = 0; j < m; j++) sum += ((const int32_t*)w)[j];
+    if (sum >= WIDE_M2_LIMIT) return GALA_E_OVERFLOW;
+    m2x = (int64_t)sum;
+    wide = m2x > M2_LIMIT;
+  } else if (weight_mode == GALA_W_F32) {
+    bool fits32 = false;
+    const int rs = f32_exact_scale((const float*)w, m, &shift, &m2x, &fits32);
+    if (rs) return rs;
+    f32_as_i32 = shift == 0 && fits32 && m2x <= M2_LIMIT;
+    wide = !f32_as_i32;
+  } else {
+    wide = m > M2_LIMIT;
+  }
   ctx->wmode = weight_mode;
-  bool fp = weight_mode == GALA_W_F32;
-  int rc = alloc_buffers(ctx, n, m, fp);
+  int rc = alloc_buffers(ctx, n, m, wide);
   if (rc) return rc;
+  ctx->wide = wide;
+  ctx->wshift = shift;
+  ctx->m2_wide = m2x;
   hipStream_t s = ctx->stream;
   HIPOK(hipEventRecord(ctx->ev[0], s));
   int64_t* dend = ctx->off[1];  // staging of the inclusive ends
@@ -959,12 +1041,25 @@ int upload(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr
   k_offsets_from_incl<<<std::min((n + 256) / 256 + 1, 8192), 256, 0, s>>>(dend, n, ctx->off[0]);
   if (m > 0) HIPOK(hipMemcpyAsync(ctx->nbr[0], nbr, sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
   ctx->unit0 = weight_mode == GALA_W_NONE;
-  if (weight_mode == GALA_W_I32 && m > 0)
-    HIPOK(hipMemcpyAsync(ctx->w[0], w, sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+  ctx->f32in = weight_mode == GALA_W_F32 && wide;
+  const int nbw = (int)std::min<int64_t>((m + 255) / 256, 8192);
+  if (weight_mode == GALA_W_I32 && m > 0) {
+    if (!wide) {
+      HIPOK(hipMemcpyAsync(ctx->w[0], w, sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+    } else {  // widened to the int64 sums of the wide path on the device
+      int32_t* tmp = (int32_t*)ctx->c_sval;  // staging (8 bytes per entry available)
+      HIPOK(hipMemcpyAsync(tmp, w, sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+      k_i32_to_i64<<<nbw, 256, 0, s>>>(tmp, (int64_t*)ctx->w[0], m);
+    }
+  }
   if (weight_mode == GALA_W_F32 && m > 0) {
-    float* tmpf = (float*)ctx->c_sval;  // staging (8 bytes per entry available)
-    HIPOK(hipMemcpyAsync(tmpf, w, sizeof(float) * m, hipMemcpyHostToDevice, s));
-    k_f32_to_f64<<<(int)std::min<int64_t>((m + 255) / 256, 8192), 256, 0, s>>>(tmpf, (double*)ctx->w[0], m);
+    if (wide) {  // kept as float32 in HBM; the kernels scale each weight by 2^shift on load
+      HIPOK(hipMemcpyAsync(ctx->w[0], w, sizeof(float) * m, hipMemcpyHostToDevice, s));
+    } else {  // integer-valued floats: the int32 path
+      float* tmp = (float*)ctx->c_sval;
+      HIPOK(hipMemcpyAsync(tmp, w, sizeof(float) * m, hipMemcpyHostToDevice, s));
+      k_f32_to_i32<<<nbw, 256, 0, s>>>(tmp, (int32_t*)ctx->w[0], m);
+    }
   }
   HIPOK(hipEventRecord(ctx->ev[1], s));
   HIPOK(hipEventSynchronize(ctx->ev[1]));
@@ -1104,7 +1199,7 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
   fill<int32_t>(ctx->b_rounds, G, 0, s);
   fill<double>(ctx->b_cur_mod, G, 0.0, s);
   copy_i32(ctx->b_gid[0], ctx->b_gid[1], ctx->n0, s);
-  reset_counters(ctx->ctr, m2max, (double)m2max, s);  // the decide totals start at zero
+  reset_counters(ctx->ctr, m2max, s);  // the decide totals start at zero
   HIPOK(hipEventRecord(ctx->ev[4], s));
   k_iota<<<std::min((ctx->n0 + 255) / 256, 8192), 256, 0, s>>>(ctx->fin, ctx->n0);
   int gcur = 1;  // b_gid index of the round graph
@@ -1139,7 +1234,7 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     // one iteration for every running graph; a graph that stopped is inert
     // integer path: In from the movers' rows ({cur, prev} pairs, reset by the
     // mask after each sweep), so decide-and-move writes no dirty marks
-    constexpr bool incr = !T::FP;
+    constexpr bool incr = true;
     DevState<T> sdec = st;
     if (incr) sdec.dirty = nullptr;
     auto body = [&](bool first) {
@@ -1356,10 +1451,10 @@ int gala_run(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gal
   if (ctx->n0 == 0) return GALA_E_INVALID;
   (void)hipSetDevice(ctx->device);
   // the input buffers are read-only: a context can run the same graph again
-  return is_fp(ctx) ? run_all<FltW>(ctx, icb, rcb, user, out) : run_all<IntW>(ctx, icb, rcb, user, out);
+  return is_wide(ctx) ? run_all<WideW>(ctx, icb, rcb, user, out) : run_all<IntW>(ctx, icb, rcb, user, out);
 }
 
-// weight-update kernel variants per bin.  Dirty-row path (FP, tracked runs):
+// weight-update kernel variants per bin.  Dirty-row path (tracked runs):
 // 1 = pipelined (bins 0-4), 2 = one thread per vertex (bins 0-3).  Incremental
 // integer path: narrower lane groups for bins 1-5 (launch_weight_moved).
 static int weight_variants(int bin) { return bin <= 5 ? 3 : 1; }
@@ -1424,7 +1519,7 @@ int gala_autotune(gala_ctx* ctx, int32_t* variants_out) {
     }
     gala_result r;
     for (int rep = 0; rep < 2 && rc == GALA_OK; rep++)  // the second run is timed
-      rc = is_fp(ctx) ? run_all<FltW>(ctx, nullptr, nullptr, nullptr, &r) : run_all<IntW>(ctx, nullptr, nullptr, nullptr, &r);
+      rc = is_wide(ctx) ? run_all<WideW>(ctx, nullptr, nullptr, nullptr, &r) : run_all<IntW>(ctx, nullptr, nullptr, nullptr, &r);
     if (rc) break;
     for (int b = 0; b < NBINS; b++) {
       if (k < decide_variants(b) && r.bin_decide_ms[b] < best_d[b]) { best_d[b] = r.bin_decide_ms[b]; bd[b] = k; }
@@ -1513,7 +1608,7 @@ int gala_phase_decide(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const i
   partition_bins(1, ctx->order, ctx->bstart_full, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front,
                  ctx->part_tmp, s);
   DevState<IntW> st = dstate<IntW>(ctx);
-  reset_counters(ctx->ctr, m2, (double)m2, s);
+  reset_counters(ctx->ctr, m2, s);
   launch_build_cv<IntW>(st, n, s);
   for (int b = 0; b < NBINS; b++)
     launch_decide<IntW>(b, g, st, ctx->front, ctx->bstart_front, caps[b], (int32_t)m2, s, ctx->dvar[b]);
@@ -1542,7 +1637,7 @@ int gala_phase_weight(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const i
   HIPOK(hipMemcpyAsync(ctx->Tot, Tot, 4ll * n, hipMemcpyHostToDevice, s));
   HIPOK(hipMemcpyAsync(ctx->cur, cur, 4ll * n, hipMemcpyHostToDevice, s));
   DevState<IntW> st = dstate<IntW>(ctx);
-  reset_counters(ctx->ctr, min_tot, (double)min_tot, s);
+  reset_counters(ctx->ctr, min_tot, s);
   // degree-0 vertices are in no bin: In stays 0 and active 1, as the
   // reference's zero-initialised In and fill(active, 1) leave them
   fill<int32_t>((int32_t*)ctx->In, n, 0, s);
@@ -1555,6 +1650,17 @@ int gala_phase_weight(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const i
   HIPOK(hipMemcpy(active_out, ctx->active, n, hipMemcpyDeviceToHost));
   ctx->have_graph = false;
   return GALA_OK;
+}
+
+int gala_phase_modularity(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr, const void* w,
+                          int weight_mode, const int32_t* comm, double* q_out) {
+  if (!ctx || n < 0 || !q_out || weight_mode < 0 || weight_mode > 2) return GALA_E_INVALID;
+  if (n > 0 && (!end_incl || !nbr || !comm)) return GALA_E_INVALID;
+  for (int32_t i = 0; i < n; i++)
+    if (end_incl[i] < (i == 0 ? 0 : end_incl[i - 1])) return GALA_E_FORMAT;
+  if (weight_mode != GALA_W_NONE && n > 0 && end_incl[n - 1] > 0 && !w) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  return modularity_host(ctx->device, ctx->stream, n, end_incl, nbr, w, weight_mode, comm, q_out);
 }
 
 int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr,
@@ -1570,7 +1676,7 @@ int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const
   int32_t caps[NBINS], nitems = 0;
   if ((rc = round_setup<IntW>(ctx, g, caps, &nitems))) return rc;  // degree bins for the staging kernels
   HIPOK(hipMemcpy(ctx->prev, comm, 4ll * n, hipMemcpyHostToDevice));
-  reset_counters(ctx->ctr, 0, 0.0, ctx->stream);
+  reset_counters(ctx->ctr, 0, ctx->stream);
   int32_t C = 0;
   int64_t m = 0;
   if ((rc = contract<IntW>(ctx, g, ctx->prev, 1, caps, &C, &m, true))) return rc;

@@ -36,10 +36,10 @@ __host__ __device__ inline int bin_of_degree(int64_t d) {
   return 8;
 }
 
-// First bin that uses the global hash arena (FP64 tables do not fit 16384
-// slots in LDS, so the FP path sends bin 7 to the arena as well).
+// First bin that uses the global hash arena (tables of 8-byte sums do not fit
+// 16384 slots in LDS, so the wide path sends bin 7 to the arena as well).
 template <class T>
-constexpr int global_bin_min() { return T::FP ? 7 : 8; }
+constexpr int global_bin_min() { return T::WIDE ? 7 : 8; }
 
 // Learned-frontier MLP: 6 features -> 64 hidden (ReLU) -> 1 logit.
 // Parameter vector layout (fp32): W1[16][64] (feature rows 6..15 are zero
@@ -50,8 +50,9 @@ constexpr int MLP_NPARAM = MLP_B2 + 1;
 
 // Per-iteration device counters (one small struct read back per iteration).
 struct Counters {
-  unsigned long long sum_in;     // sum over v of In[v]            (exact, int path)
-  unsigned long long sum_tot2;   // sum over c of Tot[c]^2         (exact, int path)
+  unsigned long long sum_in;     // sum over v of In[v]            (exact, <= m2 < 2^62)
+  unsigned long long sum_tot2;   // sum over c of Tot[c]^2, low 64 bits (exact)
+  unsigned long long sum_tot2_hi;// ... high 64 bits (wide path: Tot^2 up to 2^124)
   long long min_tot;             // min non-zero Tot (zero -> m2)
   unsigned long long moved;      // vertices that changed community this sweep
   unsigned long long swept;      // vertices evaluated by decide-and-move
@@ -59,22 +60,18 @@ struct Counters {
   unsigned long long active;     // vertices kept in the next frontier
   int nonempty;                  // communities with size > 0
   int overflow;                  // hash-table overflow flag (must stay 0)
-  double sum_in_f;               // FP path: sum In (FP64)
-  double sum_tot2_f;             // FP path: sum Tot^2 (FP64)
-  double min_tot_f;              // FP path: min non-zero Tot
   unsigned long long bin_swept[NBINS];  // decide-and-move per degree bin
   unsigned long long bin_edges[NBINS];
 };
 
 // Per-workgroup partial results of the O(V) reductions (k_apply, k_prune).
-// One workgroup combines them in a fixed order afterwards, so thousands of
-// workgroups do not serialise on the same counter addresses and FP64 sums are
-// reproducible run to run.
+// One workgroup combines them afterwards, so thousands of workgroups do not
+// serialise on the same counter addresses.  Every sum is an exact integer.
 struct BlockPart {
-  unsigned long long s0;  // apply: sum Tot^2          prune: sum In
-  unsigned long long s1;  // apply: min non-zero Tot   prune: vertices kept active
-  double f0;              // FP path: apply sum Tot^2, prune sum In
-  unsigned long long f1;  // FP path: apply min Tot (bits of a non-negative double)
+  unsigned long long s0;  // apply: sum Tot^2 (low 64 bits)  prune: sum In
+  unsigned long long s1;  // apply: min non-zero Tot          prune: vertices kept active
+  unsigned long long s0hi;// apply: sum Tot^2 (high 64 bits)
+  unsigned long long pad0;
   long long i0;           // apply: non-empty communities
   long long pad[3];
 };
@@ -84,48 +81,56 @@ constexpr int64_t HPIN_BYTES = 1 << 20;  // pinned host scratch of the engine's 
 // Community view of a vertex for one sweep: its community, that community's
 // Tot and whether the community is a singleton, so one gather gives all
 // three.  Integer path: 8 bytes, Tot in the low 31 bits of ts (Tot <= m2 <
-// 2^31) and the singleton flag in its sign bit.  FP path: 16 bytes.
+// 2^31) and the singleton flag in its sign bit.  Wide path: 16 bytes.
 struct alignas(8) CvI {
   int32_t c;
   int32_t ts;
 };
-struct alignas(16) CvF {
+struct alignas(16) CvW {
   int32_t c;
   int32_t size;
-  double tot;
+  int64_t tot;
 };
 __host__ __device__ inline int32_t cv_tot(const CvI& e) { return e.ts & 0x7fffffff; }
-__host__ __device__ inline double cv_tot(const CvF& e) { return e.tot; }
+__host__ __device__ inline int64_t cv_tot(const CvW& e) { return e.tot; }
 // 1 for a singleton community, some other positive size otherwise
 __host__ __device__ inline int32_t cv_sz(const CvI& e) { return e.ts < 0 ? 1 : 2; }
-__host__ __device__ inline int32_t cv_sz(const CvF& e) { return e.size; }
+__host__ __device__ inline int32_t cv_sz(const CvW& e) { return e.size; }
 __host__ __device__ inline void cv_set(CvI& e, int32_t c, int32_t size, int32_t tot) {
   e.c = c;
   e.ts = size == 1 ? (int32_t)((uint32_t)tot | 0x80000000u) : tot;
 }
-__host__ __device__ inline void cv_set(CvF& e, int32_t c, int32_t size, double tot) {
+__host__ __device__ inline void cv_set(CvW& e, int32_t c, int32_t size, int64_t tot) {
   e.c = c;
   e.size = size;
   e.tot = tot;
 }
 
-// Weight traits: exact integer path and FP64-accumulating float path.
+// Weight traits.  Both are exact integer arithmetic:
+//  IntW  -- int32 weights and sums, int64 gains; m2 <= 1e9 (the fast path for
+//           unweighted and int32-weighted graphs of up to 1e9 directed entries).
+//  WideW -- int64 weights and sums, int128 gains; m2 < 2^62.  Float32 weights
+//           run here as exact integers W = w * 2^k (k the smallest shift that
+//           makes every weight integral; decisions and Q are invariant under
+//           the common scale), as do integer graphs with m2 > 1e9.  The input
+//           round may keep its float32 weights in HBM (DevGraph::wf) and scale
+//           them in registers; contracted rounds carry int64 sums.
 struct IntW {
   using w_t = int32_t;     // edge weight as stored
   using acc_t = int32_t;   // per-community neighbour weight sum (<= K < 2^31)
   using tot_t = int32_t;   // K, Self, Tot, In (all <= m2 < 2^31)
   using gain_t = long long;
   using cv_t = CvI;
-  static constexpr bool FP = false;
+  static constexpr bool WIDE = false;
 };
-struct FltW {
-  using w_t = double;      // fp32 input weights are widened on upload; contracted
-                           // rounds carry FP64 sums
-  using acc_t = double;
-  using tot_t = double;
-  using gain_t = double;
-  using cv_t = CvF;
-  static constexpr bool FP = true;
+struct WideW {
+  using w_t = int64_t;
+  using acc_t = int64_t;
+  using tot_t = int64_t;
+  using gain_t = __int128;
+  using cv_t = CvW;
+  static constexpr bool WIDE = true;
 };
+constexpr int64_t WIDE_M2_LIMIT = (int64_t)1 << 62;  // every sum <= m2 fits int64, every product int128
 
 }  // namespace gala

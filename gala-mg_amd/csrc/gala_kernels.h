@@ -25,7 +25,7 @@ void fill(V* p, int64_t n, V x, hipStream_t s);
 void copy_i32(const int32_t* a, int32_t* b, int64_t n, hipStream_t s);
 // prev = cur after an accepted sweep; cp (if not null) = {cur, cur}
 void accept_sweep(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, hipStream_t s);
-void reset_counters(Counters* c, long long min_init, double min_init_f, hipStream_t s);
+void reset_counters(Counters* c, long long min_init, hipStream_t s);
 
 // ---- Louvain phase kernels (kernels_louvain.hip) ----
 // Graph view on the device.  off has n+1 exclusive offsets.
@@ -39,8 +39,23 @@ struct DevGraph {
   const int64_t* off;
   const int64_t* end;
   const int32_t* nbr;
-  const typename T::w_t* w;  // nullptr => every weight is 1
+  const typename T::w_t* w;  // nullptr (and wf nullptr) => every weight is 1
+  // wide path, input round of a float32-weighted graph: the weights as read
+  // from the file, each scaled exactly to an integer by wsc = 2^k on load
+  const float* wf = nullptr;
+  double wsc = 1.0;
+  __host__ __device__ bool weighted() const { return w != nullptr || wf != nullptr; }
 };
+
+// Weight of entry j as the trait's integer.  A float32 weight times 2^k is an
+// integer below 2^62 (checked on upload), so both conversions are exact.
+template <class T>
+__device__ __forceinline__ typename T::w_t load_w(const DevGraph<T>& g, int64_t j) {
+  if constexpr (T::WIDE) {
+    if (g.wf) return (typename T::w_t)((double)g.wf[j] * g.wsc);
+  }
+  return g.w ? g.w[j] : (typename T::w_t)1;
+}
 
 // entries of padding after every CSR neighbour / weight array (vector row loads)
 constexpr int64_t CSR_PAD = 16;
@@ -288,5 +303,9 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
 
 // final[v] = label[final[v]]
 void compose_map(int32_t* fin, int32_t n0, const int32_t* label, hipStream_t s);
+
+// ---- modularity of a given partition (modularity.hip) ----
+int modularity_host(int device, hipStream_t s, int32_t n, const int64_t* end_incl, const int32_t* nbr, const void* w,
+                    int wmode, const int32_t* comm, double* q_out);
 
 }  // namespace gala

@@ -365,16 +365,15 @@ void copy_i32(const int32_t* a, int32_t* b, int64_t n, hipStream_t s) {
   k_copy_i32<<<nb, 256, 0, s>>>(a, b, n);
 }
 
-__global__ void k_reset_counters(Counters* c, long long min_init, double min_init_f) {
+__global__ void k_reset_counters(Counters* c, long long min_init) {
   if (threadIdx.x == 0) {
-    c->sum_in = 0; c->sum_tot2 = 0; c->min_tot = min_init; c->moved = 0; c->swept = 0;
+    c->sum_in = 0; c->sum_tot2 = 0; c->sum_tot2_hi = 0; c->min_tot = min_init; c->moved = 0; c->swept = 0;
     c->swept_edges = 0; c->active = 0; c->nonempty = 0; c->overflow = 0;
-    c->sum_in_f = 0; c->sum_tot2_f = 0; c->min_tot_f = min_init_f;
   }
   if (threadIdx.x < NBINS) { c->bin_swept[threadIdx.x] = 0; c->bin_edges[threadIdx.x] = 0; }
 }
-void reset_counters(Counters* c, long long min_init, double min_init_f, hipStream_t s) {
-  k_reset_counters<<<1, 64, 0, s>>>(c, min_init, min_init_f);
+void reset_counters(Counters* c, long long min_init, hipStream_t s) {
+  k_reset_counters<<<1, 64, 0, s>>>(c, min_init);
 }
 
 }  // namespace gala

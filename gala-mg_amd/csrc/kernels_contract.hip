@@ -18,7 +18,7 @@
 //   L <= 2048     128-thread workgroup, same with 16 keys per thread
 //   L <= 4096     256-thread workgroup
 //   L <= LT       1024-thread workgroup: LT-slot LDS table (LT = 16384, 8192 on
-//                 the FP path), overflow into a global slice, register bitonic
+//                 the wide path), overflow into a global slice, register bitonic
 //                 sort of <= LT keys (LDS only for exchanges between waves)
 //   L >  LT       dense rows: the segment is cut into chunks, one workgroup per
 //                 chunk pre-aggregates in LDS and adds into a C-wide accumulator
@@ -170,7 +170,7 @@ __device__ __forceinline__ void stage_span(const DevGraph<T>& g, int64_t o, int6
     for (int r = 0; r < STG_U; r++) {
       const int64_t j = j0 + (int64_t)r * NT;
       u[r] = j < d ? g.nbr[o + j] : 0;
-      w[r] = j < d && g.w ? (acc)g.w[o + j] : acc(1);
+      w[r] = j < d ? (acc)load_w<T>(g, o + j) : acc(1);
     }
 #pragma unroll
     for (int r = 0; r < STG_U; r++) u[r] = j0 + (int64_t)r * NT < d ? label[u[r]] : 0;
@@ -889,7 +889,7 @@ __global__ void k_plan_stage(const int32_t* __restrict__ C, const int32_t* __res
 template <class T>
 int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>& cb, ContractPlan* plan,
                      hipStream_t s) {
-  constexpr long long LT = T::FP ? 8192 : 16384;
+  constexpr long long LT = T::WIDE ? 8192 : 16384;
   const int32_t n = g.n;
   fill<int32_t>(cb.cnt, n, 0, s);
   // the staging cursors are free until contract_build: row-length sums by community
@@ -973,7 +973,7 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
 template <class T>
 int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan& plan,
                    ContractOut<T>& out, int64_t* m_out, hipStream_t s) {
-  constexpr int LT = T::FP ? 8192 : 16384;
+  constexpr int LT = T::WIDE ? 8192 : 16384;
   const int32_t n = g.n;
   const int32_t C = plan.C;
   fill<int64_t>(cb.ecur, C, 0, s);
@@ -1122,6 +1122,6 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
   template int contract_build<T>(const DevGraph<T>&, ContractBufs<T>&, const ContractPlan&,         \
                                  ContractOut<T>&, int64_t*, hipStream_t);
 GALA_CINST(IntW)
-GALA_CINST(FltW)
+GALA_CINST(WideW)
 
 }  // namespace gala

@@ -339,11 +339,11 @@ void launch_w1_bf16(const float* prm, short* w1b, hipStream_t s) {
 
 template void launch_mlp_filter<IntW>(const DevGraph<IntW>&, DevState<IntW>&, const int32_t*, int, int, const float*,
                                       const short*, float, uint8_t*, hipStream_t);
-template void launch_mlp_filter<FltW>(const DevGraph<FltW>&, DevState<FltW>&, const int32_t*, int, int, const float*,
+template void launch_mlp_filter<WideW>(const DevGraph<WideW>&, DevState<WideW>&, const int32_t*, int, int, const float*,
                                       const short*, float, uint8_t*, hipStream_t);
 template void launch_features<IntW>(const DevGraph<IntW>&, DevState<IntW>&, const int32_t*, int, int, float*,
                                     hipStream_t);
-template void launch_features<FltW>(const DevGraph<FltW>&, DevState<FltW>&, const int32_t*, int, int, float*,
+template void launch_features<WideW>(const DevGraph<WideW>&, DevState<WideW>&, const int32_t*, int, int, float*,
                                     hipStream_t);
 
 }  // namespace gala

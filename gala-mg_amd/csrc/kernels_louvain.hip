@@ -55,31 +55,28 @@ __device__ __forceinline__ void lds_fence_wave() {
   __builtin_amdgcn_wave_barrier();
 }
 
-template <class T>
-__device__ __forceinline__ typename T::w_t load_w(const typename T::w_t* w, int64_t j) {
-  return w ? w[j] : (typename T::w_t)1;
-}
-
-// Scaled gain H(d) = w_d*m2 - K*(Tot_d - Tot_c + K).  Exact in int64 on the
-// integer path; on the FP path every operation is rounded separately (no FMA
-// contraction), the same sequence the FP64 oracle evaluates.
+// Scaled gain H(d) = w_d*m2 - K*(Tot_d - Tot_c + K), exact: int64 on the
+// integer path (m2 <= 1e9), int128 on the wide path (m2 < 2^62), where the
+// differences of sums stay in int64 and only the two products are widened.
 template <class T>
 __device__ __forceinline__ typename T::gain_t gain_h(typename T::gain_t a, typename T::gain_t m2,
                                                      typename T::gain_t K, typename T::gain_t Td,
                                                      typename T::gain_t Tc) {
-  if constexpr (T::FP) {
-    const double t = __dadd_rn(__dsub_rn(Td, Tc), K);
-    return __dsub_rn(__dmul_rn(a, m2), __dmul_rn(K, t));
+  using G = typename T::gain_t;
+  if constexpr (T::WIDE) {
+    const int64_t t = (int64_t)Td - (int64_t)Tc + (int64_t)K;
+    return (G)(int64_t)a * (G)(int64_t)m2 - (G)(int64_t)K * (G)t;
   } else {
     return a * m2 - K * (Td - Tc + K);
   }
 }
 
-// Move test H(best) - m2*(w_c - S) > 0 (same rounding discipline).
+// Move test H(best) - m2*(w_c - S) > 0.
 template <class T>
 __device__ __forceinline__ typename T::gain_t gain_net(typename T::gain_t h, typename T::gain_t m2,
                                                        typename T::gain_t wc, typename T::gain_t S) {
-  if constexpr (T::FP) return __dsub_rn(h, __dmul_rn(m2, __dsub_rn(wc, S)));
+  using G = typename T::gain_t;
+  if constexpr (T::WIDE) return h - (G)(int64_t)m2 * (G)((int64_t)wc - (int64_t)S);
   else return h - m2 * (wc - S);
 }
 
@@ -165,7 +162,7 @@ __device__ __forceinline__ typename T::tot_t vertex_m2(const DevState<T>& st, in
 
 // Community view of u.  In the first sweep of a round every vertex is a
 // singleton with Tot = K, so the view is built from one 4-byte Tot gather
-// (8 bytes on the FP path) instead of reading the full view, which is then
+// (8 bytes on the wide path) instead of reading the full view, which is then
 // not built for that sweep.
 template <class T, bool FIRST>
 __device__ __forceinline__ typename T::cv_t view_of(const DevState<T>& st, int u) {
@@ -205,7 +202,7 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
     if (d >= 8) continue;  // bins 2 and up: launch_init_rows, once the bins are partitioned
     tt k = 0, self = 0;
     for (int64_t j = o; j < e; j++) {
-      tt w = (tt)load_w<T>(g.w, j);
+      tt w = (tt)load_w<T>(g, j);
       k += w;
       if (g.nbr[j] == v) self = w;
     }
@@ -229,7 +226,7 @@ __global__ __launch_bounds__(256) void k_init_wide(DevGraph<T> g, DevState<T> st
     const int64_t o = g.off[v], e = g.end[v];
     tt k = 0, self = 0;
     for (int64_t j = o + lane; j < e; j += W) {
-      tt w = (tt)load_w<T>(g.w, j);
+      tt w = (tt)load_w<T>(g, j);
       k += w;
       if (g.nbr[j] == v) self = w;
     }
@@ -265,7 +262,7 @@ __global__ __launch_bounds__(1024) void k_init_huge(DevGraph<T> g, DevState<T> s
       for (int q = 0; q < U; q++) {
         const int64_t j = j0 + (int64_t)q * NT;
         u[q] = j < e ? g.nbr[j] : -1;
-        w[q] = j < e ? (tt)load_w<T>(g.w, j) : tt(0);
+        w[q] = j < e ? (tt)load_w<T>(g, j) : tt(0);
       }
 #pragma unroll
       for (int q = 0; q < U; q++) {
@@ -364,7 +361,7 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
       const typename T::cv_t eu = st.cv[ul];
       cl = eu.c;
       tl = cv_tot(eu);
-      wl = (acc)load_w<T>(g.w, o + lane);
+      wl = (acc)load_w<T>(g, o + lane);
     }
     acc s = wl, wc = (cl == c) ? wl : acc(0);
     bool leader = cl >= 0;
@@ -522,7 +519,7 @@ __global__ __launch_bounds__(256) void k_decide_pipe(DevGraph<T> g, DevState<T> 
       vC = vB; dC = dB; evC = evB; KC = KB; SC = SB;
       if (lane < dB) {
         uC = g.nbr[oB + lane];
-        wC = (acc)load_w<T>(g.w, oB + lane);
+        wC = (acc)load_w<T>(g, oB + lane);
       }
     }
     okB = okA;
@@ -595,7 +592,7 @@ __global__ __launch_bounds__(256) void k_decide_tpv(DevGraph<T> g, DevState<T> s
                                                      const int32_t* __restrict__ front,
                                                      const int32_t* __restrict__ bstart, int bin,
                                                      typename T::tot_t m2_all) {
-  static_assert(!T::FP, "integer path only");
+  static_assert(!T::WIDE, "integer path only");
   using acc = typename T::acc_t;
   using tt = typename T::tot_t;
   using G = typename T::gain_t;
@@ -816,7 +813,7 @@ __global__ __launch_bounds__(256) void k_decide_hpipe(DevGraph<T> g, DevState<T>
       for (int k = 0; k < EPL; k++) {
         if (lane + k * W < dB) {
           uC[k] = g.nbr[oB + lane + k * W];
-          wC[k] = (acc)load_w<T>(g.w, oB + lane + k * W);
+          wC[k] = (acc)load_w<T>(g, oB + lane + k * W);
         }
       }
     }
@@ -857,7 +854,7 @@ __global__ __launch_bounds__(256) void k_decide_hpipe(DevGraph<T> g, DevState<T>
 // the Tot and size of its own community-view entry, and clears the slot.  The
 // tables are zeroed once at kernel start and stay zero between vertices, so
 // the per-vertex cost is O(d / W) per lane instead of O(next_pow2(2d) / W),
-// and a slot is 8 bytes of LDS (12 on the FP path).
+// and a slot is 8 bytes of LDS (12 on the wide path).
 // ---------------------------------------------------------------------------
 // FIRST: the first sweep of a round (singleton partition, duplicate-free
 // rows): every entry other than the self-loop is its own candidate with the
@@ -1009,7 +1006,7 @@ __global__ __launch_bounds__(256) void k_decide_hlead(DevGraph<T> g, DevState<T>
       for (int k = 0; k < EPL; k++) {
         if (lane + k * W < dB) {
           uC[k] = g.nbr[oB + lane + k * W];
-          wC[k] = (acc)load_w<T>(g.w, oB + lane + k * W);
+          wC[k] = (acc)load_w<T>(g, oB + lane + k * W);
         }
       }
     }
@@ -1095,7 +1092,7 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
       for (int r = 0; r < U; r++) {
         const int j = j0 + r * W;
         un[r] = j < d ? g.nbr[o + j] : -1;
-        wv[r] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+        wv[r] = j < d ? (acc)load_w<T>(g, o + j) : acc(0);
       }
 #pragma unroll
       for (int r = 0; r < U; r++)
@@ -1213,7 +1210,7 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
       for (int q = 0; q < U; q++) {
         const int j = j0 + q * NT;
         cl[q] = j < d ? g.nbr[o + j] : -1;
-        wv[q] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+        wv[q] = j < d ? (acc)load_w<T>(g, o + j) : acc(0);
       }
 #pragma unroll
       for (int q = 0; q < U; q++) cl[q] = cl[q] >= 0 ? st.prev[cl[q]] : -1;
@@ -1332,7 +1329,7 @@ __global__ __launch_bounds__(NT) void k_decide_blead(DevGraph<T> g, DevState<T> 
     for (int k = 0; k < EPL; k++) {
       const int j = threadIdx.x + k * NT;
       u[k] = j < d ? g.nbr[o + j] : -1;
-      wv[k] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+      wv[k] = j < d ? (acc)load_w<T>(g, o + j) : acc(0);
     }
 #pragma unroll
     for (int k = 0; k < EPL; k++) eu[k] = u[k] >= 0 ? st.cv[u[k]] : CV{};
@@ -1470,7 +1467,7 @@ __global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> 
       for (int r = 0; r < U; r++) {
         const int j = j0 + r * NT;
         cu[r] = j < d ? g.nbr[o + j] : -1;
-        wv[r] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+        wv[r] = j < d ? (acc)load_w<T>(g, o + j) : acc(0);
       }
 #pragma unroll
       for (int r = 0; r < U; r++) cu[r] = cu[r] >= 0 ? st.prev[cu[r]] : -1;
@@ -1663,7 +1660,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(DevGraph<T> g, DevState<T> st, 
     for (int r = 0; r < PER; r++) {
       const int j = j0 + r * NT + threadIdx.x;
       cu[r] = j < j1 ? g.nbr[o + j] : -1;
-      wv[r] = j < j1 ? (acc)load_w<T>(g.w, o + j) : acc(0);
+      wv[r] = j < j1 ? (acc)load_w<T>(g, o + j) : acc(0);
     }
 #pragma unroll
     for (int r = 0; r < PER; r++) cu[r] = cu[r] >= 0 ? st.prev[cu[r]] : -1;
@@ -1810,7 +1807,7 @@ __global__ __launch_bounds__(1024) void k_hub_resolve(DevGraph<T> g, DevState<T>
       for (int r = 0; r < U; r++) {
         const int j = j0 + r * gridDim.y * NT;
         u[r] = j < d ? g.nbr[o + j] : -1;
-        w[r] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+        w[r] = j < d ? (acc)load_w<T>(g, o + j) : acc(0);
       }
 #pragma unroll
       for (int r = 0; r < U; r++) u[r] = u[r] >= 0 ? st.prev[u[r]] : -1;
@@ -1872,7 +1869,7 @@ __global__ __launch_bounds__(1024) void k_hub_part(DevGraph<T> g, DevState<T> st
         for (int r = 0; r < U; r++) {
           const int j = j0 + r * NT;
           cu[r] = j < d ? g.nbr[o + j] : -1;
-          wv[r] = j < d ? (acc)load_w<T>(g.w, o + j) : acc(0);
+          wv[r] = j < d ? (acc)load_w<T>(g, o + j) : acc(0);
         }
 #pragma unroll
         for (int r = 0; r < U; r++) cu[r] = cu[r] >= 0 ? st.prev[cu[r]] : -1;
@@ -2013,7 +2010,7 @@ __global__ __launch_bounds__(256) void k_decide_first(DevGraph<T> g, DevState<T>
     for (int j = lane; j < d; j += W) {
       const int u = g.nbr[o + j];
       if (u != v) {
-        const G hh = gain_h<T>((G)load_w<T>(g.w, o + j), (G)m2, (G)K, (G)st.Tot[u], (G)Tc);
+        const G hh = gain_h<T>((G)load_w<T>(g, o + j), (G)m2, (G)K, (G)st.Tot[u], (G)Tc);
         if (hh > 0 && better(hh, u, h, bd)) { h = hh; bd = u; }
       }
     }
@@ -2069,7 +2066,7 @@ __global__ __launch_bounds__(NT) void k_decide_first_block(DevGraph<T> g, DevSta
     for (int j = threadIdx.x; j < d; j += NT) {
       const int u = g.nbr[o + j];
       if (u != v) {
-        const G hh = gain_h<T>((G)load_w<T>(g.w, o + j), (G)m2, (G)K, (G)st.Tot[u], (G)Tc);
+        const G hh = gain_h<T>((G)load_w<T>(g, o + j), (G)m2, (G)K, (G)st.Tot[u], (G)Tc);
         if (hh > 0 && better(hh, u, h, bd)) { h = hh; bd = u; }
       }
     }
@@ -2141,10 +2138,10 @@ int decide_first_variants(int bin) {
 }
 
 // one thread per vertex on the integer path, the pipelined W-lane kernel on
-// the FP path (its per-entry FP64 totals would not fit the registers)
+// the wide path (its per-entry 8-byte totals would not fit the registers)
 #define TPV_OR(DMAX, FIRST, W, PER)                                                                       \
   do {                                                                                                     \
-    if constexpr (T::FP)                                                                                   \
+    if constexpr (T::WIDE)                                                                                   \
       k_decide_pipe<W, FIRST, T><<<grid_for(cap, PER, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2);   \
     else                                                                                                   \
       k_decide_tpv<DMAX, FIRST, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); \
@@ -2154,7 +2151,7 @@ template <class T>
 void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
                    const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s, int variant) {
   if (cap <= 0) return;
-  constexpr int HUB_LT = T::FP ? 8192 : 16384;
+  constexpr int HUB_LT = T::WIDE ? 8192 : 16384;
   if (variant < 0 || variant >= decide_variants(bin)) variant = 0;
   const int b = bin;
   switch (bin * 8 + variant) {
@@ -2192,20 +2189,20 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
     // bin 5: 128..511
     case 40: k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 41:
-      if constexpr (T::FP)  // 8 groups x 1024 slots of FP64 values and totals exceed the LDS
+      if constexpr (T::WIDE)  // 8 groups x 1024 slots of 8-byte sums and totals exceed the LDS
         k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       else
         k_decide_wave<1024, 32, T><<<grid_for(cap, 8, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       break;
     case 42:
-      if constexpr (T::FP)
+      if constexpr (T::WIDE)
         k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       else
         k_decide_hpipe<1024, 64, 8, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       break;
     case 43: k_decide_hlead<1024, 64, 8, false, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 44:
-      if constexpr (T::FP)  // 8 groups x 1024 FP64 slots exceed the LDS with the keys
+      if constexpr (T::WIDE)  // 8 groups x 1024 8-byte slots exceed the LDS with the keys
         k_decide_hlead<1024, 64, 8, false, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       else
         k_decide_hlead<1024, 32, 16, false, T><<<grid_for(cap, 32, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
@@ -2215,13 +2212,13 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
     case 49: k_decide_blead<256, 8192, 16, 1, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2, 0); break;
     // bin 7: 4096..8191
     case 56:
-      if constexpr (T::FP)
+      if constexpr (T::WIDE)
         k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0);
       else
         k_decide_block<1024, 16384, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
       break;
     case 57:
-      if constexpr (T::FP)  // 16384 FP64 slots exceed the LDS: the hub kernel
+      if constexpr (T::WIDE)  // 16384 8-byte slots exceed the LDS: the hub kernel
         k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0);
       else
         k_decide_blead<1024, 16384, 8, 1, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0);
@@ -2238,7 +2235,7 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
       if (variant == 2)
         k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, HUB_LT);
       if (variant == 3) {
-        if constexpr (T::FP)
+        if constexpr (T::WIDE)
           k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, above);
         else
           k_decide_blead<1024, 16384, 16, 2, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b,
@@ -2321,18 +2318,11 @@ void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const i
 template <class T>
 __global__ __launch_bounds__(256) void k_apply(DevState<T> st, int32_t n, typename T::tot_t m2) {
   using tt = typename T::tot_t;
-  __shared__ unsigned long long sh_t2[4];
-  __shared__ double sh_t2f[4];
+  __shared__ __int128 sh_t2[4];
   __shared__ unsigned long long sh_min[4];
   __shared__ int sh_ne[4];
-  unsigned long long t2 = 0, mn;
-  double t2f = 0;
-  if (T::FP) {
-    double m2d = (double)m2;
-    mn = __double_as_longlong(m2d);
-  } else {
-    mn = (unsigned long long)(long long)m2;
-  }
+  __int128 t2 = 0;  // sum Tot^2: <= m2^2 < 2^124 on the wide path
+  unsigned long long mn = (unsigned long long)(long long)m2;
   int ne = 0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     tt t = st.Tot[c] + st.dTot[c];
@@ -2342,84 +2332,69 @@ __global__ __launch_bounds__(256) void k_apply(DevState<T> st, int32_t n, typena
     st.dTot[c] = 0;
     st.dsize[c] = 0;
     ne += sz > 0;
-    if (T::FP) {
-      double tf = (double)t;
-      t2f += tf * tf;
-      if (tf > 0) {
-        unsigned long long b = (unsigned long long)__double_as_longlong(tf);
-        mn = b < mn ? b : mn;
-      }
-    } else {
-      long long ti = (long long)t;
-      t2 += (unsigned long long)(ti * ti);
-      if (ti != 0 && (unsigned long long)ti < mn) mn = (unsigned long long)ti;
-    }
+    const long long ti = (long long)t;
+    t2 += (__int128)ti * ti;
+    if (ti != 0 && (unsigned long long)ti < mn) mn = (unsigned long long)ti;
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  t2 = sum_xor<unsigned long long, 64>(t2);
-  t2f = sum_xor<double, 64>(t2f);
+  t2 = sum_xor<__int128, 64>(t2);
   ne = sum_xor<int, 64>(ne);
 #pragma unroll
   for (int k = 32; k >= 1; k >>= 1) {
     unsigned long long o = __shfl_xor(mn, k, 64);
     mn = o < mn ? o : mn;
   }
-  if (lane == 0) { sh_t2[wid] = t2; sh_t2f[wid] = t2f; sh_min[wid] = mn; sh_ne[wid] = ne; }
+  if (lane == 0) { sh_t2[wid] = t2; sh_min[wid] = mn; sh_ne[wid] = ne; }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int q = 1; q < 4; q++) {
-      t2 += sh_t2[q]; t2f += sh_t2f[q]; ne += sh_ne[q];
+      t2 += sh_t2[q]; ne += sh_ne[q];
       mn = sh_min[q] < mn ? sh_min[q] : mn;
     }
     BlockPart bp{};
-    bp.s0 = t2;
+    bp.s0 = (unsigned long long)t2;
+    bp.s0hi = (unsigned long long)(t2 >> 64);
     bp.s1 = mn;
-    bp.f0 = t2f;
-    bp.f1 = mn;
     bp.i0 = ne;
     st.part[blockIdx.x] = bp;
   }
 }
 
-// Combines the nb workgroup partials of k_apply (fixed order) into the counters.
+// Combines the nb workgroup partials of k_apply into the counters (the
+// counters were reset before the sweep; this one workgroup is the only writer
+// of sum Tot^2, so its two halves are stored, not added atomically).
 template <class T>
 __global__ __launch_bounds__(256) void k_apply_fin(DevState<T> st, int nb) {
-  __shared__ unsigned long long sh_t2[4], sh_min[4];
-  __shared__ double sh_t2f[4];
+  __shared__ __int128 sh_t2[4];
+  __shared__ unsigned long long sh_min[4];
   __shared__ long long sh_ne[4];
-  unsigned long long t2 = 0, mn = ~0ull;
-  double t2f = 0;
+  __int128 t2 = 0;
+  unsigned long long mn = ~0ull;
   long long ne = 0;
   for (int b = threadIdx.x; b < nb; b += 256) {
     const BlockPart p = st.part[b];
-    t2 += p.s0;
-    t2f += p.f0;
+    t2 += (__int128)(((unsigned __int128)p.s0hi << 64) | p.s0);
     ne += p.i0;
     mn = p.s1 < mn ? p.s1 : mn;
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  t2 = sum_xor<unsigned long long, 64>(t2);
-  t2f = sum_xor<double, 64>(t2f);
+  t2 = sum_xor<__int128, 64>(t2);
   ne = sum_xor<long long, 64>(ne);
 #pragma unroll
   for (int k = 32; k >= 1; k >>= 1) {
     unsigned long long o = __shfl_xor(mn, k, 64);
     mn = o < mn ? o : mn;
   }
-  if (lane == 0) { sh_t2[wid] = t2; sh_t2f[wid] = t2f; sh_min[wid] = mn; sh_ne[wid] = ne; }
+  if (lane == 0) { sh_t2[wid] = t2; sh_min[wid] = mn; sh_ne[wid] = ne; }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int q = 1; q < 4; q++) {
-      t2 += sh_t2[q]; t2f += sh_t2f[q]; ne += sh_ne[q];
+      t2 += sh_t2[q]; ne += sh_ne[q];
       mn = sh_min[q] < mn ? sh_min[q] : mn;
     }
-    if (T::FP) {
-      atomicAdd(&st.ctr->sum_tot2_f, t2f);
-      atomicMin((unsigned long long*)&st.ctr->min_tot_f, mn);
-    } else {
-      atomic_add_ull(&st.ctr->sum_tot2, t2);
-      atomicMin((unsigned long long*)&st.ctr->min_tot, mn);
-    }
+    st.ctr->sum_tot2 = (unsigned long long)t2;
+    st.ctr->sum_tot2_hi = (unsigned long long)(t2 >> 64);
+    atomicMin((unsigned long long*)&st.ctr->min_tot, mn);
     if (ne) atomicAdd(&st.ctr->nonempty, (int)ne);
   }
 }
@@ -2442,13 +2417,15 @@ __device__ __forceinline__ uint8_t prune_keep(typename T::tot_t in, typename T::
                                               typename T::tot_t S, typename T::tot_t Tc,
                                               typename T::tot_t minTot, typename T::tot_t m2) {
   using G = typename T::gain_t;
-  G P = (G)m2 * (2 * (G)in - (G)S - (G)K) + (G)K * ((G)minTot - ((G)Tc - (G)K));
+  // the bracketed differences are sums of at most 2*m2 in magnitude: int64
+  const int64_t a = 2 * (int64_t)in - (int64_t)S - (int64_t)K;
+  const int64_t b = (int64_t)minTot - ((int64_t)Tc - (int64_t)K);
+  const G P = (G)(int64_t)m2 * (G)a + (G)(int64_t)K * (G)b;
   return P > 0 ? 0 : 1;
 }
 
 template <class T>
 __device__ __forceinline__ typename T::tot_t read_min_tot(const Counters* c) {
-  if (T::FP) return (typename T::tot_t)c->min_tot_f;
   return (typename T::tot_t)c->min_tot;
 }
 
@@ -2468,7 +2445,7 @@ __device__ __forceinline__ void row_in_span(const DevGraph<T>& g, const DevState
     for (int r = 0; r < WR_U; r++) {
       const int j = j0 + r * NT;
       u[r] = j < d ? g.nbr[o + j] : -1;
-      w[r] = j < d ? (tt)load_w<T>(g.w, o + j) : tt(0);
+      w[r] = j < d ? (tt)load_w<T>(g, o + j) : tt(0);
     }
 #pragma unroll
     for (int r = 0; r < WR_U; r++) cu[r] = u[r] >= 0 ? st.cur[u[r]] : -1;
@@ -2527,7 +2504,7 @@ __global__ __launch_bounds__(256) void k_weight_tpv(DevGraph<T> g, DevState<T> s
 #pragma unroll
     for (int k = 0; k < DMAX; k++) {
       us[k] = k < d ? g.nbr[o + k] : v;
-      ws[k] = k < d ? (tt)load_w<T>(g.w, o + k) : tt(0);
+      ws[k] = k < d ? (tt)load_w<T>(g, o + k) : tt(0);
     }
     tt x = 0;
     int mvn = 0;
@@ -2618,7 +2595,7 @@ __global__ __launch_bounds__(256) void k_weight_pipe(DevGraph<T> g, DevState<T> 
       for (int k = 0; k < EPL; k++) {
         if (lane + k * W < dC) {
           uD[k] = g.nbr[oC + lane + k * W];
-          wD[k] = (acc)load_w<T>(g.w, oC + lane + k * W);
+          wD[k] = (acc)load_w<T>(g, oC + lane + k * W);
         }
       }
     }
@@ -2675,7 +2652,7 @@ __global__ __launch_bounds__(NT) void k_weight_block(DevGraph<T> g, DevState<T> 
 }
 
 // ---------------------------------------------------------------------------
-// Incremental weight update (integer path): only the rows of the vertices that
+// Incremental weight update: only the rows of the vertices that
 // moved in the sweep (cur != prev, read as one {cur, prev} pair) are read.  A mover v recomputes In[v] over
 // its row; for every neighbour u that did not move, the entry's contribution
 // to In[u] changes by w * ([cur[v] == c_u] - [prev[v] == c_u]), which v adds
@@ -2704,7 +2681,7 @@ __device__ __forceinline__ typename T::tot_t moved_row_span(const DevGraph<T>& g
     for (int r = 0; r < MV_U; r++) {
       const int j = j0 + r * NT;
       u[r] = j < d ? g.nbr[o + j] : -1;
-      w[r] = j < d ? (tt)load_w<T>(g.w, o + j) : tt(0);
+      w[r] = j < d ? (tt)load_w<T>(g, o + j) : tt(0);
     }
 #pragma unroll
     for (int r = 0; r < MV_U; r++) cpu[r] = u[r] >= 0 ? st.cp[u[r]] : make_int2(-1, -2);  // {cur[u], prev[u]}
@@ -2776,7 +2753,6 @@ template <class T>
 void launch_weight_moved(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* order,
                          const int32_t* bstart, int cap, hipStream_t s, int variant) {
   if (cap <= 0) return;
-  static_assert(!T::FP, "integer path only: the FP path keeps a deterministic summation order");
   static const int lanes[3][6] = {{4, 8, 16, 32, 64, 64}, {4, 4, 8, 16, 32, 32}, {4, 4, 8, 8, 16, 32}};
   if (variant < 0 || variant > 2) variant = 0;
   if (bin <= 5) {
@@ -2841,10 +2817,8 @@ template <class T>
 __global__ __launch_bounds__(256) void k_prune(DevState<T> st, int32_t n, typename T::tot_t m2) {
   using tt = typename T::tot_t;
   __shared__ unsigned long long s_in[4], s_act[4];
-  __shared__ double s_inf[4];
   const tt minTot = read_min_tot<T>(st.ctr);
-  unsigned long long insum = 0, act = 0;
-  double insumf = 0;
+  unsigned long long insum = 0, act = 0;  // sum In <= m2 < 2^62
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     if (st.binid[v] < 0) continue;
     const tt x = st.In[v];
@@ -2853,48 +2827,40 @@ __global__ __launch_bounds__(256) void k_prune(DevState<T> st, int32_t n, typena
     if (st.track && !st.dirty[v]) st.nbr_moved[v] = 0.f;  // no neighbour moved
     st.dirty[v] = 0;
     act += keep;
-    if (T::FP) insumf += (double)x;
-    else insum += (unsigned long long)(long long)x;
+    insum += (unsigned long long)(long long)x;
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   insum = sum_xor<unsigned long long, 64>(insum);
   act = sum_xor<unsigned long long, 64>(act);
-  insumf = sum_xor<double, 64>(insumf);
-  if (lane == 0) { s_in[wid] = insum; s_act[wid] = act; s_inf[wid] = insumf; }
+  if (lane == 0) { s_in[wid] = insum; s_act[wid] = act; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int q = 1; q < 4; q++) { insum += s_in[q]; act += s_act[q]; insumf += s_inf[q]; }
+    for (int q = 1; q < 4; q++) { insum += s_in[q]; act += s_act[q]; }
     BlockPart bp{};
     bp.s0 = insum;
     bp.s1 = act;
-    bp.f0 = insumf;
     st.part[blockIdx.x] = bp;
   }
 }
 
-// Combines the nb workgroup partials of k_prune (fixed order) into the counters.
+// Combines the nb workgroup partials of k_prune into the counters.
 template <class T>
 __global__ __launch_bounds__(256) void k_prune_fin(DevState<T> st, int nb) {
   __shared__ unsigned long long s_in[4], s_act[4];
-  __shared__ double s_inf[4];
   unsigned long long insum = 0, act = 0;
-  double insumf = 0;
   for (int b = threadIdx.x; b < nb; b += 256) {
     const BlockPart p = st.part[b];
     insum += p.s0;
     act += p.s1;
-    insumf += p.f0;
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   insum = sum_xor<unsigned long long, 64>(insum);
   act = sum_xor<unsigned long long, 64>(act);
-  insumf = sum_xor<double, 64>(insumf);
-  if (lane == 0) { s_in[wid] = insum; s_act[wid] = act; s_inf[wid] = insumf; }
+  if (lane == 0) { s_in[wid] = insum; s_act[wid] = act; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int q = 1; q < 4; q++) { insum += s_in[q]; act += s_act[q]; insumf += s_inf[q]; }
-    if (T::FP) { if (insumf != 0) atomicAdd(&st.ctr->sum_in_f, insumf); }
-    else atomic_add_ull(&st.ctr->sum_in, insum);
+    for (int q = 1; q < 4; q++) { insum += s_in[q]; act += s_act[q]; }
+    atomic_add_ull(&st.ctr->sum_in, insum);
     atomic_add_ull(&st.ctr->active, act);
   }
 }
@@ -2922,8 +2888,10 @@ void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t 
                                  const int32_t*, int, bool, hipStream_t, int);                    \
   template void launch_prune<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t);
 GALA_INST(IntW)
-GALA_INST(FltW)
+GALA_INST(WideW)
 template void launch_weight_moved<IntW>(int, const DevGraph<IntW>&, DevState<IntW>&, const int32_t*, const int32_t*,
                                         int, hipStream_t, int);
+template void launch_weight_moved<WideW>(int, const DevGraph<WideW>&, DevState<WideW>&, const int32_t*,
+                                         const int32_t*, int, hipStream_t, int);
 
 }  // namespace gala

@@ -26,7 +26,7 @@ EXPORTS = [
     "gala_gen_rmat", "gala_set_logger", "gala_set_mlp", "gala_get_mlp", "gala_mlp_train",
     "gala_mlp_forward", "gala_mlp_grad", "gala_batch_set", "gala_batch_run", "gala_batch_graph_results",
     "gala_batch_communities", "gala_gen_sbm_batch", "gala_set_variants", "gala_get_variants", "gala_autotune",
-    "gala_variant_counts",
+    "gala_variant_counts", "gala_phase_modularity",
 ]
 N_VARIANT_SLOTS = 27  # kernel variant per bin: 9 general sweeps, 9 first sweeps, 9 weight updates
 
@@ -75,7 +75,8 @@ class Result(ctypes.Structure):
                 ("learned_frontier", ctypes.c_int64), ("learned_kept", ctypes.c_int64),
                 ("bin_first_bytes", ctypes.c_int64 * 9), ("bin_first_edges", ctypes.c_int64 * 9),
                 ("bin_first_ms", ctypes.c_double * 9), ("bin_launches", ctypes.c_int32 * 9),
-                ("bin_first_launches", ctypes.c_int32 * 9)]
+                ("bin_first_launches", ctypes.c_int32 * 9), ("wide", ctypes.c_int32),
+                ("weight_shift", ctypes.c_int32)]
 
     def to_dict(self):
         r = self.rounds
@@ -99,7 +100,8 @@ class Result(ctypes.Structure):
                     learned_frontier=self.learned_frontier, learned_kept=self.learned_kept,
                     bin_first_bytes=list(self.bin_first_bytes), bin_first_edges=list(self.bin_first_edges),
                     bin_first_ms=list(self.bin_first_ms), bin_launches=list(self.bin_launches),
-                    bin_first_launches=list(self.bin_first_launches))
+                    bin_first_launches=list(self.bin_first_launches), wide=self.wide,
+                    weight_shift=self.weight_shift)
 
 
 ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -159,6 +161,8 @@ def lib():
                                         I32P, I32P, I32P, ctypes.c_int64, I32P, U8P]
         L.gala_phase_contract.argtypes = [VP, ctypes.c_int32, I64P, I32P, I32P, I32P, I32P, I64P,
                                           I32P, I64P, I32P, I64P]
+        L.gala_phase_modularity.argtypes = [VP, ctypes.c_int32, I64P, I32P, VP, ctypes.c_int, I32P,
+                                            ctypes.POINTER(ctypes.c_double)]
         L.gala_read_bin.argtypes = [ctypes.c_char_p, ctypes.c_int, I32P, I64P, I64P, I32P, VP]
         L.gala_write_bin.argtypes = [ctypes.c_char_p, ctypes.c_int32, I64P, I32P, VP, ctypes.c_int]
         L.gala_ingest_text.argtypes = [ctypes.c_char_p, ctypes.c_int, I32P, I64P, I64P, I32P, VP]
@@ -421,10 +425,10 @@ class Engine:
                     d["end"] = np.ctypeslib.as_array(end, shape=(max(C, 1),))[:C].copy()
                     m = int(d["end"][-1]) if C else 0
                     d["nbr"] = np.ctypeslib.as_array(nbr, shape=(max(m, 1),))[:m].copy()
-                    if wi:
+                    if wi:  # exact integer weights (the wide path's scaled sums)
                         d["w"] = np.ctypeslib.as_array(wi, shape=(max(m, 1),))[:m].copy()
-                    if wf:
-                        d["w"] = np.ctypeslib.as_array(wf, shape=(max(m, 1),))[:m].copy()
+                    if wf:  # the wide path: the same sums on the input weights' scale
+                        d["wf"] = np.ctypeslib.as_array(wf, shape=(max(m, 1),))[:m].copy()
                 rounds.append(d)
 
         # no callback unless the caller collects its output: a round callback
@@ -580,6 +584,21 @@ class Engine:
         _check(lib().gala_phase_contract(*args, None, _p(end, ctypes.c_int64), _p(nbr, ctypes.c_int32),
                                          _p(ww, ctypes.c_int64)), "gala_phase_contract")
         return nid, C.value, end[:C.value], nbr[:m.value], ww[:m.value]
+
+
+    def phase_modularity(self, g, comm):
+        """Modularity of `comm` on g (gala_phase_modularity, host arrays)."""
+        comm = np.ascontiguousarray(comm, np.int32)
+        q = ctypes.c_double()
+        w = None if g.weights is None else g.weights
+        _check(lib().gala_phase_modularity(self.ctx, g.vertex_num, _p(g.degrees, ctypes.c_int64),
+                                           _p(g.neighbors, ctypes.c_int32),
+                                           None if w is None else w.ctypes.data_as(ctypes.c_void_p),
+                                           g.weight_mode if w is not None else W_NONE,
+                                           _p(comm, ctypes.c_int32), ctypes.byref(q)),
+               "gala_phase_modularity")
+        return q.value
+
 
 
 def louvain_gpu(graph, min_modularity=1e-6, device=0):

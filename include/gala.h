@@ -35,7 +35,8 @@ extern "C" {
 /* weight modes of the .bin weight section (out of band, as the reference's -w flag) */
 #define GALA_W_NONE 0 /* no weight section: every weight is 1 (graph.cpp:205-212) */
 #define GALA_W_I32 1  /* int32 weights (graph.cpp:201-204) */
-#define GALA_W_F32 2  /* IEEE-754 float32 weights in the same slot (FP64 accumulation) */
+#define GALA_W_F32 2  /* IEEE-754 float32 weights in the same slot, run in exact
+                         arithmetic on their values (scaled to integers) */
 
 typedef struct gala_ctx gala_ctx;
 
@@ -107,6 +108,12 @@ typedef struct {
   double bin_first_ms[9];      /* profile_bins=1 */
   int32_t bin_launches[9];     /* decide launches per bin, general sweeps */
   int32_t bin_first_launches[9];
+  /* Arithmetic of the run.  wide = 0: int32 sums, int64 gains (m2 <= 1e9).
+   * wide = 1: int64 sums, int128 gains (m2 < 2^62): float32 weights run as the
+   * exact integers w * 2^weight_shift, as do integer graphs with m2 > 1e9.
+   * Both are exact: Q and every decision are those of exact arithmetic. */
+  int32_t wide;
+  int32_t weight_shift;
 } gala_result;
 
 /* Version string of the library. */
@@ -163,6 +170,16 @@ int gala_phase_weight(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const i
 int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr,
                         const int32_t* w, const int32_t* comm, int32_t* out_C, int64_t* out_m,
                         int32_t* new_id, int64_t* out_end_incl, int32_t* out_nbr, int64_t* out_w);
+
+/* Modularity of the partition comm[n] (community ids in [0, n)) of a graph
+ * (modularity_op, kernel_functions.cuh:77-87, as louvain_main_process.cu:
+ * 538-548,990-999 evaluate it): Q = sum In / m2 - sum Tot^2 / m2^2 from exact
+ * integer sums (float32 weights scaled to integers as gala_run does), so it
+ * equals gala_run's Q for the same partition bit for bit.  Host arrays;
+ * GALA_E_INVALID for an id out of range or a non-positive weight,
+ * GALA_E_OVERFLOW when the scaled m2 reaches 2^62; an edgeless graph gives 0. */
+int gala_phase_modularity(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr, const void* w,
+                          int weight_mode, const int32_t* comm, double* q_out);
 
 /* ---- learned frontier (no counterpart in the reference) ----
  * A 6 -> 64 -> 1 MLP (BF16 MFMA, FP32 accumulation, FP32 master weights)

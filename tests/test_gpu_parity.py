@@ -441,27 +441,3 @@ def test_full_size_chung_lu():
     tot = np.bincount(comm, weights=deg, minlength=g.vertex_num).astype(np.int64)
     q = s_in / m2 - float(int((tot * tot).sum())) / (m2 * m2)
     assert q == res["modularity"]
-
-
-@pytest.mark.parametrize("scale", [12, 15])
-def test_weighted_fp32_rmat(scale):
-    # fp32 weights U[0.5, 2], FP64 accumulation: final Q within 1e-9 of the FP64 oracle
-    say(f"weighted fp32 rmat scale {scale}")
-    g = gala.gen_rmat(scale, 16, seed=scale)
-    eng = gala.Engine()
-    eng.set_graph(g)
-    res = eng.run()
-    ref = oracle.louvain_f(g.vertex_num, g.degrees, g.neighbors, g.weights)
-    assert abs(res["modularity"] - ref["Q"]) <= 1e-9, (res["modularity"], ref["Q"])
-    assert res["n_communities"] == ref["nc"]
-    comm = eng.communities(g.vertex_num)
-    # the reported Q is the modularity of the returned partition
-    off = g.offsets()
-    src = np.repeat(np.arange(g.vertex_num), np.diff(off))
-    w = g.weights.astype(np.float64)
-    m2 = w.sum()
-    K = np.bincount(src, weights=w, minlength=g.vertex_num)
-    tot = np.bincount(comm, weights=K, minlength=g.vertex_num)
-    q = w[comm[g.neighbors] == comm[src]].sum() / m2 - (tot ** 2).sum() / m2 ** 2
-    assert abs(q - res["modularity"]) <= 1e-9
-    eng.close()
