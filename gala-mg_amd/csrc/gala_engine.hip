@@ -1392,6 +1392,8 @@ int gala_create(gala_ctx** out, int device, const gala_opts* opts) {
   if (opts) ctx->opts = *opts;
   else gala_default_opts(&ctx->opts);
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return GALA_E_HIP; }
+  // device address of the no-allocation entry point's static scratch on this GPU
+  if (int rs = modularity_scratch_setup(device)) { (void)hipStreamDestroy(ctx->stream); delete ctx; return rs; }
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) { gala_destroy(ctx); return GALA_E_HIP; }
   for (auto& e : ctx->bev)
@@ -1661,6 +1663,14 @@ int gala_phase_modularity(gala_ctx* ctx, int32_t n, const int64_t* end_incl, con
   if (weight_mode != GALA_W_NONE && n > 0 && end_incl[n - 1] > 0 && !w) return GALA_E_INVALID;
   (void)hipSetDevice(ctx->device);
   return modularity_host(ctx->device, ctx->stream, n, end_incl, nbr, w, weight_mode, comm, q_out);
+}
+
+int gala_modularity_noalloc(gala_ctx* ctx, int32_t n, const int64_t* d_end_incl, const int32_t* d_nbr,
+                            const void* d_w, int weight_mode, const int32_t* d_comm, double* q_out) {
+  if (!ctx || n < 0 || !q_out || weight_mode < 0 || weight_mode > 2) return GALA_E_INVALID;
+  if (n > 0 && (!d_end_incl || !d_nbr || !d_comm || (weight_mode != GALA_W_NONE && !d_w))) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  return modularity_noalloc(ctx->device, ctx->stream, n, d_end_incl, d_nbr, d_w, weight_mode, d_comm, q_out);
 }
 
 int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr,

@@ -305,7 +305,10 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
 void compose_map(int32_t* fin, int32_t n0, const int32_t* label, hipStream_t s);
 
 // ---- modularity of a given partition (modularity.hip) ----
+int modularity_scratch_setup(int device);
 int modularity_host(int device, hipStream_t s, int32_t n, const int64_t* end_incl, const int32_t* nbr, const void* w,
                     int wmode, const int32_t* comm, double* q_out);
+int modularity_noalloc(int device, hipStream_t s, int32_t n, const int64_t* d_end_incl, const int32_t* d_nbr,
+                       const void* d_w, int wmode, const int32_t* d_comm, double* q_out);
 
 }  // namespace gala

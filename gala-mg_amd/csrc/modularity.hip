@@ -10,8 +10,14 @@
 // engine's wide path does) and Q is finished with three FP64 operations, so it
 // equals the engine's per-iteration Q bit for bit.
 //
-// gala_phase_modularity takes host arrays and allocates its device copies
-// and its scratch (Tot per community, workgroup partials).
+// Two entry points share the kernels:
+//   gala_phase_modularity   host arrays in; allocates its device copies and
+//                           its scratch (Tot per community, workgroup partials)
+//   gala_modularity_noalloc device arrays in; allocates nothing: its scratch is
+//                           the file-scope array `scratch` below, one copy per
+//                           GPU, held until the process ends.
+#include <mutex>
+
 #include "../../include/gala.h"
 #include "gala_device.h"
 #include "gala_kernels.h"
@@ -45,6 +51,20 @@ inline int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
 inline int64_t off_parts(int64_t n) { return align256(8 * n); }
 inline int64_t off_scal(int64_t n) { return off_parts(n) + align256(sizeof(ModPart) * MOD_PARTS); }
 inline int64_t scratch_bytes(int64_t n) { return off_scal(n) + align256(sizeof(ModScalars)); }
+
+// The no-allocation entry point's scratch: communities of graphs of up to
+// GALA_NOALLOC_MAX_N vertices (32 MiB of Tot plus the partials).
+constexpr int64_t NOALLOC_BYTES = ((8 * (int64_t)GALA_NOALLOC_MAX_N + 255) & ~int64_t(255)) +
+                                  ((sizeof(ModPart) * MOD_PARTS + 255) & ~int64_t(255)) +
+                                  ((sizeof(ModScalars) + 255) & ~int64_t(255));
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// Device address of `scratch` on every GPU (each holds its own copy).  The
+// array's name is a host-side stand-in: only these addresses are passed to
+// kernels.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
 
 __device__ __forceinline__ unsigned __int128 u128(unsigned long long lo, unsigned long long hi) {
   return ((unsigned __int128)hi << 64) | lo;
@@ -243,6 +263,17 @@ int finish_q(const ModScalars& r, double* q_out) {
 
 }  // namespace
 
+// Records the address of `scratch` on `device` (the current device), once.
+int modularity_scratch_setup(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return GALA_E_INVALID;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return GALA_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GALA_E_HIP;
+  g_scratch_ptr[device] = p;
+  return GALA_OK;
+}
+
 int modularity_host(int device, hipStream_t s, int32_t n, const int64_t* end_incl, const int32_t* nbr, const void* w,
                     int wmode, const int32_t* comm, double* q_out) {
   const int64_t m = n > 0 ? end_incl[n - 1] : 0;
@@ -266,6 +297,28 @@ int modularity_host(int device, hipStream_t s, int32_t n, const int64_t* end_inc
   if (!rc) rc = modularity_on(n, m, d_end, d_nbr, d_w, wmode, d_comm, d_scr, s, &r);
   (void)hipFree(d_buf);
   (void)hipFree(d_scr);
+  if (rc) return rc;
+  return finish_q(r, q_out);
+}
+
+int modularity_noalloc(int device, hipStream_t s, int32_t n, const int64_t* d_end_incl, const int32_t* d_nbr,
+                       const void* d_w, int wmode, const int32_t* d_comm, double* q_out) {
+  if (n > GALA_NOALLOC_MAX_N) return GALA_E_INVALID;
+  if (device < 0 || device >= MAX_DEVICES) return GALA_E_INVALID;
+  void* base;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_ptr[device];
+  }
+  if (!base) return GALA_E_INVALID;  // gala_create sets it up
+  // m: the last inclusive row end, read from the caller's device array
+  int64_t m = 0;
+  if (n > 0) {
+    if (hipMemcpyAsync(&m, d_end_incl + (n - 1), 8, hipMemcpyDeviceToHost, s) != hipSuccess) return GALA_E_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return GALA_E_HIP;
+  }
+  ModScalars r{};
+  const int rc = modularity_on(n, m, d_end_incl, d_nbr, d_w, wmode, d_comm, (unsigned char*)base, s, &r);
   if (rc) return rc;
   return finish_q(r, q_out);
 }

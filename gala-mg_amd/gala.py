@@ -26,8 +26,9 @@ EXPORTS = [
     "gala_gen_rmat", "gala_set_logger", "gala_set_mlp", "gala_get_mlp", "gala_mlp_train",
     "gala_mlp_forward", "gala_mlp_grad", "gala_batch_set", "gala_batch_run", "gala_batch_graph_results",
     "gala_batch_communities", "gala_gen_sbm_batch", "gala_set_variants", "gala_get_variants", "gala_autotune",
-    "gala_variant_counts", "gala_phase_modularity",
+    "gala_variant_counts", "gala_phase_modularity", "gala_modularity_noalloc",
 ]
+NOALLOC_MAX_N = 1 << 22  # GALA_NOALLOC_MAX_N
 N_VARIANT_SLOTS = 27  # kernel variant per bin: 9 general sweeps, 9 first sweeps, 9 weight updates
 
 
@@ -163,6 +164,9 @@ def lib():
                                           I32P, I64P, I32P, I64P]
         L.gala_phase_modularity.argtypes = [VP, ctypes.c_int32, I64P, I32P, VP, ctypes.c_int, I32P,
                                             ctypes.POINTER(ctypes.c_double)]
+        L.gala_modularity_noalloc.argtypes = [VP, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                              ctypes.POINTER(ctypes.c_double)]
         L.gala_read_bin.argtypes = [ctypes.c_char_p, ctypes.c_int, I32P, I64P, I64P, I32P, VP]
         L.gala_write_bin.argtypes = [ctypes.c_char_p, ctypes.c_int32, I64P, I32P, VP, ctypes.c_int]
         L.gala_ingest_text.argtypes = [ctypes.c_char_p, ctypes.c_int, I32P, I64P, I64P, I32P, VP]
@@ -599,6 +603,12 @@ class Engine:
                "gala_phase_modularity")
         return q.value
 
+    def modularity_noalloc(self, n, d_end_incl, d_nbr, d_w, weight_mode, d_comm):
+        """gala_modularity_noalloc on device addresses (ints): returns (rc, Q)."""
+        q = ctypes.c_double()
+        rc = lib().gala_modularity_noalloc(self.ctx, n, d_end_incl, d_nbr, d_w, weight_mode, d_comm,
+                                           ctypes.byref(q))
+        return rc, q.value
 
 
 def louvain_gpu(graph, min_modularity=1e-6, device=0):

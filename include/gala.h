@@ -181,6 +181,20 @@ int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const
 int gala_phase_modularity(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr, const void* w,
                           int weight_mode, const int32_t* comm, double* q_out);
 
+/* The same from DEVICE arrays (on ctx's device, read on ctx's stream), with
+ * no allocation of any kind: no counterpart in the reference; for callers
+ * that cannot allocate device memory (serving loops, code under stream
+ * capture, harnesses that forbid hipMalloc).  Its scratch is a static device
+ * array of the library, one copy per GPU, sized for n <= GALA_NOALLOC_MAX_N
+ * (32 MiB of per-community totals plus 192 KiB of partials), held until the
+ * process ends; larger n gives GALA_E_INVALID.  Every call on one GPU uses
+ * that GPU's copy: calls that overlap on one GPU (several threads, several
+ * contexts or streams) race and must be serialised by the caller.  Blocks
+ * until q_out is written. */
+#define GALA_NOALLOC_MAX_N (1 << 22)
+int gala_modularity_noalloc(gala_ctx* ctx, int32_t n, const int64_t* d_end_incl, const int32_t* d_nbr,
+                            const void* d_w, int weight_mode, const int32_t* d_comm, double* q_out);
+
 /* ---- learned frontier (no counterpart in the reference) ----
  * A 6 -> 64 -> 1 MLP (BF16 MFMA, FP32 accumulation, FP32 master weights)
  * predicts which vertices the next sweep moves; with opts.frontier_mode = 1 a
