@@ -186,7 +186,7 @@ int gala_phase_modularity(gala_ctx* ctx, int32_t n, const int64_t* end_incl, con
  * that cannot allocate device memory (serving loops, code under stream
  * capture, harnesses that forbid hipMalloc).  Its scratch is a static device
  * array of the library, one copy per GPU, sized for n <= GALA_NOALLOC_MAX_N
- * (32 MiB of per-community totals plus 192 KiB of partials), held until the
+ * (32 MiB of per-community totals plus 128 KiB of partials), held until the
  * process ends; larger n gives GALA_E_INVALID.  Every call on one GPU uses
  * that GPU's copy: calls that overlap on one GPU (several threads, several
  * contexts or streams) race and must be serialised by the caller.  Blocks
