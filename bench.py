@@ -4,7 +4,9 @@ unweighted) on one MI355X.
 
 One step = one full multi-round Louvain run (all rounds, contraction included)
 on the graph already resident in HBM (the input CSR stays read-only on the
-device between steps).  value = edge-TEPS = undirected input edges / T_e2e.
+device between steps).  value = edge-TEPS = undirected input edges / T_e2e,
+with T_e2e the host wall clock of the timed steps (each run ends with a
+device synchronisation); the engine's own HIP-event time is reported beside it.
 
 Also reported:
   roofline      the dominant decide-and-move kernel (largest total time in a run
@@ -13,15 +15,21 @@ Also reported:
                 degree d, 12*d + 40 with implicit unit weights; first sweeps
                 12*d + 40 / 8*d + 40) over its launch time; the whole decide
                 phase is reported beside it
+  contraction   the contraction phase against HBM peak: per round
+                12*2E + 8*2E' + 16*V + 8*C bytes (read the round CSR and the
+                community of every entry, write the contracted CSR) over its time
   cpu_baseline  the oracle (exact-integer C port of the same algorithm, OpenMP)
-                on the same graph on the host, 1 run
+                on the same graph on the host's CPU share, 1 run; plus one
+                thread on a bounded sample (a 500k-vertex graph of the same kind)
   parity        final partition of the last GPU step == oracle's, bit for bit
-  learned       the learned-frontier form on the same graph (network trained on
-                SBM move logs, gala-mg_amd/models/frontier_mlp.json): time per
-                run, speedup over the plain path and final-Q difference
+  learned       the learned-frontier form (network trained on SBM move logs,
+                gala-mg_amd/models/frontier_mlp.json) on held-out seeds
+                1001-1003 of the same workload: time per run, speedup over the
+                plain path on the same graph and final-Q difference
   batched       4,096 independent SBMs run as one union against the same graphs
                 run one at a time
-  weighted_rmat R-MAT scale 24 with fp32 weights (FP64 accumulation)
+  weighted_rmat R-MAT scale 24 with fp32 weights (run exactly: the weights
+                scaled to integers, int64 sums, int128 gains)
 """
 import argparse
 import json
@@ -48,9 +56,10 @@ KERNEL_GENERAL = [
     ["k_decide_wave<256, 64, IntW>", "k_decide_wave<256, 32, IntW>", "k_decide_wave<256, 16, IntW>",
      "k_decide_hpipe<256, 32, 4, IntW>", "k_decide_hpipe<256, 16, 8, IntW>",
      "k_decide_hlead<256, 32, 4, false, IntW>", "k_decide_hlead<256, 16, 8, false, IntW>",
-     "k_decide_hlead<256, 8, 16, false, IntW>"],
+     "k_decide_hlead<256, 8, 16, false, IntW>", "k_decide_glead<256, 16, 8, false, IntW>"],
     ["k_decide_wave<1024, 64, IntW>", "k_decide_wave<1024, 32, IntW>", "k_decide_hpipe<1024, 64, 8, IntW>",
-     "k_decide_hlead<1024, 64, 8, false, IntW>", "k_decide_hlead<1024, 32, 16, false, IntW>"],
+     "k_decide_hlead<1024, 64, 8, false, IntW>", "k_decide_hlead<1024, 32, 16, false, IntW>",
+     "k_decide_glead<1024, 64, 8, false, IntW>"],
     ["k_decide_block<256, 8192, IntW>", "k_decide_blead<256, 8192, 16, 1, IntW>"],
     ["k_decide_block<1024, 16384, IntW>", "k_decide_blead<1024, 16384, 8, 1, IntW>"],
     ["k_decide_hub<16384, IntW>", "k_hub_plan/agg/scan/commit<IntW>",
@@ -65,8 +74,9 @@ KERNEL_FIRST = [
     ["k_decide_first<16, IntW>", "k_decide_pipe<16, true, IntW>", "k_decide_tpv<15, true, IntW>"],
     ["k_decide_first<32, IntW>", "k_decide_pipe<32, true, IntW>", "k_decide_tpv<31, true, IntW>",
      "k_decide_hlead<1, 16, 2, true, IntW>"],
-    ["k_decide_first<64, IntW>", "k_decide_first<32, IntW>", "k_decide_hlead<1, 16, 8, true, IntW>"],
-    ["k_decide_first<64, IntW>", "k_decide_hlead<1, 64, 8, true, IntW>"],
+    ["k_decide_first<64, IntW>", "k_decide_first<32, IntW>", "k_decide_hlead<1, 16, 8, true, IntW>",
+     "k_decide_glead<1, 16, 8, true, IntW>"],
+    ["k_decide_first<64, IntW>", "k_decide_hlead<1, 64, 8, true, IntW>", "k_decide_glead<1, 64, 8, true, IntW>"],
     ["k_decide_first_block<256, IntW>"],
     ["k_decide_first_block<1024, IntW>"],
     ["k_decide_first_block<1024, IntW>"]]
@@ -106,31 +116,78 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def learned_leg(gala, g, args, plain):
-    """Same graph, learned frontier: K timed runs after one warmup."""
+def cpu_threads(args):
+    """Host threads for the CPU legs: the CPU share this process may use (its
+    affinity, and OMP_NUM_THREADS where the host sets it), unless given."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+
+
+def contraction_roofline(r):
+    """Contraction against HBM peak: per round 12*2E + 8*2E' + 16*V + 8*C bytes
+    (read the round CSR with each entry's community, write the contracted CSR)."""
+    rounds = []
+    tot_b = 0
+    for i in range(r["rounds"]):
+        V, m_in, m_out, C = (r["round_vertices"][i], r["round_entries"][i], r["round_contracted_entries"][i],
+                             r["round_nc"][i])
+        b = 12 * m_in + 8 * m_out + 16 * V + 8 * C
+        ms = r["round_build_ms"][i]
+        tot_b += b
+        rounds.append({"round": i, "vertices": V, "entries": m_in, "contracted_entries": m_out,
+                       "communities": C, "bytes": b, "ms": ms,
+                       "GBps": b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0})
+    ms = r["t_contract_ms"]
+    gbs = tot_b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "algorithmic_bytes": tot_b, "time_ms": ms, "rounds": rounds}
+
+
+def learned_leg(gala, args):
+    """Learned frontier on held-out seeds 1001-1003 of the workload (the network
+    never saw them: it is trained on 1,024-vertex SBM move logs).  Per seed:
+    the plain path and the learned form on the same graph, K timed runs each
+    after one warmup."""
     import numpy as np
     import train_frontier as TF
     if not os.path.exists(TF.DEFAULT_MODEL):
         log("learned leg skipped: no trained model")
         return None
     params, tau = TF.load(TF.DEFAULT_MODEL)
-    eng = gala.Engine(device=0, frontier_mode=1, learned_tau=tau)
-    eng.set_mlp(params)
-    eng.set_graph(g)
-    eng.run()
-    runs = [eng.run() for _ in range(args.steps)]
-    eng.close()
-    r = runs[-1]
-    ms = float(np.mean([x["t_total_ms"] for x in runs]))
-    plain_ms = plain["t_total_ms"]
-    out = {"ms_per_step": ms, "speedup_vs_plain": plain_ms / ms, "plain_ms": plain_ms,
-           "modularity": r["modularity"], "dQ": r["modularity"] - plain["modularity"], "tau": tau,
-           "learned_sweeps": r["learned_sweeps"], "certify_sweeps": r["certify_sweeps"],
-           "kept_fraction": (r["learned_kept"] / r["learned_frontier"]) if r["learned_frontier"] else None,
-           "swept_edges": r["swept_edges"], "plain_swept_edges": plain["swept_edges"],
-           "phase_ms": {"decide": r["t_decide_ms"], "update": r["t_update_ms"], "contract": r["t_contract_ms"]}}
-    log(f"learned: {ms:.2f} ms (plain {plain_ms:.2f}) Q={r['modularity']:.6f} dQ={out['dQ']:+.6f}")
-    return out
+    per_seed = []
+    for seed in (1001, 1002, 1003):
+        g = gala.gen_chung_lu(args.n, 2.1, 17.35, 15000.0, seed)
+        res = {}
+        for mode in (0, 1):
+            eng = gala.Engine(device=0, frontier_mode=mode, learned_tau=tau)
+            if mode:
+                eng.set_mlp(params)
+            eng.set_graph(g)
+            eng.run()
+            runs = [eng.run() for _ in range(args.steps)]
+            eng.close()
+            res[mode] = (float(np.mean([x["t_total_ms"] for x in runs])), runs[-1])
+        (pms, rp), (lms, rl) = res[0], res[1]
+        per_seed.append({"seed": seed, "plain_ms": pms, "learned_ms": lms, "speedup_vs_plain": pms / lms,
+                         "plain_q": rp["modularity"], "learned_q": rl["modularity"],
+                         "dQ": rl["modularity"] - rp["modularity"], "learned_sweeps": rl["learned_sweeps"],
+                         "certify_sweeps": rl["certify_sweeps"],
+                         "kept_fraction": (rl["learned_kept"] / rl["learned_frontier"]) if rl["learned_frontier"] else None,
+                         "swept_edges": rl["swept_edges"], "plain_swept_edges": rp["swept_edges"],
+                         "phase_ms": {"decide": rl["t_decide_ms"], "update": rl["t_update_ms"],
+                                      "contract": rl["t_contract_ms"]},
+                         "plain_phase_ms": {"decide": rp["t_decide_ms"], "update": rp["t_update_ms"],
+                                            "contract": rp["t_contract_ms"]}})
+        log(f"learned seed {seed}: {lms:.2f} ms vs plain {pms:.2f} ms ({pms / lms:.2f}x) "
+            f"dQ={rl['modularity'] - rp['modularity']:+.6f}")
+    sp = [x["speedup_vs_plain"] for x in per_seed]
+    return {"tau": tau, "seeds": per_seed, "speedup_vs_plain": float(np.mean(sp)),
+            "ms_per_step": float(np.mean([x["learned_ms"] for x in per_seed])),
+            "plain_ms": float(np.mean([x["plain_ms"] for x in per_seed])),
+            "dQ": max((x["dQ"] for x in per_seed), key=abs)}
 
 
 def batch_leg(gala, args):
@@ -199,8 +256,10 @@ def batch_leg(gala, args):
 
 def rmat_leg(gala, args):
     """Weighted R-MAT: scale S (2^S vertices), edge factor 16, a=0.57 b=c=0.19,
-    fp32 weights uniform [0.5, 2.0], self-loops/duplicates merged; FP64
-    accumulation.  Optionally checked against the FP64 oracle (Q within 1e-9)."""
+    fp32 weights uniform [0.5, 2.0], self-loops/duplicates merged; run exactly
+    (weights scaled to integers, int64 sums, int128 gains).  Optionally checked
+    against the exact oracle (identical partition and Q) and the FP64 oracle
+    (Q within 1e-9)."""
     import numpy as np
     t0 = time.time()
     g = gala.gen_rmat(args.rmat_scale, 16, 0.57, 0.19, 0.19, 0, 0.5, 2.0)
@@ -221,18 +280,26 @@ def rmat_leg(gala, args):
            "edge_teps": g.edge_num / (ms * 1e-3), "modularity": r["modularity"], "rounds": r["rounds"],
            "iterations": r["total_iterations"],
            "phase_ms": {"decide": r["t_decide_ms"], "update": r["t_update_ms"], "contract": r["t_contract_ms"]},
-           "decide_GBps": dm_gbs, "dtype": "f64 (fp32 weights)", "oracle_dq": None}
+           "decide_GBps": dm_gbs, "wide": r["wide"], "weight_shift": r["weight_shift"],
+           "dtype": "int64 sums / int128 gains (fp32 weights scaled exactly)", "oracle_dq": None,
+           "contraction": contraction_roofline(r)}
     log(f"rmat: {ms:.1f} ms Q={r['modularity']:.9f} rounds {r['rounds']} decide {dm_gbs:.0f} GB/s")
     if args.rmat_oracle:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        oracle.set_threads(args.cpu_threads or min(16, os.cpu_count() or 1))
+        oracle.set_threads(cpu_threads(args))
+        t2 = time.perf_counter()
+        W, _ = oracle.scale_f32(g.weights)
+        ex = oracle.louvain(g.vertex_num, g.degrees, g.neighbors, W)
+        out["exact_oracle_s"] = time.perf_counter() - t2
+        out["exact_oracle_identical"] = bool(np.array_equal(ex["final_comm"], comm)) and ex["Q"] == r["modularity"]
         t2 = time.perf_counter()
         ref = oracle.louvain_f(g.vertex_num, g.degrees, g.neighbors, g.weights)
         out["oracle_dq"] = r["modularity"] - ref["Q"]
         out["oracle_s"] = time.perf_counter() - t2
         out["oracle_same_partition"] = bool(np.array_equal(ref["final_comm"], comm))
-        log(f"rmat oracle: Q={ref['Q']:.12f} dQ={out['oracle_dq']:.3e} ({out['oracle_s']:.1f} s)")
+        log(f"rmat oracles: exact identical {out['exact_oracle_identical']} ({out['exact_oracle_s']:.1f} s); "
+            f"FP64 dQ={out['oracle_dq']:.3e} ({out['oracle_s']:.1f} s)")
     return out
 
 
@@ -305,7 +372,7 @@ def main():
     wall = time.perf_counter() - t1
     comm = eng.communities(g.vertex_num)
     eng.close()
-    learned = None if args.no_learned else learned_leg(gala, g, args, runs[-1])
+    learned = None if args.no_learned else learned_leg(gala, args)
     batched = batch_leg(gala, args) if args.batch_graphs > 0 else None
     rmat = rmat_leg(gala, args) if args.rmat_scale > 0 else None
     # one more run with per-bin HIP events (outside the timed steps)
@@ -328,8 +395,9 @@ def main():
     # the dominant decide kernel: largest total time over one run
     dom = max(bins, key=lambda x: x["ms"])
 
-    ms = float(np.mean([r["t_total_ms"] for r in runs]))
+    dev_ms = float(np.mean([r["t_total_ms"] for r in runs]))
     wall_ms = wall * 1e3 / args.steps
+    ms = wall_ms  # the headline: host wall clock per step
     edges = g.edge_num
     last = runs[-1]
     # decide-and-move algorithmic bytes (per swept vertex of degree d): 16d + 40,
@@ -344,15 +412,32 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        th = args.cpu_threads or min(16, os.cpu_count() or 1)
+        th = cpu_threads(args)
         oracle.set_threads(th)
         t2 = time.perf_counter()
         ref = oracle.louvain(g.vertex_num, g.degrees, g.neighbors)
         cpu_s = time.perf_counter() - t2
         parity = bool(np.array_equal(ref["final_comm"], comm)) and ref["Q"] == last["modularity"]
-        cpu = {"value": edges / cpu_s, "unit": "edge-TEPS", "cores": th, "kind": "port",
-               "sample": f"full workload, one end-to-end run ({cpu_s:.1f} s, {th} OpenMP threads)"}
-        log(f"cpu oracle: {cpu_s:.2f} s Q={ref['Q']:.6f} parity={parity}")
+        cpu = {"value": edges / cpu_s, "unit": "edge-TEPS", "cores": th, "nproc": os.cpu_count(), "kind": "port",
+               "sample": f"full workload, one end-to-end run ({cpu_s:.1f} s, {th} OpenMP threads: the CPU share "
+                         f"of this process, nproc {os.cpu_count()})"}
+        log(f"cpu oracle: {cpu_s:.2f} s on {th} threads (nproc {os.cpu_count()}) Q={ref['Q']:.6f} parity={parity}")
+        # one thread, on a bounded sample of the same workload: a 500k-vertex
+        # Chung-Lu graph with the same exponent, mean degree and cap
+        gs = gala.gen_chung_lu(500_000, 2.1, 17.35, 15000.0, args.seed)
+        oracle.set_threads(1)
+        t2 = time.perf_counter()
+        oracle.louvain(gs.vertex_num, gs.degrees, gs.neighbors)
+        s1 = time.perf_counter() - t2
+        oracle.set_threads(th)
+        t2 = time.perf_counter()
+        oracle.louvain(gs.vertex_num, gs.degrees, gs.neighbors)
+        sn = time.perf_counter() - t2
+        cpu["single_thread"] = {"value": gs.edge_num / s1, "unit": "edge-TEPS", "cores": 1, "seconds": s1,
+                                "sample": f"Chung-Lu 500k vertices / {gs.edge_num} edges (gamma 2.1, mean degree "
+                                          f"17.35, cap 15,000, seed {args.seed}), one end-to-end run",
+                                "same_sample_all_threads": {"value": gs.edge_num / sn, "cores": th, "seconds": sn}}
+        log(f"cpu oracle sample: 1 thread {s1:.2f} s, {th} threads {sn:.2f} s")
 
     out = {
         "metric": "end-to-end Louvain s + edge-TEPS, 4M/35M power-law; dQ learned vs plain",
@@ -383,6 +468,9 @@ def main():
         "cpu_baseline": cpu,
         "seconds_e2e": ms * 1e-3,
         "wall_ms_per_step": wall_ms,
+        "device_ms_per_step": dev_ms,
+        "edge_teps_device": edges / (dev_ms * 1e-3),
+        "contraction": contraction_roofline(last),
         "modularity": last["modularity"],
         "communities": last["n_communities"],
         "rounds": last["rounds"],

@@ -108,7 +108,7 @@ struct gala_ctx {
   int32_t *c_cnt = nullptr, *c_nid = nullptr, *c_label = nullptr, *c_skey = nullptr, *c_dcnt = nullptr,
           *c_big_list = nullptr, *c_nbig = nullptr, *c_i32_tmp = nullptr, *c_i32_scalar = nullptr;
   int64_t *c_ecnt = nullptr, *c_eoff = nullptr, *c_ecur = nullptr, *c_big_tab = nullptr,
-          *c_i64_tmp = nullptr, *c_i64_scalar = nullptr;
+          *c_i64_tmp = nullptr, *c_i64_scalar = nullptr, *c_mtotal = nullptr;
   void* c_sval = nullptr;
   int8_t* c_tier = nullptr;
   int32_t *c_tlist = nullptr, *c_tstart = nullptr;
@@ -159,11 +159,11 @@ struct gala_ctx {
 
   // decide kernel variant per degree bin, general and first sweeps; the
   // defaults are what gala_autotune picks on the 4M-vertex power-law graph
-  // (one thread per vertex below degree 32, the pipelined leader-hash kernels
-  // for 32..511, leader evaluation in the workgroup kernels for 512..8191,
-  // every hub of 8192+ split by community over workgroups)
-  int dvar[NBINS] = {3, 3, 3, 3, 6, 3, 1, 1, 6};
-  int fvar[NBINS] = {1, 2, 2, 2, 2, 1, 0, 0, 0};
+  // (one thread per vertex below degree 32, the leader-hash kernels with
+  // resident workgroups for 32..511, leader evaluation in the workgroup kernels
+  // for 512..8191, every hub of 8192+ split by community over workgroups)
+  int dvar[NBINS] = {3, 3, 3, 3, 8, 5, 1, 1, 6};
+  int fvar[NBINS] = {1, 2, 2, 2, 3, 2, 0, 0, 0};
   // weight update: the incremental integer path with narrow lane groups
   // (variant 2); the dirty-row path (tracked runs) reads variant 2 as one
   // thread per vertex for bins 0-3
@@ -280,6 +280,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool wide) {
   OWN(ctx->c_ecnt, N + 1); OWN(ctx->c_eoff, N + 1); OWN(ctx->c_ecur, N);
   OWN(ctx->c_big_tab, N + 1); OWN(ctx->c_i64_tmp, 2 * (N + 1) + scan_tmp_elems(N));
   OWN(ctx->c_i64_scalar, 16);
+  OWN(ctx->c_mtotal, 1);
   if (!ctx->ctr_h) {
     hipError_t e = hipHostMalloc((void**)&ctx->ctr_h, 2 * sizeof(Counters), hipHostMallocDefault);
     if (e != hipSuccess) { ctx->last_hip = e; free_all(ctx); return GALA_E_OOM; }
@@ -811,7 +812,8 @@ ContractBufs<T> cbufs(gala_ctx* ctx) {
   cb.tier = ctx->c_tier; cb.tlist = ctx->c_tlist; cb.tstart = ctx->c_tstart; cb.part_tmp = ctx->part_tmp;
   cb.i64_scalar = ctx->c_i64_scalar;
   cb.hpin = ctx->hpin;
-  cb.hpin_bytes = HPIN_BYTES;
+  cb.hpin_bytes = HPIN_BYTES - 256;  // the last 256 bytes: the round's entry-count readback
+  cb.mtotal = ctx->c_mtotal;
   return cb;
 }
 
@@ -891,6 +893,7 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
 
   HIPOK(hipEventRecord(ctx->ev[4], s));
   k_iota<<<std::min((ctx->n0 + 255) / 256, 8192), 256, 0, s>>>(ctx->fin, ctx->n0);
+  int64_t m_in = ctx->m0;  // directed entries of the round graph
   double prev_mod = -1, cur_mod = -1;
   int round = 0;
   std::vector<int32_t> hostvec;
@@ -907,11 +910,21 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
     int32_t C = 0;
     int64_t m = 0;
     const int dst = dst_of(src);
-    rc = contract<T>(ctx, g, ctx->prev, dst, ro.caps, &C, &m, ctx->opts.dump_rounds != 0);
+    const bool canonical = ctx->opts.dump_rounds != 0;
+    rc = contract<T>(ctx, g, ctx->prev, dst, ro.caps, &C, &m, canonical);
     if (rc) return rc;
     compose_map(ctx->fin, ctx->n0, ctx->c_label, s);
     HIPOK(hipEventRecord(ctx->ev[7], s));
+    int64_t* m_rd = (int64_t*)(ctx->hpin + HPIN_BYTES - 64);  // entries of the contracted graph
+    if (!canonical) HIPOK(hipMemcpyAsync(m_rd, ctx->c_mtotal, 8, hipMemcpyDeviceToHost, s));
     HIPOK(hipStreamSynchronize(s));
+    const int64_t m_out = canonical ? m : *m_rd;
+    if (round < 64) {
+      R.round_vertices[round] = g.n;
+      R.round_entries[round] = m_in;
+      R.round_contracted_entries[round] = m_out;
+    }
+    m_in = m_out;
     float t_round = 0, t_build = 0;
     (void)hipEventElapsedTime(&t_round, ctx->ev[5], ctx->ev[6]);
     (void)hipEventElapsedTime(&t_build, ctx->ev[6], ctx->ev[7]);

@@ -275,6 +275,7 @@ struct ContractBufs {
   // lists that do not fit go through pageable copies
   char* hpin;
   int64_t hpin_bytes;
+  int64_t* mtotal;  // [1] entries of a gapped contracted graph
 };
 
 template <class T>

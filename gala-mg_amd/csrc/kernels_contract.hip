@@ -88,10 +88,18 @@ __global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ c
   }
 }
 
-// row ends of a gapped contracted graph: end[c] = off[c] + distinct count
-__global__ void k_row_ends(const int64_t* __restrict__ off, const int32_t* __restrict__ dcnt, int32_t C,
-                           int64_t* __restrict__ end) {
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) end[c] = off[c] + dcnt[c];
+// row ends of a gapped contracted graph: end[c] = off[c] + distinct count;
+// *total (zero on entry) receives the number of entries of the new graph
+__global__ __launch_bounds__(256) void k_row_ends(const int64_t* __restrict__ off, const int32_t* __restrict__ dcnt,
+                                                   int32_t C, int64_t* __restrict__ end,
+                                                   unsigned long long* __restrict__ total) {
+  unsigned long long t = 0;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+    end[c] = off[c] + dcnt[c];
+    t += (unsigned long long)dcnt[c];
+  }
+  t = wave_sum(t);
+  if ((threadIdx.x & 63) == 0 && t) atomicAdd(total, t);
 }
 
 // entries of every new community: ecnt[nid[c]] = row-length sum of c
@@ -1095,7 +1103,8 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     if (hipStreamSynchronize(s) != hipSuccess) return -1;  // the host tile vector is in flight until here
   }
   if (out.end) {  // gapped: every row stays at the start of its staging segment
-    k_row_ends<<<nblk(C), 256, 0, s>>>(cb.eoff, cb.dcnt, C, out.end);
+    fill<int64_t>(cb.mtotal, 1, 0, s);
+    k_row_ends<<<nblk(C), 256, 0, s>>>(cb.eoff, cb.dcnt, C, out.end, (unsigned long long*)cb.mtotal);
     return 0;
   }
   scan_exclusive<int32_t, int64_t>(cb.dcnt, out.off, C, cb.i64_tmp, out.off + C, s);

@@ -1039,6 +1039,155 @@ __global__ __launch_bounds__(256) void k_decide_hlead(DevGraph<T> g, DevState<T>
 }
 
 // ---------------------------------------------------------------------------
+// decide-and-move, leader-hash with one vertex at a time per group (32..511).
+// W lanes per vertex, EPL row entries per lane: the lanes load their entries
+// and the entries' community views together, insert (community -> summed
+// weight) into the group's LDS table, and the lane whose insert claimed a slot
+// evaluates that community once and clears the slot (as k_decide_hlead), but
+// without the four-stage software pipeline: a workgroup stays resident (the
+// grid is sized to the CUs) and its groups loop over the frontier, so many
+// groups' independent gathers are in flight per CU while each group's
+// instruction stream stays short.  FIRST: the first sweep of a round
+// (singleton partition, duplicate-free rows): every entry but the self-loop is
+// its own candidate and no table is used (TMAX = 1).
+// ---------------------------------------------------------------------------
+template <int TMAX, int W, int EPL, bool FIRST, class T>
+__global__ __launch_bounds__(256) void k_decide_glead(DevGraph<T> g, DevState<T> st,
+                                                       const int32_t* __restrict__ front,
+                                                       const int32_t* __restrict__ bstart, int bin,
+                                                       typename T::tot_t m2_all) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  using CV = typename T::cv_t;
+  constexpr int GPB = 256 / W;
+  constexpr int LEAD = 1 << 30;
+  __shared__ int s_keys[GPB][TMAX];
+  __shared__ acc s_vals[GPB][TMAX];
+  __shared__ unsigned long long s_moved, s_swept, s_edges;
+  for (int t = threadIdx.x; t < GPB * TMAX; t += blockDim.x) {
+    (&s_keys[0][0])[t] = 0;
+    (&s_vals[0][0])[t] = acc(0);
+  }
+  if (threadIdx.x == 0) { s_moved = 0; s_swept = 0; s_edges = 0; }
+  __syncthreads();
+  const int lo = bstart[bin], hi = bstart[bin + 1];
+  const int lane = threadIdx.x & (W - 1), grp = threadIdx.x / W;
+  int* keys = s_keys[grp];
+  acc* vals = s_vals[grp];
+  unsigned moved = 0, swept = 0, edges = 0;
+  for (int i = lo + blockIdx.x * GPB + grp; i < hi; i += gridDim.x * GPB) {  // uniform per group
+    const int v = front[i];
+    const tt m2 = vertex_m2<T>(st, v, m2_all);
+    const int64_t o = g.off[v];
+    const int d = (int)(g.end[v] - o);  // < W * EPL by bin construction
+    const CV ev = view_of<T, FIRST>(st, v);
+    const tt K = st.K[v];
+    const tt S = st.Self[v];
+    int u[EPL];
+    acc wv[EPL];
+#pragma unroll
+    for (int k = 0; k < EPL; k++) {
+      const int j = lane + k * W;
+      u[k] = j < d ? g.nbr[o + j] : -1;
+      wv[k] = j < d ? (acc)load_w<T>(g, o + j) : acc(0);
+    }
+    CV eu[EPL];
+#pragma unroll
+    for (int k = 0; k < EPL; k++) eu[k] = u[k] >= 0 ? view_of<T, FIRST>(st, u[k]) : CV{};
+    const int c = ev.c;
+    G h = 0;
+    int bd = INT_MAX, bsz = 0;
+    acc wc = 0;
+    if constexpr (FIRST) {
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        const int cl = eu[k].c;
+        if (u[k] >= 0 && cl != c) {
+          const G hh = gain_h<T>((G)wv[k], (G)m2, (G)K, (G)cv_tot(eu[k]), (G)cv_tot(ev));
+          if (hh > 0 && better(hh, cl, h, bd)) { h = hh; bd = cl; bsz = cv_sz(eu[k]); }
+        }
+      }
+    } else {
+      int lg = ilog2_ceil(2 * d);
+      if (lg < 1) lg = 1;
+      const int tmask = (1 << lg) - 1;  // next_pow2(2d) <= TMAX by bin construction
+      int sl[EPL];
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        sl[k] = -1;
+        if (u[k] >= 0) {
+          const int cl = eu[k].c;
+          unsigned p = hash_slot(cl, lg);
+          for (int probe = 0; probe <= tmask; probe++) {
+            const int kk = atomicCAS(&keys[p], 0, cl + 1);
+            if (kk == 0 || kk == cl + 1) {
+              atomicAdd(&vals[p], wv[k]);
+              sl[k] = (int)p | (kk == 0 ? LEAD : 0);
+              break;
+            }
+            p = (p + 1) & tmask;
+          }
+        }
+      }
+      lds_fence_wave();
+#pragma unroll
+      for (int k = 0; k < EPL; k++) {
+        if (sl[k] >= LEAD) {
+          const int p = sl[k] & (LEAD - 1);
+          const acc a = vals[p];
+          keys[p] = 0;
+          vals[p] = acc(0);
+          const int cl = eu[k].c;
+          if (cl == c) {
+            wc = a;
+          } else {
+            const G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)cv_tot(eu[k]), (G)cv_tot(ev));
+            if (hh > 0 && better(hh, cl, h, bd)) { h = hh; bd = cl; bsz = cv_sz(eu[k]); }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = W / 2; k >= 1; k >>= 1) {
+      const G ho = __shfl_xor(h, k, W);
+      const int dd = __shfl_xor(bd, k, W);
+      const int so = __shfl_xor(bsz, k, W);
+      if (better(ho, dd, h, bd)) { h = ho; bd = dd; bsz = so; }
+    }
+    if constexpr (FIRST) wc = (acc)S;
+    else wc = sum_xor<acc, W>(wc);
+    int mv = 0;
+    if (lane == 0) {
+      swept++;
+      edges += d;
+      mv = commit_move_sz<T>(v, c, h, bd, bsz, cv_sz(ev), wc, K, S, m2, st);
+      moved += mv;
+    }
+    if (st.dirty && __shfl(mv, 0, W)) {
+#pragma unroll
+      for (int k = 0; k < EPL; k++)
+        if (u[k] >= 0) st.dirty[u[k]] = 1;
+      if (lane == 0) st.dirty[v] = 1;
+    }
+    if constexpr (!FIRST) lds_fence_wave();  // every slot cleared before the group's next vertex inserts
+  }
+  if (lane == 0) {
+    if (moved) atomicAdd(&s_moved, (unsigned long long)moved);
+    if (swept) atomicAdd(&s_swept, (unsigned long long)swept);
+    if (edges) atomicAdd(&s_edges, (unsigned long long)edges);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomic_add_ull(&st.ctr->moved, s_moved);
+    atomic_add_ull(&st.ctr->swept, s_swept);
+    atomic_add_ull(&st.ctr->swept_edges, s_edges);
+    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
+    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // decide-and-move, W lanes per vertex with a per-group LDS hash (8..511)
 // Phase 1 inserts (community -> summed weight) with LDS CAS/add; phase 2 scans
 // the table so each distinct community is evaluated once (one Tot gather per
@@ -2110,9 +2259,11 @@ static int grid_for(int cap, int per_block, int max_blocks) {
 //   bin 4:    0 = one wave per vertex, 1 = 32 lanes, 2 = 16 lanes,
 //             3 = pipelined, 32 lanes x 4 entries, 4 = pipelined, 16 lanes x 8 entries,
 //             5 = pipelined leader hash, 32 lanes x 4 entries, 6 = the same, 16 x 8,
-//             7 = the same, 8 x 16
+//             7 = the same, 8 x 16, 8 = leader hash without the pipeline, 16 x 8
+//             (resident workgroups loop over the frontier)
 //   bin 5:    0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 64 lanes x 8 entries,
-//             3 = pipelined leader hash, 64 lanes x 8 entries, 4 = the same, 32 x 16
+//             3 = pipelined leader hash, 64 lanes x 8 entries, 4 = the same, 32 x 16,
+//             5 = leader hash without the pipeline, 64 x 8
 //   bins 6-7: 0 = one workgroup per vertex, table scan, 1 = one workgroup per
 //             vertex, leader evaluation (bin 7 FP: the hub kernel)
 //   bin 8:    0 = one workgroup per hub, 1 = hubs split over row and table
@@ -2127,13 +2278,14 @@ static int grid_for(int cap, int per_block, int max_blocks) {
 // vertices in flight per group), 2 = one thread per vertex (integer path; FP
 // falls back to 1), 3 (bin 3) = pipelined leader kernel, 16 lanes x 2 entries;
 // bin 4 has 0 = one wave per vertex, 1 = 32 lanes, 2 = pipelined, 16 lanes x
-// 8 entries; bin 5 has 0 = one wave per vertex, 1 = pipelined, 64 lanes x 8.
+// 8 entries, 3 = without the pipeline, 16 x 8; bin 5 has 0 = one wave per
+// vertex, 1 = pipelined, 64 lanes x 8, 2 = without the pipeline, 64 x 8.
 int decide_variants(int bin) {
-  static const int nv[NBINS] = {4, 4, 5, 5, 8, 5, 2, 2, 7};
+  static const int nv[NBINS] = {4, 4, 5, 5, 9, 6, 2, 2, 7};
   return bin >= 0 && bin < NBINS ? nv[bin] : 1;
 }
 int decide_first_variants(int bin) {
-  static const int nv[NBINS] = {3, 3, 3, 4, 3, 2, 1, 1, 1};
+  static const int nv[NBINS] = {3, 3, 3, 4, 4, 3, 1, 1, 1};
   return bin >= 0 && bin < NBINS ? nv[bin] : 1;
 }
 
@@ -2154,80 +2306,82 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
   constexpr int HUB_LT = T::WIDE ? 8192 : 16384;
   if (variant < 0 || variant >= decide_variants(bin)) variant = 0;
   const int b = bin;
-  switch (bin * 8 + variant) {
+  switch (bin * 16 + variant) {
     // bin 0: 1..3
     case 0: k_decide_ring<4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 1: k_decide_wave<8, 4, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 2: k_decide_pipe<4, false, T><<<grid_for(cap, 256, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 3: TPV_OR(3, false, 4, 256); break;
     // bin 1: 4..7
-    case 8: k_decide_ring<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 9: k_decide_wave<16, 8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 10: k_decide_pipe<8, false, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 11: TPV_OR(7, false, 8, 128); break;
+    case 16: k_decide_ring<8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 17: k_decide_wave<16, 8, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 18: k_decide_pipe<8, false, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 19: TPV_OR(7, false, 8, 128); break;
     // bin 2: 8..15
-    case 16: k_decide_ring<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 17: k_decide_wave<32, 16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 18: k_decide_pipe<16, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 19: TPV_OR(15, false, 16, 64); break;
-    case 20: k_decide_hlead<32, 8, 2, false, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 32: k_decide_ring<16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 33: k_decide_wave<32, 16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 34: k_decide_pipe<16, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 35: TPV_OR(15, false, 16, 64); break;
+    case 36: k_decide_hlead<32, 8, 2, false, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     // bin 3: 16..31
-    case 24: k_decide_ring<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 25: k_decide_wave<64, 32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 26: k_decide_pipe<32, false, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 27: TPV_OR(31, false, 32, 32); break;
-    case 28: k_decide_hlead<64, 16, 2, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 48: k_decide_ring<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 49: k_decide_wave<64, 32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 50: k_decide_pipe<32, false, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 51: TPV_OR(31, false, 32, 32); break;
+    case 52: k_decide_hlead<64, 16, 2, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     // bin 4: 32..127
-    case 32: k_decide_wave<256, 64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 33: k_decide_wave<256, 32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 34: k_decide_wave<256, 16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 35: k_decide_hpipe<256, 32, 4, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 36: k_decide_hpipe<256, 16, 8, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 37: k_decide_hlead<256, 32, 4, false, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 38: k_decide_hlead<256, 16, 8, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 39: k_decide_hlead<256, 8, 16, false, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 64: k_decide_wave<256, 64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 65: k_decide_wave<256, 32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 66: k_decide_wave<256, 16, T><<<grid_for(cap, 16, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 67: k_decide_hpipe<256, 32, 4, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 68: k_decide_hpipe<256, 16, 8, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 69: k_decide_hlead<256, 32, 4, false, T><<<grid_for(cap, 32, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 70: k_decide_hlead<256, 16, 8, false, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 71: k_decide_hlead<256, 8, 16, false, T><<<grid_for(cap, 128, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 72: k_decide_glead<256, 16, 8, false, T><<<grid_for(cap, 16, 1024), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     // bin 5: 128..511
-    case 40: k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 41:
+    case 80: k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 81:
       if constexpr (T::WIDE)  // 8 groups x 1024 slots of 8-byte sums and totals exceed the LDS
         k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       else
         k_decide_wave<1024, 32, T><<<grid_for(cap, 8, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       break;
-    case 42:
+    case 82:
       if constexpr (T::WIDE)
         k_decide_wave<1024, 64, T><<<grid_for(cap, 4, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       else
         k_decide_hpipe<1024, 64, 8, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       break;
-    case 43: k_decide_hlead<1024, 64, 8, false, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 44:
+    case 83: k_decide_hlead<1024, 64, 8, false, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 84:
       if constexpr (T::WIDE)  // 8 groups x 1024 8-byte slots exceed the LDS with the keys
         k_decide_hlead<1024, 64, 8, false, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       else
         k_decide_hlead<1024, 32, 16, false, T><<<grid_for(cap, 32, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2);
       break;
+    case 85: k_decide_glead<1024, 64, 8, false, T><<<grid_for(cap, 4, 1024), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     // bin 6: 512..4095
-    case 48: k_decide_block<256, 8192, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
-    case 49: k_decide_blead<256, 8192, 16, 1, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2, 0); break;
+    case 96: k_decide_block<256, 8192, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 97: k_decide_blead<256, 8192, 16, 1, T><<<grid_for(cap, 1, 2048), 256, 0, s>>>(g, st, front, bstart, b, m2, 0); break;
     // bin 7: 4096..8191
-    case 56:
+    case 112:
       if constexpr (T::WIDE)
         k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0);
       else
         k_decide_block<1024, 16384, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2);
       break;
-    case 57:
+    case 113:
       if constexpr (T::WIDE)  // 16384 8-byte slots exceed the LDS: the hub kernel
         k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0);
       else
         k_decide_blead<1024, 16384, 8, 1, T><<<grid_for(cap, 1, 512), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0);
       break;
     // bin 8: >= 8192
-    case 64: k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0); break;
-    case 65:
-    case 66:
-    case 67: {
+    case 128: k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, 0); break;
+    case 129:
+    case 130:
+    case 131: {
       // 1: every hub split; 2: only rows longer than the LDS table, the rest
       // one workgroup per hub; 3: the same with the two-pass leader kernel
       // for rows shorter than the LDS table (integer path; FP: as 2)
@@ -2248,9 +2402,9 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
                                                               (const HubPart<T>*)st.hub_part, m2, nullptr);
       break;
     }
-    case 68:
-    case 69:
-    case 70: {
+    case 132:
+    case 133:
+    case 134: {
       // 4: rows longer than the LDS table split by community over workgroups
       // (k_hub_part), the others one workgroup per hub; hubs whose partition
       // overflowed are redone by the global-table split.  5: the same with
@@ -2303,8 +2457,10 @@ void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const i
     case 32: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 33: k_decide_first<32, T><<<grid_for(cap, 8, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 34: k_decide_hlead<1, 16, 8, true, T><<<grid_for(cap, 64, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 35: k_decide_glead<1, 16, 8, true, T><<<grid_for(cap, 16, 1024), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 40: k_decide_first<64, T><<<grid_for(cap, 4, 8192), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 41: k_decide_hlead<1, 64, 8, true, T><<<grid_for(cap, 16, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
+    case 42: k_decide_glead<1, 64, 8, true, T><<<grid_for(cap, 4, 1024), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 48: k_decide_first_block<256, T><<<grid_for(cap, 1, 4096), 256, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 56: k_decide_first_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2); break;
     case 64: k_decide_first_block<1024, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2); break;

@@ -77,7 +77,8 @@ class Result(ctypes.Structure):
                 ("bin_first_bytes", ctypes.c_int64 * 9), ("bin_first_edges", ctypes.c_int64 * 9),
                 ("bin_first_ms", ctypes.c_double * 9), ("bin_launches", ctypes.c_int32 * 9),
                 ("bin_first_launches", ctypes.c_int32 * 9), ("wide", ctypes.c_int32),
-                ("weight_shift", ctypes.c_int32)]
+                ("weight_shift", ctypes.c_int32), ("round_vertices", ctypes.c_int32 * 64),
+                ("round_entries", ctypes.c_int64 * 64), ("round_contracted_entries", ctypes.c_int64 * 64)]
 
     def to_dict(self):
         r = self.rounds
@@ -102,7 +103,9 @@ class Result(ctypes.Structure):
                     bin_first_bytes=list(self.bin_first_bytes), bin_first_edges=list(self.bin_first_edges),
                     bin_first_ms=list(self.bin_first_ms), bin_launches=list(self.bin_launches),
                     bin_first_launches=list(self.bin_first_launches), wide=self.wide,
-                    weight_shift=self.weight_shift)
+                    weight_shift=self.weight_shift, round_vertices=list(self.round_vertices[:r]),
+                    round_entries=list(self.round_entries[:r]),
+                    round_contracted_entries=list(self.round_contracted_entries[:r]))
 
 
 ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,

@@ -114,6 +114,11 @@ typedef struct {
    * Both are exact: Q and every decision are those of exact arithmetic. */
   int32_t wide;
   int32_t weight_shift;
+  /* Per round: vertices and directed entries of the round graph, entries of
+   * the graph its contraction produced (the contraction's byte count). */
+  int32_t round_vertices[64];
+  int64_t round_entries[64];
+  int64_t round_contracted_entries[64];
 } gala_result;
 
 /* Version string of the library. */
