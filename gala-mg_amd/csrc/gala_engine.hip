@@ -78,8 +78,8 @@ struct gala_ctx {
   int src = 0;  // buffer holding the current round graph
 
   // per-vertex state (sized n0)
-  void *K = nullptr, *Self = nullptr, *Tot = nullptr, *In = nullptr, *dTot = nullptr;
-  int32_t *size = nullptr, *dsize = nullptr, *prev = nullptr, *cur = nullptr;
+  void *K = nullptr, *Self = nullptr, *Tot = nullptr, *In = nullptr;
+  int32_t *size = nullptr, *prev = nullptr, *cur = nullptr;
   uint8_t* active = nullptr;
   uint8_t* dirty = nullptr;
   int8_t* binid = nullptr;
@@ -108,7 +108,7 @@ struct gala_ctx {
   int32_t *c_cnt = nullptr, *c_nid = nullptr, *c_label = nullptr, *c_skey = nullptr, *c_dcnt = nullptr,
           *c_big_list = nullptr, *c_nbig = nullptr, *c_i32_tmp = nullptr, *c_i32_scalar = nullptr;
   int64_t *c_ecnt = nullptr, *c_eoff = nullptr, *c_ecur = nullptr, *c_big_tab = nullptr,
-          *c_i64_tmp = nullptr, *c_i64_scalar = nullptr, *c_mtotal = nullptr;
+          *c_i64_tmp = nullptr, *c_i64_scalar = nullptr, *c_mtotal = nullptr, *c_voff = nullptr;
   void* c_sval = nullptr;
   int8_t* c_tier = nullptr;
   int32_t *c_tlist = nullptr, *c_tstart = nullptr;
@@ -250,8 +250,8 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool wide) {
   }
   { char* p;
     OWN(p, N * tsz); ctx->K = p; OWN(p, N * tsz); ctx->Self = p; OWN(p, N * tsz); ctx->Tot = p;
-    OWN(p, N * tsz); ctx->In = p; OWN(p, N * tsz); ctx->dTot = p; }
-  OWN(ctx->size, N); OWN(ctx->dsize, N); OWN(ctx->prev, N); OWN(ctx->cur, N);
+    OWN(p, N * tsz); ctx->In = p; }
+  OWN(ctx->size, N); OWN(ctx->prev, N); OWN(ctx->cur, N);
   OWN(ctx->active, N); OWN(ctx->dirty, N); OWN(ctx->binid, N);
   OWN(ctx->order, N); OWN(ctx->front, N); OWN(ctx->fin, N);
   { char* p; OWN(p, N * 16); ctx->cv = p; }
@@ -281,6 +281,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool wide) {
   OWN(ctx->c_big_tab, N + 1); OWN(ctx->c_i64_tmp, 2 * (N + 1) + scan_tmp_elems(N));
   OWN(ctx->c_i64_scalar, 16);
   OWN(ctx->c_mtotal, 1);
+  OWN(ctx->c_voff, N);
   if (!ctx->ctr_h) {
     hipError_t e = hipHostMalloc((void**)&ctx->ctr_h, 2 * sizeof(Counters), hipHostMallocDefault);
     if (e != hipSuccess) { ctx->last_hip = e; free_all(ctx); return GALA_E_OOM; }
@@ -491,8 +492,7 @@ DevState<T> dstate(gala_ctx* ctx) {
   using tt = typename T::tot_t;
   DevState<T> st;
   st.K = (tt*)ctx->K; st.Self = (tt*)ctx->Self; st.Tot = (tt*)ctx->Tot; st.In = (tt*)ctx->In;
-  st.dTot = (tt*)ctx->dTot;
-  st.size = ctx->size; st.dsize = ctx->dsize; st.prev = ctx->prev; st.cur = ctx->cur;
+  st.size = ctx->size; st.prev = ctx->prev; st.cur = ctx->cur;
   st.active = ctx->active; st.dirty = ctx->dirty; st.binid = ctx->binid; st.hub_off = ctx->hub_off;
   st.hub_keys = ctx->hub_keys; st.hub_vals = (typename T::acc_t*)ctx->hub_vals;
   st.hub_rc = ctx->hub_rc; st.hub_rw = (typename T::acc_t*)ctx->hub_rw;
@@ -760,6 +760,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       // them touches keep the In of the learned sweep, which is exact.
       ctx->res.certify_sweeps++;
       reset_counters(ctx->ctr, (long long)m2, s);
+      settle_pairs(ctx->cp, g.n, s);  // the learned sweep's moves are in Tot and size already
       partition_bins(2, fnext, bnext, ctx->binid, ctx->keep, nf0, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
       DevState<T> sc = st;
       sc.Tot = (tt*)ctx->tot_snap;
@@ -814,6 +815,7 @@ ContractBufs<T> cbufs(gala_ctx* ctx) {
   cb.hpin = ctx->hpin;
   cb.hpin_bytes = HPIN_BYTES - 256;  // the last 256 bytes: the round's entry-count readback
   cb.mtotal = ctx->c_mtotal;
+  cb.voff = ctx->c_voff;
   return cb;
 }
 

@@ -25,6 +25,9 @@ void fill(V* p, int64_t n, V x, hipStream_t s);
 void copy_i32(const int32_t* a, int32_t* b, int64_t n, hipStream_t s);
 // prev = cur after an accepted sweep; cp (if not null) = {cur, cur}
 void accept_sweep(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, hipStream_t s);
+// cp = {cur, cur} without touching prev: the moves of a sweep already applied
+// to Tot and size are not applied again by the next update
+void settle_pairs(int2* cp, int64_t n, hipStream_t s);
 void reset_counters(Counters* c, long long min_init, hipStream_t s);
 
 // ---- Louvain phase kernels (kernels_louvain.hip) ----
@@ -76,8 +79,8 @@ inline int64_t hub_part_elems(int64_t m) { return 4 * m / HUB_TCH + m / 8192 + 2
 
 template <class T>
 struct DevState {
-  typename T::tot_t *K, *Self, *Tot, *In, *dTot;
-  int32_t *size, *dsize, *prev, *cur;
+  typename T::tot_t *K, *Self, *Tot, *In;
+  int32_t *size, *prev, *cur;
   uint8_t* active;
   uint8_t* dirty;    // In must be recomputed (the vertex or a neighbour moved)
   int8_t* binid;
@@ -158,7 +161,8 @@ template <class T>
 void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t* front,
                          const int32_t* bstart, int cap, typename T::tot_t m2, hipStream_t s, int variant = 0);
 
-// Tot += dTot, size += dsize, sums of Tot^2, min Tot, non-empty count.
+// Tot and size take the sweep's moves (from the {cur, prev} pairs, tile sums
+// per community), then sums of Tot^2, min Tot, non-empty count.
 template <class T>
 void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);
 
@@ -276,6 +280,7 @@ struct ContractBufs {
   char* hpin;
   int64_t hpin_bytes;
   int64_t* mtotal;  // [1] entries of a gapped contracted graph
+  int64_t* voff;    // [n] offset of every vertex's row inside its community's staging segment
 };
 
 template <class T>

@@ -359,6 +359,18 @@ void accept_sweep(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, hipStr
   k_accept<<<nb, 256, 0, s>>>(cur, prev, cp, n);
 }
 
+__global__ void k_settle_pairs(int2* __restrict__ cp, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int2 p = cp[i];
+    if (p.x != p.y) cp[i] = make_int2(p.x, p.x);
+  }
+}
+void settle_pairs(int2* cp, int64_t n, hipStream_t s) {
+  if (n <= 0 || !cp) return;
+  int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  k_settle_pairs<<<nb, 256, 0, s>>>(cp, n);
+}
+
 void copy_i32(const int32_t* a, int32_t* b, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);

@@ -42,24 +42,35 @@ __host__ __device__ __forceinline__ int clog2(long long x) {
   return l;
 }
 
-// member count and row-length sum of every community.  A workgroup takes a
+// member count and row-length sum of every community, and the offset of every
+// vertex's row inside its community's staging segment.  A workgroup takes a
 // tile of CT vertices at a time and pre-aggregates them in an LDS hash keyed by
-// community, so a large community costs one global atomic per tile instead of
-// one per member (per-member atomics on a few hot addresses serialise).
-constexpr int CT = 1024, CTS = 2 * CT;
+// community: each vertex's row length is added to its community's tile sum
+// (the returned partial sum is the row's offset inside the tile's share), and
+// each community's tile sum is added to the global sum with one atomic whose
+// returned value is the tile's share's offset inside the segment.  So a large
+// community costs one global atomic per tile instead of one per member
+// (per-member atomics on a few hot addresses serialise), and staging needs no
+// atomics at all.
+constexpr int CT = 1024, CTS = 2 * CT, CT_PT = CT / 256;
 __global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ comm, const int64_t* __restrict__ off,
                                                    const int64_t* __restrict__ end, int32_t n, int32_t* __restrict__ cnt,
-                                                   unsigned long long* __restrict__ dsum) {
+                                                   unsigned long long* __restrict__ dsum, int64_t* __restrict__ voff) {
   __shared__ int s_key[CTS];
   __shared__ int s_cnt[CTS];
   __shared__ unsigned long long s_sum[CTS];
+  __shared__ unsigned long long s_base[CTS];
   for (int t = threadIdx.x; t < CTS; t += blockDim.x) { s_key[t] = 0; s_cnt[t] = 0; s_sum[t] = 0; }
   __syncthreads();
   constexpr int LG = 11;  // CTS = 2^LG
   for (int64_t base = (int64_t)blockIdx.x * CT; base < n; base += (int64_t)gridDim.x * CT) {
-    for (int j = threadIdx.x; j < CT; j += blockDim.x) {
-      const int64_t v = base + j;
-      if (v >= n) break;
+    int slot[CT_PT];
+    unsigned long long loc[CT_PT];
+#pragma unroll
+    for (int r = 0; r < CT_PT; r++) {
+      slot[r] = -1;
+      const int64_t v = base + threadIdx.x + r * 256;
+      if (v >= n) continue;
       const int c = comm[v];
       const unsigned long long d = (unsigned long long)(end[v] - off[v]);
       unsigned p = ((unsigned)c * 2654435769u) >> (32 - LG);
@@ -67,7 +78,8 @@ __global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ c
         const int k = atomicCAS(&s_key[p], 0, c + 1);
         if (k == 0 || k == c + 1) {
           atomicAdd(&s_cnt[p], 1);
-          if (d) atomicAdd(&s_sum[p], d);
+          loc[r] = atomicAdd(&s_sum[p], d);
+          slot[r] = (int)p;
           break;
         }
         p = (p + 1) & (CTS - 1);
@@ -78,7 +90,16 @@ __global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ c
       const int k = s_key[t];
       if (k) {
         atomicAdd(&cnt[k - 1], s_cnt[t]);
-        if (s_sum[t]) atomicAdd(&dsum[k - 1], s_sum[t]);
+        s_base[t] = atomicAdd(&dsum[k - 1], s_sum[t]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < CT_PT; r++)
+      if (slot[r] >= 0) voff[base + threadIdx.x + r * 256] = (int64_t)(s_base[slot[r]] + loc[r]);
+    __syncthreads();
+    for (int t = threadIdx.x; t < CTS; t += blockDim.x) {
+      if (s_key[t]) {
         s_key[t] = 0;
         s_cnt[t] = 0;
         s_sum[t] = 0;
@@ -194,13 +215,13 @@ __device__ __forceinline__ void stage_span(const DevGraph<T>& g, int64_t o, int6
 }
 
 // rows of the vertices order[bstart[blo] .. bstart[bhi]) (the round's degree
-// bins): NT lanes per vertex, one reservation each
+// bins): NT lanes per vertex, each row at its offset from k_comm_sums
 template <int NT, class T>
 __global__ __launch_bounds__(256) void k_stage_rows(DevGraph<T> g, const int32_t* __restrict__ order,
                                                     const int32_t* __restrict__ bstart, int blo, int bhi,
                                                     const int32_t* __restrict__ label,
                                                     const int64_t* __restrict__ eoff,
-                                                    unsigned long long* __restrict__ ecur,
+                                                    const int64_t* __restrict__ voff,
                                                     int32_t* __restrict__ skey,
                                                     typename T::acc_t* __restrict__ sval) {
   constexpr int G = 256 / NT;  // vertices per block step
@@ -210,48 +231,32 @@ __global__ __launch_bounds__(256) void k_stage_rows(DevGraph<T> g, const int32_t
     const int v = order[i];
     const int64_t o = g.off[v];
     const int64_t d = g.end[v] - o;
-    const int nc = label[v];
-    int64_t base = 0;
-    if (lane == 0) base = eoff[nc] + (int64_t)atomicAdd(&ecur[nc], (unsigned long long)d);
-    base = __shfl(base, 0, NT > 64 ? 64 : NT);
-    if (NT > 64) {  // broadcast across the waves of the group through LDS
-      __shared__ int64_t s_b;
-      if (threadIdx.x == 0) s_b = base;
-      __syncthreads();
-      base = s_b;
-      __syncthreads();
-    }
+    const int64_t base = eoff[label[v]] + voff[v];  // the row's place in its community's segment
     stage_span<NT>(g, o, d, lane, label, skey, sval, base);
   }
 }
 
 // rows of order[bstart[blo] .. bstart[bhi]) cut into STAGE_CH-entry chunks:
-// workgroup (x, y) stages chunks y, y + gridDim.y, ... of row x, one
-// reservation per chunk, so a hub row of 10^5 entries is spread over
-// gridDim.y workgroups instead of one
+// workgroup (x, y) stages chunks y, y + gridDim.y, ... of row x, so a hub row
+// of 10^5 entries is spread over gridDim.y workgroups instead of one
 constexpr int STAGE_CH = 4096;
 template <class T>
 __global__ __launch_bounds__(256) void k_stage_chunks(DevGraph<T> g, const int32_t* __restrict__ order,
                                                       const int32_t* __restrict__ bstart, int blo, int bhi,
                                                       const int32_t* __restrict__ label,
                                                       const int64_t* __restrict__ eoff,
-                                                      unsigned long long* __restrict__ ecur,
+                                                      const int64_t* __restrict__ voff,
                                                       int32_t* __restrict__ skey,
                                                       typename T::acc_t* __restrict__ sval) {
-  __shared__ int64_t s_b;
   const int i = bstart[blo] + blockIdx.x;
   if (i >= bstart[bhi]) return;  // uniform per workgroup
   const int v = order[i];
   const int64_t o = g.off[v];
   const int64_t d = g.end[v] - o;
-  const int nc = label[v];
+  const int64_t row = eoff[label[v]] + voff[v];
   for (int64_t c0 = (int64_t)blockIdx.y * STAGE_CH; c0 < d; c0 += (int64_t)gridDim.y * STAGE_CH) {
     const int64_t len = d - c0 < STAGE_CH ? d - c0 : STAGE_CH;
-    if (threadIdx.x == 0) s_b = eoff[nc] + (int64_t)atomicAdd(&ecur[nc], (unsigned long long)len);
-    __syncthreads();
-    const int64_t base = s_b;
-    stage_span<256>(g, o + c0, len, (int)threadIdx.x, label, skey, sval, base);
-    __syncthreads();  // s_b is rewritten by the next chunk
+    stage_span<256>(g, o + c0, len, (int)threadIdx.x, label, skey, sval, row + c0);
   }
 }
 
@@ -903,7 +908,8 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
   // the staging cursors are free until contract_build: row-length sums by community
   unsigned long long* dsum = (unsigned long long*)cb.ecur;
   fill<int64_t>(cb.ecur, n, 0, s);
-  k_comm_sums<<<std::max(1, std::min((n + CT - 1) / CT, 2048)), 256, 0, s>>>(comm, g.off, g.end, n, cb.cnt, dsum);
+  k_comm_sums<<<std::max(1, std::min((n + CT - 1) / CT, 2048)), 256, 0, s>>>(comm, g.off, g.end, n, cb.cnt, dsum,
+                                                                             cb.voff);
   uint8_t* flag = (uint8_t*)cb.i32_tmp;  // n bytes fit in the n-int scratch
   k_nonempty<<<nblk(n), 256, 0, s>>>(cb.cnt, n, flag);
   scan_exclusive<uint8_t, int32_t>(flag, cb.nid, n, cb.i32_tmp + (n + 3) / 4 + 1, cb.i32_scalar, s);
@@ -984,23 +990,23 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
   constexpr int LT = T::WIDE ? 8192 : 16384;
   const int32_t n = g.n;
   const int32_t C = plan.C;
-  fill<int64_t>(cb.ecur, C, 0, s);
-  unsigned long long* ecur = (unsigned long long*)cb.ecur;
-  // W lanes per row, W matched to the round's degree bins, one reservation per row
+  // W lanes per row, W matched to the round's degree bins; every row goes to
+  // its offset from k_comm_sums
   const int32_t* cp = plan.caps;
-  k_stage_rows<4, T><<<nblk(cp[0], 64, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 0, 1, cb.label, cb.eoff, ecur,
+  const int64_t* voff = cb.voff;
+  k_stage_rows<4, T><<<nblk(cp[0], 64, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 0, 1, cb.label, cb.eoff, voff,
                                                            cb.skey, cb.sval);
-  k_stage_rows<8, T><<<nblk(cp[1], 32, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 1, 2, cb.label, cb.eoff, ecur,
+  k_stage_rows<8, T><<<nblk(cp[1], 32, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 1, 2, cb.label, cb.eoff, voff,
                                                            cb.skey, cb.sval);
-  k_stage_rows<16, T><<<nblk(cp[2], 16, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 2, 3, cb.label, cb.eoff, ecur,
+  k_stage_rows<16, T><<<nblk(cp[2], 16, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 2, 3, cb.label, cb.eoff, voff,
                                                             cb.skey, cb.sval);
-  k_stage_rows<32, T><<<nblk(cp[3], 8, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 3, 4, cb.label, cb.eoff, ecur,
+  k_stage_rows<32, T><<<nblk(cp[3], 8, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 3, 4, cb.label, cb.eoff, voff,
                                                            cb.skey, cb.sval);
   k_stage_rows<64, T><<<nblk(cp[4] + cp[5] + cp[6], 4, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 4, 7, cb.label,
-                                                                           cb.eoff, ecur, cb.skey, cb.sval);
+                                                                           cb.eoff, voff, cb.skey, cb.sval);
   if (cp[7] + cp[8] > 0)
     k_stage_chunks<T><<<dim3(cp[7] + cp[8], 16), 256, 0, s>>>(g, cb.order, cb.bstart, 7, NBINS, cb.label, cb.eoff,
-                                                              ecur, cb.skey, cb.sval);
+                                                              voff, cb.skey, cb.sval);
   fill<int32_t>(cb.dcnt, C + 1, 0, s);
   const int canon = plan.canonical ? 1 : 0;
   const int32_t* ts = plan.tstart;

@@ -100,16 +100,7 @@ __device__ __forceinline__ bool commit_move(int v, int c, typename T::gain_t bes
     st.last_gain[v] = (float)((double)gain / ((double)m2 * (double)K));
     if (nc != c) st.moved_it[v] = st.iter;
   }
-  typename T::tot_t* dTot = st.dTot;
-  int32_t* dsize = st.dsize;
-  if (nc != c) {
-    atomicAdd(&dTot[nc], K);
-    atomicAdd(&dTot[c], -K);
-    atomicAdd(&dsize[nc], 1);
-    atomicAdd(&dsize[c], -1);
-    return true;
-  }
-  return false;
+  return nc != c;  // Tot and size take the movers' deltas after the sweep (launch_apply)
 }
 
 // commit_move with the sizes of the best and the own community already at hand
@@ -131,14 +122,7 @@ __device__ __forceinline__ bool commit_move_sz(int v, int c, typename T::gain_t 
     st.last_gain[v] = (float)((double)gain / ((double)m2 * (double)K));
     if (nc != c) st.moved_it[v] = st.iter;
   }
-  if (nc != c) {
-    atomicAdd(&st.dTot[nc], K);
-    atomicAdd(&st.dTot[c], -K);
-    atomicAdd(&st.dsize[nc], 1);
-    atomicAdd(&st.dsize[c], -1);
-    return true;
-  }
-  return false;
+  return nc != c;
 }
 
 // A mover invalidates In of itself and of every neighbour (plain byte stores:
@@ -191,8 +175,6 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
     st.cur[v] = v;
     if (st.cp) st.cp[v] = make_int2(v, v);
     st.active[v] = 1;
-    st.dTot[v] = 0;
-    st.dsize[v] = 0;
     st.dirty[v] = 0;
     if (st.track) {
       st.last_gain[v] = 0.f;
@@ -2468,8 +2450,73 @@ void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const i
 }
 
 // ---------------------------------------------------------------------------
-// apply: Tot += dTot, size += dsize (louvain_main_process.cu:949-953),
-// min non-zero Tot (Tot_op, kernel_functions.cuh:112-121), sum Tot^2, non-empty count.
+// Tot and size of every community after a sweep (louvain_main_process.cu:
+// 949-953 adds the movers' deltas, which the reference's decide kernels gather
+// with per-mover atomics, kernel_functions.cu:397-400).  Here one pass reads
+// the {cur, prev} pair of every vertex; a workgroup takes tiles of TD_T
+// vertices, sums the movers' +K/+1 (new community) and -K/-1 (old community)
+// per community in an LDS hash, and adds each community's tile sum with one
+// global atomic.  When many movers join one community (late rounds of
+// power-law graphs: tens of thousands into one community in one sweep),
+// per-mover atomics on that address serialise; the tile sums keep it to one
+// atomic per tile.  Exact integer sums.
+// ---------------------------------------------------------------------------
+constexpr int TD_T = 512, TD_S = 4 * TD_T, TD_LG = 11;  // <= 2 keys per vertex: half load at most
+template <class T>
+__global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n) {
+  using tt = typename T::tot_t;
+  __shared__ int s_key[TD_S];
+  __shared__ int s_cnt[TD_S];
+  __shared__ tt s_tot[TD_S];
+  __shared__ int s_list[TD_S];
+  __shared__ int s_n;
+  for (int t = threadIdx.x; t < TD_S; t += blockDim.x) { s_key[t] = 0; s_cnt[t] = 0; s_tot[t] = tt(0); }
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  auto add = [&](int c, tt k, int one) {
+    unsigned p = hash_slot(c, TD_LG);
+    for (int probe = 0; probe < TD_S; probe++) {  // at most 2 * TD_T keys: a free slot exists
+      const int kk = atomicCAS(&s_key[p], 0, c + 1);
+      if (kk == 0 || kk == c + 1) {
+        if (kk == 0) s_list[atomicAdd(&s_n, 1)] = (int)p;
+        atomicAdd(&s_cnt[p], one);
+        if (k) atomicAdd(&s_tot[p], one > 0 ? k : -k);
+        return;
+      }
+      p = (p + 1) & (TD_S - 1);
+    }
+  };
+  for (int64_t base = (int64_t)blockIdx.x * TD_T; base < n; base += (int64_t)gridDim.x * TD_T) {
+    for (int j = threadIdx.x; j < TD_T; j += blockDim.x) {
+      const int64_t v = base + j;
+      if (v >= n) break;
+      const int2 cp = st.cp[v];  // {cur, prev}
+      if (cp.x != cp.y) {
+        const tt k = st.K[v];
+        add(cp.x, k, 1);
+        add(cp.y, k, -1);
+      }
+    }
+    __syncthreads();
+    const int ns = s_n;
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+      const int p = s_list[i];
+      const int c = s_key[p] - 1;
+      if (s_cnt[p]) atomicAdd(&st.size[c], s_cnt[p]);
+      if (s_tot[p]) atomicAdd(&st.Tot[c], s_tot[p]);
+      s_key[p] = 0;
+      s_cnt[p] = 0;
+      s_tot[p] = tt(0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// min non-zero Tot (Tot_op, kernel_functions.cuh:112-121), sum Tot^2 and the
+// non-empty count over the rebuilt Tot and size.
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void k_apply(DevState<T> st, int32_t n, typename T::tot_t m2) {
@@ -2481,12 +2528,8 @@ __global__ __launch_bounds__(256) void k_apply(DevState<T> st, int32_t n, typena
   unsigned long long mn = (unsigned long long)(long long)m2;
   int ne = 0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-    tt t = st.Tot[c] + st.dTot[c];
-    int sz = st.size[c] + st.dsize[c];
-    st.Tot[c] = t;
-    st.size[c] = sz;
-    st.dTot[c] = 0;
-    st.dsize[c] = 0;
+    const tt t = st.Tot[c];
+    const int sz = st.size[c];
     ne += sz > 0;
     const long long ti = (long long)t;
     t2 += (__int128)ti * ti;
@@ -2559,6 +2602,7 @@ template <class T>
 void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s) {
   if (n <= 0) return;
   const int nb = std::min((n + 255) / 256, MAX_PARTS);
+  k_tot_delta<T><<<std::min((n + TD_T - 1) / TD_T, 2048), 256, 0, s>>>(st, n);
   k_apply<T><<<nb, 256, 0, s>>>(st, n, m2);
   k_apply_fin<T><<<1, 256, 0, s>>>(st, nb);
 }
