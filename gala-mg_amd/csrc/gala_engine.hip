@@ -150,6 +150,11 @@ struct gala_ctx {
   int32_t* b_scalars = nullptr;        // [0] nlive, [1] nopen, [2] iteration counter
   unsigned long long *b_gin = nullptr, *b_gtot2 = nullptr;
   int32_t* b_host = nullptr;           // pinned readback of the scalars
+  // vertices (of positive degree) of the graphs still iterating, grouped by
+  // degree bin: the per-sweep O(n) passes visit only these; [1] is scratch
+  int32_t* b_vl[2] = {nullptr, nullptr};
+  int32_t* b_lbs[2] = {nullptr, nullptr};  // their bin starts (NBINS + 1)
+  uint8_t* b_vlive = nullptr;
   int32_t b_m2max = 0;
   std::vector<void*> bowned;
   bool have_batch = false, have_batch_result = false;
@@ -1157,6 +1162,8 @@ int batch_upload(gala_ctx* ctx, int32_t G, const int32_t* vstart, int32_t n, con
   BOWN(ctx->b_qprev, G); BOWN(ctx->b_cur_mod, G); BOWN(ctx->b_iters, G); BOWN(ctx->b_tot_iters, G);
   BOWN(ctx->b_rounds, G); BOWN(ctx->b_ncomm, G); BOWN(ctx->b_scalars, 4); BOWN(ctx->b_gin, G);
   BOWN(ctx->b_gtot2, G);
+  for (int k = 0; k < 2; k++) { BOWN(ctx->b_vl[k], N); BOWN(ctx->b_lbs[k], NBINS + 1); }
+  BOWN(ctx->b_vlive, N);
   if (!ctx->b_host) {
     hipError_t e = hipHostMalloc((void**)&ctx->b_host, 4 * sizeof(int32_t), hipHostMallocDefault);
     if (e != hipSuccess) { ctx->last_hip = e; free_batch(ctx); return GALA_E_OOM; }
@@ -1243,37 +1250,47 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     launch_prune<T>(st, g.n, m2max, s);
     launch_graph_sums(gid, ctx->binid, (const int32_t*)ctx->In, (const int32_t*)ctx->Tot, g.n, b.g_in, b.g_tot2, s);
     launch_batch_step(b, it_dev, min_mod, max_it, s);
-    launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, true, it_dev, nullptr, s);
-    partition_bins(1, ctx->order, ctx->bstart_full, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front,
-                 ctx->part_tmp, s);
+    // the round's first frontier, and the list of the live graphs' vertices
+    int32_t *LA = ctx->b_vl[0], *LB = ctx->b_vl[1], *BA = ctx->b_lbs[0], *BB = ctx->b_lbs[1];
+    const int32_t* nA = BA + NBINS;  // length of the live list (device)
+    launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, true, it_dev, nullptr, s, ctx->order,
+                      ctx->bstart_full + NBINS, ctx->b_vlive);
+    partition_bins(1, ctx->order, ctx->bstart_full, ctx->binid, ctx->b_vlive, nitems, LA, BA, ctx->part_tmp, s);
+    partition_bins(1, LA, BA, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp, s, nA);
     // one iteration for every running graph; a graph that stopped is inert
     // integer path: In from the movers' rows ({cur, prev} pairs, reset by the
     // mask after each sweep), so decide-and-move writes no dirty marks
     constexpr bool incr = true;
     DevState<T> sdec = st;
     if (incr) sdec.dirty = nullptr;
+    // Every O(n) pass of a sweep visits only the live list (a graph that
+    // stopped is inert), which shrinks as graphs converge; the list is
+    // re-compacted after every sweep (fixed buffers, so a captured body
+    // replays correctly).
     auto body = [&](bool first) {
       launch_batch_reset(ctx->ctr, m2max, nlive, s);
-      launch_build_cv<T>(st, g.n, s);
+      launch_build_cv<T>(st, g.n, s, LA, nA);
       for (int k = 0; k < NBINS; k++) {
         if (first) launch_decide_first<T>(k, g, sdec, ctx->front, ctx->bstart_front, caps[k], m2max, s, ctx->fvar[k]);
         else launch_decide<T>(k, g, sdec, ctx->front, ctx->bstart_front, caps[k], m2max, s, ctx->dvar[k]);
       }
-      launch_apply<T>(st, g.n, m2max, s);
+      launch_apply<T>(st, g.n, m2max, s, LA, nA);
       if constexpr (incr) {
-        for (int k = 0; k < NBINS; k++)
-          launch_weight_moved<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], s, ctx->wvar[k]);
+        for (int k = 0; k < NBINS; k++) launch_weight_moved<T>(k, g, st, LA, BA, caps[k], s, ctx->wvar[k]);
       } else {
-        for (int k = 0; k < NBINS; k++)
-          launch_weight<T>(k, g, st, ctx->order, ctx->bstart_full, caps[k], true, s, ctx->wvar[k]);
+        for (int k = 0; k < NBINS; k++) launch_weight<T>(k, g, st, LA, BA, caps[k], true, s, ctx->wvar[k]);
       }
-      launch_prune<T>(st, g.n, m2max, s);
+      launch_prune<T>(st, g.n, m2max, s, LA, nA);
       launch_graph_sums(gid, ctx->binid, (const int32_t*)ctx->In, (const int32_t*)ctx->Tot, g.n, b.g_in, b.g_tot2,
-                        s);
+                        s, LA, nA);
       launch_batch_step(b, it_dev, min_mod, max_it, s);
-      launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, false, it_dev, incr ? st.cp : nullptr, s);
-      partition_bins(1, ctx->order, ctx->bstart_full, ctx->binid, ctx->active, nitems, ctx->front,
-                     ctx->bstart_front, ctx->part_tmp, s);
+      launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, false, it_dev, incr ? st.cp : nullptr, s, LA,
+                        nA, ctx->b_vlive);
+      partition_bins(1, LA, BA, ctx->binid, ctx->b_vlive, nitems, LB, BB, ctx->part_tmp, s, nA);
+      copy_list(LB, LA, BB + NBINS, nitems, s);
+      copy_i32(BB, BA, NBINS + 1, s);
+      partition_bins(1, LA, BA, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp, s,
+                     nA);
     };
     auto live_after = [&](int32_t* out_live) -> int {
       HIPOK(hipMemcpyAsync(ctx->b_host, nlive, sizeof(int32_t), hipMemcpyDeviceToHost, s));

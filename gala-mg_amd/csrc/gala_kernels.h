@@ -18,11 +18,14 @@ int64_t part_tmp_elems(int64_t n);
 // order already grouped by bin, with its NBINS+1 bin starts in order_bstart
 // (device), and keep that grouping: a stable compaction.
 void partition_bins(int mode, const int32_t* order, const int32_t* order_bstart, const int8_t* binid,
-                    const uint8_t* active, int64_t n, int32_t* out, int32_t* bstart, int32_t* tmp, hipStream_t s);
+                    const uint8_t* active, int64_t n, int32_t* out, int32_t* bstart, int32_t* tmp, hipStream_t s,
+                    const int32_t* n_dev = nullptr);
 
 template <class V>
 void fill(V* p, int64_t n, V x, hipStream_t s);
 void copy_i32(const int32_t* a, int32_t* b, int64_t n, hipStream_t s);
+// b[0 .. min(*cnt, n)) = a[...]: a list whose length is on the device
+void copy_list(const int32_t* a, int32_t* b, const int32_t* cnt, int64_t n, hipStream_t s);
 // prev = cur after an accepted sweep; cp (if not null) = {cur, cur}
 void accept_sweep(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, hipStream_t s);
 // cp = {cur, cur} without touching prev: the moves of a sweep already applied
@@ -145,8 +148,12 @@ void launch_init_rows(const DevGraph<T>& g, DevState<T>& st, const int32_t* orde
 // decide-and-move for one bin over front[bstart[b] .. bstart[b+1]) (device counts);
 // `cap` = host upper bound of that range (used only to size the grid).
 // cv[u] = {prev[u], size[prev[u]], Tot[prev[u]]} for u < n, before a sweep.
+// Vertex lists: the O(n) passes below take an optional device list of the
+// vertices to visit (vlist[0 .. *vcnt)); without one they visit 0..n-1.  The
+// batched path passes the vertices of the graphs still iterating.
 template <class T>
-void launch_build_cv(DevState<T>& st, int32_t n, hipStream_t s);
+void launch_build_cv(DevState<T>& st, int32_t n, hipStream_t s, const int32_t* vlist = nullptr,
+                     const int32_t* vcnt = nullptr);
 
 // variant: kernel variant of the bin (0 = default); see decide_variants().
 template <class T>
@@ -164,7 +171,8 @@ void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const i
 // Tot and size take the sweep's moves (from the {cur, prev} pairs, tile sums
 // per community), then sums of Tot^2, min Tot, non-empty count.
 template <class T>
-void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);
+void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s, const int32_t* vlist = nullptr,
+                  const int32_t* vcnt = nullptr);
 
 // In[v] over order[bstart[b] .. bstart[b+1]) (only the dirty vertices when only_dirty).
 // variant 1: pipelined kernels for bins 0-4 (same result).
@@ -180,14 +188,17 @@ void launch_weight_moved(int bin, const DevGraph<T>& g, DevState<T>& st, const i
                          const int32_t* bstart, int cap, hipStream_t s, int variant = 0);
 // prune flags, sum In, active count; clears the dirty marks.
 template <class T>
-void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s);
+void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s, const int32_t* vlist = nullptr,
+                  const int32_t* vcnt = nullptr);
 
 // ---- batched graphs (kernels_batch.hip) ----
 void launch_graph_sums(const int32_t* gid, const int8_t* binid, const int32_t* In, const int32_t* Tot, int32_t n,
-                       unsigned long long* g_in, unsigned long long* g_tot2, hipStream_t s);
+                       unsigned long long* g_in, unsigned long long* g_tot2, hipStream_t s,
+                       const int32_t* vlist = nullptr, const int32_t* vcnt = nullptr);
 void launch_batch_step(const BatchDev& b, const int32_t* it_dev, double min_mod, int max_it, hipStream_t s);
 void launch_batch_mask(const BatchDev& b, const int32_t* gid, const int32_t* cur, int32_t* prev, uint8_t* active,
-                       int32_t n, bool first, int32_t* it_dev, int2* cp, hipStream_t s);
+                       int32_t n, bool first, int32_t* it_dev, int2* cp, hipStream_t s, const int32_t* vlist = nullptr,
+                       const int32_t* vcnt = nullptr, uint8_t* vlive = nullptr);
 void launch_batch_live(const BatchDev& b, int32_t* it_dev, hipStream_t s);
 void launch_batch_round(const BatchDev& b, int round, double min_mod, int max_rounds, int32_t* nopen, hipStream_t s);
 void launch_batch_gid_next(const int32_t* gid, const int32_t* cnt, const int32_t* nid, int32_t n, int32_t* gid_next,

@@ -33,14 +33,18 @@ __device__ __forceinline__ void wave_seg_add(unsigned long long* tab, int key, u
 // communities (community c belongs to the graph of vertex c).
 __global__ __launch_bounds__(256) void k_graph_sums(const int32_t* __restrict__ gid, const int8_t* __restrict__ binid,
                                                      const int32_t* __restrict__ In, const int32_t* __restrict__ Tot,
-                                                     int32_t n, unsigned long long* g_in, unsigned long long* g_tot2) {
+                                                     int32_t n, unsigned long long* g_in, unsigned long long* g_tot2,
+                                                     const int32_t* __restrict__ vlist,
+                                                     const int32_t* __restrict__ vcnt) {
   const int lane = threadIdx.x & 63;
+  if (vlist) n = *vcnt;  // listed vertices (grouped by bin, ascending ids within a bin)
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t v = base + lane;
+    const int64_t i = base + lane;
     int key = -1;
     unsigned long long xi = 0, xt = 0;
-    if (v < n) {
+    if (i < n) {
+      const int v = vlist ? vlist[i] : (int)i;
       key = gid[v];
       const long long t = Tot[v];
       xt = (unsigned long long)(t * t);
@@ -80,14 +84,20 @@ __global__ void k_batch_step(BatchDev b, const int32_t* __restrict__ it_dev, dou
 
 // prev = cur for the graphs whose sweep was accepted; vertices of graphs that
 // stopped leave the frontier.  first: the round's first frontier is every
-// vertex of a live graph.  Advances the iteration counter.
+// vertex of a live graph.  Advances the iteration counter.  vlive (if given)
+// receives whether the vertex's graph iterates on: the next sweep's list.
 __global__ __launch_bounds__(256) void k_batch_mask(BatchDev b, const int32_t* __restrict__ gid,
                                                      const int32_t* __restrict__ cur, int32_t* __restrict__ prev,
                                                      uint8_t* __restrict__ active, int32_t n, int first,
-                                                     int32_t* __restrict__ it_dev, int2* __restrict__ cp) {
+                                                     int32_t* __restrict__ it_dev, int2* __restrict__ cp,
+                                                     const int32_t* __restrict__ vlist,
+                                                     const int32_t* __restrict__ vcnt, uint8_t* __restrict__ vlive) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *it_dev += 1;
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+  if (vlist) n = *vcnt;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int v = vlist ? vlist[i] : i;
     const int g = gid[v];
+    if (vlive) vlive[v] = b.live[g];
     if (first) {
       active[v] = b.live[g];
       continue;
@@ -168,16 +178,19 @@ static int blocks_for(int64_t n, int per, int cap) {
 }
 
 void launch_graph_sums(const int32_t* gid, const int8_t* binid, const int32_t* In, const int32_t* Tot, int32_t n,
-                       unsigned long long* g_in, unsigned long long* g_tot2, hipStream_t s) {
+                       unsigned long long* g_in, unsigned long long* g_tot2, hipStream_t s, const int32_t* vlist,
+                       const int32_t* vcnt) {
   if (n <= 0) return;
-  k_graph_sums<<<blocks_for(n, 256, 8192), 256, 0, s>>>(gid, binid, In, Tot, n, g_in, g_tot2);
+  k_graph_sums<<<blocks_for(n, 256, 8192), 256, 0, s>>>(gid, binid, In, Tot, n, g_in, g_tot2, vlist, vcnt);
 }
 void launch_batch_step(const BatchDev& b, const int32_t* it_dev, double min_mod, int max_it, hipStream_t s) {
   k_batch_step<<<blocks_for(b.G, 256, 1 << 20), 256, 0, s>>>(b, it_dev, min_mod, max_it);
 }
 void launch_batch_mask(const BatchDev& b, const int32_t* gid, const int32_t* cur, int32_t* prev, uint8_t* active,
-                       int32_t n, bool first, int32_t* it_dev, int2* cp, hipStream_t s) {
-  k_batch_mask<<<blocks_for(n, 256, 8192), 256, 0, s>>>(b, gid, cur, prev, active, n, first ? 1 : 0, it_dev, cp);
+                       int32_t n, bool first, int32_t* it_dev, int2* cp, hipStream_t s, const int32_t* vlist,
+                       const int32_t* vcnt, uint8_t* vlive) {
+  k_batch_mask<<<blocks_for(n, 256, 8192), 256, 0, s>>>(b, gid, cur, prev, active, n, first ? 1 : 0, it_dev, cp, vlist,
+                                                        vcnt, vlive);
 }
 void launch_batch_live(const BatchDev& b, int32_t* it_dev, hipStream_t s) {
   k_batch_live<<<blocks_for(b.G, 256, 1 << 20), 256, 0, s>>>(b, it_dev);

@@ -226,8 +226,9 @@ __device__ __forceinline__ bool cmp_keep(int mode, const uint8_t* __restrict__ f
 
 __global__ __launch_bounds__(CMP_T) void k_cmp_count(int mode, const int32_t* __restrict__ order,
                                                      const uint8_t* __restrict__ flag, int64_t n,
-                                                     int32_t* __restrict__ cnt) {
+                                                     int32_t* __restrict__ cnt, const int32_t* __restrict__ n_dev) {
   __shared__ int32_t sh[17];
+  if (n_dev && *n_dev < n) n = *n_dev;  // the list's length on the device (n is its bound)
   const int64_t base = (int64_t)blockIdx.x * CMP_TILE;
   int c = 0;
 #pragma unroll
@@ -244,8 +245,9 @@ __global__ __launch_bounds__(CMP_T) void k_cmp_count(int mode, const int32_t* __
 // at the total
 __global__ __launch_bounds__(1024) void k_cmp_top(int32_t* __restrict__ cnt, int nblk,
                                                   const int32_t* __restrict__ bstart_in, int64_t n,
-                                                  int32_t* __restrict__ bstart) {
+                                                  int32_t* __restrict__ bstart, const int32_t* __restrict__ n_dev) {
   __shared__ int32_t sh[17];
+  if (n_dev && *n_dev < n) n = *n_dev;
   int32_t carry = 0;
   for (int c0 = 0; c0 < nblk; c0 += 1024) {
     const int i = c0 + threadIdx.x;
@@ -262,8 +264,10 @@ __global__ __launch_bounds__(CMP_T) void k_cmp_scatter(int mode, const int32_t* 
                                                        const uint8_t* __restrict__ flag, int64_t n,
                                                        const int32_t* __restrict__ cnt,
                                                        const int32_t* __restrict__ bstart_in,
-                                                       int32_t* __restrict__ out, int32_t* __restrict__ bstart) {
+                                                       int32_t* __restrict__ out, int32_t* __restrict__ bstart,
+                                                       const int32_t* __restrict__ n_dev) {
   __shared__ int32_t wsum[CMP_T / 64];
+  if (n_dev && *n_dev < n) n = *n_dev;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const int64_t wbase = (int64_t)blockIdx.x * CMP_TILE + (int64_t)w * 64 * CMP_IPT;
@@ -301,12 +305,13 @@ __global__ __launch_bounds__(CMP_T) void k_cmp_scatter(int mode, const int32_t* 
 }
 
 void partition_bins(int mode, const int32_t* order, const int32_t* order_bstart, const int8_t* binid,
-                    const uint8_t* active, int64_t n, int32_t* out, int32_t* bstart, int32_t* tmp, hipStream_t s) {
+                    const uint8_t* active, int64_t n, int32_t* out, int32_t* bstart, int32_t* tmp, hipStream_t s,
+                    const int32_t* n_dev) {
   if (mode != 0) {
     const int nblk = std::max<int>(1, (int)((n + CMP_TILE - 1) / CMP_TILE));
-    k_cmp_count<<<nblk, CMP_T, 0, s>>>(mode, order, active, n, tmp);
-    k_cmp_top<<<1, 1024, 0, s>>>(tmp, nblk, order_bstart, n, bstart);
-    k_cmp_scatter<<<nblk, CMP_T, 0, s>>>(mode, order, active, n, tmp, order_bstart, out, bstart);
+    k_cmp_count<<<nblk, CMP_T, 0, s>>>(mode, order, active, n, tmp, n_dev);
+    k_cmp_top<<<1, 1024, 0, s>>>(tmp, nblk, order_bstart, n, bstart, n_dev);
+    k_cmp_scatter<<<nblk, CMP_T, 0, s>>>(mode, order, active, n, tmp, order_bstart, out, bstart, n_dev);
     return;
   }
   int nblk = (int)((n + PART_TILE - 1) / PART_TILE);
@@ -369,6 +374,18 @@ void settle_pairs(int2* cp, int64_t n, hipStream_t s) {
   if (n <= 0 || !cp) return;
   int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
   k_settle_pairs<<<nb, 256, 0, s>>>(cp, n);
+}
+
+__global__ void k_copy_list(const int32_t* __restrict__ a, int32_t* __restrict__ b, const int32_t* __restrict__ cnt,
+                            int64_t n) {
+  const int64_t m = *cnt < n ? *cnt : n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+void copy_list(const int32_t* a, int32_t* b, const int32_t* cnt, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  k_copy_list<<<nb, 256, 0, s>>>(a, b, cnt, n);
 }
 
 void copy_i32(const int32_t* a, int32_t* b, int64_t n, hipStream_t s) {

@@ -290,8 +290,11 @@ void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s) {
 // community view for the sweep: cv[u] = {prev[u], size[prev[u]], Tot[prev[u]]}
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_build_cv(DevState<T> st, int32_t n) {
-  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(256) void k_build_cv(DevState<T> st, int32_t n, const int32_t* __restrict__ vlist,
+                                                  const int32_t* __restrict__ vcnt) {
+  if (vlist) n = *vcnt;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int u = vlist ? vlist[i] : i;
     const int c = st.prev[u];
     typename T::cv_t e;
     cv_set(e, c, st.size[c], st.Tot[c]);
@@ -300,9 +303,9 @@ __global__ __launch_bounds__(256) void k_build_cv(DevState<T> st, int32_t n) {
 }
 
 template <class T>
-void launch_build_cv(DevState<T>& st, int32_t n, hipStream_t s) {
+void launch_build_cv(DevState<T>& st, int32_t n, hipStream_t s, const int32_t* vlist, const int32_t* vcnt) {
   if (n <= 0) return;
-  k_build_cv<T><<<std::min((n + 255) / 256, 8192), 256, 0, s>>>(st, n);
+  k_build_cv<T><<<std::min((n + 255) / 256, 8192), 256, 0, s>>>(st, n, vlist, vcnt);
 }
 
 // ---------------------------------------------------------------------------
@@ -2463,7 +2466,8 @@ void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const i
 // ---------------------------------------------------------------------------
 constexpr int TD_T = 512, TD_S = 4 * TD_T, TD_LG = 11;  // <= 2 keys per vertex: half load at most
 template <class T>
-__global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n) {
+__global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n, const int32_t* __restrict__ vlist,
+                                                   const int32_t* __restrict__ vcnt) {
   using tt = typename T::tot_t;
   __shared__ int s_key[TD_S];
   __shared__ int s_cnt[TD_S];
@@ -2486,10 +2490,12 @@ __global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n) {
       p = (p + 1) & (TD_S - 1);
     }
   };
+  if (vlist) n = *vcnt;
   for (int64_t base = (int64_t)blockIdx.x * TD_T; base < n; base += (int64_t)gridDim.x * TD_T) {
     for (int j = threadIdx.x; j < TD_T; j += blockDim.x) {
-      const int64_t v = base + j;
-      if (v >= n) break;
+      const int64_t i = base + j;
+      if (i >= n) break;
+      const int v = vlist ? vlist[i] : (int)i;
       const int2 cp = st.cp[v];  // {cur, prev}
       if (cp.x != cp.y) {
         const tt k = st.K[v];
@@ -2519,7 +2525,8 @@ __global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n) {
 // non-empty count over the rebuilt Tot and size.
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_apply(DevState<T> st, int32_t n, typename T::tot_t m2) {
+__global__ __launch_bounds__(256) void k_apply(DevState<T> st, int32_t n, typename T::tot_t m2,
+                                               const int32_t* __restrict__ vlist, const int32_t* __restrict__ vcnt) {
   using tt = typename T::tot_t;
   __shared__ __int128 sh_t2[4];
   __shared__ unsigned long long sh_min[4];
@@ -2527,7 +2534,9 @@ __global__ __launch_bounds__(256) void k_apply(DevState<T> st, int32_t n, typena
   __int128 t2 = 0;  // sum Tot^2: <= m2^2 < 2^124 on the wide path
   unsigned long long mn = (unsigned long long)(long long)m2;
   int ne = 0;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+  if (vlist) n = *vcnt;  // communities are vertex ids: those of the listed vertices
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = vlist ? vlist[i] : i;
     const tt t = st.Tot[c];
     const int sz = st.size[c];
     ne += sz > 0;
@@ -2599,11 +2608,12 @@ __global__ __launch_bounds__(256) void k_apply_fin(DevState<T> st, int nb) {
 }
 
 template <class T>
-void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s) {
+void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s, const int32_t* vlist,
+                  const int32_t* vcnt) {
   if (n <= 0) return;
   const int nb = std::min((n + 255) / 256, MAX_PARTS);
-  k_tot_delta<T><<<std::min((n + TD_T - 1) / TD_T, 2048), 256, 0, s>>>(st, n);
-  k_apply<T><<<nb, 256, 0, s>>>(st, n, m2);
+  k_tot_delta<T><<<std::min((n + TD_T - 1) / TD_T, 2048), 256, 0, s>>>(st, n, vlist, vcnt);
+  k_apply<T><<<nb, 256, 0, s>>>(st, n, m2, vlist, vcnt);
   k_apply_fin<T><<<1, 256, 0, s>>>(st, nb);
 }
 
@@ -3014,12 +3024,15 @@ void launch_weight(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
 // (the prune test of compute_In, kernel_functions.cu:944,1028)
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_prune(DevState<T> st, int32_t n, typename T::tot_t m2) {
+__global__ __launch_bounds__(256) void k_prune(DevState<T> st, int32_t n, typename T::tot_t m2,
+                                               const int32_t* __restrict__ vlist, const int32_t* __restrict__ vcnt) {
   using tt = typename T::tot_t;
   __shared__ unsigned long long s_in[4], s_act[4];
   const tt minTot = read_min_tot<T>(st.ctr);
   unsigned long long insum = 0, act = 0;  // sum In <= m2 < 2^62
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+  if (vlist) n = *vcnt;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int v = vlist ? vlist[i] : i;
     if (st.binid[v] < 0) continue;
     const tt x = st.In[v];
     const uint8_t keep = prune_keep<T>(x, st.K[v], st.Self[v], st.Tot[st.cur[v]], minTot, vertex_m2<T>(st, v, m2));
@@ -3066,27 +3079,30 @@ __global__ __launch_bounds__(256) void k_prune_fin(DevState<T> st, int nb) {
 }
 
 template <class T>
-void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s) {
+void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s, const int32_t* vlist,
+                  const int32_t* vcnt) {
   if (n <= 0) return;
   const int nb = std::min((n + 255) / 256, MAX_PARTS);
-  k_prune<T><<<nb, 256, 0, s>>>(st, n, m2);
+  k_prune<T><<<nb, 256, 0, s>>>(st, n, m2, vlist, vcnt);
   k_prune_fin<T><<<1, 256, 0, s>>>(st, nb);
 }
 
 // explicit instantiations
 #define GALA_INST(T)                                                                              \
   template void launch_init<T>(const DevGraph<T>&, DevState<T>&, hipStream_t);                    \
-  template void launch_build_cv<T>(DevState<T>&, int32_t, hipStream_t);                           \
+  template void launch_build_cv<T>(DevState<T>&, int32_t, hipStream_t, const int32_t*, const int32_t*);                           \
   template void launch_init_rows<T>(const DevGraph<T>&, DevState<T>&, const int32_t*, const int32_t*,         \
                                     const int32_t*, hipStream_t);                                  \
   template void launch_decide<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,           \
                                  const int32_t*, int, typename T::tot_t, hipStream_t, int);       \
   template void launch_decide_first<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,     \
                                        const int32_t*, int, typename T::tot_t, hipStream_t, int); \
-  template void launch_apply<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t);           \
+  template void launch_apply<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t, const int32_t*,        \
+                                const int32_t*);           \
   template void launch_weight<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,           \
                                  const int32_t*, int, bool, hipStream_t, int);                    \
-  template void launch_prune<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t);
+  template void launch_prune<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t, const int32_t*,        \
+                                const int32_t*);
 GALA_INST(IntW)
 GALA_INST(WideW)
 template void launch_weight_moved<IntW>(int, const DevGraph<IntW>&, DevState<IntW>&, const int32_t*, const int32_t*,
