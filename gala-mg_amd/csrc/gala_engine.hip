@@ -124,6 +124,7 @@ struct gala_ctx {
   float* last_gain = nullptr;
   int32_t* moved_it = nullptr;
   float* nbr_moved = nullptr;
+  int32_t* churn = nullptr;
   int32_t* front2 = nullptr;   // predicted movers of a learned sweep, then its complement
   int32_t* bstart2 = nullptr;
   int32_t* front3 = nullptr;   // second frontier buffer (the pre-sweep frontier survives a sweep)
@@ -261,7 +262,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool wide) {
   OWN(ctx->order, N); OWN(ctx->front, N); OWN(ctx->fin, N);
   { char* p; OWN(p, N * 16); ctx->cv = p; }
   OWN(ctx->cp, N);
-  OWN(ctx->last_gain, N); OWN(ctx->moved_it, N); OWN(ctx->nbr_moved, N);
+  OWN(ctx->last_gain, N); OWN(ctx->moved_it, N); OWN(ctx->nbr_moved, N); OWN(ctx->churn, N);
   OWN(ctx->front2, N); OWN(ctx->bstart2, NBINS + 1); OWN(ctx->keep, N);
   OWN(ctx->front3, N); OWN(ctx->bstart3, NBINS + 1); OWN(ctx->size_snap, N);
   { char* p; OWN(p, N * tsz); ctx->tot_snap = p; }
@@ -638,10 +639,11 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   // decide-and-move over one frontier, timed into t_dm (and per bin)
   // build_cv: refresh the community view first (a certification sweep reads
   // the view of the pre-sweep state, which is still in place)
-  // Plain runs update In from the movers' rows alone (no dirty marks); the
-  // tracked (learned / logged) runs rescan dirty rows, which yields the
-  // neighbour-moved fractions.
-  const bool incr = !track;
+  // In is updated from the movers' rows alone (no dirty marks); tracked
+  // (learned / logged) runs also count each community's churn for the
+  // features.
+  const bool incr = true;
+  if (track) st.churn = ctx->churn;
   auto decide = [&](DevState<T>& sd0, const int32_t* fr, const int32_t* bs, bool first, bool build_cv) -> int {
     DevState<T> sd = sd0;
     if (incr) sd.dirty = nullptr;
@@ -659,7 +661,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   // Tot/size update, In of the invalidated rows, prune, next frontier and the
   // counters; accumulates the sweep's work and timings
   auto update = [&](DevState<T>& sp, bool first, Counters* out) -> int {
-    launch_apply<T>(st, g.n, m2, s);
+    launch_apply<T>(sp, g.n, m2, s);
     // In changes only where a vertex or one of its neighbours moved
     if (incr) {
       for (int b = 0; b < NBINS; b++)
@@ -772,7 +774,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       sc.size = ctx->size_snap;
       if ((rc = decide(sc, ctx->front2, ctx->bstart2, false, false))) return rc;
       DevState<T> sp = st;
-      sp.track = 0;  // keep the neighbour-moved fractions of the learned sweep's rows
+      sp.churn = nullptr;  // keep the churn of the learned sweep for the features
       if ((rc = update(sp, false, &c))) return rc;
       nfront = (int32_t)c.active;
       moved += (int64_t)c.moved;

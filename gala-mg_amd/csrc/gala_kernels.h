@@ -103,7 +103,8 @@ struct DevState {
   int iter;          // sweep number within the round (1 = first sweep)
   float* last_gain;  // normalised net gain of the best move at the vertex's last sweep
   int32_t* moved_it; // sweep in which the vertex last moved (-1: never this round)
-  float* nbr_moved;  // fraction of neighbours that moved in the latest sweep
+  float* nbr_moved;  // fraction of neighbours that moved (dirty-row In updates of tracked runs)
+  int32_t* churn = nullptr;  // per community: members that joined or left in the latest sweep
   // batched graphs (block-diagonal union): graph of every vertex and the m2 of
   // every graph; null for a single graph
   const int32_t* gid = nullptr;
@@ -210,8 +211,10 @@ void launch_batch_reset(Counters* c, long long min_init, int32_t* nlive, hipStre
 // Features of vertex v before sweep it (all in [-1, 1]):
 //   f0 log2(1 + degree)/16, f1 In/K (weight fraction inside its community),
 //   f2 normalised net gain of its best move at its last sweep,
-//   f3 moved in sweep it-1, f4 fraction of neighbours that moved in sweep it-1,
-//   f5 min(it, 16)/16.
+//   f3 moved in sweep it-1, f4 churn of its community in sweep it-1 (members
+//   that joined or left, over its size, at most 1), f5 min(it, 16)/16.
+// All of them are O(1) per vertex, so tracked runs keep the incremental In
+// update (no row rescans).
 template <class T>
 void launch_mlp_filter(const DevGraph<T>& g, DevState<T>& st, const int32_t* front, int nfront, int it,
                        const float* prm, const short* w1b, float tau, uint8_t* keep, hipStream_t s);

@@ -42,7 +42,10 @@ __device__ __forceinline__ void vertex_features(const DevGraph<T>& g, const DevS
   const float lg = st.last_gain[v];
   f[2] = lg < -1.0f ? -1.0f : (lg > 1.0f ? 1.0f : lg);
   f[3] = st.moved_it[v] == it - 1 ? 1.0f : 0.0f;
-  f[4] = st.nbr_moved[v];
+  const int c = st.cur[v];
+  const int sz = st.size[c];
+  const float ch = sz > 0 ? (float)st.churn[c] / (float)sz : 0.0f;
+  f[4] = ch < 1.0f ? ch : 1.0f;
   f[5] = (float)(it < 16 ? it : 16) * (1.0f / 16.0f);
 }
 

@@ -2473,8 +2473,9 @@ __global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n, co
   __shared__ int s_cnt[TD_S];
   __shared__ tt s_tot[TD_S];
   __shared__ int s_list[TD_S];
+  __shared__ int s_mv[TD_S];  // members that joined or left (tracked runs)
   __shared__ int s_n;
-  for (int t = threadIdx.x; t < TD_S; t += blockDim.x) { s_key[t] = 0; s_cnt[t] = 0; s_tot[t] = tt(0); }
+  for (int t = threadIdx.x; t < TD_S; t += blockDim.x) { s_key[t] = 0; s_cnt[t] = 0; s_tot[t] = tt(0); s_mv[t] = 0; }
   if (threadIdx.x == 0) s_n = 0;
   __syncthreads();
   auto add = [&](int c, tt k, int one) {
@@ -2485,6 +2486,7 @@ __global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n, co
         if (kk == 0) s_list[atomicAdd(&s_n, 1)] = (int)p;
         atomicAdd(&s_cnt[p], one);
         if (k) atomicAdd(&s_tot[p], one > 0 ? k : -k);
+        if (st.churn) atomicAdd(&s_mv[p], 1);
         return;
       }
       p = (p + 1) & (TD_S - 1);
@@ -2510,9 +2512,11 @@ __global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n, co
       const int c = s_key[p] - 1;
       if (s_cnt[p]) atomicAdd(&st.size[c], s_cnt[p]);
       if (s_tot[p]) atomicAdd(&st.Tot[c], s_tot[p]);
+      if (st.churn) atomicAdd(&st.churn[c], s_mv[p]);
       s_key[p] = 0;
       s_cnt[p] = 0;
       s_tot[p] = tt(0);
+      s_mv[p] = 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) s_n = 0;
@@ -2612,6 +2616,7 @@ void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t 
                   const int32_t* vcnt) {
   if (n <= 0) return;
   const int nb = std::min((n + 255) / 256, MAX_PARTS);
+  if (st.churn) fill<int32_t>(st.churn, n, 0, s);  // this sweep's churn replaces the last one's
   k_tot_delta<T><<<std::min((n + TD_T - 1) / TD_T, 2048), 256, 0, s>>>(st, n, vlist, vcnt);
   k_apply<T><<<nb, 256, 0, s>>>(st, n, m2, vlist, vcnt);
   k_apply_fin<T><<<1, 256, 0, s>>>(st, nb);
@@ -3037,7 +3042,7 @@ __global__ __launch_bounds__(256) void k_prune(DevState<T> st, int32_t n, typena
     const tt x = st.In[v];
     const uint8_t keep = prune_keep<T>(x, st.K[v], st.Self[v], st.Tot[st.cur[v]], minTot, vertex_m2<T>(st, v, m2));
     st.active[v] = keep;
-    if (st.track && !st.dirty[v]) st.nbr_moved[v] = 0.f;  // no neighbour moved
+    if (st.track && st.dirty && !st.dirty[v]) st.nbr_moved[v] = 0.f;  // no neighbour moved
     st.dirty[v] = 0;
     act += keep;
     insum += (unsigned long long)(long long)x;
