@@ -51,6 +51,10 @@ struct gala_ctx {
   gala_opts opts{};
   hipError_t last_hip = hipSuccess;
 
+  // last device text ingest (two-call protocol) and its key: path + weight mode
+  gala::IngestResult ingest;
+  std::string ingest_key;
+
   // input graph (sizes) and mode
   int32_t n0 = 0;
   int64_t m0 = 0;
@@ -1685,6 +1689,60 @@ int gala_phase_weight(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const i
   HIPOK(hipMemcpy(In_out, ctx->In, 4ll * n, hipMemcpyDeviceToHost));
   HIPOK(hipMemcpy(active_out, ctx->active, n, hipMemcpyDeviceToHost));
   ctx->have_graph = false;
+  return GALA_OK;
+}
+
+// Device text ingest: a first call (NULL arrays) parses the file on the GPU
+// and keeps the CSR in the context; a call with arrays copies it out (and
+// ingests first when the context holds another file or mode).
+int gala_ingest_text_device(gala_ctx* ctx, const char* path, int weight_mode, int32_t* n, int64_t* m,
+                            int64_t* end_incl, int32_t* nbr, void* w) {
+  if (!ctx || !path || !n || !m || weight_mode < GALA_W_NONE || weight_mode > GALA_W_F32) return GALA_E_INVALID;
+  std::string key;
+  try {
+    key = std::string(path) + "\n" + std::to_string(weight_mode);
+  } catch (const std::exception&) {
+    return GALA_E_OOM;
+  }
+  if (!end_incl || ctx->ingest_key != key) {
+    ctx->ingest_key.clear();
+    int rc;
+    try {
+      rc = ingest_text_device(ctx->device, ctx->stream, path, weight_mode, &ctx->ingest);
+    } catch (const std::exception&) {
+      rc = GALA_E_OOM;
+    }
+    if (rc) return rc;
+    ctx->ingest_key = key;
+  }
+  const IngestResult& r = ctx->ingest;
+  *n = r.n;
+  *m = (int64_t)r.nbr.size();
+  if (end_incl && r.n) std::memcpy(end_incl, r.end.data(), 8 * (size_t)r.n);
+  if (nbr && !r.nbr.empty()) std::memcpy(nbr, r.nbr.data(), 4 * r.nbr.size());
+  if (w && weight_mode != GALA_W_NONE && !r.w.empty()) std::memcpy(w, r.w.data(), 4 * r.w.size());
+  return GALA_OK;
+}
+
+int gala_load_text(gala_ctx* ctx, const char* path, int weight_mode) {
+  int32_t n = 0;
+  int64_t m = 0;
+  int rc = gala_ingest_text_device(ctx, path, weight_mode, &n, &m, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const IngestResult& r = ctx->ingest;
+  return upload(ctx, n, r.end.data(), r.nbr.data(), r.w.empty() ? nullptr : r.w.data(), weight_mode);
+}
+
+int gala_ingest_times(gala_ctx* ctx, double* read_ms, double* load_ms, double* device_ms, double* d2h_ms,
+                      double* wall_ms) {
+  if (!ctx || ctx->ingest_key.empty()) return GALA_E_INVALID;
+  const IngestResult& r = ctx->ingest;
+  if (read_ms) *read_ms = r.read_ms;
+  if (load_ms) *load_ms = r.h2d_ms;
+  if (device_ms) *device_ms = r.device_ms;
+  if (d2h_ms) *d2h_ms = r.d2h_ms;
+  if (wall_ms) *wall_ms = r.wall_ms;
   return GALA_OK;
 }
 

@@ -324,6 +324,17 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
 // final[v] = label[final[v]]
 void compose_map(int32_t* fin, int32_t n0, const int32_t* label, hipStream_t s);
 
+// ---- text edge list -> CSR on the device (ingest.hip) ----
+struct IngestResult {
+  int32_t n = 0;
+  std::vector<int64_t> end;  // inclusive row ends
+  std::vector<int32_t> nbr;
+  std::vector<uint32_t> w;   // float32 or int32 bits, empty when unweighted
+  double read_ms = 0, h2d_ms = 0, device_ms = 0, d2h_ms = 0, wall_ms = 0;
+  int64_t bytes = 0;
+};
+int ingest_text_device(int device, hipStream_t s, const char* path, int weight_mode, IngestResult* out);
+
 // ---- modularity of a given partition (modularity.hip) ----
 int modularity_scratch_setup(int device);
 int modularity_host(int device, hipStream_t s, int32_t n, const int64_t* end_incl, const int32_t* nbr, const void* w,

@@ -16,10 +16,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
-#include <fstream>
-#include <map>
-#include <sstream>
+#include <exception>
 #include <string>
 #include <thread>
 #include <atomic>
@@ -58,7 +57,8 @@ struct Csr {
   int32_t n = 0;
   std::vector<int64_t> end;  // inclusive ends
   std::vector<int32_t> nbr;
-  std::vector<float> w;
+  std::vector<float> w;     // float32 weights
+  std::vector<int32_t> wi;  // int32 weights (text ingest, GALA_W_I32)
 };
 
 void csr_from_pairs(int32_t n, const std::vector<std::pair<int32_t, int32_t>>& e, Csr& out) {
@@ -166,6 +166,178 @@ struct BatchCache {
   BatchCsr u;
 } g_batch;
 
+// ---- text edge list ingest (Graph::Graph, graph.cpp:18-151) ----
+// Line rules: split at '\n'; empty and '#' lines skipped; tokens separated by
+// ' ' '\t' '\r' '\v' '\f'; an edge line starts with two integer tokens
+// [+-]?[0-9]+ (other lines are skipped); ids outside [0, INT32_MAX] are a
+// format error; the optional third token is the weight (int32 integer, or a
+// decimal of at most 40 significant digits rounded by strtod).  Rows are
+// bucketed by source in line order, each bucket stably sorted by neighbour,
+// and repeated entries summed in that order (float64 then float32, or int64
+// range-checked to int32) -- the per-row std::map of graph.cpp:54-75.
+bool tok_ws(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+
+bool int_token(const char* p, const char* q, bool* neg, uint64_t* mag) {
+  *neg = false;
+  if (p < q && (*p == '+' || *p == '-')) *neg = *p++ == '-';
+  if (p == q) return false;
+  uint64_t v = 0;
+  for (; p < q; p++) {
+    if (*p < '0' || *p > '9') return false;
+    if (v < (1ull << 40)) v = v * 10 + (uint64_t)(*p - '0');
+  }
+  *mag = v;
+  return true;
+}
+
+// decimal token grammar and the 40-significant-digit limit
+bool decimal_token(const char* p, const char* q) {
+  if (p < q && (*p == '+' || *p == '-')) p++;
+  int64_t digits = 0, first = -1, last = -1;
+  bool point = false;
+  for (; p < q && ((*p >= '0' && *p <= '9') || *p == '.'); p++) {
+    if (*p == '.') {
+      if (point) return false;
+      point = true;
+      continue;
+    }
+    if (*p != '0') {
+      if (first < 0) first = digits;
+      last = digits;
+    }
+    digits++;
+  }
+  if (digits == 0 || (first >= 0 && last - first + 1 > 40)) return false;
+  if (p == q) return true;
+  if (*p != 'e' && *p != 'E') return false;
+  if (++p < q && (*p == '+' || *p == '-')) p++;
+  if (p == q) return false;
+  for (; p < q; p++)
+    if (*p < '0' || *p > '9') return false;
+  return true;
+}
+
+int ingest_text_host(const char* path, int wmode, Csr& out) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return GALA_E_IO;
+  std::string txt;
+  char buf[1 << 16];
+  for (size_t k; (k = std::fread(buf, 1, sizeof buf, f)) > 0;) txt.append(buf, k);
+  const bool bad_read = std::ferror(f) != 0;
+  std::fclose(f);
+  if (bad_read) return GALA_E_IO;
+  std::vector<int32_t> ea, eb, wi;
+  std::vector<double> wd;
+  int32_t maxid = -1;
+  const char* t = txt.data();
+  const char* tend = t + txt.size();
+  std::string tok;
+  for (size_t ls = 0; ls <= txt.size();) {
+    const char* s = t + ls;
+    const char* e = static_cast<const char*>(std::memchr(s, '\n', (size_t)(tend - s)));
+    if (!e) e = tend;
+    ls = (size_t)(e - t) + 1;
+    if (s < e && *s != '#') {
+      const char* p = s;
+      const char* tk[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+      for (int k = 0; k < 3; k++) {
+        while (p < e && tok_ws(*p)) p++;
+        tk[k][0] = p;
+        while (p < e && !tok_ws(*p)) p++;
+        tk[k][1] = p;
+      }
+      bool n1, n2;
+      uint64_t a, b;
+      if (tk[0][0] < tk[0][1] && tk[1][0] < tk[1][1] && int_token(tk[0][0], tk[0][1], &n1, &a) &&
+          int_token(tk[1][0], tk[1][1], &n2, &b)) {
+        if ((n1 && a) || (n2 && b) || a > INT32_MAX || b > INT32_MAX) return GALA_E_FORMAT;
+        const bool has_w = tk[2][0] < tk[2][1];
+        if (wmode == GALA_W_I32) {
+          bool nw = false;
+          uint64_t mw = 1;
+          if (has_w && (!int_token(tk[2][0], tk[2][1], &nw, &mw) || mw > (nw ? 2147483648ull : 2147483647ull)))
+            return GALA_E_FORMAT;
+          wi.push_back(nw ? (int32_t)(-(int64_t)mw) : (int32_t)mw);
+        } else if (wmode == GALA_W_F32) {
+          double x = 1.0;
+          if (has_w) {
+            if (!decimal_token(tk[2][0], tk[2][1])) return GALA_E_FORMAT;
+            tok.assign(tk[2][0], tk[2][1]);
+            x = std::strtod(tok.c_str(), nullptr);
+            if (std::isinf(x)) return GALA_E_FORMAT;
+          }
+          wd.push_back(x);
+        }
+        ea.push_back((int32_t)a);
+        eb.push_back((int32_t)b);
+        maxid = std::max(maxid, (int32_t)std::max(a, b));
+      }
+    }
+  }
+  const int64_t E = (int64_t)ea.size();
+  out = Csr();
+  if (E == 0) return GALA_OK;
+  // renumbering of the linked ids, ascending: a table over [0, maxid] when the
+  // ids are dense enough, else the sorted distinct ids searched per endpoint
+  int32_t nn = 0;
+  if ((int64_t)maxid + 1 <= 8 * E + (1 << 20)) {
+    std::vector<int32_t> renum((size_t)maxid + 1, -1);
+    for (int64_t i = 0; i < E; i++) renum[ea[i]] = renum[eb[i]] = 0;
+    for (auto& r : renum)
+      if (r == 0) r = nn++;
+    for (int64_t i = 0; i < E; i++) { ea[i] = renum[ea[i]]; eb[i] = renum[eb[i]]; }
+  } else {
+    std::vector<int32_t> ids(ea);
+    ids.insert(ids.end(), eb.begin(), eb.end());
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    nn = (int32_t)ids.size();
+    auto rank = [&](int32_t x) { return (int32_t)(std::lower_bound(ids.begin(), ids.end(), x) - ids.begin()); };
+    for (int64_t i = 0; i < E; i++) { ea[i] = rank(ea[i]); eb[i] = rank(eb[i]); }
+  }
+  // bucket entries by row in line order (a->b, then b->a)
+  std::vector<int64_t> start((size_t)nn + 1, 0);
+  for (int64_t i = 0; i < E; i++) { start[ea[i] + 1]++; start[eb[i] + 1]++; }
+  for (int32_t v = 0; v < nn; v++) start[v + 1] += start[v];
+  std::vector<int64_t> cur(start.begin(), start.end() - 1);
+  std::vector<std::pair<int32_t, int64_t>> ent((size_t)(2 * E));  // (neighbour, line)
+  for (int64_t i = 0; i < E; i++) {
+    const int32_t ra = ea[i], rb = eb[i];
+    ent[cur[ra]++] = {rb, i};
+    ent[cur[rb]++] = {ra, i};
+  }
+  out.n = nn;
+  out.end.assign(nn, 0);
+  for (int32_t v = 0; v < nn; v++) {
+    auto b0 = ent.begin() + start[v], b1 = ent.begin() + start[v + 1];
+    std::stable_sort(b0, b1, [](const std::pair<int32_t, int64_t>& x, const std::pair<int32_t, int64_t>& y) {
+      return x.first < y.first;
+    });
+    for (auto it = b0; it != b1;) {
+      auto jt = it;
+      double sd = 0;
+      int64_t si = 0;
+      for (; jt != b1 && jt->first == it->first; ++jt) {
+        if (wmode == GALA_W_F32) sd = jt == it ? wd[jt->second] : sd + wd[jt->second];
+        if (wmode == GALA_W_I32) si += wi[jt->second];
+      }
+      out.nbr.push_back(it->first);
+      if (wmode == GALA_W_F32) {
+        const float fw = (float)sd;
+        if (!std::isfinite(fw)) return GALA_E_FORMAT;
+        out.w.push_back(fw);
+      }
+      if (wmode == GALA_W_I32) {
+        if (si < INT32_MIN || si > INT32_MAX) return GALA_E_FORMAT;
+        out.wi.push_back((int32_t)si);
+      }
+      it = jt;
+    }
+    out.end[v] = (int64_t)out.nbr.size();
+  }
+  return GALA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -226,57 +398,18 @@ int gala_write_bin(const char* path, int32_t n, const int64_t* end_incl, const i
 
 int gala_ingest_text(const char* path, int weight_mode, int32_t* n, int64_t* m, int64_t* end_incl,
                      int32_t* nbr, void* w) {
-  if (!path || !n || !m) return GALA_E_INVALID;
+  if (!path || !n || !m || weight_mode < GALA_W_NONE || weight_mode > GALA_W_F32) return GALA_E_INVALID;
   std::string key = std::string("txt:") + path + ":" + std::to_string(weight_mode);
   if (g_cache.key != key) {
-    std::ifstream fin(path);
-    if (!fin.is_open()) return GALA_E_IO;
-    const bool wt = weight_mode != GALA_W_NONE;
-    std::vector<std::vector<std::pair<int64_t, double>>> rows;
-    std::string line;
-    while (std::getline(fin, line)) {
-      if (line.empty() || line[0] == '#') continue;
-      std::istringstream iss(line);
-      long long a, b;
-      double x = 1;
-      if (!(iss >> a >> b)) continue;
-      if (wt) iss >> x;
-      if (a < 0 || b < 0 || a > INT32_MAX || b > INT32_MAX) return GALA_E_FORMAT;
-      size_t need = (size_t)std::max(a, b) + 1;
-      if (rows.size() < need) rows.resize(need);
-      rows[a].push_back({b, x});
-      rows[b].push_back({a, x});
+    try {
+      Csr c;
+      int rc = ingest_text_host(path, weight_mode, c);
+      if (rc) return rc;
+      g_cache.key = key;
+      g_cache.csr = std::move(c);
+    } catch (const std::exception&) {
+      return GALA_E_OOM;
     }
-    // per-row dedup, ascending (graph.cpp:54-75); weights summed when weighted
-    std::vector<int32_t> renum(rows.size(), -1);
-    std::vector<char> linked(rows.size(), 0);
-    for (size_t i = 0; i < rows.size(); i++)
-      for (auto& p : rows[i]) { linked[i] = 1; linked[p.first] = 1; }
-    int32_t nn = 0;
-    for (size_t i = 0; i < rows.size(); i++)
-      if (linked[i]) renum[i] = nn++;
-    Csr c;
-    c.n = nn;
-    c.end.assign(nn, 0);
-    int64_t pos = 0;
-    for (size_t i = 0; i < rows.size(); i++) {
-      if (!linked[i]) continue;
-      std::map<int32_t, double> mp;
-      for (auto& p : rows[i]) {
-        auto it = mp.find(renum[p.first]);
-        if (it == mp.end()) mp.emplace(renum[p.first], p.second);
-        else if (wt) it->second += p.second;
-      }
-      // a self-loop line pushes the entry twice into its own row; keep one entry
-      for (auto& kv : mp) {
-        c.nbr.push_back(kv.first);
-        if (wt) c.w.push_back((float)kv.second);
-        pos++;
-      }
-      c.end[renum[i]] = pos;
-    }
-    g_cache.key = key;
-    g_cache.csr = std::move(c);
   }
   const Csr& c = g_cache.csr;
   *n = c.n;
@@ -284,8 +417,7 @@ int gala_ingest_text(const char* path, int weight_mode, int32_t* n, int64_t* m, 
   if (end_incl) std::memcpy(end_incl, c.end.data(), 8 * c.n);
   if (nbr) std::memcpy(nbr, c.nbr.data(), 4 * c.nbr.size());
   if (w && weight_mode == GALA_W_F32) std::memcpy(w, c.w.data(), 4 * c.w.size());
-  if (w && weight_mode == GALA_W_I32)
-    for (size_t j = 0; j < c.w.size(); j++) ((int32_t*)w)[j] = (int32_t)std::lround(c.w[j]);
+  if (w && weight_mode == GALA_W_I32) std::memcpy(w, c.wi.data(), 4 * c.wi.size());
   return GALA_OK;
 }
 

@@ -27,6 +27,7 @@ EXPORTS = [
     "gala_mlp_forward", "gala_mlp_grad", "gala_batch_set", "gala_batch_run", "gala_batch_graph_results",
     "gala_batch_communities", "gala_gen_sbm_batch", "gala_set_variants", "gala_get_variants", "gala_autotune",
     "gala_variant_counts", "gala_phase_modularity", "gala_modularity_noalloc",
+    "gala_ingest_text_device", "gala_load_text", "gala_ingest_times",
 ]
 NOALLOC_MAX_N = 1 << 22  # GALA_NOALLOC_MAX_N
 N_VARIANT_SLOTS = 27  # kernel variant per bin: 9 general sweeps, 9 first sweeps, 9 weight updates
@@ -173,6 +174,10 @@ def lib():
         L.gala_read_bin.argtypes = [ctypes.c_char_p, ctypes.c_int, I32P, I64P, I64P, I32P, VP]
         L.gala_write_bin.argtypes = [ctypes.c_char_p, ctypes.c_int32, I64P, I32P, VP, ctypes.c_int]
         L.gala_ingest_text.argtypes = [ctypes.c_char_p, ctypes.c_int, I32P, I64P, I64P, I32P, VP]
+        L.gala_ingest_text_device.argtypes = [VP, ctypes.c_char_p, ctypes.c_int, I32P, I64P, I64P, I32P, VP]
+        L.gala_load_text.argtypes = [VP, ctypes.c_char_p, ctypes.c_int]
+        DP = ctypes.POINTER(ctypes.c_double)
+        L.gala_ingest_times.argtypes = [VP, DP, DP, DP, DP, DP]
         L.gala_gen_sbm.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_uint64, I32P, I64P, I64P, I32P]
         L.gala_gen_chung_lu.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_double,
@@ -269,19 +274,25 @@ class Graph:
                f"write {path}")
 
     @staticmethod
-    def from_text(path, weight_mode=W_NONE):
-        """Graph::Graph(file, is_wt) (graph.cpp:18-151), i.e. preprocess's ingest."""
+    def from_text(path, weight_mode=W_NONE, engine=None):
+        """Graph::Graph(file, is_wt) (graph.cpp:18-151), i.e. preprocess's ingest:
+        on the host (gala_ingest_text), or on engine's GPU when an Engine is
+        given (gala_ingest_text_device; same output byte for byte)."""
         L = lib()
+        if engine is None:
+            fn, pre = L.gala_ingest_text, ()
+        else:
+            fn, pre = L.gala_ingest_text_device, (engine.ctx,)
         n, m = ctypes.c_int32(), ctypes.c_int64()
-        _check(L.gala_ingest_text(path.encode(), weight_mode, ctypes.byref(n), ctypes.byref(m), None,
-                                  None, None), f"ingest {path}")
+        _check(fn(*pre, path.encode(), weight_mode, ctypes.byref(n), ctypes.byref(m), None, None, None),
+               f"ingest {path}")
         end = np.zeros(max(n.value, 1), np.int64)
         nbr = np.zeros(max(m.value, 1), np.int32)
         w = None if weight_mode == W_NONE else np.zeros(max(m.value, 1),
                                                           np.float32 if weight_mode == W_F32 else np.int32)
-        _check(L.gala_ingest_text(path.encode(), weight_mode, ctypes.byref(n), ctypes.byref(m),
-                                  _p(end, ctypes.c_int64), _p(nbr, ctypes.c_int32),
-                                  None if w is None else w.ctypes.data_as(VP)), f"ingest {path}")
+        _check(fn(*pre, path.encode(), weight_mode, ctypes.byref(n), ctypes.byref(m),
+                  _p(end, ctypes.c_int64), _p(nbr, ctypes.c_int32),
+                  None if w is None else w.ctypes.data_as(VP)), f"ingest {path}")
         return Graph(end[:n.value], nbr[:m.value], None if w is None else w[:m.value], weight_mode)
 
 
@@ -415,6 +426,21 @@ class Engine:
 
     def load_bin(self, path, weight_mode=W_NONE):
         _check(lib().gala_load_bin(self.ctx, path.encode(), weight_mode), "gala_load_bin")
+
+    def load_text(self, path, weight_mode=W_NONE):
+        """gala_load_text: device ingest of an edge list, set as the graph."""
+        _check(lib().gala_load_text(self.ctx, path.encode(), weight_mode), "gala_load_text")
+
+    def ingest_text(self, path, weight_mode=W_NONE):
+        """gala_ingest_text_device: the edge list as a Graph, ingested on the GPU."""
+        return Graph.from_text(path, weight_mode, engine=self)
+
+    def ingest_times(self):
+        """Timings (ms) of the last device ingest: read, load (file to HBM),
+        device, d2h, wall."""
+        v = [ctypes.c_double() for _ in range(5)]
+        _check(lib().gala_ingest_times(self.ctx, *[ctypes.byref(x) for x in v]), "gala_ingest_times")
+        return dict(zip(("read_ms", "load_ms", "device_ms", "d2h_ms", "wall_ms"), (x.value for x in v)))
 
     def run(self, trace=None, rounds=None):
         """gala_run.  trace: list receiving (round, it, Q, moved, cur|None);

@@ -176,6 +176,23 @@ int gala_phase_contract(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const
                         const int32_t* w, const int32_t* comm, int32_t* out_C, int64_t* out_m,
                         int32_t* new_id, int64_t* out_end_incl, int32_t* out_nbr, int64_t* out_w);
 
+/* Text edge list -> CSR on the GPU, with the line rules and output of
+ * gala_ingest_text (host IO section below); replaces Graph::Graph(file, is_wt),
+ * graph.cpp:18-151, the step the preprocess CLI runs (preprocess.cpp:43): the file is streamed
+ * into HBM, newlines are found, lines parsed, ids renumbered, entries sorted
+ * and deduplicated there; the output equals gala_ingest_text's byte for byte.
+ * Two-call protocol: a call with NULL arrays ingests and keeps the CSR in the
+ * context; a call with arrays copies it out.  Up to 2^30 edge lines. */
+int gala_ingest_text_device(gala_ctx* ctx, const char* path, int weight_mode, int32_t* n, int64_t* m,
+                            int64_t* end_incl, int32_t* nbr, void* w);
+/* Device ingest of a text edge list, then set as the context's graph (as gala_load_bin). */
+int gala_load_text(gala_ctx* ctx, const char* path, int weight_mode);
+/* Timings of the last device ingest (any pointer may be NULL): host file read,
+ * file-to-HBM pipeline (read + copy), device passes (parse to dedup, including
+ * their allocations), copy-out, and the whole call. */
+int gala_ingest_times(gala_ctx* ctx, double* read_ms, double* load_ms, double* device_ms, double* d2h_ms,
+                      double* wall_ms);
+
 /* Modularity of the partition comm[n] (community ids in [0, n)) of a graph
  * (modularity_op, kernel_functions.cuh:77-87, as louvain_main_process.cu:
  * 538-548,990-999 evaluate it): Q = sum In / m2 - sum Tot^2 / m2^2 from exact
@@ -294,9 +311,23 @@ int gala_read_bin(const char* path, int weight_mode, int32_t* n, int64_t* m, int
 int gala_write_bin(const char* path, int32_t n, const int64_t* end_incl, const int32_t* nbr,
                    const void* w, int weight_mode);
 /* Text edge list -> CSR with the reference preprocess semantics (graph.cpp:18-151):
- * '#' comments, both directions, per-row dedup in ascending order (weights summed
- * when weighted), dense renumbering of linked vertices.  Self-loops are kept as
- * one entry (the reference truncates them).  Two-call protocol as gala_read_bin. */
+ * both directions pushed, per-row dedup in ascending order (weights summed when
+ * weighted), dense renumbering of the linked vertices in ascending original id.
+ * Self-loops are kept as one entry (the reference truncates them).
+ * Line rules (shared with gala_ingest_text_device):
+ *   - the file is split at '\n'; an empty line or one starting with '#' is skipped;
+ *   - tokens are separated by ' ', '\t', '\r', '\v', '\f';
+ *   - an edge line's first two tokens are integers [+-]?[0-9]+; a line whose
+ *     first two tokens are not both integers is skipped (a header, a '%' line);
+ *   - an id below 0 or above INT32_MAX is GALA_E_FORMAT;
+ *   - weighted: the third token is the weight (1 when absent), further tokens
+ *     are ignored.  GALA_W_I32: an int32 integer token, summed in int64 per
+ *     entry and range-checked to int32.  GALA_W_F32: a decimal
+ *     [+-]?(D+(.D*)?|.D+)([eE][+-]?D+)? of at most 40 significant digits,
+ *     rounded to the nearest double (ties to even); repeated entries are summed
+ *     in float64 in line order and rounded once to float32.  A malformed weight,
+ *     one that rounds to an infinity, or a sum out of range is GALA_E_FORMAT.
+ * Two-call protocol as gala_read_bin. */
 int gala_ingest_text(const char* path, int weight_mode, int32_t* n, int64_t* m, int64_t* end_incl,
                      int32_t* nbr, void* w);
 

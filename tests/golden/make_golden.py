@@ -9,6 +9,11 @@
                      community counts on that graph, plus the final partition
   karate_readme.json : the known-answer trajectory printed in the reference
                      README (README.md:58-89), typed in as data
+  snap_sparse.txt(.bin) : a SNAP-style edge list (seeded): '#' header lines,
+                     blank lines, ids scattered below 2^24, leading zeros and
+                     '+' signs, tab / multi-space separators, repeated and
+                     reversed edges, no self-loops (the reference truncates
+                     those); the .bin is the reference preprocess's output
 
 Run from the repository root after `make -C oracle && make -C gala-mg_amd`.
 """
@@ -45,11 +50,45 @@ def ref_preprocess(txt):
     return txt + ".bin"
 
 
+def snap_sparse_text(seed=7, lines=1500, pool=400):
+    rng = np.random.default_rng(seed)
+    ids = np.unique(rng.integers(0, 1 << 24, pool * 2))[:pool]
+    rng.shuffle(ids)
+    pick = lambda: int(ids[int(pool * rng.random() ** 2)])  # noqa: E731  (skewed toward the first ids)
+    out = ["# Directed graph (each unordered pair of nodes is saved once)",
+           "# Nodes: ? Edges: ?", "# FromNodeId\tToNodeId"]
+    seen = []
+    for i in range(lines):
+        r = rng.random()
+        if r < 0.02:
+            out.append("")
+            continue
+        if r < 0.10 and seen:
+            a, b = seen[int(rng.integers(len(seen)))]
+            if rng.random() < 0.5:
+                a, b = b, a
+        else:
+            a, b = pick(), pick()
+            while b == a:
+                b = pick()
+            seen.append((a, b))
+        fa = f"{a:07d}" if rng.random() < 0.05 else (f"+{a}" if rng.random() < 0.03 else str(a))
+        sep = ["\t", " ", "   ", " \t "][int(rng.integers(4))]
+        lead = " " if rng.random() < 0.03 else ""
+        tail = " " if rng.random() < 0.05 else ""
+        out.append(f"{lead}{fa}{sep}{b}{tail}")
+    return "\n".join(out) + "\n"
+
+
 def main():
     with open(os.path.join(HERE, "karate_readme.json"), "w") as f:
         json.dump(README_KARATE, f, indent=1)
     if os.path.exists(REF_PRE):
         ref_preprocess(os.path.join(HERE, "karate.txt"))
+        snap = os.path.join(HERE, "snap_sparse.txt")
+        with open(snap, "w") as f:
+            f.write(snap_sparse_text())
+        ref_preprocess(snap)
     g = gala.gen_sbm(1024, 16, 0.10, 0.002, 0)
     off = g.offsets()
     assert np.all(np.diff(off) > 0), "SBM fixture has an isolated vertex; pick another seed"
