@@ -257,7 +257,8 @@ def test_device_hard_decimals_bulk(engine, tmp_path):
             toks.append(f"{int(rng.integers(1, 10**15))}e{int(rng.integers(-40, 40))}")
         else:
             toks.append(HARD_DECIMALS[i % len(HARD_DECIMALS)])
-    toks = [t for t in toks if "inf" not in t and "nan" not in t and "e+3" not in t[-5:]]
+    # finite, and within float32 once summed (every pair occurs once)
+    toks = [t for t in toks if "inf" not in t and "nan" not in t and abs(float(t)) < 3e38]
     lines = [f"{i} {i + 1} {t}" for i, t in enumerate(toks)]
     path = _write(tmp_path, "dec.txt", "\n".join(lines) + "\n")
     try:
