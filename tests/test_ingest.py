@@ -50,6 +50,9 @@ def crafted_cases():
         ("weights_extra_tokens", "1 2 3 junk more\n2 3 4\n", (0, 1, 2)),
         ("hard_decimals", dec + "\n", (0, 2)),
         ("int_weights_edge", "1 2 2147483647\n3 4 -2147483648\n5 6 -7\n", (0, 1)),
+        # lines longer than a 4 KiB tile and its staged kilobyte before it
+        ("long_lines", "1 2\n" + " " * 6000 + "3 4 5\n#" + "y" * 9000 + "\n6\t" + " " * 5000 + "7 8\n" +
+         "x" * 5000 + " 9 10\n11 12", (0, 1, 2)),
     ]
 
 
