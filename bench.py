@@ -30,6 +30,9 @@ Also reported:
                 run one at a time
   weighted_rmat R-MAT scale 24 with fp32 weights (run exactly: the weights
                 scaled to integers, int64 sums, int128 gains)
+  text_ingest   the same graph as a 34.4M-line text edge list, ingested on the
+                GPU (gala_ingest_text_device) against the host ingest and the
+                reference's own preprocess on a 2M-line sample
 """
 import argparse
 import json
@@ -303,6 +306,92 @@ def rmat_leg(gala, args):
     return out
 
 
+def write_edge_list(path, g, chunk=4_000_000):
+    """The graph's undirected edges (u < v) as a SNAP-style text edge list:
+    a '#' header line, then "u<TAB>v" per line."""
+    import numpy as np
+    off = np.concatenate([[0], g.degrees])
+    src = np.repeat(np.arange(g.vertex_num, dtype=np.int64), np.diff(off))
+    dst = g.neighbors.astype(np.int64)
+    sel = src < dst
+    u, v = src[sel], dst[sel]
+    width = len(str(max(g.vertex_num - 1, 0)))
+    ids = np.arange(g.vertex_num, dtype=np.int64)
+    digits = ((ids[:, None] // (10 ** np.arange(width - 1, -1, -1, dtype=np.int64))[None, :]) % 10 + 48).astype(
+        np.uint8)
+    nd = 1 + sum((ids >= 10 ** k).astype(np.int64) for k in range(1, width))  # digits of each id
+    keep_id = np.arange(width)[None, :] >= (width - nd)[:, None]
+    with open(path, "wb") as f:
+        f.write(b"# FromNodeId\tToNodeId\n")
+        for s0 in range(0, len(u), chunk):
+            uu, vv = u[s0:s0 + chunk], v[s0:s0 + chunk]
+            n = len(uu)
+            row = np.concatenate([digits[uu], np.full((n, 1), 9, np.uint8), digits[vv],
+                                  np.full((n, 1), 10, np.uint8)], axis=1)
+            keep = np.concatenate([keep_id[uu], np.ones((n, 1), bool), keep_id[vv], np.ones((n, 1), bool)], axis=1)
+            f.write(row[keep].tobytes())
+    return len(u)
+
+
+def ingest_leg(gala, g, args):
+    """Text edge-list ingest of the same graph (one line per undirected edge):
+    gala_ingest_text_device (file read, copy to HBM, parse, renumber, sort,
+    dedup, copy-out) against the host gala_ingest_text (same output, checked
+    byte for byte) and the reference's own preprocess (graph.cpp:18-151, built
+    from its sources into oracle/_ref/) on the first 2M lines."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    tmp = tempfile.mkdtemp(prefix="gala_ingest_")
+    path = os.path.join(tmp, "edges.txt")
+    t0 = time.time()
+    lines = write_edge_list(path, g)
+    size = os.path.getsize(path)
+    log(f"ingest: {lines} lines, {size / 1e6:.1f} MB text written in {time.time() - t0:.1f}s")
+    eng = gala.Engine(device=0)
+    runs = []
+    for i in range(1 + max(args.steps, 1)):
+        d = eng.ingest_text(path)
+        runs.append(eng.ingest_times())
+    t = runs[-1]
+    dev_ms = float(np.mean([r["device_ms"] for r in runs[1:]]))
+    wall_ms = float(np.mean([r["wall_ms"] for r in runs[1:]]))
+    t2 = time.perf_counter()
+    h = gala.Graph.from_text(path)
+    host_s = time.perf_counter() - t2
+    same = bool(np.array_equal(h.degrees, d.degrees) and np.array_equal(h.neighbors, d.neighbors))
+    out = {"lines": int(lines), "text_bytes": int(size), "vertices": int(d.vertex_num),
+           "entries": int(len(d.neighbors)), "wall_ms": wall_ms, "device_ms": dev_ms,
+           "read_ms": t["read_ms"], "load_ms": t["load_ms"], "d2h_ms": t["d2h_ms"],
+           "lines_per_s": lines / (wall_ms * 1e-3), "text_GBps_wall": size / (wall_ms * 1e-3) / 1e9,
+           "text_GBps_device": size / (dev_ms * 1e-3) / 1e9, "host_ingest_s": host_s,
+           "device_equals_host": same, "reference": None}
+    ref = os.path.join(ROOT, "oracle", "_ref", "preprocess_ref")
+    if os.path.exists(ref):
+        sample = os.path.join(tmp, "sample.txt")
+        with open(path, "rb") as fi, open(sample, "wb") as fo:
+            for k, line in enumerate(fi):
+                if k > 2_000_000:
+                    break
+                fo.write(line)
+        t2 = time.perf_counter()
+        subprocess.run([ref, "-f", sample], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        ref_s = time.perf_counter() - t2
+        ds = eng.ingest_text(sample)
+        rb = gala.Graph.load_bin_graph(sample + ".bin")
+        out["reference"] = {"kind": "reference", "sample": "first 2,000,000 edge lines", "seconds": ref_s,
+                            "lines_per_s": 2_000_000 / ref_s, "cores": 1,
+                            "device_equals_reference": bool(np.array_equal(rb.degrees, ds.degrees) and
+                                                            np.array_equal(rb.neighbors, ds.neighbors))}
+    eng.close()
+    for f in os.listdir(tmp):
+        os.remove(os.path.join(tmp, f))
+    os.rmdir(tmp)
+    log(f"ingest: device {wall_ms:.1f} ms wall ({dev_ms:.1f} ms device passes), host {host_s:.2f} s, "
+        f"identical {same}; reference {out['reference']['seconds'] if out['reference'] else None}")
+    return out
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of known size: FETCH_SIZE and WRITE_SIZE of
     its dispatches against mib MiB read and written give the factor that turns
@@ -330,6 +419,7 @@ def main():
     ap.add_argument("--rmat-scale", type=int, default=24, help="scale of the weighted R-MAT leg (0: skip it)")
     ap.add_argument("--rmat-oracle", action="store_true", help="check the R-MAT leg against the FP64 oracle")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-frontier leg")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the text-ingest leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times and exit (scales the HBM byte counters)")
     args = ap.parse_args()
@@ -375,6 +465,7 @@ def main():
     learned = None if args.no_learned else learned_leg(gala, args)
     batched = batch_leg(gala, args) if args.batch_graphs > 0 else None
     rmat = rmat_leg(gala, args) if args.rmat_scale > 0 else None
+    ingest = None if args.no_ingest else ingest_leg(gala, g, args)
     # one more run with per-bin HIP events (outside the timed steps)
     peng = gala.Engine(device=0, profile_bins=True)
     peng.set_graph(g)
@@ -479,6 +570,7 @@ def main():
         "learned": learned,
         "batched": batched,
         "weighted_rmat": rmat,
+        "text_ingest": ingest,
         "dQ_learned_vs_plain": learned["dQ"] if learned else None,
     }
     print(json.dumps(out))

@@ -564,6 +564,21 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   return GALA_OK;
 }
 
+// The decide-and-move totals of the counters: the sums of their slots.
+void fold_slots(Counters* c) {
+  c->moved = c->swept = c->swept_edges = 0;
+  for (int b = 0; b < NBINS; b++) c->bin_swept[b] = c->bin_edges[b] = 0;
+  for (const DecideSlot& sl : c->slot) {
+    c->moved += sl.moved;
+    c->swept += sl.swept;
+    c->swept_edges += sl.swept_edges;
+    for (int b = 0; b < NBINS; b++) {
+      c->bin_swept[b] += sl.bin_swept[b];
+      c->bin_edges[b] += sl.bin_edges[b];
+    }
+  }
+}
+
 // Reads ctr[0] into out and, when saved is given, the copy kept in ctr[1] as
 // well, in one transfer and one synchronisation.
 template <class T>
@@ -572,8 +587,10 @@ int read_counters(gala_ctx* ctx, Counters* out, Counters* saved = nullptr) {
   HIPOK(hipMemcpyAsync(ctx->ctr_h, ctx->ctr, k * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
   HIPOK(hipStreamSynchronize(ctx->stream));
   *out = ctx->ctr_h[0];
+  fold_slots(out);
   if (saved) {
     *saved = ctx->ctr_h[1];
+    fold_slots(saved);
     if (saved->overflow) return GALA_E_OVERFLOW;
   }
   if (out->overflow) return GALA_E_OVERFLOW;

@@ -49,6 +49,16 @@ constexpr int MLP_W1 = 0, MLP_B1 = 16 * MLP_H, MLP_W2 = MLP_B1 + MLP_H, MLP_B2 =
 constexpr int MLP_NPARAM = MLP_B2 + 1;
 
 // Per-iteration device counters (one small struct read back per iteration).
+// Decide-and-move counts, spread over CTR_SLOTS slots of 256 bytes: the
+// workgroups of a launch add to slot (workgroup id % CTR_SLOTS), so their
+// end-of-launch atomics land on different L2 lines instead of queueing on one.
+constexpr int CTR_SLOTS = 32;
+struct alignas(256) DecideSlot {
+  unsigned long long moved, swept, swept_edges, pad;
+  unsigned long long bin_swept[NBINS];
+  unsigned long long bin_edges[NBINS];
+};
+
 struct Counters {
   unsigned long long sum_in;     // sum over v of In[v]            (exact, <= m2 < 2^62)
   unsigned long long sum_tot2;   // sum over c of Tot[c]^2, low 64 bits (exact)
@@ -62,6 +72,9 @@ struct Counters {
   int overflow;                  // hash-table overflow flag (must stay 0)
   unsigned long long bin_swept[NBINS];  // decide-and-move per degree bin
   unsigned long long bin_edges[NBINS];
+  // written by the decide kernels; moved / swept / swept_edges / bin_* above
+  // are their sums, formed on the host when the counters are read
+  DecideSlot slot[CTR_SLOTS];
 };
 
 // Per-workgroup partial results of the O(V) reductions (k_apply, k_prune).

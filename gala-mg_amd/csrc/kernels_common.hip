@@ -400,9 +400,11 @@ __global__ void k_reset_counters(Counters* c, long long min_init) {
     c->swept_edges = 0; c->active = 0; c->nonempty = 0; c->overflow = 0;
   }
   if (threadIdx.x < NBINS) { c->bin_swept[threadIdx.x] = 0; c->bin_edges[threadIdx.x] = 0; }
+  unsigned long long* sl = (unsigned long long*)c->slot;
+  for (int i = threadIdx.x; i < (int)(sizeof(c->slot) / 8); i += blockDim.x) sl[i] = 0;
 }
 void reset_counters(Counters* c, long long min_init, hipStream_t s) {
-  k_reset_counters<<<1, 64, 0, s>>>(c, min_init);
+  k_reset_counters<<<1, 256, 0, s>>>(c, min_init);
 }
 
 }  // namespace gala

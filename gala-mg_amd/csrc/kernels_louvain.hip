@@ -138,6 +138,23 @@ __device__ __forceinline__ void atomic_add_ull(unsigned long long* p, V x) {
   if (x) atomicAdd(p, (unsigned long long)x);
 }
 
+// Decide-and-move counts of one workgroup, added to slot blockIdx.x % CTR_SLOTS
+// of the counters: thousands of workgroups adding to one address serialise in
+// L2 at the end of every launch; the host sums the slots when it reads them.
+template <class A, class B, class C>
+__device__ __forceinline__ void add_decide_counts(Counters* c, int bin, A moved, B swept, C edges) {
+  DecideSlot& sl = c->slot[blockIdx.x % CTR_SLOTS];
+  if (moved) atomic_add_ull(&sl.moved, moved);
+  if (swept) {
+    atomic_add_ull(&sl.swept, swept);
+    atomic_add_ull(&sl.bin_swept[bin], swept);
+  }
+  if (edges) {
+    atomic_add_ull(&sl.swept_edges, edges);
+    atomic_add_ull(&sl.bin_edges[bin], edges);
+  }
+}
+
 // m2 of the vertex's graph: the batch's per-graph m2 when graphs are batched.
 template <class T>
 __device__ __forceinline__ typename T::tot_t vertex_m2(const DevState<T>& st, int v, typename T::tot_t m2_all) {
@@ -388,11 +405,7 @@ __global__ __launch_bounds__(256) void k_decide_ring(DevGraph<T> g, DevState<T> 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, s_moved);
-    atomic_add_ull(&st.ctr->swept, s_swept);
-    atomic_add_ull(&st.ctr->swept_edges, s_edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+    add_decide_counts(st.ctr, bin, s_moved, s_swept, s_edges);
   }
 }
 
@@ -527,11 +540,7 @@ __global__ __launch_bounds__(256) void k_decide_pipe(DevGraph<T> g, DevState<T> 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, s_moved);
-    atomic_add_ull(&st.ctr->swept, s_swept);
-    atomic_add_ull(&st.ctr->swept_edges, s_edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+    add_decide_counts(st.ctr, bin, s_moved, s_swept, s_edges);
   }
 }
 
@@ -656,11 +665,7 @@ __global__ __launch_bounds__(256) void k_decide_tpv(DevGraph<T> g, DevState<T> s
   if (edges) atomicAdd(&s_edges, (unsigned long long)edges);
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, s_moved);
-    atomic_add_ull(&st.ctr->swept, s_swept);
-    atomic_add_ull(&st.ctr->swept_edges, s_edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+    add_decide_counts(st.ctr, bin, s_moved, s_swept, s_edges);
   }
 }
 
@@ -822,11 +827,7 @@ __global__ __launch_bounds__(256) void k_decide_hpipe(DevGraph<T> g, DevState<T>
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, s_moved);
-    atomic_add_ull(&st.ctr->swept, s_swept);
-    atomic_add_ull(&st.ctr->swept_edges, s_edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+    add_decide_counts(st.ctr, bin, s_moved, s_swept, s_edges);
   }
 }
 
@@ -1015,11 +1016,7 @@ __global__ __launch_bounds__(256) void k_decide_hlead(DevGraph<T> g, DevState<T>
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, s_moved);
-    atomic_add_ull(&st.ctr->swept, s_swept);
-    atomic_add_ull(&st.ctr->swept_edges, s_edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+    add_decide_counts(st.ctr, bin, s_moved, s_swept, s_edges);
   }
 }
 
@@ -1164,11 +1161,7 @@ __global__ __launch_bounds__(256) void k_decide_glead(DevGraph<T> g, DevState<T>
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, s_moved);
-    atomic_add_ull(&st.ctr->swept, s_swept);
-    atomic_add_ull(&st.ctr->swept_edges, s_edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+    add_decide_counts(st.ctr, bin, s_moved, s_swept, s_edges);
   }
 }
 
@@ -1282,11 +1275,7 @@ __global__ __launch_bounds__(256) void k_decide_wave(DevGraph<T> g, DevState<T> 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, s_moved);
-    atomic_add_ull(&st.ctr->swept, s_swept);
-    atomic_add_ull(&st.ctr->swept_edges, s_edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+    add_decide_counts(st.ctr, bin, s_moved, s_swept, s_edges);
   }
 }
 
@@ -1399,11 +1388,7 @@ __global__ __launch_bounds__(NT) void k_decide_block(DevGraph<T> g, DevState<T> 
     if (st.dirty && s_mv) mark_row(st.dirty, g.nbr, o, d, v, threadIdx.x, NT);
   }
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, moved);
-    atomic_add_ull(&st.ctr->swept, swept);
-    atomic_add_ull(&st.ctr->swept_edges, edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], edges);
+    add_decide_counts(st.ctr, bin, moved, swept, edges);
   }
 }
 
@@ -1541,11 +1526,7 @@ __global__ __launch_bounds__(NT) void k_decide_blead(DevGraph<T> g, DevState<T> 
     }
   }
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, moved);
-    atomic_add_ull(&st.ctr->swept, swept);
-    atomic_add_ull(&st.ctr->swept_edges, edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], edges);
+    add_decide_counts(st.ctr, bin, moved, swept, edges);
   }
 }
 
@@ -1685,11 +1666,7 @@ __global__ __launch_bounds__(1024) void k_decide_hub(DevGraph<T> g, DevState<T> 
     if (st.dirty && s_mv) mark_row(st.dirty, g.nbr, o, d, v, threadIdx.x, NT);
   }
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, moved);
-    atomic_add_ull(&st.ctr->swept, swept);
-    atomic_add_ull(&st.ctr->swept_edges, edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], edges);
+    add_decide_counts(st.ctr, bin, moved, swept, edges);
   }
 }
 
@@ -2105,11 +2082,7 @@ __global__ __launch_bounds__(256) void k_hub_commit(DevGraph<T> g, DevState<T> s
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, moved);
-    atomic_add_ull(&st.ctr->swept, swept);
-    atomic_add_ull(&st.ctr->swept_edges, edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], edges);
+    add_decide_counts(st.ctr, bin, moved, swept, edges);
   }
 }
 
@@ -2166,11 +2139,7 @@ __global__ __launch_bounds__(256) void k_decide_first(DevGraph<T> g, DevState<T>
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, s_moved);
-    atomic_add_ull(&st.ctr->swept, s_swept);
-    atomic_add_ull(&st.ctr->swept_edges, s_edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], s_swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], s_edges);
+    add_decide_counts(st.ctr, bin, s_moved, s_swept, s_edges);
   }
 }
 
@@ -2221,11 +2190,7 @@ __global__ __launch_bounds__(NT) void k_decide_first_block(DevGraph<T> g, DevSta
     if (st.dirty && s_mv) mark_row(st.dirty, g.nbr, o, d, v, threadIdx.x, NT);
   }
   if (threadIdx.x == 0) {
-    atomic_add_ull(&st.ctr->moved, moved);
-    atomic_add_ull(&st.ctr->swept, swept);
-    atomic_add_ull(&st.ctr->swept_edges, edges);
-    atomic_add_ull(&st.ctr->bin_swept[bin], swept);
-    atomic_add_ull(&st.ctr->bin_edges[bin], edges);
+    add_decide_counts(st.ctr, bin, moved, swept, edges);
   }
 }
 
