@@ -290,7 +290,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool wide) {
   OWN(ctx->c_ecnt, N + 1); OWN(ctx->c_eoff, N + 1); OWN(ctx->c_ecur, N);
   OWN(ctx->c_big_tab, N + 1); OWN(ctx->c_i64_tmp, 2 * (N + 1) + scan_tmp_elems(N));
   OWN(ctx->c_i64_scalar, 16);
-  OWN(ctx->c_mtotal, 1);
+  OWN(ctx->c_mtotal, 16 * MTOTAL_SLOTS);
   OWN(ctx->c_voff, N);
   if (!ctx->ctr_h) {
     hipError_t e = hipHostMalloc((void**)&ctx->ctr_h, 2 * sizeof(Counters), hipHostMallocDefault);
@@ -466,11 +466,19 @@ int f32_exact_scale(const float* w, int64_t m, int32_t* shift, int64_t* m2, bool
 __global__ void k_iota(int32_t* __restrict__ p, int32_t n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
 }
-__global__ void k_sum_w64(const int32_t* __restrict__ w, int64_t m, Counters* c) {
+// c->sum_in += sum of w: one atomic per workgroup of 256
+__global__ __launch_bounds__(256) void k_sum_w64(const int32_t* __restrict__ w, int64_t m, Counters* c) {
+  __shared__ long long s_w[4];
   long long s = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
     s += w[i];
-  if (s) atomicAdd(&c->sum_in, (unsigned long long)s);
+  for (int k = 32; k >= 1; k >>= 1) s += __shfl_xor(s, k, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    if (s) atomicAdd(&c->sum_in, (unsigned long long)s);
+  }
 }
 
 struct RoundOut {
@@ -841,7 +849,7 @@ ContractBufs<T> cbufs(gala_ctx* ctx) {
   cb.tier = ctx->c_tier; cb.tlist = ctx->c_tlist; cb.tstart = ctx->c_tstart; cb.part_tmp = ctx->part_tmp;
   cb.i64_scalar = ctx->c_i64_scalar;
   cb.hpin = ctx->hpin;
-  cb.hpin_bytes = HPIN_BYTES - 256;  // the last 256 bytes: the round's entry-count readback
+  cb.hpin_bytes = HPIN_BYTES - 8 * 16 * MTOTAL_SLOTS;  // the tail: the round's entry-count slots
   cb.mtotal = ctx->c_mtotal;
   cb.voff = ctx->c_voff;
   return cb;
@@ -945,10 +953,15 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
     if (rc) return rc;
     compose_map(ctx->fin, ctx->n0, ctx->c_label, s);
     HIPOK(hipEventRecord(ctx->ev[7], s));
-    int64_t* m_rd = (int64_t*)(ctx->hpin + HPIN_BYTES - 64);  // entries of the contracted graph
-    if (!canonical) HIPOK(hipMemcpyAsync(m_rd, ctx->c_mtotal, 8, hipMemcpyDeviceToHost, s));
+    // entries of the contracted graph: the sum of the slots of c_mtotal
+    int64_t* m_rd = (int64_t*)(ctx->hpin + HPIN_BYTES - 8 * 16 * MTOTAL_SLOTS);
+    if (!canonical)
+      HIPOK(hipMemcpyAsync(m_rd, ctx->c_mtotal, 8 * 16 * MTOTAL_SLOTS, hipMemcpyDeviceToHost, s));
     HIPOK(hipStreamSynchronize(s));
-    const int64_t m_out = canonical ? m : *m_rd;
+    int64_t m_slots = 0;
+    if (!canonical)
+      for (int k = 0; k < MTOTAL_SLOTS; k++) m_slots += m_rd[16 * k];
+    const int64_t m_out = canonical ? m : m_slots;
     if (round < 64) {
       R.round_vertices[round] = g.n;
       R.round_entries[round] = m_in;

@@ -53,6 +53,7 @@ constexpr int MLP_NPARAM = MLP_B2 + 1;
 // workgroups of a launch add to slot (workgroup id % CTR_SLOTS), so their
 // end-of-launch atomics land on different L2 lines instead of queueing on one.
 constexpr int CTR_SLOTS = 32;
+constexpr int MTOTAL_SLOTS = 32;  // slots of the contracted-entry count (one 128-byte line each)
 struct alignas(256) DecideSlot {
   unsigned long long moved, swept, swept_edges, pad;
   unsigned long long bin_swept[NBINS];

@@ -293,7 +293,7 @@ struct ContractBufs {
   // lists that do not fit go through pageable copies
   char* hpin;
   int64_t hpin_bytes;
-  int64_t* mtotal;  // [1] entries of a gapped contracted graph
+  int64_t* mtotal;  // [16 * MTOTAL_SLOTS] entries of a gapped contracted graph: the sum of mtotal[16 k]
   int64_t* voff;    // [n] offset of every vertex's row inside its community's staging segment
 };
 

@@ -57,29 +57,40 @@ __global__ __launch_bounds__(256) void k_graph_sums(const int32_t* __restrict__ 
 
 // One thread per graph: Q of the sweep just done, then the plain rule.
 // it == 0 is the singleton partition of the round (Q becomes the reference).
+// *p += the number of lanes of the wave with flag set; every lane calls it
+// (one atomic per wave: thousands of per-graph atomics on one counter queue
+// behind each other in L2)
+__device__ __forceinline__ void wave_count_add(int32_t* p, bool flag) {
+  const unsigned long long m = __ballot(flag);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(p, (int32_t)__popcll(m));
+}
+
 __global__ void k_batch_step(BatchDev b, const int32_t* __restrict__ it_dev, double min_mod, int max_it) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= b.G) return;
-  const int it = *it_dev;
-  const unsigned long long si = b.g_in[g], st2 = b.g_tot2[g];
-  b.g_in[g] = 0;
-  b.g_tot2[g] = 0;
-  b.accept[g] = 0;
-  if (!b.live[g]) return;
-  const double q = q_exact_dev(si, st2, (double)b.m2g[g]);
-  if (it == 0) {
+  // returns whether graph g iterates on (counted in *b.nlive)
+  auto step = [&]() -> bool {
+    if (g >= b.G) return false;
+    const int it = *it_dev;
+    const unsigned long long si = b.g_in[g], st2 = b.g_tot2[g];
+    b.g_in[g] = 0;
+    b.g_tot2[g] = 0;
+    b.accept[g] = 0;
+    if (!b.live[g]) return false;
+    const double q = q_exact_dev(si, st2, (double)b.m2g[g]);
+    if (it == 0) {
+      b.qprev[g] = q;
+      return true;
+    }
+    b.iters[g] = it;
+    if (__dsub_rn(q, b.qprev[g]) < min_mod || it >= max_it) {
+      b.live[g] = 0;  // revert: prev keeps the previous assignment
+      return false;
+    }
     b.qprev[g] = q;
-    atomicAdd(b.nlive, 1);
-    return;
-  }
-  b.iters[g] = it;
-  if (__dsub_rn(q, b.qprev[g]) < min_mod || it >= max_it) {
-    b.live[g] = 0;  // revert: prev keeps the previous assignment
-    return;
-  }
-  b.qprev[g] = q;
-  b.accept[g] = 1;
-  atomicAdd(b.nlive, 1);
+    b.accept[g] = 1;
+    return true;
+  };
+  wave_count_add(b.nlive, step());
 }
 
 // prev = cur for the graphs whose sweep was accepted; vertices of graphs that
@@ -125,22 +136,26 @@ __global__ void k_batch_live(BatchDev b, int32_t* __restrict__ it_dev) {
 // of the round, the round-level stop test, totals.
 __global__ void k_batch_round(BatchDev b, int round, double min_mod, int max_rounds, int32_t* __restrict__ nopen) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= b.G || b.done[g]) return;
-  if (!(b.m2g[g] > 0)) {  // no edge: every vertex stays alone, Q = 0
-    b.done[g] = 1;
-    b.cur_mod[g] = 0.0;
-    return;
-  }
-  const double cm = b.qprev[g];
-  const double pm = round > 0 ? b.cur_mod[g] : -1.0;
-  b.cur_mod[g] = cm;
-  b.rounds[g] = round + 1;
-  b.tot_iters[g] += b.iters[g];
-  if (!(__dsub_rn(cm, pm) >= min_mod) || round + 1 >= max_rounds) {
-    b.done[g] = 1;
-  } else {
-    atomicAdd(nopen, 1);
-  }
+  // returns whether graph g goes on to another round (counted in *nopen)
+  auto end_round = [&]() -> bool {
+    if (g >= b.G || b.done[g]) return false;
+    if (!(b.m2g[g] > 0)) {  // no edge: every vertex stays alone, Q = 0
+      b.done[g] = 1;
+      b.cur_mod[g] = 0.0;
+      return false;
+    }
+    const double cm = b.qprev[g];
+    const double pm = round > 0 ? b.cur_mod[g] : -1.0;
+    b.cur_mod[g] = cm;
+    b.rounds[g] = round + 1;
+    b.tot_iters[g] += b.iters[g];
+    if (!(__dsub_rn(cm, pm) >= min_mod) || round + 1 >= max_rounds) {
+      b.done[g] = 1;
+      return false;
+    }
+    return true;
+  };
+  wave_count_add(nopen, end_round());
 }
 
 // Graph of every vertex of the contracted union: new id nid[c] of each
@@ -155,8 +170,17 @@ __global__ __launch_bounds__(256) void k_batch_gid_next(const int32_t* __restric
 // Communities of the final union graph per graph (gid_final is non-decreasing).
 __global__ __launch_bounds__(256) void k_batch_count(const int32_t* __restrict__ gid, int32_t n,
                                                       int32_t* __restrict__ per_graph) {
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-    atomicAdd(&per_graph[gid[v]], 1);
+  // graphs own contiguous vertex ranges: a wave mostly adds to one graph
+  for (int v0 = blockIdx.x * blockDim.x; v0 < n; v0 += gridDim.x * blockDim.x) {
+    const int v = v0 + threadIdx.x;
+    const int key = v < n ? gid[v] : -1;
+    const int k0 = __shfl(key, 0, 64);
+    if (__all(key == k0)) {
+      if ((threadIdx.x & 63) == 0 && k0 >= 0) atomicAdd(&per_graph[k0], 64);
+    } else if (key >= 0) {
+      atomicAdd(&per_graph[key], 1);
+    }
+  }
 }
 
 // Per-iteration counters of the union (the decide counters keep accumulating).

@@ -109,18 +109,26 @@ __global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ c
   }
 }
 
-// row ends of a gapped contracted graph: end[c] = off[c] + distinct count;
-// *total (zero on entry) receives the number of entries of the new graph
+// row ends of a gapped contracted graph: end[c] = off[c] + distinct count.
+// The number of entries of the new graph is the sum of the MTOTAL_SLOTS
+// slots of total (zero on entry; slot k at total[16 k], one L2 line each),
+// workgroup b adding its sum to slot b % MTOTAL_SLOTS.
 __global__ __launch_bounds__(256) void k_row_ends(const int64_t* __restrict__ off, const int32_t* __restrict__ dcnt,
                                                    int32_t C, int64_t* __restrict__ end,
                                                    unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long s_t[4];
   unsigned long long t = 0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
     end[c] = off[c] + dcnt[c];
     t += (unsigned long long)dcnt[c];
   }
   t = wave_sum(t);
-  if ((threadIdx.x & 63) == 0 && t) atomicAdd(total, t);
+  if ((threadIdx.x & 63) == 0) s_t[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    t = s_t[0] + s_t[1] + s_t[2] + s_t[3];
+    if (t) atomicAdd(&total[16 * (blockIdx.x % MTOTAL_SLOTS)], t);
+  }
 }
 
 // entries of every new community: ecnt[nid[c]] = row-length sum of c
@@ -1109,7 +1117,7 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     if (hipStreamSynchronize(s) != hipSuccess) return -1;  // the host tile vector is in flight until here
   }
   if (out.end) {  // gapped: every row stays at the start of its staging segment
-    fill<int64_t>(cb.mtotal, 1, 0, s);
+    fill<int64_t>(cb.mtotal, 16 * MTOTAL_SLOTS, 0, s);
     k_row_ends<<<nblk(C), 256, 0, s>>>(cb.eoff, cb.dcnt, C, out.end, (unsigned long long*)cb.mtotal);
     return 0;
   }
