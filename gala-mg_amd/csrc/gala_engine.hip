@@ -178,6 +178,10 @@ struct gala_ctx {
   // (variant 2); the dirty-row path (tracked runs) reads variant 2 as one
   // thread per vertex for bins 0-3
   int wvar[NBINS] = {2, 2, 2, 2, 2, 2, 0, 0, 0};
+  // degree-bin cut points (lowest degree of every bin); gala_autotune may
+  // lower those of bins 4..8
+  BinCuts cuts = {{BIN_LO[0], BIN_LO[1], BIN_LO[2], BIN_LO[3], BIN_LO[4], BIN_LO[5], BIN_LO[6], BIN_LO[7],
+                   BIN_LO[8]}};
 
   std::vector<void*> owned;
   int32_t cap_n = -1;  // sizes the owned buffers were allocated for
@@ -512,6 +516,7 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.K = (tt*)ctx->K; st.Self = (tt*)ctx->Self; st.Tot = (tt*)ctx->Tot; st.In = (tt*)ctx->In;
   st.size = ctx->size; st.prev = ctx->prev; st.cur = ctx->cur;
   st.active = ctx->active; st.dirty = ctx->dirty; st.binid = ctx->binid; st.hub_off = ctx->hub_off;
+  st.cuts = ctx->cuts;
   st.hub_keys = ctx->hub_keys; st.hub_vals = (typename T::acc_t*)ctx->hub_vals;
   st.hub_rc = ctx->hub_rc; st.hub_rw = (typename T::acc_t*)ctx->hub_rw;
   st.hub_plan = ctx->hub_plan; st.hub_part = ctx->hub_part; st.hub_fail = ctx->hub_fail;
@@ -1564,6 +1569,69 @@ int gala_variant_counts(int32_t* counts_out) {
   return GALA_OK;
 }
 
+int gala_set_bin_cuts(gala_ctx* ctx, const int32_t* lo) {
+  if (!ctx || !lo || !bin_cuts_valid(lo)) return GALA_E_INVALID;
+  for (int b = 0; b < NBINS; b++) ctx->cuts.lo[b] = lo[b];
+  ctx->have_result = false;
+  return GALA_OK;
+}
+
+int gala_get_bin_cuts(gala_ctx* ctx, int32_t* lo) {
+  if (!ctx || !lo) return GALA_E_INVALID;
+  for (int b = 0; b < NBINS; b++) lo[b] = ctx->cuts.lo[b];
+  return GALA_OK;
+}
+
+// Cut points by coordinate descent: for bins 8 down to 4, the lower cut is
+// tried at 1, 7/8, 3/4, 5/8 and 1/2 of its default (never at or below the
+// previous bin's default cut) and the one with the least decide + update time
+// over the best of two timed plain runs is kept.  Binning changes which kernel
+// evaluates a vertex, never the decision, so every run is identical but for time.
+int tune_bin_cuts(gala_ctx* ctx) {
+  const gala_opts saved = ctx->opts;
+  ctx->opts.profile_bins = 0;
+  ctx->opts.frontier_mode = 0;
+  auto timed = [&](double* ms) -> int {
+    gala_result r;
+    double best = 1e300;
+    for (int rep = 0; rep < 3; rep++) {  // the first run warms up
+      int rc = is_wide(ctx) ? run_all<WideW>(ctx, nullptr, nullptr, nullptr, &r)
+                            : run_all<IntW>(ctx, nullptr, nullptr, nullptr, &r);
+      if (rc) return rc;
+      if (rep > 0) best = std::min(best, r.t_decide_ms + r.t_update_ms);
+    }
+    *ms = best;
+    return GALA_OK;
+  };
+  double cur_ms = 0;
+  int rc = timed(&cur_ms);
+  for (int b = NBINS - 1; b >= BIN_TUNE_FIRST && rc == GALA_OK; b--) {
+    const int keep = ctx->cuts.lo[b];
+    int best_lo = keep;
+    for (int num : {8, 7, 6, 5, 4}) {
+      const int lo = std::max(BIN_LO[b - 1] + 1, BIN_LO[b] * num / 8);
+      if (lo == keep) continue;
+      ctx->cuts.lo[b] = lo;
+      double ms = 0;
+      if ((rc = timed(&ms)) != GALA_OK) break;
+      if (ms < cur_ms) { cur_ms = ms; best_lo = lo; }
+    }
+    ctx->cuts.lo[b] = best_lo;
+  }
+  ctx->opts = saved;
+  ctx->have_result = false;
+  return rc;
+}
+
+int gala_autotune_bins(gala_ctx* ctx, int32_t* lo_out) {
+  if (!ctx || !ctx->have_graph || ctx->n0 == 0) return GALA_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  int rc = tune_bin_cuts(ctx);
+  if (rc) return rc;
+  if (lo_out) gala_get_bin_cuts(ctx, lo_out);
+  return GALA_OK;
+}
+
 int gala_autotune(gala_ctx* ctx, int32_t* variants_out) {
   if (!ctx || !ctx->have_graph || ctx->n0 == 0) return GALA_E_INVALID;
   (void)hipSetDevice(ctx->device);
@@ -1605,6 +1673,7 @@ int gala_autotune(gala_ctx* ctx, int32_t* variants_out) {
   }
   ctx->have_result = false;
   if (rc) return rc;
+  if ((rc = tune_bin_cuts(ctx)) != GALA_OK) return rc;  // then the cut points, with those kernels
   if (variants_out) gala_get_variants(ctx, variants_out);
   return GALA_OK;
 }

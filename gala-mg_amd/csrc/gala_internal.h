@@ -23,17 +23,31 @@ namespace gala {
 constexpr int NBINS = 9;
 constexpr int BIN_LO[NBINS] = {1, 4, 8, 16, 32, 128, 512, 4096, 8192};
 
-__host__ __device__ inline int bin_of_degree(int64_t d) {
+// Lowest degree of every bin.  The autotuner may lower the cut points of bins
+// 4..8 (never raise them, never below the previous default cut + 1): every
+// kernel of a bin then receives rows no longer than its default range allows
+// (the LDS tables and per-thread row buffers are sized for that maximum) and
+// handles shorter rows like any others; bins 0..3 keep their cuts, so rows
+// below 8 entries still get K and Self from the init kernel.
+struct BinCuts {
+  int lo[NBINS];
+};
+constexpr int BIN_TUNE_FIRST = 4;  // first bin whose lower cut may move
+
+__host__ __device__ inline int bin_of_degree(int64_t d, const BinCuts& c) {
   if (d <= 0) return -1;
-  if (d < 4) return 0;
-  if (d < 8) return 1;
-  if (d < 16) return 2;
-  if (d < 32) return 3;
-  if (d < 128) return 4;
-  if (d < 512) return 5;
-  if (d < 4096) return 6;
-  if (d < 8192) return 7;
-  return 8;
+  int b = NBINS - 1;
+  while (b > 0 && d < c.lo[b]) b--;
+  return b;
+}
+
+// cuts valid: bins 0..3 at their defaults; bin b >= 4 in (BIN_LO[b-1], BIN_LO[b]]
+inline bool bin_cuts_valid(const int32_t* lo) {
+  for (int b = 0; b < NBINS; b++) {
+    if (b < BIN_TUNE_FIRST && lo[b] != BIN_LO[b]) return false;
+    if (b >= BIN_TUNE_FIRST && (lo[b] <= BIN_LO[b - 1] || lo[b] > BIN_LO[b])) return false;
+  }
+  return true;
 }
 
 // First bin that uses the global hash arena (tables of 8-byte sums do not fit

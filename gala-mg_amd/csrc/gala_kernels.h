@@ -87,6 +87,8 @@ struct DevState {
   uint8_t* active;
   uint8_t* dirty;    // In must be recomputed (the vertex or a neighbour moved)
   int8_t* binid;
+  BinCuts cuts = {{BIN_LO[0], BIN_LO[1], BIN_LO[2], BIN_LO[3], BIN_LO[4], BIN_LO[5], BIN_LO[6], BIN_LO[7],
+                   BIN_LO[8]}};  // degree bin cut points (binid)
   int64_t* hub_off;  // per-vertex offset into the global hash arena (bin 8), in slots
   int32_t* hub_keys;
   typename T::acc_t* hub_vals;

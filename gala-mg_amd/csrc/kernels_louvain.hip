@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += gridDim.x * blockDim.x) {
     int64_t o = g.off[v], e = g.end[v];
     int64_t d = e - o;
-    st.binid[v] = (int8_t)bin_of_degree(d);
+    st.binid[v] = (int8_t)bin_of_degree(d, st.cuts);
     st.size[v] = 1;
     st.prev[v] = v;
     st.cur[v] = v;

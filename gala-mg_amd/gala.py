@@ -28,7 +28,9 @@ EXPORTS = [
     "gala_batch_communities", "gala_gen_sbm_batch", "gala_set_variants", "gala_get_variants", "gala_autotune",
     "gala_variant_counts", "gala_phase_modularity", "gala_modularity_noalloc",
     "gala_ingest_text_device", "gala_load_text", "gala_ingest_times",
+    "gala_set_bin_cuts", "gala_get_bin_cuts", "gala_autotune_bins",
 ]
+BIN_LO_DEFAULT = (1, 4, 8, 16, 32, 128, 512, 4096, 8192)
 NOALLOC_MAX_N = 1 << 22  # GALA_NOALLOC_MAX_N
 N_VARIANT_SLOTS = 27  # kernel variant per bin: 9 general sweeps, 9 first sweeps, 9 weight updates
 
@@ -204,6 +206,9 @@ def lib():
         L.gala_set_variants.argtypes = [VP, I32P]
         L.gala_get_variants.argtypes = [VP, I32P]
         L.gala_autotune.argtypes = [VP, I32P]
+        L.gala_set_bin_cuts.argtypes = [VP, I32P]
+        L.gala_get_bin_cuts.argtypes = [VP, I32P]
+        L.gala_autotune_bins.argtypes = [VP, I32P]
         L.gala_variant_counts.argtypes = [I32P]
         _lib = L
     return _lib
@@ -484,10 +489,27 @@ class Engine:
         return v
 
     def autotune(self):
-        """Times every decide variant on the loaded graph; keeps the fastest per bin."""
+        """Times every decide variant on the loaded graph and keeps the fastest
+        per bin, then tunes the degree-bin cut points (get_bin_cuts)."""
         v = np.zeros(N_VARIANT_SLOTS, np.int32)
         _check(lib().gala_autotune(self.ctx, _p(v, ctypes.c_int32)), "gala_autotune")
         return v
+
+    def set_bin_cuts(self, lo):
+        lo = np.ascontiguousarray(lo, np.int32)
+        assert lo.size == 9
+        _check(lib().gala_set_bin_cuts(self.ctx, _p(lo, ctypes.c_int32)), "gala_set_bin_cuts")
+
+    def get_bin_cuts(self):
+        lo = np.zeros(9, np.int32)
+        _check(lib().gala_get_bin_cuts(self.ctx, _p(lo, ctypes.c_int32)), "gala_get_bin_cuts")
+        return lo
+
+    def autotune_bins(self):
+        """gala_autotune_bins: the degree-bin cut points, tuned on the loaded graph."""
+        lo = np.zeros(9, np.int32)
+        _check(lib().gala_autotune_bins(self.ctx, _p(lo, ctypes.c_int32)), "gala_autotune_bins")
+        return lo
 
     # ---- batched graphs ----
     def batch_set(self, vstart, g):

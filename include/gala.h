@@ -261,7 +261,21 @@ int gala_mlp_grad(gala_ctx* ctx, const float* params, const float* X, const uint
 #define GALA_NVARIANTS 27
 int gala_set_variants(gala_ctx* ctx, const int32_t* variants);
 int gala_get_variants(gala_ctx* ctx, int32_t* variants);
+/* Chooses the variants, then the degree-bin cut points (gala_autotune_bins). */
 int gala_autotune(gala_ctx* ctx, int32_t* variants_out);
+/* Degree-bin cut points: lo[b] is the lowest degree of bin b (GALA_NBINS
+ * entries; defaults 1, 4, 8, 16, 32, 128, 512, 4096, 8192 -- the reference
+ * cuts its bins at fixed degrees, louvain_main_process.cu:276-288).  Bins 0-3
+ * keep their cuts; the cut of bin b >= 4 may be lowered down to one above the
+ * default cut of bin b - 1 (each kernel's tables are sized for its bin's
+ * default range, so no bin may receive longer rows): GALA_E_INVALID
+ * otherwise.  gala_autotune_bins picks them by timed plain runs on the loaded
+ * graph (coordinate descent, bins 8 down to 4, cuts at 1, 7/8 .. 1/2 of the
+ * default); results are unchanged by construction. */
+#define GALA_NBINS 9
+int gala_set_bin_cuts(gala_ctx* ctx, const int32_t* lo);
+int gala_get_bin_cuts(gala_ctx* ctx, int32_t* lo);
+int gala_autotune_bins(gala_ctx* ctx, int32_t* lo_out);
 /* Number of kernel variants of every slot (GALA_NVARIANTS entries, same order
  * as gala_set_variants): a valid variant of slot i is 0 .. counts[i]-1.  Needs
  * no device and no context. */

@@ -125,9 +125,57 @@ def test_autotune_keeps_results():
     counts = gala.variant_counts()
     assert all(0 <= v[i] < counts[i] for i in range(len(v)))
     assert np.array_equal(eng.get_variants(), v)
+    lo = eng.get_bin_cuts()
+    d = gala.BIN_LO_DEFAULT
+    assert all(lo[b] == d[b] for b in range(4)) and all(d[b - 1] < lo[b] <= d[b] for b in range(4, 9)), lo
     r1 = eng.run()
     assert r1["modularity"] == r0["modularity"] and np.array_equal(eng.communities(g.vertex_num), c0)
     eng.close()
+
+
+def lowest_cuts():
+    d = gala.BIN_LO_DEFAULT
+    return [d[b] if b < 4 else d[b - 1] + 1 for b in range(9)]
+
+
+def test_bin_cuts_validation(engine):
+    say("bin cut validation")
+    d = list(gala.BIN_LO_DEFAULT)
+    assert list(engine.get_bin_cuts()) == d
+    for bad in ([0] + d[1:], d[:3] + [15] + d[4:], d[:4] + [16] + d[5:], d[:8] + [8193],
+                d[:5] + [32] + d[6:], d[:8] + [4096]):
+        with pytest.raises(gala.GalaError):
+            engine.set_bin_cuts(bad)
+    engine.set_bin_cuts(lowest_cuts())
+    assert list(engine.get_bin_cuts()) == lowest_cuts()
+    engine.set_bin_cuts(d)
+
+
+@pytest.mark.parametrize("cuts", ["lowest", "mixed"])
+def test_bin_cuts_match_oracle(cuts):
+    # every kernel gets shorter rows than its default range when the cuts are
+    # lowered: the trajectory must not change
+    say(f"bin cuts {cuts}")
+    g = gala.gen_chung_lu(200000, 2.1, 17.35, 12000.0, 0)
+    eng = gala.Engine(dump_iterations=True)
+    lo = lowest_cuts() if cuts == "lowest" else [1, 4, 8, 16, 20, 100, 300, 3000, 6000]
+    eng.set_bin_cuts(lo)
+    full_trace_compare(eng, g)
+    eng.close()
+
+
+def test_bin_cuts_variants_split_hub():
+    # the lowered hub cut sends rows of 4097+ entries through every bin-8 path
+    say("bin cuts with split hub variants")
+    g = star_plus_random(30000, 40000, 3)
+    for variant in (0, 2, 4, 6):
+        eng = gala.Engine(dump_iterations=True)
+        eng.set_bin_cuts(lowest_cuts())
+        v = eng.get_variants()
+        v[8] = variant
+        eng.set_variants(v)
+        full_trace_compare(eng, g)
+        eng.close()
 
 
 def test_weighted_int_graph():
