@@ -70,7 +70,8 @@ KERNEL_GENERAL = [
      "k_decide_blead<1024, 16384, 16, 2, IntW> + k_hub_plan/agg/scan/commit<IntW>",
      "k_decide_hub<16384, IntW> + k_hub_part<16384, IntW>",
      "k_decide_hub<16384, IntW> + k_hub_part<16384, IntW> (one part per hub)",
-     "k_hub_part<16384, IntW>"]]
+     "bin-8 split sequence: k_hub_plan, k_hub_resolve, k_hub_part<16384, IntW>, k_hub_commit "
+     "(+ overflow check: k_hub_plan, k_hub_agg, k_hub_scan, k_hub_commit)"]]
 KERNEL_FIRST = [
     ["k_decide_first<4, IntW>", "k_decide_pipe<4, true, IntW>", "k_decide_tpv<3, true, IntW>"],
     ["k_decide_first<8, IntW>", "k_decide_pipe<8, true, IntW>", "k_decide_tpv<7, true, IntW>"],
@@ -94,16 +95,29 @@ KERNEL_FIRST = [
 # inside the timed process, so the figures are recorded here and reported
 # beside the live timing of the same kernel.
 PMC_FETCH_FACTOR, PMC_WRITE_FACTOR = 2.00, 1.00
+KERNEL_GENERAL_BIN8_SPLIT = KERNEL_GENERAL[8][6]
 PMC_KIB = {  # kernel: (FETCH_SIZE KiB, WRITE_SIZE KiB) per dispatch
-    "k_decide_hlead<256, 16, 8, false, IntW>": (190300.5, 14015.4),
-    "k_decide_hlead<1024, 64, 8, false, IntW>": (106035.1, 1917.2),
-    "k_decide_hlead<1, 16, 8, true, IntW>": (105188.1, 9030.6),
-    "k_decide_blead<256, 8192, 16, 1, IntW>": (123872.1, 627.8),
-    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50893.3, 83.4),
-    "k_decide_tpv<31, false, IntW>": (102932.4, 13240.2),
-    "k_decide_tpv<15, false, IntW>": (86090.8, 13929.8),
-    "k_decide_tpv<7, false, IntW>": (64310.8, 13583.2),
-    "k_decide_tpv<3, false, IntW>": (25668.5, 6774.4),
+    # the bin-8 sequence timed as one unit by the per-bin events: the sums of
+    # its kernels per sweep (k_hub_part 195902.1 / 26.4, k_hub_resolve
+    # 11215.9 / 12015.0, two k_hub_plan 8.9 / 3.4, two k_hub_commit 84.2 / 41.7,
+    # k_hub_agg 15.0 / 0.0, k_hub_scan 19.0 / 0.0)
+    KERNEL_GENERAL_BIN8_SPLIT: (207338.1, 12131.2),
+    "k_decide_glead<256, 16, 8, false, IntW>": (192786.4, 5203.8),
+    "k_decide_blead<256, 8192, 16, 1, IntW>": (124312.4, 439.4),
+    "k_decide_glead<1024, 64, 8, false, IntW>": (107285.2, 1013.9),
+    "k_decide_glead<1, 16, 8, true, IntW>": (105256.9, 4198.4),
+    "k_decide_tpv<31, false, IntW>": (103881.2, 4875.9),
+    "k_decide_tpv<15, false, IntW>": (85595.5, 6750.1),
+    "k_decide_first_block<256, IntW>": (77622.5, 529.3),
+    "k_decide_tpv<7, false, IntW>": (64110.4, 8618.9),
+    "k_decide_glead<1, 64, 8, true, IntW>": (58005.1, 758.1),
+    "k_decide_tpv<15, true, IntW>": (56267.9, 6048.1),
+    "k_decide_tpv<31, true, IntW>": (55472.6, 4154.0),
+    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50872.6, 71.5),
+    "k_decide_first_block<1024, IntW>": (49534.6, 89.2),
+    "k_decide_tpv<7, true, IntW>": (41387.0, 7068.0),
+    "k_decide_tpv<3, false, IntW>": (25581.1, 5841.3),
+    "k_decide_pipe<4, true, IntW>": (16197.9, 4381.6),
 }
 
 
@@ -420,6 +434,8 @@ def main():
     ap.add_argument("--rmat-oracle", action="store_true", help="check the R-MAT leg against the FP64 oracle")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-frontier leg")
     ap.add_argument("--no-ingest", action="store_true", help="skip the text-ingest leg")
+    ap.add_argument("--ingest-only", action="store_true",
+                    help="only the text-ingest leg (prints its JSON object; for profiling the ingest kernels)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times and exit (scales the HBM byte counters)")
     args = ap.parse_args()
@@ -438,6 +454,9 @@ def main():
     t0 = time.time()
     g = gala.gen_chung_lu(args.n, 2.1, 17.35, 15000.0, args.seed)
     log(f"graph: n={g.vertex_num} undirected_edges={g.edge_num} gen {time.time() - t0:.1f}s")
+    if args.ingest_only:
+        print(json.dumps(ingest_leg(gala, g, args)))
+        return
 
     eng = gala.Engine(device=0)
     eng.set_graph(g)
