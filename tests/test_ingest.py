@@ -287,3 +287,33 @@ def test_device_two_call_protocol_and_load_text(engine, tmp_path):
     engine.set_graph(host_ingest(b, gala.W_I32))
     r2 = engine.run()
     assert r1["modularity"] == r2["modularity"] and r1["n_communities"] == r2["n_communities"]
+
+
+# ---------------------------------------------------------------------------
+# the preprocess drop-in CLI (preprocess.cpp:7-47), on the host and with -g on the GPU
+# ---------------------------------------------------------------------------
+CLI_PRE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gala-mg_amd", "preprocess")
+
+
+def run_preprocess(src, out_dir, gpu=None):
+    cmd = [CLI_PRE, "-f", src, "-o", str(out_dir)] + ([] if gpu is None else ["-g", str(gpu)])
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0] == os.path.join(str(out_dir), os.path.basename(src) + ".bin")
+    assert lines[-1] == "store successfully. "
+    return open(lines[0], "rb").read()
+
+
+@pytest.mark.parametrize("name", ["karate.txt", "snap_sparse.txt"])
+def test_preprocess_cli_matches_reference(tmp_path, name):
+    src = os.path.join(GOLDEN, name)
+    assert run_preprocess(src, tmp_path) == open(src + ".bin", "rb").read()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["karate.txt", "snap_sparse.txt"])
+def test_preprocess_cli_on_gpu_matches_reference(tmp_path, name):
+    src = os.path.join(GOLDEN, name)
+    print(f"preprocess -g 0 {name}", flush=True)
+    assert run_preprocess(src, tmp_path, gpu=0) == open(src + ".bin", "rb").read()
