@@ -48,6 +48,8 @@ hipError_t dalloc(P** p, int64_t elems) {
 struct gala_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // long-row degree bins beside the others (over_bins)
+  hipEvent_t evf[2] = {};         // fork / join of the two streams
   gala_opts opts{};
   hipError_t last_hip = hipSuccess;
 
@@ -99,6 +101,7 @@ struct gala_ctx {
   int32_t* hub_rc = nullptr;  // resolved split-hub rows (hub_cap slots each)
   void* hub_rw = nullptr;
   int64_t hub_cap = 0;
+  int64_t hub_dmax = -1;  // longest row of the round graph's hub bins (round_setup; -1: unknown)
   int32_t* hub_plan = nullptr;  // split hub decide scratch
   void* hub_part = nullptr;
   uint8_t* hub_fail = nullptr;
@@ -383,17 +386,21 @@ int grow_dense(gala_ctx* ctx, const ContractPlan& plan, bool wide) {
 // --------------------------------------------------------------------------
 // per-vertex hub-table offsets for the bins that use the global arena
 // --------------------------------------------------------------------------
+// per hub row: its table size next_pow2(2d); *dmax (zero before) = the longest row
 __global__ void k_hub_sizes(const int32_t* __restrict__ order, const int32_t* __restrict__ bstart,
                             int gbin, const int64_t* __restrict__ off, const int64_t* __restrict__ end,
-                            int64_t* __restrict__ tsz) {
+                            int64_t* __restrict__ tsz, int64_t* __restrict__ dmax) {
   const int lo = bstart[gbin], hi = bstart[NBINS];
+  int64_t mx = 0;
   for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
     int v = order[i];
     int64_t d = end[v] - off[v];
     int64_t t = 1;
     while (t < 2 * d) t <<= 1;
     tsz[i - lo] = t;
+    mx = d > mx ? d : mx;
   }
+  if (mx > 0) atomicMax((unsigned long long*)dmax, (unsigned long long)mx);
 }
 __global__ void k_hub_scatter(const int32_t* __restrict__ order, const int32_t* __restrict__ bstart,
                               int gbin, const int64_t* __restrict__ pos, int64_t* __restrict__ hub_off) {
@@ -405,12 +412,13 @@ __global__ void k_offsets_from_incl(const int64_t* __restrict__ end_incl, int32_
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x)
     off[i] = i == 0 ? 0 : end_incl[i - 1];
 }
-// gathers the round set-up readback into one 72-byte block (see round_setup)
+// gathers the round set-up readback into one 80-byte block (see round_setup)
 __global__ void k_setup_stage(const int32_t* __restrict__ bstart, const int64_t* __restrict__ slots,
-                              int64_t* __restrict__ stage) {
+                              const int64_t* __restrict__ dmax, int64_t* __restrict__ stage) {
   const int t = threadIdx.x;
   if (t <= NBINS) ((int32_t*)stage)[t] = bstart[t];
   if (t == 0) stage[8] = *slots;
+  if (t == 1) stage[9] = *dmax;
 }
 
 __global__ void k_i32_to_i64(const int32_t* __restrict__ a, int64_t* __restrict__ b, int64_t n) {
@@ -520,6 +528,7 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.hub_keys = ctx->hub_keys; st.hub_vals = (typename T::acc_t*)ctx->hub_vals;
   st.hub_rc = ctx->hub_rc; st.hub_rw = (typename T::acc_t*)ctx->hub_rw;
   st.hub_plan = ctx->hub_plan; st.hub_part = ctx->hub_part; st.hub_fail = ctx->hub_fail;
+  st.hub_dmax = ctx->hub_dmax;
   st.ctr = ctx->ctr;
   st.track = 0;
   st.iter = 0;
@@ -552,18 +561,21 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   // runs over n slots (entries past the hub count are zero)
   const int gb = global_bin_min<T>();
   const int64_t nn = std::max<int32_t>(g.n, 1);
-  fill<int64_t>(ctx->hub_tmp, nn, 0, s);
+  // hub_tmp[0, nn): table sizes; hub_tmp[nn]: the longest hub row
+  fill<int64_t>(ctx->hub_tmp, nn + 1, 0, s);
   k_hub_sizes<<<std::min<int64_t>((nn + 255) / 256, 1024), 256, 0, s>>>(ctx->order, ctx->bstart_full, gb, g.off,
-                                                                         g.end, ctx->hub_tmp);
+                                                                         g.end, ctx->hub_tmp, ctx->hub_tmp + nn);
   int64_t* pos = ctx->hub_tmp + nn + 1;
   int64_t* tot = ctx->scan64_tmp;
   scan_exclusive<int64_t, int64_t>(ctx->hub_tmp, pos, nn, ctx->c_i64_tmp, tot, s);
-  // one transfer: bytes [0, 40) the NBINS + 1 bin starts, [64, 72) the hub-table slots
-  k_setup_stage<<<1, 64, 0, s>>>(ctx->bstart_full, tot, ctx->c_i64_scalar);
-  HIPOK(hipMemcpyAsync(ctx->hpin, ctx->c_i64_scalar, 72, hipMemcpyDeviceToHost, s));
+  // one transfer: bytes [0, 40) the NBINS + 1 bin starts, [64, 72) the hub-table
+  // slots, [72, 80) the longest hub row
+  k_setup_stage<<<1, 64, 0, s>>>(ctx->bstart_full, tot, ctx->hub_tmp + nn, ctx->c_i64_scalar);
+  HIPOK(hipMemcpyAsync(ctx->hpin, ctx->c_i64_scalar, 80, hipMemcpyDeviceToHost, s));
   HIPOK(hipStreamSynchronize(s));
   const int32_t* hb = (const int32_t*)ctx->hpin;
   const int64_t* hs = (const int64_t*)(ctx->hpin + 64);
+  ctx->hub_dmax = hs[1];
   for (int b = 0; b < NBINS; b++) caps[b] = hb[b + 1] - hb[b];
   *nitems = hb[NBINS];
   launch_init_rows<T>(g, st, ctx->order, ctx->bstart_full, caps, s);
@@ -607,6 +619,28 @@ int read_counters(gala_ctx* ctx, Counters* out, Counters* saved = nullptr) {
     if (saved->overflow) return GALA_E_OVERFLOW;
   }
   if (out->overflow) return GALA_E_OVERFLOW;
+  return GALA_OK;
+}
+
+// Launches one kernel per degree bin.  concurrent: bins CONC_BIN.. (few long
+// rows, one LDS-heavy workgroup per vertex, long tails) go on the second
+// stream beside the short-row bins, so the two halves share the CUs instead
+// of draining one after the other; both halves write disjoint vertices (and
+// add exact integer deltas where they meet).  Joined back before returning.
+constexpr int CONC_BIN = 6;
+template <class F>
+int over_bins(gala_ctx* ctx, bool concurrent, F&& launch) {
+  hipStream_t s = ctx->stream;
+  if (!concurrent) {
+    for (int b = 0; b < NBINS; b++) launch(b, s);
+    return GALA_OK;
+  }
+  HIPOK(hipEventRecord(ctx->evf[0], s));
+  HIPOK(hipStreamWaitEvent(ctx->stream2, ctx->evf[0], 0));
+  for (int b = CONC_BIN; b < NBINS; b++) launch(b, ctx->stream2);
+  for (int b = 0; b < CONC_BIN; b++) launch(b, s);
+  HIPOK(hipEventRecord(ctx->evf[1], ctx->stream2));
+  HIPOK(hipStreamWaitEvent(s, ctx->evf[1], 0));
   return GALA_OK;
 }
 
@@ -666,6 +700,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   const int max_it = ctx->opts.max_iterations > 0 ? ctx->opts.max_iterations : 100000;
   const float tau = ctx->opts.learned_tau > 0 ? ctx->opts.learned_tau : 0.5f;
   const bool prof = ctx->opts.profile_bins != 0;
+  const bool conc = !prof && !ctx->opts.serial_bins;  // long-row bins on the second stream
   const size_t tsz = sizeof(tt);
   double t_dm = 0, t_upd = 0;
   std::vector<float> hfeat;
@@ -683,12 +718,20 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     if (incr) sd.dirty = nullptr;
     HIPOK(hipEventRecord(ctx->ev[0], s));
     if (build_cv) launch_build_cv<T>(sd, g.n, s);
-    for (int b = 0; b < NBINS; b++) {
-      if (prof) HIPOK(hipEventRecord(ctx->bev[b], s));
-      if (first) launch_decide_first<T>(b, g, sd, fr, bs, caps[b], m2, s, ctx->fvar[b]);
-      else launch_decide<T>(b, g, sd, fr, bs, caps[b], m2, s, ctx->dvar[b]);
+    if (prof) {  // per-bin events: one stream
+      for (int b = 0; b < NBINS; b++) {
+        HIPOK(hipEventRecord(ctx->bev[b], s));
+        if (first) launch_decide_first<T>(b, g, sd, fr, bs, caps[b], m2, s, ctx->fvar[b]);
+        else launch_decide<T>(b, g, sd, fr, bs, caps[b], m2, s, ctx->dvar[b]);
+      }
+      HIPOK(hipEventRecord(ctx->bev[NBINS], s));
+    } else {
+      int r = over_bins(ctx, conc, [&](int b, hipStream_t sb) {
+        if (first) launch_decide_first<T>(b, g, sd, fr, bs, caps[b], m2, sb, ctx->fvar[b]);
+        else launch_decide<T>(b, g, sd, fr, bs, caps[b], m2, sb, ctx->dvar[b]);
+      });
+      if (r) return r;
     }
-    if (prof) HIPOK(hipEventRecord(ctx->bev[NBINS], s));
     HIPOK(hipEventRecord(ctx->ev[1], s));
     return GALA_OK;
   };
@@ -698,8 +741,10 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     launch_apply<T>(sp, g.n, m2, s);
     // In changes only where a vertex or one of its neighbours moved
     if (incr) {
-      for (int b = 0; b < NBINS; b++)
-        launch_weight_moved<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], s, ctx->wvar[b]);
+      int r = over_bins(ctx, conc, [&](int b, hipStream_t sb) {
+        launch_weight_moved<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], sb, ctx->wvar[b]);
+      });
+      if (r) return r;
     }
     if (!incr)
       for (int b = 0; b < NBINS; b++)
@@ -1465,8 +1510,20 @@ int gala_create(gala_ctx** out, int device, const gala_opts* opts) {
   if (opts) ctx->opts = *opts;
   else gala_default_opts(&ctx->opts);
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return GALA_E_HIP; }
+  if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GALA_E_HIP;
+  }
   // device address of the no-allocation entry point's static scratch on this GPU
-  if (int rs = modularity_scratch_setup(device)) { (void)hipStreamDestroy(ctx->stream); delete ctx; return rs; }
+  if (int rs = modularity_scratch_setup(device)) {
+    (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream2);
+    delete ctx;
+    return rs;
+  }
+  for (auto& e : ctx->evf)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { gala_destroy(ctx); return GALA_E_HIP; }
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) { gala_destroy(ctx); return GALA_E_HIP; }
   for (auto& e : ctx->bev)
@@ -1479,6 +1536,7 @@ void gala_destroy(gala_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   free_all(ctx);
   if (ctx->mlp_prm) (void)hipFree(ctx->mlp_prm);
   if (ctx->mlp_w1b) (void)hipFree(ctx->mlp_w1b);
@@ -1489,6 +1547,9 @@ void gala_destroy(gala_ctx* ctx) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->bev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->evf)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

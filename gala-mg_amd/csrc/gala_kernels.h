@@ -99,6 +99,10 @@ struct DevState {
   int32_t* hub_plan = nullptr;  // split hub decide: chunk starts (hub_plan_elems)
   void* hub_part = nullptr;     // split hub decide: HubPart<T>[hub_part_elems]
   uint8_t* hub_fail = nullptr;  // community-split hub decide: per-hub overflow flags
+  // longest row of the round graph's hub bins (host-side bound from the round
+  // set-up; -1: unknown): lets the launcher skip the split-hub kernels that no
+  // row of the round can need
+  int64_t hub_dmax = -1;
   Counters* ctr;
   // learned-frontier features (tracked only when track != 0)
   int track;

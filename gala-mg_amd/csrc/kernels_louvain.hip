@@ -1702,8 +1702,8 @@ __device__ __forceinline__ int hub_find(const int32_t* __restrict__ start, int n
 // part_keys > 0: instead of table chunks, plan ceil(d / part_keys) community
 // partitions per hub (k_hub_part) for rows of up to part_max entries; longer
 // rows (whose P workgroups would each stream the whole row) are marked in
-// fail_out for the global-table split.  only != null: plan only the hubs
-// marked there.
+// fail_out for the global-table split, every other hub's flag is cleared.
+// only != null: plan only the hubs marked there.
 template <class T>
 __global__ __launch_bounds__(1024) void k_hub_plan(DevGraph<T> g, const int32_t* __restrict__ front,
                                                    const int32_t* __restrict__ bstart, int bin,
@@ -1721,10 +1721,11 @@ __global__ __launch_bounds__(1024) void k_hub_plan(DevGraph<T> g, const int32_t*
     if (i < nh) {
       const int v = front[lo + i];
       const int64_t d = g.end[v] - g.off[v];
+      uint8_t f = 0;
       if (d > split_above && (!only || only[i])) {
         if (part_keys > 0) {
           if (d <= part_max) nt = (int)((d + part_keys - 1) / part_keys);
-          else fail_out[i] = 1;
+          else f = 1;
         } else {
           int64_t tsz = 1;
           while (tsz < 2 * d) tsz <<= 1;
@@ -1732,6 +1733,7 @@ __global__ __launch_bounds__(1024) void k_hub_plan(DevGraph<T> g, const int32_t*
           nt = (int)((tsz + HUB_TCH - 1) / HUB_TCH);
         }
       }
+      if (fail_out) fail_out[i] = f;
     }
     int tot_c, tot_t;
     const int xc = block_excl_scan<int>(nc, sh, &tot_c);
@@ -1934,6 +1936,10 @@ __global__ __launch_bounds__(1024) void k_hub_resolve(DevGraph<T> g, DevState<T>
   }
 }
 
+// The table is evaluated by a scan, the Tot gathers of eight slots per lane
+// issued together.  (Holding a row of up to 16,384 entries in registers with
+// leader evaluation from 8-byte community views measured slower on the
+// power-law workload: the views' gathers hit L2 less often than prev[].)
 template <int LT, class T>
 __global__ __launch_bounds__(1024) void k_hub_part(DevGraph<T> g, DevState<T> st, const int32_t* __restrict__ front,
                                                     const int32_t* __restrict__ bstart, int bin,
@@ -1942,7 +1948,12 @@ __global__ __launch_bounds__(1024) void k_hub_part(DevGraph<T> g, DevState<T> st
   using acc = typename T::acc_t;
   using tt = typename T::tot_t;
   using G = typename T::gain_t;
-  constexpr int NT = 1024, PMAX = 64, U = 8, LG = __builtin_ctz(LT);
+  // PMAX: probes before a part gives up (its hub then takes the fallback);
+  // at most 5/8 of the table is expected to fill, so only a skewed partition
+  // reaches it
+  constexpr int NT = 1024, PMAX = 256, U = 8, LG = __builtin_ctz(LT);
+  constexpr int SB = 8;  // table scan: slots per lane per batch of Tot gathers
+  static_assert(LT % (NT * SB) == 0, "table scan batches");
   __shared__ int s_keys[LT];
   __shared__ acc s_vals[LT];
   __shared__ G r_h[NT / 64];
@@ -1961,10 +1972,14 @@ __global__ __launch_bounds__(1024) void k_hub_part(DevGraph<T> g, DevState<T> st
     const tt m2 = vertex_m2<T>(st, v, m2_all);
     const int64_t o = g.off[v];
     const int d = (int)(g.end[v] - o);
-    const bool res = P >= HUB_RESOLVE_P;  // uniform per workgroup
-    const int64_t rb = res ? st.hub_off[v] : 0;  // the row resolved by k_hub_resolve
+    const tt K = st.K[v];
+    G h = 0;
+    int bd = INT_MAX;
+    acc wc = 0;
     if (threadIdx.x == 0) s_fail = 0;
     __syncthreads();  // also orders the table clears before the inserts
+    const bool res = P >= HUB_RESOLVE_P;  // uniform per workgroup
+    const int64_t rb = res ? st.hub_off[v] : 0;  // the row resolved by k_hub_resolve
     for (int j0 = threadIdx.x; j0 < d; j0 += NT * U) {
       int cu[U];
       acc wv[U];
@@ -2002,23 +2017,32 @@ __global__ __launch_bounds__(1024) void k_hub_part(DevGraph<T> g, DevState<T> st
     __syncthreads();
     const bool failed = s_fail != 0;
     const int c = st.prev[v];
-    const tt K = st.K[v];
     const tt Tc = st.Tot[c];
-    G h = 0;
-    int bd = INT_MAX;
-    acc wc = 0;
-    for (int t = threadIdx.x; t < LT; t += NT) {
-      const int k = s_keys[t];
-      if (k) {
-        const int dd = k - 1;
-        const acc a = s_vals[t];
-        s_keys[t] = 0;
-        s_vals[t] = acc(0);
-        if (!failed) {
+    for (int t0 = threadIdx.x; t0 < LT; t0 += NT * SB) {
+      int kk[SB];
+      acc av[SB];
+      tt tv[SB];
+#pragma unroll
+      for (int r = 0; r < SB; r++) {
+        const int t = t0 + r * NT;
+        kk[r] = s_keys[t];
+        av[r] = acc(0);
+        if (kk[r]) {
+          av[r] = s_vals[t];
+          s_keys[t] = 0;
+          s_vals[t] = acc(0);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < SB; r++) tv[r] = (kk[r] && !failed && kk[r] - 1 != c) ? st.Tot[kk[r] - 1] : tt(0);
+#pragma unroll
+      for (int r = 0; r < SB; r++) {
+        if (kk[r] && !failed) {
+          const int dd = kk[r] - 1;
           if (dd == c) {
-            wc = a;
+            wc = av[r];
           } else {
-            const G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)st.Tot[dd], (G)Tc);
+            const G hh = gain_h<T>((G)av[r], (G)m2, (G)K, (G)tv[r], (G)Tc);
             if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; }
           }
         }
@@ -2036,39 +2060,122 @@ __global__ __launch_bounds__(1024) void k_hub_part(DevGraph<T> g, DevState<T> st
       part[q].h = h;
       part[q].d = bd;
       part[q].wc = wc;
-      if (failed) fail[i] = 1;
+      if (s_fail) fail[i] = 1;
     }
     __syncthreads();
   }
 }
 
+// Reads a slot of a global table and leaves it zero (one atomic at L2).
+template <class V>
+__device__ __forceinline__ V atomic_take(V* p) {
+  if constexpr (sizeof(V) == 8) return (V)atomicExch((unsigned long long*)p, 0ull);
+  else return (V)atomicExch((int*)p, 0);
+}
+
+// One workgroup aggregates the whole row of hub v in its global slice
+// (next_pow2(2d) slots, so every distinct community fits), evaluates every
+// community and leaves the slice zero; thread 0 receives the best candidate
+// and w_c.  The fallback of a hub whose community part overflowed its LDS
+// table: rare (a skewed hash), so one workgroup suffices.
+template <int NT, class T>
+__device__ void hub_global_decide(const DevGraph<T>& g, const DevState<T>& st, int v, int64_t o, int d,
+                                  typename T::tot_t m2, typename T::gain_t* r_h, int* r_d,
+                                  typename T::acc_t* r_wc, typename T::gain_t& h, int& bd,
+                                  typename T::acc_t& wc) {
+  using acc = typename T::acc_t;
+  using tt = typename T::tot_t;
+  using G = typename T::gain_t;
+  const int glg = ilog2_ceil(2 * d);
+  const int gsz = 1 << glg;
+  int* gkeys = st.hub_keys + st.hub_off[v];
+  acc* gvals = st.hub_vals + st.hub_off[v];
+  for (int j = threadIdx.x; j < d; j += NT) {
+    const int cl = st.prev[g.nbr[o + j]];
+    const acc w = (acc)load_w<T>(g, o + j);
+    unsigned p = hash_slot(cl, glg);
+    int probe = 0;
+    for (; probe < gsz; probe++) {
+      const int k = atomicCAS(&gkeys[p], 0, cl + 1);
+      if (k == 0 || k == cl + 1) { atomicAdd(&gvals[p], w); break; }
+      p = (p + 1) & (gsz - 1);
+    }
+    if (probe == gsz) st.ctr->overflow = 1;
+  }
+  __threadfence();  // every insert performed at L2 before the table is read
+  __syncthreads();
+  const int c = st.prev[v];
+  const tt K = st.K[v];
+  const tt Tc = st.Tot[c];
+  h = 0;
+  bd = INT_MAX;
+  wc = 0;
+  for (int t = threadIdx.x; t < gsz; t += NT) {
+    const int k = atomic_take(&gkeys[t]);
+    if (k) {
+      const acc a = atomic_take(&gvals[t]);
+      const int dd = k - 1;
+      if (dd == c) {
+        wc = a;
+      } else {
+        const G hh = gain_h<T>((G)a, (G)m2, (G)K, (G)st.Tot[dd], (G)Tc);
+        if (hh > 0 && better(hh, dd, h, bd)) { h = hh; bd = dd; }
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  argmax_xor<G, 64>(h, bd);
+  wc = sum_xor<acc, 64>(wc);
+  if (lane == 0) { r_h[wid] = h; r_d[wid] = bd; r_wc[wid] = wc; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < NT / 64; w++) {
+      if (better(r_h[w], r_d[w], h, bd)) { h = r_h[w]; bd = r_d[w]; }
+      wc += r_wc[w];
+    }
+  }
+}
+
+// Combines each split hub's part results in part order (the same sums every
+// run) and commits the move.  skip[i] set: a part of hub i overflowed, or the
+// row was too long to split; with inline_fb (no too-long rows this sweep) the
+// hub is decided here from its global slice, otherwise the global-table split
+// kernels redo it.
 template <class T>
 __global__ __launch_bounds__(256) void k_hub_commit(DevGraph<T> g, DevState<T> st, const int32_t* __restrict__ front,
                                                     const int32_t* __restrict__ bstart, int bin,
                                                     const int32_t* __restrict__ plan,
                                                     const HubPart<T>* __restrict__ part, typename T::tot_t m2_all,
-                                                    const uint8_t* __restrict__ skip) {
+                                                    const uint8_t* __restrict__ skip, int inline_fb) {
   using acc = typename T::acc_t;
   using G = typename T::gain_t;
   constexpr int NT = 256;
   __shared__ int s_mv;
+  __shared__ G r_h[NT / 64];
+  __shared__ int r_d[NT / 64];
+  __shared__ acc r_wc[NT / 64];
   const int lo = bstart[bin], nh = bstart[bin + 1] - lo;
   const int32_t* tstart = plan + nh + 1;
   unsigned moved = 0, swept = 0;
   unsigned long long edges = 0;
   for (int i = blockIdx.x; i < nh; i += gridDim.x) {
     // no chunks: left to k_decide_hub; skipped: redone by the fallback
-    if (tstart[i] == tstart[i + 1] || (skip && skip[i])) continue;  // uniform per workgroup
+    const bool failed = skip && skip[i];
+    if (tstart[i] == tstart[i + 1] || (failed && !inline_fb)) continue;  // uniform per workgroup
     const int v = front[lo + i];
     const int64_t o = g.off[v];
     const int d = (int)(g.end[v] - o);
+    G h = 0;
+    int bd = INT_MAX;
+    acc wc = 0;
+    if (failed)
+      hub_global_decide<NT, T>(g, st, v, o, d, vertex_m2<T>(st, v, m2_all), r_h, r_d, r_wc, h, bd, wc);
     if (threadIdx.x == 0) {
-      G h = 0;
-      int bd = INT_MAX;
-      acc wc = 0;
-      for (int q = tstart[i]; q < tstart[i + 1]; q++) {  // chunk order: the same sums every run
-        if (better(part[q].h, part[q].d, h, bd)) { h = part[q].h; bd = part[q].d; }
-        wc += part[q].wc;
+      if (!failed) {
+        for (int q = tstart[i]; q < tstart[i + 1]; q++) {  // chunk order: the same sums every run
+          if (better(part[q].h, part[q].d, h, bd)) { h = part[q].h; bd = part[q].d; }
+          wc += part[q].wc;
+        }
       }
       const int c = st.prev[v];
       swept++;
@@ -2349,7 +2456,7 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
       k_hub_agg<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
       k_hub_scan<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, (HubPart<T>*)st.hub_part, m2);
       k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan,
-                                                              (const HubPart<T>*)st.hub_part, m2, nullptr);
+                                                              (const HubPart<T>*)st.hub_part, m2, nullptr, 0);
       break;
     }
     case 132:
@@ -2366,19 +2473,27 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
       HubPart<T>* hp = (HubPart<T>*)st.hub_part;
       if (above > 0)
         k_decide_hub<HUB_LT, T><<<grid_for(cap, 1, 1024), 1024, 0, s>>>(g, st, front, bstart, b, m2, above);
-      fill<uint8_t>(st.hub_fail, cap, 0, s);
       // rows beyond 16 partitions go to the global-table split straight away
       const int64_t part_max = variant == 5 ? INT64_MAX : 16ll * PART_KEYS;
+      // from the round's longest hub row: whether any row can be too long to
+      // split, or have enough parts to be resolved first (unknown: both)
+      const bool long_rows = st.hub_dmax < 0 || st.hub_dmax > part_max;
+      const bool resolve = st.hub_dmax < 0 || st.hub_dmax > (int64_t)(HUB_RESOLVE_P - 1) * PART_KEYS;
       k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, PART_KEYS, nullptr, part_max,
-                                        st.hub_fail);
-      k_hub_resolve<T><<<dim3(std::min(cap, 512), 8), 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
+                                        st.hub_fail);  // also clears the flags
+      if (resolve)
+        k_hub_resolve<T><<<dim3(std::min(cap, 512), 8), 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
       k_hub_part<HUB_LT, T><<<2048, 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, st.hub_fail, m2);
+      // without too-long rows an overflowed hub is decided by the commit itself
       k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2,
-                                                              st.hub_fail);
-      k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, 0, st.hub_fail, 0, nullptr);
-      k_hub_agg<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
-      k_hub_scan<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2);
-      k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2, nullptr);
+                                                              st.hub_fail, long_rows ? 0 : 1);
+      if (long_rows) {
+        k_hub_plan<T><<<1, 1024, 0, s>>>(g, front, bstart, b, st.hub_plan, above, 0, st.hub_fail, 0, nullptr);
+        k_hub_agg<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
+        k_hub_scan<T><<<2048, 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2);
+        k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2, nullptr,
+                                                                0);
+      }
       break;
     }
   }

@@ -57,7 +57,7 @@ class Opts(ctypes.Structure):
                 ("dump_rounds", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("frontier_mode", ctypes.c_int), ("learned_tau", ctypes.c_float),
                 ("profile_bins", ctypes.c_int), ("no_hip_graphs", ctypes.c_int),
-                ("reserved", ctypes.c_int * 6)]
+                ("serial_bins", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
 
 
 class Result(ctypes.Structure):
@@ -395,7 +395,7 @@ class Engine:
 
     def __init__(self, device=0, min_modularity=1e-6, dump_iterations=False, dump_rounds=False,
                  max_rounds=0, max_iterations=0, profile_bins=False, frontier_mode=0, learned_tau=0.5,
-                 hip_graphs=True):
+                 hip_graphs=True, serial_bins=False):
         L = lib()
         self.opts = Opts()
         L.gala_default_opts(ctypes.byref(self.opts))
@@ -408,6 +408,7 @@ class Engine:
         self.opts.frontier_mode = int(frontier_mode)
         self.opts.learned_tau = float(learned_tau)
         self.opts.no_hip_graphs = 0 if hip_graphs else 1
+        self.opts.serial_bins = int(serial_bins)
         self.ctx = VP()
         _check(L.gala_create(ctypes.byref(self.ctx), device, ctypes.byref(self.opts)), "gala_create")
 

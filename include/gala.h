@@ -52,7 +52,9 @@ typedef struct {
   int profile_bins;      /* 1: time every degree bin's decide launch with HIP events */
   int no_hip_graphs;     /* batched runs replay each round's iteration body from a
                             captured hipGraph; 1 = launch kernel by kernel instead */
-  int reserved[6];
+  int serial_bins;       /* single-graph runs launch the long-row degree bins (512+)
+                            on a second stream beside the others; 1 = one stream */
+  int reserved[5];
 } gala_opts;
 
 /* Called after every iteration (Iteration:0 is Q before any move).

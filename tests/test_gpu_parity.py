@@ -81,6 +81,16 @@ def test_chung_lu_hub_bins(seed):
     eng.close()
 
 
+def test_serial_bins_match_oracle():
+    # every degree bin on one stream (the long-row bins otherwise run on a
+    # second stream beside the others): the same trajectory
+    say("serial bins")
+    g = gala.gen_chung_lu(200000, 2.1, 17.35, 12000.0, 1)
+    eng = gala.Engine(dump_iterations=True, serial_bins=True)
+    full_trace_compare(eng, g)
+    eng.close()
+
+
 def variant_set(k):
     return np.array([min(k, c - 1) for c in gala.variant_counts()], np.int32)
 
@@ -344,6 +354,38 @@ def test_phase_decide_hub_overflow(engine):
         got = engine.phase_decide(g, m2, K, Self, Tot, size, prev)
         exp = oracle.decide_step(g.vertex_num, g.degrees, g.neighbors, w, m2, K, Self, Tot, size, prev)
         assert np.array_equal(got, exp), rep
+
+
+@pytest.mark.parametrize("variant", [5, 6])
+def test_phase_decide_hub_full_part_table(variant):
+    # a 16,383-degree hub whose neighbours all sit in distinct communities: with
+    # one part per hub (variant 5) its 16,384-slot LDS table cannot take them
+    # all, so the commit kernel decides the hub from its global slice; with two
+    # parts (variant 6) the table holds them.  The second call checks that the
+    # slice and the table were left zeroed.
+    say(f"phase decide full part table, variant {variant}")
+    g = star_plus_random(16383, 20000, 4)
+    rng = np.random.default_rng(8)
+    n = g.vertex_num
+    off = g.offsets()
+    K = np.diff(off).astype(np.int64)
+    Self = np.zeros(n, np.int64)
+    prev = rng.permutation(n).astype(np.int32)  # every vertex its own community
+    Tot = np.zeros(n, np.int64)
+    Tot[prev] = K
+    size = np.zeros(n, np.int32)
+    size[prev] = 1
+    m2 = int(K.sum())
+    w = np.ones(len(g.neighbors), np.int64)
+    eng = gala.Engine()
+    v = eng.get_variants()
+    v[8] = variant
+    eng.set_variants(v)
+    for rep in range(2):
+        got = eng.phase_decide(g, m2, K, Self, Tot, size, prev)
+        exp = oracle.decide_step(g.vertex_num, g.degrees, g.neighbors, w, m2, K, Self, Tot, size, prev)
+        assert np.array_equal(got, exp), rep
+    eng.close()
 
 
 @pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
