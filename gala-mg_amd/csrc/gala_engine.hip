@@ -627,16 +627,23 @@ int read_counters(gala_ctx* ctx, Counters* out, Counters* saved = nullptr) {
 // stream beside the short-row bins, so the two halves share the CUs instead
 // of draining one after the other; both halves write disjoint vertices (and
 // add exact integer deltas where they meet).  Joined back before returning.
+// pre2(stream): independent work queued on the second stream ahead of the
+// long-row bins (on the one stream ahead of every bin when not concurrent).
 constexpr int CONC_BIN = 6;
-template <class F>
-int over_bins(gala_ctx* ctx, bool concurrent, F&& launch) {
+struct NoPre {
+  void operator()(hipStream_t) const {}
+};
+template <class F, class P = NoPre>
+int over_bins(gala_ctx* ctx, bool concurrent, F&& launch, P&& pre2 = P()) {
   hipStream_t s = ctx->stream;
   if (!concurrent) {
+    pre2(s);
     for (int b = 0; b < NBINS; b++) launch(b, s);
     return GALA_OK;
   }
   HIPOK(hipEventRecord(ctx->evf[0], s));
   HIPOK(hipStreamWaitEvent(ctx->stream2, ctx->evf[0], 0));
+  pre2(ctx->stream2);
   for (int b = CONC_BIN; b < NBINS; b++) launch(b, ctx->stream2);
   for (int b = 0; b < CONC_BIN; b++) launch(b, s);
   HIPOK(hipEventRecord(ctx->evf[1], ctx->stream2));
@@ -738,13 +745,20 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   // Tot/size update, In of the invalidated rows, prune, next frontier and the
   // counters; accumulates the sweep's work and timings
   auto update = [&](DevState<T>& sp, bool first, Counters* out) -> int {
-    launch_apply<T>(sp, g.n, m2, s);
-    // In changes only where a vertex or one of its neighbours moved
+    // Tot/size from the movers (on the second stream when concurrent) beside
+    // In: In changes only where a vertex or one of its neighbours moved, so
+    // only the movers' rows are read.  (Recomputing every In after the first
+    // sweep of a round, when most vertices move, measured slower.)
     if (incr) {
-      int r = over_bins(ctx, conc, [&](int b, hipStream_t sb) {
-        launch_weight_moved<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], sb, ctx->wvar[b]);
-      });
+      int r = over_bins(
+          ctx, conc,
+          [&](int b, hipStream_t sb) {
+            launch_weight_moved<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], sb, ctx->wvar[b]);
+          },
+          [&](hipStream_t s2) { launch_apply<T>(sp, g.n, m2, s2); });
       if (r) return r;
+    } else {
+      launch_apply<T>(sp, g.n, m2, s);
     }
     if (!incr)
       for (int b = 0; b < NBINS; b++)
