@@ -49,6 +49,31 @@ __device__ __forceinline__ V wave_sum(V x) {
   return x;
 }
 
+// Equal keys within a wave, combined before an LDS atomic insert: while the
+// first pending lane's key is held by at least two pending lanes, their values
+// are summed across the wave and that lane inserts the sum once; the lanes
+// left (keys not repeated in the wave) insert their own values.  Many lanes
+// adding into one LDS slot otherwise serialise on that address (a community
+// whose row entries almost all carry a few keys).  Every lane of the wave
+// must call it; insert(key, value) is called for valid lanes only.
+template <class V, class F>
+__device__ __forceinline__ void wave_combine_insert(int key, V w, bool valid, F&& insert) {
+  const int lane = threadIdx.x & 63;
+  bool pending = valid;
+  for (int round = 0; round < 64; round++) {
+    const unsigned long long m = __ballot(pending);
+    if (!m) return;
+    const int src = __ffsll((unsigned long long)m) - 1;
+    const int lk = __shfl(key, src, 64);
+    const bool mine = pending && key == lk;
+    if (__popcll(__ballot(mine)) < 2) break;  // the first pending key is not repeated
+    const V s = wave_sum<V>(mine ? w : V(0));
+    if (lane == src) insert(lk, s);
+    pending = pending && !mine;
+  }
+  if (pending) insert(key, w);
+}
+
 // Exclusive workgroup scan for blockDim.x <= 1024; returns the prefix and
 // *total = the workgroup sum.  sh needs 17 elements.  Contains barriers: every
 // thread of the workgroup must call it.

@@ -50,6 +50,7 @@ struct gala_ctx {
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // long-row degree bins beside the others (over_bins)
   hipEvent_t evf[2] = {};         // fork / join of the two streams
+  hipEvent_t evr[2] = {};         // batched runs: readback of the live-graph count, one per iteration in flight
   gala_opts opts{};
   hipError_t last_hip = hipSuccess;
 
@@ -167,6 +168,7 @@ struct gala_ctx {
   std::vector<void*> bowned;
   bool have_batch = false, have_batch_result = false;
   std::vector<int32_t> b_comm_local;   // per-vertex community within its graph
+  bool b_comm_ready = false;           // b_comm_local read back from fin (gala_batch_communities)
   std::vector<double> b_q;
   std::vector<int32_t> b_nc, b_rd, b_it;
 
@@ -192,7 +194,6 @@ struct gala_ctx {
   bool cap_wide = false;
   gala_result res{};
   bool have_result = false;
-  std::vector<int32_t> comm_host;
 
   hipEvent_t ev[8] = {};
   hipEvent_t bev[NBINS + 1] = {};  // per-bin decide timing (opts.profile_bins)
@@ -964,6 +965,8 @@ template <class T>
 int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala_result* out) {
   using tt = typename T::tot_t;
   hipStream_t s = ctx->stream;
+  ctx->have_result = false;  // fin is overwritten
+  ctx->have_batch_result = false;
   gala_result& R = ctx->res;
   std::memset(&R, 0, sizeof R);
   R.n_vertices = ctx->n0;
@@ -1095,9 +1098,7 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
   (void)hipEventElapsedTime(&tt_ms, ctx->ev[4], ctx->ev[3]);
   R.t_total_ms = tt_ms;
   R.modularity = cur_mod;
-  ctx->comm_host.resize(ctx->n0);
-  HIPOK(hipMemcpy(ctx->comm_host.data(), ctx->fin, sizeof(int32_t) * ctx->n0, hipMemcpyDeviceToHost));
-  ctx->have_result = true;
+  ctx->have_result = true;  // the composed partition stays in fin until gala_get_communities
   if (out) *out = R;
   return GALA_OK;
 }
@@ -1265,7 +1266,7 @@ int batch_upload(gala_ctx* ctx, int32_t G, const int32_t* vstart, int32_t n, con
   for (int k = 0; k < 2; k++) { BOWN(ctx->b_vl[k], N); BOWN(ctx->b_lbs[k], NBINS + 1); }
   BOWN(ctx->b_vlive, N);
   if (!ctx->b_host) {
-    hipError_t e = hipHostMalloc((void**)&ctx->b_host, 4 * sizeof(int32_t), hipHostMallocDefault);
+    hipError_t e = hipHostMalloc((void**)&ctx->b_host, 4 * sizeof(int32_t), hipHostMallocDefault);  // [0] [1] scalars, [2] [3] per-iteration live counts
     if (e != hipSuccess) { ctx->last_hip = e; free_batch(ctx); return GALA_E_OOM; }
   }
   hipStream_t s = ctx->stream;
@@ -1287,6 +1288,8 @@ int batch_upload(gala_ctx* ctx, int32_t G, const int32_t* vstart, int32_t n, con
 int run_batch(gala_ctx* ctx, gala_batch_result* out) {
   using T = IntW;
   hipStream_t s = ctx->stream;
+  ctx->have_result = false;  // fin is overwritten
+  ctx->have_batch_result = false;
   gala_batch_result R{};
   R.n_graphs = ctx->b_G;
   const int32_t G = ctx->b_G;
@@ -1306,6 +1309,7 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
       for (int32_t v = ctx->b_vstart[gg]; v < ctx->b_vstart[gg + 1]; v++)
         ctx->b_comm_local[v] = v - ctx->b_vstart[gg];
     }
+    ctx->b_comm_ready = true;
     ctx->have_batch_result = true;
     if (out) *out = R;
     return GALA_OK;
@@ -1367,17 +1371,24 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     // stopped is inert), which shrinks as graphs converge; the list is
     // re-compacted after every sweep (fixed buffers, so a captured body
     // replays correctly).
-    auto body = [&](bool first) {
+    // The long-row bins' decide and In update run on the second stream
+    // beside the others (forked and joined inside a captured body too).
+    const bool conc = !ctx->opts.serial_bins;
+    auto body = [&](bool first) -> int {
       launch_batch_reset(ctx->ctr, m2max, nlive, s);
       launch_build_cv<T>(st, g.n, s, LA, nA);
-      for (int k = 0; k < NBINS; k++) {
-        if (first) launch_decide_first<T>(k, g, sdec, ctx->front, ctx->bstart_front, caps[k], m2max, s, ctx->fvar[k]);
-        else launch_decide<T>(k, g, sdec, ctx->front, ctx->bstart_front, caps[k], m2max, s, ctx->dvar[k]);
-      }
-      launch_apply<T>(st, g.n, m2max, s, LA, nA);
+      int r = over_bins(ctx, conc, [&](int k, hipStream_t sb) {
+        if (first) launch_decide_first<T>(k, g, sdec, ctx->front, ctx->bstart_front, caps[k], m2max, sb, ctx->fvar[k]);
+        else launch_decide<T>(k, g, sdec, ctx->front, ctx->bstart_front, caps[k], m2max, sb, ctx->dvar[k]);
+      });
+      if (r) return r;
       if constexpr (incr) {
-        for (int k = 0; k < NBINS; k++) launch_weight_moved<T>(k, g, st, LA, BA, caps[k], s, ctx->wvar[k]);
+        r = over_bins(
+            ctx, conc, [&](int k, hipStream_t sb) { launch_weight_moved<T>(k, g, st, LA, BA, caps[k], sb, ctx->wvar[k]); },
+            [&](hipStream_t s2) { launch_apply<T>(st, g.n, m2max, s2, LA, nA); });
+        if (r) return r;
       } else {
+        launch_apply<T>(st, g.n, m2max, s, LA, nA);
         for (int k = 0; k < NBINS; k++) launch_weight<T>(k, g, st, LA, BA, caps[k], true, s, ctx->wvar[k]);
       }
       launch_prune<T>(st, g.n, m2max, s, LA, nA);
@@ -1391,6 +1402,7 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
       copy_i32(BB, BA, NBINS + 1, s);
       partition_bins(1, LA, BA, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp, s,
                      nA);
+      return GALA_OK;
     };
     auto live_after = [&](int32_t* out_live) -> int {
       HIPOK(hipMemcpyAsync(ctx->b_host, nlive, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1402,29 +1414,53 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     if ((rc = live_after(&live))) return rc;
     if (round < 64) R.round_open[round] = live;
     hipGraphExec_t exec = nullptr;
-    int it = 0;
-    while (live > 0 && it < max_it) {
-      it++;
-      if (it == 1 || !graphs) {
-        body(it == 1 && unique_rows);
+    // Iteration k + 1 is queued before the live count of iteration k is read
+    // (a graph that stopped is inert, so the one iteration queued past the
+    // last one changes nothing): the host waits one iteration behind and the
+    // GPU does not idle on the round trip.  Each queued iteration is followed
+    // by the copy of its live count into its own pinned slot and an event.
+    auto launch_it = [&](int k) -> int {
+      if (k == 1 || !graphs) {
+        if (int r = body(k == 1 && unique_rows)) return r;
       } else {
         if (!exec) {
           hipGraph_t gr = nullptr;
           HIPOK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-          body(false);
+          const int rb = body(false);
           hipError_t e = hipStreamEndCapture(s, &gr);
+          if (rb) { if (gr) (void)hipGraphDestroy(gr); return rb; }
           if (e != hipSuccess) { ctx->last_hip = e; return GALA_E_HIP; }
           e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
           (void)hipGraphDestroy(gr);
           if (e != hipSuccess) { ctx->last_hip = e; return GALA_E_HIP; }
         }
-        hipError_t e = hipGraphLaunch(exec, s);
-        if (e != hipSuccess) { ctx->last_hip = e; (void)hipGraphExecDestroy(exec); return GALA_E_HIP; }
+        HIPOK(hipGraphLaunch(exec, s));
       }
-      rc = live_after(&live);
-      if (rc) { if (exec) (void)hipGraphExecDestroy(exec); return rc; }
+      HIPOK(hipMemcpyAsync(ctx->b_host + 2 + (k & 1), nlive, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      HIPOK(hipEventRecord(ctx->evr[k & 1], s));
+      return GALA_OK;
+    };
+    int it = 0;     // iterations queued
+    int known = 0;  // iterations whose live count has been read (the live count of iteration `known` is `live`)
+    if (live > 0 && max_it > 0) {
+      rc = launch_it(++it);
+      while (!rc) {
+        if (it < max_it) rc = launch_it(++it);  // the live count of iteration it - 2 was positive
+        if (rc) break;
+        known++;
+        const hipError_t e = hipEventSynchronize(ctx->evr[known & 1]);
+        if (e != hipSuccess) {
+          ctx->last_hip = e;
+          rc = GALA_E_HIP;
+          break;
+        }
+        live = ctx->b_host[2 + (known & 1)];
+        if (live == 0 || known >= max_it) break;
+      }
     }
     if (exec) (void)hipGraphExecDestroy(exec);
+    if (rc) return rc;
+    it = known;  // iterations that did work (a queued one past them is inert)
     HIPOK(hipEventRecord(ctx->ev[6], s));
     // contraction of the whole union under the accepted assignments
     int32_t C = 0;
@@ -1468,8 +1504,8 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
   if (rc) return rc;
   R.swept_vertices = (int64_t)c.swept;
   R.swept_edges = (int64_t)c.swept_edges;
-  // per-graph results; communities renumbered within each graph (the final
-  // union's vertices are grouped by graph in ascending order)
+  // per-graph results (the communities stay in fin, renumbered within each
+  // graph when they are requested)
   fill<int32_t>(ctx->b_ncomm, G, 0, s);
   launch_batch_count(ctx->b_gid[gcur], g.n, ctx->b_ncomm, s);
   ctx->b_q.resize(G); ctx->b_nc.resize(G); ctx->b_rd.resize(G); ctx->b_it.resize(G);
@@ -1477,16 +1513,9 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
   HIPOK(hipMemcpyAsync(ctx->b_nc.data(), ctx->b_ncomm, sizeof(int32_t) * G, hipMemcpyDeviceToHost, s));
   HIPOK(hipMemcpyAsync(ctx->b_rd.data(), ctx->b_rounds, sizeof(int32_t) * G, hipMemcpyDeviceToHost, s));
   HIPOK(hipMemcpyAsync(ctx->b_it.data(), ctx->b_tot_iters, sizeof(int32_t) * G, hipMemcpyDeviceToHost, s));
-  ctx->b_comm_local.resize(ctx->n0);
-  HIPOK(hipMemcpyAsync(ctx->b_comm_local.data(), ctx->fin, sizeof(int32_t) * ctx->n0, hipMemcpyDeviceToHost, s));
   HIPOK(hipStreamSynchronize(s));
-  int64_t first = 0;
-  for (int32_t gg = 0; gg < G; gg++) {
-    for (int32_t v = ctx->b_vstart[gg]; v < ctx->b_vstart[gg + 1]; v++) ctx->b_comm_local[v] -= (int32_t)first;
-    first += ctx->b_nc[gg];
-  }
+  ctx->b_comm_ready = false;  // read back on request (gala_batch_communities)
   ctx->have_batch_result = true;
-  ctx->have_result = false;
   if (out) *out = R;
   return GALA_OK;
 }
@@ -1538,6 +1567,8 @@ int gala_create(gala_ctx** out, int device, const gala_opts* opts) {
   }
   for (auto& e : ctx->evf)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { gala_destroy(ctx); return GALA_E_HIP; }
+  for (auto& e : ctx->evr)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { gala_destroy(ctx); return GALA_E_HIP; }
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) { gala_destroy(ctx); return GALA_E_HIP; }
   for (auto& e : ctx->bev)
@@ -1562,6 +1593,8 @@ void gala_destroy(gala_ctx* ctx) {
   for (auto& e : ctx->bev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->evf)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->evr)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1779,6 +1812,19 @@ int gala_batch_graph_results(gala_ctx* ctx, double* modularity, int32_t* n_commu
 
 int gala_batch_communities(gala_ctx* ctx, int32_t* comm) {
   if (!ctx || !comm || !ctx->have_batch_result) return GALA_E_INVALID;
+  if (!ctx->b_comm_ready) {
+    // the composed union partition, renumbered within each graph (the final
+    // union's vertices are grouped by graph in ascending order)
+    (void)hipSetDevice(ctx->device);
+    ctx->b_comm_local.resize(ctx->n0);
+    HIPOK(hipMemcpy(ctx->b_comm_local.data(), ctx->fin, sizeof(int32_t) * ctx->n0, hipMemcpyDeviceToHost));
+    int64_t first = 0;
+    for (int32_t gg = 0; gg < ctx->b_G; gg++) {
+      for (int32_t v = ctx->b_vstart[gg]; v < ctx->b_vstart[gg + 1]; v++) ctx->b_comm_local[v] -= (int32_t)first;
+      first += ctx->b_nc[gg];
+    }
+    ctx->b_comm_ready = true;
+  }
   std::memcpy(comm, ctx->b_comm_local.data(), sizeof(int32_t) * ctx->b_comm_local.size());
   return GALA_OK;
 }
@@ -1791,7 +1837,8 @@ int gala_partial_result(gala_ctx* ctx, gala_result* out) {
 
 int gala_get_communities(gala_ctx* ctx, int32_t* out_n0) {
   if (!ctx || !out_n0 || !ctx->have_result) return GALA_E_INVALID;
-  std::memcpy(out_n0, ctx->comm_host.data(), sizeof(int32_t) * ctx->comm_host.size());
+  (void)hipSetDevice(ctx->device);
+  HIPOK(hipMemcpy(out_n0, ctx->fin, sizeof(int32_t) * ctx->n0, hipMemcpyDeviceToHost));
   return GALA_OK;
 }
 

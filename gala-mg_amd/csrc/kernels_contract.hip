@@ -412,15 +412,27 @@ __global__ __launch_bounds__(NT) void k_merge_mid(const int64_t* __restrict__ ec
     const int tsz = 1 << lg;
     for (int t = threadIdx.x; t < tsz; t += NT) { h_k[t] = 0; h_v[t] = 0; }
     __syncthreads();
-    for (int j = threadIdx.x; j < L; j += NT) {
-      const int key = skey[base + j];
-      const acc w = sval[base + j];
-      unsigned p = chash(key, lg);
-      for (;;) {
-        const int k = atomicCAS(&h_k[p], 0, key + 1);
-        if (k == 0 || k == key + 1) { atomicAdd(&h_v[p], w); break; }
-        p = (p + 1) & (tsz - 1);
-      }
+    // L <= LMAX = NT * U: every entry of a thread is loaded before the
+    // inserts; equal keys of a wave are combined first
+    constexpr int U = LMAX / NT;
+    int key[U];
+    acc w[U];
+#pragma unroll
+    for (int r = 0; r < U; r++) {
+      const long long j = threadIdx.x + r * NT;
+      key[r] = j < L ? skey[base + j] : -1;
+      w[r] = j < L ? sval[base + j] : acc(0);
+    }
+#pragma unroll
+    for (int r = 0; r < U; r++) {
+      wave_combine_insert<acc>(key[r], w[r], key[r] >= 0, [&](int kk, acc ww) {
+        unsigned p = chash(kk, lg);
+        for (;;) {
+          const int k = atomicCAS(&h_k[p], 0, kk + 1);
+          if (k == 0 || k == kk + 1) { atomicAdd(&h_v[p], ww); break; }
+          p = (p + 1) & (tsz - 1);
+        }
+      });
     }
     __syncthreads();
     int cnt = 0;
@@ -479,25 +491,36 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
     for (int t = threadIdx.x; t < LT; t += NT) { s_k[t] = 0; s_v[t] = 0; }
     if (threadIdx.x == 0) s_ovf = 0;
     __syncthreads();
-    for (long long j = threadIdx.x; j < L; j += NT) {
-      const int key = skey[base + j];
-      const acc w = sval[base + j];
-      unsigned p = chash(key, LLG);
-      bool placed = false;
-      for (int probe = 0; probe < PMAX; probe++) {
-        const int k = atomicCAS(&s_k[p], 0, key + 1);
-        if (k == 0 || k == key + 1) { atomicAdd(&s_v[p], w); placed = true; break; }
-        p = (p + 1) & (LT - 1);
-      }
-      if (!placed) {
-        s_ovf = 1;
-        unsigned q = chash(key, glg);
-        for (;;) {
-          const int k = atomicCAS(&keys[q], 0, key + 1);
-          if (k == 0 || k == key + 1) { atomicAdd(&vals[q], w); break; }
-          q = (q + 1) & (unsigned)(gsz - 1);
+    // L <= LT = NT * PER: every entry of a thread is loaded before the
+    // inserts; equal keys of a wave are combined first
+    int key[PER];
+    acc w[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const long long j = threadIdx.x + r * NT;
+      key[r] = j < L ? skey[base + j] : -1;
+      w[r] = j < L ? sval[base + j] : acc(0);
+    }
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      wave_combine_insert<acc>(key[r], w[r], key[r] >= 0, [&](int kk, acc ww) {
+        unsigned p = chash(kk, LLG);
+        bool placed = false;
+        for (int probe = 0; probe < PMAX; probe++) {
+          const int k = atomicCAS(&s_k[p], 0, kk + 1);
+          if (k == 0 || k == kk + 1) { atomicAdd(&s_v[p], ww); placed = true; break; }
+          p = (p + 1) & (LT - 1);
         }
-      }
+        if (!placed) {
+          s_ovf = 1;
+          unsigned q = chash(kk, glg);
+          for (;;) {
+            const int k = atomicCAS(&keys[q], 0, kk + 1);
+            if (k == 0 || k == kk + 1) { atomicAdd(&vals[q], ww); break; }
+            q = (q + 1) & (unsigned)(gsz - 1);
+          }
+        }
+      });
     }
     __syncthreads();
     const bool ovf = s_ovf != 0;
@@ -570,31 +593,53 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
 // packed offset of the tile, len = pairs, first/nch = the row's first tile
 // and tile count.
 // ---------------------------------------------------------------------------
+// A chunk holds at most LT/2 = NT * U entries, so the table never fills.
+// Every entry of a thread is in registers before the inserts, and the next
+// chunk's entries are loaded before this chunk's table is read out, so a
+// workgroup (one per CU: the table takes 128 KB of LDS) overlaps its HBM
+// reads with its LDS work.  Equal keys of a wave are combined before the LDS
+// atomics (a dense row's entries often carry only a few distinct keys); the
+// table is cleared as it is read out.
 template <int LT, class T>
 __global__ __launch_bounds__(1024) void k_dense_agg(const DenseChunk* __restrict__ chunks, int nch,
                                                     const int64_t* __restrict__ eoff, int32_t* __restrict__ skey,
                                                     typename T::acc_t* __restrict__ sval, int32_t* __restrict__ ccnt) {
   using acc = typename T::acc_t;
-  constexpr int NT = 1024, LLG = __builtin_ctz(LT);
+  constexpr int NT = 1024, LLG = __builtin_ctz(LT), U = LT / 2 / NT;
   __shared__ int s_k[LT];
   __shared__ acc s_v[LT];
   __shared__ int s_sh[17];
-  for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
-    const DenseChunk cc = chunks[ch];
-    const int64_t base = eoff[cc.c] + cc.start;
-    for (int t = threadIdx.x; t < LT; t += NT) { s_k[t] = 0; s_v[t] = 0; }
-    __syncthreads();
-    // a chunk holds at most LT/2 entries, so the table never fills
-    for (int j = threadIdx.x; j < cc.len; j += NT) {
-      const int key = skey[base + j];
-      const acc w = sval[base + j];
-      unsigned p = chash(key, LLG);
-      for (;;) {
-        const int k = atomicCAS(&s_k[p], 0, key + 1);
-        if (k == 0 || k == key + 1) { atomicAdd(&s_v[p], w); break; }
-        p = (p + 1) & (LT - 1);
-      }
+  for (int t = threadIdx.x; t < LT; t += NT) { s_k[t] = 0; s_v[t] = 0; }
+  int key[U];
+  acc w[U];
+  int ch = blockIdx.x;
+  int64_t base = 0;
+  auto load = [&](int c2, int64_t& b2) {
+    const DenseChunk cc = chunks[c2];
+    b2 = eoff[cc.c] + cc.start;
+#pragma unroll
+    for (int r = 0; r < U; r++) {
+      const int j = threadIdx.x + r * NT;
+      key[r] = j < cc.len ? skey[b2 + j] : -1;
+      w[r] = j < cc.len ? sval[b2 + j] : acc(0);
     }
+  };
+  if (ch < nch) load(ch, base);
+  __syncthreads();  // table cleared
+  for (; ch < nch; ch += gridDim.x) {
+#pragma unroll
+    for (int r = 0; r < U; r++) {
+      wave_combine_insert<acc>(key[r], w[r], key[r] >= 0, [&](int kk, acc ww) {
+        unsigned p = chash(kk, LLG);
+        for (;;) {
+          const int k = atomicCAS(&s_k[p], 0, kk + 1);
+          if (k == 0 || k == kk + 1) { atomicAdd(&s_v[p], ww); break; }
+          p = (p + 1) & (LT - 1);
+        }
+      });
+    }
+    const int64_t cbase = base;
+    if (ch + (int)gridDim.x < nch) load(ch + gridDim.x, base);  // in flight during the read-out
     __syncthreads();
     int cnt = 0;
     for (int t = threadIdx.x; t < LT; t += NT) cnt += s_k[t] != 0;
@@ -603,9 +648,11 @@ __global__ __launch_bounds__(1024) void k_dense_agg(const DenseChunk* __restrict
     for (int t = threadIdx.x; t < LT; t += NT) {
       const int k = s_k[t];
       if (k) {
-        skey[base + q] = k - 1;
-        sval[base + q] = s_v[t];
+        skey[cbase + q] = k - 1;
+        sval[cbase + q] = s_v[t];
         q++;
+        s_k[t] = 0;
+        s_v[t] = acc(0);
       }
     }
     if (threadIdx.x == 0) ccnt[ch] = D;
