@@ -174,12 +174,13 @@ def learned_leg(gala, args):
         log("learned leg skipped: no trained model")
         return None
     params, tau = TF.load(TF.DEFAULT_MODEL)
+    settings = TF.load_settings(TF.DEFAULT_MODEL)
     per_seed = []
     for seed in (1001, 1002, 1003):
         g = gala.gen_chung_lu(args.n, 2.1, 17.35, 15000.0, seed)
         res = {}
         for mode in (0, 1):
-            eng = gala.Engine(device=0, frontier_mode=mode, learned_tau=tau)
+            eng = gala.Engine(device=0, frontier_mode=mode, **settings)
             if mode:
                 eng.set_mlp(params)
             eng.set_graph(g)
@@ -191,7 +192,7 @@ def learned_leg(gala, args):
         per_seed.append({"seed": seed, "plain_ms": pms, "learned_ms": lms, "speedup_vs_plain": pms / lms,
                          "plain_q": rp["modularity"], "learned_q": rl["modularity"],
                          "dQ": rl["modularity"] - rp["modularity"], "learned_sweeps": rl["learned_sweeps"],
-                         "certify_sweeps": rl["certify_sweeps"],
+                         "certify_sweeps": rl["certify_sweeps"], "certify_empty": rl["certify_empty"],
                          "kept_fraction": (rl["learned_kept"] / rl["learned_frontier"]) if rl["learned_frontier"] else None,
                          "swept_edges": rl["swept_edges"], "plain_swept_edges": rp["swept_edges"],
                          "phase_ms": {"decide": rl["t_decide_ms"], "update": rl["t_update_ms"],
@@ -201,7 +202,7 @@ def learned_leg(gala, args):
         log(f"learned seed {seed}: {lms:.2f} ms vs plain {pms:.2f} ms ({pms / lms:.2f}x) "
             f"dQ={rl['modularity'] - rp['modularity']:+.6f}")
     sp = [x["speedup_vs_plain"] for x in per_seed]
-    return {"tau": tau, "seeds": per_seed, "speedup_vs_plain": float(np.mean(sp)),
+    return {"tau": tau, "settings": settings, "seeds": per_seed, "speedup_vs_plain": float(np.mean(sp)),
             "ms_per_step": float(np.mean([x["learned_ms"] for x in per_seed])),
             "plain_ms": float(np.mean([x["plain_ms"] for x in per_seed])),
             "dQ": max((x["dQ"] for x in per_seed), key=abs)}

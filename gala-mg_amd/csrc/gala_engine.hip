@@ -660,7 +660,9 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   using tt = typename T::tot_t;
   hipStream_t s = ctx->stream;
   const double m2d = (double)m2;
-  const bool learned = ctx->opts.frontier_mode == 1 && ctx->have_mlp;
+  // learned sweeps in the rounds whose graph is large enough for the filter
+  // to save more than it costs
+  const bool learned = ctx->opts.frontier_mode == 1 && ctx->have_mlp && g.n >= ctx->opts.learned_min_vertices;
   const bool logging = ctx->logger != nullptr;
   const int track = (learned || logging) ? 1 : 0;
   int32_t caps[NBINS], nitems = 0;
@@ -743,44 +745,13 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     HIPOK(hipEventRecord(ctx->ev[1], s));
     return GALA_OK;
   };
-  // Tot/size update, In of the invalidated rows, prune, next frontier and the
-  // counters; accumulates the sweep's work and timings
-  auto update = [&](DevState<T>& sp, bool first, Counters* out) -> int {
-    // Tot/size from the movers (on the second stream when concurrent) beside
-    // In: In changes only where a vertex or one of its neighbours moved, so
-    // only the movers' rows are read.  (Recomputing every In after the first
-    // sweep of a round, when most vertices move, measured slower.)
-    if (incr) {
-      int r = over_bins(
-          ctx, conc,
-          [&](int b, hipStream_t sb) {
-            launch_weight_moved<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], sb, ctx->wvar[b]);
-          },
-          [&](hipStream_t s2) { launch_apply<T>(sp, g.n, m2, s2); });
-      if (r) return r;
-    } else {
-      launch_apply<T>(sp, g.n, m2, s);
-    }
-    if (!incr)
-      for (int b = 0; b < NBINS; b++)
-        launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s, ctx->wvar[b]);
-    launch_prune<T>(sp, g.n, m2, s);
-    partition_bins(1, ctx->order, ctx->bstart_full, ctx->binid, ctx->active, nitems, fnext, bnext, ctx->part_tmp,
-                   s);
-    HIPOK(hipEventRecord(ctx->ev[2], s));
-    Counters c0;
-    int r = read_counters<T>(ctx, out, q0_pending ? &c0 : nullptr);
-    if (r) return r;
-    if (q0_pending) {
-      q0_pending = false;
-      qprev = q_from(c0, m2d);
-      if (icb) icb(user, round, 0, qprev, 0, nullptr, g.n);
-    }
-    float a = 0, b2 = 0;
+  // the decide time of the sweep (events ev[0] -> ev[1]) and its work, from
+  // the sweep's counters
+  auto account_decide = [&](bool first, const Counters& cs) {
+    const Counters* out = &cs;
+    float a = 0;
     (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
-    (void)hipEventElapsedTime(&b2, ctx->ev[1], ctx->ev[2]);
     t_dm += a;
-    t_upd += b2;
     ctx->res.swept_vertices += (int64_t)out->swept;
     ctx->res.swept_edges += (int64_t)out->swept_edges;
     if (!g.weighted()) {
@@ -811,6 +782,43 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
         ctx->res.bin_launches[b] += launched;
       }
     }
+  };
+  // Tot/size update, In of the invalidated rows, prune, next frontier and the
+  // counters; accumulates the sweep's work and timings
+  // fr / bs: the vertices the sweep decided (only they can have moved)
+  auto update = [&](DevState<T>& sp, bool first, const int32_t* fr, const int32_t* bs, Counters* out) -> int {
+    // Tot/size from the movers (on the second stream when concurrent) beside
+    // In: In changes only where a vertex or one of its neighbours moved, so
+    // only the movers' rows are read.  (Recomputing every In after the first
+    // sweep of a round, when most vertices move, measured slower.)
+    if (incr) {
+      int r = over_bins(
+          ctx, conc,
+          [&](int b, hipStream_t sb) { launch_weight_moved<T>(b, g, st, fr, bs, caps[b], sb, ctx->wvar[b]); },
+          [&](hipStream_t s2) { launch_apply<T>(sp, g.n, m2, s2, nullptr, nullptr, fr, bs + NBINS); });
+      if (r) return r;
+    } else {
+      launch_apply<T>(sp, g.n, m2, s, nullptr, nullptr, fr, bs + NBINS);
+    }
+    if (!incr)
+      for (int b = 0; b < NBINS; b++)
+        launch_weight<T>(b, g, st, ctx->order, ctx->bstart_full, caps[b], true, s, ctx->wvar[b]);
+    launch_prune<T>(sp, g.n, m2, s);
+    partition_bins(1, ctx->order, ctx->bstart_full, ctx->binid, ctx->active, nitems, fnext, bnext, ctx->part_tmp,
+                   s);
+    HIPOK(hipEventRecord(ctx->ev[2], s));
+    Counters c0;
+    int r = read_counters<T>(ctx, out, q0_pending ? &c0 : nullptr);
+    if (r) return r;
+    if (q0_pending) {
+      q0_pending = false;
+      qprev = q_from(c0, m2d);
+      if (icb) icb(user, round, 0, qprev, 0, nullptr, g.n);
+    }
+    float b2 = 0;
+    (void)hipEventElapsedTime(&b2, ctx->ev[1], ctx->ev[2]);
+    t_upd += b2;
+    account_decide(first, *out);
     std::swap(fcur, fnext);
     std::swap(bcur, bnext);
     return GALA_OK;
@@ -825,7 +833,8 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     const bool first = it == 1 && unique_rows;  // singleton partition: no aggregation needed
     const int32_t nf0 = nfront;
     if (lsweep) {
-      launch_mlp_filter<T>(g, st, fcur, nfront, it, ctx->mlp_prm, ctx->mlp_w1b, tau, ctx->keep, s);
+      launch_mlp_filter<T>(g, st, fcur, nfront, it, ctx->mlp_prm, ctx->mlp_w1b, tau, ctx->opts.learned_min_degree,
+                           ctx->keep, s);
       partition_bins(1, fcur, bcur, ctx->binid, ctx->keep, nfront, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
       // pre-sweep Tot and size, read again if the sweep has to be certified
       HIPOK(hipMemcpyAsync(ctx->tot_snap, st.Tot, tsz * g.n, hipMemcpyDeviceToDevice, s));
@@ -844,7 +853,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       HIPOK(hipStreamSynchronize(s));
       ctx->logger(ctx->logger_user, round, it, nfront, hfeat.data(), hlab.data());
     }
-    if ((rc = update(st, first, &c))) return rc;
+    if ((rc = update(st, first, lsweep ? ctx->front2 : fcur, lsweep ? ctx->bstart2 : bcur, &c))) return rc;
     nfront = (int32_t)c.active;
     int64_t moved = (int64_t)c.moved;
     q = q_from(c, m2d);
@@ -867,12 +876,22 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       sc.Tot = (tt*)ctx->tot_snap;
       sc.size = ctx->size_snap;
       if ((rc = decide(sc, ctx->front2, ctx->bstart2, false, false))) return rc;
-      DevState<T> sp = st;
-      sp.churn = nullptr;  // keep the churn of the learned sweep for the features
-      if ((rc = update(sp, false, &c))) return rc;
-      nfront = (int32_t)c.active;
-      moved += (int64_t)c.moved;
-      q = q_from(c, m2d);
+      Counters cc;
+      if ((rc = read_counters<T>(ctx, &cc))) return rc;
+      if (cc.moved == 0) {
+        // the rest of the frontier moved nobody: the plain sweep from this
+        // state is the learned one, whose Q already stalled (its update stands;
+        // the round ends on it, reverted)
+        ctx->res.certify_empty++;
+        account_decide(false, cc);
+      } else {
+        DevState<T> sp = st;
+        sp.churn = nullptr;  // keep the churn of the learned sweep for the features
+        if ((rc = update(sp, false, ctx->front2, ctx->bstart2, &c))) return rc;
+        nfront = (int32_t)c.active;
+        moved += (int64_t)c.moved;
+        q = q_from(c, m2d);
+      }
     }
     if (icb) {
       const int32_t* vec = nullptr;

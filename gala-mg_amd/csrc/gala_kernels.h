@@ -178,8 +178,12 @@ void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const i
 // Tot and size take the sweep's moves (from the {cur, prev} pairs, tile sums
 // per community), then sums of Tot^2, min Tot, non-empty count.
 template <class T>
+// Tot/size take the deltas of the movers among dlist[0 .. *dcnt) (the swept
+// vertices: only they can have moved; every vertex when null, else vlist),
+// then sum Tot^2, min Tot and the non-empty count over the communities of
+// vlist[0 .. *vcnt) (all n when null).
 void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s, const int32_t* vlist = nullptr,
-                  const int32_t* vcnt = nullptr);
+                  const int32_t* vcnt = nullptr, const int32_t* dlist = nullptr, const int32_t* dcnt = nullptr);
 
 // In[v] over order[bstart[b] .. bstart[b+1]) (only the dirty vertices when only_dirty).
 // variant 1: pipelined kernels for bins 0-4 (same result).
@@ -223,7 +227,7 @@ void launch_batch_reset(Counters* c, long long min_init, int32_t* nlive, hipStre
 // update (no row rescans).
 template <class T>
 void launch_mlp_filter(const DevGraph<T>& g, DevState<T>& st, const int32_t* front, int nfront, int it,
-                       const float* prm, const short* w1b, float tau, uint8_t* keep, hipStream_t s);
+                       const float* prm, const short* w1b, float tau, int dmin, uint8_t* keep, hipStream_t s);
 template <class T>
 void launch_features(const DevGraph<T>& g, DevState<T>& st, const int32_t* front, int nfront, int it, float* out,
                      hipStream_t s);

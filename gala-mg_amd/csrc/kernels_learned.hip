@@ -92,7 +92,7 @@ __device__ __forceinline__ void mlp_logits(const f32x4 acc[4], const float* __re
 template <class T>
 __global__ __launch_bounds__(256) void k_mlp_filter(DevGraph<T> g, DevState<T> st, const int32_t* __restrict__ front,
                                                      int nfront, int it, const float* __restrict__ prm,
-                                                     const short* __restrict__ w1b, float tau,
+                                                     const short* __restrict__ w1b, float tau, int dmin,
                                                      uint8_t* __restrict__ keep) {
   const int lane = threadIdx.x & 63;
   const int nw = gridDim.x * (blockDim.x >> 6);
@@ -116,7 +116,10 @@ __global__ __launch_bounds__(256) void k_mlp_filter(DevGraph<T> g, DevState<T> s
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int rr = base + 4 * (lane >> 4) + i;
-        if (rr < nfront) keep[front[rr]] = sigmoidf(z[i]) > tau ? 1 : 0;
+        if (rr < nfront) {
+          const int v = front[rr];
+          keep[v] = (sigmoidf(z[i]) > tau || g.end[v] - g.off[v] < dmin) ? 1 : 0;
+        }
       }
     }
   }
@@ -301,10 +304,11 @@ __global__ void k_w1_bf16(const float* __restrict__ prm, short* __restrict__ w1b
 // ---------------------------------------------------------------------------
 template <class T>
 void launch_mlp_filter(const DevGraph<T>& g, DevState<T>& st, const int32_t* front, int nfront, int it,
-                       const float* prm, const short* w1b, float tau, uint8_t* keep, hipStream_t s) {
+                       const float* prm, const short* w1b, float tau, int dmin, uint8_t* keep, hipStream_t s) {
   if (nfront <= 0) return;
   const int waves = (nfront + 15) / 16;
-  k_mlp_filter<T><<<std::min((waves + 3) / 4, 8192), 256, 0, s>>>(g, st, front, nfront, it, prm, w1b, tau, keep);
+  k_mlp_filter<T><<<std::min((waves + 3) / 4, 8192), 256, 0, s>>>(g, st, front, nfront, it, prm, w1b, tau, dmin,
+                                                                   keep);
 }
 
 template <class T>
@@ -341,9 +345,9 @@ void launch_w1_bf16(const float* prm, short* w1b, hipStream_t s) {
 }
 
 template void launch_mlp_filter<IntW>(const DevGraph<IntW>&, DevState<IntW>&, const int32_t*, int, int, const float*,
-                                      const short*, float, uint8_t*, hipStream_t);
+                                      const short*, float, int, uint8_t*, hipStream_t);
 template void launch_mlp_filter<WideW>(const DevGraph<WideW>&, DevState<WideW>&, const int32_t*, int, int, const float*,
-                                      const short*, float, uint8_t*, hipStream_t);
+                                      const short*, float, int, uint8_t*, hipStream_t);
 template void launch_features<IntW>(const DevGraph<IntW>&, DevState<IntW>&, const int32_t*, int, int, float*,
                                     hipStream_t);
 template void launch_features<WideW>(const DevGraph<WideW>&, DevState<WideW>&, const int32_t*, int, int, float*,

@@ -57,7 +57,8 @@ class Opts(ctypes.Structure):
                 ("dump_rounds", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("frontier_mode", ctypes.c_int), ("learned_tau", ctypes.c_float),
                 ("profile_bins", ctypes.c_int), ("no_hip_graphs", ctypes.c_int),
-                ("serial_bins", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+                ("serial_bins", ctypes.c_int), ("learned_min_degree", ctypes.c_int),
+                ("learned_min_vertices", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 class Result(ctypes.Structure):
@@ -81,7 +82,8 @@ class Result(ctypes.Structure):
                 ("bin_first_ms", ctypes.c_double * 9), ("bin_launches", ctypes.c_int32 * 9),
                 ("bin_first_launches", ctypes.c_int32 * 9), ("wide", ctypes.c_int32),
                 ("weight_shift", ctypes.c_int32), ("round_vertices", ctypes.c_int32 * 64),
-                ("round_entries", ctypes.c_int64 * 64), ("round_contracted_entries", ctypes.c_int64 * 64)]
+                ("round_entries", ctypes.c_int64 * 64), ("round_contracted_entries", ctypes.c_int64 * 64),
+                ("certify_empty", ctypes.c_int32)]
 
     def to_dict(self):
         r = self.rounds
@@ -108,7 +110,8 @@ class Result(ctypes.Structure):
                     bin_first_launches=list(self.bin_first_launches), wide=self.wide,
                     weight_shift=self.weight_shift, round_vertices=list(self.round_vertices[:r]),
                     round_entries=list(self.round_entries[:r]),
-                    round_contracted_entries=list(self.round_contracted_entries[:r]))
+                    round_contracted_entries=list(self.round_contracted_entries[:r]),
+                    certify_empty=self.certify_empty)
 
 
 ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -395,7 +398,7 @@ class Engine:
 
     def __init__(self, device=0, min_modularity=1e-6, dump_iterations=False, dump_rounds=False,
                  max_rounds=0, max_iterations=0, profile_bins=False, frontier_mode=0, learned_tau=0.5,
-                 hip_graphs=True, serial_bins=False):
+                 hip_graphs=True, serial_bins=False, learned_min_degree=0, learned_min_vertices=0):
         L = lib()
         self.opts = Opts()
         L.gala_default_opts(ctypes.byref(self.opts))
@@ -409,6 +412,8 @@ class Engine:
         self.opts.learned_tau = float(learned_tau)
         self.opts.no_hip_graphs = 0 if hip_graphs else 1
         self.opts.serial_bins = int(serial_bins)
+        self.opts.learned_min_degree = int(learned_min_degree)
+        self.opts.learned_min_vertices = int(learned_min_vertices)
         self.ctx = VP()
         _check(L.gala_create(ctypes.byref(self.ctx), device, ctypes.byref(self.opts)), "gala_create")
 

@@ -54,7 +54,11 @@ typedef struct {
                             captured hipGraph; 1 = launch kernel by kernel instead */
   int serial_bins;       /* single-graph runs launch the long-row degree bins (512+)
                             on a second stream beside the others; 1 = one stream */
-  int reserved[5];
+  int learned_min_degree;/* learned sweeps keep every vertex with fewer row entries
+                            than this (0: the network filters every vertex) */
+  int learned_min_vertices; /* rounds whose graph has fewer vertices run plain
+                               sweeps only (0: every round is filtered) */
+  int reserved[3];
 } gala_opts;
 
 /* Called after every iteration (Iteration:0 is Q before any move).
@@ -121,6 +125,8 @@ typedef struct {
   int32_t round_vertices[64];
   int64_t round_entries[64];
   int64_t round_contracted_entries[64];
+  int32_t certify_empty;  /* of the certify_sweeps, those in which the rest of the
+                             frontier moved nobody (their update pass is skipped) */
 } gala_result;
 
 /* Version string of the library. */

@@ -26,8 +26,10 @@ def trained():
     return p, tau
 
 
-def run(g, mode, params=None, tau=0.5, trace=None):
-    eng = gala.Engine(frontier_mode=mode, learned_tau=tau, dump_iterations=trace is not None)
+def run(g, mode, params=None, tau=0.5, trace=None, **settings):
+    if "learned_tau" in settings:
+        tau = settings.pop("learned_tau")
+    eng = gala.Engine(frontier_mode=mode, learned_tau=tau, dump_iterations=trace is not None, **settings)
     if params is not None:
         eng.set_mlp(params)
     eng.set_graph(g)
@@ -156,6 +158,31 @@ def test_learned_sbm_quality(trained):
     assert abs(rl["modularity"] - rp["modularity"]) <= 0.002
 
 
+@pytest.mark.gpu
+def test_degree_proportional_logs_and_filter_settings():
+    # the power-law training logs are sampled in proportion to the degree (the
+    # work a kept vertex costs): their mean degree exceeds the full logs'; a
+    # learned run that filters only long rows of large rounds keeps every
+    # short-row vertex of the frontier and still ends within 0.002 of plain
+    say("degree-proportional logs")
+    g = gala.gen_chung_lu(50000, 2.1, 17.35, 3000.0, 11)
+    eng = gala.Engine()
+    Xf, yf = TF.collect_logs(eng, [g])
+    Xs, ys = TF.collect_logs(eng, [g], per_sweep=2000, seed=1)
+    eng.close()
+    assert 0 < len(ys) < len(yf)
+    assert TF.degree_of(Xs).mean() > 2 * TF.degree_of(Xf).mean()
+    nobody = np.zeros(gala.MLP_NPARAM, np.float32)
+    nobody[-1] = -30.0  # predicts no mover: only the rows kept by the settings are swept
+    rp, cp = run(g, 0)
+    rl, cl = run(g, 1, nobody, learned_min_degree=16, learned_min_vertices=10000)
+    assert rl["learned_sweeps"] > 0 and 0 < rl["learned_kept"] < rl["learned_frontier"]
+    assert abs(rl["modularity"] - rp["modularity"]) <= 0.002
+    # rounds below the vertex bound run plain sweeps only
+    rb, cb = run(g, 1, nobody, learned_min_vertices=10 ** 9)
+    assert rb["learned_sweeps"] == 0 and rb["modularity"] == rp["modularity"] and np.array_equal(cb, cp)
+
+
 def q_of_partition(g, comm):
     """Modularity of `comm` on an unweighted graph from exact integer sums,
     finished in the order the engine uses: sumIn/m2 - sumTot2/m2^2."""
@@ -170,16 +197,18 @@ def q_of_partition(g, comm):
 
 @pytest.mark.gpu
 def test_committed_model_on_held_out_4M():
-    # the committed network (trained on SBM move logs) on a held-out seed of the
-    # 4.0M-vertex power-law workload: final Q within 0.002 of the plain run and
-    # equal to the modularity of the returned partition
+    # the committed network (trained on move logs of other seeds of the
+    # power-law family) with its stored filter settings, on a held-out seed of
+    # the 4.0M-vertex workload: final Q within 0.002 of the plain run and equal
+    # to the modularity of the returned partition
     if not os.path.exists(TF.DEFAULT_MODEL):
         pytest.skip("no committed model")
-    params, tau = TF.load(TF.DEFAULT_MODEL)
+    params, _ = TF.load(TF.DEFAULT_MODEL)
+    settings = TF.load_settings(TF.DEFAULT_MODEL)
     say("committed model, Chung-Lu 4.0M seed 1001")
     g = gala.gen_chung_lu(4_000_000, 2.1, 17.35, 15000.0, 1001)
     rp, cp = run(g, 0)
-    rl, cl = run(g, 1, params, tau)
+    rl, cl = run(g, 1, params, **settings)
     say(f"  Q plain {rp['modularity']:.6f} learned {rl['modularity']:.6f} "
         f"kept {rl['learned_kept']}/{rl['learned_frontier']} certified {rl['certify_sweeps']}")
     assert rl["learned_sweeps"] > 0
