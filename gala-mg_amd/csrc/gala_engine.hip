@@ -93,6 +93,7 @@ struct gala_ctx {
   int32_t *order = nullptr, *front = nullptr, *fin = nullptr;
   void* cv = nullptr;  // community view, 8 (int) or 16 (wide) bytes per vertex
   int2* cp = nullptr;  // {cur, prev} per vertex (integer path, incremental In)
+  unsigned long long* tsd = nullptr;  // packed Tot/size deltas of a sweep (integer path; zero between sweeps)
   BlockPart* part = nullptr;  // workgroup partials of the O(V) reductions
   int32_t *bstart_full = nullptr, *bstart_front = nullptr;
   int64_t* hub_off = nullptr;
@@ -274,6 +275,8 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool wide) {
   OWN(ctx->order, N); OWN(ctx->front, N); OWN(ctx->fin, N);
   { char* p; OWN(p, N * 16); ctx->cv = p; }
   OWN(ctx->cp, N);
+  OWN(ctx->tsd, N);
+  fill<int64_t>((int64_t*)ctx->tsd, N, 0, ctx->stream);
   OWN(ctx->last_gain, N); OWN(ctx->moved_it, N); OWN(ctx->nbr_moved, N); OWN(ctx->churn, N);
   OWN(ctx->front2, N); OWN(ctx->bstart2, NBINS + 1); OWN(ctx->keep, N);
   OWN(ctx->front3, N); OWN(ctx->bstart3, NBINS + 1); OWN(ctx->size_snap, N);
@@ -539,6 +542,7 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.cv = (typename T::cv_t*)ctx->cv;
   st.part = ctx->part;
   st.cp = ctx->cp;
+  st.tsd = T::WIDE ? nullptr : ctx->tsd;
   return st;
 }
 

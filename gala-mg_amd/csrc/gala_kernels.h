@@ -124,6 +124,11 @@ struct DevState {
   // the commit of a swept vertex writes it, accept_sweep resets it to
   // {cur, cur}, so the incremental In update reads both with one gather
   int2* cp = nullptr;
+  // integer path: each community's Tot and size deltas of a sweep packed in
+  // one word (Tot delta * 2^32 + size delta, two's complement), so a
+  // (community, tile) sum is one global atomic instead of two; k_apply adds
+  // them to Tot and size and leaves the words zero (null: separate atomics)
+  unsigned long long* tsd = nullptr;
 };
 
 // Per-graph state of a batch (one entry per graph; integer path).

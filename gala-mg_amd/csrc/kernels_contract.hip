@@ -86,20 +86,34 @@ __global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ c
       }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < CTS; t += blockDim.x) {
-      const int k = s_key[t];
-      if (k) {
-        atomicAdd(&cnt[k - 1], s_cnt[t]);
-        s_base[t] = atomicAdd(&dsum[k - 1], s_sum[t]);
+    // the tile's communities: every global atomic of a thread is issued before
+    // any of their returned offsets is used (one L2 round trip, not one per slot)
+    constexpr int FL = CTS / 256;
+    int fk[FL];
+    unsigned long long fb[FL];
+#pragma unroll
+    for (int r = 0; r < FL; r++) fk[r] = s_key[threadIdx.x + r * 256];
+#pragma unroll
+    for (int r = 0; r < FL; r++) {
+      const int t = threadIdx.x + r * 256;
+      fb[r] = 0;
+      if (fk[r]) {
+        atomicAdd(&cnt[fk[r] - 1], s_cnt[t]);
+        fb[r] = atomicAdd(&dsum[fk[r] - 1], s_sum[t]);
       }
     }
+#pragma unroll
+    for (int r = 0; r < FL; r++)
+      if (fk[r]) s_base[threadIdx.x + r * 256] = fb[r];
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < CT_PT; r++)
       if (slot[r] >= 0) voff[base + threadIdx.x + r * 256] = (int64_t)(s_base[slot[r]] + loc[r]);
     __syncthreads();
-    for (int t = threadIdx.x; t < CTS; t += blockDim.x) {
-      if (s_key[t]) {
+#pragma unroll
+    for (int r = 0; r < FL; r++) {
+      const int t = threadIdx.x + r * 256;
+      if (fk[r]) {
         s_key[t] = 0;
         s_cnt[t] = 0;
         s_sum[t] = 0;

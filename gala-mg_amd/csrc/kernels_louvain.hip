@@ -2483,7 +2483,12 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
                                         st.hub_fail);  // also clears the flags
       if (resolve)
         k_hub_resolve<T><<<dim3(std::min(cap, 512), 8), 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan);
-      k_hub_part<HUB_LT, T><<<2048, 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, st.hub_fail, m2);
+      // at most cap hubs of at most ceil(dmax / PART_KEYS) parts each: no
+      // workgroup (128 KB of LDS, one per CU) is launched only to find no part
+      int64_t pmax = 16;
+      if (st.hub_dmax >= 0) pmax = std::min<int64_t>(16, std::max<int64_t>(1, (st.hub_dmax + PART_KEYS - 1) / PART_KEYS));
+      const int part_grid = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (int64_t)cap * pmax));
+      k_hub_part<HUB_LT, T><<<part_grid, 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, st.hub_fail, m2);
       // without too-long rows an overflowed hub is decided by the commit itself
       k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2,
                                                               st.hub_fail, long_rows ? 0 : 1);
@@ -2590,8 +2595,18 @@ __global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n, co
     for (int i = threadIdx.x; i < ns; i += blockDim.x) {
       const int p = s_list[i];
       const int c = s_key[p] - 1;
-      if (s_cnt[p]) atomicAdd(&st.size[c], s_cnt[p]);
-      if (s_tot[p]) atomicAdd(&st.Tot[c], s_tot[p]);
+      bool packed = false;
+      if constexpr (!T::WIDE) {
+        if (st.tsd) {  // one atomic for both deltas (|deltas| < 2^31: the product is exact)
+          const long long pk = (long long)s_tot[p] * 4294967296ll + (long long)s_cnt[p];
+          if (pk) atomicAdd(&st.tsd[c], (unsigned long long)pk);
+          packed = true;
+        }
+      }
+      if (!packed) {
+        if (s_cnt[p]) atomicAdd(&st.size[c], s_cnt[p]);
+        if (s_tot[p]) atomicAdd(&st.Tot[c], s_tot[p]);
+      }
       if (st.churn) atomicAdd(&st.churn[c], s_mv[p]);
       s_key[p] = 0;
       s_cnt[p] = 0;
@@ -2621,8 +2636,23 @@ __global__ __launch_bounds__(256) void k_apply(DevState<T> st, int32_t n, typena
   if (vlist) n = *vcnt;  // communities are vertex ids: those of the listed vertices
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int c = vlist ? vlist[i] : i;
-    const tt t = st.Tot[c];
-    const int sz = st.size[c];
+    tt t = st.Tot[c];
+    int sz = st.size[c];
+    if constexpr (!T::WIDE) {
+      if (st.tsd) {  // the sweep's packed deltas (k_tot_delta)
+        const unsigned long long D = st.tsd[c];
+        if (D) {
+          st.tsd[c] = 0;
+          // D = dTot * 2^32 + dsize modulo 2^64 with |dTot|, |dsize| < 2^31: exact
+          const int32_t ds = (int32_t)(uint32_t)D;
+          const int32_t dt = (int32_t)(((long long)D - (long long)ds) / 4294967296ll);
+          t += dt;
+          sz += ds;
+          st.Tot[c] = t;
+          st.size[c] = sz;
+        }
+      }
+    }
     ne += sz > 0;
     const long long ti = (long long)t;
     t2 += (__int128)ti * ti;
