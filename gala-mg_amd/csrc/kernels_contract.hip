@@ -53,9 +53,13 @@ __host__ __device__ __forceinline__ int clog2(long long x) {
 // (per-member atomics on a few hot addresses serialise), and staging needs no
 // atomics at all.
 constexpr int CT = 1024, CTS = 2 * CT, CT_PT = CT / 256;
+// packed (every row-length sum < 2^32): dsum[c] holds count * 2^32 + row-length
+// sum, so a (community, tile) pair costs one global atomic (k_unpack_sums
+// separates them); otherwise cnt and dsum take one atomic each.
 __global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ comm, const int64_t* __restrict__ off,
                                                    const int64_t* __restrict__ end, int32_t n, int32_t* __restrict__ cnt,
-                                                   unsigned long long* __restrict__ dsum, int64_t* __restrict__ voff) {
+                                                   unsigned long long* __restrict__ dsum, int64_t* __restrict__ voff,
+                                                   int packed) {
   __shared__ int s_key[CTS];
   __shared__ int s_cnt[CTS];
   __shared__ unsigned long long s_sum[CTS];
@@ -98,8 +102,12 @@ __global__ __launch_bounds__(256) void k_comm_sums(const int32_t* __restrict__ c
       const int t = threadIdx.x + r * 256;
       fb[r] = 0;
       if (fk[r]) {
-        atomicAdd(&cnt[fk[r] - 1], s_cnt[t]);
-        fb[r] = atomicAdd(&dsum[fk[r] - 1], s_sum[t]);
+        if (packed) {
+          fb[r] = atomicAdd(&dsum[fk[r] - 1], ((unsigned long long)s_cnt[t] << 32) + s_sum[t]) & 0xffffffffull;
+        } else {
+          atomicAdd(&cnt[fk[r] - 1], s_cnt[t]);
+          fb[r] = atomicAdd(&dsum[fk[r] - 1], s_sum[t]);
+        }
       }
     }
 #pragma unroll
@@ -152,6 +160,17 @@ __global__ void k_ecount_comm(const int32_t* __restrict__ cnt, const int32_t* __
     if (cnt[c] > 0) ecnt[nid[c]] = (int64_t)dsum[c];
 }
 
+// packed community sums -> member counts, non-empty flags, row-length sums
+__global__ void k_unpack_sums(unsigned long long* __restrict__ dsum, int32_t n, int32_t* __restrict__ cnt,
+                              uint8_t* __restrict__ flag) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const unsigned long long x = dsum[c];
+    const int32_t k = (int32_t)(x >> 32);
+    cnt[c] = k;
+    flag[c] = k > 0;
+    dsum[c] = x & 0xffffffffull;
+  }
+}
 __global__ void k_nonempty(const int32_t* __restrict__ cnt, int32_t n, uint8_t* __restrict__ flag) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
     flag[c] = cnt[c] > 0;
@@ -973,14 +992,18 @@ int contract_prepare(const DevGraph<T>& g, const int32_t* comm, ContractBufs<T>&
                      hipStream_t s) {
   constexpr long long LT = T::WIDE ? 8192 : 16384;
   const int32_t n = g.n;
-  fill<int32_t>(cb.cnt, n, 0, s);
+  // count and row-length sum of a community in one word while every sum is
+  // below 2^32 (g.m bounds the round's entries)
+  const int packed = g.m < (1ll << 32) ? 1 : 0;
+  if (!packed) fill<int32_t>(cb.cnt, n, 0, s);
   // the staging cursors are free until contract_build: row-length sums by community
   unsigned long long* dsum = (unsigned long long*)cb.ecur;
   fill<int64_t>(cb.ecur, n, 0, s);
   k_comm_sums<<<std::max(1, std::min((n + CT - 1) / CT, 2048)), 256, 0, s>>>(comm, g.off, g.end, n, cb.cnt, dsum,
-                                                                             cb.voff);
+                                                                             cb.voff, packed);
   uint8_t* flag = (uint8_t*)cb.i32_tmp;  // n bytes fit in the n-int scratch
-  k_nonempty<<<nblk(n), 256, 0, s>>>(cb.cnt, n, flag);
+  if (packed) k_unpack_sums<<<nblk(n), 256, 0, s>>>(dsum, n, cb.cnt, flag);
+  else k_nonempty<<<nblk(n), 256, 0, s>>>(cb.cnt, n, flag);
   scan_exclusive<uint8_t, int32_t>(flag, cb.nid, n, cb.i32_tmp + (n + 3) / 4 + 1, cb.i32_scalar, s);
   k_label<<<nblk(n), 256, 0, s>>>(comm, cb.nid, n, cb.label);
   // Everything up to the one readback below is sized by n >= C, so C, the
