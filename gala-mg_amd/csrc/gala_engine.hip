@@ -198,6 +198,7 @@ struct gala_ctx {
 
   hipEvent_t ev[8] = {};
   hipEvent_t bev[NBINS + 1] = {};  // per-bin decide timing (opts.profile_bins)
+  hipEvent_t kev[2] = {};          // the bin-8 part kernel's own time (opts.profile_bins)
 };
 
 namespace {
@@ -733,6 +734,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     HIPOK(hipEventRecord(ctx->ev[0], s));
     if (build_cv) launch_build_cv<T>(sd, g.n, s);
     if (prof) {  // per-bin events: one stream
+      sd.kev = ctx->kev;
       for (int b = 0; b < NBINS; b++) {
         HIPOK(hipEventRecord(ctx->bev[b], s));
         if (first) launch_decide_first<T>(b, g, sd, fr, bs, caps[b], m2, s, ctx->fvar[b]);
@@ -756,6 +758,13 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     float a = 0;
     (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
     t_dm += a;
+    const int hv = ctx->dvar[NBINS - 1];
+    if (prof && !first && caps[NBINS - 1] > 0 && hv >= 4 && hv <= 6) {  // the split's part kernel ran
+      float kp = 0;
+      (void)hipEventElapsedTime(&kp, ctx->kev[0], ctx->kev[1]);
+      ctx->res.hub_part_ms += kp;
+      ctx->res.hub_part_launches++;
+    }
     ctx->res.swept_vertices += (int64_t)out->swept;
     ctx->res.swept_edges += (int64_t)out->swept_edges;
     if (!g.weighted()) {
@@ -1596,6 +1605,8 @@ int gala_create(gala_ctx** out, int device, const gala_opts* opts) {
     if (hipEventCreate(&e) != hipSuccess) { gala_destroy(ctx); return GALA_E_HIP; }
   for (auto& e : ctx->bev)
     if (hipEventCreate(&e) != hipSuccess) { gala_destroy(ctx); return GALA_E_HIP; }
+  for (auto& e : ctx->kev)
+    if (hipEventCreate(&e) != hipSuccess) { gala_destroy(ctx); return GALA_E_HIP; }
   *out = ctx;
   return GALA_OK;
 }
@@ -1614,6 +1625,8 @@ void gala_destroy(gala_ctx* ctx) {
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->bev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->kev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->evf)
     if (e) (void)hipEventDestroy(e);

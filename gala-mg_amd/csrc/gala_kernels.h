@@ -129,6 +129,9 @@ struct DevState {
   // (community, tile) sum is one global atomic instead of two; k_apply adds
   // them to Tot and size and leaves the words zero (null: separate atomics)
   unsigned long long* tsd = nullptr;
+  // profiling (per-bin timing): two events recorded around the bin-8 split's
+  // part kernel, which does all of that bin's row work (null: not recorded)
+  hipEvent_t* kev = nullptr;
 };
 
 // Per-graph state of a batch (one entry per graph; integer path).

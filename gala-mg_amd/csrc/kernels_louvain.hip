@@ -2488,7 +2488,9 @@ void launch_decide(int bin, const DevGraph<T>& g, DevState<T>& st, const int32_t
       int64_t pmax = 16;
       if (st.hub_dmax >= 0) pmax = std::min<int64_t>(16, std::max<int64_t>(1, (st.hub_dmax + PART_KEYS - 1) / PART_KEYS));
       const int part_grid = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (int64_t)cap * pmax));
+      if (st.kev) (void)hipEventRecord(st.kev[0], s);
       k_hub_part<HUB_LT, T><<<part_grid, 1024, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, st.hub_fail, m2);
+      if (st.kev) (void)hipEventRecord(st.kev[1], s);
       // without too-long rows an overflowed hub is decided by the commit itself
       k_hub_commit<T><<<grid_for(cap, 1, 1024), 256, 0, s>>>(g, st, front, bstart, b, st.hub_plan, hp, m2,
                                                               st.hub_fail, long_rows ? 0 : 1);

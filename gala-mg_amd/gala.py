@@ -83,7 +83,8 @@ class Result(ctypes.Structure):
                 ("bin_first_launches", ctypes.c_int32 * 9), ("wide", ctypes.c_int32),
                 ("weight_shift", ctypes.c_int32), ("round_vertices", ctypes.c_int32 * 64),
                 ("round_entries", ctypes.c_int64 * 64), ("round_contracted_entries", ctypes.c_int64 * 64),
-                ("certify_empty", ctypes.c_int32)]
+                ("certify_empty", ctypes.c_int32), ("hub_part_launches", ctypes.c_int32),
+                ("hub_part_ms", ctypes.c_double)]
 
     def to_dict(self):
         r = self.rounds
@@ -111,7 +112,8 @@ class Result(ctypes.Structure):
                     weight_shift=self.weight_shift, round_vertices=list(self.round_vertices[:r]),
                     round_entries=list(self.round_entries[:r]),
                     round_contracted_entries=list(self.round_contracted_entries[:r]),
-                    certify_empty=self.certify_empty)
+                    certify_empty=self.certify_empty, hub_part_launches=self.hub_part_launches,
+                    hub_part_ms=self.hub_part_ms)
 
 
 ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,

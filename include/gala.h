@@ -127,6 +127,9 @@ typedef struct {
   int64_t round_contracted_entries[64];
   int32_t certify_empty;  /* of the certify_sweeps, those in which the rest of the
                              frontier moved nobody (their update pass is skipped) */
+  int32_t hub_part_launches; /* profile_bins=1: general sweeps that ran the bin-8
+                                split's part kernel, and its own time */
+  double hub_part_ms;
 } gala_result;
 
 /* Version string of the library. */
