@@ -85,11 +85,22 @@ struct Counters {
   unsigned long long active;     // vertices kept in the next frontier
   int nonempty;                  // communities with size > 0
   int overflow;                  // hash-table overflow flag (must stay 0)
+  double step_q;                 // Q of the sweep as the device's stopping rule saw it
+  int step_accept;               // ... and its decision (1: accepted, prev = cur)
+  int step_pad;
   unsigned long long bin_swept[NBINS];  // decide-and-move per degree bin
   unsigned long long bin_edges[NBINS];
   // written by the decide kernels; moved / swept / swept_edges / bin_* above
   // are their sums, formed on the host when the counters are read
   DecideSlot slot[CTR_SLOTS];
+};
+
+// Device state of a round's stopping rule when the host reads each sweep's
+// counters one sweep late (the sweep after it is already queued).
+struct RoundCtl {
+  double qprev;  // Q of the last accepted sweep (of the singleton partition before sweep 1)
+  int live;      // 1 while the round goes on; 0 once a sweep was refused
+  int accept;    // the last sweep's decision: 1 = accepted (prev = cur before the next sweep)
 };
 
 // Per-workgroup partial results of the O(V) reductions (k_apply, k_prune).

@@ -32,6 +32,18 @@ void accept_sweep(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, hipStr
 // to Tot and size are not applied again by the next update
 void settle_pairs(int2* cp, int64_t n, hipStream_t s);
 void reset_counters(Counters* c, long long min_init, hipStream_t s);
+// Stopping rule of a round on the device (louvain_main_process.cu:1004-1007,
+// 1026-1031).  round_start: qprev = Q of the counters (the singleton
+// partition), live = 1.  round_step, after sweep `it`'s update: the sweep is
+// accepted iff live, Q - qprev >= min_mod and it < max_it; then qprev = Q,
+// else live = 0 and the next frontier's bin starts (NBINS + 1) are zeroed, so
+// a sweep already queued behind it finds no vertex.  The Q and the decision
+// are also written into the counters for the host.
+void round_start(const Counters* c, RoundCtl* rc, double m2, hipStream_t s);
+void round_step(Counters* c, RoundCtl* rc, double m2, double min_mod, int it, int max_it, int32_t* bnext,
+                hipStream_t s);
+// accept_sweep when the last round_step accepted its sweep
+void accept_sweep_if(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, const RoundCtl* rc, hipStream_t s);
 
 // ---- Louvain phase kernels (kernels_louvain.hip) ----
 // Graph view on the device.  off has n+1 exclusive offsets.

@@ -15,10 +15,19 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const In* __restrict__ i
                                                         V* __restrict__ part) {
   __shared__ V sh[17];
   int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
-  V s = 0;
+  // all loads first, at indices clamped into [0, n) (n >= 1 for a launched
+  // scan), so that none waits behind a branch for the one before it
+  In x[SCAN_IPT];
+#pragma unroll
   for (int k = 0; k < SCAN_IPT; k++) {
-    int64_t i = base + (int64_t)k * SCAN_T + threadIdx.x;
-    if (i < n) s += (V)in[i];
+    const int64_t i = base + (int64_t)k * SCAN_T + threadIdx.x;
+    x[k] = in[i < n ? i : n - 1];
+  }
+  V s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_IPT; k++) {
+    const int64_t i = base + (int64_t)k * SCAN_T + threadIdx.x;
+    s += i < n ? (V)x[k] : V(0);
   }
   V tot;
   block_excl_scan(s, sh, &tot);
@@ -44,12 +53,18 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_down(const In* __restrict__ in,
                                                       const V* __restrict__ part, V* __restrict__ out) {
   __shared__ V sh[17];
   int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_IPT;
+  In x[SCAN_IPT];
+#pragma unroll
+  for (int k = 0; k < SCAN_IPT; k++) {
+    const int64_t i = base + k;
+    x[k] = in[i < n ? i : n - 1];  // clamped, unconditional (as in k_scan_reduce)
+  }
   V loc[SCAN_IPT];
   V s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_IPT; k++) {
-    int64_t i = base + k;
-    loc[k] = i < n ? (V)in[i] : V(0);
+    const int64_t i = base + k;
+    loc[k] = i < n ? (V)x[k] : V(0);
     s += loc[k];
   }
   V tot;
@@ -90,15 +105,42 @@ template void scan_exclusive<uint8_t, int64_t>(const uint8_t*, int64_t*, int64_t
 // ---------------------------------------------------------------------------
 constexpr int PART_T = 256, PART_IPT = 8, PART_TILE = PART_T * PART_IPT;
 
-__device__ __forceinline__ void part_item(int mode, const int32_t* order, const int8_t* binid,
-                                          const uint8_t* active, int64_t i, int& v, int& key) {
+// The items base + k * stride (k < IPT): item id and key (-1: past n or
+// dropped).  Every load is issued without a branch, at indices clamped into
+// [0, n), so a thread's loads are in flight together instead of one behind
+// another.  n >= 1.
+template <int IPT>
+__device__ __forceinline__ void part_items(int mode, const int32_t* __restrict__ order,
+                                           const int8_t* __restrict__ binid, const uint8_t* __restrict__ active,
+                                           int64_t base, int64_t stride, int64_t n, int (&v)[IPT], int (&key)[IPT]) {
+  int64_t idx[IPT];
+#pragma unroll
+  for (int k = 0; k < IPT; k++) {
+    const int64_t i = base + k * stride;
+    idx[k] = i < n ? i : n - 1;
+  }
+  int8_t b[IPT];
   if (mode == 0) {
-    v = (int)i;
-    key = binid[i];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) b[k] = binid[idx[k]];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+      v[k] = (int)idx[k];
+      key[k] = base + k * stride < n ? (int)b[k] : -1;
+    }
   } else {
     // mode 1 keeps the flagged vertices of `order`, mode 2 the unflagged ones
-    v = order[i];
-    key = (active[v] != 0) == (mode == 1) ? (int)binid[v] : -1;
+#pragma unroll
+    for (int k = 0; k < IPT; k++) v[k] = order[idx[k]];
+    uint8_t a[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+      a[k] = active[v[k]];
+      b[k] = binid[v[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < IPT; k++)
+      key[k] = base + k * stride < n && (a[k] != 0) == (mode == 1) ? (int)b[k] : -1;
   }
 }
 
@@ -113,14 +155,13 @@ __global__ __launch_bounds__(PART_T) void k_part_count(int mode, const int32_t* 
   int c[NBINS];
 #pragma unroll
   for (int b = 0; b < NBINS; b++) c[b] = 0;
-  for (int k = 0; k < PART_IPT; k++) {
-    int64_t i = base + k;
-    if (i < n) {
-      int v, key;
-      part_item(mode, order, binid, active, i, v, key);
+  if (n > 0) {  // uniform
+    int vs[PART_IPT], ks[PART_IPT];
+    part_items<PART_IPT>(mode, order, binid, active, base, 1, n, vs, ks);
 #pragma unroll
-      for (int b = 0; b < NBINS; b++) c[b] += (key == b);
-    }
+    for (int k = 0; k < PART_IPT; k++)
+#pragma unroll
+      for (int b = 0; b < NBINS; b++) c[b] += (ks[k] == b);
   }
 #pragma unroll
   for (int b = 0; b < NBINS; b++) {
@@ -182,15 +223,16 @@ __global__ __launch_bounds__(PART_T) void k_part_scatter(int mode, const int32_t
   int c[NBINS];
 #pragma unroll
   for (int b = 0; b < NBINS; b++) c[b] = 0;
+  if (n > 0) {  // uniform
+    part_items<PART_IPT>(mode, order, binid, active, base, 1, n, vs, ks);
+  } else {
 #pragma unroll
-  for (int k = 0; k < PART_IPT; k++) {
-    int64_t i = base + k;
-    ks[k] = -1;
-    vs[k] = 0;
-    if (i < n) part_item(mode, order, binid, active, i, vs[k], ks[k]);
+    for (int k = 0; k < PART_IPT; k++) { vs[k] = 0; ks[k] = -1; }
+  }
+#pragma unroll
+  for (int k = 0; k < PART_IPT; k++)
 #pragma unroll
     for (int b = 0; b < NBINS; b++) c[b] += (ks[k] == b);
-  }
   int pos[NBINS];
 #pragma unroll
   for (int b = 0; b < NBINS; b++) {
@@ -220,10 +262,6 @@ int64_t part_tmp_elems(int64_t n) { return (int64_t)NBINS * ((n + PART_TILE - 1)
 // ---------------------------------------------------------------------------
 constexpr int CMP_T = 256, CMP_IPT = 16, CMP_TILE = CMP_T * CMP_IPT;
 
-__device__ __forceinline__ bool cmp_keep(int mode, const uint8_t* __restrict__ flag, int v) {
-  return (flag[v] != 0) == (mode == 1);
-}
-
 __global__ __launch_bounds__(CMP_T) void k_cmp_count(int mode, const int32_t* __restrict__ order,
                                                      const uint8_t* __restrict__ flag, int64_t n,
                                                      int32_t* __restrict__ cnt, const int32_t* __restrict__ n_dev) {
@@ -231,10 +269,19 @@ __global__ __launch_bounds__(CMP_T) void k_cmp_count(int mode, const int32_t* __
   if (n_dev && *n_dev < n) n = *n_dev;  // the list's length on the device (n is its bound)
   const int64_t base = (int64_t)blockIdx.x * CMP_TILE;
   int c = 0;
+  if (n > 0) {  // uniform; loads at clamped indices, none behind a branch
+    int vs[CMP_IPT];
 #pragma unroll
-  for (int k = 0; k < CMP_IPT; k++) {
-    const int64_t i = base + k * CMP_T + threadIdx.x;
-    if (i < n) c += cmp_keep(mode, flag, order[i]) ? 1 : 0;
+    for (int k = 0; k < CMP_IPT; k++) {
+      const int64_t i = base + k * CMP_T + threadIdx.x;
+      vs[k] = order[i < n ? i : n - 1];
+    }
+    uint8_t f[CMP_IPT];
+#pragma unroll
+    for (int k = 0; k < CMP_IPT; k++) f[k] = flag[vs[k]];
+#pragma unroll
+    for (int k = 0; k < CMP_IPT; k++)
+      c += (base + k * CMP_T + threadIdx.x < n && (f[k] != 0) == (mode == 1)) ? 1 : 0;
   }
   int32_t tot;
   (void)block_excl_scan<int32_t>(c, sh, &tot);
@@ -272,14 +319,25 @@ __global__ __launch_bounds__(CMP_T) void k_cmp_scatter(int mode, const int32_t* 
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const int64_t wbase = (int64_t)blockIdx.x * CMP_TILE + (int64_t)w * 64 * CMP_IPT;
   int vs[CMP_IPT];
+  uint8_t f[CMP_IPT];
+  if (n > 0) {  // uniform; loads at clamped indices, none behind a branch
+#pragma unroll
+    for (int k = 0; k < CMP_IPT; k++) {
+      const int64_t i = wbase + k * 64 + lane;
+      vs[k] = order[i < n ? i : n - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < CMP_IPT; k++) f[k] = flag[vs[k]];
+  } else {
+#pragma unroll
+    for (int k = 0; k < CMP_IPT; k++) { vs[k] = 0; f[k] = 0; }
+  }
   unsigned keep = 0;
   int wc = 0;
 #pragma unroll
   for (int k = 0; k < CMP_IPT; k++) {
     const int64_t i = wbase + k * 64 + lane;
-    const bool ok = i < n;
-    vs[k] = ok ? order[i] : 0;
-    const bool kp = ok && cmp_keep(mode, flag, vs[k]);
+    const bool kp = i < n && (f[k] != 0) == (mode == 1);
     keep |= (kp ? 1u : 0u) << k;
     wc += __popcll(__ballot(kp));
   }
@@ -394,10 +452,80 @@ void copy_i32(const int32_t* a, int32_t* b, int64_t n, hipStream_t s) {
   k_copy_i32<<<nb, 256, 0, s>>>(a, b, n);
 }
 
+// Unsigned 128-bit integer to double, rounded to nearest even: the 64 bits
+// from the leading one down, with every lower bit folded into bit 0 as a
+// sticky bit, round exactly as the whole value does (the rounding position is
+// 11 bits above bit 0); the scale by 2^sh is exact.
+__device__ __forceinline__ double u128_to_double(unsigned long long hi, unsigned long long lo) {
+  if (hi == 0) return (double)lo;
+  const int lz = __clzll(hi);  // 0..63
+  const int sh = 64 - lz;      // bits below the 64-bit window: 1..64
+  unsigned long long top = sh == 64 ? hi : (hi << lz) | (lo >> sh);
+  const unsigned long long rest = sh == 64 ? lo : lo & ((1ull << sh) - 1);
+  top |= rest != 0 ? 1ull : 0ull;
+  return ldexp((double)top, sh);
+}
+
+// Q = sum_in/m2 - sum_tot2/m2^2, each operation rounded once, in the order the
+// host evaluates it
+__device__ __forceinline__ double q_of_counters(const Counters* c, double m2) {
+  const double t2 = u128_to_double(c->sum_tot2_hi, c->sum_tot2);
+  return __dsub_rn(__ddiv_rn((double)c->sum_in, m2), __ddiv_rn(t2, __dmul_rn(m2, m2)));
+}
+
+__global__ void k_round_start(const Counters* __restrict__ c, RoundCtl* __restrict__ rc, double m2) {
+  if (threadIdx.x == 0) {
+    rc->qprev = q_of_counters(c, m2);
+    rc->live = 1;
+    rc->accept = 0;
+  }
+}
+void round_start(const Counters* c, RoundCtl* rc, double m2, hipStream_t s) {
+  k_round_start<<<1, 64, 0, s>>>(c, rc, m2);
+}
+
+__global__ void k_round_step(Counters* __restrict__ c, RoundCtl* __restrict__ rc, double m2, double min_mod, int it,
+                             int max_it, int32_t* __restrict__ bnext) {
+  __shared__ int s_acc;
+  if (threadIdx.x == 0) {
+    const double q = q_of_counters(c, m2);
+    const int acc = rc->live && !(__dsub_rn(q, rc->qprev) < min_mod) && it < max_it ? 1 : 0;
+    c->step_q = q;
+    c->step_accept = acc;
+    rc->accept = acc;
+    if (acc) rc->qprev = q;
+    else rc->live = 0;
+    s_acc = acc;
+  }
+  __syncthreads();
+  if (!s_acc && threadIdx.x <= NBINS) bnext[threadIdx.x] = 0;
+}
+void round_step(Counters* c, RoundCtl* rc, double m2, double min_mod, int it, int max_it, int32_t* bnext,
+                hipStream_t s) {
+  k_round_step<<<1, 64, 0, s>>>(c, rc, m2, min_mod, it, max_it, bnext);
+}
+
+__global__ void k_accept_if(const int32_t* __restrict__ cur, int32_t* __restrict__ prev, int2* __restrict__ cp,
+                            int64_t n, const RoundCtl* __restrict__ rc) {
+  if (!rc->accept) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = cur[i];
+    prev[i] = c;
+    if (cp) cp[i] = make_int2(c, c);
+  }
+}
+void accept_sweep_if(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, const RoundCtl* rc, hipStream_t s) {
+  if (n <= 0) return;
+  int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  k_accept_if<<<nb, 256, 0, s>>>(cur, prev, cp, n, rc);
+}
+
 __global__ void k_reset_counters(Counters* c, long long min_init) {
   if (threadIdx.x == 0) {
     c->sum_in = 0; c->sum_tot2 = 0; c->sum_tot2_hi = 0; c->min_tot = min_init; c->moved = 0; c->swept = 0;
     c->swept_edges = 0; c->active = 0; c->nonempty = 0; c->overflow = 0;
+    c->step_q = 0; c->step_accept = 0; c->step_pad = 0;
   }
   if (threadIdx.x < NBINS) { c->bin_swept[threadIdx.x] = 0; c->bin_edges[threadIdx.x] = 0; }
   unsigned long long* sl = (unsigned long long*)c->slot;

@@ -58,7 +58,8 @@ class Opts(ctypes.Structure):
                 ("frontier_mode", ctypes.c_int), ("learned_tau", ctypes.c_float),
                 ("profile_bins", ctypes.c_int), ("no_hip_graphs", ctypes.c_int),
                 ("serial_bins", ctypes.c_int), ("learned_min_degree", ctypes.c_int),
-                ("learned_min_vertices", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+                ("learned_min_vertices", ctypes.c_int), ("queue_sweeps", ctypes.c_int),
+                ("reserved", ctypes.c_int * 2)]
 
 
 class Result(ctypes.Structure):
@@ -84,7 +85,8 @@ class Result(ctypes.Structure):
                 ("weight_shift", ctypes.c_int32), ("round_vertices", ctypes.c_int32 * 64),
                 ("round_entries", ctypes.c_int64 * 64), ("round_contracted_entries", ctypes.c_int64 * 64),
                 ("certify_empty", ctypes.c_int32), ("hub_part_launches", ctypes.c_int32),
-                ("hub_part_ms", ctypes.c_double)]
+                ("hub_part_ms", ctypes.c_double), ("queued_rounds", ctypes.c_int32),
+                ("inert_sweeps", ctypes.c_int32)]
 
     def to_dict(self):
         r = self.rounds
@@ -113,7 +115,8 @@ class Result(ctypes.Structure):
                     round_entries=list(self.round_entries[:r]),
                     round_contracted_entries=list(self.round_contracted_entries[:r]),
                     certify_empty=self.certify_empty, hub_part_launches=self.hub_part_launches,
-                    hub_part_ms=self.hub_part_ms)
+                    hub_part_ms=self.hub_part_ms, queued_rounds=self.queued_rounds,
+                    inert_sweeps=self.inert_sweeps)
 
 
 ITER_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -400,7 +403,8 @@ class Engine:
 
     def __init__(self, device=0, min_modularity=1e-6, dump_iterations=False, dump_rounds=False,
                  max_rounds=0, max_iterations=0, profile_bins=False, frontier_mode=0, learned_tau=0.5,
-                 hip_graphs=True, serial_bins=False, learned_min_degree=0, learned_min_vertices=0):
+                 hip_graphs=True, serial_bins=False, learned_min_degree=0, learned_min_vertices=0,
+                 queue_sweeps=False):
         L = lib()
         self.opts = Opts()
         L.gala_default_opts(ctypes.byref(self.opts))
@@ -416,6 +420,7 @@ class Engine:
         self.opts.serial_bins = int(serial_bins)
         self.opts.learned_min_degree = int(learned_min_degree)
         self.opts.learned_min_vertices = int(learned_min_vertices)
+        self.opts.queue_sweeps = int(queue_sweeps)
         self.ctx = VP()
         _check(L.gala_create(ctypes.byref(self.ctx), device, ctypes.byref(self.opts)), "gala_create")
 

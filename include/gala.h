@@ -31,6 +31,7 @@ extern "C" {
 #define GALA_E_HIP -4       /* HIP runtime error / no device */
 #define GALA_E_OVERFLOW -5  /* exact-integer range exceeded (m2 > 1e9) or hash overflow */
 #define GALA_E_INVALID -6   /* bad argument or state (no graph, m2 == 0, ...) */
+#define GALA_E_INTERNAL -7  /* internal consistency check failed (a bug: please report) */
 
 /* weight modes of the .bin weight section (out of band, as the reference's -w flag) */
 #define GALA_W_NONE 0 /* no weight section: every weight is 1 (graph.cpp:205-212) */
@@ -58,7 +59,10 @@ typedef struct {
                             than this (0: the network filters every vertex) */
   int learned_min_vertices; /* rounds whose graph has fewer vertices run plain
                                sweeps only (0: every round is filtered) */
-  int reserved[3];
+  int queue_sweeps;      /* 1: plain rounds queue each sweep before the previous
+                            one's counters are read, the stopping rule applied on
+                            the device (0: the host reads every sweep first) */
+  int reserved[2];
 } gala_opts;
 
 /* Called after every iteration (Iteration:0 is Q before any move).
@@ -130,6 +134,10 @@ typedef struct {
   int32_t hub_part_launches; /* profile_bins=1: general sweeps that ran the bin-8
                                 split's part kernel, and its own time */
   double hub_part_ms;
+  int32_t queued_rounds;  /* rounds run with the device-side stopping rule (sweep k+1
+                             queued before sweep k's counters are read) */
+  int32_t inert_sweeps;   /* sweeps queued after the round's last one: their frontier
+                             is emptied on the device, so they move nobody */
 } gala_result;
 
 /* Version string of the library. */

@@ -91,6 +91,62 @@ def test_serial_bins_match_oracle():
     eng.close()
 
 
+def test_queued_rounds_match_oracle():
+    # sweeps queued ahead of the host's readback, the stopping rule applied on
+    # the device: every iteration's Q and moved count, the round counts and
+    # the final partition equal the oracle's; every round ran queued and each
+    # round that stopped on its Q left one sweep with an emptied frontier
+    say("queued rounds")
+    g = gala.gen_chung_lu(200000, 2.1, 17.35, 12000.0, 0)
+    eng = gala.Engine(queue_sweeps=True)
+    res, _ = full_trace_compare(eng, g, keep_vectors=False)
+    eng.close()
+    assert res["queued_rounds"] == res["rounds"]
+    assert res["inert_sweeps"] == res["rounds"]
+
+
+def same_run(a, b):
+    for k in ("modularity", "rounds", "round_nc", "round_iters", "round_q", "total_iterations",
+              "swept_vertices", "swept_edges", "decide_bytes", "bin_swept_edges", "round_contracted_entries"):
+        assert a[k] == b[k], k
+
+
+@pytest.mark.parametrize("case", ["karate", "sbm1024", "star", "rmat_fp32", "cap2"])
+def test_queued_rounds_equal_synchronous(golden, case):
+    # the same runs with the host reading every sweep before queueing the next
+    say(f"queued vs synchronous: {case}")
+    kw = {}
+    if case == "karate":
+        g = gala.Graph.load_bin_graph(os.path.join(golden, "karate.txt.bin"))
+    elif case == "sbm1024":
+        g = gala.Graph.load_bin_graph(os.path.join(golden, "sbm1024_s0.bin"))
+    elif case == "star":
+        g = star_plus_random(30000, 40000, 3)
+    elif case == "rmat_fp32":  # the wide path: 128-bit sums of Tot^2
+        g = gala.gen_rmat(16, seed=5)
+    else:  # at most 2 sweeps a round: rounds cut by the cap queue no sweep after it
+        g = gala.gen_chung_lu(50000, 2.1, 17.35, 3000.0, 2)
+        kw = dict(max_iterations=2)
+    out = []
+    for queue in (True, False):
+        trace = []
+        eng = gala.Engine(queue_sweeps=queue, **kw)
+        eng.set_graph(g)
+        r = eng.run(trace=trace)
+        out.append((r, trace, eng.communities(g.vertex_num)))
+        eng.close()
+    (rq, tq, cq), (rs, ts, cs) = out
+    same_run(rq, rs)
+    assert [t[:4] for t in tq] == [t[:4] for t in ts]
+    assert np.array_equal(cq, cs)
+    assert rq["queued_rounds"] == rq["rounds"] and rs["queued_rounds"] == 0
+    assert rs["inert_sweeps"] == 0
+    capped = sum(1 for it in rq["round_iters"] if kw and it >= kw["max_iterations"])
+    assert rq["inert_sweeps"] == rq["rounds"] - capped
+    if case == "cap2":
+        assert capped > 0 and max(rq["round_iters"]) == 2
+
+
 def variant_set(k):
     return np.array([min(k, c - 1) for c in gala.variant_counts()], np.int32)
 
