@@ -134,7 +134,6 @@ struct gala_ctx {
   float* last_gain = nullptr;
   int32_t* moved_it = nullptr;
   float* nbr_moved = nullptr;
-  int32_t* churn = nullptr;
   int32_t* front2 = nullptr;   // predicted movers of a learned sweep, then its complement
   int32_t* bstart2 = nullptr;
   int32_t* front3 = nullptr;   // second frontier buffer (the pre-sweep frontier survives a sweep)
@@ -280,7 +279,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool wide) {
   OWN(ctx->cp, N);
   OWN(ctx->tsd, N);
   fill<int64_t>((int64_t*)ctx->tsd, N, 0, ctx->stream);
-  OWN(ctx->last_gain, N); OWN(ctx->moved_it, N); OWN(ctx->nbr_moved, N); OWN(ctx->churn, N);
+  OWN(ctx->last_gain, N); OWN(ctx->moved_it, N); OWN(ctx->nbr_moved, N);
   OWN(ctx->front2, N); OWN(ctx->bstart2, NBINS + 1); OWN(ctx->keep, N);
   OWN(ctx->front3, N); OWN(ctx->bstart3, NBINS + 1); OWN(ctx->size_snap, N);
   { char* p; OWN(p, N * tsz); ctx->tot_snap = p; }
@@ -726,11 +725,8 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   // decide-and-move over one frontier, timed into t_dm (and per bin)
   // build_cv: refresh the community view first (a certification sweep reads
   // the view of the pre-sweep state, which is still in place)
-  // In is updated from the movers' rows alone (no dirty marks); tracked
-  // (learned / logged) runs also count each community's churn for the
-  // features.
+  // In is updated from the movers' rows alone (no dirty marks)
   const bool incr = true;
-  if (track) st.churn = ctx->churn;
   // ev: the events recorded at its start and end (ctx->ev unless given)
   auto decide = [&](DevState<T>& sd0, const int32_t* fr, const int32_t* bs, bool first, bool build_cv,
                     hipEvent_t* ev = nullptr) -> int {
@@ -984,9 +980,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
         ctx->res.certify_empty++;
         account_decide(false, cc);
       } else {
-        DevState<T> sp = st;
-        sp.churn = nullptr;  // keep the churn of the learned sweep for the features
-        if ((rc = update(sp, false, ctx->front2, ctx->bstart2, &c))) return rc;
+        if ((rc = update(st, false, ctx->front2, ctx->bstart2, &c))) return rc;
         nfront = (int32_t)c.active;
         moved += (int64_t)c.moved;
         q = q_from(c, m2d);

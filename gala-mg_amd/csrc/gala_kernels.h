@@ -122,7 +122,6 @@ struct DevState {
   float* last_gain;  // normalised net gain of the best move at the vertex's last sweep
   int32_t* moved_it; // sweep in which the vertex last moved (-1: never this round)
   float* nbr_moved;  // fraction of neighbours that moved (dirty-row In updates of tracked runs)
-  int32_t* churn = nullptr;  // per community: members that joined or left in the latest sweep
   // batched graphs (block-diagonal union): graph of every vertex and the m2 of
   // every graph; null for a single graph
   const int32_t* gid = nullptr;
@@ -241,8 +240,8 @@ void launch_batch_reset(Counters* c, long long min_init, int32_t* nlive, hipStre
 // Features of vertex v before sweep it (all in [-1, 1]):
 //   f0 log2(1 + degree)/16, f1 In/K (weight fraction inside its community),
 //   f2 normalised net gain of its best move at its last sweep,
-//   f3 moved in sweep it-1, f4 churn of its community in sweep it-1 (members
-//   that joined or left, over its size, at most 1), f5 min(it, 16)/16.
+//   f3 moved in sweep it-1, f4 log(1 + size of its community) / log(1 + n),
+//   f5 min(it, 16)/16.
 // All of them are O(1) per vertex, so tracked runs keep the incremental In
 // update (no row rescans).
 template <class T>

@@ -42,10 +42,10 @@ __device__ __forceinline__ void vertex_features(const DevGraph<T>& g, const DevS
   const float lg = st.last_gain[v];
   f[2] = lg < -1.0f ? -1.0f : (lg > 1.0f ? 1.0f : lg);
   f[3] = st.moved_it[v] == it - 1 ? 1.0f : 0.0f;
-  const int c = st.cur[v];
-  const int sz = st.size[c];
-  const float ch = sz > 0 ? (float)st.churn[c] / (float)sz : 0.0f;
-  f[4] = ch < 1.0f ? ch : 1.0f;
+  // the size of its community on the scale of the graph: log(1 + size) /
+  // log(1 + n), the same range for graphs of every size
+  const int sz = st.size[st.cur[v]];
+  f[4] = g.n > 1 ? __log2f(1.0f + (float)sz) / __log2f(1.0f + (float)g.n) : 0.0f;
   f[5] = (float)(it < 16 ? it : 16) * (1.0f / 16.0f);
 }
 
@@ -98,10 +98,23 @@ __global__ __launch_bounds__(256) void k_mlp_filter(DevGraph<T> g, DevState<T> s
   const int nw = gridDim.x * (blockDim.x >> 6);
   for (int base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16; base < nfront; base += nw * 16) {
     const int r = base + (lane & 15);
+    int v = -1;
+    int64_t d = 0;
+    if (r < nfront) {
+      v = front[r];
+      d = g.end[v] - g.off[v];
+    }
+    // rows shorter than dmin are kept without a prediction: a wave whose 16
+    // rows are all short (the frontier is grouped by degree bin, so most
+    // waves of the short-row bins are) skips the features and the MFMAs
+    if (!__any(r < nfront && d >= dmin)) {
+      if (lane < 16 && r < nfront) keep[v] = 1;
+      continue;
+    }
     float x[4] = {0.f, 0.f, 0.f, 0.f};
     if (r < nfront) {
       float f[MLP_IN];
-      vertex_features<T>(g, st, front[r], it, f);
+      vertex_features<T>(g, st, v, it, f);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const int k = 4 * (lane >> 4) + j;

@@ -2560,9 +2560,8 @@ __global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n, co
   __shared__ int s_cnt[TD_S];
   __shared__ tt s_tot[TD_S];
   __shared__ int s_list[TD_S];
-  __shared__ int s_mv[TD_S];  // members that joined or left (tracked runs)
   __shared__ int s_n;
-  for (int t = threadIdx.x; t < TD_S; t += blockDim.x) { s_key[t] = 0; s_cnt[t] = 0; s_tot[t] = tt(0); s_mv[t] = 0; }
+  for (int t = threadIdx.x; t < TD_S; t += blockDim.x) { s_key[t] = 0; s_cnt[t] = 0; s_tot[t] = tt(0); }
   if (threadIdx.x == 0) s_n = 0;
   __syncthreads();
   auto add = [&](int c, tt k, int one) {
@@ -2573,7 +2572,6 @@ __global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n, co
         if (kk == 0) s_list[atomicAdd(&s_n, 1)] = (int)p;
         atomicAdd(&s_cnt[p], one);
         if (k) atomicAdd(&s_tot[p], one > 0 ? k : -k);
-        if (st.churn) atomicAdd(&s_mv[p], 1);
         return;
       }
       p = (p + 1) & (TD_S - 1);
@@ -2609,11 +2607,9 @@ __global__ __launch_bounds__(256) void k_tot_delta(DevState<T> st, int32_t n, co
         if (s_cnt[p]) atomicAdd(&st.size[c], s_cnt[p]);
         if (s_tot[p]) atomicAdd(&st.Tot[c], s_tot[p]);
       }
-      if (st.churn) atomicAdd(&st.churn[c], s_mv[p]);
       s_key[p] = 0;
       s_cnt[p] = 0;
       s_tot[p] = tt(0);
-      s_mv[p] = 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) s_n = 0;
@@ -2728,7 +2724,6 @@ void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t 
                   const int32_t* vcnt, const int32_t* dlist, const int32_t* dcnt) {
   if (n <= 0) return;
   const int nb = std::min((n + 255) / 256, MAX_PARTS);
-  if (st.churn) fill<int32_t>(st.churn, n, 0, s);  // this sweep's churn replaces the last one's
   if (!dlist) { dlist = vlist; dcnt = vcnt; }
   k_tot_delta<T><<<std::min((n + TD_T - 1) / TD_T, 2048), 256, 0, s>>>(st, n, dlist, dcnt);
   k_apply<T><<<nb, 256, 0, s>>>(st, n, m2, vlist, vcnt);
