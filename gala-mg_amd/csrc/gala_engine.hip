@@ -82,6 +82,7 @@ struct gala_ctx {
   bool unit0 = true;
   bool f32in = false;  // input weights kept as float32 in w[0] (wide path, scaled on load)
   bool unique0 = false;  // input rows strictly ascending (duplicate-free)
+  bool noself0 = false;  // input rows hold no self-loop
   int src = 0;  // buffer holding the current round graph
 
   // per-vertex state (sized n0)
@@ -559,11 +560,12 @@ double q_from(const Counters& c, double m2) {
 // caps[b] = vertices of bin b; *nitems = vertices in any bin (degree > 0),
 // i.e. the valid prefix of ctx->order.
 template <class T>
-int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nitems, int track = 0) {
+int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nitems, int track = 0,
+                bool degree_init = false) {
   hipStream_t s = ctx->stream;
   DevState<T> st = dstate<T>(ctx);
   st.track = track;
-  launch_init<T>(g, st, s);
+  launch_init<T>(g, st, s, degree_init);
   partition_bins(0, nullptr, nullptr, ctx->binid, nullptr, g.n, ctx->order, ctx->bstart_full, ctx->part_tmp, s);
   // hub-table sizes and their prefix sums before the one readback: the scan
   // runs over n slots (entries past the hub count are zero)
@@ -586,7 +588,7 @@ int round_setup(gala_ctx* ctx, const DevGraph<T>& g, int32_t* caps, int32_t* nit
   ctx->hub_dmax = hs[1];
   for (int b = 0; b < NBINS; b++) caps[b] = hb[b + 1] - hb[b];
   *nitems = hb[NBINS];
-  launch_init_rows<T>(g, st, ctx->order, ctx->bstart_full, caps, s);
+  if (!degree_init) launch_init_rows<T>(g, st, ctx->order, ctx->bstart_full, caps, s);
   const int nh = hb[NBINS] - hb[gb];
   if (nh > 0) {
     int rc = grow_hub(ctx, hs[0], T::WIDE);
@@ -673,7 +675,9 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   const bool logging = ctx->logger != nullptr;
   const int track = (learned || logging) ? 1 : 0;
   int32_t caps[NBINS], nitems = 0;
-  int rc = round_setup<T>(ctx, g, caps, &nitems, track);
+  // the input graph with unit weights and no self-loop: K = degree, Self = 0
+  const bool degree_init = round == 0 && ctx->unit0 && ctx->noself0;
+  int rc = round_setup<T>(ctx, g, caps, &nitems, track, degree_init);
   if (rc) return rc;
   DevState<T> st = dstate<T>(ctx);
   st.track = track;
@@ -1228,11 +1232,13 @@ int upload(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr
   for (int64_t j = 0; j < m; j++)
     if (nbr[j] < 0 || nbr[j] >= n) return GALA_E_FORMAT;
   bool unique = true;  // strictly ascending rows (what preprocess and the generators write)
-  for (int32_t i = 0, j0 = 0; i < n && unique; i++) {
-    int64_t a = i == 0 ? 0 : end_incl[i - 1];
-    for (int64_t j = a + 1; j < end_incl[i]; j++)
-      if (nbr[j] <= nbr[j - 1]) { unique = false; break; }
-    (void)j0;
+  bool noself = true;  // no row holds its own vertex
+  for (int32_t i = 0; i < n; i++) {
+    const int64_t a = i == 0 ? 0 : end_incl[i - 1];
+    for (int64_t j = a; j < end_incl[i]; j++) {
+      if (j > a && nbr[j] <= nbr[j - 1]) unique = false;
+      if (nbr[j] == i) noself = false;
+    }
   }
   if (weight_mode != GALA_W_NONE && m > 0 && !w) return GALA_E_INVALID;
   if (weight_mode == GALA_W_I32)
@@ -1302,6 +1308,7 @@ int upload(gala_ctx* ctx, int32_t n, const int64_t* end_incl, const int32_t* nbr
   ctx->n0 = n;
   ctx->m0 = m;
   ctx->unique0 = unique;
+  ctx->noself0 = noself;
   ctx->have_graph = true;
   ctx->have_result = false;
   return GALA_OK;

@@ -166,7 +166,7 @@ struct BatchDev {
 // rows of 8 entries and more come from launch_init_rows, once the bins are
 // partitioned (order, bstart: the bin-grouped vertices; caps: host bin sizes).
 template <class T>
-void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s);
+void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s, bool degree_only = false);
 template <class T>
 void launch_init_rows(const DevGraph<T>& g, DevState<T>& st, const int32_t* order, const int32_t* bstart,
                       const int32_t* caps, hipStream_t s);

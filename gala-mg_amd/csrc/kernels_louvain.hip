@@ -180,8 +180,10 @@ __device__ __forceinline__ typename T::cv_t view_of(const DevState<T>& st, int u
 // init: K, Self, Tot=K, size=1, prev=cur=v, bin id, active=1
 // (init_communities, kernel_functions.cu:34-73)
 // ---------------------------------------------------------------------------
+// degree_only: unit weights and no self-loop (the input graph, checked on
+// upload): K = Tot = degree and Self = 0 without reading the rows
 template <class T>
-__global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> st) {
+__global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> st, int degree_only) {
   using tt = typename T::tot_t;
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += gridDim.x * blockDim.x) {
     int64_t o = g.off[v], e = g.end[v];
@@ -197,6 +199,12 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
       st.last_gain[v] = 0.f;
       st.moved_it[v] = -1;
       st.nbr_moved[v] = 0.f;
+    }
+    if (degree_only) {
+      st.K[v] = (tt)d;
+      st.Tot[v] = (tt)d;
+      st.Self[v] = tt(0);
+      continue;
     }
     if (d >= 8) continue;  // bins 2 and up: launch_init_rows, once the bins are partitioned
     tt k = 0, self = 0;
@@ -297,10 +305,10 @@ void launch_init_rows(const DevGraph<T>& g, DevState<T>& st, const int32_t* orde
 }
 
 template <class T>
-void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s) {
+void launch_init(const DevGraph<T>& g, DevState<T>& st, hipStream_t s, bool degree_only) {
   if (g.n <= 0) return;
   int nb = std::min((g.n + 255) / 256, 16384);
-  k_init_small<T><<<nb, 256, 0, s>>>(g, st);
+  k_init_small<T><<<nb, 256, 0, s>>>(g, st, degree_only ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -3202,7 +3210,7 @@ void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t 
 
 // explicit instantiations
 #define GALA_INST(T)                                                                              \
-  template void launch_init<T>(const DevGraph<T>&, DevState<T>&, hipStream_t);                    \
+  template void launch_init<T>(const DevGraph<T>&, DevState<T>&, hipStream_t, bool);                    \
   template void launch_build_cv<T>(DevState<T>&, int32_t, hipStream_t, const int32_t*, const int32_t*);                           \
   template void launch_init_rows<T>(const DevGraph<T>&, DevState<T>&, const int32_t*, const int32_t*,         \
                                     const int32_t*, hipStream_t);                                  \
