@@ -94,6 +94,7 @@ struct gala_ctx {
   int32_t *order = nullptr, *front = nullptr, *fin = nullptr;
   void* cv = nullptr;  // community view, 8 (int) or 16 (wide) bytes per vertex
   int2* cp = nullptr;  // {cur, prev} per vertex (integer path, incremental In)
+  int32_t* cm = nullptr;  // cur with bit 31 set for a vertex that moved in the sweep in progress
   unsigned long long* tsd = nullptr;  // packed Tot/size deltas of a sweep (integer path; zero between sweeps)
   BlockPart* part = nullptr;  // workgroup partials of the O(V) reductions
   int32_t *bstart_full = nullptr, *bstart_front = nullptr;
@@ -278,6 +279,7 @@ int alloc_buffers(gala_ctx* ctx, int32_t n, int64_t m, bool wide) {
   OWN(ctx->order, N); OWN(ctx->front, N); OWN(ctx->fin, N);
   { char* p; OWN(p, N * 16); ctx->cv = p; }
   OWN(ctx->cp, N);
+  OWN(ctx->cm, N);
   OWN(ctx->tsd, N);
   fill<int64_t>((int64_t*)ctx->tsd, N, 0, ctx->stream);
   OWN(ctx->last_gain, N); OWN(ctx->moved_it, N); OWN(ctx->nbr_moved, N);
@@ -546,6 +548,7 @@ DevState<T> dstate(gala_ctx* ctx) {
   st.cv = (typename T::cv_t*)ctx->cv;
   st.part = ctx->part;
   st.cp = ctx->cp;
+  st.cm = ctx->cm;
   st.tsd = T::WIDE ? nullptr : ctx->tsd;
   return st;
 }
@@ -885,7 +888,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       std::swap(bcur, bnext);
       const bool more = it < max_it;
       if (more) {  // sweep it + 1
-        accept_sweep_if(ctx->cur, ctx->prev, st.cp, g.n, ctx->rctl, s);
+        accept_sweep_if(ctx->cur, ctx->prev, st.cp, st.cm, g.n, ctx->rctl, s);
         st.iter = it + 1;
         reset_counters(ctx->ctr, (long long)m2, s);
         if ((rc = decide(st, fcur, bcur, false, true, ctx->sev[(it + 1) & 1]))) return fail(rc);
@@ -969,7 +972,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       // them touches keep the In of the learned sweep, which is exact.
       ctx->res.certify_sweeps++;
       reset_counters(ctx->ctr, (long long)m2, s);
-      settle_pairs(ctx->cp, g.n, s);  // the learned sweep's moves are in Tot and size already
+      settle_pairs(ctx->cp, ctx->cm, g.n, s);  // the learned sweep's moves are in Tot and size already
       partition_bins(2, fnext, bnext, ctx->binid, ctx->keep, nf0, ctx->front2, ctx->bstart2, ctx->part_tmp, s);
       DevState<T> sc = st;
       sc.Tot = (tt*)ctx->tot_snap;
@@ -1001,7 +1004,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     }
     // plain rule, with revert (louvain_main_process.cu:1004-1007,1026-1031)
     if (q - qprev < ctx->opts.min_modularity || it >= max_it) break;
-    accept_sweep(ctx->cur, ctx->prev, st.cp, g.n, s);  // prev = cur (louvain_main_process.cu:1029)
+    accept_sweep(ctx->cur, ctx->prev, st.cp, st.cm, g.n, s);  // prev = cur (louvain_main_process.cu:1029)
     qprev = q;
   }
   ro->q = qprev;
@@ -1476,7 +1479,7 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
     // the round's first frontier, and the list of the live graphs' vertices
     int32_t *LA = ctx->b_vl[0], *LB = ctx->b_vl[1], *BA = ctx->b_lbs[0], *BB = ctx->b_lbs[1];
     const int32_t* nA = BA + NBINS;  // length of the live list (device)
-    launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, true, it_dev, nullptr, s, ctx->order,
+    launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, true, it_dev, nullptr, nullptr, s, ctx->order,
                       ctx->bstart_full + NBINS, ctx->b_vlive);
     partition_bins(1, ctx->order, ctx->bstart_full, ctx->binid, ctx->b_vlive, nitems, LA, BA, ctx->part_tmp, s);
     partition_bins(1, LA, BA, ctx->binid, ctx->active, nitems, ctx->front, ctx->bstart_front, ctx->part_tmp, s, nA);
@@ -1514,8 +1517,8 @@ int run_batch(gala_ctx* ctx, gala_batch_result* out) {
       launch_graph_sums(gid, ctx->binid, (const int32_t*)ctx->In, (const int32_t*)ctx->Tot, g.n, b.g_in, b.g_tot2,
                         s, LA, nA);
       launch_batch_step(b, it_dev, min_mod, max_it, s);
-      launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, false, it_dev, incr ? st.cp : nullptr, s, LA,
-                        nA, ctx->b_vlive);
+      launch_batch_mask(b, gid, ctx->cur, ctx->prev, ctx->active, g.n, false, it_dev, incr ? st.cp : nullptr,
+                        incr ? st.cm : nullptr, s, LA, nA, ctx->b_vlive);
       partition_bins(1, LA, BA, ctx->binid, ctx->b_vlive, nitems, LB, BB, ctx->part_tmp, s, nA);
       copy_list(LB, LA, BB + NBINS, nitems, s);
       copy_i32(BB, BA, NBINS + 1, s);

@@ -26,11 +26,11 @@ void fill(V* p, int64_t n, V x, hipStream_t s);
 void copy_i32(const int32_t* a, int32_t* b, int64_t n, hipStream_t s);
 // b[0 .. min(*cnt, n)) = a[...]: a list whose length is on the device
 void copy_list(const int32_t* a, int32_t* b, const int32_t* cnt, int64_t n, hipStream_t s);
-// prev = cur after an accepted sweep; cp (if not null) = {cur, cur}
-void accept_sweep(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, hipStream_t s);
-// cp = {cur, cur} without touching prev: the moves of a sweep already applied
+// prev = cur after an accepted sweep; cp (if not null) = {cur, cur}, cm = cur
+void accept_sweep(const int32_t* cur, int32_t* prev, int2* cp, int32_t* cm, int64_t n, hipStream_t s);
+// cp = {cur, cur} and cm = cur without touching prev: the moves of a sweep already applied
 // to Tot and size are not applied again by the next update
-void settle_pairs(int2* cp, int64_t n, hipStream_t s);
+void settle_pairs(int2* cp, int32_t* cm, int64_t n, hipStream_t s);
 void reset_counters(Counters* c, long long min_init, hipStream_t s);
 // Stopping rule of a round on the device (louvain_main_process.cu:1004-1007,
 // 1026-1031).  round_start: qprev = Q of the counters (the singleton
@@ -43,7 +43,8 @@ void round_start(const Counters* c, RoundCtl* rc, double m2, hipStream_t s);
 void round_step(Counters* c, RoundCtl* rc, double m2, double min_mod, int it, int max_it, int32_t* bnext,
                 hipStream_t s);
 // accept_sweep when the last round_step accepted its sweep
-void accept_sweep_if(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, const RoundCtl* rc, hipStream_t s);
+void accept_sweep_if(const int32_t* cur, int32_t* prev, int2* cp, int32_t* cm, int64_t n, const RoundCtl* rc,
+                     hipStream_t s);
 
 // ---- Louvain phase kernels (kernels_louvain.hip) ----
 // Graph view on the device.  off has n+1 exclusive offsets.
@@ -135,6 +136,10 @@ struct DevState {
   // the commit of a swept vertex writes it, accept_sweep resets it to
   // {cur, cur}, so the incremental In update reads both with one gather
   int2* cp = nullptr;
+  // the same in 4 bytes for the neighbour gathers of the incremental In
+  // update: cur, with bit 31 set while the vertex has moved in the sweep in
+  // progress (half the bytes of cp, so twice the vertices per cached line)
+  int32_t* cm = nullptr;
   // integer path: each community's Tot and size deltas of a sweep packed in
   // one word (Tot delta * 2^32 + size delta, two's complement), so a
   // (community, tile) sum is one global atomic instead of two; k_apply adds
@@ -227,7 +232,8 @@ void launch_graph_sums(const int32_t* gid, const int8_t* binid, const int32_t* I
                        const int32_t* vlist = nullptr, const int32_t* vcnt = nullptr);
 void launch_batch_step(const BatchDev& b, const int32_t* it_dev, double min_mod, int max_it, hipStream_t s);
 void launch_batch_mask(const BatchDev& b, const int32_t* gid, const int32_t* cur, int32_t* prev, uint8_t* active,
-                       int32_t n, bool first, int32_t* it_dev, int2* cp, hipStream_t s, const int32_t* vlist = nullptr,
+                       int32_t n, bool first, int32_t* it_dev, int2* cp, int32_t* cm, hipStream_t s,
+                       const int32_t* vlist = nullptr,
                        const int32_t* vcnt = nullptr, uint8_t* vlive = nullptr);
 void launch_batch_live(const BatchDev& b, int32_t* it_dev, hipStream_t s);
 void launch_batch_round(const BatchDev& b, int round, double min_mod, int max_rounds, int32_t* nopen, hipStream_t s);

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_batch_mask(BatchDev b, const int32_t* _
                                                      const int32_t* __restrict__ cur, int32_t* __restrict__ prev,
                                                      uint8_t* __restrict__ active, int32_t n, int first,
                                                      int32_t* __restrict__ it_dev, int2* __restrict__ cp,
-                                                     const int32_t* __restrict__ vlist,
+                                                     int32_t* __restrict__ cm, const int32_t* __restrict__ vlist,
                                                      const int32_t* __restrict__ vcnt, uint8_t* __restrict__ vlive) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *it_dev += 1;
   if (vlist) n = *vcnt;
@@ -116,6 +116,7 @@ __global__ __launch_bounds__(256) void k_batch_mask(BatchDev b, const int32_t* _
     int p = prev[v];
     if (b.accept[g]) prev[v] = p = cur[v];
     if (cp) cp[v] = make_int2(p, p);  // no vertex has moved in the next sweep yet
+    if (cm) cm[v] = p;
     if (!b.live[g]) active[v] = 0;
   }
 }
@@ -211,10 +212,10 @@ void launch_batch_step(const BatchDev& b, const int32_t* it_dev, double min_mod,
   k_batch_step<<<blocks_for(b.G, 256, 1 << 20), 256, 0, s>>>(b, it_dev, min_mod, max_it);
 }
 void launch_batch_mask(const BatchDev& b, const int32_t* gid, const int32_t* cur, int32_t* prev, uint8_t* active,
-                       int32_t n, bool first, int32_t* it_dev, int2* cp, hipStream_t s, const int32_t* vlist,
-                       const int32_t* vcnt, uint8_t* vlive) {
-  k_batch_mask<<<blocks_for(n, 256, 8192), 256, 0, s>>>(b, gid, cur, prev, active, n, first ? 1 : 0, it_dev, cp, vlist,
-                                                        vcnt, vlive);
+                       int32_t n, bool first, int32_t* it_dev, int2* cp, int32_t* cm, hipStream_t s,
+                       const int32_t* vlist, const int32_t* vcnt, uint8_t* vlive) {
+  k_batch_mask<<<blocks_for(n, 256, 8192), 256, 0, s>>>(b, gid, cur, prev, active, n, first ? 1 : 0, it_dev, cp, cm,
+                                                        vlist, vcnt, vlive);
 }
 void launch_batch_live(const BatchDev& b, int32_t* it_dev, hipStream_t s) {
   k_batch_live<<<blocks_for(b.G, 256, 1 << 20), 256, 0, s>>>(b, it_dev);

@@ -407,31 +407,35 @@ __global__ void k_copy_i32(const int32_t* __restrict__ a, int32_t* __restrict__ 
     b[i] = a[i];
 }
 __global__ void k_accept(const int32_t* __restrict__ cur, int32_t* __restrict__ prev, int2* __restrict__ cp,
-                         int64_t n) {
+                         int32_t* __restrict__ cm, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int c = cur[i];
     prev[i] = c;
     cp[i] = make_int2(c, c);
+    if (cm) cm[i] = c;
   }
 }
-void accept_sweep(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, hipStream_t s) {
+void accept_sweep(const int32_t* cur, int32_t* prev, int2* cp, int32_t* cm, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   if (!cp) { copy_i32(cur, prev, n, s); return; }
   int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
-  k_accept<<<nb, 256, 0, s>>>(cur, prev, cp, n);
+  k_accept<<<nb, 256, 0, s>>>(cur, prev, cp, cm, n);
 }
 
-__global__ void k_settle_pairs(int2* __restrict__ cp, int64_t n) {
+__global__ void k_settle_pairs(int2* __restrict__ cp, int32_t* __restrict__ cm, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int2 p = cp[i];
-    if (p.x != p.y) cp[i] = make_int2(p.x, p.x);
+    if (p.x != p.y) {
+      cp[i] = make_int2(p.x, p.x);
+      if (cm) cm[i] = p.x;
+    }
   }
 }
-void settle_pairs(int2* cp, int64_t n, hipStream_t s) {
+void settle_pairs(int2* cp, int32_t* cm, int64_t n, hipStream_t s) {
   if (n <= 0 || !cp) return;
   int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
-  k_settle_pairs<<<nb, 256, 0, s>>>(cp, n);
+  k_settle_pairs<<<nb, 256, 0, s>>>(cp, cm, n);
 }
 
 __global__ void k_copy_list(const int32_t* __restrict__ a, int32_t* __restrict__ b, const int32_t* __restrict__ cnt,
@@ -506,19 +510,21 @@ void round_step(Counters* c, RoundCtl* rc, double m2, double min_mod, int it, in
 }
 
 __global__ void k_accept_if(const int32_t* __restrict__ cur, int32_t* __restrict__ prev, int2* __restrict__ cp,
-                            int64_t n, const RoundCtl* __restrict__ rc) {
+                            int32_t* __restrict__ cm, int64_t n, const RoundCtl* __restrict__ rc) {
   if (!rc->accept) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int c = cur[i];
     prev[i] = c;
     if (cp) cp[i] = make_int2(c, c);
+    if (cm) cm[i] = c;
   }
 }
-void accept_sweep_if(const int32_t* cur, int32_t* prev, int2* cp, int64_t n, const RoundCtl* rc, hipStream_t s) {
+void accept_sweep_if(const int32_t* cur, int32_t* prev, int2* cp, int32_t* cm, int64_t n, const RoundCtl* rc,
+                     hipStream_t s) {
   if (n <= 0) return;
   int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
-  k_accept_if<<<nb, 256, 0, s>>>(cur, prev, cp, n, rc);
+  k_accept_if<<<nb, 256, 0, s>>>(cur, prev, cp, cm, n, rc);
 }
 
 __global__ void k_reset_counters(Counters* c, long long min_init) {

@@ -96,6 +96,7 @@ __device__ __forceinline__ bool commit_move(int v, int c, typename T::gain_t bes
   }
   st.cur[v] = nc;
   if (st.cp) st.cp[v] = make_int2(nc, c);
+  if (st.cm) st.cm[v] = nc != c ? (int)((unsigned)nc | 0x80000000u) : nc;
   if (st.track) {
     st.last_gain[v] = (float)((double)gain / ((double)m2 * (double)K));
     if (nc != c) st.moved_it[v] = st.iter;
@@ -118,6 +119,7 @@ __device__ __forceinline__ bool commit_move_sz(int v, int c, typename T::gain_t 
   }
   st.cur[v] = nc;
   if (st.cp) st.cp[v] = make_int2(nc, c);
+  if (st.cm) st.cm[v] = nc != c ? (int)((unsigned)nc | 0x80000000u) : nc;
   if (st.track) {
     st.last_gain[v] = (float)((double)gain / ((double)m2 * (double)K));
     if (nc != c) st.moved_it[v] = st.iter;
@@ -193,6 +195,7 @@ __global__ __launch_bounds__(256) void k_init_small(DevGraph<T> g, DevState<T> s
     st.prev[v] = v;
     st.cur[v] = v;
     if (st.cp) st.cp[v] = make_int2(v, v);
+    if (st.cm) st.cm[v] = v;
     st.active[v] = 1;
     st.dirty[v] = 0;
     if (st.track) {
@@ -3007,20 +3010,22 @@ __device__ __forceinline__ typename T::tot_t moved_row_span(const DevGraph<T>& g
   for (int j0 = lane; j0 < d; j0 += NT * MV_U) {
     int u[MV_U];
     tt w[MV_U];
-    int2 cpu[MV_U];
+    int cmu[MV_U];
 #pragma unroll
     for (int r = 0; r < MV_U; r++) {
       const int j = j0 + r * NT;
       u[r] = j < d ? g.nbr[o + j] : -1;
       w[r] = j < d ? (tt)load_w<T>(g, o + j) : tt(0);
     }
+    // cur[u] with the moved flag in bit 31; -1 past the row (community
+    // 0x7fffffff, moved: no contribution)
 #pragma unroll
-    for (int r = 0; r < MV_U; r++) cpu[r] = u[r] >= 0 ? st.cp[u[r]] : make_int2(-1, -2);  // {cur[u], prev[u]}
+    for (int r = 0; r < MV_U; r++) cmu[r] = u[r] >= 0 ? st.cm[u[r]] : -1;
 #pragma unroll
     for (int r = 0; r < MV_U; r++) {
-      const int cu = cpu[r].x;
+      const int cu = cmu[r] & 0x7fffffff;
       if (cu == c) x += w[r];
-      if (cu == cpu[r].y) {  // u stayed: its In gains or loses this entry
+      if (cmu[r] >= 0) {  // u stayed: its In gains or loses this entry
         const tt dl = (cu == c ? w[r] : tt(0)) - (cu == p ? w[r] : tt(0));
         if (dl) atomicAdd(&st.In[u[r]], dl);
       }
