@@ -686,7 +686,7 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
   st.track = track;
   // Iteration 0: Q on the singleton partition (louvain_main_process.cu:538-548)
   reset_counters(ctx->ctr, (long long)m2, s);
-  launch_apply<T>(st, g.n, m2, s);
+  launch_apply<T>(st, g.n, m2, s, nullptr, nullptr, nullptr, nullptr, false);  // nobody has moved yet
   if (unique_rows) {
     // singleton partition, at most one self entry per row: In = Self
     HIPOK(hipMemcpyAsync(st.In, st.Self, sizeof(tt) * g.n, hipMemcpyDeviceToDevice, s));
@@ -888,7 +888,8 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
       std::swap(bcur, bnext);
       const bool more = it < max_it;
       if (more) {  // sweep it + 1
-        accept_sweep_if(ctx->cur, ctx->prev, st.cp, st.cm, g.n, ctx->rctl, s);
+        // the swept frontier (now in fnext / bnext) holds every vertex that moved
+        accept_list(ctx->cur, ctx->prev, st.cp, st.cm, fnext, bnext + NBINS, nitems, ctx->rctl, s);
         st.iter = it + 1;
         reset_counters(ctx->ctr, (long long)m2, s);
         if ((rc = decide(st, fcur, bcur, false, true, ctx->sev[(it + 1) & 1]))) return fail(rc);
@@ -1004,7 +1005,10 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     }
     // plain rule, with revert (louvain_main_process.cu:1004-1007,1026-1031)
     if (q - qprev < ctx->opts.min_modularity || it >= max_it) break;
-    accept_sweep(ctx->cur, ctx->prev, st.cp, st.cm, g.n, s);  // prev = cur (louvain_main_process.cu:1029)
+    // prev = cur (louvain_main_process.cu:1029); after a plain sweep only its
+    // frontier (now in fnext / bnext) can hold movers
+    if (lsweep) accept_sweep(ctx->cur, ctx->prev, st.cp, st.cm, g.n, s);
+    else accept_list(ctx->cur, ctx->prev, st.cp, st.cm, fnext, bnext + NBINS, nitems, nullptr, s);
     qprev = q;
   }
   ro->q = qprev;

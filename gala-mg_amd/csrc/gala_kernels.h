@@ -42,9 +42,10 @@ void reset_counters(Counters* c, long long min_init, hipStream_t s);
 void round_start(const Counters* c, RoundCtl* rc, double m2, hipStream_t s);
 void round_step(Counters* c, RoundCtl* rc, double m2, double min_mod, int it, int max_it, int32_t* bnext,
                 hipStream_t s);
-// accept_sweep when the last round_step accepted its sweep
-void accept_sweep_if(const int32_t* cur, int32_t* prev, int2* cp, int32_t* cm, int64_t n, const RoundCtl* rc,
-                     hipStream_t s);
+// accept_sweep over the vertices list[0, *cnt) (the swept frontier; *cnt <=
+// nmax), when rc is null or the last round_step accepted its sweep
+void accept_list(const int32_t* cur, int32_t* prev, int2* cp, int32_t* cm, const int32_t* list, const int32_t* cnt,
+                 int64_t nmax, const RoundCtl* rc, hipStream_t s);
 
 // ---- Louvain phase kernels (kernels_louvain.hip) ----
 // Graph view on the device.  off has n+1 exclusive offsets.
@@ -201,13 +202,15 @@ void launch_decide_first(int bin, const DevGraph<T>& g, DevState<T>& st, const i
 
 // Tot and size take the sweep's moves (from the {cur, prev} pairs, tile sums
 // per community), then sums of Tot^2, min Tot, non-empty count.
-template <class T>
 // Tot/size take the deltas of the movers among dlist[0 .. *dcnt) (the swept
 // vertices: only they can have moved; every vertex when null, else vlist),
 // then sum Tot^2, min Tot and the non-empty count over the communities of
-// vlist[0 .. *vcnt) (all n when null).
+// vlist[0 .. *vcnt) (all n when null).  moves: false when no vertex has moved
+// since Tot and size were set (skips the delta pass).
+template <class T>
 void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s, const int32_t* vlist = nullptr,
-                  const int32_t* vcnt = nullptr, const int32_t* dlist = nullptr, const int32_t* dcnt = nullptr);
+                  const int32_t* vcnt = nullptr, const int32_t* dlist = nullptr, const int32_t* dcnt = nullptr,
+                  bool moves = true);
 
 // In[v] over order[bstart[b] .. bstart[b+1]) (only the dirty vertices when only_dirty).
 // variant 1: pipelined kernels for bins 0-4 (same result).

@@ -509,23 +509,30 @@ void round_step(Counters* c, RoundCtl* rc, double m2, double min_mod, int it, in
   k_round_step<<<1, 64, 0, s>>>(c, rc, m2, min_mod, it, max_it, bnext);
 }
 
-__global__ void k_accept_if(const int32_t* __restrict__ cur, int32_t* __restrict__ prev, int2* __restrict__ cp,
-                            int32_t* __restrict__ cm, int64_t n, const RoundCtl* __restrict__ rc) {
-  if (!rc->accept) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int c = cur[i];
-    prev[i] = c;
-    if (cp) cp[i] = make_int2(c, c);
-    if (cm) cm[i] = c;
+// accept_sweep over the vertices of a list whose length is on the device:
+// the vertices of a sweep's frontier are the only ones whose cur can differ
+// from prev (every other vertex already has prev = cur, cp = {cur, cur} and
+// cm = cur).  rc (if given): only when the last round_step accepted.
+__global__ void k_accept_list(const int32_t* __restrict__ cur, int32_t* __restrict__ prev, int2* __restrict__ cp,
+                              int32_t* __restrict__ cm, const int32_t* __restrict__ list,
+                              const int32_t* __restrict__ cnt, int64_t nmax, const RoundCtl* __restrict__ rc) {
+  if (rc && !rc->accept) return;
+  const int64_t m = *cnt < nmax ? *cnt : nmax;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    const int v = list[i];
+    const int c = cur[v];
+    prev[v] = c;
+    cp[v] = make_int2(c, c);
+    if (cm) cm[v] = c;
   }
 }
-void accept_sweep_if(const int32_t* cur, int32_t* prev, int2* cp, int32_t* cm, int64_t n, const RoundCtl* rc,
-                     hipStream_t s) {
-  if (n <= 0) return;
-  int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
-  k_accept_if<<<nb, 256, 0, s>>>(cur, prev, cp, cm, n, rc);
+void accept_list(const int32_t* cur, int32_t* prev, int2* cp, int32_t* cm, const int32_t* list, const int32_t* cnt,
+                 int64_t nmax, const RoundCtl* rc, hipStream_t s) {
+  if (nmax <= 0) return;
+  int nb = (int)std::min<int64_t>((nmax + 255) / 256, 8192);
+  k_accept_list<<<nb, 256, 0, s>>>(cur, prev, cp, cm, list, cnt, nmax, rc);
 }
+
 
 __global__ void k_reset_counters(Counters* c, long long min_init) {
   if (threadIdx.x == 0) {

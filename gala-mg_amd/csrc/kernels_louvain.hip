@@ -2732,11 +2732,11 @@ __global__ __launch_bounds__(256) void k_apply_fin(DevState<T> st, int nb) {
 
 template <class T>
 void launch_apply(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t s, const int32_t* vlist,
-                  const int32_t* vcnt, const int32_t* dlist, const int32_t* dcnt) {
+                  const int32_t* vcnt, const int32_t* dlist, const int32_t* dcnt, bool moves) {
   if (n <= 0) return;
   const int nb = std::min((n + 255) / 256, MAX_PARTS);
   if (!dlist) { dlist = vlist; dcnt = vcnt; }
-  k_tot_delta<T><<<std::min((n + TD_T - 1) / TD_T, 2048), 256, 0, s>>>(st, n, dlist, dcnt);
+  if (moves) k_tot_delta<T><<<std::min((n + TD_T - 1) / TD_T, 2048), 256, 0, s>>>(st, n, dlist, dcnt);
   k_apply<T><<<nb, 256, 0, s>>>(st, n, m2, vlist, vcnt);
   k_apply_fin<T><<<1, 256, 0, s>>>(st, nb);
 }
@@ -3224,7 +3224,7 @@ void launch_prune(DevState<T>& st, int32_t n, typename T::tot_t m2, hipStream_t 
   template void launch_decide_first<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,     \
                                        const int32_t*, int, typename T::tot_t, hipStream_t, int); \
   template void launch_apply<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t, const int32_t*,        \
-                                const int32_t*, const int32_t*, const int32_t*);                  \
+                                const int32_t*, const int32_t*, const int32_t*, bool);            \
   template void launch_weight<T>(int, const DevGraph<T>&, DevState<T>&, const int32_t*,           \
                                  const int32_t*, int, bool, hipStream_t, int);                    \
   template void launch_prune<T>(DevState<T>&, int32_t, typename T::tot_t, hipStream_t, const int32_t*,        \
