@@ -21,9 +21,9 @@
 //                 the wide path), overflow into a global slice, register bitonic
 //                 sort of <= LT keys (LDS only for exchanges between waves)
 //   L >  LT       dense rows: the segment is cut into chunks, one workgroup per
-//                 chunk pre-aggregates in LDS and adds into a C-wide accumulator
-//                 for the row (HBM is plentiful); a tiled scan of the accumulator
-//                 then emits the row already in key order
+//                 chunk pre-aggregates in LDS; the chunks' pairs are packed row
+//                 after row, radix sorted by key within each row and their runs
+//                 merged, so the row comes out in key order
 // -> exclusive scan of the distinct counts -> compaction into the new CSR.
 #include "gala_kernels.h"
 #include "gala_device.h"
