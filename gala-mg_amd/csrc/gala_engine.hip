@@ -1139,19 +1139,32 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
     int64_t m = 0;
     const int dst = dst_of(src);
     const bool canonical = ctx->opts.dump_rounds != 0;
-    rc = contract<T>(ctx, g, ctx->prev, dst, ro.caps, &C, &m, canonical);
-    if (rc) return rc;
-    compose_map(ctx->fin, ctx->n0, ctx->c_label, s);
-    HIPOK(hipEventRecord(ctx->ev[7], s));
-    // entries of the contracted graph: the sum of the slots of c_mtotal
-    int64_t* m_rd = (int64_t*)(ctx->hpin + HPIN_BYTES - 8 * 16 * MTOTAL_SLOTS);
-    if (!canonical)
-      HIPOK(hipMemcpyAsync(m_rd, ctx->c_mtotal, 8 * 16 * MTOTAL_SLOTS, hipMemcpyDeviceToHost, s));
-    HIPOK(hipStreamSynchronize(s));
-    int64_t m_slots = 0;
-    if (!canonical)
-      for (int k = 0; k < MTOTAL_SLOTS; k++) m_slots += m_rd[16 * k];
-    const int64_t m_out = canonical ? m : m_slots;
+    // A round whose first sweep was refused keeps the singleton partition:
+    // its contraction is the identity on duplicate-free rows (C = n, the
+    // same rows under the same ids) and composes nothing, so it is skipped
+    // and the round graph stays.  (Q then equals the last round's: the run
+    // ends here unless min_modularity <= 0.)
+    const bool identity = ro.iters == 1 && unique_rows && !rcb;
+    int64_t m_out = m_in;
+    if (identity) {
+      C = g.n;
+      HIPOK(hipEventRecord(ctx->ev[7], s));
+      HIPOK(hipStreamSynchronize(s));
+    } else {
+      rc = contract<T>(ctx, g, ctx->prev, dst, ro.caps, &C, &m, canonical);
+      if (rc) return rc;
+      compose_map(ctx->fin, ctx->n0, ctx->c_label, s);
+      HIPOK(hipEventRecord(ctx->ev[7], s));
+      // entries of the contracted graph: the sum of the slots of c_mtotal
+      int64_t* m_rd = (int64_t*)(ctx->hpin + HPIN_BYTES - 8 * 16 * MTOTAL_SLOTS);
+      if (!canonical)
+        HIPOK(hipMemcpyAsync(m_rd, ctx->c_mtotal, 8 * 16 * MTOTAL_SLOTS, hipMemcpyDeviceToHost, s));
+      HIPOK(hipStreamSynchronize(s));
+      int64_t m_slots = 0;
+      if (!canonical)
+        for (int k = 0; k < MTOTAL_SLOTS; k++) m_slots += m_rd[16 * k];
+      m_out = canonical ? m : m_slots;
+    }
     if (round < 64) {
       R.round_vertices[round] = g.n;
       R.round_entries[round] = m_in;
@@ -1207,8 +1220,10 @@ int run_all(gala_ctx* ctx, gala_iter_cb icb, gala_round_cb rcb, void* user, gala
       }
     }
     // next round's graph = contracted graph
-    src = dst;
-    g = graph_at<T>(ctx, src, C, m);
+    if (!identity) {
+      src = dst;
+      g = graph_at<T>(ctx, src, C, m);
+    }
     round++;
     R.rounds = round;
     R.n_communities = C;

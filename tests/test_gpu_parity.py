@@ -147,6 +147,29 @@ def test_queued_rounds_equal_synchronous(golden, case):
         assert capped > 0 and max(rq["round_iters"]) == 2
 
 
+def test_identity_round_contraction_skipped():
+    # a last round whose first sweep is refused keeps the singleton partition;
+    # its contraction (the identity) is skipped unless the round callback
+    # asks for the contracted graph: both runs report the same rounds,
+    # community counts, entries, Q and partition
+    say("identity round")
+    g = gala.gen_chung_lu(200000, 2.1, 17.35, 12000.0, 3)
+    out = []
+    for with_rounds in (False, True):
+        eng = gala.Engine()
+        eng.set_graph(g)
+        r = eng.run(rounds=[] if with_rounds else None)
+        out.append((r, eng.communities(g.vertex_num)))
+        eng.close()
+    (a, ca), (b, cb) = out
+    assert a["round_iters"][-1] == 1  # the workload ends on such a round
+    for k in ("modularity", "rounds", "round_nc", "round_q", "round_iters", "round_entries",
+              "round_contracted_entries", "round_vertices", "n_communities"):
+        assert a[k] == b[k], k
+    assert np.array_equal(ca, cb)
+    assert a["round_build_ms"][-1] < b["round_build_ms"][-1]
+
+
 def variant_set(k):
     return np.array([min(k, c - 1) for c in gala.variant_counts()], np.int32)
 
