@@ -739,7 +739,10 @@ __global__ __launch_bounds__(256) void k_rs_hist(const DenseChunk* __restrict__ 
   }
 }
 
-// stable scatter of every tile's pairs to their digit positions
+// stable scatter of every tile's pairs to their digit positions.  The tile is
+// first ordered by digit in LDS, so that the global writes of a digit's run
+// come from consecutive threads to consecutive addresses (written straight
+// from the ranking, every lane of a wave would hit a different line).
 template <int LT, int RB, class T>
 __global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restrict__ tiles, int ntiles,
                                                      const int32_t* __restrict__ kin,
@@ -749,8 +752,13 @@ __global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restric
   using acc = typename T::acc_t;
   constexpr int NW = 16, R = (LT / 2) / (NW * 64);  // rounds of 64 pairs per wave
   constexpr int NB = 1 << RB;
+  static_assert(NB <= 1024, "one digit per thread in the digit scan");
   __shared__ int wcnt[NW][NB];
   __shared__ int64_t gofs[NB];
+  __shared__ int lstart[NB];  // the tile's first position of each digit
+  __shared__ int s_key[LT / 2];
+  __shared__ acc s_val[LT / 2];
+  __shared__ int sh[17];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned long long below_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -787,24 +795,36 @@ __global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restric
       rk[r] = base + below;
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < NB; d += 1024) {  // exclusive prefix over the waves, per digit
-      int run = 0;
+    int dtot = 0;  // the tile's pairs of digit threadIdx.x
+    if (threadIdx.x < NB) {  // exclusive prefix over the waves, per digit
+      const int d = threadIdx.x;
       for (int q = 0; q < NW; q++) {
         const int x = wcnt[q][d];
-        wcnt[q][d] = run;
-        run += x;
+        wcnt[q][d] = dtot;
+        dtot += x;
       }
     }
+    int32_t tot;
+    const int dstart = block_excl_scan<int32_t>(dtot, sh, &tot);
+    if (threadIdx.x < NB) lstart[threadIdx.x] = dstart;
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int i = (w * R + r) * 64 + lane;
       if (i < n) {
         const int d = (key[r] >> shift) & (NB - 1);
-        const int64_t pos = gofs[d] + wcnt[w][d] + rk[r];
-        kout[pos] = key[r];
-        vout[pos] = val[r];
+        const int lp = lstart[d] + wcnt[w][d] + rk[r];
+        s_key[lp] = key[r];
+        s_val[lp] = val[r];
       }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 1024) {
+      const int k = s_key[i];
+      const int d = (k >> shift) & (NB - 1);
+      const int64_t pos = gofs[d] + (i - lstart[d]);
+      kout[pos] = k;
+      vout[pos] = s_val[i];
     }
     __syncthreads();
   }
