@@ -617,11 +617,10 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
 //     absolute output position, and k_rs_scatter ranks each tile's pairs
 //     stably (per wave with RB ballots, then across waves) and writes them to
 //     the other packed buffer.
-//  3. k_rs_heads / scan / k_rs_emit / k_rs_sum: equal keys are adjacent;
-//     each run head writes its key at the run's rank from the start of the
-//     row's staged segment, and every pair adds its value into that slot, so
-//     the row's distinct pairs end up in ascending key order, dcnt[c] = their
-//     count.
+//  3. k_rs_heads / scan over tiles / k_rs_emit: equal keys are adjacent;
+//     each run head writes its key and the run's sum at the run's rank from
+//     the start of the row's staged segment, so the row's distinct pairs end
+//     up in ascending key order, dcnt[c] = their count.
 // A tile table entry (DenseChunk) holds: slot = row, c = community, start =
 // packed offset of the tile, len = pairs, first/nch = the row's first tile
 // and tile count.
@@ -830,74 +829,69 @@ __global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restric
   }
 }
 
-// run heads of the sorted packed pairs (the first pair of a row is a head)
+// Run merge of the sorted packed pairs: equal keys are adjacent, and the
+// first pair of a row or of a new key is a run head.  k_rs_heads counts each
+// tile's heads; after an exclusive scan over the tiles, k_rs_emit ranks every
+// head inside its tile (block scan) and writes its key with the run's sum at
+// that rank from the start of the row's staged segment (a run holds at most
+// one pair per chunk of the row, so a head sums it by walking forward, past
+// the tile's end if needed); the row's first tile writes its distinct count.
+constexpr int RS_T = 1024, RS_PT = 8;  // k_rs_emit: threads and pairs per thread (tiles <= LT/2 = RS_T * RS_PT)
 __global__ __launch_bounds__(256) void k_rs_heads(const DenseChunk* __restrict__ tiles, int ntiles,
-                                                  const int32_t* __restrict__ keys, uint8_t* __restrict__ heads) {
+                                                  const int32_t* __restrict__ keys, int64_t* __restrict__ tcnt) {
+  __shared__ int64_t sh[17];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const DenseChunk tt = tiles[t];
     const int64_t seg0 = tiles[tt.first].start;
+    int64_t c = 0;
     for (int i = threadIdx.x; i < tt.len; i += 256) {
       const int64_t g = tt.start + i;
-      heads[g] = (g == seg0 || keys[g] != keys[g - 1]) ? 1 : 0;
+      c += (g == seg0 || keys[g] != keys[g - 1]) ? 1 : 0;
     }
+    int64_t tot;
+    (void)block_excl_scan<int64_t>(c, sh, &tot);
+    if (threadIdx.x == 0) tcnt[t] = tot;
   }
 }
 
-// every run head writes the row's i-th distinct key (value zeroed), the
-// row's first tile its distinct count ...
 template <class T>
-__global__ __launch_bounds__(256) void k_rs_emit(const DenseChunk* __restrict__ tiles, int ntiles,
-                                                 const int32_t* __restrict__ keys, const uint8_t* __restrict__ heads,
-                                                 const int64_t* __restrict__ hs, const int64_t* __restrict__ eoff,
-                                                 int32_t* __restrict__ skey, typename T::acc_t* __restrict__ sval,
-                                                 int32_t* __restrict__ dcnt) {
+__global__ __launch_bounds__(RS_T) void k_rs_emit(const DenseChunk* __restrict__ tiles, int ntiles,
+                                                  const int32_t* __restrict__ keys,
+                                                  const typename T::acc_t* __restrict__ vals,
+                                                  const int64_t* __restrict__ tpos, const int64_t* __restrict__ eoff,
+                                                  int32_t* __restrict__ skey, typename T::acc_t* __restrict__ sval,
+                                                  int32_t* __restrict__ dcnt) {
+  using acc = typename T::acc_t;
+  __shared__ int32_t sh[17];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const DenseChunk tt = tiles[t];
     const DenseChunk tl = tiles[tt.first + tt.nch - 1];
     const int64_t seg0 = tiles[tt.first].start, seg1 = tl.start + tl.len;
-    const int64_t out0 = eoff[tt.c];
-    if (t == tt.first && threadIdx.x == 0) dcnt[tt.c] = (int32_t)(hs[seg1] - hs[seg0]);
-    for (int i = threadIdx.x; i < tt.len; i += 256) {
-      const int64_t g = tt.start + i;
-      if (!heads[g]) continue;
-      const int64_t o = out0 + (hs[g] - hs[seg0]);
-      skey[o] = keys[g];
-      sval[o] = 0;
-    }
-  }
-}
-
-// ... then every pair's value goes into its run's slot (runs hold at most
-// one pair per chunk of the row): summed per wave first, one atomic per run
-// and wave
-template <class T>
-__global__ __launch_bounds__(256) void k_rs_sum(const DenseChunk* __restrict__ tiles, int ntiles,
-                                                const typename T::acc_t* __restrict__ vals,
-                                                const uint8_t* __restrict__ heads, const int64_t* __restrict__ hs,
-                                                const int64_t* __restrict__ eoff, typename T::acc_t* __restrict__ sval) {
-  using acc = typename T::acc_t;
-  const int lane = threadIdx.x & 63;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const DenseChunk tt = tiles[t];
-    const int64_t seg0 = tiles[tt.first].start;
-    const int64_t out0 = eoff[tt.c] - hs[seg0];
-    for (int i0 = 0; i0 < tt.len; i0 += 256) {  // uniform trip count per workgroup
-      const int i = i0 + threadIdx.x;
-      const bool ok = i < tt.len;
-      const int64_t g = tt.start + i;
-      // run slot of the pair (runs are contiguous), -1 past the tile end
-      const int64_t r = ok ? out0 + hs[g] - (heads[g] ? 0 : 1) : -1;
-      acc x = ok ? vals[g] : acc(0);
-      // segmented suffix sums over the wave: the first lane of each run in
-      // the wave then holds the run's wave-local sum and adds it once
+    const int64_t row0 = tpos[tt.first];
+    if (t == tt.first && threadIdx.x == 0) dcnt[tt.c] = (int32_t)(tpos[tt.first + tt.nch] - row0);
+    const int i0 = threadIdx.x * RS_PT;
+    bool hd[RS_PT];
+    int c = 0;
 #pragma unroll
-      for (int k = 1; k < 64; k <<= 1) {
-        const acc xo = __shfl_down(x, k, 64);
-        const int64_t ro = __shfl_down(r, k, 64);
-        if (lane + k < 64 && ro == r) x += xo;
-      }
-      const int64_t rp = __shfl_up(r, 1, 64);
-      if (ok && (lane == 0 || rp != r)) atomicAdd(&sval[r], x);
+    for (int k = 0; k < RS_PT; k++) {
+      const int i = i0 + k;
+      const int64_t g = tt.start + i;
+      hd[k] = i < tt.len && (g == seg0 || keys[g] != keys[g - 1]);
+      c += hd[k] ? 1 : 0;
+    }
+    int32_t tot;
+    int r = block_excl_scan<int32_t>(c, sh, &tot);
+    const int64_t out0 = eoff[tt.c] + (tpos[t] - row0);
+#pragma unroll
+    for (int k = 0; k < RS_PT; k++) {
+      if (!hd[k]) continue;
+      const int64_t g = tt.start + i0 + k;
+      const int key = keys[g];
+      acc x = vals[g];
+      for (int64_t h = g + 1; h < seg1 && keys[h] == key; h++) x += vals[h];
+      skey[out0 + r] = key;
+      sval[out0 + r] = x;
+      r++;
     }
   }
 }
@@ -905,7 +899,7 @@ __global__ __launch_bounds__(256) void k_rs_sum(const DenseChunk* __restrict__ t
 // Scratch layout of the dense-row sort (offsets in bytes, 256-byte aligned).
 constexpr int64_t RS_NB_MAX = 1024;  // buckets of the widest radix digit (10 bits)
 struct DenseLayout {
-  int64_t ccnt, pstart, bound, tiles, hist, hscan, k0, k1, v0, v1, heads, hs, tmp, total;
+  int64_t ccnt, pstart, bound, tiles, hist, hscan, k0, k1, v0, v1, tcnt, tpos, tmp, total;
 };
 // nch chunks, nrows rows, P staged entries; tiles of the packed rows are at
 // most nch + nrows
@@ -924,8 +918,8 @@ static DenseLayout dense_layout(int64_t nch, int64_t nrows, int64_t P, int64_t v
   L.k1 = o; o = al(o + 4 * P);
   L.v0 = o; o = al(o + vsz * P);
   L.v1 = o; o = al(o + vsz * P);
-  L.heads = o; o = al(o + P);
-  L.hs = o; o = al(o + 8 * (P + 1));
+  L.tcnt = o; o = al(o + 8 * nt);
+  L.tpos = o; o = al(o + 8 * (nt + 1));
   L.tmp = o; o = al(o + 8 * (scan_tmp_elems(std::max<int64_t>(P, RS_NB_MAX * nt)) + 1));
   L.total = o;
   return L;
@@ -1160,8 +1154,8 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     int64_t* hscan = (int64_t*)(d + L.hscan);
     int32_t* kb[2] = {(int32_t*)(d + L.k0), (int32_t*)(d + L.k1)};
     acc* vb[2] = {(acc*)(d + L.v0), (acc*)(d + L.v1)};
-    uint8_t* heads = (uint8_t*)(d + L.heads);
-    int64_t* hs = (int64_t*)(d + L.hs);
+    int64_t* tcnt = (int64_t*)(d + L.tcnt);
+    int64_t* tpos = (int64_t*)(d + L.tpos);
     int64_t* tmp = (int64_t*)(d + L.tmp);
     if (hipMemcpyAsync(cb.chunks, plan.chunks.data(), sizeof(DenseChunk) * nch, hipMemcpyHostToDevice, s) !=
         hipSuccess)
@@ -1213,11 +1207,10 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     }
     const int32_t* kf = kb[(npass - 1) & 1];
     const acc* vf = vb[(npass - 1) & 1];
-    const int64_t Pk = hb[nrows];  // packed pairs
-    k_rs_heads<<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kf, heads);
-    scan_exclusive<uint8_t, int64_t>(heads, hs, Pk, tmp, hs + Pk, s);
-    k_rs_emit<T><<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kf, heads, hs, cb.eoff, cb.skey, cb.sval, cb.dcnt);
-    k_rs_sum<T><<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, vf, heads, hs, cb.eoff, cb.sval);
+    static_assert(LT / 2 <= RS_T * RS_PT, "a tile fits one k_rs_emit workgroup");
+    k_rs_heads<<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kf, tcnt);
+    scan_exclusive<int64_t, int64_t>(tcnt, tpos, ntl, tmp, tpos + ntl, s);
+    k_rs_emit<T><<<std::min(ntl, 4096), RS_T, 0, s>>>(tiles, ntl, kf, vf, tpos, cb.eoff, cb.skey, cb.sval, cb.dcnt);
     if (hipStreamSynchronize(s) != hipSuccess) return -1;  // the host tile vector is in flight until here
   }
   if (out.end) {  // gapped: every row stays at the start of its staging segment
