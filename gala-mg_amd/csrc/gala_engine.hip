@@ -1006,8 +1006,10 @@ int run_round(gala_ctx* ctx, const DevGraph<T>& g, typename T::tot_t m2, int rou
     // plain rule, with revert (louvain_main_process.cu:1004-1007,1026-1031)
     if (q - qprev < ctx->opts.min_modularity || it >= max_it) break;
     // prev = cur (louvain_main_process.cu:1029); after a plain sweep only its
-    // frontier (now in fnext / bnext) can hold movers
-    if (lsweep) accept_sweep(ctx->cur, ctx->prev, st.cp, st.cm, g.n, s);
+    // frontier (now in fnext / bnext) can hold movers: a short frontier is
+    // visited through its list, a long one (its gathers and scattered writes
+    // cost more than streaming every vertex) by the pass over all n
+    if (lsweep || 4 * (int64_t)c.swept > (int64_t)g.n) accept_sweep(ctx->cur, ctx->prev, st.cp, st.cm, g.n, s);
     else accept_list(ctx->cur, ctx->prev, st.cp, st.cm, fnext, bnext + NBINS, nitems, nullptr, s);
     qprev = q;
   }

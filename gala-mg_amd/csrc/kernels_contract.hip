@@ -836,7 +836,7 @@ __global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restric
 // that rank from the start of the row's staged segment (a run holds at most
 // one pair per chunk of the row, so a head sums it by walking forward, past
 // the tile's end if needed); the row's first tile writes its distinct count.
-constexpr int RS_T = 1024, RS_PT = 8;  // k_rs_emit: threads and pairs per thread (tiles <= LT/2 = RS_T * RS_PT)
+constexpr int RS_T = 1024;  // k_rs_emit threads
 __global__ __launch_bounds__(256) void k_rs_heads(const DenseChunk* __restrict__ tiles, int ntiles,
                                                   const int32_t* __restrict__ keys, int64_t* __restrict__ tcnt) {
   __shared__ int64_t sh[17];
@@ -862,37 +862,46 @@ __global__ __launch_bounds__(RS_T) void k_rs_emit(const DenseChunk* __restrict__
                                                   int32_t* __restrict__ skey, typename T::acc_t* __restrict__ sval,
                                                   int32_t* __restrict__ dcnt) {
   using acc = typename T::acc_t;
-  __shared__ int32_t sh[17];
+  constexpr int NW = RS_T / 64;
+  __shared__ int32_t wh[NW];  // heads in each wave's segment of the tile
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const DenseChunk tt = tiles[t];
     const DenseChunk tl = tiles[tt.first + tt.nch - 1];
     const int64_t seg0 = tiles[tt.first].start, seg1 = tl.start + tl.len;
     const int64_t row0 = tpos[tt.first];
     if (t == tt.first && threadIdx.x == 0) dcnt[tt.c] = (int32_t)(tpos[tt.first + tt.nch] - row0);
-    const int i0 = threadIdx.x * RS_PT;
-    bool hd[RS_PT];
-    int c = 0;
-#pragma unroll
-    for (int k = 0; k < RS_PT; k++) {
-      const int i = i0 + k;
+    // wave w owns the tile's pairs [s0, s1), in steps of 64 consecutive pairs
+    const int seg = (tt.len + NW * 64 - 1) / (NW * 64) * 64;
+    const int s0 = w * seg, s1 = min(tt.len, s0 + seg);
+    int cnt = 0;
+    for (int b = s0; b < s1; b += 64) {  // uniform per wave
+      const int i = b + lane;
       const int64_t g = tt.start + i;
-      hd[k] = i < tt.len && (g == seg0 || keys[g] != keys[g - 1]);
-      c += hd[k] ? 1 : 0;
+      const bool hd = i < s1 && (g == seg0 || keys[g] != keys[g - 1]);
+      cnt += __popcll(__ballot(hd));
     }
-    int32_t tot;
-    int r = block_excl_scan<int32_t>(c, sh, &tot);
-    const int64_t out0 = eoff[tt.c] + (tpos[t] - row0);
-#pragma unroll
-    for (int k = 0; k < RS_PT; k++) {
-      if (!hd[k]) continue;
-      const int64_t g = tt.start + i0 + k;
-      const int key = keys[g];
-      acc x = vals[g];
-      for (int64_t h = g + 1; h < seg1 && keys[h] == key; h++) x += vals[h];
-      skey[out0 + r] = key;
-      sval[out0 + r] = x;
-      r++;
+    if (lane == 0) wh[w] = cnt;
+    __syncthreads();
+    int64_t r = eoff[tt.c] + (tpos[t] - row0);
+    for (int q = 0; q < w; q++) r += wh[q];
+    for (int b = s0; b < s1; b += 64) {
+      const int i = b + lane;
+      const int64_t g = tt.start + i;
+      const bool hd = i < s1 && (g == seg0 || keys[g] != keys[g - 1]);
+      const unsigned long long bal = __ballot(hd);
+      if (hd) {
+        const int key = keys[g];
+        acc x = vals[g];
+        for (int64_t h = g + 1; h < seg1 && keys[h] == key; h++) x += vals[h];
+        const int64_t o = r + __popcll(bal & below);
+        skey[o] = key;
+        sval[o] = x;
+      }
+      r += __popcll(bal);
     }
+    __syncthreads();
   }
 }
 
@@ -1207,7 +1216,6 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     }
     const int32_t* kf = kb[(npass - 1) & 1];
     const acc* vf = vb[(npass - 1) & 1];
-    static_assert(LT / 2 <= RS_T * RS_PT, "a tile fits one k_rs_emit workgroup");
     k_rs_heads<<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kf, tcnt);
     scan_exclusive<int64_t, int64_t>(tcnt, tpos, ntl, tmp, tpos + ntl, s);
     k_rs_emit<T><<<std::min(ntl, 4096), RS_T, 0, s>>>(tiles, ntl, kf, vf, tpos, cb.eoff, cb.skey, cb.sval, cb.dcnt);
