@@ -831,12 +831,17 @@ __global__ __launch_bounds__(1024) void k_rs_scatter(const DenseChunk* __restric
 
 // Run merge of the sorted packed pairs: equal keys are adjacent, and the
 // first pair of a row or of a new key is a run head.  k_rs_heads counts each
-// tile's heads; after an exclusive scan over the tiles, k_rs_emit ranks every
-// head inside its tile (block scan) and writes its key with the run's sum at
-// that rank from the start of the row's staged segment (a run holds at most
-// one pair per chunk of the row, so a head sums it by walking forward, past
-// the tile's end if needed); the row's first tile writes its distinct count.
+// tile's heads; after an exclusive scan over the tiles, k_rs_emit ranks the
+// heads (per wave segment of the tile, with ballots) and writes each head's
+// key at its rank from the start of the row's staged segment with the sum of
+// its run inside the head's 64-pair window (a segmented reduction over the
+// window).  A window's leading pairs whose head lies before the window (a run
+// longer than the rest of its window: hot keys of long rows meet one pair per
+// chunk) leave their partial sum and the run's slot in cont; k_rs_cont adds
+// those once every head has been written.  The row's first tile writes its
+// distinct count.
 constexpr int RS_T = 1024;  // k_rs_emit threads
+constexpr int RS_WIN = 128;  // 64-pair windows per tile (tiles <= LT/2 = 8192 pairs)
 __global__ __launch_bounds__(256) void k_rs_heads(const DenseChunk* __restrict__ tiles, int ntiles,
                                                   const int32_t* __restrict__ keys, int64_t* __restrict__ tcnt) {
   __shared__ int64_t sh[17];
@@ -860,7 +865,8 @@ __global__ __launch_bounds__(RS_T) void k_rs_emit(const DenseChunk* __restrict__
                                                   const typename T::acc_t* __restrict__ vals,
                                                   const int64_t* __restrict__ tpos, const int64_t* __restrict__ eoff,
                                                   int32_t* __restrict__ skey, typename T::acc_t* __restrict__ sval,
-                                                  int32_t* __restrict__ dcnt) {
+                                                  int32_t* __restrict__ dcnt, int64_t* __restrict__ cont_slot,
+                                                  typename T::acc_t* __restrict__ cont_val) {
   using acc = typename T::acc_t;
   constexpr int NW = RS_T / 64;
   __shared__ int32_t wh[NW];  // heads in each wave's segment of the tile
@@ -868,11 +874,10 @@ __global__ __launch_bounds__(RS_T) void k_rs_emit(const DenseChunk* __restrict__
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const DenseChunk tt = tiles[t];
-    const DenseChunk tl = tiles[tt.first + tt.nch - 1];
-    const int64_t seg0 = tiles[tt.first].start, seg1 = tl.start + tl.len;
+    const int64_t seg0 = tiles[tt.first].start;
     const int64_t row0 = tpos[tt.first];
     if (t == tt.first && threadIdx.x == 0) dcnt[tt.c] = (int32_t)(tpos[tt.first + tt.nch] - row0);
-    // wave w owns the tile's pairs [s0, s1), in steps of 64 consecutive pairs
+    // wave w owns the tile's pairs [s0, s1), in windows of 64 consecutive pairs
     const int seg = (tt.len + NW * 64 - 1) / (NW * 64) * 64;
     const int s0 = w * seg, s1 = min(tt.len, s0 + seg);
     int cnt = 0;
@@ -884,20 +889,33 @@ __global__ __launch_bounds__(RS_T) void k_rs_emit(const DenseChunk* __restrict__
     }
     if (lane == 0) wh[w] = cnt;
     __syncthreads();
-    int64_t r = eoff[tt.c] + (tpos[t] - row0);
+    int64_t r = eoff[tt.c] + (tpos[t] - row0);  // output slot of the next head
     for (int q = 0; q < w; q++) r += wh[q];
     for (int b = s0; b < s1; b += 64) {
       const int i = b + lane;
       const int64_t g = tt.start + i;
-      const bool hd = i < s1 && (g == seg0 || keys[g] != keys[g - 1]);
+      const bool ok = i < s1;
+      const bool hd = ok && (g == seg0 || keys[g] != keys[g - 1]);
       const unsigned long long bal = __ballot(hd);
+      // segment of the lane inside the window: heads at or before it (0: the
+      // pairs before the window's first head)
+      const int sid = __popcll(bal & (below | (1ull << lane)));
+      acc x = ok ? vals[g] : acc(0);
+#pragma unroll
+      for (int k = 1; k < 64; k <<= 1) {  // suffix sums within the segment
+        const acc xo = __shfl_down(x, k, 64);
+        const int so = __shfl_down(sid, k, 64);
+        if (lane + k < 64 && so == sid) x += xo;
+      }
       if (hd) {
-        const int key = keys[g];
-        acc x = vals[g];
-        for (int64_t h = g + 1; h < seg1 && keys[h] == key; h++) x += vals[h];
         const int64_t o = r + __popcll(bal & below);
-        skey[o] = key;
+        skey[o] = keys[g];
         sval[o] = x;
+      }
+      if (lane == 0) {
+        const int win = t * RS_WIN + b / 64;
+        cont_slot[win] = hd ? -1 : r - 1;
+        cont_val[win] = x;
       }
       r += __popcll(bal);
     }
@@ -905,10 +923,25 @@ __global__ __launch_bounds__(RS_T) void k_rs_emit(const DenseChunk* __restrict__
   }
 }
 
+// the continuation sums of the windows, once every head's slot holds its key
+template <class T>
+__global__ __launch_bounds__(256) void k_rs_cont(const DenseChunk* __restrict__ tiles, int ntiles,
+                                                 const int64_t* __restrict__ cont_slot,
+                                                 const typename T::acc_t* __restrict__ cont_val,
+                                                 typename T::acc_t* __restrict__ sval) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < (int64_t)ntiles * RS_WIN;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int t = (int)(q / RS_WIN), j = (int)(q % RS_WIN);
+    if (j * 64 >= tiles[t].len) continue;
+    const int64_t slot = cont_slot[q];
+    if (slot >= 0) atomicAdd(&sval[slot], cont_val[q]);
+  }
+}
+
 // Scratch layout of the dense-row sort (offsets in bytes, 256-byte aligned).
 constexpr int64_t RS_NB_MAX = 1024;  // buckets of the widest radix digit (10 bits)
 struct DenseLayout {
-  int64_t ccnt, pstart, bound, tiles, hist, hscan, k0, k1, v0, v1, tcnt, tpos, tmp, total;
+  int64_t ccnt, pstart, bound, tiles, hist, hscan, k0, k1, v0, v1, tcnt, tpos, cslot, cval, tmp, total;
 };
 // nch chunks, nrows rows, P staged entries; tiles of the packed rows are at
 // most nch + nrows
@@ -929,6 +962,8 @@ static DenseLayout dense_layout(int64_t nch, int64_t nrows, int64_t P, int64_t v
   L.v1 = o; o = al(o + vsz * P);
   L.tcnt = o; o = al(o + 8 * nt);
   L.tpos = o; o = al(o + 8 * (nt + 1));
+  L.cslot = o; o = al(o + 8 * RS_WIN * nt);
+  L.cval = o; o = al(o + vsz * RS_WIN * nt);
   L.tmp = o; o = al(o + 8 * (scan_tmp_elems(std::max<int64_t>(P, RS_NB_MAX * nt)) + 1));
   L.total = o;
   return L;
@@ -1165,6 +1200,8 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     acc* vb[2] = {(acc*)(d + L.v0), (acc*)(d + L.v1)};
     int64_t* tcnt = (int64_t*)(d + L.tcnt);
     int64_t* tpos = (int64_t*)(d + L.tpos);
+    int64_t* cslot = (int64_t*)(d + L.cslot);
+    acc* cval = (acc*)(d + L.cval);
     int64_t* tmp = (int64_t*)(d + L.tmp);
     if (hipMemcpyAsync(cb.chunks, plan.chunks.data(), sizeof(DenseChunk) * nch, hipMemcpyHostToDevice, s) !=
         hipSuccess)
@@ -1218,7 +1255,11 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     const acc* vf = vb[(npass - 1) & 1];
     k_rs_heads<<<std::min(ntl, 8192), 256, 0, s>>>(tiles, ntl, kf, tcnt);
     scan_exclusive<int64_t, int64_t>(tcnt, tpos, ntl, tmp, tpos + ntl, s);
-    k_rs_emit<T><<<std::min(ntl, 4096), RS_T, 0, s>>>(tiles, ntl, kf, vf, tpos, cb.eoff, cb.skey, cb.sval, cb.dcnt);
+    static_assert(LT / 2 <= RS_WIN * 64, "a tile's windows fit its continuation slots");
+    k_rs_emit<T><<<std::min(ntl, 4096), RS_T, 0, s>>>(tiles, ntl, kf, vf, tpos, cb.eoff, cb.skey, cb.sval, cb.dcnt,
+                                                      cslot, cval);
+    k_rs_cont<T><<<(int)std::min<int64_t>(((int64_t)ntl * RS_WIN + 255) / 256, 8192), 256, 0, s>>>(tiles, ntl, cslot,
+                                                                                                   cval, cb.sval);
     if (hipStreamSynchronize(s) != hipSuccess) return -1;  // the host tile vector is in flight until here
   }
   if (out.end) {  // gapped: every row stays at the start of its staging segment
