@@ -97,23 +97,23 @@ KERNEL_FIRST = [
 PMC_FETCH_FACTOR, PMC_WRITE_FACTOR = 2.00, 1.00
 KERNEL_HUB_PART = "k_hub_part<16384, IntW>"  # the part kernel of the bin-8 split sequence
 PMC_KIB = {  # kernel: (FETCH_SIZE KiB, WRITE_SIZE KiB) per dispatch
-    KERNEL_HUB_PART: (196649.8, 26.4),
-    "k_decide_glead<256, 16, 8, false, IntW>": (192773.9, 5200.2),
-    "k_decide_blead<256, 8192, 16, 1, IntW>": (124280.8, 439.7),
-    "k_decide_glead<1024, 64, 8, false, IntW>": (107251.4, 1011.9),
-    "k_decide_glead<1, 16, 8, true, IntW>": (105251.8, 4201.6),
-    "k_decide_tpv<31, false, IntW>": (103810.0, 4873.6),
-    "k_decide_tpv<15, false, IntW>": (85618.8, 6749.0),
-    "k_decide_first_block<256, IntW>": (77605.5, 529.6),
-    "k_decide_tpv<7, false, IntW>": (64117.3, 8618.8),
-    "k_decide_tpv<15, true, IntW>": (56302.1, 6047.8),
-    "k_decide_glead<1, 64, 8, true, IntW>": (58021.9, 759.8),
-    "k_decide_tpv<31, true, IntW>": (55568.1, 4153.7),
-    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50894.1, 71.4),
-    "k_decide_first_block<1024, IntW>": (49516.5, 89.3),
-    "k_decide_tpv<7, true, IntW>": (41378.5, 7064.7),
-    "k_decide_tpv<3, false, IntW>": (25586.6, 5841.1),
-    "k_decide_pipe<4, true, IntW>": (16195.9, 4386.1),
+    KERNEL_HUB_PART: (196728.6, 26.4),
+    "k_decide_glead<256, 16, 8, false, IntW>": (193495.5, 7287.8),
+    "k_decide_blead<256, 8192, 16, 1, IntW>": (124297.0, 581.3),
+    "k_decide_glead<1024, 64, 8, false, IntW>": (107478.9, 1440.9),
+    "k_decide_glead<1, 16, 8, true, IntW>": (105906.3, 5875.1),
+    "k_decide_tpv<31, false, IntW>": (103958.0, 6660.8),
+    "k_decide_tpv<15, false, IntW>": (85836.1, 9327.7),
+    "k_decide_first_block<256, IntW>": (77673.7, 672.1),
+    "k_decide_tpv<7, false, IntW>": (64394.7, 11764.8),
+    "k_decide_glead<1, 64, 8, true, IntW>": (58196.3, 1062.9),
+    "k_decide_tpv<15, true, IntW>": (56391.9, 8347.8),
+    "k_decide_tpv<31, true, IntW>": (55553.7, 5671.7),
+    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50893.7, 84.2),
+    "k_decide_first_block<1024, IntW>": (49494.3, 104.7),
+    "k_decide_tpv<7, true, IntW>": (41655.5, 9656.4),
+    "k_decide_tpv<3, false, IntW>": (25765.3, 8027.0),
+    "k_decide_pipe<4, true, IntW>": (16394.0, 5986.9),
 }
 
 
