@@ -141,17 +141,20 @@ def cpu_threads(args):
 
 def contraction_roofline(r):
     """Contraction against HBM peak: per round 12*2E + 8*2E' + 16*V + 8*C bytes
-    (read the round CSR with each entry's community, write the contracted CSR)."""
+    (read the round CSR with each entry's community, write the contracted CSR).
+    A round whose first sweep was refused keeps the singleton partition; the
+    engine skips its contraction (the identity), so it counts no bytes."""
     rounds = []
     tot_b = 0
     for i in range(r["rounds"]):
         V, m_in, m_out, C = (r["round_vertices"][i], r["round_entries"][i], r["round_contracted_entries"][i],
                              r["round_nc"][i])
-        b = 12 * m_in + 8 * m_out + 16 * V + 8 * C
+        skipped = r["round_iters"][i] == 1 and C == V and m_out == m_in
+        b = 0 if skipped else 12 * m_in + 8 * m_out + 16 * V + 8 * C
         ms = r["round_build_ms"][i]
         tot_b += b
         rounds.append({"round": i, "vertices": V, "entries": m_in, "contracted_entries": m_out,
-                       "communities": C, "bytes": b, "ms": ms,
+                       "communities": C, "bytes": b, "ms": ms, "skipped": skipped,
                        "GBps": b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0})
     ms = r["t_contract_ms"]
     gbs = tot_b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -161,7 +164,7 @@ def contraction_roofline(r):
 
 def learned_leg(gala, args):
     """Learned frontier on held-out seeds 1001-1003 of the workload (the network
-    never saw them: it is trained on 1,024-vertex SBM move logs).  Per seed:
+    never saw them: it is trained on move logs of seeds 1-32 at 1M vertices).  Per seed:
     the plain path and the learned form on the same graph, K timed runs each
     after one warmup."""
     import numpy as np
