@@ -244,12 +244,15 @@ __device__ __forceinline__ void stage_span(const DevGraph<T>& g, int64_t o, int6
     }
 #pragma unroll
     for (int r = 0; r < STG_U; r++) u[r] = j0 + (int64_t)r * NT < d ? label[u[r]] : 0;
+    // unit weights (an unweighted input round): the merges take every staged
+    // value as 1, so none is written
+    const bool unit = !g.weighted();
 #pragma unroll
     for (int r = 0; r < STG_U; r++) {
       const int64_t j = j0 + (int64_t)r * NT;
       if (j < d) {
         skey[base + j] = u[r];
-        sval[base + j] = w[r];
+        if (!unit) sval[base + j] = w[r];
       }
     }
   }
@@ -311,7 +314,7 @@ __global__ __launch_bounds__(256) void k_merge_tiny(const int64_t* __restrict__ 
                                                      const int32_t* __restrict__ tstart,
                                                      int32_t* __restrict__ skey,
                                                      typename T::acc_t* __restrict__ sval,
-                                                     int32_t* __restrict__ dcnt) {
+                                                     int32_t* __restrict__ dcnt, int unit) {
   using acc = typename T::acc_t;
   const int lane = threadIdx.x & (W - 1);
   const int gbase = (threadIdx.x & 63) & ~(W - 1);  // first lane of the group in its wave
@@ -324,7 +327,7 @@ __global__ __launch_bounds__(256) void k_merge_tiny(const int64_t* __restrict__ 
     const int64_t base = eoff[c];
     int key = INT_MAX;
     acc val = 0;
-    if (lane < L) { key = skey[base + lane]; val = sval[base + lane]; }
+    if (lane < L) { key = skey[base + lane]; val = unit ? acc(1) : sval[base + lane]; }
     // bitonic sort over the group's W lanes (ascending)
 #pragma unroll
     for (int k = 2; k <= W; k <<= 1) {
@@ -427,7 +430,7 @@ __global__ __launch_bounds__(NT) void k_merge_mid(const int64_t* __restrict__ ec
                                                    const int32_t* __restrict__ tstart, int tr,
                                                    int32_t* __restrict__ skey,
                                                    typename T::acc_t* __restrict__ sval,
-                                                   int32_t* __restrict__ dcnt, int canonical) {
+                                                   int32_t* __restrict__ dcnt, int canonical, int unit) {
   using acc = typename T::acc_t;
   constexpr int TMAX = 2 * LMAX;
   __shared__ int h_k[TMAX];
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(NT) void k_merge_mid(const int64_t* __restrict__ ec
     for (int r = 0; r < U; r++) {
       const long long j = threadIdx.x + r * NT;
       key[r] = j < L ? skey[base + j] : -1;
-      w[r] = j < L ? sval[base + j] : acc(0);
+      w[r] = j < L ? (unit ? acc(1) : sval[base + j]) : acc(0);
     }
 #pragma unroll
     for (int r = 0; r < U; r++) {
@@ -504,7 +507,7 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
                                                      typename T::acc_t* __restrict__ arena_vals,
                                                      int32_t* __restrict__ skey,
                                                      typename T::acc_t* __restrict__ sval,
-                                                     int32_t* __restrict__ dcnt, int canonical) {
+                                                     int32_t* __restrict__ dcnt, int canonical, int unit) {
   using acc = typename T::acc_t;
   constexpr int NT = 1024, PMAX = 32, PER = LT / NT;
   constexpr int LLG = __builtin_ctz(LT);
@@ -532,7 +535,7 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
     for (int r = 0; r < PER; r++) {
       const long long j = threadIdx.x + r * NT;
       key[r] = j < L ? skey[base + j] : -1;
-      w[r] = j < L ? sval[base + j] : acc(0);
+      w[r] = j < L ? (unit ? acc(1) : sval[base + j]) : acc(0);
     }
 #pragma unroll
     for (int r = 0; r < PER; r++) {
@@ -635,7 +638,8 @@ __global__ __launch_bounds__(1024) void k_merge_big(const int64_t* __restrict__ 
 template <int LT, class T>
 __global__ __launch_bounds__(1024) void k_dense_agg(const DenseChunk* __restrict__ chunks, int nch,
                                                     const int64_t* __restrict__ eoff, int32_t* __restrict__ skey,
-                                                    typename T::acc_t* __restrict__ sval, int32_t* __restrict__ ccnt) {
+                                                    typename T::acc_t* __restrict__ sval, int32_t* __restrict__ ccnt,
+                                                    int unit) {
   using acc = typename T::acc_t;
   constexpr int NT = 1024, LLG = __builtin_ctz(LT), U = LT / 2 / NT;
   __shared__ int s_k[LT];
@@ -653,7 +657,7 @@ __global__ __launch_bounds__(1024) void k_dense_agg(const DenseChunk* __restrict
     for (int r = 0; r < U; r++) {
       const int j = threadIdx.x + r * NT;
       key[r] = j < cc.len ? skey[b2 + j] : -1;
-      w[r] = j < cc.len ? sval[b2 + j] : acc(0);
+      w[r] = j < cc.len ? (unit ? acc(1) : sval[b2 + j]) : acc(0);
     }
   };
   if (ch < nch) load(ch, base);
@@ -1159,30 +1163,31 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
                                                               voff, cb.skey, cb.sval);
   fill<int32_t>(cb.dcnt, C + 1, 0, s);
   const int canon = plan.canonical ? 1 : 0;
+  const int unit = g.weighted() ? 0 : 1;  // staging wrote keys only: every staged value is 1
   const int32_t* ts = plan.tstart;
   auto tcount = [&](int t) { return ts[t + 1] - ts[t]; };
   if (tcount(TIER_T16) > 0)
     k_merge_tiny<16, TIER_T16, T><<<nblk(tcount(TIER_T16), 16, 16384), 256, 0, s>>>(
-        cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey, cb.sval, cb.dcnt);
+        cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey, cb.sval, cb.dcnt, unit);
   if (tcount(TIER_T32) > 0)
     k_merge_tiny<32, TIER_T32, T><<<nblk(tcount(TIER_T32), 8, 16384), 256, 0, s>>>(
-        cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey, cb.sval, cb.dcnt);
+        cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey, cb.sval, cb.dcnt, unit);
   if (tcount(TIER_T64) > 0)
     k_merge_tiny<64, TIER_T64, T><<<nblk(tcount(TIER_T64), 4, 16384), 256, 0, s>>>(
-        cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey, cb.sval, cb.dcnt);
+        cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey, cb.sval, cb.dcnt, unit);
   if (tcount(TIER_MID1) > 0)
     k_merge_mid<64, 64, 512, T><<<nblk(tcount(TIER_MID1), 1, 16384), 64, 0, s>>>(
-        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID1, cb.skey, cb.sval, cb.dcnt, canon);
+        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID1, cb.skey, cb.sval, cb.dcnt, canon, unit);
   if (tcount(TIER_MID2) > 0)
     k_merge_mid<128, 512, 2048, T><<<nblk(tcount(TIER_MID2), 1, 8192), 128, 0, s>>>(
-        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID2, cb.skey, cb.sval, cb.dcnt, canon);
+        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID2, cb.skey, cb.sval, cb.dcnt, canon, unit);
   if (tcount(TIER_MID3) > 0)
     k_merge_mid<256, 2048, 4096, T><<<nblk(tcount(TIER_MID3), 1, 4096), 256, 0, s>>>(
-        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID3, cb.skey, cb.sval, cb.dcnt, canon);
+        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID3, cb.skey, cb.sval, cb.dcnt, canon, unit);
   if (plan.nbig > 0)
     k_merge_big<LT, T><<<std::min(plan.nbig, 2048), 1024, 0, s>>>(cb.ecnt, cb.eoff, C, cb.big_list, plan.nbig,
                                                                    cb.big_tab, cb.arena_keys, cb.arena_vals,
-                                                                   cb.skey, cb.sval, cb.dcnt, canon);
+                                                                   cb.skey, cb.sval, cb.dcnt, canon, unit);
   if (plan.ndense > 0) {
     using acc = typename T::acc_t;
     const int nch = (int)plan.chunks.size();
@@ -1206,7 +1211,8 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
     if (hipMemcpyAsync(cb.chunks, plan.chunks.data(), sizeof(DenseChunk) * nch, hipMemcpyHostToDevice, s) !=
         hipSuccess)
       return -1;
-    k_dense_agg<LT, T><<<std::min(nch, 4096), 1024, 0, s>>>(cb.chunks, nch, cb.eoff, cb.skey, cb.sval, ccnt);
+    k_dense_agg<LT, T><<<std::min(nch, 4096), 1024, 0, s>>>(cb.chunks, nch, cb.eoff, cb.skey, cb.sval, ccnt,
+                                                              unit);
     scan_exclusive<int32_t, int64_t>(ccnt, pstart, nch, tmp, pstart + nch, s);
     // pass 0 reads kb[1]; pass p writes kb[p & 1]
     k_dense_pack<T><<<std::min(nch, 8192), 256, 0, s>>>(cb.chunks, nch, ccnt, pstart, cb.eoff, cb.skey, cb.sval,
