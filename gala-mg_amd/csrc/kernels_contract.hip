@@ -1154,10 +1154,10 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
                                                            cb.skey, cb.sval);
   k_stage_rows<16, T><<<nblk(cp[2], 16, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 2, 3, cb.label, cb.eoff, voff,
                                                             cb.skey, cb.sval);
-  k_stage_rows<32, T><<<nblk(cp[3], 8, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 3, 4, cb.label, cb.eoff, voff,
-                                                           cb.skey, cb.sval);
-  k_stage_rows<64, T><<<nblk(cp[4] + cp[5] + cp[6], 4, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 4, 7, cb.label,
-                                                                           cb.eoff, voff, cb.skey, cb.sval);
+  k_stage_rows<32, T><<<nblk(cp[3] + cp[4], 8, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 3, 5, cb.label, cb.eoff,
+                                                                   voff, cb.skey, cb.sval);
+  k_stage_rows<64, T><<<nblk(cp[5] + cp[6], 4, 8192), 256, 0, s>>>(g, cb.order, cb.bstart, 5, 7, cb.label, cb.eoff,
+                                                                   voff, cb.skey, cb.sval);
   if (cp[7] + cp[8] > 0)
     k_stage_chunks<T><<<dim3(cp[7] + cp[8], 16), 256, 0, s>>>(g, cb.order, cb.bstart, 7, NBINS, cb.label, cb.eoff,
                                                               voff, cb.skey, cb.sval);
