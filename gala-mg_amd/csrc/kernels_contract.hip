@@ -190,10 +190,11 @@ __global__ void k_dense_lengths(const int32_t* __restrict__ list, int32_t nlist,
 
 // classify rows: 4096 < L <= lt -> big_list (with the size of its global
 // overflow slice); L > lt -> dense_list
-// Row length tiers of the merge: 0: 1..16, 1: 17..32, 2: 33..64, 3: 65..512,
-// 4: 513..2048, 5: 2049..4096; longer rows go to the big and dense lists,
-// empty rows nowhere.
-constexpr int TIER_T16 = 0, TIER_T32 = 1, TIER_T64 = 2, TIER_MID1 = 3, TIER_MID2 = 4, TIER_MID3 = 5;
+// Row length tiers of the merge: 0: 1..16, 1: 17..32, 2: 33..64, 3: 65..128,
+// 4: 129..512, 5: 513..2048, 6: 2049..4096; longer rows go to the big and
+// dense lists, empty rows nowhere.
+constexpr int TIER_T16 = 0, TIER_T32 = 1, TIER_T64 = 2, TIER_MID0 = 3, TIER_MID1 = 4, TIER_MID2 = 5,
+              TIER_MID3 = 6;
 
 // C is read on the device (rows c >= C get tier -1), so the host needs no
 // readback before the launch; n is an upper bound of C.
@@ -209,7 +210,7 @@ __global__ void k_classify(const int64_t* __restrict__ ecnt, const int32_t* __re
     }
     const long long L = ecnt[c];
     tier[c] = L == 0 ? -1 : L <= 16 ? TIER_T16 : L <= 32 ? TIER_T32 : L <= 64 ? TIER_T64
-            : L <= 512 ? TIER_MID1 : L <= 2048 ? TIER_MID2 : L <= 4096 ? TIER_MID3 : -1;
+            : L <= 128 ? TIER_MID0 : L <= 512 ? TIER_MID1 : L <= 2048 ? TIER_MID2 : L <= 4096 ? TIER_MID3 : -1;
     if (L > lt) {
       dense_list[atomicAdd(&counts[1], 1)] = c;
     } else if (L > 4096) {
@@ -1175,8 +1176,11 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
   if (tcount(TIER_T64) > 0)
     k_merge_tiny<64, TIER_T64, T><<<nblk(tcount(TIER_T64), 4, 16384), 256, 0, s>>>(
         cb.ecnt, cb.eoff, cb.tlist, cb.tstart, cb.skey, cb.sval, cb.dcnt, unit);
+  if (tcount(TIER_MID0) > 0)
+    k_merge_mid<64, 64, 128, T><<<nblk(tcount(TIER_MID0), 1, 16384), 64, 0, s>>>(
+        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID0, cb.skey, cb.sval, cb.dcnt, canon, unit);
   if (tcount(TIER_MID1) > 0)
-    k_merge_mid<64, 64, 512, T><<<nblk(tcount(TIER_MID1), 1, 16384), 64, 0, s>>>(
+    k_merge_mid<64, 128, 512, T><<<nblk(tcount(TIER_MID1), 1, 16384), 64, 0, s>>>(
         cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID1, cb.skey, cb.sval, cb.dcnt, canon, unit);
   if (tcount(TIER_MID2) > 0)
     k_merge_mid<128, 512, 2048, T><<<nblk(tcount(TIER_MID2), 1, 8192), 128, 0, s>>>(
