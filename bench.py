@@ -97,23 +97,23 @@ KERNEL_FIRST = [
 PMC_FETCH_FACTOR, PMC_WRITE_FACTOR = 2.00, 1.00
 KERNEL_HUB_PART = "k_hub_part<16384, IntW>"  # the part kernel of the bin-8 split sequence
 PMC_KIB = {  # kernel: (FETCH_SIZE KiB, WRITE_SIZE KiB) per dispatch
-    KERNEL_HUB_PART: (196863.1, 26.4),
-    "k_decide_glead<256, 16, 8, false, IntW>": (193551.5, 7296.8),
-    "k_decide_blead<256, 8192, 16, 1, IntW>": (124326.2, 581.6),
-    "k_decide_glead<1024, 64, 8, false, IntW>": (107501.5, 1437.6),
-    "k_decide_glead<1, 16, 8, true, IntW>": (105929.2, 5869.3),
-    "k_decide_tpv<31, false, IntW>": (103958.3, 6661.4),
-    "k_decide_tpv<15, false, IntW>": (85826.1, 9330.1),
-    "k_decide_first_block<256, IntW>": (77699.5, 670.5),
-    "k_decide_tpv<7, false, IntW>": (64394.9, 11763.8),
-    "k_decide_glead<1, 64, 8, true, IntW>": (58207.7, 1062.5),
-    "k_decide_tpv<15, true, IntW>": (56311.6, 8347.5),
-    "k_decide_tpv<31, true, IntW>": (55535.4, 5670.8),
-    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50887.5, 84.3),
-    "k_decide_first_block<1024, IntW>": (49483.4, 104.7),
-    "k_decide_tpv<7, true, IntW>": (41646.8, 9657.7),
-    "k_decide_tpv<3, false, IntW>": (25764.0, 8028.5),
-    "k_decide_pipe<4, true, IntW>": (16398.2, 5979.7),
+    KERNEL_HUB_PART: (196579.4, 26.4),
+    "k_decide_glead<256, 16, 8, false, IntW>": (193549.7, 7297.2),
+    "k_decide_blead<256, 8192, 16, 1, IntW>": (124307.2, 580.9),
+    "k_decide_glead<1024, 64, 8, false, IntW>": (107487.8, 1435.2),
+    "k_decide_glead<1, 16, 8, true, IntW>": (105921.8, 5873.8),
+    "k_decide_tpv<31, false, IntW>": (103942.9, 6658.9),
+    "k_decide_tpv<15, false, IntW>": (85842.5, 9328.2),
+    "k_decide_first_block<256, IntW>": (77677.7, 671.9),
+    "k_decide_tpv<7, false, IntW>": (64395.3, 11762.7),
+    "k_decide_glead<1, 64, 8, true, IntW>": (58202.0, 1064.4),
+    "k_decide_tpv<15, true, IntW>": (56325.9, 8347.8),
+    "k_decide_tpv<31, true, IntW>": (55564.2, 5671.5),
+    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50886.7, 84.3),
+    "k_decide_first_block<1024, IntW>": (49491.1, 104.8),
+    "k_decide_tpv<7, true, IntW>": (41647.0, 9655.5),
+    "k_decide_tpv<3, false, IntW>": (25763.4, 8027.7),
+    "k_decide_pipe<4, true, IntW>": (16399.8, 5980.9),
 }
 
 
