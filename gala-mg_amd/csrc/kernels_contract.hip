@@ -191,10 +191,11 @@ __global__ void k_dense_lengths(const int32_t* __restrict__ list, int32_t nlist,
 // classify rows: 4096 < L <= lt -> big_list (with the size of its global
 // overflow slice); L > lt -> dense_list
 // Row length tiers of the merge: 0: 1..16, 1: 17..32, 2: 33..64, 3: 65..128,
-// 4: 129..512, 5: 513..2048, 6: 2049..4096; longer rows go to the big and
-// dense lists, empty rows nowhere.
-constexpr int TIER_T16 = 0, TIER_T32 = 1, TIER_T64 = 2, TIER_MID0 = 3, TIER_MID1 = 4, TIER_MID2 = 5,
-              TIER_MID3 = 6;
+// 4: 129..512, 5: 513..1024, 6: 1025..2048, 7: 2049..4096; longer rows go to
+// the big and dense lists, empty rows nowhere.
+constexpr int TIER_T16 = 0, TIER_T32 = 1, TIER_T64 = 2, TIER_MID0 = 3, TIER_MID1 = 4, TIER_MID1K = 5,
+              TIER_MID2 = 6, TIER_MID3 = 7;
+static_assert(TIER_MID3 < NBINS, "the tiers are partitioned like degree bins");
 
 // C is read on the device (rows c >= C get tier -1), so the host needs no
 // readback before the launch; n is an upper bound of C.
@@ -210,7 +211,7 @@ __global__ void k_classify(const int64_t* __restrict__ ecnt, const int32_t* __re
     }
     const long long L = ecnt[c];
     tier[c] = L == 0 ? -1 : L <= 16 ? TIER_T16 : L <= 32 ? TIER_T32 : L <= 64 ? TIER_T64
-            : L <= 128 ? TIER_MID0 : L <= 512 ? TIER_MID1 : L <= 2048 ? TIER_MID2 : L <= 4096 ? TIER_MID3 : -1;
+            : L <= 128 ? TIER_MID0 : L <= 512 ? TIER_MID1 : L <= 1024 ? TIER_MID1K : L <= 2048 ? TIER_MID2 : L <= 4096 ? TIER_MID3 : -1;
     if (L > lt) {
       dense_list[atomicAdd(&counts[1], 1)] = c;
     } else if (L > 4096) {
@@ -1182,8 +1183,11 @@ int contract_build(const DevGraph<T>& g, ContractBufs<T>& cb, const ContractPlan
   if (tcount(TIER_MID1) > 0)
     k_merge_mid<64, 128, 512, T><<<nblk(tcount(TIER_MID1), 1, 16384), 64, 0, s>>>(
         cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID1, cb.skey, cb.sval, cb.dcnt, canon, unit);
+  if (tcount(TIER_MID1K) > 0)
+    k_merge_mid<128, 512, 1024, T><<<nblk(tcount(TIER_MID1K), 1, 8192), 128, 0, s>>>(
+        cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID1K, cb.skey, cb.sval, cb.dcnt, canon, unit);
   if (tcount(TIER_MID2) > 0)
-    k_merge_mid<128, 512, 2048, T><<<nblk(tcount(TIER_MID2), 1, 8192), 128, 0, s>>>(
+    k_merge_mid<128, 1024, 2048, T><<<nblk(tcount(TIER_MID2), 1, 8192), 128, 0, s>>>(
         cb.ecnt, cb.eoff, C, cb.tlist, cb.tstart, TIER_MID2, cb.skey, cb.sval, cb.dcnt, canon, unit);
   if (tcount(TIER_MID3) > 0)
     k_merge_mid<256, 2048, 4096, T><<<nblk(tcount(TIER_MID3), 1, 4096), 256, 0, s>>>(
