@@ -97,23 +97,23 @@ KERNEL_FIRST = [
 PMC_FETCH_FACTOR, PMC_WRITE_FACTOR = 2.00, 1.00
 KERNEL_HUB_PART = "k_hub_part<16384, IntW>"  # the part kernel of the bin-8 split sequence
 PMC_KIB = {  # kernel: (FETCH_SIZE KiB, WRITE_SIZE KiB) per dispatch
-    KERNEL_HUB_PART: (196579.4, 26.4),
-    "k_decide_glead<256, 16, 8, false, IntW>": (193549.7, 7297.2),
-    "k_decide_blead<256, 8192, 16, 1, IntW>": (124307.2, 580.9),
-    "k_decide_glead<1024, 64, 8, false, IntW>": (107487.8, 1435.2),
-    "k_decide_glead<1, 16, 8, true, IntW>": (105921.8, 5873.8),
-    "k_decide_tpv<31, false, IntW>": (103942.9, 6658.9),
-    "k_decide_tpv<15, false, IntW>": (85842.5, 9328.2),
-    "k_decide_first_block<256, IntW>": (77677.7, 671.9),
-    "k_decide_tpv<7, false, IntW>": (64395.3, 11762.7),
-    "k_decide_glead<1, 64, 8, true, IntW>": (58202.0, 1064.4),
-    "k_decide_tpv<15, true, IntW>": (56325.9, 8347.8),
-    "k_decide_tpv<31, true, IntW>": (55564.2, 5671.5),
-    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50886.7, 84.3),
-    "k_decide_first_block<1024, IntW>": (49491.1, 104.8),
-    "k_decide_tpv<7, true, IntW>": (41647.0, 9655.5),
-    "k_decide_tpv<3, false, IntW>": (25763.4, 8027.7),
-    "k_decide_pipe<4, true, IntW>": (16399.8, 5980.9),
+    KERNEL_HUB_PART: (196282.3, 26.4),
+    "k_decide_glead<256, 16, 8, false, IntW>": (193498.7, 7295.9),
+    "k_decide_blead<256, 8192, 16, 1, IntW>": (124281.4, 580.8),
+    "k_decide_glead<1024, 64, 8, false, IntW>": (107480.0, 1433.8),
+    "k_decide_glead<1, 16, 8, true, IntW>": (105940.6, 5874.6),
+    "k_decide_tpv<31, false, IntW>": (103958.6, 6662.0),
+    "k_decide_tpv<15, false, IntW>": (85845.0, 9329.2),
+    "k_decide_first_block<256, IntW>": (77679.1, 671.7),
+    "k_decide_tpv<7, false, IntW>": (64397.8, 11764.5),
+    "k_decide_glead<1, 64, 8, true, IntW>": (58218.3, 1062.2),
+    "k_decide_tpv<15, true, IntW>": (56296.8, 8347.0),
+    "k_decide_tpv<31, true, IntW>": (55553.6, 5671.7),
+    "k_decide_blead<1024, 16384, 8, 1, IntW>": (50872.7, 84.3),
+    "k_decide_first_block<1024, IntW>": (49557.4, 104.9),
+    "k_decide_tpv<7, true, IntW>": (41649.4, 9658.7),
+    "k_decide_tpv<3, false, IntW>": (25767.5, 8027.9),
+    "k_decide_pipe<4, true, IntW>": (16396.7, 5978.3),
 }
 
 
